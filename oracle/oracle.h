@@ -1,0 +1,153 @@
+/*
+ * oracle.h -- CPU restatement of Rumi-Renderer's hot path, used ONLY as the
+ * checker in tests/, in __graft_entry__.smoke() and in bench.py's CPU-baseline
+ * leg.  Nothing in the product (rumi-renderer_amd/) links, loads or calls it.
+ *
+ * Every function cites the reference source it restates (paths relative to
+ * the Rumi-Renderer source tree).  Arithmetic keeps the reference's FP32
+ * operation order; the library is compiled with -ffp-contract=off so no
+ * multiply-add is fused.
+ *
+ * Parity pins (what checks this file, besides the product it checks):
+ *   or_triangle_intersect / or_sphere_intersect / or_rect_intersect /
+ *   or_bbox_intersect : closed-form cases in tests/test_oracle_geometry.py
+ *   or_bvh_build_median6 : structural invariants (leaf size, median cut,
+ *                          sortedness on the split axis, bounds nesting)
+ *   or_travel_*          : brute force over all primitives (BVH-independent
+ *                          closest hit; ties within 1e-6 relative t)
+ */
+#ifndef RUMI_ORACLE_H
+#define RUMI_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Ray::_epsilon (src/Kernel/ray.cpp:4). */
+#define OR_EPS 1e-4f
+
+/* Ray (src/Kernel/ray.h:9-31): tmin is "closest hit so far", 0 = none. */
+typedef struct {
+    float o[3];
+    float d[3];
+    float tmin;
+    float tmax;
+} or_ray;
+
+/* Primitive kinds of the shape table. */
+enum { OR_TRI = 0, OR_SPHERE = 1, OR_RECT = 2 };
+
+/* One scene primitive, restating the data members the reference's shapes
+ * keep (src/Shape/triangle.h:11-17, sphere.h, rect.h). */
+typedef struct {
+    int kind;
+    float a[3], b[3], c[3];  /* tri: v0,v1,v2; sphere: a=centre; rect: a=p, b=l, c=w */
+    float n[3];              /* rect: unit normal */
+    float r;                 /* sphere radius */
+    float length, width, inv_area; /* rect */
+    int emissive;            /* material Le() != 0 (rect back-face rule, sphere normal) */
+    int le_textured;         /* sphere: IsLe() && texture (image-based light) */
+} or_prim;
+
+/* ------------------------------------------------------------------ */
+/* Intersectors.  Each returns 1 when the primitive is accepted and then
+ * has written ray->tmin = t, exactly like the reference.                */
+int or_triangle_intersect(const float v0[3], const float v1[3], const float v2[3],
+                          or_ray* ray, float* u, float* v, float* det);
+int or_sphere_intersect(const or_prim* s, or_ray* ray, int in_object);
+int or_rect_intersect(const or_prim* q, or_ray* ray);
+int or_prim_intersect(const or_prim* p, or_ray* ray, int in_object);
+
+/* BBox::Intersect (src/Kernel/bbox.cpp:30-92). */
+int or_bbox_intersect(const float lo[3], const float hi[3], const or_ray* ray,
+                      float* tnear, float* tfar);
+
+/* Shape::Bound overrides: triangle.cpp:120-148, sphere.cpp:139-145,
+ * rect.cpp:141-157. */
+void or_prim_bound(const or_prim* p, float lo[3], float hi[3]);
+
+/* ------------------------------------------------------------------ */
+/* Flat BVH: every node carries its own box, as BVHNode does
+ * (src/Acceleration/bvh.h:9-36).  left/right < 0 marks a leaf whose
+ * primitives are order[first .. first+count). */
+typedef struct {
+    int n_nodes;
+    int root;
+    int n_prims;
+    float* lo;     /* 3 * n_nodes */
+    float* hi;     /* 3 * n_nodes */
+    int* left;
+    int* right;
+    int* first;
+    int* count;
+    int* order;    /* n_prims: primitive index per leaf slot */
+} or_bvh;
+
+/* BVHAccel::BuildBVH + CreateBVHNode (src/Acceleration/bvh.cpp:48-148):
+ * median split on the longest axis after a shell sort of box centres,
+ * sort skipped when the axis repeats, leaves of at most 6.  Input is the
+ * per-primitive bounds (prim_lo/prim_hi, 3 floats each) in scene order. */
+int or_bvh_build_median6(const float* prim_lo, const float* prim_hi, int n, or_bvh* out);
+
+/* Rebuild the flat layout from a child-box BVH (two child boxes per
+ * interior node, child ref >= 0 = interior node, < 0 = leaf), the form the
+ * GPU library exports.  Leaf ref encoding: ~((first << 4) | count). */
+int or_bvh_from_childbox(int n_inner, const float* child_boxes /* 12 per node */,
+                         const int* child_refs /* 2 per node */,
+                         const float root_lo[3], const float root_hi[3], int root_ref,
+                         const int* leaf_prims, int n_prims, or_bvh* out);
+
+void or_bvh_free(or_bvh* b);
+
+/* Traversal statistics: interior nodes entered, primitive tests. */
+typedef struct {
+    int64_t nodes;
+    int64_t prims;
+} or_stats;
+
+/* BVHAccel::Travel (src/Acceleration/bvh.cpp:150-212).
+ *   any      = 0: closest hit (isect != NULL); 1: shadow ray (isect == NULL),
+ *              return after the first leaf that produced a hit.
+ *   ordered  = 0: reference order (push right, descend left).
+ *            = 1: near child first by box entry distance (the GPU's order
+ *              for SAH trees); same node and primitive tests.
+ * Returns the primitive index (scene order) or -1; ray->tmin holds t. */
+int or_travel(const or_bvh* bvh, const or_prim* prims, or_ray* ray, int any,
+              int ordered, int in_object, or_stats* st);
+
+/* Batched form over a triangle mesh given as indexed vertices (the
+ * product's input format).  rays: n x 8 floats (ox,oy,oz,tmin,dx,dy,dz,tmax).
+ * Outputs prim id (-1 miss) and t (0 on miss) per ray; per-ray node and
+ * primitive counts when node_cnt / prim_cnt are non-NULL. */
+int or_trace_mesh(const or_bvh* bvh, const float* xyz, const uint32_t* idx, int nt,
+                  const float* rays, int n, int any, int ordered,
+                  int32_t* prim_out, float* t_out, int32_t* node_cnt, int32_t* prim_cnt);
+
+/* Brute force closest hit over all triangles in index order with the
+ * Triangle::Intersect acceptance rules (BVH-independent pin). */
+int or_brute_mesh(const float* xyz, const uint32_t* idx, int nt, const float* rays, int n,
+                  int32_t* prim_out, float* t_out);
+
+/* Convenience: build the scene-order triangle primitive table. */
+int or_mesh_prims(const float* xyz, const uint32_t* idx, int nt, or_prim* out);
+int or_mesh_bounds(const float* xyz, const uint32_t* idx, int nt, float* lo, float* hi);
+
+/* Query the layout of a median6 BVH built by or_bvh_build_median6 (for
+ * structural comparison with the product's reference-mode builder). */
+int or_bvh_nodes(const or_bvh* b);
+void or_bvh_copy(const or_bvh* b, float* lo, float* hi, int* left, int* right,
+                 int* first, int* count, int* order, int* root);
+
+/* Median6 build + traversal driver over a triangle mesh, for timing the
+ * reference's CPU path (bench CPU baseline).  Returns the seconds spent in
+ * the traversal loop only (std-like monotonic clock), -1 on error. */
+double or_time_median6_mesh(const float* xyz, const uint32_t* idx, int nt,
+                            const float* rays, int n, int any,
+                            int32_t* prim_out, float* t_out, double* build_seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,240 @@
+"""GPU traversal parity: the HIP kernels (through the C ABI) against the oracle.
+
+Bar: hit primitive ids and hit distances bit-exact against the oracle's
+traversal of the same tree in the same visiting order; against brute force
+(BVH-independent) ids equal up to ties within 1e-6 relative t.
+"""
+import numpy as np
+import pytest
+
+from conftest import tie_ok
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def closest(rumi, ctx, bvh, rays, flags=0, uv=False):
+    import torch
+    n = rays.shape[0]
+    r = dev(rays)
+    hits = torch.full((max(n, 1), 2), -7, dtype=torch.int32, device="cuda")[:n]
+    uvt = torch.zeros((max(n, 1), 2), dtype=torch.float32, device="cuda")[:n] if uv else None
+    rumi.trace_closest(ctx, bvh, r, hits, uvt, flags)
+    torch.cuda.synchronize()
+    p, t = rumi.split_hits(hits.cpu().numpy())
+    if uv:
+        return p, t, uvt.cpu().numpy()
+    return p, t
+
+
+def anyhit(rumi, ctx, bvh, rays, flags=0):
+    import torch
+    n = rays.shape[0]
+    out = torch.full((max(n, 1),), -7, dtype=torch.int32, device="cuda")[:n]
+    rumi.trace_any(ctx, bvh, dev(rays), out, flags)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def counts(rumi, ctx, bvh, rays, any_hit=False, flags=0):
+    import torch
+    n = rays.shape[0]
+    hits = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+    nc = torch.empty(n, dtype=torch.int32, device="cuda")
+    pc = torch.empty(n, dtype=torch.int32, device="cuda")
+    rumi.trace_count(ctx, bvh, dev(rays), any_hit, hits, nc, pc, flags)
+    torch.cuda.synchronize()
+    p, t = rumi.split_hits(hits.cpu().numpy())
+    return p, t, nc.cpu().numpy(), pc.cpu().numpy()
+
+
+def bits(t):
+    return t.view(np.int32)
+
+
+@pytest.fixture(scope="module")
+def mesh(rumi):
+    return rumi.make_icosphere(12, 1, 0.05)
+
+
+@pytest.fixture(scope="module")
+def rays(rumi):
+    return rumi.make_rays_uniform(40000, 2, 2.1)
+
+
+@pytest.mark.parametrize("max_leaf", [1, 4, 8])
+def test_closest_sah_bitexact_vs_oracle(rumi, oracle, ctx, mesh, rays, max_leaf):
+    xyz, idx = mesh
+    b = rumi.Bvh(ctx, xyz, idx, max_leaf=max_leaf)
+    ob = oracle.MedianBvh.from_export(b.export(), idx.shape[0])
+    p, t = closest(rumi, ctx, b, rays)
+    op, ot = ob.trace_mesh(xyz, idx, rays, ordered=True)
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(bits(t), bits(ot))
+    bp, bt = oracle.brute_mesh(xyz, idx, rays[:5000])
+    assert ((p[:5000] == bp) | tie_ok(t[:5000], bt)).all()
+    assert (p >= 0).sum() > 1000
+
+
+def test_closest_reference_tree_and_order_bitexact(rumi, oracle, ctx, mesh, rays):
+    """REF_MEDIAN6 tree + reference visiting order == the oracle's restatement
+    of BVHAccel::BuildBVH + Travel, including which of two tied triangles wins."""
+    xyz, idx = mesh
+    b = rumi.Bvh(ctx, xyz, idx, mode=rumi.RUMI_BVH_REF_MEDIAN6)
+    lo, hi = oracle.mesh_bounds(xyz, idx)
+    ob = oracle.MedianBvh(lo, hi)
+    p, t = closest(rumi, ctx, b, rays)
+    op, ot = ob.trace_mesh(xyz, idx, rays, ordered=False)
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(bits(t), bits(ot))
+    pa = anyhit(rumi, ctx, b, rays)
+    oa, _ = ob.trace_mesh(xyz, idx, rays, any_hit=True, ordered=False)
+    np.testing.assert_array_equal(pa, oa)
+
+
+@pytest.mark.parametrize("flags,ordered", [(1, False), (2, True)])
+def test_order_override_flags(rumi, oracle, ctx, mesh, rays, flags, ordered):
+    xyz, idx = mesh
+    b = rumi.Bvh(ctx, xyz, idx)
+    ob = oracle.MedianBvh.from_export(b.export(), idx.shape[0])
+    p, t = closest(rumi, ctx, b, rays, flags=flags)
+    op, ot = ob.trace_mesh(xyz, idx, rays, ordered=ordered)
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(bits(t), bits(ot))
+
+
+def test_any_hit_bitexact_and_status(rumi, oracle, ctx, mesh, rays):
+    xyz, idx = mesh
+    b = rumi.Bvh(ctx, xyz, idx)
+    ob = oracle.MedianBvh.from_export(b.export(), idx.shape[0])
+    pa = anyhit(rumi, ctx, b, rays)
+    oa, _ = ob.trace_mesh(xyz, idx, rays, any_hit=True, ordered=True)
+    np.testing.assert_array_equal(pa, oa)
+    pc, _ = closest(rumi, ctx, b, rays)
+    np.testing.assert_array_equal(pa >= 0, pc >= 0)
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_counts_match_oracle(rumi, oracle, ctx, mesh, rays, any_hit):
+    xyz, idx = mesh
+    b = rumi.Bvh(ctx, xyz, idx)
+    ob = oracle.MedianBvh.from_export(b.export(), idx.shape[0])
+    p, t, nc, pc = counts(rumi, ctx, b, rays, any_hit=any_hit)
+    op, ot, onc, opc = ob.trace_mesh(xyz, idx, rays, any_hit=any_hit, ordered=True, counts=True)
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(nc, onc)
+    np.testing.assert_array_equal(pc, opc)
+
+
+def test_barycentrics_reconstruct_hit_point(rumi, ctx, mesh, rays):
+    xyz, idx = mesh
+    b = rumi.Bvh(ctx, xyz, idx)
+    p, t, uv = closest(rumi, ctx, b, rays, uv=True)
+    m = p >= 0
+    tri = idx[p[m]]
+    v0, v1, v2 = (xyz[tri[:, k]].astype(np.float64) for k in range(3))
+    u, v = uv[m, 0:1].astype(np.float64), uv[m, 1:2].astype(np.float64)
+    on_tri = v0 + u * (v1 - v0) + v * (v2 - v0)
+    on_ray = rays[m, :3].astype(np.float64) + t[m, None].astype(np.float64) * rays[m, 4:7]
+    assert np.abs(on_tri - on_ray).max() < 1e-5
+    assert (u >= 0).all() and (v >= 0).all() and (u + v <= 1 + 1e-7).all()
+
+
+def test_edge_cases(rumi, oracle, ctx, mesh, rays):
+    xyz, idx = mesh
+    b = rumi.Bvh(ctx, xyz, idx)
+    ob = oracle.MedianBvh.from_export(b.export(), idx.shape[0])
+    # ragged sizes around the block size
+    for n in (1, 63, 64, 127, 128, 129, 1031):
+        p, t = closest(rumi, ctx, b, rays[:n])
+        op, ot = ob.trace_mesh(xyz, idx, rays[:n], ordered=True)
+        np.testing.assert_array_equal(p, op)
+    # n = 0 is a no-op
+    p, t = closest(rumi, ctx, b, rays[:0])
+    assert p.size == 0
+    # zero direction: BVHAccel::Travel returns -1 (bvh.cpp:161-162)
+    z = rays[:64].copy()
+    z[:, 4:7] = 0
+    p, t = closest(rumi, ctx, b, z)
+    assert (p == -1).all() and (t == 0).all()
+    # finite tmax (shadow segment shorter than the hit) misses; longer finds it
+    p0, t0 = closest(rumi, ctx, b, rays[:4000])
+    seg = rays[:4000].copy()
+    hit = p0 >= 0
+    seg[hit, 7] = t0[hit] * 0.5
+    p1, _ = closest(rumi, ctx, b, seg)
+    assert (p1[hit] == -1).all()
+    a1 = anyhit(rumi, ctx, b, seg)
+    assert (a1[hit] == -1).all()
+    op1, _ = ob.trace_mesh(xyz, idx, seg, ordered=True)
+    np.testing.assert_array_equal(p1, op1)
+    # a caller-supplied "closest so far" (tmin > 0) is honoured
+    pre = rays[:4000].copy()
+    pre[hit, 3] = t0[hit] * np.float32(0.999)
+    p2, t2 = closest(rumi, ctx, b, pre)
+    op2, ot2 = ob.trace_mesh(xyz, idx, pre, ordered=True)
+    np.testing.assert_array_equal(p2, op2)
+    np.testing.assert_array_equal(bits(t2), bits(ot2))
+
+
+def test_empty_and_single_triangle_meshes(rumi, oracle, ctx, rays):
+    xyz = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
+    empty = rumi.Bvh(ctx, xyz, np.zeros((0, 3), np.uint32))
+    p, t = closest(rumi, ctx, empty, rays[:500])
+    assert (p == -1).all()
+    assert (anyhit(rumi, ctx, empty, rays[:500]) == -1).all()
+    one = rumi.Bvh(ctx, xyz, np.array([[0, 1, 2]], np.uint32))
+    bp, bt = oracle.brute_mesh(xyz, np.array([[0, 1, 2]], np.uint32), rays)
+    p, t = closest(rumi, ctx, one, rays)
+    np.testing.assert_array_equal(p, bp)
+    np.testing.assert_array_equal(bits(t), bits(bt))
+
+
+def test_errors_are_reported_not_crashing(rumi, ctx, mesh):
+    import torch
+    xyz, idx = mesh
+    b = rumi.Bvh(ctx, xyz, idx)
+    host_only = rumi.Bvh(None, xyz, idx)
+    r = torch.zeros((8, 8), dtype=torch.float32, device="cuda")
+    h = torch.zeros((8, 2), dtype=torch.int32, device="cuda")
+    with pytest.raises(rumi.RumiError):
+        rumi.trace_closest(ctx, host_only, r, h)            # BVH not on the device
+    with pytest.raises(rumi.RumiError):
+        rumi.trace_closest(ctx, b, r, h, flags=3)           # exclusive flags
+    with pytest.raises(rumi.RumiError):
+        rumi.trace_closest(ctx, b, r, None)                 # null output
+
+
+def test_full_config_icosphere_1p3m_8m_rays(rumi, oracle, ctx):
+    """The 1,310,720-triangle noisy icosphere with 2^23 incoherent rays: the
+    whole batch is checked through size-independent properties, a 65,536-ray
+    slice bit-exactly against the oracle's traversal of the same tree."""
+    import torch
+    xyz, idx = rumi.make_icosphere(256, 1, 0.05)
+    rays = rumi.make_rays_uniform(1 << 23, 2, 2.1)
+    b = rumi.Bvh(ctx, xyz, idx)
+    assert b.info().n_prims == 1310720
+    p, t = closest(rumi, ctx, b, rays)
+    pa = anyhit(rumi, ctx, b, rays)
+    np.testing.assert_array_equal(pa >= 0, p >= 0)
+    m = p >= 0
+    hp = rays[m, :3].astype(np.float64) + t[m, None].astype(np.float64) * rays[m, 4:7]
+    r = np.linalg.norm(hp, axis=1)
+    assert r.min() > 0.95 - 1e-4 and r.max() < 1.05 + 1e-4
+    inside = np.linalg.norm(rays[:, :3].astype(np.float64), axis=1) < 0.95
+    assert m[inside].all()
+    ob = oracle.MedianBvh.from_export(b.export(), idx.shape[0])
+    sl = slice(1 << 22, (1 << 22) + 65536)
+    op, ot = ob.trace_mesh(xyz, idx, rays[sl], ordered=True)
+    np.testing.assert_array_equal(p[sl], op)
+    np.testing.assert_array_equal(bits(t[sl]), bits(ot))
+    oa, _ = ob.trace_mesh(xyz, idx, rays[sl], any_hit=True, ordered=True)
+    np.testing.assert_array_equal(pa[sl], oa)
+    bp, bt = oracle.brute_mesh(xyz, idx, rays[:128])
+    assert ((p[:128] == bp) | tie_ok(t[:128], bt)).all()
+    del b
+    torch.cuda.synchronize()
