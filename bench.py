@@ -41,7 +41,23 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 23)
     ap.add_argument("--cpu-rays", type=int, default=1 << 20,
                     help="rays in the bounded CPU-baseline sample (0 = skip)")
+    ap.add_argument("--no-render", action="store_true", help="skip the path-tracing legs")
+    ap.add_argument("--render-runs", type=int, default=3)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (the "
+                         "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy} MiB x 5")
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -111,6 +127,8 @@ def main():
     achieved_c = b_ray_c * n / (ms_c * 1e-3) / 1e9
     achieved_a = b_ray_a * n / (ms_a * 1e-3) / 1e9
 
+    pt = None if args.no_render else render_legs(rumi, ctx, args, log)
+
     cpu = None
     if args.cpu_rays > 0:
         import oracle
@@ -157,8 +175,87 @@ def main():
             "bytes_per_ray": round(b_ray_a, 1), "nodes_per_ray": round(nodes_a, 3),
             "tris_per_ray": round(tris_a, 3),
         },
+        "path_tracing": pt,
     }
     print(json.dumps(out), flush=True)
+
+
+def render_legs(rumi, ctx, args, log):
+    """Path tracing: the 512^2 x 16 spp Cornell box (depth 5) and the 512^2 x
+    32 spp media scene (depth 16), samples/s on the GPU; rel-MSE against the
+    committed 4096-spp oracle reference at 64^2, GPU vs the oracle at equal
+    spp; CPU baseline = the oracle on one thread over a bounded sample."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "rumi-renderer_amd"))
+    import scenes
+
+    out = {}
+    legs = [("cornell_pt", scenes.cornell_box, 512, 512, 16, 5, "cornell64_pt_d5"),
+            ("media_pt", scenes.cornell_media, 512, 512, 32, 16, "media64_pt_d16")]
+    for name, factory, w, h, spp, depth, golden in legs:
+        sc = factory(w, h)
+        scene = rumi.Scene(ctx, sc)
+        img, st = scene.render(w, h, spp, depth, seed=1)  # warm-up (allocates work buffers)
+        torch.cuda.synchronize()
+        times = []
+        for k in range(args.render_runs):
+            t0 = time.perf_counter()
+            img, st = scene.render(w, h, spp, depth, seed=10 + k)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        sec = min(times)
+        sps = st.samples / sec
+        # rel-MSE at equal spp against the committed 64^2 4096-spp reference:
+        # the GPU's renders here, the CPU's from the baseline leg below.
+        ref = np.load(os.path.join(ROOT, "tests", "golden", golden + ".npz"))
+        sc64 = factory(64, 64)
+        s64 = rumi.Scene(ctx, sc64)
+        g_vals = []
+        for k in range(4):
+            gi, _ = s64.render(64, 64, spp, depth, seed=200 + k)
+            torch.cuda.synchronize()
+            g_vals.append(rel_mse(gi.cpu().numpy(), ref["mean"]))
+        s64.close()
+        # CPU baseline leg: the oracle (FP64 restatement of PathTracer::Trace)
+        # on one thread, 4 renders of 64^2 at the same spp; their images give
+        # the CPU rel-MSE at equal spp.
+        import oracle
+        c_vals = []
+        t0 = time.perf_counter()
+        for k in range(4):
+            ci, _ = oracle.render_pt(sc64, 64, 64, spp, depth, seed=300 + k, nthreads=1)
+            c_vals.append(rel_mse(ci, ref["mean"]))
+        cpu_sec = time.perf_counter() - t0
+        cpu_sps = 4 * 64 * 64 * spp / cpu_sec
+        out[name] = {
+            "samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4),
+            "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
+            "max_depth": depth, "extension_rays": int(st.extension_rays),
+            "shadow_rays": int(st.shadow_rays), "iterations": int(st.iterations),
+            "rays_per_s": round((st.extension_rays + st.shadow_rays) / sec, 1),
+            "rel_mse_64px": {"gpu": float(np.mean(g_vals)), "cpu_oracle": float(np.mean(c_vals)),
+                             "ratio": float(np.mean(g_vals) / np.mean(c_vals)),
+                             "spp": spp, "renders": 4},
+            "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
+                             "kind": "port",
+                             "sample": f"4 renders of 64x64 at {spp} spp, the oracle's FP64 "
+                                       f"restatement of PathTracer::Trace ({cpu_sec:.2f}s)"},
+            "speedup_vs_cpu": round(sps / cpu_sps, 1),
+        }
+        log(f"{name}: {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.1f} ms), "
+            f"rel-MSE ratio {out[name]['rel_mse_64px']['ratio']:.3f}, CPU {cpu_sps:.0f} samples/s")
+        scene.close()
+    return out
+
+
+def rel_mse(img, ref):
+    """mean over pixels and channels of (I - R)^2 / (R^2 + 0.01)."""
+    import numpy as np
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
 
 
 def _cpu_model():

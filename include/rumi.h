@@ -151,6 +151,128 @@ int rumi_trace_count(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* dev_ray
                      uint32_t n, int32_t any, rumi_hit* dev_hits, int32_t* dev_node_count,
                      int32_t* dev_prim_count, uint32_t flags, void* stream);
 
+/* ----------------------------------------------------------------- scenes */
+/* A scene replaces the reference's Scene object graph (src/Scene/scene.h:
+ * 16-53): shapes with materials, area lights, an optional homogeneous medium
+ * and a pinhole camera, uploaded once and rendered many times. */
+typedef struct rumi_scene rumi_scene;
+
+enum {
+    RUMI_SHAPE_TRIANGLE = 0, /* Triangle(v0, v1, v2) + Normal(n0, n1, n2) (src/Shape/triangle.cpp) */
+    RUMI_SHAPE_SPHERE = 1,   /* Sphere(centre, radius) (src/Shape/sphere.cpp) */
+    RUMI_SHAPE_RECT = 2,     /* Rect(p, l, w) (src/Shape/rect.cpp) */
+    RUMI_SHAPE_MESH = 3      /* indexed triangle mesh, one material */
+};
+
+enum {
+    RUMI_MAT_MATTE = 0,      /* Matte(kd, color) (src/Material/matte.cpp) */
+    RUMI_MAT_GLOSSY = 1,     /* Glossy(kd, ks, exponent, color) (src/Material/glossy.cpp) */
+    RUMI_MAT_SPECULAR = 2,   /* Specular(kr = kd, color) (src/Material/specular.cpp) */
+    RUMI_MAT_EMISSIVE = 3,   /* Emissive(ls = kd, color) (src/Material/emissive.cpp) */
+    RUMI_MAT_DIELECTRIC = 4, /* Dielectric(eta_in, eta_out, absorption) (src/Material/dielectric.cpp) */
+    RUMI_MAT_SUBSURFACE = 5  /* Subsurface(eta_in, eta_out, sigma_s, sigma_a, density) (src/Material/subsurface.cpp) */
+};
+
+typedef struct {
+    int32_t kind;        /* RUMI_MAT_* */
+    float kd;            /* Matte/Glossy kd, Specular kr, Emissive ls */
+    float ks;            /* Glossy ks */
+    float exponent;      /* Glossy Phong exponent */
+    float color[3];      /* RGB albedo / specular colour / emitted colour */
+    float eta_in, eta_out;
+    float absorption[3]; /* Dielectric Beer-Lambert coefficients */
+    float sigma_s[3], sigma_a[3];
+    float density;       /* Subsurface sampling density (0 = 1) */
+    int32_t reserved[4];
+} rumi_material;
+
+typedef struct {
+    int32_t kind;       /* RUMI_SHAPE_* */
+    int32_t material;   /* index into rumi_scene_desc.materials */
+    int32_t is_light;   /* 1: an AreaLight on this shape (rect or sphere, emissive material) */
+    int32_t texture;    /* index into rumi_scene_desc.textures or -1 (multiplies radiance,
+                           PathTracer::Trace pathtracer.cpp:343-344) */
+    float v[9];         /* triangle v0 v1 v2 | sphere centre in v[0..2] | rect p, l, w */
+    float n[9];         /* triangle vertex normals (all zero: geometric normal) */
+    float radius;       /* sphere */
+    /* RUMI_SHAPE_MESH (host memory, copied at scene creation): */
+    const float* xyz;   uint32_t nv;
+    const uint32_t* idx; uint32_t nt;
+    const float* normals; /* nv x 3 or NULL (geometric normals) */
+    const float* uv;      /* nv x 2 texture coordinates or NULL */
+} rumi_shape;
+
+typedef struct {
+    float position[3];
+    float basis[9];     /* row-major rotation: world direction = basis * camera direction;
+                           the camera looks along its +z (Pinhole, src/Camera/pinhole.cpp:30-51) */
+    float fov_deg;      /* full field of view; the reference fixes 20 (pinhole.cpp:36) */
+    float y_ratio;      /* image height / width */
+} rumi_camera;
+
+enum { RUMI_MEDIUM_NONE = 0, RUMI_MEDIUM_BOX = 1, RUMI_MEDIUM_SPHERE = 2 };
+
+typedef struct {
+    int32_t bound;      /* RUMI_MEDIUM_*: BOX is HomogeneousVolume(BBox, ...) (src/Volume/
+                           homogeneousvolume.cpp:27-38); SPHERE is an extension */
+    float lo[3], hi[3];
+    float center[3], radius;
+    float sigma_s[3], sigma_a[3];
+    float g;            /* Henyey-Greenstein anisotropy; 0 = the reference's Isotropic phase */
+} rumi_medium;
+
+typedef struct {
+    int32_t width, height;
+    const float* rgb;   /* host, width * height * 3, row 0 = v = 0 */
+} rumi_texture;
+
+typedef struct {
+    const rumi_shape* shapes;       uint32_t n_shapes;
+    const rumi_material* materials; uint32_t n_materials;
+    const rumi_texture* textures;   uint32_t n_textures;
+    rumi_camera camera;
+    rumi_medium medium;
+} rumi_scene_desc;
+
+/* Builds the scene's BVH (binned SAH over all primitives) and uploads it
+ * with the shading tables.  Replaces Scene::Build's object graph + BVHAccel
+ * (src/Scene/scene.cpp:135-204). */
+int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* desc, rumi_scene** out);
+int rumi_scene_destroy(rumi_scene* scene);
+
+/* ---------------------------------------------------------------- render */
+enum { RUMI_INTEGRATOR_PT = 0, RUMI_INTEGRATOR_BDPT = 1 };
+
+typedef struct {
+    int32_t integrator;     /* RUMI_INTEGRATOR_* */
+    int32_t width, height;
+    int32_t spp;
+    int32_t max_depth;      /* PathTracer(scene, max_depth): surface/medium vertices per path */
+    int32_t n_light, n_eye; /* BidirectionalPathTracer(scene, nL, nE) vertex caps */
+    float mis_beta;         /* BDPT MIS exponent: 1 balance (the reference's), 2 power */
+    uint64_t seed;
+    uint32_t queue_capacity;    /* wavefront paths in flight, 0 = automatic */
+    uint32_t regen_threshold;   /* refill the queue when fewer paths are alive, 0 = always */
+} rumi_render_desc;
+
+typedef struct {
+    uint64_t samples;
+    uint64_t extension_rays;
+    uint64_t shadow_rays;
+    uint64_t iterations;
+    double milliseconds;   /* device time between the first and last stage (HIP events) */
+} rumi_render_stats;
+
+/* Renders one frame: Scene::Rendering's pixel x sample loop
+ * (src/Scene/scene.cpp:206-258) over Integrator::Trace(w, h, camera, film)
+ * (PathTracer src/Integrator/pathtracer.cpp:117-354).  dev_rgb receives the
+ * per-pixel mean radiance, height x width x 3 floats RGB, row 0 at the top of
+ * the image as Film::Pixel indexes it (src/Camera/film.cpp:83-97).  Samples
+ * whose radiance is NaN are dropped, as Scene::Rendering drops them.
+ * stats may be NULL.  Synchronises the stream before returning. */
+int rumi_render(rumi_ctx* ctx, const rumi_scene* scene, const rumi_render_desc* desc,
+                float* dev_rgb, rumi_render_stats* stats, void* stream);
+
 /* ------------------------------------------------------------ workloads */
 /* Seeded geodesic icosphere of frequency f (20 f^2 triangles, 10 f^2 + 2
  * vertices) with each vertex at radius 1 + amplitude * (2u - 1), u from

@@ -244,3 +244,107 @@ def time_median6_mesh(xyz, idx, rays, any_hit=False):
     secs = lib().or_time_median6_mesh(_p(xyz), _p(idx), idx.shape[0], _p(rays), n,
                                       int(any_hit), _p(prim), _p(t), byref(bs))
     return prim, t, secs, bs.value
+
+
+# ------------------------------------------------------------------ renderer
+MAT_KINDS = {"matte": 0, "glossy": 1, "specular": 2, "emissive": 3, "dielectric": 4,
+             "subsurface": 5}
+
+
+def scene_arrays(scene: dict):
+    """The scene dict (rumi-renderer_amd/scenes.py form) as the oracle's flat
+    records: shapes (n, 32), materials (m, 32), camera (16,), medium (16,) or None."""
+    rows = []
+    for s in scene["shapes"]:
+        k = s["kind"]
+        if k == "mesh":
+            xyz = np.asarray(s["xyz"], np.float32)
+            idx = np.asarray(s["idx"], np.int64)
+            nrm = s.get("normals")
+            recs = np.zeros((idx.shape[0], 32), np.float32)
+            recs[:, 0] = OR_TRI
+            recs[:, 1] = s["material"]
+            recs[:, 3:12] = xyz[idx].reshape(-1, 9)
+            if nrm is not None:
+                recs[:, 13:22] = np.asarray(nrm, np.float32)[idx].reshape(-1, 9)
+            else:
+                e1 = xyz[idx[:, 1]] - xyz[idx[:, 0]]
+                e2 = xyz[idx[:, 2]] - xyz[idx[:, 0]]
+                c = np.cross(e1.astype(np.float64), e2.astype(np.float64))
+                c /= np.linalg.norm(c, axis=1, keepdims=True)
+                recs[:, 13:22] = np.tile(c, (1, 3)).astype(np.float32)
+            rows.append(recs)
+            continue
+        r = np.zeros((1, 32), np.float32)
+        r[0, 1] = s["material"]
+        r[0, 2] = 1.0 if s.get("light") else 0.0
+        if k == "tri":
+            r[0, 0] = OR_TRI
+            r[0, 3:12] = np.asarray(s["v"], np.float32).reshape(9)
+            r[0, 13:22] = np.asarray(s["n"], np.float32).reshape(9)
+        elif k == "sphere":
+            r[0, 0] = OR_SPHERE
+            r[0, 3:6] = s["center"]
+            r[0, 12] = s["radius"]
+        elif k == "rect":
+            r[0, 0] = OR_RECT
+            r[0, 3:6] = s["p"]
+            r[0, 6:9] = s["l"]
+            r[0, 9:12] = s["w"]
+        rows.append(r)
+    shapes = np.ascontiguousarray(np.concatenate(rows, 0))
+    mats = np.zeros((len(scene["materials"]), 32), np.float32)
+    for i, m in enumerate(scene["materials"]):
+        mats[i, 0] = MAT_KINDS[m["kind"]]
+        mats[i, 1] = m.get("kd", 0.0)
+        mats[i, 2] = m.get("ks", 0.0)
+        mats[i, 3] = m.get("exponent", 0.0)
+        mats[i, 4:7] = m.get("color", (1, 1, 1))
+        mats[i, 7] = m.get("eta_in", 1.0)
+        mats[i, 8] = m.get("eta_out", 1.0)
+        mats[i, 9:12] = m.get("absorption", (0, 0, 0))
+        mats[i, 12:15] = m.get("sigma_s", (0, 0, 0))
+        mats[i, 15:18] = m.get("sigma_a", (0, 0, 0))
+        mats[i, 18] = m.get("density", 1.0)
+    cam = scene["camera"]
+    c = np.zeros(16, np.float32)
+    c[0:3] = cam["position"]
+    c[3:12] = np.asarray(cam["basis"], np.float32).reshape(9)
+    c[12] = np.float32(np.tan(np.pi / np.float32(180.0) * np.float64(cam["fov_deg"]) * 0.5))
+    c[13] = cam.get("y_ratio", 1.0)
+    med = None
+    if scene.get("medium"):
+        md = scene["medium"]
+        med = np.zeros(16, np.float32)
+        med[0] = {"box": 1, "sphere": 2}[md["bound"]]
+        if md["bound"] == "box":
+            med[1:4] = md["lo"]
+            med[4:7] = md["hi"]
+        else:
+            med[1:4] = md["center"]
+            med[7] = md["radius"]
+        med[8:11] = md["sigma_s"]
+        med[11:14] = md["sigma_a"]
+        med[14] = md.get("g", 0.0)
+    return shapes, mats, c, med
+
+
+def render_pt(scene: dict, width: int, height: int, spp: int, max_depth: int, seed: int = 0,
+              nthreads: int = 1):
+    """FP64 restatement of PathTracer::Trace over the scene; returns the
+    per-pixel mean radiance and mean of squares, (height, width, 3) float64."""
+    L = lib()
+    if not hasattr(L, "_render_bound"):
+        L.or_render_pt.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,
+                                   c_int, c_int, c_int, ctypes.c_uint64, c_int, c_void_p,
+                                   c_void_p]
+        L._render_bound = True
+    shapes, mats, cam, med = scene_arrays(scene)
+    mean = np.zeros((height, width, 3), np.float64)
+    m2 = np.zeros((height, width, 3), np.float64)
+    rc = L.or_render_pt(_p(shapes), shapes.shape[0], _p(mats), mats.shape[0], _p(cam),
+                        _p(med) if med is not None else None, width, height, spp, max_depth,
+                        seed, nthreads, _p(mean), _p(m2))
+    if rc != 0:
+        raise RuntimeError(f"or_render_pt failed ({rc})")
+    return mean, m2

@@ -147,6 +147,28 @@ double or_time_median6_mesh(const float* xyz, const uint32_t* idx, int nt,
                             const float* rays, int n, int any,
                             int32_t* prim_out, float* t_out, double* build_seconds);
 
+/* ------------------------------------------------------------------ */
+/* FP64 path tracer (oracle_render.c): PathTracer::Trace(w, h, ...)
+ * (src/Integrator/pathtracer.cpp:117-354) over a flat scene description.
+ *   shapes    n_shapes x 32 floats: [0] kind (OR_TRI/OR_SPHERE/OR_RECT),
+ *             [1] material, [2] area-light flag, [3..11] a,b,c (tri v0 v1 v2;
+ *             sphere centre in a; rect p, l, w), [12] radius, [13..21] tri
+ *             vertex normals n0 n1 n2.
+ *   mats      n_mats x 32 floats: [0] kind (0 matte, 1 glossy, 2 specular,
+ *             3 emissive, 4 dielectric, 5 subsurface), [1] kd (specular kr,
+ *             emissive ls), [2] ks, [3] Phong exponent, [4..6] rgb, [7] eta_in,
+ *             [8] eta_out, [9..11] absorption, [12..14] sigma_s,
+ *             [15..17] sigma_a, [18] density.
+ *   camera    16 floats: position, row-major 3x3 basis, tan(fov/2), y_ratio.
+ *   medium    16 floats or NULL: [0] 0 none / 1 box / 2 sphere, [1..3] lo or
+ *             centre, [4..6] hi, [7] radius, [8..10] sigma_s, [11..13]
+ *             sigma_a, [14] Henyey-Greenstein g (0 = the reference's isotropic).
+ * Writes the per-pixel mean (and mean of squares when m2_rgb != NULL), RGB,
+ * row-major, y = 0 at the top row as the reference's film indexes it. */
+int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                 const float* camera, const float* medium, int width, int height, int spp,
+                 int max_depth, uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,1136 @@
+/*
+ * oracle_render.c -- FP64 CPU restatement of Rumi-Renderer's path tracer,
+ * materials, area lights, homogeneous medium and subsurface walk.
+ * TEST INFRASTRUCTURE ONLY (the checker; never linked into the product).
+ *
+ * The per-sample algorithm follows PathTracer::Trace(w, h, ...)
+ * (src/Integrator/pathtracer.cpp:117-354) literally: a forward loop that
+ * records one Path per vertex, then a backward loop that evaluates the
+ * direct lighting and accumulates L = direct + vol * fr * L * cos / pdf.
+ * All arithmetic is double precision.  Randomness: one PCG32 stream per
+ * pixel, uniform doubles in [0, 1) (the reference draws rand()/RAND_MAX in
+ * [0, 1]); pixel jitter and light points are drawn continuously instead of
+ * from the reference's fixed stratified point sets.
+ *
+ * Extensions beyond the reference (flags in the medium record): a
+ * sphere-bounded medium and a Henyey-Greenstein phase function; with the
+ * box bound and g = 0 the reference's code paths are taken.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI_D 3.14159265358979323846
+#define INV_PI_D 0.3183098865475128
+#define EPS_D 1e-4
+
+typedef struct {
+    double x, y, z;
+} v3;
+
+static inline v3 mk(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 mul(v3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 u, v3 v)
+{
+    return mk(u.y * v.z - v.y * u.z, u.z * v.x - v.z * u.x, u.x * v.y - v.x * u.y);
+}
+static inline double len(v3 a) { return sqrt(dot(a, a)); }
+static inline v3 norm(v3 a) { double il = 1.0 / sqrt(dot(a, a)); return mul(a, il); }
+static inline double vget(v3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+typedef struct {
+    double r, g, b;
+} col;
+static inline col cmk(double r, double g, double b) { col c = {r, g, b}; return c; }
+static inline col cs(double s) { return cmk(s, s, s); }
+static inline col cadd(col a, col b) { return cmk(a.r + b.r, a.g + b.g, a.b + b.b); }
+static inline col cmul(col a, col b) { return cmk(a.r * b.r, a.g * b.g, a.b * b.b); }
+static inline col cscale(col a, double s) { return cmk(a.r * s, a.g * s, a.b * s); }
+static inline col cdiv(col a, col b) { return cmk(a.r / b.r, a.g / b.g, a.b / b.b); }
+static inline double cavg(col a) { return (a.r + a.g + a.b) / 3.0; }
+static inline int cnan(col a) { return a.r != a.r || a.g != a.g || a.b != a.b; }
+static inline int cis0(col a) { return a.r == 0.0 && a.g == 0.0 && a.b == 0.0; }
+
+/* ------------------------------------------------------------------ RNG */
+typedef struct {
+    uint64_t state, inc;
+} pcg_t;
+
+static uint32_t pcg_next(pcg_t* g)
+{
+    uint64_t old = g->state;
+    g->state = old * 6364136223846793005ULL + g->inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+
+static void pcg_seed(pcg_t* g, uint64_t initstate, uint64_t initseq)
+{
+    g->state = 0;
+    g->inc = (initseq << 1u) | 1u;
+    pcg_next(g);
+    g->state += initstate;
+    pcg_next(g);
+}
+
+static inline double rnd(pcg_t* g) { return (double)pcg_next(g) * (1.0 / 4294967296.0); }
+
+/* ------------------------------------------------------------------ scene */
+enum { MAT_MATTE = 0, MAT_GLOSSY = 1, MAT_SPECULAR = 2, MAT_EMISSIVE = 3,
+       MAT_DIELECTRIC = 4, MAT_SUBSURFACE = 5 };
+
+typedef struct {
+    int kind;
+    double kd, ks, e, kd_pdf, phong_norm;
+    col color;
+    double eta_in, eta_out;
+    col absorption;
+    col sigma_s, sigma_a, sigma_t;
+    double avg_sigmat, absorb;
+    int opaque;
+} mat_t;
+
+typedef struct {
+    int kind; /* OR_TRI / OR_SPHERE / OR_RECT */
+    int mat;
+    int light;
+    v3 a, b, c;       /* tri v0 v1 v2; sphere centre a; rect p l w */
+    double r;
+    v3 n0, n1, n2;    /* tri vertex normals */
+    v3 n;             /* rect normal */
+    double length, width, inv_area;
+    double lo[3], hi[3];
+} shape_t;
+
+typedef struct {
+    int kind; /* 0 none, 1 box (reference), 2 sphere */
+    double lo[3], hi[3];
+    v3 c;
+    double r;
+    col sigma_s, sigma_a, sigma_t;
+    double avg_sigma_t, absorb, g;
+} medium_t;
+
+typedef struct {
+    v3 p;
+    double R[9];
+    double tanfov, y_ratio;
+} camera_t;
+
+typedef struct {
+    int first, count, left, right;
+    double lo[3], hi[3];
+} bnode_t;
+
+typedef struct {
+    shape_t* shapes;
+    int n_shapes;
+    mat_t* mats;
+    int n_mats;
+    int* lights;
+    int n_lights;
+    medium_t med;
+    camera_t cam;
+    bnode_t* nodes;
+    int n_nodes;
+    int* order;
+} scene_t;
+
+/* Ray + hit record, the reference's Ray/Intersection pair. */
+typedef struct {
+    v3 o, d;
+    double tmin, tmax;
+} ray_t;
+
+/* --------------------------------------------------------- intersectors */
+static int hit_tri(const shape_t* s, ray_t* ray, double* u_out, double* v_out, double* det_out)
+{
+    v3 e1 = sub(s->b, s->a), e2 = sub(s->c, s->a);
+    v3 p = cross(ray->d, e2);
+    double det = dot(p, e1);
+    if (-1e-8 < det && det < 1e-8) return 0;
+    double inv = 1.0 / det;
+    v3 tv = sub(ray->o, s->a);
+    double u = dot(p, tv) * inv;
+    if (u < 0.0 || u > 1.0) return 0;
+    v3 q = cross(tv, e1);
+    double v = dot(q, ray->d) * inv;
+    if (v < 0.0 || u + v > 1.0) return 0;
+    double t = dot(q, e2) * inv;
+    if (t < EPS_D) return 0;
+    if (ray->tmin > EPS_D && t > ray->tmin) return 0;
+    if (ray->tmax - t < EPS_D) return 0;
+    ray->tmin = t;
+    *u_out = u;
+    *v_out = v;
+    *det_out = det;
+    return 1;
+}
+
+static int hit_sphere(const shape_t* s, ray_t* ray, int in_object)
+{
+    v3 oc = sub(ray->o, s->a);
+    double B = 2.0 * dot(ray->d, oc);
+    double C = dot(oc, oc) - s->r * s->r;
+    double disc = B * B - 4.0 * C;
+    if (disc < 0.0) return 0;
+    double t0, t1;
+    if (disc == 0.0) {
+        t0 = t1 = -B * 0.5;
+    } else {
+        double sq = sqrt(disc);
+        t0 = (-B - sq) * 0.5;
+        t1 = (-B + sq) * 0.5;
+    }
+    if (t0 < 0.0 && t1 < 0.0) return 0;
+    if (in_object) {
+        double tmp = t0;
+        t0 = t1;
+        t1 = tmp;
+    }
+    if (t0 < EPS_D) return 0;
+    if (ray->tmax - t0 < EPS_D) return 0;
+    if (t0 > ray->tmin && ray->tmin > 0.0) return 0;
+    ray->tmin = t0;
+    return 1;
+}
+
+static int hit_rect(const shape_t* q, const mat_t* m, ray_t* ray)
+{
+    double cost = dot(ray->d, q->n);
+    if (cost == 0.0) return 0;
+    double t = -dot(sub(ray->o, q->a), q->n) / cost;
+    if (t < 0.0 || t < EPS_D) return 0;
+    if (ray->tmax - t < EPS_D) return 0;
+    if (m->kind != MAT_EMISSIVE && (0.0 < cost && cost < 1.0)) return 0;
+    if (t > ray->tmin && ray->tmin > 0.0) return 0;
+    v3 ip = add(ray->o, mul(ray->d, t));
+    v3 onp = sub(ip, q->a);
+    double il = dot(onp, q->b) * (q->inv_area * q->width);
+    if (il < 0.0 || il > q->length) return 0;
+    double iw = dot(onp, q->c) * (q->inv_area * q->length);
+    if (iw < 0.0 || iw > q->width) return 0;
+    ray->tmin = t;
+    return 1;
+}
+
+typedef struct {
+    int obj;
+    double u, v, det;
+} hitrec_t;
+
+static int hit_prim(const scene_t* sc, int i, ray_t* ray, int in_object, hitrec_t* h)
+{
+    const shape_t* s = &sc->shapes[i];
+    double u = 0, v = 0, det = 0;
+    int ok = 0;
+    if (s->kind == OR_TRI)
+        ok = hit_tri(s, ray, &u, &v, &det);
+    else if (s->kind == OR_SPHERE)
+        ok = hit_sphere(s, ray, in_object);
+    else
+        ok = hit_rect(s, &sc->mats[s->mat], ray);
+    if (ok && h) {
+        h->obj = i;
+        h->u = u;
+        h->v = v;
+        h->det = det;
+    }
+    return ok;
+}
+
+static int slab(const double* lo, const double* hi, const ray_t* r, double* tn, double* tf)
+{
+    double t0, t1, tmp;
+    int k;
+    *tn = -INFINITY;
+    *tf = INFINITY;
+    for (k = 0; k < 3; ++k) {
+        double inv = 1.0 / vget(r->d, k);
+        double o = vget(r->o, k);
+        t0 = (lo[k] - o) * inv;
+        t1 = (hi[k] - o) * inv;
+        if (t0 > t1) { tmp = t0; t0 = t1; t1 = tmp; }
+        if (k == 0) {
+            *tn = t0;
+            *tf = t1;
+            continue;
+        }
+        if (*tn > t1 || t0 > *tf) return 0;
+        if (*tn < t0) *tn = t0;
+        if (*tf > t1) *tf = t1;
+    }
+    return 1;
+}
+
+/* Closest hit (any = 0) or shadow query (any = 1): BVHAccel::Travel. */
+static int travel(const scene_t* sc, ray_t* ray, int any, int in_object, hitrec_t* h)
+{
+    int stack[128], sp = 0, found = -1;
+    if (sc->n_nodes == 0) return -1;
+    if (ray->d.x == 0.0 && ray->d.y == 0.0 && ray->d.z == 0.0) return -1;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const bnode_t* nd = &sc->nodes[stack[--sp]];
+        double tn, tf;
+        if (!(slab(nd->lo, nd->hi, ray, &tn, &tf) && tf >= 0.0 && ray->tmax >= tn &&
+              (ray->tmin == 0.0 || ray->tmin > tn)))
+            continue;
+        if (nd->left < 0) {
+            int i;
+            for (i = nd->first; i < nd->first + nd->count; ++i)
+                if (hit_prim(sc, sc->order[i], ray, in_object, any ? 0 : h)) found = sc->order[i];
+            if (any && ray->tmin > 0.0) return found;
+        } else {
+            stack[sp++] = nd->right;
+            stack[sp++] = nd->left;
+        }
+    }
+    return found;
+}
+
+static void build_bvh(scene_t* sc)
+{
+    int n = sc->n_shapes, i;
+    sc->order = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+    for (i = 0; i < n; ++i) sc->order[i] = i;
+    sc->nodes = (bnode_t*)malloc(sizeof(bnode_t) * (2 * (size_t)n + 1));
+    sc->n_nodes = 0;
+    if (n == 0) return;
+    /* explicit work stack: (node, first, count) */
+    int* work = (int*)malloc(sizeof(int) * 3 * (2 * (size_t)n + 1));
+    int wp = 0;
+    sc->nodes[0].first = 0;
+    sc->nodes[0].count = n;
+    sc->n_nodes = 1;
+    work[wp++] = 0;
+    while (wp > 0) {
+        int ni = work[--wp];
+        bnode_t* nd = &sc->nodes[ni];
+        int f = nd->first, c = nd->count, k;
+        for (k = 0; k < 3; ++k) {
+            nd->lo[k] = INFINITY;
+            nd->hi[k] = -INFINITY;
+        }
+        double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (i = f; i < f + c; ++i) {
+            const shape_t* s = &sc->shapes[sc->order[i]];
+            for (k = 0; k < 3; ++k) {
+                double m = 0.5 * (s->lo[k] + s->hi[k]);
+                if (s->lo[k] < nd->lo[k]) nd->lo[k] = s->lo[k];
+                if (s->hi[k] > nd->hi[k]) nd->hi[k] = s->hi[k];
+                if (m < clo[k]) clo[k] = m;
+                if (m > chi[k]) chi[k] = m;
+            }
+        }
+        nd->left = nd->right = -1;
+        if (c <= 4) continue;
+        int ax = 0;
+        for (k = 1; k < 3; ++k)
+            if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+        double mid = 0.5 * (clo[ax] + chi[ax]);
+        int l = f, r = f + c - 1;
+        while (l <= r) {
+            const shape_t* s = &sc->shapes[sc->order[l]];
+            if (0.5 * (s->lo[ax] + s->hi[ax]) < mid) {
+                ++l;
+            } else {
+                int t = sc->order[l];
+                sc->order[l] = sc->order[r];
+                sc->order[r] = t;
+                --r;
+            }
+        }
+        int nl = l - f;
+        if (nl == 0 || nl == c) nl = c / 2;
+        int li = sc->n_nodes++, ri = sc->n_nodes++;
+        nd = &sc->nodes[ni];
+        nd->left = li;
+        nd->right = ri;
+        sc->nodes[li].first = f;
+        sc->nodes[li].count = nl;
+        sc->nodes[ri].first = f + nl;
+        sc->nodes[ri].count = c - nl;
+        work[wp++] = ri;
+        work[wp++] = li;
+    }
+    free(work);
+}
+
+/* ------------------------------------------------------- shading helpers */
+/* OrthonormalBasis (src/Kernel/vector.h:70-95). */
+static void basis(v3 v0, v3* v1, v3* v2)
+{
+    if (v0.x == 0.0 && v0.z == 0.0 && (v0.y == 1.0 || v0.y == -1.0)) {
+        *v1 = mk(1, 0, 0);
+        *v2 = v0.y == 0.0 ? mk(0, 0, 1) : mk(0, 0, -1);
+    } else {
+        v3 c = cross(v0, mk(0, 1, 0));
+        *v1 = norm(c);
+        if (v1->x != v1->x || v1->y != v1->y || v1->z != v1->z) *v1 = mk(1, 0, 0);
+        *v2 = cross(v0, *v1);
+    }
+}
+
+static v3 frame(v3 u, v3 v, v3 w, double su, double sv, double sw)
+{
+    return add(add(mul(u, su), mul(v, sv)), mul(w, sw));
+}
+
+/* Lambertian::BSDFSampling (src/Kernel/bsdf.cpp:43-69). */
+static v3 lambert_sample(v3 n, pcg_t* g, double* cost, double* pdf)
+{
+    v3 u, w;
+    basis(n, &u, &w);
+    double phi = rnd(g) * 2.0 * PI_D;
+    double theta = acos(1.0 - rnd(g) * 2.0) * 0.5;
+    double st = sin(theta);
+    v3 wi = frame(u, n, w, st * sin(phi), cos(theta), st * cos(phi));
+    *cost = dot(wi, n);
+    if (*cost <= 0.0) *cost = 1e-4;
+    *pdf = INV_PI_D * *cost;
+    if (*pdf <= 0.0) *pdf = 1e-4;
+    return wi;
+}
+
+/* Phong::F (bsdf.cpp:91-104), white lobe. */
+static double phong_f(const mat_t* m, v3 n, v3 wi, v3 wo)
+{
+    double c = dot(wi, n);
+    if (c <= 0.0) c = 1e-4;
+    v3 r = add(neg(wi), mul(n, c * 2.0));
+    double ca = pow(dot(wo, r), m->e);
+    return ca >= 1e-4 ? m->phong_norm * ca : m->phong_norm * 1e-4;
+}
+
+/* Phong::BSDFSampling (bsdf.cpp:127-170): the returned cosine is 1. */
+static v3 phong_sample(const mat_t* m, v3 wo, v3 n, pcg_t* g, double* cost, double* pdf)
+{
+    double won = dot(wo, n);
+    if (won <= 0.0) won = 1e-4;
+    v3 v = add(neg(wo), mul(n, won * 2.0));
+    v3 u, w;
+    basis(v, &u, &w);
+    double phi = rnd(g) * 2.0 * PI_D;
+    double theta = acos(pow(rnd(g), 1.0 / (1.0 + m->e)));
+    double su = sin(theta) * sin(phi), sv = cos(theta), sw = sin(theta) * cos(phi);
+    v3 wi = frame(u, v, w, su, sv, sw);
+    *cost = dot(wi, n);
+    if (*cost < 0.0) {
+        *cost = -*cost;
+        wi = frame(u, v, w, -su, sv, -sw);
+    }
+    if (*cost == 0.0) *cost = 1e-4;
+    double ca = pow(dot(wi, v), m->e);
+    *pdf = ca >= 1e-4 ? m->phong_norm * ca : m->phong_norm * 1e-4;
+    if (*pdf <= 0.0) *pdf = 1e-4;
+    *cost = 1.0;
+    return wi;
+}
+
+/* FresnelReflection::Fresnel (bsdf.cpp:255-283). */
+static double fresnel(v3 dir, v3 n, double eta_in, double eta_out)
+{
+    double eta_i = eta_out, eta_t = eta_in, ci = -dot(dir, n);
+    if (ci < 0.0) {
+        eta_i = eta_in;
+        eta_t = eta_out;
+        ci = -ci;
+    }
+    double inv_eta = 1.0 / (eta_t / eta_i);
+    double ct = 1.0 - (1.0 - ci * ci) * (inv_eta * inv_eta);
+    if (ct <= 0.0) return 1.0;
+    ct = sqrt(ct);
+    double rs = (eta_i * ci - eta_t * ct) / (eta_i * ci + eta_t * ct);
+    double rp = (eta_i * ct - eta_t * ci) / (eta_i * ct + eta_t * ci);
+    return (rs * rs + rp * rp) * 0.5;
+}
+
+/* FresnelReflection::BSDFSampling (bsdf.cpp:232-253). */
+static v3 fresnel_reflect(v3 wo, v3 n, double* cost)
+{
+    *cost = dot(wo, n);
+    double won = *cost;
+    if (won < 1e-4) {
+        if (won > -1e-4)
+            won = 1e-4;
+        else {
+            n = neg(n);
+            won = -won;
+        }
+    }
+    return add(neg(wo), mul(n, 2.0 * won));
+}
+
+/* FresnelTransparency::BSDFSampling (bsdf.cpp:322-342). */
+static v3 fresnel_transmit(const mat_t* m, v3 wo, v3 n, double* cost)
+{
+    double eta = m->eta_in / m->eta_out;
+    double ci = *cost = dot(wo, n);
+    if (ci < 0.0) {
+        n = neg(n);
+        eta = m->eta_out / m->eta_in;
+        ci = -ci;
+    }
+    double inv = 1.0 / eta;
+    double ct = sqrt(1.0 - (1.0 - ci * ci) * (inv * inv));
+    return add(mul(neg(wo), inv), mul(n, ci * inv - ct));
+}
+
+/* ------------------------------------------------------------ medium */
+static int med_intersect(const medium_t* md, const ray_t* r, double* tn, double* tf)
+{
+    *tn = 0.0;
+    *tf = 0.0;
+    if (md->kind == 1) {
+        if (!slab(md->lo, md->hi, r, tn, tf)) return 0;
+    } else {
+        v3 oc = sub(r->o, md->c);
+        double B = dot(r->d, oc), C = dot(oc, oc) - md->r * md->r;
+        double A = dot(r->d, r->d);
+        double disc = B * B - A * C;
+        if (disc < 0.0) return 0;
+        double sq = sqrt(disc);
+        *tn = (-B - sq) / A;
+        *tf = (-B + sq) / A;
+    }
+    if (*tn != *tn || *tf != *tf) return 0;
+    if (0.0 < r->tmin && r->tmin < *tn) return 0;
+    if (*tf < 0.0) return 0;
+    return 1;
+}
+
+static double med_maxdist(double thit, double tn, double tf)
+{
+    if (tn < 0.0) {
+        if (thit == 0.0) return tf;
+        return thit < tf ? thit : tf;
+    }
+    if (thit == 0.0) return tf - tn;
+    return thit < tf ? thit - tn : tf - tn;
+}
+
+static col exp_tau(col st, double d)
+{
+    return cmk(exp(st.r * -d), exp(st.g * -d), exp(st.b * -d));
+}
+
+/* HomogeneousVolume::DistanceSampling (homogeneousvolume.cpp:168-215) and
+ * Subsurface::DistanceSampling (subsurface.cpp:49-99): returns 1 when a
+ * scattering distance s was sampled; *absorbed when the roulette killed it. */
+static int distance_sample(col sigma_t, double avg, double absorb, double smax, pcg_t* g,
+                           double* s, int* absorbed, col* tau, col* pdf)
+{
+    double xi = rnd(g);
+    if (xi >= 1.0 - exp(avg * -smax)) {
+        *tau = exp_tau(sigma_t, smax);
+        if (cis0(*tau)) *tau = cs(1e-4);
+        *pdf = *tau;
+        return 0;
+    }
+    double ds = -log(1.0 - xi) / avg;
+    if (smax - ds < 1e-4) {
+        *tau = exp_tau(sigma_t, smax);
+        if (cis0(*tau)) *tau = cs(1e-4);
+        *pdf = *tau;
+        return 0;
+    }
+    if (rnd(g) < absorb || ds < 1e-4) {
+        *s = -1.0;
+        *absorbed = 1;
+        return 0;
+    }
+    *s = ds;
+    *tau = exp_tau(sigma_t, ds);
+    *pdf = cmul(*tau, sigma_t);
+    if (cis0(*pdf)) *pdf = cs(1e-4);
+    return 1;
+}
+
+/* Henyey-Greenstein value for the cosine between propagation d and wi. */
+static double hg_eval(double g, double cos_dw)
+{
+    double den = 1.0 + g * g - 2.0 * g * cos_dw;
+    return (1.0 - g * g) / (4.0 * PI_D * den * sqrt(den));
+}
+
+/* Isotropic::IsotropicSampling (phase.cpp:40-56) for g = 0; HG otherwise. */
+static v3 phase_sample(double g, v3 d, pcg_t* gen, double* pdf)
+{
+    v3 v = d, u, w;
+    basis(v, &u, &w);
+    double phi = rnd(gen) * 2.0 * PI_D;
+    double ct;
+    if (g == 0.0) {
+        ct = cos(acos(rnd(gen) * 2.0 - 1.0));
+        *pdf = 1.0 / (4.0 * PI_D);
+    } else {
+        double x = rnd(gen);
+        double sq = (1.0 - g * g) / (1.0 - g + 2.0 * g * x);
+        ct = (1.0 + g * g - sq * sq) / (2.0 * g);
+        if (ct > 1.0) ct = 1.0;
+        if (ct < -1.0) ct = -1.0;
+        *pdf = hg_eval(g, ct);
+    }
+    double st = sqrt(1.0 - ct * ct);
+    return frame(u, v, w, st * sin(phi), ct, st * cos(phi));
+}
+
+/* ------------------------------------------------------------ lights */
+typedef struct {
+    v3 p, n;
+    double pdf; /* (1 / n_lights) * (1 / area) */
+    col le;
+} lsample_t;
+
+static void light_sample(const scene_t* sc, pcg_t* g, lsample_t* ls)
+{
+    int li = (int)(rnd(g) * sc->n_lights);
+    if (li >= sc->n_lights) li = sc->n_lights - 1;
+    const shape_t* s = &sc->shapes[sc->lights[li]];
+    const mat_t* m = &sc->mats[s->mat];
+    double area;
+    if (s->kind == OR_RECT) {
+        double a = rnd(g), b = rnd(g);
+        ls->p = add(add(s->a, mul(s->b, a)), mul(s->c, b));
+        ls->n = s->n;
+        area = s->length * s->width;
+    } else {
+        double z = 1.0 - 2.0 * rnd(g), phi = 2.0 * PI_D * rnd(g);
+        double r = sqrt(1.0 - z * z > 0 ? 1.0 - z * z : 0.0);
+        v3 dir = mk(r * cos(phi), r * sin(phi), z);
+        ls->p = add(s->a, mul(dir, s->r));
+        ls->n = norm(sub(ls->p, s->a));
+        area = 4.0 * PI_D * s->r * s->r;
+    }
+    ls->pdf = (1.0 / sc->n_lights) * (1.0 / area);
+    ls->le = cscale(m->color, m->kd);
+}
+
+/* HomogeneousVolume::ShadowRayTr (homogeneousvolume.cpp:56-93): a grey
+ * Bernoulli transmittance, 0 or 1. */
+static double shadow_tr(const scene_t* sc, const ray_t* shadow, pcg_t* g)
+{
+    const medium_t* md = &sc->med;
+    double tn, tf;
+    if (md->kind == 0) return 1.0;
+    if (!med_intersect(md, shadow, &tn, &tf)) return 1.0;
+    double smax = med_maxdist(shadow->tmax, tn, tf);
+    double xi = rnd(g);
+    if (xi < 1.0 - exp(md->avg_sigma_t * -smax)) return 0.0;
+    return 1.0;
+}
+
+/* Scene::VisibilityTest (src/Scene/scene.cpp:99-123). */
+static int visibility(const scene_t* sc, v3 x, pcg_t* g, lsample_t* ls, v3* wi, ray_t* shadow)
+{
+    light_sample(sc, g, ls);
+    *wi = norm(sub(ls->p, x));
+    shadow->o = x;
+    shadow->d = *wi;
+    shadow->tmin = 0.0;
+    shadow->tmax = len(sub(ls->p, x));
+    if (dot(shadow->d, ls->n) >= 0.0) return 0;
+    ray_t r = *shadow;
+    travel(sc, &r, 1, 0, 0);
+    if (r.tmin > 0.0) return 0;
+    return 1;
+}
+
+/* AreaLight::G (arealight.cpp:106-112). */
+static double geom(v3 x, const lsample_t* ls, v3 wi)
+{
+    double d = len(sub(x, ls->p));
+    double c = -dot(wi, ls->n);
+    if (c < 0.0) c = 0.0;
+    return c / (d * d);
+}
+
+/* Matte::DirectLighting (matte.cpp:54-82). */
+static col matte_direct(const scene_t* sc, const mat_t* m, v3 p, v3 n, pcg_t* g)
+{
+    lsample_t ls;
+    v3 wi;
+    ray_t sh;
+    if (!visibility(sc, p, g, &ls, &wi, &sh)) return cs(0.0);
+    col f = cscale(m->color, INV_PI_D * m->kd);
+    double c = dot(wi, n);
+    if (c < 0.0) c = 0.0;
+    col lo = cscale(cmul(f, ls.le), c * geom(p, &ls, wi) / ls.pdf);
+    if (sc->med.kind) lo = cscale(lo, shadow_tr(sc, &sh, g));
+    return lo;
+}
+
+/* HomogeneousVolume::DirectLighting (homogeneousvolume.cpp:231-250), with
+ * the HG value in place of 1/(4 pi) when g != 0. */
+static col medium_direct(const scene_t* sc, v3 p, v3 d, pcg_t* g)
+{
+    lsample_t ls;
+    v3 wi;
+    ray_t sh;
+    if (!visibility(sc, p, g, &ls, &wi, &sh)) return cs(0.0);
+    double tau = shadow_tr(sc, &sh, g);
+    double fp = sc->med.g == 0.0 ? 1.0 / (4.0 * PI_D) : hg_eval(sc->med.g, dot(d, wi));
+    return cscale(cmul(ls.le, sc->med.sigma_s), tau * fp * geom(p, &ls, wi) / ls.pdf);
+}
+
+/* ------------------------------------------------------------ path tracer */
+typedef struct {
+    /* vertex */
+    v3 p, n;
+    double costheta_i, pdf_path, pdf_wi;
+    int onsurface;
+    /* ray */
+    v3 o, d;
+    double tmin;
+    /* info */
+    int obj;
+    int le, in_object;
+    double rand;
+    double vs, vsmax;
+    col vtau, vpdf;
+} pv_t;
+
+static void pv_clean(pv_t* v)
+{
+    memset(v, 0, sizeof(*v));
+    v->costheta_i = 1.0;
+    v->pdf_path = 1.0;
+    v->pdf_wi = 1.0;
+    v->onsurface = 1;
+    v->obj = -1;
+    v->rand = 1.0;
+    v->vtau = cs(1.0);
+    v->vpdf = cs(1.0);
+}
+
+/* Surface point and normal after a closest hit: Triangle/Sphere/Rect
+ * Intersect's vertex writes (triangle.cpp:104-114, sphere.cpp:110-129,
+ * rect.cpp:112-118). */
+static void surface(const scene_t* sc, const hitrec_t* h, const ray_t* r, v3* p, v3* n)
+{
+    const shape_t* s = &sc->shapes[h->obj];
+    const mat_t* m = &sc->mats[s->mat];
+    *p = add(r->o, mul(r->d, r->tmin));
+    if (s->kind == OR_TRI) {
+        *n = add(add(mul(s->n0, 1.0 - h->u - h->v), mul(s->n1, h->u)), mul(s->n2, h->v));
+        if (h->det < 0.0 && m->opaque) *n = neg(*n);
+    } else if (s->kind == OR_SPHERE) {
+        if (m->kind == MAT_EMISSIVE && !(len(sub(r->o, s->a)) > s->r))
+            *n = norm(sub(s->a, *p));
+        else
+            *n = norm(sub(*p, s->a));
+    } else {
+        *n = s->n;
+    }
+}
+
+/* PathTracer::Trace(w, h, camera, film) (pathtracer.cpp:117-354). */
+static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, pv_t* P)
+{
+    const medium_t* md = &sc->med;
+    int i, di;
+    for (i = 0; i < max_depth; ++i) pv_clean(&P[i]);
+    pv_t* cur = &P[0];
+    cur->o = cam_ray.o;
+    cur->d = cam_ray.d;
+    int le = 1, in_object = 0, absorb = 0;
+    cur->le = le;
+    cur->in_object = in_object;
+    for (di = 0; di < max_depth; ++di) {
+        cur = &P[di];
+        ray_t r = {cur->o, cur->d, 0.0, 1e300};
+        hitrec_t h = {-1, 0, 0, 0};
+        int hit = travel(sc, &r, 0, cur->in_object, &h);
+        cur->tmin = r.tmin;
+        if (hit >= 0) {
+            cur->obj = hit;
+            surface(sc, &h, &r, &cur->p, &cur->n);
+        }
+        if (md->kind && !cur->in_object) {
+            double tn, tf;
+            if (med_intersect(md, &r, &tn, &tf)) {
+                cur->vsmax = med_maxdist(r.tmin, tn, tf);
+                if (distance_sample(md->sigma_t, md->avg_sigma_t, md->absorb, cur->vsmax, g, &cur->vs,
+                                    &absorb, &cur->vtau, &cur->vpdf)) {
+                    cur->p = add(r.o, mul(r.d, tn < 0.0 ? cur->vs : tn + cur->vs));
+                    cur->onsurface = 0;
+                }
+            }
+            if (absorb) break;
+        }
+        double xi = 1.0, cost = 1.0, pdf_wi = 1.0, pdf_path = 1.0;
+        v3 w = mk(0, 0, 0);
+        if (md->kind && cur->vs > 0.0 && !cur->in_object) {
+            le = 0;
+            in_object = 0;
+            w = phase_sample(md->g, r.d, g, &pdf_wi);
+        } else {
+            if (cur->obj < 0) break;
+            const mat_t* m = &sc->mats[sc->shapes[cur->obj].mat];
+            if (m->kind == MAT_EMISSIVE) break;
+            v3 wo = neg(r.d), n = cur->n;
+            int sample_bsdf = 1;
+            if (m->kind == MAT_SUBSURFACE && in_object) {
+                /* Subsurface::PhaseSampling (subsurface.cpp:135-155) */
+                int ab = 0;
+                if (distance_sample(m->sigma_t, m->avg_sigmat, m->absorb, r.tmin, g, &cur->vs, &ab,
+                                    &cur->vtau, &cur->vpdf)) {
+                    le = 1;
+                    in_object = 1;
+                    cur->p = add(r.o, mul(r.d, cur->vs));
+                    w = phase_sample(0.0, neg(wo), g, &pdf_wi);
+                    sample_bsdf = 0;
+                } else {
+                    if (ab) cur->vs = -1.0;
+                    if (cur->vs < 0.0) break;
+                }
+            }
+            if (sample_bsdf) {
+                switch (m->kind) {
+                case MAT_MATTE:
+                    le = 0;
+                    in_object = 0;
+                    xi = 1.0;
+                    w = lambert_sample(n, g, &cost, &pdf_wi);
+                    break;
+                case MAT_GLOSSY:
+                    xi = rnd(g);
+                    if (m->kd != 0.0 && xi <= m->kd_pdf) {
+                        w = lambert_sample(n, g, &cost, &pdf_wi);
+                        le = 0;
+                        pdf_path = m->kd_pdf;
+                    } else {
+                        w = phong_sample(m, wo, n, g, &cost, &pdf_wi);
+                        le = 1;
+                        pdf_path = 1.0 - m->kd_pdf;
+                    }
+                    in_object = 0;
+                    break;
+                case MAT_SPECULAR: {
+                    le = 1;
+                    in_object = 0;
+                    xi = 1.0;
+                    double won = dot(wo, n);
+                    if (won <= 0.0) won = 1e-4;
+                    w = add(neg(wo), mul(n, 2.0 * won));
+                    cost = 1.0;
+                    pdf_wi = 1.0;
+                    break;
+                }
+                case MAT_DIELECTRIC:
+                case MAT_SUBSURFACE: {
+                    double kr = fresnel(neg(wo), n, m->eta_in, m->eta_out);
+                    double kt = 1.0 - kr;
+                    double c;
+                    if (kr != 0.0 && rnd(g) <= kr) {
+                        w = fresnel_reflect(wo, n, &c);
+                        in_object = c < 0.0;
+                        xi = 1.0;
+                        pdf_path = kr;
+                    } else {
+                        c = 0.0;
+                        w = fresnel_transmit(m, wo, n, &c);
+                        in_object = c > 0.0;
+                        xi = 0.0;
+                        pdf_path = kt;
+                    }
+                    le = 1;
+                    cost = 1.0;
+                    pdf_wi = 1.0;
+                    break;
+                }
+                }
+            }
+        }
+        cur->rand = xi;
+        cur->costheta_i = cost;
+        cur->pdf_wi = pdf_wi;
+        cur->pdf_path = pdf_path;
+        if (di + 1 > max_depth - 1) break;
+        pv_t* nx = &P[di + 1];
+        nx->o = cur->p;
+        nx->d = w;
+        nx->le = le;
+        nx->in_object = in_object;
+    }
+    if (di >= max_depth) di = max_depth - 1;
+
+    col L = cs(0.0);
+    for (; di >= 0; --di) {
+        cur = &P[di];
+        v3 n = cur->n, wo = neg(cur->d);
+        v3 w = di == max_depth - 1 ? mk(0, 0, 0) : P[di + 1].d;
+        col direct = cs(0.0), indirect, fr = cs(1.0), vol = cs(1.0);
+        double cost = cur->costheta_i;
+        double pdf = cur->pdf_wi * cur->pdf_path;
+        if (md->kind && cur->vs != 0.0 && !cur->in_object) {
+            if (cur->vs == -1.0) continue;
+            if (cur->vs > 0.0) {
+                direct = medium_direct(sc, cur->p, cur->d, g);
+                double fp = md->g == 0.0 ? 1.0 / (4.0 * PI_D) : hg_eval(md->g, dot(cur->d, w));
+                indirect = cscale(cmul(L, md->sigma_s), fp / pdf);
+                vol = cdiv(cur->vtau, cur->vpdf);
+                L = cmul(cadd(direct, indirect), vol);
+            }
+        } else {
+            if (cur->obj < 0) continue;
+            const shape_t* s = &sc->shapes[cur->obj];
+            const mat_t* m = &sc->mats[s->mat];
+            int rinfo_in = cur->in_object;
+            switch (m->kind) {
+            case MAT_MATTE:
+                fr = cscale(m->color, INV_PI_D * m->kd);
+                direct = matte_direct(sc, m, cur->p, n, g);
+                break;
+            case MAT_GLOSSY:
+                if (m->kd != 0.0 && cur->rand <= m->kd_pdf) {
+                    fr = cscale(m->color, INV_PI_D * m->kd);
+                    direct = cscale(matte_direct(sc, m, cur->p, n, g), 1.0 / m->kd_pdf);
+                } else {
+                    fr = cs(phong_f(m, n, w, wo) * m->ks);
+                }
+                break;
+            case MAT_SPECULAR:
+                fr = cscale(m->color, m->kd);
+                break;
+            case MAT_EMISSIVE:
+                if (rinfo_in == 0 && dot(n, wo) > 0.0) fr = cscale(m->color, m->kd);
+                else fr = cs(0.0);
+                if (rinfo_in == 0 && cur->le && -dot(cur->d, n) > 0.0)
+                    direct = cscale(m->color, m->kd);
+                break;
+            case MAT_DIELECTRIC:
+            case MAT_SUBSURFACE:
+                if (m->kind == MAT_SUBSURFACE && cur->vs != 0.0) {
+                    vol = cdiv(cur->vtau, cur->vpdf);
+                    vol = cscale(cmul(vol, m->sigma_s), 1.0 / (4.0 * PI_D));
+                    break;
+                }
+                if (cur->rand == 1.0) {
+                    fr = cs(cur->pdf_path);
+                } else {
+                    double eta = rinfo_in ? m->eta_in / m->eta_out : m->eta_out / m->eta_in;
+                    fr = cs(1.0 / (eta * eta) * cur->pdf_path);
+                }
+                if (rinfo_in) {
+                    if (m->kind == MAT_DIELECTRIC)
+                        fr = cmul(fr, exp_tau(m->absorption, cur->tmin));
+                    else
+                        fr = cmul(fr, cdiv(cur->vtau, cur->vpdf));
+                }
+                break;
+            }
+            indirect = cscale(cmul(cmul(vol, fr), L), cost / pdf);
+            L = cadd(direct, indirect);
+            if (md->kind && !cur->in_object) L = cmul(L, cdiv(cur->vtau, cur->vpdf));
+        }
+    }
+    return L;
+}
+
+/* Pinhole::CastViewRay (src/Camera/pinhole.cpp:97-136), no lens. */
+static ray_t camera_ray(const camera_t* c, int W, int H, int x, int y, double sx, double sy)
+{
+    double hw = W * 0.5, hh = H * 0.5;
+    double px = -(((double)x - hw) + sx) / hw * c->tanfov;
+    double py = -(((double)y - hh) + sy) / hh * c->tanfov * c->y_ratio;
+    v3 l = norm(mk(px, py, 1.0));
+    ray_t r;
+    r.o = c->p;
+    r.d = mk(c->R[0] * l.x + c->R[1] * l.y + c->R[2] * l.z,
+             c->R[3] * l.x + c->R[4] * l.y + c->R[5] * l.z,
+             c->R[6] * l.x + c->R[7] * l.y + c->R[8] * l.z);
+    r.tmin = 0.0;
+    r.tmax = 1e300;
+    return r;
+}
+
+/* ------------------------------------------------------------ scene load */
+static void load_scene(scene_t* sc, const float* shapes, int n_shapes, const float* mats,
+                       int n_mats, const float* cam, const float* med)
+{
+    int i, k;
+    memset(sc, 0, sizeof(*sc));
+    sc->n_shapes = n_shapes;
+    sc->shapes = (shape_t*)calloc((size_t)(n_shapes > 0 ? n_shapes : 1), sizeof(shape_t));
+    sc->lights = (int*)malloc(sizeof(int) * (n_shapes > 0 ? n_shapes : 1));
+    sc->n_mats = n_mats;
+    sc->mats = (mat_t*)calloc((size_t)(n_mats > 0 ? n_mats : 1), sizeof(mat_t));
+    for (i = 0; i < n_mats; ++i) {
+        const float* m = mats + 32 * (size_t)i;
+        mat_t* o = &sc->mats[i];
+        o->kind = (int)m[0];
+        o->kd = m[1];
+        o->ks = m[2];
+        o->e = m[3];
+        o->color = cmk(m[4], m[5], m[6]);
+        o->eta_in = m[7];
+        o->eta_out = m[8];
+        o->absorption = cmk(m[9], m[10], m[11]);
+        o->sigma_s = cmk(m[12], m[13], m[14]);
+        o->sigma_a = cmk(m[15], m[16], m[17]);
+        o->sigma_t = cadd(o->sigma_s, o->sigma_a);
+        double density = m[18] > 0 ? m[18] : 1.0;
+        o->avg_sigmat = (o->sigma_t.r + o->sigma_t.g + o->sigma_t.b) / 3.0 * density;
+        if (o->kind == MAT_SUBSURFACE)
+            o->absorb = (o->sigma_a.r / o->sigma_t.r + o->sigma_a.g / o->sigma_t.g +
+                         o->sigma_a.b / o->sigma_t.b) / 3.0;
+        o->kd_pdf = (o->kd + o->ks) > 0 ? o->kd / (o->kd + o->ks) : 0.0;
+        o->phong_norm = (o->e + 1.0) / (2.0 * PI_D);
+        o->opaque = !(o->kind == MAT_DIELECTRIC || o->kind == MAT_SUBSURFACE);
+    }
+    for (i = 0; i < n_shapes; ++i) {
+        const float* s = shapes + 32 * (size_t)i;
+        shape_t* o = &sc->shapes[i];
+        o->kind = (int)s[0];
+        o->mat = (int)s[1];
+        o->light = s[2] != 0.0f;
+        o->a = mk(s[3], s[4], s[5]);
+        o->b = mk(s[6], s[7], s[8]);
+        o->c = mk(s[9], s[10], s[11]);
+        o->r = s[12];
+        o->n0 = mk(s[13], s[14], s[15]);
+        o->n1 = mk(s[16], s[17], s[18]);
+        o->n2 = mk(s[19], s[20], s[21]);
+        if (o->kind == OR_TRI) {
+            for (k = 0; k < 3; ++k) {
+                double a = vget(o->a, k), b = vget(o->b, k), c = vget(o->c, k);
+                o->lo[k] = fmin(a, fmin(b, c));
+                o->hi[k] = fmax(a, fmax(b, c));
+            }
+        } else if (o->kind == OR_SPHERE) {
+            for (k = 0; k < 3; ++k) {
+                o->lo[k] = vget(o->a, k) - o->r;
+                o->hi[k] = vget(o->a, k) + o->r;
+            }
+        } else {
+            o->n = norm(cross(o->b, o->c));
+            o->length = len(o->b);
+            o->width = len(o->c);
+            o->inv_area = 1.0 / (o->length * o->width);
+            /* full parallelogram bounds (a box slightly larger than the
+               reference's two-corner box for skewed rects: conservative) */
+            for (k = 0; k < 3; ++k) {
+                double p0 = vget(o->a, k), l = vget(o->b, k), w = vget(o->c, k);
+                o->lo[k] = p0 + fmin(0.0, l) + fmin(0.0, w);
+                o->hi[k] = p0 + fmax(0.0, l) + fmax(0.0, w);
+            }
+        }
+        if (o->light) sc->lights[sc->n_lights++] = i;
+    }
+    sc->cam.p = mk(cam[0], cam[1], cam[2]);
+    for (k = 0; k < 9; ++k) sc->cam.R[k] = cam[3 + k];
+    sc->cam.tanfov = cam[12];
+    sc->cam.y_ratio = cam[13];
+    medium_t* md = &sc->med;
+    md->kind = med ? (int)med[0] : 0;
+    if (md->kind) {
+        for (k = 0; k < 3; ++k) {
+            md->lo[k] = med[1 + k];
+            md->hi[k] = med[4 + k];
+        }
+        md->c = mk(med[1], med[2], med[3]);
+        md->r = med[7];
+        md->sigma_s = cmk(med[8], med[9], med[10]);
+        md->sigma_a = cmk(med[11], med[12], med[13]);
+        md->sigma_t = cadd(md->sigma_s, md->sigma_a);
+        md->avg_sigma_t = cavg(md->sigma_t);
+        md->absorb = cavg(cdiv(md->sigma_a, md->sigma_t));
+        md->g = med[14];
+    }
+    build_bvh(sc);
+}
+
+static void free_scene(scene_t* sc)
+{
+    free(sc->shapes);
+    free(sc->mats);
+    free(sc->lights);
+    free(sc->nodes);
+    free(sc->order);
+}
+
+typedef struct {
+    const scene_t* sc;
+    int W, H, spp, max_depth, row0, row1;
+    uint64_t seed;
+    double* mean;
+    double* m2;
+} job_t;
+
+static void* render_rows(void* arg)
+{
+    job_t* j = (job_t*)arg;
+    pv_t* P = (pv_t*)malloc(sizeof(pv_t) * (size_t)(j->max_depth > 0 ? j->max_depth : 1));
+    int x, y, s;
+    for (y = j->row0; y < j->row1; ++y)
+        for (x = 0; x < j->W; ++x) {
+            pcg_t g;
+            pcg_seed(&g, j->seed, (uint64_t)y * (uint64_t)j->W + (uint64_t)x);
+            col sum = cs(0.0), sq = cs(0.0);
+            for (s = 0; s < j->spp; ++s) {
+                double sx = rnd(&g), sy = rnd(&g);
+                ray_t r = camera_ray(&j->sc->cam, j->W, j->H, x, y, sx, sy);
+                col L = trace_pt(j->sc, r, j->max_depth, &g, P);
+                if (cnan(L)) continue; /* Scene::Rendering drops NaN samples */
+                sum = cadd(sum, L);
+                sq = cadd(sq, cmul(L, L));
+            }
+            size_t o = 3 * ((size_t)y * j->W + x);
+            j->mean[o + 0] = sum.r / j->spp;
+            j->mean[o + 1] = sum.g / j->spp;
+            j->mean[o + 2] = sum.b / j->spp;
+            if (j->m2) {
+                j->m2[o + 0] = sq.r / j->spp;
+                j->m2[o + 1] = sq.g / j->spp;
+                j->m2[o + 2] = sq.b / j->spp;
+            }
+        }
+    free(P);
+    return 0;
+}
+
+int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                 const float* camera, const float* medium, int width, int height, int spp,
+                 int max_depth, uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb)
+{
+    if (width <= 0 || height <= 0 || spp <= 0 || max_depth <= 0 || !mean_rgb) return 1;
+    scene_t sc;
+    load_scene(&sc, shapes, n_shapes, mats, n_mats, camera, medium);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > height) nthreads = height;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+    job_t* jobs = (job_t*)malloc(sizeof(job_t) * nthreads);
+    int t;
+    for (t = 0; t < nthreads; ++t) {
+        jobs[t].sc = &sc;
+        jobs[t].W = width;
+        jobs[t].H = height;
+        jobs[t].spp = spp;
+        jobs[t].max_depth = max_depth;
+        jobs[t].row0 = height * t / nthreads;
+        jobs[t].row1 = height * (t + 1) / nthreads;
+        jobs[t].seed = seed;
+        jobs[t].mean = mean_rgb;
+        jobs[t].m2 = m2_rgb;
+        if (nthreads == 1)
+            render_rows(&jobs[t]);
+        else
+            pthread_create(&th[t], 0, render_rows, &jobs[t]);
+    }
+    if (nthreads > 1)
+        for (t = 0; t < nthreads; ++t) pthread_join(th[t], 0);
+    free(th);
+    free(jobs);
+    free_scene(&sc);
+    return 0;
+}

@@ -43,6 +43,58 @@ class BvhInfo(Structure):
                 ("sah_cost", c_double), ("build_seconds", c_double)]
 
 
+class Material(Structure):
+    _fields_ = [("kind", c_int32), ("kd", c_float), ("ks", c_float), ("exponent", c_float),
+                ("color", c_float * 3), ("eta_in", c_float), ("eta_out", c_float),
+                ("absorption", c_float * 3), ("sigma_s", c_float * 3), ("sigma_a", c_float * 3),
+                ("density", c_float), ("reserved", c_int32 * 4)]
+
+
+class Shape(Structure):
+    _fields_ = [("kind", c_int32), ("material", c_int32), ("is_light", c_int32),
+                ("texture", c_int32), ("v", c_float * 9), ("n", c_float * 9), ("radius", c_float),
+                ("xyz", c_void_p), ("nv", c_uint32), ("idx", c_void_p), ("nt", c_uint32),
+                ("normals", c_void_p), ("uv", c_void_p)]
+
+
+class Camera(Structure):
+    _fields_ = [("position", c_float * 3), ("basis", c_float * 9), ("fov_deg", c_float),
+                ("y_ratio", c_float)]
+
+
+class Medium(Structure):
+    _fields_ = [("bound", c_int32), ("lo", c_float * 3), ("hi", c_float * 3),
+                ("center", c_float * 3), ("radius", c_float), ("sigma_s", c_float * 3),
+                ("sigma_a", c_float * 3), ("g", c_float)]
+
+
+class Texture(Structure):
+    _fields_ = [("width", c_int32), ("height", c_int32), ("rgb", c_void_p)]
+
+
+class SceneDesc(Structure):
+    _fields_ = [("shapes", c_void_p), ("n_shapes", c_uint32), ("materials", c_void_p),
+                ("n_materials", c_uint32), ("textures", c_void_p), ("n_textures", c_uint32),
+                ("camera", Camera), ("medium", Medium)]
+
+
+class RenderDesc(Structure):
+    _fields_ = [("integrator", c_int32), ("width", c_int32), ("height", c_int32),
+                ("spp", c_int32), ("max_depth", c_int32), ("n_light", c_int32),
+                ("n_eye", c_int32), ("mis_beta", c_float), ("seed", c_uint64),
+                ("queue_capacity", c_uint32), ("regen_threshold", c_uint32)]
+
+
+class RenderStats(Structure):
+    _fields_ = [("samples", c_uint64), ("extension_rays", c_uint64), ("shadow_rays", c_uint64),
+                ("iterations", c_uint64), ("milliseconds", c_double)]
+
+
+RUMI_SHAPE_TRIANGLE, RUMI_SHAPE_SPHERE, RUMI_SHAPE_RECT, RUMI_SHAPE_MESH = 0, 1, 2, 3
+MATERIAL_KINDS = {"matte": 0, "glossy": 1, "specular": 2, "emissive": 3, "dielectric": 4,
+                  "subsurface": 5}
+RUMI_INTEGRATOR_PT, RUMI_INTEGRATOR_BDPT = 0, 1
+
 # Every function include/rumi.h declares, with its ctypes signature.
 _SIGNATURES = {
     "rumi_last_error": (c_char_p, []),
@@ -60,6 +112,10 @@ _SIGNATURES = {
                                c_void_p]),
     "rumi_trace_count": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_int32, c_void_p,
                                  c_void_p, c_void_p, c_uint32, c_void_p]),
+    "rumi_scene_create": (c_int, [c_void_p, POINTER(SceneDesc), POINTER(c_void_p)]),
+    "rumi_scene_destroy": (c_int, [c_void_p]),
+    "rumi_render": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
+                            POINTER(RenderStats), c_void_p]),
     "rumi_make_icosphere": (c_int, [c_uint32, c_uint64, c_float, c_void_p, c_void_p]),
     "rumi_make_rays_uniform": (c_int, [c_uint64, c_uint64, c_float, c_void_p]),
     "rumi_pcg32_sequence": (c_int, [c_uint64, c_uint64, c_uint32, c_void_p]),
@@ -224,3 +280,120 @@ def split_hits(hits_i32: np.ndarray):
     """(n, 2) int32 hit records -> (prim int32, t float32)."""
     h = np.ascontiguousarray(hits_i32, np.int32)
     return h[:, 0].copy(), h[:, 1].copy().view(np.float32)
+
+
+# --------------------------------------------------------------- scenes
+def _f3(x):
+    return (c_float * 3)(*[float(v) for v in x])
+
+
+class Scene:
+    """A device scene built from the dict form in scenes.py (shapes,
+    materials, textures, camera, medium); see include/rumi.h for fields."""
+
+    def __init__(self, ctx: Context, desc: dict):
+        self.ctx = ctx
+        self._keep = []
+        mats = (Material * len(desc["materials"]))()
+        for i, m in enumerate(desc["materials"]):
+            mm = mats[i]
+            mm.kind = MATERIAL_KINDS[m["kind"]]
+            mm.kd = m.get("kd", 0.0)
+            mm.ks = m.get("ks", 0.0)
+            mm.exponent = m.get("exponent", 0.0)
+            mm.color = _f3(m.get("color", (1, 1, 1)))
+            mm.eta_in = m.get("eta_in", 1.0)
+            mm.eta_out = m.get("eta_out", 1.0)
+            mm.absorption = _f3(m.get("absorption", (0, 0, 0)))
+            mm.sigma_s = _f3(m.get("sigma_s", (0, 0, 0)))
+            mm.sigma_a = _f3(m.get("sigma_a", (0, 0, 0)))
+            mm.density = m.get("density", 1.0)
+        shapes = (Shape * len(desc["shapes"]))()
+        for i, sd in enumerate(desc["shapes"]):
+            sh = shapes[i]
+            sh.material = sd["material"]
+            sh.is_light = 1 if sd.get("light") else 0
+            sh.texture = sd.get("texture", -1)
+            k = sd["kind"]
+            if k == "tri":
+                sh.kind = RUMI_SHAPE_TRIANGLE
+                sh.v = (c_float * 9)(*np.asarray(sd["v"], np.float32).reshape(9).tolist())
+                if sd.get("n") is not None:
+                    sh.n = (c_float * 9)(*np.asarray(sd["n"], np.float32).reshape(9).tolist())
+            elif k == "sphere":
+                sh.kind = RUMI_SHAPE_SPHERE
+                sh.v = (c_float * 9)(*(list(map(float, sd["center"])) + [0.0] * 6))
+                sh.radius = sd["radius"]
+            elif k == "rect":
+                sh.kind = RUMI_SHAPE_RECT
+                sh.v = (c_float * 9)(*[float(x) for x in (*sd["p"], *sd["l"], *sd["w"])])
+            elif k == "mesh":
+                sh.kind = RUMI_SHAPE_MESH
+                xyz = np.ascontiguousarray(sd["xyz"], np.float32)
+                idx = np.ascontiguousarray(sd["idx"], np.uint32)
+                self._keep += [xyz, idx]
+                sh.xyz, sh.nv, sh.idx, sh.nt = _ptr(xyz), xyz.shape[0], _ptr(idx), idx.shape[0]
+                if sd.get("normals") is not None:
+                    nrm = np.ascontiguousarray(sd["normals"], np.float32)
+                    self._keep.append(nrm)
+                    sh.normals = _ptr(nrm)
+                if sd.get("uv") is not None:
+                    uv = np.ascontiguousarray(sd["uv"], np.float32)
+                    self._keep.append(uv)
+                    sh.uv = _ptr(uv)
+            else:
+                raise ValueError(f"unknown shape kind {k}")
+        texs = (Texture * max(1, len(desc.get("textures", []))))()
+        for i, t in enumerate(desc.get("textures", [])):
+            rgb = np.ascontiguousarray(t, np.float32)
+            self._keep.append(rgb)
+            texs[i].height, texs[i].width = rgb.shape[0], rgb.shape[1]
+            texs[i].rgb = _ptr(rgb)
+        d = SceneDesc()
+        d.shapes, d.n_shapes = ctypes.addressof(shapes), len(desc["shapes"])
+        d.materials, d.n_materials = ctypes.addressof(mats), len(desc["materials"])
+        d.textures, d.n_textures = ctypes.addressof(texs), len(desc.get("textures", []))
+        cam = desc["camera"]
+        d.camera.position = _f3(cam["position"])
+        d.camera.basis = (c_float * 9)(*np.asarray(cam["basis"], np.float32).reshape(9).tolist())
+        d.camera.fov_deg = cam["fov_deg"]
+        d.camera.y_ratio = cam.get("y_ratio", 1.0)
+        med = desc.get("medium")
+        if med:
+            d.medium.bound = {"box": 1, "sphere": 2}[med["bound"]]
+            d.medium.lo = _f3(med.get("lo", (0, 0, 0)))
+            d.medium.hi = _f3(med.get("hi", (0, 0, 0)))
+            d.medium.center = _f3(med.get("center", (0, 0, 0)))
+            d.medium.radius = med.get("radius", 0.0)
+            d.medium.sigma_s = _f3(med["sigma_s"])
+            d.medium.sigma_a = _f3(med["sigma_a"])
+            d.medium.g = med.get("g", 0.0)
+        h = c_void_p()
+        check(lib().rumi_scene_create(ctx.handle, byref(d), byref(h)))
+        self.handle = h
+        self._keep = []
+
+    def render(self, width, height, spp, max_depth, seed=0, out=None, queue_capacity=0,
+               regen_threshold=0, integrator=RUMI_INTEGRATOR_PT, n_light=0, n_eye=0,
+               mis_beta=1.0, stream=None):
+        """Returns (device tensor height x width x 3 of mean radiance, RenderStats)."""
+        import torch
+        if out is None:
+            out = torch.empty((height, width, 3), dtype=torch.float32, device="cuda")
+        rd = RenderDesc(integrator, width, height, spp, max_depth, n_light, n_eye, mis_beta,
+                        seed, queue_capacity, regen_threshold)
+        st = RenderStats()
+        check(lib().rumi_render(self.ctx.handle, self.handle, byref(rd), _ptr(out), byref(st),
+                                _stream(stream)))
+        return out, st
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().rumi_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,73 +1,15 @@
-// BVH traversal kernels for gfx950 and their C-ABI launchers.
-//
-// One ray per lane (wave64).  The traversal stack lives in LDS, one column
-// per lane (entry k of lane t at k * BLOCK + t, so the 64 lanes of a wave
-// touch 64 consecutive dwords whatever their depths); trees deeper than the
-// LDS stack spill the excess to a private per-lane array.  A stack entry is
-// (child ref, entry distance of its box): the box test runs when the parent
-// is entered, the "closer than the current hit" test when the entry is
-// popped, which is the moment BVHAccel::Travel tests a popped node
-// (src/Acceleration/bvh.cpp:164-169).
-//
-// Every FP32 expression keeps the reference's operation order and the file
-// is compiled with -ffp-contract=off and correctly rounded division, so the
-// box and triangle tests give the bits BBox::Intersect (src/Kernel/bbox.cpp:
-// 30-92) and Triangle::Intersect (src/Shape/triangle.cpp:41-118) give.
+// Ray-query kernels for gfx950 (closest hit, any hit, instrumented) and their
+// C-ABI launchers.  One ray per lane (wave64); the traversal itself is the
+// shared device routine in traverse.h.
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
+#include "traverse.h"
 
 using namespace rumi;
+using namespace rumi::dev;
 
 namespace {
-
-constexpr float kEps = 1e-4f;  // Ray::_epsilon (src/Kernel/ray.cpp:4)
-
-struct RootInfo {
-    float lo[3];
-    float hi[3];
-    int32_t ref;
-};
-
-// BBox::Intersect with a precomputed reciprocal direction (1.0f / d is the
-// same value whether computed per node or once per ray).
-__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
-                                     float ox, float oy, float oz, float ix, float iy, float iz,
-                                     float& tn, float& tf) {
-    float t0 = (lx - ox) * ix;
-    float t1 = (hx - ox) * ix;
-    float tmp;
-    if (t0 > t1) { tmp = t0; t0 = t1; t1 = tmp; }
-    tn = t0;
-    tf = t1;
-    t0 = (ly - oy) * iy;
-    t1 = (hy - oy) * iy;
-    if (t0 > t1) { tmp = t0; t0 = t1; t1 = tmp; }
-    if (tn > t1 || t0 > tf) return false;
-    if (tn < t0) tn = t0;
-    if (tf > t1) tf = t1;
-    t0 = (lz - oz) * iz;
-    t1 = (hz - oz) * iz;
-    if (t0 > t1) { tmp = t0; t0 = t1; t1 = tmp; }
-    if (tn > t1 || t0 > tf) return false;
-    if (tn < t0) tn = t0;
-    if (tf > t1) tf = t1;
-    return true;
-}
-
-// The tmin-independent part of the node accept test (bvh.cpp:167-168).
-__device__ __forceinline__ bool box_static(float lx, float ly, float lz, float hx, float hy,
-                                           float hz, float ox, float oy, float oz, float ix,
-                                           float iy, float iz, float tmax, float& tn) {
-    float tf;
-    return slab(lx, ly, lz, hx, hy, hz, ox, oy, oz, ix, iy, iz, tn, tf) && tf >= 0.0f &&
-           tmax >= tn;
-}
-
-// The tmin part (bvh.cpp:169): no hit yet, or the box starts before it.
-__device__ __forceinline__ bool closer(float tmin, float tn) {
-    return tmin == 0.0f || tmin > tn;
-}
 
 template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool COUNT, bool SPILL>
 __global__ __launch_bounds__(BLOCK) void k_trace(
@@ -77,153 +19,18 @@ __global__ __launch_bounds__(BLOCK) void k_trace(
     int32_t* __restrict__ cnt_prims) {
     __shared__ int32_t s_ref[STACK * BLOCK];
     __shared__ float s_tn[STACK * BLOCK];
-    constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
-    int32_t p_ref[kSpill];
-    float p_tn[kSpill];
-
     const uint32_t tid = threadIdx.x;
     const uint32_t i = blockIdx.x * BLOCK + tid;
     if (i >= n) return;
-
     const float4 r0 = rays[2 * (size_t)i + 0];
     const float4 r1 = rays[2 * (size_t)i + 1];
-    const float ox = r0.x, oy = r0.y, oz = r0.z;
-    const float dx = r1.x, dy = r1.y, dz = r1.z, tmax = r1.w;
-    float tmin = r0.w;
-    int32_t hit = -1;
+    Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, r0.w};
     float hu = 0.0f, hv = 0.0f;
     int32_t nn = 0, np = 0;
-
-    if (!(dx == 0.0f && dy == 0.0f && dz == 0.0f)) {
-        const float ix = 1.0f / dx;
-        const float iy = 1.0f / dy;
-        const float iz = 1.0f / dz;
-        float tn;
-        bool go = box_static(root.lo[0], root.lo[1], root.lo[2], root.hi[0], root.hi[1],
-                             root.hi[2], ox, oy, oz, ix, iy, iz, tmax, tn) &&
-                  closer(tmin, tn);
-        int32_t cur = root.ref;
-        int sp = 0;
-        while (go) {
-            if (cur >= 0) {
-                const float4* np4 = nodes + 4 * (size_t)cur;
-                const float4 a = np4[0];
-                const float4 b = np4[1];
-                const float4 c = np4[2];
-                const float4 d = np4[3];
-                if (COUNT) ++nn;
-                const int32_t ref0 = __float_as_int(d.x);
-                const int32_t ref1 = __float_as_int(d.y);
-                float tn0, tn1;
-                const bool h0 = box_static(a.x, a.y, a.z, a.w, b.x, b.y, ox, oy, oz, ix, iy, iz,
-                                           tmax, tn0);
-                const bool h1 = box_static(b.z, b.w, c.x, c.y, c.z, c.w, ox, oy, oz, ix, iy, iz,
-                                           tmax, tn1);
-                const bool a0 = h0 && closer(tmin, tn0);
-                const bool a1 = h1 && closer(tmin, tn1);
-                int32_t push_ref = 0;
-                float push_tn = 0.0f;
-                bool do_push = false;
-                bool descend = true;
-                if (ORDERED) {
-                    if (a0 && a1) {
-                        const bool swap = tn1 < tn0;
-                        cur = swap ? ref1 : ref0;
-                        push_ref = swap ? ref0 : ref1;
-                        push_tn = swap ? tn0 : tn1;
-                        do_push = true;
-                    } else if (a0) {
-                        cur = ref0;
-                    } else if (a1) {
-                        cur = ref1;
-                    } else {
-                        descend = false;
-                    }
-                } else {
-                    // Reference order: the right child waits on the stack,
-                    // the left child is visited next.
-                    do_push = a1;
-                    push_ref = ref1;
-                    push_tn = tn1;
-                    if (a0)
-                        cur = ref0;
-                    else
-                        descend = false;
-                }
-                if (do_push) {
-                    if (!SPILL || sp < STACK) {
-                        s_ref[sp * BLOCK + tid] = push_ref;
-                        s_tn[sp * BLOCK + tid] = push_tn;
-                    } else {
-                        p_ref[sp - STACK] = push_ref;
-                        p_tn[sp - STACK] = push_tn;
-                    }
-                    ++sp;
-                }
-                if (descend) continue;
-            } else {
-                const int32_t enc = ~cur;
-                const int32_t first = enc >> 4;
-                const int32_t count = enc & 15;
-                for (int32_t k = 0; k < count; ++k) {
-                    const float4* t4 = tris + 3 * (size_t)(first + k);
-                    const float4 ta = t4[0];
-                    const float4 tb = t4[1];
-                    const float4 tc = t4[2];
-                    if (COUNT) ++np;
-                    const float v0x = ta.x, v0y = ta.y, v0z = ta.z;
-                    const float e1x = ta.w, e1y = tb.x, e1z = tb.y;
-                    const float e2x = tb.z, e2y = tb.w, e2z = tc.x;
-                    // p = Cross(d, e2)
-                    const float px = dy * e2z - e2y * dz;
-                    const float py = dz * e2x - e2z * dx;
-                    const float pz = dx * e2y - e2x * dy;
-                    const float det = px * e1x + py * e1y + pz * e1z;
-                    if (-1e-8f < det && det < 1e-8f) continue;
-                    const float inv_det = 1.0f / det;
-                    const float tx = ox - v0x, ty = oy - v0y, tz = oz - v0z;
-                    const float u = (px * tx + py * ty + pz * tz) * inv_det;
-                    if (u < 0.0f || u > 1.0f) continue;
-                    // q = Cross(T, e1)
-                    const float qx = ty * e1z - e1y * tz;
-                    const float qy = tz * e1x - e1z * tx;
-                    const float qz = tx * e1y - e1x * ty;
-                    const float v = (qx * dx + qy * dy + qz * dz) * inv_det;
-                    if (v < 0.0f || u + v > 1.0f) continue;
-                    const float t = (qx * e2x + qy * e2y + qz * e2z) * inv_det;
-                    if (t < kEps) continue;
-                    if (tmin > kEps && t > tmin) continue;
-                    if (tmax - t < kEps) continue;
-                    tmin = t;
-                    hit = __float_as_int(tc.y);
-                    hu = u;
-                    hv = v;
-                }
-                if (ANY && tmin > 0.0f) break;
-            }
-            // Pop until an entry still starts before the current hit.
-            go = false;
-            while (sp > 0) {
-                --sp;
-                int32_t r;
-                float t;
-                if (!SPILL || sp < STACK) {
-                    r = s_ref[sp * BLOCK + tid];
-                    t = s_tn[sp * BLOCK + tid];
-                } else {
-                    r = p_ref[sp - STACK];
-                    t = p_tn[sp - STACK];
-                }
-                if (closer(tmin, t)) {
-                    cur = r;
-                    go = true;
-                    break;
-                }
-            }
-        }
-    }
+    const int32_t hit = traverse<BLOCK, STACK, ANY, ORDERED, false, COUNT, SPILL>(
+        nodes, tris, nullptr, root, r, false, hu, hv, nn, np, s_ref, s_tn, tid);
     if (ANY && any_out) any_out[i] = hit;
-    if (hits) hits[i] = make_int2(hit, __float_as_int(tmin));
+    if (hits) hits[i] = make_int2(hit, __float_as_int(r.tmin));
     if (!ANY && uv) uv[i] = make_float2(hu, hv);
     if (COUNT) {
         cnt_nodes[i] = nn;

@@ -1,0 +1,302 @@
+// Device-side BVH traversal shared by the ray-query kernels (trace.hip) and
+// the wavefront renderer's extension and shadow stages (render.hip).
+//
+// Stack: LDS, one column per lane (entry k of lane t at k * BLOCK + t);
+// trees deeper than the LDS stack spill the excess to a private array.  A
+// stack entry is (child ref, entry distance of its box): the box test runs
+// when the parent is entered and the "closer than the current hit" test when
+// the entry is popped, the moment BVHAccel::Travel tests a popped node
+// (src/Acceleration/bvh.cpp:164-169).
+//
+// All FP32 expressions keep the reference's operation order; the library is
+// compiled with -ffp-contract=off and IEEE division, so the box, triangle,
+// sphere and rect tests produce the bits of BBox::Intersect
+// (src/Kernel/bbox.cpp:30-92), Triangle::Intersect (src/Shape/triangle.cpp:
+// 41-118), Sphere::Intersect (src/Shape/sphere.cpp:51-132) and
+// Rect::Intersect (src/Shape/rect.cpp:51-122).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace rumi {
+namespace dev {
+
+constexpr float kEps = 1e-4f;  // Ray::_epsilon (src/Kernel/ray.cpp:4)
+
+// Primitive tags stored in word c.z of a leaf record (mixed scenes only).
+constexpr int32_t kTagTri = 0;
+constexpr int32_t kTagSphere = 1;
+constexpr int32_t kTagRect = 2;
+
+struct RootInfo {
+    float lo[3];
+    float hi[3];
+    int32_t ref;
+};
+
+// BBox::Intersect with a precomputed reciprocal direction (1.0f / d is the
+// same value whether computed per node or once per ray).
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
+                                     float ox, float oy, float oz, float ix, float iy, float iz,
+                                     float& tn, float& tf) {
+    float t0 = (lx - ox) * ix;
+    float t1 = (hx - ox) * ix;
+    float tmp;
+    if (t0 > t1) { tmp = t0; t0 = t1; t1 = tmp; }
+    tn = t0;
+    tf = t1;
+    t0 = (ly - oy) * iy;
+    t1 = (hy - oy) * iy;
+    if (t0 > t1) { tmp = t0; t0 = t1; t1 = tmp; }
+    if (tn > t1 || t0 > tf) return false;
+    if (tn < t0) tn = t0;
+    if (tf > t1) tf = t1;
+    t0 = (lz - oz) * iz;
+    t1 = (hz - oz) * iz;
+    if (t0 > t1) { tmp = t0; t0 = t1; t1 = tmp; }
+    if (tn > t1 || t0 > tf) return false;
+    if (tn < t0) tn = t0;
+    if (tf > t1) tf = t1;
+    return true;
+}
+
+// The tmin-independent part of the node accept test (bvh.cpp:167-168).
+__device__ __forceinline__ bool box_static(float lx, float ly, float lz, float hx, float hy,
+                                           float hz, float ox, float oy, float oz, float ix,
+                                           float iy, float iz, float tmax, float& tn) {
+    float tf;
+    return slab(lx, ly, lz, hx, hy, hz, ox, oy, oz, ix, iy, iz, tn, tf) && tf >= 0.0f &&
+           tmax >= tn;
+}
+
+// The tmin part (bvh.cpp:169): no hit yet, or the box starts before it.
+__device__ __forceinline__ bool closer(float tmin, float tn) { return tmin == 0.0f || tmin > tn; }
+
+struct Ray {
+    float ox, oy, oz, dx, dy, dz, tmax;
+    float tmin;  // closest hit so far (0 = none), updated by the tests
+};
+
+// Triangle::Intersect on a leaf record (v0, e1 = v1 - v0, e2 = v2 - v0).
+__device__ __forceinline__ bool hit_triangle(const float4& ta, const float4& tb, const float4& tc,
+                                             Ray& r, float& u_out, float& v_out) {
+    const float e1x = ta.w, e1y = tb.x, e1z = tb.y;
+    const float e2x = tb.z, e2y = tb.w, e2z = tc.x;
+    const float px = r.dy * e2z - e2y * r.dz;
+    const float py = r.dz * e2x - e2z * r.dx;
+    const float pz = r.dx * e2y - e2x * r.dy;
+    const float det = px * e1x + py * e1y + pz * e1z;
+    if (-1e-8f < det && det < 1e-8f) return false;
+    const float inv_det = 1.0f / det;
+    const float tx = r.ox - ta.x, ty = r.oy - ta.y, tz = r.oz - ta.z;
+    const float u = (px * tx + py * ty + pz * tz) * inv_det;
+    if (u < 0.0f || u > 1.0f) return false;
+    const float qx = ty * e1z - e1y * tz;
+    const float qy = tz * e1x - e1z * tx;
+    const float qz = tx * e1y - e1x * ty;
+    const float v = (qx * r.dx + qy * r.dy + qz * r.dz) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    const float t = (qx * e2x + qy * e2y + qz * e2z) * inv_det;
+    if (t < kEps) return false;
+    if (r.tmin > kEps && t > r.tmin) return false;
+    if (r.tmax - t < kEps) return false;
+    r.tmin = t;
+    u_out = u;
+    v_out = v;
+    return true;
+}
+
+// Sphere::Intersect: a = centre.xyz, radius.  A = 1 (unit direction assumed),
+// far root first inside an object.
+__device__ __forceinline__ bool hit_sphere(const float4& a, Ray& r, bool in_object) {
+    const float ocx = r.ox - a.x, ocy = r.oy - a.y, ocz = r.oz - a.z;
+    const float A = 1.0f;
+    const float B = 2.0f * (r.dx * ocx + r.dy * ocy + r.dz * ocz);
+    const float C = (ocx * ocx + ocy * ocy + ocz * ocz) - a.w * a.w;
+    const float disc = B * B - 4 * A * C;
+    if (disc < 0.0f) return false;
+    const float inv_2a = 1.0f / (2.0f * A);
+    float t0, t1;
+    if (disc == 0.0f || disc == -0.0f) {
+        t0 = t1 = -B * inv_2a;
+    } else {
+        const float s = sqrtf(disc);
+        t0 = (-B - s) * inv_2a;
+        t1 = (-B + s) * inv_2a;
+    }
+    if (t0 < 0.0f && t1 < 0.0f) return false;
+    if (in_object) {
+        const float tmp = t0;
+        t0 = t1;
+        t1 = tmp;
+    }
+    if (t0 < kEps) return false;
+    if (r.tmax - t0 < kEps) return false;
+    if (t0 > r.tmin && r.tmin > 0.0f) return false;
+    r.tmin = t0;
+    return true;
+}
+
+// Rect record in the side table: p.xyz n.x | n.yz l.xy | l.z w.xyz |
+// (inv_area * width) (inv_area * length) length width.
+__device__ __forceinline__ bool hit_rect(const float4* rect, bool emissive, Ray& r) {
+    const float4 q0 = rect[0], q1 = rect[1], q2 = rect[2], q3 = rect[3];
+    const float nx = q0.w, ny = q1.x, nz = q1.y;
+    const float cost = r.dx * nx + r.dy * ny + r.dz * nz;
+    if (cost == 0.0f || cost == -0.0f) return false;
+    const float opx = r.ox - q0.x, opy = r.oy - q0.y, opz = r.oz - q0.z;
+    const float t = -(opx * nx + opy * ny + opz * nz) / cost;
+    if (t < 0.0f) return false;
+    if (t < kEps) return false;
+    if (r.tmax - t < kEps) return false;
+    if (!emissive && (0.0f < cost && cost < 1.0f)) return false;
+    if (t > r.tmin && r.tmin > 0.0f) return false;
+    const float ipx = r.ox + r.dx * t, ipy = r.oy + r.dy * t, ipz = r.oz + r.dz * t;
+    const float ux = ipx - q0.x, uy = ipy - q0.y, uz = ipz - q0.z;
+    const float il = (ux * q1.z + uy * q1.w + uz * q2.x) * q3.x;
+    if (il < 0.0f || il > q3.z) return false;
+    const float iw = (ux * q2.y + uy * q2.z + uz * q2.w) * q3.y;
+    if (iw < 0.0f || iw > q3.w) return false;
+    r.tmin = t;
+    return true;
+}
+
+// One traversal.  Returns the hit primitive's input index or -1; r.tmin
+// holds the hit distance.  MIXED enables sphere/rect leaf records (tag in
+// c.z, rect side table index in c.x, emissive flag in c.w).
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool MIXED, bool COUNT, bool SPILL>
+__device__ __forceinline__ int32_t traverse(const float4* __restrict__ nodes,
+                                            const float4* __restrict__ prims,
+                                            const float4* __restrict__ rects, const RootInfo& root,
+                                            Ray& r, bool in_object, float& hu, float& hv,
+                                            int32_t& nn, int32_t& np, int32_t* s_ref, float* s_tn,
+                                            uint32_t tid) {
+    constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
+    int32_t p_ref[kSpill];
+    float p_tn[kSpill];
+    int32_t hit = -1;
+    if (r.dx == 0.0f && r.dy == 0.0f && r.dz == 0.0f) return -1;
+    const float ix = 1.0f / r.dx;
+    const float iy = 1.0f / r.dy;
+    const float iz = 1.0f / r.dz;
+    float tn;
+    bool go = box_static(root.lo[0], root.lo[1], root.lo[2], root.hi[0], root.hi[1], root.hi[2],
+                         r.ox, r.oy, r.oz, ix, iy, iz, r.tmax, tn) &&
+              closer(r.tmin, tn);
+    int32_t cur = root.ref;
+    int sp = 0;
+    while (go) {
+        if (cur >= 0) {
+            const float4* np4 = nodes + 4 * (size_t)cur;
+            const float4 a = np4[0];
+            const float4 b = np4[1];
+            const float4 c = np4[2];
+            const float4 d = np4[3];
+            if (COUNT) ++nn;
+            const int32_t ref0 = __float_as_int(d.x);
+            const int32_t ref1 = __float_as_int(d.y);
+            float tn0, tn1;
+            const bool h0 = box_static(a.x, a.y, a.z, a.w, b.x, b.y, r.ox, r.oy, r.oz, ix, iy, iz,
+                                       r.tmax, tn0);
+            const bool h1 = box_static(b.z, b.w, c.x, c.y, c.z, c.w, r.ox, r.oy, r.oz, ix, iy, iz,
+                                       r.tmax, tn1);
+            const bool a0 = h0 && closer(r.tmin, tn0);
+            const bool a1 = h1 && closer(r.tmin, tn1);
+            int32_t push_ref = 0;
+            float push_tn = 0.0f;
+            bool do_push = false;
+            bool descend = true;
+            if (ORDERED) {
+                if (a0 && a1) {
+                    const bool swap = tn1 < tn0;
+                    cur = swap ? ref1 : ref0;
+                    push_ref = swap ? ref0 : ref1;
+                    push_tn = swap ? tn0 : tn1;
+                    do_push = true;
+                } else if (a0) {
+                    cur = ref0;
+                } else if (a1) {
+                    cur = ref1;
+                } else {
+                    descend = false;
+                }
+            } else {
+                // Reference order: the right child waits on the stack, the
+                // left child is visited next.
+                do_push = a1;
+                push_ref = ref1;
+                push_tn = tn1;
+                if (a0)
+                    cur = ref0;
+                else
+                    descend = false;
+            }
+            if (do_push) {
+                if (!SPILL || sp < STACK) {
+                    s_ref[sp * BLOCK + tid] = push_ref;
+                    s_tn[sp * BLOCK + tid] = push_tn;
+                } else {
+                    p_ref[sp - STACK] = push_ref;
+                    p_tn[sp - STACK] = push_tn;
+                }
+                ++sp;
+            }
+            if (descend) continue;
+        } else {
+            const int32_t enc = ~cur;
+            const int32_t first = enc >> 4;
+            const int32_t count = enc & 15;
+            for (int32_t k = 0; k < count; ++k) {
+                const float4* t4 = prims + 3 * (size_t)(first + k);
+                const float4 ta = t4[0];
+                const float4 tb = t4[1];
+                const float4 tc = t4[2];
+                if (COUNT) ++np;
+                float u = 0.0f, v = 0.0f;
+                bool ok;
+                if (!MIXED) {
+                    ok = hit_triangle(ta, tb, tc, r, u, v);
+                } else {
+                    const int32_t tag = __float_as_int(tc.z);
+                    if (tag == kTagTri)
+                        ok = hit_triangle(ta, tb, tc, r, u, v);
+                    else if (tag == kTagSphere)
+                        ok = hit_sphere(ta, r, in_object);
+                    else
+                        ok = hit_rect(rects + 4 * (size_t)__float_as_int(tc.x),
+                                      __float_as_int(tc.w) != 0, r);
+                }
+                if (ok) {
+                    hit = __float_as_int(tc.y);
+                    hu = u;
+                    hv = v;
+                }
+            }
+            if (ANY && r.tmin > 0.0f) break;
+        }
+        go = false;
+        while (sp > 0) {
+            --sp;
+            int32_t rr;
+            float t;
+            if (!SPILL || sp < STACK) {
+                rr = s_ref[sp * BLOCK + tid];
+                t = s_tn[sp * BLOCK + tid];
+            } else {
+                rr = p_ref[sp - STACK];
+                t = p_tn[sp - STACK];
+            }
+            if (closer(r.tmin, t)) {
+                cur = rr;
+                go = true;
+                break;
+            }
+        }
+    }
+    return hit;
+}
+
+}  // namespace dev
+}  // namespace rumi

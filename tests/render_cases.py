@@ -1,0 +1,88 @@
+"""Closed-form render cases and image statistics shared by the CPU (oracle)
+and GPU render tests."""
+import math
+
+import numpy as np
+
+
+def look_at_basis(position, target):
+    """Row-major camera basis whose third column is the view direction."""
+    d = np.asarray(target, np.float64) - np.asarray(position, np.float64)
+    d /= np.linalg.norm(d)
+    up = np.array([0.0, 1.0, 0.0])
+    cx = np.cross(up, d)
+    cx /= np.linalg.norm(cx)
+    cy = np.cross(d, cx)
+    return np.stack([cx, cy, d], axis=1).tolist()
+
+
+def _floor(material, half=5.0):
+    a, b, c, d = (-half, 0, half), (half, 0, half), (half, 0, -half), (-half, 0, -half)
+    n = [(0, 1, 0)] * 3
+    return [dict(kind="tri", v=[a, b, c], n=n, material=material),
+            dict(kind="tri", v=[a, c, d], n=n, material=material)]
+
+
+def square_light_scene(rho=0.5, le=(1.0, 2.0, 3.0), side=0.5, height=1.0):
+    """A Lambertian floor under a downward-facing square light centred at
+    height h; the camera's single pixel sees the floor point below the
+    light's centre through a 0.2 degree field of view."""
+    mats = [dict(kind="matte", kd=1.0, color=(rho, rho, rho)),
+            dict(kind="emissive", kd=1.0, color=le)]
+    shapes = _floor(0)
+    s = side
+    shapes.append(dict(kind="rect", p=(-s / 2, height, -s / 2), l=(s, 0, 0), w=(0, 0, s),
+                       material=1, light=True))
+    # steep view so the pixel footprint (about 2e-3 wide) sees a flat irradiance
+    pos = (0.0, 0.5, 0.3)
+    cam = dict(position=pos, basis=look_at_basis(pos, (0, 0, 0)), fov_deg=0.1, y_ratio=1.0)
+    return dict(materials=mats, shapes=shapes, camera=cam, medium=None)
+
+
+def square_light_radiance(rho, le, side, height):
+    """Lambertian radiance rho * Le * F, F the point-to-parallel-square form
+    factor: four corner rectangles, each (1/2pi)[X/sqrt(1+X^2) atan(Y/sqrt(1+X^2))
+    + Y/sqrt(1+Y^2) atan(X/sqrt(1+Y^2))] with X = Y = side / (2 h)."""
+    X = Y = side / (2.0 * height)
+    fc = (X / math.sqrt(1 + X * X) * math.atan(Y / math.sqrt(1 + X * X)) +
+          Y / math.sqrt(1 + Y * Y) * math.atan(X / math.sqrt(1 + Y * Y))) / (2 * math.pi)
+    F = 4.0 * fc
+    return np.array([rho * l * F for l in le])
+
+
+def mirror_scene(kr=0.9, le=(1.0, 2.0, 3.0)):
+    """A Specular floor; the camera ray reflects at the origin and meets the
+    front face of a vertical light at z = -6 (PathTracer counts emission
+    after a specular bounce, pathtracer.cpp:198-202 + emissive.cpp:60-74)."""
+    mats = [dict(kind="specular", kd=kr, color=(1.0, 1.0, 1.0)),
+            dict(kind="emissive", kd=1.0, color=le)]
+    shapes = _floor(0, half=20.0)
+    shapes.append(dict(kind="rect", p=(-1.0, 0.5, -6.0), l=(2, 0, 0), w=(0, 1, 0), material=1,
+                       light=True))
+    pos = (0.0, 0.5, 3.0)
+    cam = dict(position=pos, basis=look_at_basis(pos, (0, 0, 0)), fov_deg=0.2, y_ratio=1.0)
+    return dict(materials=mats, shapes=shapes, camera=cam, medium=None)
+
+
+def rel_mse(img, ref):
+    """mean over pixels and channels of (I - R)^2 / (R^2 + 0.01)."""
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def expected_rel_mse(ref_mean, ref_m2, spp, ref_spp):
+    """Expected rel-MSE of an unbiased spp-sample estimate against the
+    ref_spp-sample reference, from the reference's per-pixel variance."""
+    var = np.maximum(np.asarray(ref_m2, np.float64) - np.asarray(ref_mean, np.float64) ** 2, 0)
+    return float(np.mean(var * (1.0 / spp + 1.0 / ref_spp) / (np.asarray(ref_mean) ** 2 + 0.01)))
+
+
+def channel_mean_z(img, ref_mean, ref_m2, spp, ref_spp):
+    """z-score of each channel's image mean against the reference's."""
+    ref_mean = np.asarray(ref_mean, np.float64)
+    var = np.maximum(np.asarray(ref_m2, np.float64) - ref_mean ** 2, 0)
+    P = ref_mean.shape[0] * ref_mean.shape[1]
+    sigma = np.sqrt(var.sum(axis=(0, 1)) * (1.0 / spp + 1.0 / ref_spp)) / P
+    diff = np.asarray(img, np.float64).mean(axis=(0, 1)) - ref_mean.mean(axis=(0, 1))
+    return diff / sigma

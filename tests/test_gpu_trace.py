@@ -225,8 +225,15 @@ def test_full_config_icosphere_1p3m_8m_rays(rumi, oracle, ctx):
     hp = rays[m, :3].astype(np.float64) + t[m, None].astype(np.float64) * rays[m, 4:7]
     r = np.linalg.norm(hp, axis=1)
     assert r.min() > 0.95 - 1e-4 and r.max() < 1.05 + 1e-4
+    # Rays starting inside the shell almost all hit; Moller-Trumbore is not
+    # watertight, so a ray through a shared edge can pass between two
+    # triangles: every such leak must be a miss under brute force too.
     inside = np.linalg.norm(rays[:, :3].astype(np.float64), axis=1) < 0.95
-    assert m[inside].all()
+    leaks = np.flatnonzero(inside & ~m)
+    assert len(leaks) <= 16
+    if len(leaks):
+        lp, _ = oracle.brute_mesh(xyz, idx, rays[leaks])
+        assert (lp == -1).all()
     ob = oracle.MedianBvh.from_export(b.export(), idx.shape[0])
     sl = slice(1 << 22, (1 << 22) + 65536)
     op, ot = ob.trace_mesh(xyz, idx, rays[sl], ordered=True)
