@@ -247,7 +247,48 @@ def render_legs(rumi, ctx, args, log):
         log(f"{name}: {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.1f} ms), "
             f"rel-MSE ratio {out[name]['rel_mse_64px']['ratio']:.3f}, CPU {cpu_sps:.0f} samples/s")
         scene.close()
+    out["cornell_bdpt"] = bdpt_leg(rumi, ctx, args, log, scenes)
     return out
+
+
+def bdpt_leg(rumi, ctx, args, log, scenes):
+    """Bidirectional path tracing on the 1024^2 Cornell box with the
+    200k-triangle GGX icosphere and the glass sphere, path length <= 8,
+    power heuristic; samples/s at 16 spp (the full 64 spp is 4x the work)
+    and a CPU baseline of the oracle's FP64 BDPT on one thread."""
+    import torch
+    w = h = 1024
+    spp, max_len = 16, 8
+    sc = scenes.cornell_bdpt(w, h)
+    scene = rumi.Scene(ctx, sc)
+    kw = dict(integrator=rumi.RUMI_INTEGRATOR_BDPT, mis_beta=2.0, n_eye=9, n_light=9)
+    img, st = scene.render(64, 64, 1, max_len, seed=1, **kw)  # warm-up
+    torch.cuda.synchronize()
+    times = []
+    for k in range(max(1, args.render_runs - 1)):
+        t0 = time.perf_counter()
+        img, st = scene.render(w, h, spp, max_len, seed=10 + k, **kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    sec = min(times)
+    sps = st.samples / sec
+    scene.close()
+    # CPU baseline leg: the oracle's FP64 bidirectional tracer, one thread
+    import oracle
+    t0 = time.perf_counter()
+    oracle.render_bdpt(scenes.cornell_bdpt(32, 32), 32, 32, spp, max_len, n_eye=9, n_light=9,
+                       mis_beta=2.0, seed=5, nthreads=1)
+    cpu_sec = time.perf_counter() - t0
+    cpu_sps = 32 * 32 * spp / cpu_sec
+    log(f"cornell_bdpt: {sps / 1e6:.3f} Msamples/s ({sec * 1e3:.1f} ms), CPU {cpu_sps:.0f} samples/s")
+    return {"samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4),
+            "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
+            "max_path_length": max_len, "mis_beta": 2.0, "triangles": 200000 + 10,
+            "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
+                             "kind": "port",
+                             "sample": f"32x32 at {spp} spp, the oracle's FP64 bidirectional "
+                                       f"tracer ({cpu_sec:.2f}s)"},
+            "speedup_vs_cpu": round(sps / cpu_sps, 1)}
 
 
 def rel_mse(img, ref):

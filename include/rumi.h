@@ -170,7 +170,9 @@ enum {
     RUMI_MAT_SPECULAR = 2,   /* Specular(kr = kd, color) (src/Material/specular.cpp) */
     RUMI_MAT_EMISSIVE = 3,   /* Emissive(ls = kd, color) (src/Material/emissive.cpp) */
     RUMI_MAT_DIELECTRIC = 4, /* Dielectric(eta_in, eta_out, absorption) (src/Material/dielectric.cpp) */
-    RUMI_MAT_SUBSURFACE = 5  /* Subsurface(eta_in, eta_out, sigma_s, sigma_a, density) (src/Material/subsurface.cpp) */
+    RUMI_MAT_SUBSURFACE = 5, /* Subsurface(eta_in, eta_out, sigma_s, sigma_a, density) (src/Material/subsurface.cpp) */
+    RUMI_MAT_GGX = 6         /* extension: GGX microfacet reflection, Schlick Fresnel with F0 = color,
+                                roughness = alpha (Walter et al. 2007); no reference counterpart */
 };
 
 typedef struct {
@@ -183,7 +185,8 @@ typedef struct {
     float absorption[3]; /* Dielectric Beer-Lambert coefficients */
     float sigma_s[3], sigma_a[3];
     float density;       /* Subsurface sampling density (0 = 1) */
-    int32_t reserved[4];
+    float roughness;     /* GGX alpha */
+    int32_t reserved[3];
 } rumi_material;
 
 typedef struct {

@@ -248,7 +248,7 @@ def time_median6_mesh(xyz, idx, rays, any_hit=False):
 
 # ------------------------------------------------------------------ renderer
 MAT_KINDS = {"matte": 0, "glossy": 1, "specular": 2, "emissive": 3, "dielectric": 4,
-             "subsurface": 5}
+             "subsurface": 5, "ggx": 6}
 
 
 def scene_arrays(scene: dict):
@@ -306,6 +306,7 @@ def scene_arrays(scene: dict):
         mats[i, 12:15] = m.get("sigma_s", (0, 0, 0))
         mats[i, 15:18] = m.get("sigma_a", (0, 0, 0))
         mats[i, 18] = m.get("density", 1.0)
+        mats[i, 19] = m.get("roughness", 0.0)
     cam = scene["camera"]
     c = np.zeros(16, np.float32)
     c[0:3] = cam["position"]
@@ -348,3 +349,24 @@ def render_pt(scene: dict, width: int, height: int, spp: int, max_depth: int, se
     if rc != 0:
         raise RuntimeError(f"or_render_pt failed ({rc})")
     return mean, m2
+
+
+def render_bdpt(scene: dict, width: int, height: int, spp: int, max_len: int, n_eye: int = 0,
+                n_light: int = 0, mis_beta: float = 1.0, seed: int = 0, nthreads: int = 1):
+    """FP64 bidirectional path tracer (or_render_bdpt): mean radiance image."""
+    L = lib()
+    if not hasattr(L, "_bdpt_bound"):
+        L.or_render_bdpt.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_int,
+                                     c_int, c_int, c_int, c_double, ctypes.c_uint64, c_int, c_void_p]
+        L._bdpt_bound = True
+    shapes, mats, cam, med = scene_arrays(scene)
+    if med is not None:
+        raise ValueError("the bidirectional tracer has no media")
+    n_eye = n_eye or max_len + 1
+    n_light = n_light or max_len + 1
+    mean = np.zeros((height, width, 3), np.float64)
+    rc = L.or_render_bdpt(_p(shapes), shapes.shape[0], _p(mats), mats.shape[0], _p(cam), width,
+                          height, spp, n_eye, n_light, max_len, mis_beta, seed, nthreads, _p(mean))
+    if rc != 0:
+        raise RuntimeError(f"or_render_bdpt failed ({rc})")
+    return mean

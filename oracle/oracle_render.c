@@ -85,7 +85,7 @@ static inline double rnd(pcg_t* g) { return (double)pcg_next(g) * (1.0 / 4294967
 
 /* ------------------------------------------------------------------ scene */
 enum { MAT_MATTE = 0, MAT_GLOSSY = 1, MAT_SPECULAR = 2, MAT_EMISSIVE = 3,
-       MAT_DIELECTRIC = 4, MAT_SUBSURFACE = 5 };
+       MAT_DIELECTRIC = 4, MAT_SUBSURFACE = 5, MAT_GGX = 6 };
 
 typedef struct {
     int kind;
@@ -95,6 +95,7 @@ typedef struct {
     col absorption;
     col sigma_s, sigma_a, sigma_t;
     double avg_sigmat, absorb;
+    double alpha; /* GGX roughness */
     int opaque;
 } mat_t;
 
@@ -485,6 +486,54 @@ static v3 fresnel_transmit(const mat_t* m, v3 wo, v3 n, double* cost)
     return add(mul(neg(wo), inv), mul(n, ci * inv - ct));
 }
 
+/* GGX microfacet reflection (Walter et al. 2007), Schlick Fresnel with
+ * F0 = colour; an extension with no reference counterpart. */
+static double ggx_d(double a, double ch)
+{
+    double a2 = a * a, d = ch * ch * (a2 - 1.0) + 1.0;
+    return a2 / (PI_D * d * d);
+}
+
+static double ggx_lambda(double a, double c)
+{
+    double t2 = (1.0 - c * c) / (c * c);
+    return 0.5 * (-1.0 + sqrt(1.0 + a * a * t2));
+}
+
+static col ggx_f(const mat_t* m, v3 n, v3 wi, v3 wo)
+{
+    double ci = dot(n, wi), co = dot(n, wo);
+    if (ci <= 0.0 || co <= 0.0) return cs(0.0);
+    v3 h = norm(add(wi, wo));
+    double D = ggx_d(m->alpha, dot(n, h));
+    double G = 1.0 / (1.0 + ggx_lambda(m->alpha, ci) + ggx_lambda(m->alpha, co));
+    double x = 1.0 - fmax(dot(wi, h), 0.0), x5 = x * x * x * x * x;
+    col F = cmk(m->color.r + (1 - m->color.r) * x5, m->color.g + (1 - m->color.g) * x5,
+                m->color.b + (1 - m->color.b) * x5);
+    return cscale(F, D * G / (4.0 * ci * co));
+}
+
+static double ggx_pdf(const mat_t* m, v3 n, v3 wi, v3 wo)
+{
+    double ci = dot(n, wi), co = dot(n, wo);
+    if (ci <= 0.0 || co <= 0.0) return 0.0;
+    v3 h = norm(add(wi, wo));
+    double ch = dot(n, h);
+    return ggx_d(m->alpha, ch) * ch / (4.0 * dot(wo, h));
+}
+
+static v3 ggx_sample(const mat_t* m, v3 n, v3 wo, pcg_t* g)
+{
+    double u1 = rnd(g), u2 = rnd(g);
+    double t2 = m->alpha * m->alpha * u1 / (1.0 - u1);
+    double ch = 1.0 / sqrt(1.0 + t2), sh = sqrt(fmax(0.0, 1.0 - ch * ch));
+    double phi = 2.0 * PI_D * u2;
+    v3 u, w;
+    basis(n, &u, &w);
+    v3 h = frame(u, n, w, sh * sin(phi), ch, sh * cos(phi));
+    return sub(mul(h, 2.0 * dot(wo, h)), wo);
+}
+
 /* ------------------------------------------------------------ medium */
 static int med_intersect(const medium_t* md, const ray_t* r, double* tn, double* tf)
 {
@@ -826,6 +875,14 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
                     pdf_wi = 1.0;
                     break;
                 }
+                case MAT_GGX:
+                    le = 1;
+                    in_object = 0;
+                    w = ggx_sample(m, n, wo, g);
+                    cost = dot(n, w);
+                    pdf_wi = ggx_pdf(m, n, w, wo);
+                    if (!(cost > 0.0 && pdf_wi > 0.0)) cost = 0.0, pdf_wi = 1.0;
+                    break;
                 case MAT_DIELECTRIC:
                 case MAT_SUBSURFACE: {
                     double kr = fresnel(neg(wo), n, m->eta_in, m->eta_out);
@@ -901,6 +958,9 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
                 break;
             case MAT_SPECULAR:
                 fr = cscale(m->color, m->kd);
+                break;
+            case MAT_GGX:
+                fr = ggx_f(m, n, w, wo);
                 break;
             case MAT_EMISSIVE:
                 if (rinfo_in == 0 && dot(n, wo) > 0.0) fr = cscale(m->color, m->kd);
@@ -986,6 +1046,7 @@ static void load_scene(scene_t* sc, const float* shapes, int n_shapes, const flo
                          o->sigma_a.b / o->sigma_t.b) / 3.0;
         o->kd_pdf = (o->kd + o->ks) > 0 ? o->kd / (o->kd + o->ks) : 0.0;
         o->phong_norm = (o->e + 1.0) / (2.0 * PI_D);
+        o->alpha = m[19];
         o->opaque = !(o->kind == MAT_DIELECTRIC || o->kind == MAT_SUBSURFACE);
     }
     for (i = 0; i < n_shapes; ++i) {
@@ -1131,6 +1192,403 @@ int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mat
         for (t = 0; t < nthreads; ++t) pthread_join(th[t], 0);
     free(th);
     free(jobs);
+    free_scene(&sc);
+    return 0;
+}
+
+/* ====================================================================== */
+/* FP64 bidirectional path tracer.  Follows BidirectionalPathTracer::Trace
+ * (src/Integrator/bidirectionalpathtracer.cpp:121-394): an eye subpath
+ * (ConstructViewPath :396-711) and a light subpath (ConstructLightPath
+ * :713-1016) with area-measure forward/reverse pdfs, then every (s, t)
+ * connection -- t = 1 splats through the pinhole onto the film
+ * (Pinhole::FilmPlaneIntersection, pinhole.cpp:138-167) -- weighted by MIS
+ * over all strategies with exponent beta (1 = the reference's balance
+ * heuristic, MISWeight :1096-1586).  Mirror and dielectric vertices are
+ * never connected.  Materials: matte, GGX, specular, dielectric, emissive. */
+
+#define BV_DELTA 1
+#define BV_EMIT 2
+#define BV_CAM 4
+#define BV_LIGHT 8
+
+typedef struct {
+    v3 p, n;
+    double pdf_fwd, pdf_rev;
+    col beta;
+    int flags, mat, prim;
+} bvert_t;
+
+static double to_area(double pdf_w, v3 from, v3 to, v3 n_to)
+{
+    v3 d = sub(to, from);
+    double d2 = dot(d, d);
+    return pdf_w * fabs(dot(n_to, d)) / (d2 * sqrt(d2));
+}
+
+static col bsdf_f_v(const scene_t* sc, const bvert_t* v, v3 wi, v3 wo)
+{
+    const mat_t* m = &sc->mats[v->mat];
+    if (m->kind == MAT_MATTE) return cscale(m->color, m->kd * INV_PI_D);
+    if (m->kind == MAT_GGX) return ggx_f(m, v->n, wi, wo);
+    return cs(0.0);
+}
+
+static double bsdf_pdf_v(const scene_t* sc, const bvert_t* v, v3 wi, v3 wo)
+{
+    const mat_t* m = &sc->mats[v->mat];
+    if (m->kind == MAT_MATTE) {
+        double p = INV_PI_D * dot(v->n, wi); /* Lambertian::Pdfw (bsdf.cpp:34-41) */
+        return p <= 0.0 ? 1e-4 : p;
+    }
+    if (m->kind == MAT_GGX) return ggx_pdf(m, v->n, wi, wo);
+    return 0.0;
+}
+
+static int bd_walk(const scene_t* sc, bvert_t* arr, int start, int maxv, v3 o, v3 d, col beta,
+                   double pdf_w, pcg_t* g, int eye)
+{
+    int nv = start, in_obj = 0;
+    v3 prev_p = o;
+    while (nv < maxv) {
+        ray_t r = {o, d, 0.0, 1e300};
+        hitrec_t h = {-1, 0, 0, 0};
+        int hit = travel(sc, &r, 0, in_obj, &h);
+        if (hit < 0) break;
+        v3 p, n;
+        surface(sc, &h, &r, &p, &n);
+        const mat_t* m = &sc->mats[sc->shapes[hit].mat];
+        bvert_t* v = &arr[nv];
+        v->p = p;
+        v->n = n;
+        v->pdf_fwd = to_area(pdf_w, prev_p, p, n);
+        v->pdf_rev = 0.0;
+        v->beta = beta;
+        v->flags = 0;
+        v->mat = sc->shapes[hit].mat;
+        v->prim = hit;
+        ++nv;
+        if (m->kind == MAT_EMISSIVE) {
+            v->flags = BV_EMIT;
+            break;
+        }
+        int was_in = in_obj;
+        v3 wo = neg(d), w;
+        double pdf_next = 0.0, pdf_rev = 0.0;
+        int delta = 0;
+        col weight;
+        if (m->kind == MAT_MATTE) {
+            double cost, pdf;
+            w = lambert_sample(n, g, &cost, &pdf);
+            weight = cscale(m->color, m->kd * INV_PI_D * cost / pdf);
+            pdf_next = pdf;
+            double pr = INV_PI_D * dot(n, wo);
+            pdf_rev = pr <= 0.0 ? 1e-4 : pr;
+        } else if (m->kind == MAT_GGX) {
+            w = ggx_sample(m, n, wo, g);
+            double cost = dot(n, w);
+            pdf_next = ggx_pdf(m, n, w, wo);
+            if (!(cost > 0.0 && pdf_next > 0.0)) break;
+            weight = cscale(ggx_f(m, n, w, wo), cost / pdf_next);
+            pdf_rev = ggx_pdf(m, n, wo, w);
+        } else if (m->kind == MAT_SPECULAR) {
+            double won = dot(wo, n);
+            if (won <= 0.0) won = 1e-4;
+            w = add(neg(wo), mul(n, 2.0 * won));
+            weight = cscale(m->color, m->kd);
+            delta = 1;
+        } else if (m->kind == MAT_DIELECTRIC) {
+            double kr = fresnel(neg(wo), n, m->eta_in, m->eta_out), c;
+            delta = 1;
+            if (kr != 0.0 && rnd(g) <= kr) {
+                w = fresnel_reflect(wo, n, &c);
+                in_obj = c < 0.0;
+                weight = cs(1.0);
+            } else {
+                c = 0.0;
+                w = fresnel_transmit(m, wo, n, &c);
+                in_obj = c > 0.0;
+                double er = was_in ? m->eta_in / m->eta_out : m->eta_out / m->eta_in;
+                weight = cs(eye ? 1.0 / (er * er) : 1.0);
+            }
+            if (was_in) weight = cmul(weight, exp_tau(m->absorption, r.tmin));
+        } else {
+            break; /* glossy / subsurface: not handled by the bidirectional tracer */
+        }
+        if (delta) v->flags |= BV_DELTA;
+        if (nv >= 2) {
+            bvert_t* pv = &arr[nv - 2];
+            pv->pdf_rev = delta ? 0.0 : to_area(pdf_rev, p, prev_p, pv->n);
+        }
+        beta = cmul(beta, weight);
+        pdf_w = delta ? 0.0 : pdf_next;
+        prev_p = p;
+        o = p;
+        d = w;
+        if (m->kind != MAT_DIELECTRIC) in_obj = 0;
+    }
+    return nv;
+}
+
+/* Inverse of Pinhole::CastViewRay's raster mapping for a world direction. */
+static int camera_project(const camera_t* c, int W, int H, v3 dc, int* pixel, double* cos_c)
+{
+    double lx = c->R[0] * dc.x + c->R[3] * dc.y + c->R[6] * dc.z;
+    double ly = c->R[1] * dc.x + c->R[4] * dc.y + c->R[7] * dc.z;
+    double lz = c->R[2] * dc.x + c->R[5] * dc.y + c->R[8] * dc.z;
+    if (!(lz > 0.0)) return 0;
+    double px = lx / lz, py = ly / lz, hw = W * 0.5, hh = H * 0.5;
+    double rx = hw - px * hw / c->tanfov, ry = hh - py * hh / (c->tanfov * c->y_ratio);
+    if (!(rx >= 0.0 && rx < W && ry >= 0.0 && ry < H)) return 0;
+    int ix = (int)rx, iy = (int)ry;
+    if (ix > W - 1) ix = W - 1;
+    if (iy > H - 1) iy = H - 1;
+    *pixel = iy * W + ix;
+    *cos_c = lz;
+    return 1;
+}
+
+static int visible(const scene_t* sc, v3 a, v3 b)
+{
+    v3 dv = sub(b, a);
+    double dist = len(dv);
+    ray_t r = {a, mul(dv, 1.0 / dist), 0.0, dist};
+    travel(sc, &r, 1, 0, 0);
+    return !(r.tmin > 0.0);
+}
+
+static double rm0(double f) { return f != 0.0 ? f : 1.0; }
+
+typedef struct {
+    const scene_t* sc;
+    int W, H, spp, nE, nL, max_len, row0, row1;
+    double beta;
+    uint64_t seed;
+    int* light_of; /* shape -> light index or -1 */
+    double* film;  /* per-thread accumulation, W*H*3 */
+} bjob_t;
+
+static void* bdpt_rows(void* arg)
+{
+    bjob_t* j = (bjob_t*)arg;
+    const scene_t* sc = j->sc;
+    const camera_t* cam = &sc->cam;
+    double film_area = (2.0 * cam->tanfov) * (2.0 * cam->tanfov * cam->y_ratio);
+    bvert_t* E = (bvert_t*)calloc((size_t)j->nE, sizeof(bvert_t));
+    bvert_t* Lv = (bvert_t*)calloc((size_t)j->nL, sizeof(bvert_t));
+    int x, y, smp, s, t, i;
+    for (y = j->row0; y < j->row1; ++y)
+        for (x = 0; x < j->W; ++x) {
+            uint64_t pixel = (uint64_t)y * j->W + x;
+            pcg_t g;
+            pcg_seed(&g, j->seed, pixel);
+            for (smp = 0; smp < j->spp; ++smp) {
+                /* eye subpath */
+                double sx = rnd(&g), sy = rnd(&g);
+                ray_t cr = camera_ray(cam, j->W, j->H, x, y, sx, sy);
+                double hw = j->W * 0.5, hh = j->H * 0.5;
+                double lpx = -(((double)x - hw) + sx) / hw * cam->tanfov;
+                double lpy = -(((double)y - hh) + sy) / hh * cam->tanfov * cam->y_ratio;
+                double lz = 1.0 / sqrt(lpx * lpx + lpy * lpy + 1.0);
+                E[0].p = cam->p;
+                E[0].n = mk(cam->R[2], cam->R[5], cam->R[8]);
+                E[0].pdf_fwd = 1.0;
+                E[0].pdf_rev = 0.0;
+                E[0].beta = cs(1.0);
+                E[0].flags = BV_CAM;
+                double pdf_dir = 1.0 / (lz * lz * lz * film_area);
+                int nE = bd_walk(sc, E, 1, j->nE, cr.o, cr.d, cs(1.0), pdf_dir, &g, 1);
+                /* light subpath */
+                int nL = 0;
+                if (sc->n_lights > 0) {
+                    lsample_t ls;
+                    light_sample(sc, &g, &ls);
+                    Lv[0].p = ls.p;
+                    Lv[0].n = ls.n;
+                    Lv[0].pdf_fwd = ls.pdf;
+                    Lv[0].pdf_rev = 0.0;
+                    Lv[0].beta = cscale(ls.le, 1.0 / ls.pdf);
+                    Lv[0].flags = BV_LIGHT;
+                    Lv[0].mat = -1;
+                    double cost, pdfd;
+                    v3 w = lambert_sample(ls.n, &g, &cost, &pdfd);
+                    col b = cscale(ls.le, cost / (ls.pdf * pdfd));
+                    nL = bd_walk(sc, Lv, 1, j->nL, ls.p, w, b, pdfd, &g, 0);
+                }
+                col own = cs(0.0);
+                for (t = 1; t <= nE; ++t)
+                    for (s = 0; s <= nL; ++s) {
+                        if (s + t < 2 || (s == 1 && t == 1) || s + t - 1 > j->max_len) continue;
+                        const bvert_t* pt = &E[t - 1];
+                        double pt_rev = 0, ptm_rev = 0, qs_rev = 0, qsm_rev = 0;
+                        int trivial = 0, splat = -1;
+                        col c;
+                        v3 va, vb;
+                        int vis = 0;
+                        if (s == 0) {
+                            if (!(pt->flags & BV_EMIT) || t < 2) continue;
+                            const bvert_t* pm = &E[t - 2];
+                            v3 to_prev = norm(sub(pm->p, pt->p));
+                            if (!(dot(pt->n, to_prev) > 0.0)) continue;
+                            const mat_t* m = &sc->mats[pt->mat];
+                            c = cmul(pt->beta, cscale(m->color, m->kd));
+                            int li = j->light_of[pt->prim];
+                            if (li < 0) {
+                                trivial = 1;
+                            } else {
+                                const shape_t* sh = &sc->shapes[sc->lights[li]];
+                                double area = sh->kind == OR_RECT ? sh->length * sh->width
+                                                                  : 4.0 * PI_D * sh->r * sh->r;
+                                pt_rev = (1.0 / sc->n_lights) * (1.0 / area);
+                                if (!(pm->flags & BV_CAM))
+                                    ptm_rev = to_area(INV_PI_D * dot(pt->n, to_prev), pt->p, pm->p, pm->n);
+                            }
+                        } else {
+                            const bvert_t* qs = &Lv[s - 1];
+                            if ((qs->flags & BV_DELTA) || (qs->flags & BV_EMIT)) continue;
+                            if (t == 1) {
+                                v3 dc = norm(sub(qs->p, cam->p));
+                                double cos_c;
+                                if (!camera_project(cam, j->W, j->H, dc, &splat, &cos_c)) continue;
+                                v3 to_cam = neg(dc);
+                                v3 dd = sub(qs->p, cam->p);
+                                double d2 = dot(dd, dd);
+                                double cos_q = dot(qs->n, to_cam);
+                                if (!(cos_q > 0.0)) continue;
+                                col fL = cs(1.0);
+                                if (s >= 2) {
+                                    const bvert_t* qm = &Lv[s - 2];
+                                    v3 to_qm = norm(sub(qm->p, qs->p));
+                                    fL = bsdf_f_v(sc, qs, to_cam, to_qm);
+                                    qsm_rev = to_area(bsdf_pdf_v(sc, qs, to_qm, to_cam), qs->p, qm->p, qm->n);
+                                }
+                                double We = 1.0 / (film_area * cos_c * cos_c * cos_c * cos_c);
+                                double G = cos_q * cos_c / d2;
+                                c = cscale(cmul(qs->beta, fL), We * G);
+                                qs_rev = to_area(1.0 / (cos_c * cos_c * cos_c * film_area), cam->p, qs->p, qs->n);
+                                va = qs->p;
+                                vb = cam->p;
+                                vis = 1;
+                            } else {
+                                if ((pt->flags & BV_DELTA) || (pt->flags & BV_EMIT)) continue;
+                                const bvert_t* pm = &E[t - 2];
+                                v3 dpq = sub(qs->p, pt->p);
+                                double d2 = dot(dpq, dpq);
+                                v3 wpq = mul(dpq, 1.0 / sqrt(d2));
+                                double cos_p = dot(pt->n, wpq), cos_q = -dot(qs->n, wpq);
+                                if (!(cos_p > 0.0 && cos_q > 0.0)) continue;
+                                v3 to_pm = norm(sub(pm->p, pt->p));
+                                col fE = bsdf_f_v(sc, pt, wpq, to_pm);
+                                col fL = cs(1.0);
+                                if (s == 1) {
+                                    pt_rev = to_area(INV_PI_D * cos_q, qs->p, pt->p, pt->n);
+                                } else {
+                                    const bvert_t* qm = &Lv[s - 2];
+                                    v3 to_qm = norm(sub(qm->p, qs->p));
+                                    fL = bsdf_f_v(sc, qs, neg(wpq), to_qm);
+                                    pt_rev = to_area(bsdf_pdf_v(sc, qs, neg(wpq), to_qm), qs->p, pt->p, pt->n);
+                                    qsm_rev = to_area(bsdf_pdf_v(sc, qs, to_qm, neg(wpq)), qs->p, qm->p, qm->n);
+                                }
+                                c = cscale(cmul(cmul(cmul(pt->beta, fE), fL), qs->beta), cos_p * cos_q / d2);
+                                qs_rev = to_area(bsdf_pdf_v(sc, pt, wpq, to_pm), pt->p, qs->p, qs->n);
+                                if (!(pm->flags & BV_CAM))
+                                    ptm_rev = to_area(bsdf_pdf_v(sc, pt, to_pm, wpq), pt->p, pm->p, pm->n);
+                                va = qs->p;
+                                vb = pt->p;
+                                vis = 1;
+                            }
+                        }
+                        if (cis0(c)) continue;
+                        double w = 1.0;
+                        if (s + t > 2 && !trivial) {
+                            double sum = 0.0, ri = 1.0;
+                            for (i = t - 1; i > 0; --i) {
+                                double pr = i == t - 1 ? pt_rev : (i == t - 2 ? ptm_rev : E[i].pdf_rev);
+                                ri *= rm0(pr) / rm0(E[i].pdf_fwd);
+                                int di = i == t - 1 ? 0 : (E[i].flags & BV_DELTA) != 0;
+                                int dp = (E[i - 1].flags & BV_DELTA) != 0;
+                                if (!di && !dp) sum += pow(ri, j->beta);
+                            }
+                            ri = 1.0;
+                            for (i = s - 1; i >= 0; --i) {
+                                double pr = i == s - 1 ? qs_rev : (i == s - 2 ? qsm_rev : Lv[i].pdf_rev);
+                                ri *= rm0(pr) / rm0(Lv[i].pdf_fwd);
+                                int di = i == s - 1 ? 0 : (Lv[i].flags & BV_DELTA) != 0;
+                                int dp = i > 0 ? (Lv[i - 1].flags & BV_DELTA) != 0 : 0;
+                                if (!di && !dp) sum += pow(ri, j->beta);
+                            }
+                            w = 1.0 / (1.0 + sum);
+                        }
+                        if (vis && !visible(sc, va, vb)) continue;
+                        c = cscale(c, w);
+                        if (cnan(c)) continue;
+                        if (splat >= 0) {
+                            double* f = j->film + 3 * (size_t)splat;
+                            f[0] += c.r;
+                            f[1] += c.g;
+                            f[2] += c.b;
+                        } else {
+                            own = cadd(own, c);
+                        }
+                    }
+                double* f = j->film + 3 * pixel;
+                f[0] += own.r;
+                f[1] += own.g;
+                f[2] += own.b;
+            }
+        }
+    free(E);
+    free(Lv);
+    return 0;
+}
+
+int or_render_bdpt(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                   const float* camera, int width, int height, int spp, int n_eye, int n_light,
+                   int max_len, double mis_beta, uint64_t seed, int nthreads, double* mean_rgb)
+{
+    if (width <= 0 || height <= 0 || spp <= 0 || n_eye < 2 || n_light < 1 || !mean_rgb) return 1;
+    scene_t sc;
+    load_scene(&sc, shapes, n_shapes, mats, n_mats, camera, 0);
+    int* light_of = (int*)malloc(sizeof(int) * (size_t)(n_shapes > 0 ? n_shapes : 1));
+    int i, t;
+    for (i = 0; i < n_shapes; ++i) light_of[i] = -1;
+    for (i = 0; i < sc.n_lights; ++i) light_of[sc.lights[i]] = i;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > height) nthreads = height;
+    size_t fn = 3 * (size_t)width * height;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+    bjob_t* jobs = (bjob_t*)malloc(sizeof(bjob_t) * nthreads);
+    for (t = 0; t < nthreads; ++t) {
+        bjob_t* b = &jobs[t];
+        b->sc = &sc;
+        b->W = width;
+        b->H = height;
+        b->spp = spp;
+        b->nE = n_eye;
+        b->nL = n_light;
+        b->max_len = max_len;
+        b->row0 = height * t / nthreads;
+        b->row1 = height * (t + 1) / nthreads;
+        b->beta = mis_beta;
+        b->seed = seed;
+        b->light_of = light_of;
+        b->film = (double*)calloc(fn, sizeof(double));
+        if (nthreads == 1)
+            bdpt_rows(b);
+        else
+            pthread_create(&th[t], 0, bdpt_rows, b);
+    }
+    if (nthreads > 1)
+        for (t = 0; t < nthreads; ++t) pthread_join(th[t], 0);
+    memset(mean_rgb, 0, sizeof(double) * fn);
+    for (t = 0; t < nthreads; ++t) {
+        size_t k;
+        for (k = 0; k < fn; ++k) mean_rgb[k] += jobs[t].film[k] / spp;
+        free(jobs[t].film);
+    }
+    free(th);
+    free(jobs);
+    free(light_of);
     free_scene(&sc);
     return 0;
 }

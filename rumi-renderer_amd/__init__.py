@@ -47,7 +47,7 @@ class Material(Structure):
     _fields_ = [("kind", c_int32), ("kd", c_float), ("ks", c_float), ("exponent", c_float),
                 ("color", c_float * 3), ("eta_in", c_float), ("eta_out", c_float),
                 ("absorption", c_float * 3), ("sigma_s", c_float * 3), ("sigma_a", c_float * 3),
-                ("density", c_float), ("reserved", c_int32 * 4)]
+                ("density", c_float), ("roughness", c_float), ("reserved", c_int32 * 3)]
 
 
 class Shape(Structure):
@@ -92,7 +92,7 @@ class RenderStats(Structure):
 
 RUMI_SHAPE_TRIANGLE, RUMI_SHAPE_SPHERE, RUMI_SHAPE_RECT, RUMI_SHAPE_MESH = 0, 1, 2, 3
 MATERIAL_KINDS = {"matte": 0, "glossy": 1, "specular": 2, "emissive": 3, "dielectric": 4,
-                  "subsurface": 5}
+                  "subsurface": 5, "ggx": 6}
 RUMI_INTEGRATOR_PT, RUMI_INTEGRATOR_BDPT = 0, 1
 
 # Every function include/rumi.h declares, with its ctypes signature.
@@ -308,6 +308,7 @@ class Scene:
             mm.sigma_s = _f3(m.get("sigma_s", (0, 0, 0)))
             mm.sigma_a = _f3(m.get("sigma_a", (0, 0, 0)))
             mm.density = m.get("density", 1.0)
+            mm.roughness = m.get("roughness", 0.0)
         shapes = (Shape * len(desc["shapes"]))()
         for i, sd in enumerate(desc["shapes"]):
             sh = shapes[i]

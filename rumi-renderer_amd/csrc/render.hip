@@ -33,8 +33,8 @@ namespace {
 
 constexpr int kBlock = 128;
 constexpr int kStack = 16;
-constexpr int kKeys = 8;     // material kinds 0..5, 6 = miss
-constexpr int kKeyMiss = 6;
+constexpr int kKeys = 8;     // material kinds 0..6, 7 = miss
+constexpr int kKeyMiss = 7;
 constexpr float kPiF = 3.14159265358979323846f;
 constexpr float kInvPiF = 0.3183098865475128f;
 
@@ -498,6 +498,55 @@ __device__ inline float fresnel(V3 dir, V3 n, float eta_in, float eta_out) {
     return (rs * rs + rp * rp) * 0.5f;
 }
 
+// GGX microfacet reflection (Walter et al. 2007): D with roughness alpha,
+// height-correlated Smith G2, Schlick Fresnel with F0 = colour.  Extension,
+// no reference counterpart.
+__device__ inline float ggx_d(float alpha, float cos_h) {
+    const float a2 = alpha * alpha;
+    const float d = cos_h * cos_h * (a2 - 1.0f) + 1.0f;
+    return a2 / (kPiF * d * d);
+}
+
+__device__ inline float ggx_lambda(float alpha, float c) {
+    const float c2 = c * c;
+    const float t2 = (1.0f - c2) / c2;
+    return 0.5f * (-1.0f + sqrtf(1.0f + alpha * alpha * t2));
+}
+
+__device__ inline V3 ggx_f(const DevMaterial& m, V3 n, V3 wi, V3 wo) {
+    const float ci = dot(n, wi), co = dot(n, wo);
+    if (ci <= 0.0f || co <= 0.0f) return v3(0, 0, 0);
+    const V3 h = normalize(wi + wo);
+    const float D = ggx_d(m.alpha, dot(n, h));
+    const float G = 1.0f / (1.0f + ggx_lambda(m.alpha, ci) + ggx_lambda(m.alpha, co));
+    const float x = 1.0f - fmaxf(dot(wi, h), 0.0f);
+    const float x5 = x * x * x * x * x;
+    const V3 F = v3(m.color[0] + (1.0f - m.color[0]) * x5, m.color[1] + (1.0f - m.color[1]) * x5,
+                    m.color[2] + (1.0f - m.color[2]) * x5);
+    return F * (D * G / (4.0f * ci * co));
+}
+
+// Solid-angle pdf of sampling wi from wo with D(h) cos(theta_h).
+__device__ inline float ggx_pdf(const DevMaterial& m, V3 n, V3 wi, V3 wo) {
+    const float ci = dot(n, wi), co = dot(n, wo);
+    if (ci <= 0.0f || co <= 0.0f) return 0.0f;
+    const V3 h = normalize(wi + wo);
+    const float ch = dot(n, h);
+    return ggx_d(m.alpha, ch) * ch / (4.0f * dot(wo, h));
+}
+
+__device__ inline V3 ggx_sample(const DevMaterial& m, V3 n, V3 wo, Rng& g) {
+    const float u1 = g.uniform(), u2 = g.uniform();
+    const float t2 = m.alpha * m.alpha * u1 / (1.0f - u1);
+    const float ch = 1.0f / sqrtf(1.0f + t2);
+    const float sh = sqrtf(fmaxf(0.0f, 1.0f - ch * ch));
+    const float phi = 2.0f * kPiF * u2;
+    V3 u, w;
+    basis(n, u, w);
+    const V3 h = frame(u, n, w, sh * sinf(phi), ch, sh * cosf(phi));
+    return h * (2.0f * dot(wo, h)) - wo;
+}
+
 // ------------------------------------------------------------ shading
 struct ShadeOut {
     uint32_t pixel;
@@ -662,6 +711,15 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
         le2 = true;
         break;
     }
+    case RUMI_MAT_GGX:
+        // emission is counted on BSDF-sampled hits (le), as after the
+        // reference's Phong lobe; no next-event estimation
+        w = ggx_sample(m, n, wo, g);
+        cost = dot(n, w);
+        pdf_w = ggx_pdf(m, n, w, wo);
+        le2 = true;
+        if (!(cost > 0.0f && pdf_w > 0.0f)) return;
+        break;
     default: {  // dielectric, subsurface boundary: Dielectric::BSDFSampling (dielectric.cpp:43-71)
         const float kr = fresnel(-wo, n, m.eta_in, m.eta_out);
         const float kt = 1.0f - kr;
@@ -733,6 +791,9 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
         break;
     case RUMI_MAT_SPECULAR:
         fr = ld3(m.color) * m.kd;
+        break;
+    case RUMI_MAT_GGX:
+        fr = ggx_f(m, n, w, wo);
         break;
     default: {
         float k;
@@ -826,7 +887,621 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, uint32_t n, Sce
     film_add(film, (uint32_t)__float_as_int(c.w), v3(c.x, c.y, c.z));
 }
 
+// ============================================================ BDPT
+// Bidirectional path tracing after BidirectionalPathTracer::Trace
+// (src/Integrator/bidirectionalpathtracer.cpp:121-394): an eye subpath
+// (ConstructViewPath, :396-711) and a light subpath (ConstructLightPath,
+// :713-1016) per sample, stored in HBM with area-measure forward and reverse
+// pdfs; then every (s, t) connection (:179-388) -- s light vertices, t eye
+// vertices, t = 1 splatting onto the film through the pinhole
+// (Pinhole::FilmPlaneIntersection, pinhole.cpp:138-167) -- weighted by MIS
+// over all strategies with exponent beta (beta = 1 is the reference's
+// balance heuristic, MISWeight :1096-1586).  Dirac vertices (mirror,
+// dielectric) are never connected and enter the MIS sums with remapped
+// pdfs.  Stages: k_bdpt_eye, k_bdpt_light, k_bdpt_connect (one thread per
+// sample and strategy, shadow ray inline).
+
+constexpr uint32_t kVDelta = 1u, kVEmitter = 2u, kVCamera = 4u, kVLight = 8u;
+
+struct BVert {
+    float4 a;  // p.xyz, pdf_fwd (area)
+    float4 b;  // shading normal.xyz, pdf_rev (area)
+    float4 c;  // beta.rgb, flags
+    float4 d;  // texture multiplier rgb, material (int bits)
+};
+
+struct BdptBufs {
+    BVert* eye;       // [vertex][sample]
+    BVert* light;
+    uint8_t* n_eye;
+    uint8_t* n_light;
+    uint32_t nb;      // samples in the batch (row stride)
+};
+
+struct BdptParams {
+    Params base;
+    int32_t n_eye, n_light, max_len;
+    float beta;       // MIS exponent
+    float film_area;  // (2 tan)(2 tan y_ratio) at plane distance 1
+    uint64_t light_seed;
+};
+
+__device__ inline V3 bv_p(const BVert& v) { return v3(v.a.x, v.a.y, v.a.z); }
+__device__ inline V3 bv_n(const BVert& v) { return v3(v.b.x, v.b.y, v.b.z); }
+__device__ inline uint32_t bv_flags(const BVert& v) { return (uint32_t)__float_as_int(v.c.w); }
+__device__ inline int bv_mat(const BVert& v) { return __float_as_int(v.d.w); }
+
+// Non-delta BSDF value f(wi, wo) at a stored vertex (matte, GGX).
+__device__ inline V3 bsdf_f(const SceneView& sv, const BVert& v, V3 wi, V3 wo) {
+    const DevMaterial& m = sv.mats[bv_mat(v)];
+    const V3 tex = v3(v.d.x, v.d.y, v.d.z);
+    if (m.kind == RUMI_MAT_MATTE) return cmul(ld3(m.color), tex) * (kInvPiF * m.kd);
+    if (m.kind == RUMI_MAT_GGX) return cmul(ggx_f(m, bv_n(v), wi, wo), tex);
+    return v3(0, 0, 0);
+}
+
+// Solid-angle pdf of sampling wi given wo at a stored vertex.
+__device__ inline float bsdf_pdf(const SceneView& sv, const BVert& v, V3 wi, V3 wo) {
+    const DevMaterial& m = sv.mats[bv_mat(v)];
+    if (m.kind == RUMI_MAT_MATTE) {
+        // Lambertian::Pdfw (bsdf.cpp:34-41)
+        const float p = kInvPiF * dot(bv_n(v), wi);
+        return p <= 0.0f ? 1e-4f : p;
+    }
+    if (m.kind == RUMI_MAT_GGX) return ggx_pdf(m, bv_n(v), wi, wo);
+    return 0.0f;
+}
+
+// Area density at `to` of a direction sampled at `from` with solid-angle pdf.
+__device__ inline float to_area(float pdf_w, V3 from, V3 to, V3 n_to) {
+    const V3 d = to - from;
+    const float d2 = dot(d, d);
+    return pdf_w * fabsf(dot(n_to, d)) / (d2 * sqrtf(d2));
+}
+
+// Camera direction pdf (Pinhole::Pdfw, pinhole.cpp:58-62) and importance
+// (Pinhole::We, :84-87) for world direction dc; raster position out.
+__device__ inline bool camera_project(const DevCamera& cam, int W, int H, V3 dc, uint32_t& pixel,
+                                      float& cos_c) {
+    const float lx = cam.R[0] * dc.x + cam.R[3] * dc.y + cam.R[6] * dc.z;
+    const float ly = cam.R[1] * dc.x + cam.R[4] * dc.y + cam.R[7] * dc.z;
+    const float lz = cam.R[2] * dc.x + cam.R[5] * dc.y + cam.R[8] * dc.z;
+    if (!(lz > 0.0f)) return false;
+    const float px = lx / lz, py = ly / lz;
+    // invert px = -((x - W/2) + sx) / (W/2) * tan: raster x = W/2 - px (W/2) / tan
+    const float hw = W * 0.5f, hh = H * 0.5f;
+    const float rx = hw - px * hw / cam.tanfov;
+    const float ry = hh - py * hh / (cam.tanfov * cam.y_ratio);
+    if (!(rx >= 0.0f && rx < (float)W && ry >= 0.0f && ry < (float)H)) return false;
+    const int ix = min((int)rx, W - 1), iy = min((int)ry, H - 1);
+    pixel = (uint32_t)iy * (uint32_t)W + (uint32_t)ix;
+    cos_c = lz;
+    return true;
+}
+
+// Surface point, shading normal and texture multiplier of a closest hit
+// (the PT shading stage's rules).
+__device__ inline void surface_at(const SceneView& sv, int32_t prim, V3 o, V3 d, float t, float bu,
+                                  float bv, V3& p, V3& n, V3& tex) {
+    const DevShape& s = sv.shapes[prim];
+    const DevMaterial& m = sv.mats[s.info[1]];
+    p = o + d * t;
+    if (s.info[0] == RUMI_SHAPE_TRIANGLE) {
+        n = (ld3(s.n0) * (1.0f - bu - bv) + ld3(s.n1) * bu) + ld3(s.n2) * bv;
+        const V3 e1 = v3(s.g0[3], s.g1[0], s.g1[1]);
+        const V3 e2 = v3(s.g1[2], s.g1[3], s.g2[0]);
+        const V3 pv = v3(d.y * e2.z - e2.y * d.z, d.z * e2.x - e2.z * d.x, d.x * e2.y - e2.x * d.y);
+        const float det = pv.x * e1.x + pv.y * e1.y + pv.z * e1.z;
+        if (det < 0.0f && m.opaque) n = -n;
+    } else if (s.info[0] == RUMI_SHAPE_SPHERE) {
+        const V3 c = v3(s.g0[0], s.g0[1], s.g0[2]);
+        if (m.kind == RUMI_MAT_EMISSIVE && !(length(o - c) > s.g0[3]))
+            n = normalize(c - p);
+        else
+            n = normalize(p - c);
+    } else {
+        n = v3(s.g0[3], s.g1[0], s.g1[1]);
+    }
+    tex = pattern(sv, s, p, bu, bv);
+}
+
+__device__ inline void store_vert(BVert* arr, uint32_t idx, V3 p, float pdf_fwd, V3 n, V3 beta,
+                                  uint32_t flags, V3 tex, int mat, float bw = 0.0f) {
+    BVert v;
+    v.a = make_float4(p.x, p.y, p.z, pdf_fwd);
+    v.b = make_float4(n.x, n.y, n.z, bw);
+    v.c = make_float4(beta.x, beta.y, beta.z, __int_as_float((int)flags));
+    v.d = make_float4(tex.x, tex.y, tex.z, __int_as_float(mat));
+    arr[idx] = v;
+}
+
+// One random walk (eye: radiance transport; light: importance transport,
+// no 1/eta^2 on refraction) writing vertices start..; returns the count.
+template <bool EYE>
+__device__ int random_walk(const SceneView& sv, BVert* arr, uint32_t nb, uint32_t j, int start,
+                           int max_verts, V3 o, V3 d, V3 beta, float pdf_w, Rng& g,
+                           int32_t* s_ref, float* s_tn, uint32_t tid, bool spill) {
+    int nv = start;
+    bool in_obj = false;
+    V3 prev_p = o;
+    while (nv < max_verts) {
+        Ray r{o.x, o.y, o.z, d.x, d.y, d.z, FLT_MAX, 0.0f};
+        float bu = 0.0f, bv = 0.0f;
+        int32_t nn = 0, np = 0;
+        const int32_t prim =
+            spill ? traverse<kBlock, kStack, false, true, true, false, true>(
+                        sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_ref, s_tn, tid)
+                  : traverse<kBlock, kStack, false, true, true, false, false>(
+                        sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_ref, s_tn, tid);
+        if (prim < 0) break;
+        V3 p, n, tex;
+        surface_at(sv, prim, o, d, r.tmin, bu, bv, p, n, tex);
+        const int mi = sv.shapes[prim].info[1];
+        const DevMaterial& m = sv.mats[mi];
+        const float pdf_fwd = to_area(pdf_w, prev_p, p, n);
+        if (m.kind == RUMI_MAT_EMISSIVE) {
+            // The walk ends on an emitter (bidirectionalpathtracer.cpp:693-699).
+            // an emitter vertex is never scattered from, so its reverse-pdf
+            // word carries the primitive id instead
+            store_vert(arr, (uint32_t)nv * nb + j, p, pdf_fwd, n, beta, kVEmitter, tex, mi,
+                       __int_as_float(prim));
+            ++nv;
+            break;
+        }
+        const uint32_t slot = (uint32_t)nv * nb + j;
+        ++nv;
+        const bool was_in = in_obj;
+        const V3 wo = -d;
+        V3 w;
+        float pdf_next = 0.0f, pdf_rev = 0.0f;
+        bool delta = false;
+        V3 weight;  // f |cos| / pdf of this bounce
+        if (m.kind == RUMI_MAT_MATTE) {
+            float cost, pdf;
+            w = lambert_sample(n, g, cost, pdf);
+            weight = cmul(ld3(m.color), tex) * (m.kd * kInvPiF * cost / pdf);
+            pdf_next = pdf;
+            const float pr = kInvPiF * dot(n, wo);
+            pdf_rev = pr <= 0.0f ? 1e-4f : pr;
+        } else if (m.kind == RUMI_MAT_GGX) {
+            w = ggx_sample(m, n, wo, g);
+            const float cost = dot(n, w);
+            pdf_next = ggx_pdf(m, n, w, wo);
+            if (!(cost > 0.0f && pdf_next > 0.0f)) {
+                store_vert(arr, slot, p, pdf_fwd, n, beta, 0u, tex, mi);
+                break;
+            }
+            weight = cmul(ggx_f(m, n, w, wo), tex) * (cost / pdf_next);
+            pdf_rev = ggx_pdf(m, n, wo, w);
+        } else if (m.kind == RUMI_MAT_SPECULAR) {
+            float won = dot(wo, n);
+            if (won <= 0.0f) won = 1e-4f;
+            w = -wo + n * (2.0f * won);
+            weight = ld3(m.color) * m.kd;
+            delta = true;
+        } else {  // dielectric
+            const float kr = fresnel(-wo, n, m.eta_in, m.eta_out);
+            const float c = dot(wo, n);
+            delta = true;
+            if (kr != 0.0f && g.uniform() <= kr) {
+                float won = c;
+                V3 nn = n;
+                if (won < 1e-4f) {
+                    if (won > -1e-4f)
+                        won = 1e-4f;
+                    else {
+                        nn = -n;
+                        won = -won;
+                    }
+                }
+                w = -wo + nn * (2.0f * won);
+                in_obj = c < 0.0f;
+                weight = v3(1, 1, 1);
+            } else {
+                float eta = m.eta_in / m.eta_out;
+                V3 nn = n;
+                float ci = c;
+                if (ci < 0.0f) {
+                    nn = -n;
+                    eta = m.eta_out / m.eta_in;
+                    ci = -ci;
+                }
+                const float inv = 1.0f / eta;
+                const float ct = sqrtf(1.0f - (1.0f - ci * ci) * (inv * inv));
+                w = -wo * inv + nn * (ci * inv - ct);
+                in_obj = c > 0.0f;
+                // Dielectric::Fr (dielectric.cpp:73-98): radiance is scaled by
+                // 1/eta^2, eta = eta_in/eta_out from inside, eta_out/eta_in from
+                // outside; importance carried by light paths is not scaled.
+                const float er = was_in ? m.eta_in / m.eta_out : m.eta_out / m.eta_in;
+                const float scale = EYE ? 1.0f / (er * er) : 1.0f;
+                weight = v3(scale, scale, scale);
+            }
+            // Beer-Lambert along a segment travelled inside (dielectric.cpp:94-95)
+            if (was_in)
+                weight = cmul(weight, v3(expf(-m.absorption[0] * r.tmin), expf(-m.absorption[1] * r.tmin),
+                                         expf(-m.absorption[2] * r.tmin)));
+        }
+        store_vert(arr, slot, p, pdf_fwd, n, beta, delta ? kVDelta : 0u, tex, mi);
+        // reverse pdf of the previous vertex (area measure at prev)
+        if (nv >= 2) {
+            BVert& pv = arr[(uint32_t)(nv - 2) * nb + j];
+            pv.b.w = delta ? 0.0f : to_area(pdf_rev, p, prev_p, v3(pv.b.x, pv.b.y, pv.b.z));
+        }
+        beta = cmul(beta, weight);
+        pdf_w = delta ? 0.0f : pdf_next;
+        prev_p = p;
+        o = p;
+        d = w;
+        if (m.kind != RUMI_MAT_DIELECTRIC) in_obj = false;
+    }
+    return nv;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bdpt_eye(BdptBufs bb, uint64_t base, BdptParams bp,
+                                                     SceneView sv, bool spill) {
+    __shared__ int32_t s_ref[kStack * kBlock];
+    __shared__ float s_tn[kStack * kBlock];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t j = blockIdx.x * kBlock + tid;
+    if (j >= bb.nb) return;
+    const Params& prm = bp.base;
+    const uint64_t k = base + j;
+    const uint32_t pixel = (uint32_t)(k % prm.pixels);
+    const uint64_t s = k / prm.pixels;
+    Rng g = rng_seed(prm.seed + s * 0x9E3779B97F4A7C15ull, pixel);
+    const DevCamera& cam = sv.cam;
+    const int x = (int)(pixel % (uint32_t)prm.width);
+    const int y = (int)(pixel / (uint32_t)prm.width);
+    const float sx = g.uniform(), sy = g.uniform();
+    const float hw = prm.width * 0.5f, hh = prm.height * 0.5f;
+    const float px = -(((float)x - hw) + sx) / hw * cam.tanfov;
+    const float py = -(((float)y - hh) + sy) / hh * cam.tanfov * cam.y_ratio;
+    const V3 l = normalize(v3(px, py, 1.0f));
+    const V3 d = v3(cam.R[0] * l.x + cam.R[1] * l.y + cam.R[2] * l.z,
+                    cam.R[3] * l.x + cam.R[4] * l.y + cam.R[5] * l.z,
+                    cam.R[6] * l.x + cam.R[7] * l.y + cam.R[8] * l.z);
+    const V3 cp = v3(cam.p[0], cam.p[1], cam.p[2]);
+    const V3 cdir = v3(cam.R[2], cam.R[5], cam.R[8]);
+    store_vert(bb.eye, j, cp, 1.0f, cdir, v3(1, 1, 1), kVCamera, v3(1, 1, 1), -1);
+    // Pinhole::Pdfw: 1 / (cos^3 film_area) at plane distance 1
+    const float pdf_dir = 1.0f / (l.z * l.z * l.z * bp.film_area);
+    const int nv = random_walk<true>(sv, bb.eye, bb.nb, j, 1, bp.n_eye, cp, d, v3(1, 1, 1), pdf_dir,
+                                     g, s_ref, s_tn, tid, spill);
+    bb.n_eye[j] = (uint8_t)nv;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bdpt_light(BdptBufs bb, uint64_t base, BdptParams bp,
+                                                       SceneView sv, bool spill) {
+    __shared__ int32_t s_ref[kStack * kBlock];
+    __shared__ float s_tn[kStack * kBlock];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t j = blockIdx.x * kBlock + tid;
+    if (j >= bb.nb) return;
+    const Params& prm = bp.base;
+    const uint64_t k = base + j;
+    const uint32_t pixel = (uint32_t)(k % prm.pixels);
+    const uint64_t s = k / prm.pixels;
+    Rng g = rng_seed(bp.light_seed + s * 0x9E3779B97F4A7C15ull, pixel);
+    if (sv.n_lights == 0) {
+        bb.n_light[j] = 0;
+        return;
+    }
+    // Scene::LightsUniformSampling (scene.cpp:91-97) + AreaLight::SolidAngleSampling
+    // (arealight.cpp:31-60): a point on the emitter, a cosine-weighted direction.
+    uint32_t li = (uint32_t)(g.uniform() * (float)sv.n_lights);
+    if (li >= sv.n_lights) li = sv.n_lights - 1;
+    const DevLight& L = sv.lights[li];
+    V3 p0, n0;
+    if (L.kind == RUMI_SHAPE_RECT) {
+        const float a = g.uniform(), b = g.uniform();
+        p0 = (ld3(L.p) + ld3(L.l) * a) + ld3(L.w) * b;
+        n0 = ld3(L.n);
+    } else {
+        const float z = 1.0f - 2.0f * g.uniform(), phi = 2.0f * kPiF * g.uniform();
+        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        p0 = ld3(L.p) + v3(rr * cosf(phi), rr * sinf(phi), z) * L.l[0];
+        n0 = normalize(p0 - ld3(L.p));
+    }
+    const float pdf_pos = 1.0f / (float)sv.n_lights * L.p[3];
+    const V3 le = ld3(L.le);
+    store_vert(bb.light, j, p0, pdf_pos, n0, le * (1.0f / pdf_pos), kVLight, v3(1, 1, 1), -1);
+    float cost, pdf_dir;
+    const V3 w = lambert_sample(n0, g, cost, pdf_dir);
+    const V3 beta = le * (cost / (pdf_pos * pdf_dir));
+    const int nv = random_walk<false>(sv, bb.light, bb.nb, j, 1, bp.n_light, p0, w, beta, pdf_dir, g,
+                                      s_ref, s_tn, tid, spill);
+    bb.n_light[j] = (uint8_t)nv;
+}
+
+__device__ inline float remap0(float f) { return f != 0.0f ? f : 1.0f; }
+
+__device__ inline float mis_pow(float r, float beta) { return beta == 1.0f ? r : powf(r, beta); }
+
+__global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t base, BdptParams bp,
+                                                         SceneView sv, const int2* strategies,
+                                                         uint32_t n_strat, float* film, bool spill) {
+    __shared__ int32_t s_ref[kStack * kBlock];
+    __shared__ float s_tn[kStack * kBlock];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + tid;
+    if (gid >= (uint64_t)n_strat * bb.nb) return;
+    const uint32_t q = (uint32_t)(gid / bb.nb);
+    const uint32_t j = (uint32_t)(gid % bb.nb);
+    const int s = strategies[q].x, t = strategies[q].y;
+    const int nE = bb.n_eye[j], nL = bb.n_light[j];
+    if (t > nE || s > nL) return;
+    const uint32_t nb = bb.nb;
+    const Params& prm = bp.base;
+    const BVert pt = bb.eye[(uint32_t)(t - 1) * nb + j];
+    const uint32_t ptf = bv_flags(pt);
+    V3 c;
+    // the sample's own pixel; a t = 1 connection splats elsewhere
+    uint32_t pixel = (uint32_t)((base + j) % prm.pixels);
+    bool mis_trivial = false;
+    float pt_rev = 0.0f, ptm_rev = 0.0f, qs_rev = 0.0f, qsm_rev = 0.0f;
+    bool visible_test = false;
+    V3 va, vb;
+    if (s == 0) {
+        if (!(ptf & kVEmitter) || t < 2) return;
+        const BVert pm = bb.eye[(uint32_t)(t - 2) * nb + j];
+        const V3 to_prev = normalize(bv_p(pm) - bv_p(pt));
+        const DevMaterial& m = sv.mats[bv_mat(pt)];
+        // Emissive::DirectLighting front-face rule (emissive.cpp:60-74)
+        if (!(dot(bv_n(pt), to_prev) > 0.0f)) return;
+        c = cmul(v3(pt.c.x, pt.c.y, pt.c.z), ld3(m.color) * m.kd);
+        const int prim = __float_as_int(pt.b.w);
+        const int li = __float_as_int(sv.shapes[prim].uv2[3]);
+        if (li < 0) {
+            // an emitter that is not a sampled light: no other strategy
+            mis_trivial = true;
+        } else {
+            const DevLight& L = sv.lights[li];
+            pt_rev = 1.0f / (float)sv.n_lights * L.p[3];
+            const float cl = fmaxf(dot(bv_n(pt), to_prev), 0.0f);
+            if (!(bv_flags(pm) & kVCamera)) ptm_rev = to_area(kInvPiF * cl, bv_p(pt), bv_p(pm), bv_n(pm));
+        }
+    } else {
+        const BVert qs = bb.light[(uint32_t)(s - 1) * nb + j];
+        const uint32_t qsf = bv_flags(qs);
+        if ((qsf & kVDelta) || (qsf & kVEmitter)) return;
+        if (t == 1) {
+            // light tracing: connect qs to the pinhole, splat (t_idx == 0 case, :367-380)
+            const V3 cp = v3(sv.cam.p[0], sv.cam.p[1], sv.cam.p[2]);
+            const V3 dc = normalize(bv_p(qs) - cp);
+            float cos_c;
+            if (!camera_project(sv.cam, prm.width, prm.height, dc, pixel, cos_c)) return;
+            const V3 to_cam = -dc;
+            const float d2 = dot(bv_p(qs) - cp, bv_p(qs) - cp);
+            V3 fL;
+            float cos_q;
+            if (s == 1) {
+                cos_q = dot(bv_n(qs), to_cam);
+                if (!(cos_q > 0.0f)) return;
+                fL = v3(1, 1, 1);
+            } else {
+                const BVert qm = bb.light[(uint32_t)(s - 2) * nb + j];
+                cos_q = dot(bv_n(qs), to_cam);
+                if (!(cos_q > 0.0f)) return;
+                fL = bsdf_f(sv, qs, to_cam, normalize(bv_p(qm) - bv_p(qs)));
+                qsm_rev = to_area(bsdf_pdf(sv, qs, normalize(bv_p(qm) - bv_p(qs)), to_cam), bv_p(qs),
+                                  bv_p(qm), bv_n(qm));
+            }
+            // Pinhole::We = 1 / (A cos^4) (pinhole.cpp:84-87); G = cos_q cos_c / d^2
+            const float We = 1.0f / (bp.film_area * cos_c * cos_c * cos_c * cos_c);
+            const float G = cos_q * cos_c / d2;
+            c = cmul(v3(qs.c.x, qs.c.y, qs.c.z), fL) * (We * G);
+            qs_rev = to_area(1.0f / (cos_c * cos_c * cos_c * bp.film_area), cp, bv_p(qs), bv_n(qs));
+            visible_test = true;
+            va = bv_p(qs);
+            vb = cp;
+        } else {
+            if ((ptf & kVDelta) || (ptf & kVEmitter)) return;
+            const BVert pm = bb.eye[(uint32_t)(t - 2) * nb + j];
+            const V3 d_pq = bv_p(qs) - bv_p(pt);
+            const float d2 = dot(d_pq, d_pq);
+            const V3 w_pq = d_pq * (1.0f / sqrtf(d2));
+            const float cos_p = dot(bv_n(pt), w_pq);
+            const float cos_q = -dot(bv_n(qs), w_pq);
+            if (!(cos_p > 0.0f && cos_q > 0.0f)) return;
+            const V3 to_pm = normalize(bv_p(pm) - bv_p(pt));
+            const V3 fE = bsdf_f(sv, pt, w_pq, to_pm);
+            V3 fL;
+            if (s == 1) {
+                fL = v3(1, 1, 1);
+                pt_rev = to_area(kInvPiF * cos_q, bv_p(qs), bv_p(pt), bv_n(pt));
+            } else {
+                const BVert qm = bb.light[(uint32_t)(s - 2) * nb + j];
+                const V3 to_qm = normalize(bv_p(qm) - bv_p(qs));
+                fL = bsdf_f(sv, qs, -w_pq, to_qm);
+                pt_rev = to_area(bsdf_pdf(sv, qs, -w_pq, to_qm), bv_p(qs), bv_p(pt), bv_n(pt));
+                qsm_rev = to_area(bsdf_pdf(sv, qs, to_qm, -w_pq), bv_p(qs), bv_p(qm), bv_n(qm));
+            }
+            const float G = cos_p * cos_q / d2;
+            c = cmul(cmul(cmul(v3(pt.c.x, pt.c.y, pt.c.z), fE), fL), v3(qs.c.x, qs.c.y, qs.c.z)) * G;
+            qs_rev = to_area(bsdf_pdf(sv, pt, w_pq, to_pm), bv_p(pt), bv_p(qs), bv_n(qs));
+            if (!(bv_flags(pm) & kVCamera))
+                ptm_rev = to_area(bsdf_pdf(sv, pt, to_pm, w_pq), bv_p(pt), bv_p(pm), bv_n(pm));
+            visible_test = true;
+            va = bv_p(qs);
+            vb = bv_p(pt);
+        }
+    }
+    if (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f) return;
+    // MIS weight over all strategies of this path (power heuristic with exponent beta)
+    float w = 1.0f;
+    if (s + t > 2 && !mis_trivial) {
+        float sum = 0.0f, ri = 1.0f;
+        for (int i = t - 1; i > 0; --i) {
+            const BVert vi = bb.eye[(uint32_t)i * nb + j];
+            const float pr = i == t - 1 ? pt_rev : (i == t - 2 ? ptm_rev : vi.b.w);
+            ri *= remap0(pr) / remap0(vi.a.w);
+            const bool di = i == t - 1 ? false : (bv_flags(vi) & kVDelta) != 0;
+            const bool dprev = (bv_flags(bb.eye[(uint32_t)(i - 1) * nb + j]) & kVDelta) != 0;
+            if (!di && !dprev) sum += mis_pow(ri, bp.beta);
+        }
+        ri = 1.0f;
+        for (int i = s - 1; i >= 0; --i) {
+            const BVert vi = bb.light[(uint32_t)i * nb + j];
+            const float pr = i == s - 1 ? qs_rev : (i == s - 2 ? qsm_rev : vi.b.w);
+            ri *= remap0(pr) / remap0(vi.a.w);
+            const bool di = i == s - 1 ? false : (bv_flags(vi) & kVDelta) != 0;
+            const bool dprev = i > 0 ? (bv_flags(bb.light[(uint32_t)(i - 1) * nb + j]) & kVDelta) != 0 : false;
+            if (!di && !dprev) sum += mis_pow(ri, bp.beta);
+        }
+        w = 1.0f / (1.0f + sum);
+    }
+    if (visible_test) {
+        const V3 dv = vb - va;
+        const float dist = length(dv);
+        const V3 dir = dv * (1.0f / dist);
+        Ray r{va.x, va.y, va.z, dir.x, dir.y, dir.z, dist, 0.0f};
+        float hu, hv;
+        int32_t nn = 0, np = 0;
+        if (spill)
+            traverse<kBlock, kStack, true, true, true, false, true>(sv.nodes, sv.prims, sv.rects, sv.root,
+                                                                    r, false, hu, hv, nn, np, s_ref, s_tn, tid);
+        else
+            traverse<kBlock, kStack, true, true, true, false, false>(sv.nodes, sv.prims, sv.rects, sv.root,
+                                                                     r, false, hu, hv, nn, np, s_ref, s_tn, tid);
+        if (r.tmin > 0.0f) return;
+    }
+    film_add(film, pixel, c * w);
+}
+
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+SceneView scene_view(const rumi_scene* sc) {
+    SceneView sv;
+    sv.nodes = (const float4*)sc->nodes;
+    sv.prims = (const float4*)sc->prims;
+    sv.rects = (const float4*)sc->rects;
+    sv.shapes = (const DevShape*)sc->shapes;
+    sv.mats = (const DevMaterial*)sc->materials;
+    sv.lights = (const DevLight*)sc->lights;
+    sv.textures = (const DevTexture*)sc->textures;
+    sv.texels = (const float*)sc->texels;
+    for (int k = 0; k < 3; ++k) {
+        sv.root.lo[k] = sc->root_lo[k];
+        sv.root.hi[k] = sc->root_hi[k];
+    }
+    sv.root.ref = sc->root_ref;
+    sv.n_lights = sc->n_lights;
+    sv.med = sc->medium;
+    sv.cam = sc->camera;
+    return sv;
+}
+
+
+int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
+                rumi_render_stats* stats, hipStream_t stream) {
+    (void)ctx;
+    if (sc->medium.bound != RUMI_MEDIUM_NONE)
+        return fail(RUMI_ERR_UNSUPPORTED, "rumi_render: the BDPT integrator does not handle media");
+    const uint32_t bad = (1u << RUMI_MAT_GLOSSY) | (1u << RUMI_MAT_SUBSURFACE);
+    if (sc->material_kinds & bad)
+        return fail(RUMI_ERR_UNSUPPORTED,
+                    "rumi_render: the BDPT integrator handles matte, GGX, specular, dielectric and "
+                    "emissive materials");
+    const int nE = desc->n_eye > 0 ? desc->n_eye : desc->max_depth + 1;
+    const int nL = desc->n_light > 0 ? desc->n_light : desc->max_depth + 1;
+    if (nE < 2 || nE > 64 || nL < 1 || nL > 64)
+        return fail(RUMI_ERR_INVALID, "rumi_render: BDPT vertex caps must be in [2,64] (eye), [1,64] (light)");
+    if (!(desc->mis_beta > 0.0f)) return fail(RUMI_ERR_INVALID, "rumi_render: mis_beta must be > 0");
+    const int max_len = desc->max_depth;
+    // all (s, t) with t >= 1, s + t >= 2, path edges s + t - 1 <= max_len,
+    // except (1, 1) (the camera ray hitting the light covers that path)
+    std::vector<int2> strat;
+    for (int t = 1; t <= nE; ++t)
+        for (int s = 0; s <= nL; ++s) {
+            if (s + t < 2 || (s == 1 && t == 1) || s + t - 1 > max_len) continue;
+            strat.push_back(make_int2(s, t));
+        }
+    const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
+    const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
+    uint32_t nb = desc->queue_capacity ? desc->queue_capacity : (1u << 19);
+    if ((uint64_t)nb > total) nb = (uint32_t)total;
+    const size_t vbytes = sizeof(BVert) * (size_t)nb;
+    const size_t need = vbytes * (size_t)(nE + nL) + 2 * (size_t)nb + sizeof(int2) * strat.size() + 4096;
+    if (sc->work_bytes < need) {
+        if (sc->work) (void)hipFree(sc->work);
+        sc->work = nullptr;
+        sc->work_bytes = 0;
+        if (hipMalloc(&sc->work, need) != hipSuccess)
+            return fail(RUMI_ERR_NO_MEMORY, "rumi_render: hipMalloc of %zu BDPT bytes failed", need);
+        sc->work_bytes = need;
+    }
+    char* w = (char*)sc->work;
+    auto take = [&](size_t bytes) {
+        char* p = w;
+        w += (bytes + 255) & ~(size_t)255;
+        return (void*)p;
+    };
+    BdptBufs bb;
+    bb.eye = (BVert*)take(vbytes * (size_t)nE);
+    bb.light = (BVert*)take(vbytes * (size_t)nL);
+    bb.n_eye = (uint8_t*)take(nb);
+    bb.n_light = (uint8_t*)take(nb);
+    int2* d_strat = (int2*)take(sizeof(int2) * strat.size());
+    if ((size_t)(w - (char*)sc->work) > sc->work_bytes)
+        return fail(RUMI_ERR_INVALID, "rumi_render: internal BDPT work-buffer layout overflow");
+    if (hipMemcpyAsync(d_strat, strat.data(), sizeof(int2) * strat.size(), hipMemcpyHostToDevice,
+                       stream) != hipSuccess)
+        return fail(RUMI_ERR_HIP, "rumi_render: strategy upload failed");
+
+    SceneView sv = scene_view(sc);
+    BdptParams bp;
+    bp.base.width = desc->width;
+    bp.base.height = desc->height;
+    bp.base.max_depth = desc->max_depth;
+    bp.base.pixels = pixels;
+    bp.base.seed = desc->seed;
+    bp.n_eye = nE;
+    bp.n_light = nL;
+    bp.max_len = max_len;
+    bp.beta = desc->mis_beta;
+    bp.film_area = (2.0f * sc->camera.tanfov) * (2.0f * sc->camera.tanfov * sc->camera.y_ratio);
+    bp.light_seed = desc->seed ^ 0x2545F4914F6CDD1Dull;
+    const bool spill = sc->max_depth > kStack;
+
+    hipEvent_t ev0, ev1;
+    (void)hipEventCreate(&ev0);
+    (void)hipEventCreate(&ev1);
+    (void)hipEventRecord(ev0, stream);
+    const size_t film_n = 3 * (size_t)pixels;
+    hipLaunchKernelGGL(k_zero, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n);
+    uint64_t batches = 0;
+    for (uint64_t base = 0; base < total; base += nb) {
+        bb.nb = (uint32_t)std::min<uint64_t>(nb, total - base);
+        hipLaunchKernelGGL(k_bdpt_eye, dim3(grid_of(bb.nb)), dim3(kBlock), 0, stream, bb, base, bp, sv, spill);
+        hipLaunchKernelGGL(k_bdpt_light, dim3(grid_of(bb.nb)), dim3(kBlock), 0, stream, bb, base, bp, sv,
+                           spill);
+        const uint64_t threads = (uint64_t)bb.nb * strat.size();
+        hipLaunchKernelGGL(k_bdpt_connect, dim3(grid_of(threads)), dim3(kBlock), 0, stream, bb, base, bp,
+                           sv, (const int2*)d_strat, (uint32_t)strat.size(), film, spill);
+        ++batches;
+    }
+    hipLaunchKernelGGL(k_scale, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n,
+                       1.0f / (float)desc->spp);
+    (void)hipEventRecord(ev1, stream);
+    int rc = RUMI_OK;
+    if (hipStreamSynchronize(stream) != hipSuccess)
+        rc = fail(RUMI_ERR_HIP, "rumi_render: %s", hipGetErrorString(hipGetLastError()));
+    else if (hipGetLastError() != hipSuccess)
+        rc = fail(RUMI_ERR_HIP, "rumi_render: a BDPT kernel launch failed");
+    float ms = 0.0f;
+    if (rc == RUMI_OK) (void)hipEventElapsedTime(&ms, ev0, ev1);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (stats) {
+        stats->samples = total;
+        stats->extension_rays = 0;
+        stats->shadow_rays = 0;
+        stats->iterations = batches;
+        stats->milliseconds = ms;
+    }
+    return rc;
+}
+
 
 }  // namespace
 
@@ -837,8 +1512,8 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
     if (!ctx) return fail(RUMI_ERR_NO_DEVICE, "rumi_render: null context");
     if (!sc || !desc || !film) return fail(RUMI_ERR_INVALID, "rumi_render: null argument");
     if (sc->device != ctx->device) return fail(RUMI_ERR_INVALID, "rumi_render: scene is on another device");
-    if (desc->integrator != RUMI_INTEGRATOR_PT)
-        return fail(RUMI_ERR_UNSUPPORTED, "rumi_render: integrator %d not available", desc->integrator);
+    if (desc->integrator != RUMI_INTEGRATOR_PT && desc->integrator != RUMI_INTEGRATOR_BDPT)
+        return fail(RUMI_ERR_UNSUPPORTED, "rumi_render: unknown integrator %d", desc->integrator);
     if (desc->width <= 0 || desc->height <= 0 || desc->spp <= 0 || desc->max_depth <= 0)
         return fail(RUMI_ERR_INVALID, "rumi_render: width, height, spp and max_depth must be > 0");
     if (desc->max_depth > 1000) return fail(RUMI_ERR_INVALID, "rumi_render: max_depth > 1000");
@@ -846,6 +1521,7 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
         return fail(RUMI_ERR_INVALID, "rumi_render: image too large");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(RUMI_ERR_HIP, "hipSetDevice failed");
     hipStream_t stream = (hipStream_t)stream_v;
+    if (desc->integrator == RUMI_INTEGRATOR_BDPT) return render_bdpt(ctx, sc, desc, film, stats, stream);
 
     const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
     const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
@@ -889,23 +1565,7 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
     if ((size_t)(w - (char*)sc->work) > sc->work_bytes)
         return fail(RUMI_ERR_INVALID, "rumi_render: internal work-buffer layout overflow");
 
-    SceneView sv;
-    sv.nodes = (const float4*)sc->nodes;
-    sv.prims = (const float4*)sc->prims;
-    sv.rects = (const float4*)sc->rects;
-    sv.shapes = (const DevShape*)sc->shapes;
-    sv.mats = (const DevMaterial*)sc->materials;
-    sv.lights = (const DevLight*)sc->lights;
-    sv.textures = (const DevTexture*)sc->textures;
-    sv.texels = (const float*)sc->texels;
-    for (int k = 0; k < 3; ++k) {
-        sv.root.lo[k] = sc->root_lo[k];
-        sv.root.hi[k] = sc->root_hi[k];
-    }
-    sv.root.ref = sc->root_ref;
-    sv.n_lights = sc->n_lights;
-    sv.med = sc->medium;
-    sv.cam = sc->camera;
+    SceneView sv = scene_view(sc);
     Params prm;
     prm.width = desc->width;
     prm.height = desc->height;

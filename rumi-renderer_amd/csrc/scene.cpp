@@ -81,7 +81,7 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
     uint32_t kinds = 0;
     for (uint32_t i = 0; i < d->n_materials; ++i) {
         const rumi_material& m = d->materials[i];
-        if (m.kind < RUMI_MAT_MATTE || m.kind > RUMI_MAT_SUBSURFACE)
+        if (m.kind < RUMI_MAT_MATTE || m.kind > RUMI_MAT_GGX)
             return fail(RUMI_ERR_INVALID, "material %u: unknown kind %d", i, m.kind);
         DevMaterial& o = mats[i];
         std::memset(&o, 0, sizeof(o));
@@ -96,6 +96,9 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
         o.eta_in = m.eta_in;
         o.eta_out = m.eta_out;
         o.density = m.density > 0.0f ? m.density : 1.0f;
+        o.alpha = m.roughness;
+        if (m.kind == RUMI_MAT_GGX && !(m.roughness > 0.0f && m.roughness <= 1.0f))
+            return fail(RUMI_ERR_INVALID, "material %u: GGX roughness must be in (0, 1]", i);
         for (int k = 0; k < 3; ++k) {
             o.color[k] = m.color[k];
             o.absorption[k] = m.absorption[k];
@@ -319,6 +322,16 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
             return fail(RUMI_ERR_UNSUPPORTED, "shape %d: only rect and sphere area lights", p.shape);
         }
         lights.push_back(L);
+    }
+
+    // light index of each emitter primitive (word uv2[3]; -1 = not a light)
+    for (uint32_t i = 0; i < n; ++i) {
+        const int32_t none = -1;
+        std::memcpy(&shapes[i].uv2[3], &none, 4);
+    }
+    for (size_t li = 0; li < lights.size(); ++li) {
+        const int32_t idx = (int32_t)li;
+        std::memcpy(&shapes[lights[li].prim].uv2[3], &idx, 4);
     }
 
     // ------------------------------------------------------------ textures

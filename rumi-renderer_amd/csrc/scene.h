@@ -14,7 +14,7 @@ struct alignas(16) DevMaterial {
     int32_t kind, opaque, pad0, pad1;
     float kd, ks, e, kd_pdf;                 // kd_pdf = kd / (kd + ks)
     float phong_norm, eta_in, eta_out, avg_sigmat;  // phong_norm = (e + 1) / 2pi
-    float absorb, density, pad2, pad3;       // absorb = avg(sigma_a / sigma_t)
+    float absorb, density, alpha, pad3;      // absorb = avg(sigma_a / sigma_t); GGX alpha
     float color[4];
     float absorption[4];
     float sigma_s[4];

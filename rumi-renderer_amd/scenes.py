@@ -83,3 +83,23 @@ def icosphere_mesh(freq: int, seed: int, amplitude: float, center, scale: float)
     xyz, idx = rumi.make_icosphere(freq, seed, amplitude)
     xyz = (xyz * np.float32(scale) + np.asarray(center, np.float32)).astype(np.float32)
     return xyz, idx
+
+
+def cornell_bdpt(width: int = 1024, height: int = 1024, freq: int = 100,
+                 roughness: float = 0.2) -> dict:
+    """The Cornell box with a seeded noisy icosphere (20 freq^2 triangles,
+    200,000 at freq 100, PCG32 seed 3, radius 1 +- 0.05) in GGX material of
+    roughness 0.2 in place of the diffuse sphere; the glass sphere stays."""
+    sc = cornell_box(width, height)
+    sc["shapes"] = [s for s in sc["shapes"] if not (s["kind"] == "sphere" and s["material"] == 0)]
+    sc["materials"].append(dict(kind="ggx", color=(0.9, 0.9, 0.9), roughness=roughness))
+    xyz, idx = icosphere_mesh(freq, 3, 0.05, center=(-0.2, 0.17, -0.15), scale=0.16)
+    sc["shapes"].append(dict(kind="mesh", xyz=xyz, idx=idx, material=len(sc["materials"]) - 1))
+    return sc
+
+
+def diffuse_box(width: int = 64, height: int = 64) -> dict:
+    """The Cornell box with both spheres Lambertian (no dirac vertices)."""
+    sc = cornell_box(width, height)
+    sc["materials"][4] = dict(kind="matte", kd=1.0, color=(0.5, 0.5, 0.5))
+    return sc
