@@ -142,6 +142,8 @@ def main():
                "host_cpu": _cpu_model(), "nproc": os.cpu_count()}
         log(f"CPU baseline: {cpu['value']:.3f} Mrays/s")
 
+    traffic = measured_traffic("closest-hit", n)
+
     out = {
         "metric": METRIC,
         "value": round(mrays_c, 2),
@@ -163,7 +165,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved_c, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved_c / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved_c / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_ray": round(b_ray_c, 1), "nodes_per_ray": round(nodes_c, 3),
             "tris_per_ray": round(tris_c, 3),
         },
@@ -178,6 +180,23 @@ def main():
         "path_tracing": pt,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(label, rays):
+    """Bytes per launch that rocprofv3's FETCH_SIZE + WRITE_SIZE counters read
+    for this kernel on this workload, each scaled by what the counter reads
+    for a device copy of known size (profiles/k_trace_traffic.json, written
+    from the two --pmc runs listed in profiles/k_trace_icosphere1p3m_8Mrays.md).
+    None when no measurement of this workload is at hand."""
+    path = os.path.join(ROOT, "profiles", "k_trace_traffic.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"][label]
+    except (OSError, KeyError, ValueError):
+        return None
+    if int(k["rays"]) != rays:
+        return None
+    return int(round(k["bytes_per_launch"]))
 
 
 def render_legs(rumi, ctx, args, log):

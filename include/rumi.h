@@ -276,6 +276,30 @@ typedef struct {
 int rumi_render(rumi_ctx* ctx, const rumi_scene* scene, const rumi_render_desc* desc,
                 float* dev_rgb, rumi_render_stats* stats, void* stream);
 
+/* ---------------------------------------------------------------- adjoint */
+/* Device buffers that rumi_adjoint overwrites; any may be NULL. */
+typedef struct {
+    float* d_medium;    /* 7 floats: dLoss/d sigma_a (rgb), sigma_s (rgb), g of the medium */
+    float* d_textures;  /* 3 floats per texel of every texture, in scene order and texel order */
+    float* d_image;     /* the primal pass's image (height x width x 3), as rumi_render writes it */
+} rumi_grad_buffers;
+
+/* Path-replay adjoint of rumi_render with the path tracer (the reference has
+ * no differentiable renderer; this extends PathTracer::Trace,
+ * src/Integrator/pathtracer.cpp:117-354).  Given dLoss/dImage (device,
+ * height x width x 3, the derivative with respect to the per-pixel mean
+ * that rumi_render writes), runs a primal pass that records every sample's
+ * radiance, then replays each path with the same random numbers and
+ * accumulates the derivative of the expected image with respect to the
+ * medium's sigma_a, sigma_s, g and to every texel the paths read.  Gradients
+ * follow the estimator as rendered, including its sampling decisions, so
+ * they match finite differences of rumi_render.  Holds 24 bytes per sample
+ * of device memory (width x height x spp < 2^32).  Synchronises the stream
+ * before returning. */
+int rumi_adjoint(rumi_ctx* ctx, const rumi_scene* scene, const rumi_render_desc* desc,
+                 const float* dev_dloss_dimage, const rumi_grad_buffers* grads,
+                 rumi_render_stats* stats, void* stream);
+
 /* ------------------------------------------------------------ workloads */
 /* Seeded geodesic icosphere of frequency f (20 f^2 triangles, 10 f^2 + 2
  * vertices) with each vertex at radius 1 + amplitude * (2u - 1), u from

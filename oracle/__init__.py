@@ -264,6 +264,9 @@ def scene_arrays(scene: dict):
             recs = np.zeros((idx.shape[0], 32), np.float32)
             recs[:, 0] = OR_TRI
             recs[:, 1] = s["material"]
+            recs[:, 22] = s.get("texture", -1)
+            if s.get("uv") is not None:
+                recs[:, 23:29] = np.asarray(s["uv"], np.float32)[idx].reshape(-1, 6)
             recs[:, 3:12] = xyz[idx].reshape(-1, 9)
             if nrm is not None:
                 recs[:, 13:22] = np.asarray(nrm, np.float32)[idx].reshape(-1, 9)
@@ -277,6 +280,7 @@ def scene_arrays(scene: dict):
             continue
         r = np.zeros((1, 32), np.float32)
         r[0, 1] = s["material"]
+        r[0, 22] = s.get("texture", -1)
         r[0, 2] = 1.0 if s.get("light") else 0.0
         if k == "tri":
             r[0, 0] = OR_TRI
@@ -330,25 +334,69 @@ def scene_arrays(scene: dict):
     return shapes, mats, c, med
 
 
+def texture_arrays(scene: dict):
+    """The scene's textures as the oracle's pool: (texels float32, table
+    int32 (n, 3) of width, height, first float), sizes in texels."""
+    texs = [np.ascontiguousarray(t, np.float32) for t in scene.get("textures", [])]
+    table = np.zeros((max(1, len(texs)), 3), np.int32)
+    off = 0
+    for i, t in enumerate(texs):
+        table[i] = (t.shape[1], t.shape[0], off)
+        off += t.size
+    pool = np.concatenate([t.reshape(-1) for t in texs]) if texs else np.zeros(3, np.float32)
+    return np.ascontiguousarray(pool), table, len(texs), off // 3
+
+
+_PT_ARGS = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+            c_int, c_int, c_int, c_int, ctypes.c_uint64, c_int]
+
+
 def render_pt(scene: dict, width: int, height: int, spp: int, max_depth: int, seed: int = 0,
               nthreads: int = 1):
     """FP64 restatement of PathTracer::Trace over the scene; returns the
     per-pixel mean radiance and mean of squares, (height, width, 3) float64."""
     L = lib()
     if not hasattr(L, "_render_bound"):
-        L.or_render_pt.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,
-                                   c_int, c_int, c_int, ctypes.c_uint64, c_int, c_void_p,
-                                   c_void_p]
+        L.or_render_pt.argtypes = _PT_ARGS + [c_void_p, c_void_p]
         L._render_bound = True
     shapes, mats, cam, med = scene_arrays(scene)
+    texels, table, n_tex, _ = texture_arrays(scene)
     mean = np.zeros((height, width, 3), np.float64)
     m2 = np.zeros((height, width, 3), np.float64)
     rc = L.or_render_pt(_p(shapes), shapes.shape[0], _p(mats), mats.shape[0], _p(cam),
-                        _p(med) if med is not None else None, width, height, spp, max_depth,
-                        seed, nthreads, _p(mean), _p(m2))
+                        _p(med) if med is not None else None, _p(texels), _p(table), n_tex,
+                        width, height, spp, max_depth, seed, nthreads, _p(mean), _p(m2))
     if rc != 0:
         raise RuntimeError(f"or_render_pt failed ({rc})")
     return mean, m2
+
+
+def adjoint_pt(scene: dict, dloss_dimage, spp: int, max_depth: int, seed: int = 0,
+               nthreads: int = 1, textures: bool = False):
+    """FP64 derivative of the expected image contracted with dloss_dimage
+    (height, width, 3).  Returns (image, grad7, grad7 standard error,
+    texel gradients (n_texels, 3) or None); grad7 = d/d(sigma_a rgb,
+    sigma_s rgb, g) of the medium."""
+    L = lib()
+    if not hasattr(L, "_adjoint_bound"):
+        L.or_adjoint_pt.argtypes = _PT_ARGS + [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              ctypes.c_size_t]
+        L._adjoint_bound = True
+    dimg = np.ascontiguousarray(dloss_dimage, np.float64)
+    height, width = dimg.shape[:2]
+    shapes, mats, cam, med = scene_arrays(scene)
+    texels, table, n_tex, n_texels = texture_arrays(scene)
+    mean = np.zeros((height, width, 3), np.float64)
+    g7 = np.zeros(7, np.float64)
+    v7 = np.zeros(7, np.float64)
+    gt = np.zeros((max(1, n_texels), 3), np.float64) if textures else None
+    rc = L.or_adjoint_pt(_p(shapes), shapes.shape[0], _p(mats), mats.shape[0], _p(cam),
+                         _p(med) if med is not None else None, _p(texels), _p(table), n_tex,
+                         width, height, spp, max_depth, seed, nthreads, _p(dimg), _p(mean),
+                         _p(g7), _p(v7), _p(gt) if textures else None, n_texels)
+    if rc != 0:
+        raise RuntimeError(f"or_adjoint_pt failed ({rc})")
+    return mean, g7, np.sqrt(v7), (gt[:n_texels] if textures else None)
 
 
 def render_bdpt(scene: dict, width: int, height: int, spp: int, max_len: int, n_eye: int = 0,

@@ -19,6 +19,7 @@
 #ifndef RUMI_ORACLE_H
 #define RUMI_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -153,21 +154,39 @@ double or_time_median6_mesh(const float* xyz, const uint32_t* idx, int nt,
  *   shapes    n_shapes x 32 floats: [0] kind (OR_TRI/OR_SPHERE/OR_RECT),
  *             [1] material, [2] area-light flag, [3..11] a,b,c (tri v0 v1 v2;
  *             sphere centre in a; rect p, l, w), [12] radius, [13..21] tri
- *             vertex normals n0 n1 n2.
+ *             vertex normals n0 n1 n2, [22] texture index (-1 none),
+ *             [23..28] tri vertex uv u0 v0 u1 v1 u2 v2.
  *   mats      n_mats x 32 floats: [0] kind (0 matte, 1 glossy, 2 specular,
- *             3 emissive, 4 dielectric, 5 subsurface), [1] kd (specular kr,
- *             emissive ls), [2] ks, [3] Phong exponent, [4..6] rgb, [7] eta_in,
- *             [8] eta_out, [9..11] absorption, [12..14] sigma_s,
- *             [15..17] sigma_a, [18] density.
+ *             3 emissive, 4 dielectric, 5 subsurface, 6 GGX), [1] kd
+ *             (specular kr, emissive ls), [2] ks, [3] Phong exponent,
+ *             [4..6] rgb, [7] eta_in, [8] eta_out, [9..11] absorption,
+ *             [12..14] sigma_s, [15..17] sigma_a, [18] density, [19] GGX
+ *             roughness.
  *   camera    16 floats: position, row-major 3x3 basis, tan(fov/2), y_ratio.
  *   medium    16 floats or NULL: [0] 0 none / 1 box / 2 sphere, [1..3] lo or
  *             centre, [4..6] hi, [7] radius, [8..10] sigma_s, [11..13]
  *             sigma_a, [14] Henyey-Greenstein g (0 = the reference's isotropic).
+ *   texels    RGB floats of every texture, row-major, row 0 = the image's
+ *             first row; tex_table: per texture width, height and the index
+ *             of its first float in texels.
  * Writes the per-pixel mean (and mean of squares when m2_rgb != NULL), RGB,
  * row-major, y = 0 at the top row as the reference's film indexes it. */
 int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
-                 const float* camera, const float* medium, int width, int height, int spp,
-                 int max_depth, uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb);
+                 const float* camera, const float* medium, const float* texels,
+                 const int* tex_table, int n_tex, int width, int height, int spp, int max_depth,
+                 uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb);
+
+/* Derivative of the expected image of or_render_pt, contracted with
+ * dloss_dimage (height x width x 3): grad7 = dLoss/d(sigma_a rgb, sigma_s
+ * rgb, g) of the medium, grad7_var the variance of that estimate,
+ * grad_tex (3 per texel, n_texels in the pool, may be NULL) dLoss/dtexel.
+ * Forward-mode derivatives carried through the backward loop, with the
+ * score of every parameter-dependent sampling decision. */
+int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                  const float* camera, const float* medium, const float* texels,
+                  const int* tex_table, int n_tex, int width, int height, int spp, int max_depth,
+                  uint64_t seed, int nthreads, const double* dloss_dimage, double* mean_rgb,
+                  double* grad7, double* grad7_var, double* grad_tex, size_t n_texels);
 
 /* FP64 bidirectional path tracer over the same scene records (no medium):
  * eye subpaths of at most n_eye vertices (camera included), light subpaths

@@ -8,9 +8,10 @@
  * records one Path per vertex, then a backward loop that evaluates the
  * direct lighting and accumulates L = direct + vol * fr * L * cos / pdf.
  * All arithmetic is double precision.  Randomness: one PCG32 stream per
- * pixel, uniform doubles in [0, 1) (the reference draws rand()/RAND_MAX in
- * [0, 1]); pixel jitter and light points are drawn continuously instead of
- * from the reference's fixed stratified point sets.
+ * sample (path tracer) or per pixel (bidirectional tracer), uniform doubles
+ * in [0, 1) (the reference draws rand()/RAND_MAX in [0, 1]); pixel jitter
+ * and light points are drawn continuously instead of from the reference's
+ * fixed stratified point sets.
  *
  * Extensions beyond the reference (flags in the medium record): a
  * sphere-bounded medium and a Henyey-Greenstein phase function; with the
@@ -109,6 +110,8 @@ typedef struct {
     v3 n;             /* rect normal */
     double length, width, inv_area;
     double lo[3], hi[3];
+    int tex;          /* texture index or -1 */
+    double uv[6];     /* tri vertex uv: u0 v0 u1 v1 u2 v2 */
 } shape_t;
 
 typedef struct {
@@ -143,6 +146,9 @@ typedef struct {
     bnode_t* nodes;
     int n_nodes;
     int* order;
+    const float* texels;    /* RGB texel pool */
+    const int* tex_table;   /* per texture: width, height, first float of its texels */
+    int n_tex;
 } scene_t;
 
 /* Ray + hit record, the reference's Ray/Intersection pair. */
@@ -666,15 +672,17 @@ static void light_sample(const scene_t* sc, pcg_t* g, lsample_t* ls)
 
 /* HomogeneousVolume::ShadowRayTr (homogeneousvolume.cpp:56-93): a grey
  * Bernoulli transmittance, 0 or 1. */
-static double shadow_tr(const scene_t* sc, const ray_t* shadow, pcg_t* g)
+static double shadow_tr(const scene_t* sc, const ray_t* shadow, pcg_t* g, double* smax_out)
 {
     const medium_t* md = &sc->med;
     double tn, tf;
+    *smax_out = 0.0;
     if (md->kind == 0) return 1.0;
     if (!med_intersect(md, shadow, &tn, &tf)) return 1.0;
     double smax = med_maxdist(shadow->tmax, tn, tf);
     double xi = rnd(g);
     if (xi < 1.0 - exp(md->avg_sigma_t * -smax)) return 0.0;
+    *smax_out = smax; /* the segment survived a Bernoulli trial of this length */
     return 1.0;
 }
 
@@ -703,32 +711,94 @@ static double geom(v3 x, const lsample_t* ls, v3 wi)
     return c / (d * d);
 }
 
-/* Matte::DirectLighting (matte.cpp:54-82). */
-static col matte_direct(const scene_t* sc, const mat_t* m, v3 p, v3 n, pcg_t* g)
+/* Matte::DirectLighting (matte.cpp:54-82).  *smax_s: medium length of the
+ * shadow segment when its Bernoulli transmittance survived (else 0). */
+static col matte_direct(const scene_t* sc, const mat_t* m, v3 p, v3 n, pcg_t* g, double* smax_s)
 {
     lsample_t ls;
     v3 wi;
     ray_t sh;
+    *smax_s = 0.0;
     if (!visibility(sc, p, g, &ls, &wi, &sh)) return cs(0.0);
     col f = cscale(m->color, INV_PI_D * m->kd);
     double c = dot(wi, n);
     if (c < 0.0) c = 0.0;
     col lo = cscale(cmul(f, ls.le), c * geom(p, &ls, wi) / ls.pdf);
-    if (sc->med.kind) lo = cscale(lo, shadow_tr(sc, &sh, g));
+    if (sc->med.kind) lo = cscale(lo, shadow_tr(sc, &sh, g, smax_s));
     return lo;
 }
 
 /* HomogeneousVolume::DirectLighting (homogeneousvolume.cpp:231-250), with
- * the HG value in place of 1/(4 pi) when g != 0. */
-static col medium_direct(const scene_t* sc, v3 p, v3 d, pcg_t* g)
+ * the HG value in place of 1/(4 pi) when g != 0.  Returns the estimate
+ * without its sigma_s factor; *cos_l is the cosine between d and the light
+ * direction, *smax_s as for matte_direct. */
+static col medium_direct0(const scene_t* sc, v3 p, v3 d, pcg_t* g, double* smax_s, double* cos_l)
 {
     lsample_t ls;
     v3 wi;
     ray_t sh;
+    *smax_s = 0.0;
+    *cos_l = 0.0;
     if (!visibility(sc, p, g, &ls, &wi, &sh)) return cs(0.0);
-    double tau = shadow_tr(sc, &sh, g);
-    double fp = sc->med.g == 0.0 ? 1.0 / (4.0 * PI_D) : hg_eval(sc->med.g, dot(d, wi));
-    return cscale(cmul(ls.le, sc->med.sigma_s), tau * fp * geom(p, &ls, wi) / ls.pdf);
+    double tau = shadow_tr(sc, &sh, g, smax_s);
+    *cos_l = dot(d, wi);
+    double fp = sc->med.g == 0.0 ? 1.0 / (4.0 * PI_D) : hg_eval(sc->med.g, *cos_l);
+    return cscale(ls.le, tau * fp * geom(p, &ls, wi) / ls.pdf);
+}
+
+/* d log HG(c; g) / dg; at g = 0 (the isotropic phase) it is 3c. */
+static double dlog_hg(double g, double c)
+{
+    return -2.0 * g / (1.0 - g * g) - 3.0 * (g - c) / (1.0 + g * g - 2.0 * g * c);
+}
+
+/* ------------------------------------------------------------ textures */
+/* ImageTexture::Pixel (imagetexture.cpp:32-61), nearest texel; *index is
+ * the texel's position in the pool. */
+static col texel(const scene_t* sc, int t, double u, double v, int* index)
+{
+    int W = sc->tex_table[3 * t], H = sc->tex_table[3 * t + 1], off = sc->tex_table[3 * t + 2];
+    int row = (int)((1.0 - v) * (H - 1));
+    int cl = (int)(u * (W - 1));
+    int y = H - 1 - row, x = cl;
+    if (y < 0) y = 0;
+    if (y > H - 1) y = H - 1;
+    if (x < 0) x = 0;
+    if (x > W - 1) x = W - 1;
+    *index = off / 3 + y * W + x;
+    const float* q = sc->texels + off + 3 * ((size_t)y * W + x);
+    return cmk(q[0], q[1], q[2]);
+}
+
+/* Texture::Pattern (texture.cpp:13-24) with the shapes' TextureMapping:
+ * Sphere (sphere.cpp:172-195, identity transform), Rect (rect.cpp:183-197);
+ * triangles interpolate their vertex uv (the reference's Shape default maps
+ * every point to (0, 0), which zero uv reproduces). */
+static col pattern(const scene_t* sc, int obj, v3 p, double bu, double bv, int* index)
+{
+    const shape_t* s = &sc->shapes[obj];
+    double u, v;
+    if (s->kind == OR_TRI) {
+        double w0 = 1.0 - bu - bv;
+        u = w0 * s->uv[0] + bu * s->uv[2] + bv * s->uv[4];
+        v = w0 * s->uv[1] + bu * s->uv[3] + bv * s->uv[5];
+    } else if (s->kind == OR_SPHERE) {
+        v3 pc = norm(sub(p, s->a));
+        double theta = acos(pc.y);
+        v3 pxz = mul(pc, sin(theta));
+        double phi = atan2(pxz.x, pxz.z);
+        u = phi >= 0.0 ? phi / (2.0 * PI_D) : (phi + 2.0 * PI_D) / (2.0 * PI_D);
+        v = 1.0 - theta / PI_D;
+    } else {
+        v3 on = sub(p, s->a);
+        u = dot(on, s->b) / (s->length * s->length);
+        v = dot(on, s->c) / (s->width * s->width);
+        if (u >= 1.0) u = 1.0;
+        if (u <= 0.0) u = 0.0;
+        if (v >= 1.0) v = 1.0;
+        if (v <= 0.0) v = 0.0;
+    }
+    return texel(sc, s->tex, u, v, index);
 }
 
 /* ------------------------------------------------------------ path tracer */
@@ -746,6 +816,7 @@ typedef struct {
     double rand;
     double vs, vsmax;
     col vtau, vpdf;
+    double bu, bv;   /* triangle barycentrics of the hit */
 } pv_t;
 
 static void pv_clean(pv_t* v)
@@ -782,8 +853,21 @@ static void surface(const scene_t* sc, const hitrec_t* h, const ray_t* r, v3* p,
     }
 }
 
+/* Forward-mode derivative of one sample's radiance, carried through the
+ * backward loop: dL[k] = dL/d theta_k per channel for the medium's
+ * (sigma_a rgb, sigma_s rgb, g), and one entry per textured vertex with
+ * dL/d texel (per channel).  Sampling decisions that depend on theta add
+ * their score d log p / d theta times the radiance they lead to, so the
+ * result is the derivative of the estimator's expectation. */
+typedef struct {
+    col dL[7];
+    int n_tex;
+    int* tix;   /* max_depth entries */
+    col* tval;
+} grad_t;
+
 /* PathTracer::Trace(w, h, camera, film) (pathtracer.cpp:117-354). */
-static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, pv_t* P)
+static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, pv_t* P, grad_t* G)
 {
     const medium_t* md = &sc->med;
     int i, di;
@@ -802,6 +886,8 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
         cur->tmin = r.tmin;
         if (hit >= 0) {
             cur->obj = hit;
+            cur->bu = h.u;
+            cur->bv = h.v;
             surface(sc, &h, &r, &cur->p, &cur->n);
         }
         if (md->kind && !cur->in_object) {
@@ -922,6 +1008,11 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
     if (di >= max_depth) di = max_depth - 1;
 
     col L = cs(0.0);
+    int k, e;
+    if (G) {
+        for (k = 0; k < 7; ++k) G->dL[k] = cs(0.0);
+        G->n_tex = 0;
+    }
     for (; di >= 0; --di) {
         cur = &P[di];
         v3 n = cur->n, wo = neg(cur->d);
@@ -932,26 +1023,67 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
         if (md->kind && cur->vs != 0.0 && !cur->in_object) {
             if (cur->vs == -1.0) continue;
             if (cur->vs > 0.0) {
-                direct = medium_direct(sc, cur->p, cur->d, g);
+                double smax_s, cos_l;
+                col dir0 = medium_direct0(sc, cur->p, cur->d, g, &smax_s, &cos_l);
+                direct = cmul(dir0, md->sigma_s);
                 double fp = md->g == 0.0 ? 1.0 / (4.0 * PI_D) : hg_eval(md->g, dot(cur->d, w));
+                col L_in = L;
                 indirect = cscale(cmul(L, md->sigma_s), fp / pdf);
                 vol = cdiv(cur->vtau, cur->vpdf);
                 L = cmul(cadd(direct, indirect), vol);
+                if (G) {
+                    /* scattering at distance ds after surviving the absorption
+                       roulette: density avg e^(-avg ds) (1 - A), A = mean(sa/st);
+                       weight vol = 1/sigma_t per channel (tau / pdf), times
+                       sigma_s and the phase value (whose sampling pdf it equals,
+                       so the direction's score is d log HG) */
+                    const double avg = md->avg_sigma_t, A = md->absorb;
+                    const double hg_w = dlog_hg(md->g, dot(cur->d, w)), hg_l = dlog_hg(md->g, cos_l);
+                    for (k = 0; k < 7; ++k) {
+                        col dd = cs(0.0), dn = cscale(cmul(G->dL[k], md->sigma_s), fp / pdf), dv = cs(0.0);
+                        double score = 0.0;
+                        if (k < 6) {
+                            const int c = k % 3;
+                            const double st = c == 0 ? md->sigma_t.r : (c == 1 ? md->sigma_t.g : md->sigma_t.b);
+                            const double sa = c == 0 ? md->sigma_a.r : (c == 1 ? md->sigma_a.g : md->sigma_a.b);
+                            const double ss = st - sa;
+                            double* ddc = c == 0 ? &dd.r : (c == 1 ? &dd.g : &dd.b);
+                            double* dnc = c == 0 ? &dn.r : (c == 1 ? &dn.g : &dn.b);
+                            double* dvc = c == 0 ? &dv.r : (c == 1 ? &dv.g : &dv.b);
+                            if (k >= 3) { /* sigma_s_c multiplies both terms of channel c */
+                                *ddc += c == 0 ? dir0.r : (c == 1 ? dir0.g : dir0.b);
+                                *dnc += (c == 0 ? L_in.r : (c == 1 ? L_in.g : L_in.b)) * fp / pdf;
+                            }
+                            *dvc = -(c == 0 ? vol.r : (c == 1 ? vol.g : vol.b)) / st;
+                            const double dA = k < 3 ? ss / (3.0 * st * st) : -sa / (3.0 * st * st);
+                            score = (1.0 / avg - cur->vs) / 3.0 - dA / (1.0 - A);
+                            if (smax_s > 0.0) dd = cadd(dd, cscale(direct, -smax_s / 3.0));
+                        } else {
+                            dd = cscale(direct, hg_l);
+                            dn = cadd(dn, cscale(indirect, hg_w));
+                        }
+                        G->dL[k] = cadd(cadd(cmul(cadd(dd, dn), vol), cmul(cadd(direct, indirect), dv)),
+                                        cscale(L, score));
+                    }
+                    for (e = 0; e < G->n_tex; ++e)
+                        G->tval[e] = cmul(cscale(cmul(G->tval[e], md->sigma_s), fp / pdf), vol);
+                }
             }
         } else {
             if (cur->obj < 0) continue;
             const shape_t* s = &sc->shapes[cur->obj];
             const mat_t* m = &sc->mats[s->mat];
             int rinfo_in = cur->in_object;
+            double smax_s = 0.0;
             switch (m->kind) {
             case MAT_MATTE:
                 fr = cscale(m->color, INV_PI_D * m->kd);
-                direct = matte_direct(sc, m, cur->p, n, g);
+                direct = matte_direct(sc, m, cur->p, n, g, &smax_s);
                 break;
             case MAT_GLOSSY:
                 if (m->kd != 0.0 && cur->rand <= m->kd_pdf) {
                     fr = cscale(m->color, INV_PI_D * m->kd);
-                    direct = cscale(matte_direct(sc, m, cur->p, n, g), 1.0 / m->kd_pdf);
+                    direct = cscale(matte_direct(sc, m, cur->p, n, g, &smax_s), 1.0 / m->kd_pdf);
                 } else {
                     fr = cs(phong_f(m, n, w, wo) * m->ks);
                 }
@@ -989,9 +1121,35 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
                 }
                 break;
             }
-            indirect = cscale(cmul(cmul(vol, fr), L), cost / pdf);
+            col k_ind = cscale(cmul(vol, fr), cost / pdf);
+            indirect = cmul(k_ind, L);
             L = cadd(direct, indirect);
-            if (md->kind && !cur->in_object) L = cmul(L, cdiv(cur->vtau, cur->vpdf));
+            col pass = cs(1.0);
+            if (md->kind && !cur->in_object) pass = cdiv(cur->vtau, cur->vpdf);
+            L = cmul(L, pass);
+            /* the segment into this vertex crossed the medium without
+               scattering: probability e^(-avg (smax - 1e-4)) */
+            const double pass_score =
+                (md->kind && !cur->in_object && cur->vsmax > 1e-4) ? -(cur->vsmax - 1e-4) / 3.0 : 0.0;
+            col tx = cs(1.0);
+            int tix = -1;
+            if (s->tex >= 0) tx = pattern(sc, cur->obj, cur->p, cur->bu, cur->bv, &tix);
+            if (G) {
+                for (k = 0; k < 7; ++k) {
+                    col d = cmul(G->dL[k], k_ind);
+                    if (k < 6 && smax_s > 0.0) d = cadd(d, cscale(direct, -smax_s / 3.0));
+                    d = cmul(d, pass);
+                    if (k < 6) d = cadd(d, cscale(L, pass_score));
+                    G->dL[k] = cmul(d, tx);
+                }
+                for (e = 0; e < G->n_tex; ++e) G->tval[e] = cmul(cmul(cmul(G->tval[e], k_ind), pass), tx);
+                if (tix >= 0) {
+                    G->tix[G->n_tex] = tix;
+                    G->tval[G->n_tex] = L; /* d (L tx) / d tx per channel */
+                    ++G->n_tex;
+                }
+            }
+            L = cmul(L, tx);
         }
     }
     return L;
@@ -1018,6 +1176,7 @@ static ray_t camera_ray(const camera_t* c, int W, int H, int x, int y, double sx
 static void load_scene(scene_t* sc, const float* shapes, int n_shapes, const float* mats,
                        int n_mats, const float* cam, const float* med)
 {
+    /* textures are attached by the caller after loading */
     int i, k;
     memset(sc, 0, sizeof(*sc));
     sc->n_shapes = n_shapes;
@@ -1062,6 +1221,8 @@ static void load_scene(scene_t* sc, const float* shapes, int n_shapes, const flo
         o->n0 = mk(s[13], s[14], s[15]);
         o->n1 = mk(s[16], s[17], s[18]);
         o->n2 = mk(s[19], s[20], s[21]);
+        o->tex = (int)s[22];
+        for (k = 0; k < 6; ++k) o->uv[k] = s[23 + k];
         if (o->kind == OR_TRI) {
             for (k = 0; k < 3; ++k) {
                 double a = vget(o->a, k), b = vget(o->b, k), c = vget(o->c, k);
@@ -1126,27 +1287,59 @@ typedef struct {
     uint64_t seed;
     double* mean;
     double* m2;
+    /* adjoint: dLoss/dImage in, per-thread sums out */
+    const double* dimg;
+    double grad[7], grad_var[7];
+    double* gtex; /* 3 per texel, or NULL */
 } job_t;
 
 static void* render_rows(void* arg)
 {
     job_t* j = (job_t*)arg;
-    pv_t* P = (pv_t*)malloc(sizeof(pv_t) * (size_t)(j->max_depth > 0 ? j->max_depth : 1));
-    int x, y, s;
+    const int md = j->max_depth > 0 ? j->max_depth : 1;
+    pv_t* P = (pv_t*)malloc(sizeof(pv_t) * (size_t)md);
+    grad_t G, *Gp = 0;
+    int x, y, s, k, e;
+    if (j->dimg) {
+        G.tix = (int*)malloc(sizeof(int) * (size_t)md);
+        G.tval = (col*)malloc(sizeof(col) * (size_t)md);
+        Gp = &G;
+        for (k = 0; k < 7; ++k) j->grad[k] = j->grad_var[k] = 0.0;
+    }
     for (y = j->row0; y < j->row1; ++y)
         for (x = 0; x < j->W; ++x) {
-            pcg_t g;
-            pcg_seed(&g, j->seed, (uint64_t)y * (uint64_t)j->W + (uint64_t)x);
+            size_t o = 3 * ((size_t)y * j->W + x);
             col sum = cs(0.0), sq = cs(0.0);
+            double psum[7] = {0}, psq[7] = {0};
             for (s = 0; s < j->spp; ++s) {
+                /* one stream per sample, so that renders at nearby parameters
+                   with one seed stay correlated sample by sample */
+                pcg_t g;
+                pcg_seed(&g, j->seed + (uint64_t)s * 0x9E3779B97F4A7C15ull,
+                         (uint64_t)y * (uint64_t)j->W + (uint64_t)x);
                 double sx = rnd(&g), sy = rnd(&g);
                 ray_t r = camera_ray(&j->sc->cam, j->W, j->H, x, y, sx, sy);
-                col L = trace_pt(j->sc, r, j->max_depth, &g, P);
+                col L = trace_pt(j->sc, r, j->max_depth, &g, P, Gp);
                 if (cnan(L)) continue; /* Scene::Rendering drops NaN samples */
                 sum = cadd(sum, L);
                 sq = cadd(sq, cmul(L, L));
+                if (Gp) {
+                    const col w = cscale(cmk(j->dimg[o], j->dimg[o + 1], j->dimg[o + 2]), 1.0 / j->spp);
+                    for (k = 0; k < 7; ++k) {
+                        const double v = w.r * G.dL[k].r + w.g * G.dL[k].g + w.b * G.dL[k].b;
+                        if (v != v) continue;
+                        psum[k] += v;
+                        psq[k] += v * v;
+                    }
+                    if (j->gtex)
+                        for (e = 0; e < G.n_tex; ++e) {
+                            double* q = j->gtex + 3 * (size_t)G.tix[e];
+                            q[0] += w.r * G.tval[e].r;
+                            q[1] += w.g * G.tval[e].g;
+                            q[2] += w.b * G.tval[e].b;
+                        }
+                }
             }
-            size_t o = 3 * ((size_t)y * j->W + x);
             j->mean[o + 0] = sum.r / j->spp;
             j->mean[o + 1] = sum.g / j->spp;
             j->mean[o + 2] = sum.b / j->spp;
@@ -1155,23 +1348,46 @@ static void* render_rows(void* arg)
                 j->m2[o + 1] = sq.g / j->spp;
                 j->m2[o + 2] = sq.b / j->spp;
             }
+            if (Gp)
+                for (k = 0; k < 7; ++k) {
+                    j->grad[k] += psum[k];
+                    /* variance of this pixel's sum of spp independent terms */
+                    if (j->spp > 1)
+                        j->grad_var[k] += (psq[k] - psum[k] * psum[k] / j->spp) * j->spp / (j->spp - 1);
+                }
         }
+    if (Gp) {
+        free(G.tix);
+        free(G.tval);
+    }
     free(P);
     return 0;
 }
 
-int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
-                 const float* camera, const float* medium, int width, int height, int spp,
-                 int max_depth, uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb)
+static int run_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                  const float* camera, const float* medium, const float* texels,
+                  const int* tex_table, int n_tex, int width, int height, int spp, int max_depth,
+                  uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb,
+                  const double* dimg, double* grad7, double* grad7_var, double* grad_tex,
+                  size_t n_texels)
 {
     if (width <= 0 || height <= 0 || spp <= 0 || max_depth <= 0 || !mean_rgb) return 1;
+    if (n_tex > 0 && (!texels || !tex_table)) return 1;
     scene_t sc;
     load_scene(&sc, shapes, n_shapes, mats, n_mats, camera, medium);
+    sc.texels = texels;
+    sc.tex_table = tex_table;
+    sc.n_tex = n_tex;
+    int i, t, k;
+    for (i = 0; i < sc.n_shapes; ++i)
+        if (sc.shapes[i].tex >= n_tex) {
+            free_scene(&sc);
+            return 1;
+        }
     if (nthreads < 1) nthreads = 1;
     if (nthreads > height) nthreads = height;
     pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
-    job_t* jobs = (job_t*)malloc(sizeof(job_t) * nthreads);
-    int t;
+    job_t* jobs = (job_t*)calloc((size_t)nthreads, sizeof(job_t));
     for (t = 0; t < nthreads; ++t) {
         jobs[t].sc = &sc;
         jobs[t].W = width;
@@ -1183,6 +1399,8 @@ int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mat
         jobs[t].seed = seed;
         jobs[t].mean = mean_rgb;
         jobs[t].m2 = m2_rgb;
+        jobs[t].dimg = dimg;
+        jobs[t].gtex = (dimg && grad_tex) ? (double*)calloc(3 * n_texels + 1, sizeof(double)) : 0;
         if (nthreads == 1)
             render_rows(&jobs[t]);
         else
@@ -1190,10 +1408,49 @@ int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mat
     }
     if (nthreads > 1)
         for (t = 0; t < nthreads; ++t) pthread_join(th[t], 0);
+    if (dimg) {
+        for (k = 0; k < 7; ++k) {
+            grad7[k] = 0.0;
+            if (grad7_var) grad7_var[k] = 0.0;
+        }
+        if (grad_tex) memset(grad_tex, 0, sizeof(double) * 3 * n_texels);
+        for (t = 0; t < nthreads; ++t) {
+            for (k = 0; k < 7; ++k) {
+                grad7[k] += jobs[t].grad[k];
+                if (grad7_var) grad7_var[k] += jobs[t].grad_var[k];
+            }
+            if (jobs[t].gtex) {
+                size_t q;
+                for (q = 0; q < 3 * n_texels; ++q) grad_tex[q] += jobs[t].gtex[q];
+                free(jobs[t].gtex);
+            }
+        }
+    }
     free(th);
     free(jobs);
     free_scene(&sc);
     return 0;
+}
+
+int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                 const float* camera, const float* medium, const float* texels,
+                 const int* tex_table, int n_tex, int width, int height, int spp, int max_depth,
+                 uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb)
+{
+    return run_pt(shapes, n_shapes, mats, n_mats, camera, medium, texels, tex_table, n_tex, width,
+                  height, spp, max_depth, seed, nthreads, mean_rgb, m2_rgb, 0, 0, 0, 0, 0);
+}
+
+int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                  const float* camera, const float* medium, const float* texels,
+                  const int* tex_table, int n_tex, int width, int height, int spp, int max_depth,
+                  uint64_t seed, int nthreads, const double* dloss_dimage, double* mean_rgb,
+                  double* grad7, double* grad7_var, double* grad_tex, size_t n_texels)
+{
+    if (!dloss_dimage || !grad7) return 1;
+    return run_pt(shapes, n_shapes, mats, n_mats, camera, medium, texels, tex_table, n_tex, width,
+                  height, spp, max_depth, seed, nthreads, mean_rgb, 0, dloss_dimage, grad7,
+                  grad7_var, grad_tex, n_texels);
 }
 
 /* ====================================================================== */

@@ -90,6 +90,10 @@ class RenderStats(Structure):
                 ("iterations", c_uint64), ("milliseconds", c_double)]
 
 
+class GradBuffers(Structure):
+    _fields_ = [("d_medium", c_void_p), ("d_textures", c_void_p), ("d_image", c_void_p)]
+
+
 RUMI_SHAPE_TRIANGLE, RUMI_SHAPE_SPHERE, RUMI_SHAPE_RECT, RUMI_SHAPE_MESH = 0, 1, 2, 3
 MATERIAL_KINDS = {"matte": 0, "glossy": 1, "specular": 2, "emissive": 3, "dielectric": 4,
                   "subsurface": 5, "ggx": 6}
@@ -116,6 +120,8 @@ _SIGNATURES = {
     "rumi_scene_destroy": (c_int, [c_void_p]),
     "rumi_render": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
                             POINTER(RenderStats), c_void_p]),
+    "rumi_adjoint": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
+                             POINTER(GradBuffers), POINTER(RenderStats), c_void_p]),
     "rumi_make_icosphere": (c_int, [c_uint32, c_uint64, c_float, c_void_p, c_void_p]),
     "rumi_make_rays_uniform": (c_int, [c_uint64, c_uint64, c_float, c_void_p]),
     "rumi_pcg32_sequence": (c_int, [c_uint64, c_uint64, c_uint32, c_void_p]),
@@ -345,8 +351,10 @@ class Scene:
             else:
                 raise ValueError(f"unknown shape kind {k}")
         texs = (Texture * max(1, len(desc.get("textures", []))))()
+        self.n_texels = 0
         for i, t in enumerate(desc.get("textures", [])):
             rgb = np.ascontiguousarray(t, np.float32)
+            self.n_texels += rgb.shape[0] * rgb.shape[1]
             self._keep.append(rgb)
             texs[i].height, texs[i].width = rgb.shape[0], rgb.shape[1]
             texs[i].rgb = _ptr(rgb)
@@ -387,6 +395,28 @@ class Scene:
         check(lib().rumi_render(self.ctx.handle, self.handle, byref(rd), _ptr(out), byref(st),
                                 _stream(stream)))
         return out, st
+
+    def adjoint(self, dloss_dimage, spp, max_depth, seed=0, textures=False, queue_capacity=0,
+                regen_threshold=0, stream=None):
+        """Path-replay adjoint of render(): dloss_dimage is a device tensor
+        height x width x 3.  Returns (primal image, 7 medium gradients
+        [sigma_a rgb, sigma_s rgb, g], texel gradients (n_texels x 3) or None,
+        RenderStats), all device tensors."""
+        import torch
+        h, w = int(dloss_dimage.shape[0]), int(dloss_dimage.shape[1])
+        if dloss_dimage.dtype != torch.float32 or not dloss_dimage.is_contiguous():
+            raise ValueError("dloss_dimage must be a contiguous float32 tensor")
+        img = torch.empty((h, w, 3), dtype=torch.float32, device=dloss_dimage.device)
+        dmed = torch.empty(7, dtype=torch.float32, device=dloss_dimage.device)
+        dtex = (torch.empty((max(1, self.n_texels), 3), dtype=torch.float32,
+                            device=dloss_dimage.device) if textures else None)
+        gb = GradBuffers(_ptr(dmed), _ptr(dtex) if textures else None, _ptr(img))
+        rd = RenderDesc(RUMI_INTEGRATOR_PT, w, h, spp, max_depth, 0, 0, 1.0, seed,
+                        queue_capacity, regen_threshold)
+        st = RenderStats()
+        check(lib().rumi_adjoint(self.ctx.handle, self.handle, byref(rd), _ptr(dloss_dimage),
+                                 byref(gb), byref(st), _stream(stream)))
+        return img, dmed, dtex, st
 
     def close(self):
         if getattr(self, "handle", None):

@@ -121,12 +121,16 @@ struct PathBuf {
     float4* beta;      // throughput rgb
     uint64_t* rng;     // PCG32 state (increment derived from the pixel)
     uint32_t* pixel;
+    uint32_t* sample;  // sample index within the frame
     uint32_t* flags;   // bit 0 le, bit 1 in_object, bits 8.. depth
+    float* pend;       // adjoint replay: score of the sampling that produced this ray
 };
 
 struct ShadowBuf {
     float4* ray;       // 2 per entry
     float4* contrib;   // rgb, pixel (int bits) in w
+    uint32_t* sample;  // adjoint passes: the sample the contribution belongs to
+    float2* score;     // adjoint replay: the connection's own score terms
 };
 
 struct Params {
@@ -200,6 +204,8 @@ __global__ __launch_bounds__(kBlock) void k_regen(PathBuf buf, uint32_t start, u
     buf.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     buf.rng[slot] = g.state;
     buf.pixel[slot] = pixel;
+    buf.sample[slot] = (uint32_t)k;
+    buf.pend[slot] = 0.0f;
     buf.flags[slot] = 1u;  // le = true, outside objects, depth 0
 }
 
@@ -341,13 +347,15 @@ __device__ inline V3 phase_sample(float g, V3 d, Rng& rng, float& pdf) {
 
 // HomogeneousVolume::ShadowRayTr (homogeneousvolume.cpp:56-93): grey
 // Bernoulli transmittance of a shadow segment, 0 or 1.
-__device__ inline float shadow_tr(const DevMedium& m, V3 o, V3 d, float tmax, Rng& g) {
+__device__ inline float shadow_tr(const DevMedium& m, V3 o, V3 d, float tmax, Rng& g, float& smax_out) {
+    smax_out = 0.0f;
     if (m.bound == 0) return 1.0f;
     float tn, tf;
     if (!med_intersect(m, o, d, 0.0f, tn, tf)) return 1.0f;
     const float smax = med_maxdist(tmax, tn, tf);
     const float xi = g.uniform();
     if (xi < 1.0f - expf(m.avg_sigma_t * -smax)) return 0.0f;
+    smax_out = smax;
     return 1.0f;
 }
 
@@ -391,21 +399,24 @@ __device__ inline float light_g(V3 x, const LightSample& ls, V3 wi) {
 }
 
 // ------------------------------------------------------------ textures
-__device__ inline V3 texel(const SceneView& sv, int tex, float u, float v) {
+__device__ inline V3 texel(const SceneView& sv, int tex, float u, float v, int& index) {
     const DevTexture t = sv.textures[tex];
     // ImageTexture::Pixel (imagetexture.cpp:32-61): nearest texel
     const int row = (int)((1.0f - v) * (float)(t.height - 1));
     const int col = (int)(u * (float)(t.width - 1));
     const int y = min(max(t.height - 1 - row, 0), t.height - 1);
     const int x = min(max(col, 0), t.width - 1);
+    index = (int)(t.offset / 3u) + y * t.width + x;
     const float* p = sv.texels + t.offset + 3 * ((size_t)y * t.width + x);
     return v3(p[0], p[1], p[2]);
 }
 
 // Shape::TextureMapping overrides: Rect (rect.cpp:183-197), Sphere
 // (sphere.cpp:172-195, identity transform); triangles interpolate vertex uv.
-__device__ inline V3 pattern(const SceneView& sv, const DevShape& s, V3 p, float bu, float bv) {
+__device__ inline V3 pattern(const SceneView& sv, const DevShape& s, V3 p, float bu, float bv,
+                             int& index) {
     const int tex = s.info[2];
+    index = -1;
     if (tex < 0) return v3(1, 1, 1);
     float u = 0.0f, v = 0.0f;
     if (s.info[0] == RUMI_SHAPE_TRIANGLE) {
@@ -428,7 +439,7 @@ __device__ inline V3 pattern(const SceneView& sv, const DevShape& s, V3 p, float
         u = fminf(fmaxf(dot(onp, L) / (len_l * len_l), 0.0f), 1.0f);
         v = fminf(fmaxf(dot(onp, W) / (len_w * len_w), 0.0f), 1.0f);
     }
-    return texel(sv, tex, u, v);
+    return texel(sv, tex, u, v, index);
 }
 
 // ------------------------------------------------------------ BSDF pieces
@@ -548,29 +559,92 @@ __device__ inline V3 ggx_sample(const DevMaterial& m, V3 n, V3 wo, Rng& g) {
 }
 
 // ------------------------------------------------------------ shading
+// Modes of the shading and shadow stages: plain rendering; the primal pass
+// of the adjoint, which also sums every sample's radiance; the adjoint
+// replay, which re-runs every path with the same random numbers and
+// accumulates parameter gradients instead of the film.
+enum { kRender = 0, kPrimal = 1, kReplay = 2 };
+constexpr int kStripes = 64;  // gradient partial sums spread over 64 addresses
+
+struct AdjCtx {
+    float* L_total;     // 3 per sample: the primal pass's radiance
+    float* L_sofar;     // 3 per sample: radiance of the vertices replayed so far
+    const float* dimg;  // dLoss/dImage, rgb per pixel
+    float inv_spp;      // the image is the mean over spp samples
+    float* gstripes;    // kStripes x 8 partial sums of the 7 medium gradients
+    float* tex_grad;    // texel-pool gradient (rgb) or null
+};
+
+// d(log) of the path weight with respect to the medium parameters
+// (sigma_a rgb, sigma_s rgb, g), times dLoss/dL x L, summed in registers.
+struct Grad7 {
+    float v[7];
+};
+
+__device__ inline void grad_grey(Grad7& G, float dlog_dsigma_bar, float S) {
+    // sigma_bar = (sigma_t.r + sigma_t.g + sigma_t.b) / 3, sigma_t = sigma_a + sigma_s
+    const float t = dlog_dsigma_bar * (1.0f / 3.0f) * S;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) G.v[k] += t;
+}
+
+// d log HG(c; g) / d g, c the cosine between propagation and scattered
+// direction (3c at g = 0, where the isotropic phase is HG's limit).
+__device__ inline float dlog_hg(float g, float c) {
+    return -2.0f * g / (1.0f - g * g) - 3.0f * (g - c) / (1.0f + g * g - 2.0f * g * c);
+}
+
+__device__ inline void flush_grad(Grad7& G, float* stripes) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        float v = G.v[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        G.v[k] = v;
+    }
+    if (lane_id() == 0) {
+        const uint32_t wave = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+        float* dst = stripes + 8 * (wave % kStripes);
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            if (G.v[k] != 0.0f) atomicAdd(dst + k, G.v[k]);
+    }
+}
+
+__device__ inline void add3(float* p, V3 c) {
+    if (is_nan(c)) return;
+    atomicAdd(p + 0, c.x);
+    atomicAdd(p + 1, c.y);
+    atomicAdd(p + 2, c.z);
+}
+
 struct ShadeOut {
-    uint32_t pixel;
+    uint32_t pixel, sample;
     bool alive;
     float4 r0, r1, beta;
     uint64_t rng;
     uint32_t flags;
+    float pend;     // score of the sampling decision made at this vertex for the next one
     bool shadow;
     float4 s0, s1, contrib;
+    float2 score;   // shadow entry: medium length of the segment (Bernoulli), d log fp / d g
 };
 
-__device__ inline void push_shadow(ShadeOut& o, V3 x, V3 wi, float dist, V3 c, uint32_t pixel) {
+__device__ inline void push_shadow(ShadeOut& o, V3 x, V3 wi, float dist, V3 c, uint32_t pixel,
+                                   float smax_s, float dlog_fp) {
     o.shadow = true;
     o.s0 = make_float4(x.x, x.y, x.z, 0.0f);
     o.s1 = make_float4(wi.x, wi.y, wi.z, dist);
     o.contrib = make_float4(c.x, c.y, c.z, __int_as_float((int)pixel));
+    o.score = make_float2(smax_s, dlog_fp);
 }
 
-__device__ inline void push_next(ShadeOut& o, V3 p, V3 w, V3 beta, uint32_t flags) {
+__device__ inline void push_next(ShadeOut& o, V3 p, V3 w, V3 beta, uint32_t flags, float pend) {
     o.alive = true;
     o.r0 = make_float4(p.x, p.y, p.z, 0.0f);
     o.r1 = make_float4(w.x, w.y, w.z, FLT_MAX);
     o.beta = make_float4(beta.x, beta.y, beta.z, 0.0f);
     o.flags = flags;
+    o.pend = pend;
 }
 
 __device__ inline uint32_t mk_flags(bool le, bool in_obj, uint32_t depth) {
@@ -578,26 +652,41 @@ __device__ inline uint32_t mk_flags(bool le, bool in_obj, uint32_t depth) {
 }
 
 // Matte::DirectLighting (matte.cpp:54-82) without the occlusion test, which
-// the shadow stage performs.
+// the shadow stage performs.  smax_s: the shadow segment's length inside
+// the medium when its Bernoulli transmittance survived (0 if none).
 __device__ inline bool matte_direct(const SceneView& sv, const DevMaterial& m, V3 p, V3 n, Rng& g,
-                                    V3& lo, V3& wi, float& dist) {
+                                    V3& lo, V3& wi, float& dist, float& smax_s) {
     LightSample ls;
+    smax_s = 0.0f;
     if (!sample_light(sv, p, g, ls, wi, dist)) return false;
     const V3 f = ld3(m.color) * (kInvPiF * m.kd);
     const float c = fmaxf(dot(wi, n), 0.0f);
     lo = cmul(f, ls.le) * (c * light_g(p, ls, wi) / ls.pdf);
-    if (sv.med.bound) lo = lo * shadow_tr(sv.med, p, wi, dist, g);
+    if (sv.med.bound) lo = lo * shadow_tr(sv.med, p, wi, dist, g, smax_s);
     return true;
 }
 
+template <int MODE>
 __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32_t flags, V3 beta,
-                      Rng& g, uint32_t pixel, int32_t prim, float t, float bu, float bv, float* film,
-                      ShadeOut& out) {
+                      Rng& g, uint32_t pixel, uint32_t sample, float pend, int32_t prim, float t,
+                      float bu, float bv, float* film, const AdjCtx& adj, Grad7& G, ShadeOut& out) {
     const bool le = (flags & 1u) != 0;
     const bool in_obj = (flags & 2u) != 0;
     const uint32_t depth = flags >> 8;
     const bool last = (int)depth + 1 > prm.max_depth - 1;
     const DevMedium& md = sv.med;
+
+    // Replay: the radiance this sample still collects from here on.
+    V3 w3 = v3(0, 0, 0);
+    float S = 0.0f;
+    if (MODE == kReplay) {
+        const float* Lt = adj.L_total + 3 * (size_t)sample;
+        const float* Ls = adj.L_sofar + 3 * (size_t)sample;
+        const V3 dl = ld3(adj.dimg + 3 * (size_t)pixel) * adj.inv_spp;
+        w3 = cmul(dl, v3(Lt[0] - Ls[0], Lt[1] - Ls[1], Lt[2] - Ls[2]));
+        S = w3.x + w3.y + w3.z;
+        if (pend != 0.0f) G.v[6] += pend * S;  // the previous vertex's phase direction
+    }
 
     // Homogeneous medium along the segment (pathtracer.cpp:156-178).
     if (md.bound && !in_obj) {
@@ -609,6 +698,27 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
             const int ev = distance_sample(md.sigma_t, md.avg_sigma_t, md.absorb, smax, g, s, ratio);
             if (ev == kAbsorb) return;
             beta = cmul(beta, ratio);
+            if (MODE == kReplay) {
+                if (ev == kPass) {
+                    // P(pass) = exp(-sigma_bar (smax - 1e-4))
+                    grad_grey(G, -fmaxf(smax - 1e-4f, 0.0f), S);
+                } else {
+                    // distance density sigma_bar exp(-sigma_bar s), survival of the
+                    // absorption roulette (1 - A), albedo weight sigma_s / sigma_t
+                    grad_grey(G, 1.0f / md.avg_sigma_t - s, S);
+                    const float inv1a = 1.0f / (1.0f - md.absorb);
+                    const float wc[3] = {w3.x, w3.y, w3.z};
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const float st = md.sigma_t[c];
+                        const float dA_da = md.sigma_s[c] / (st * st) * (1.0f / 3.0f);
+                        const float dA_ds = -md.sigma_a[c] / (st * st) * (1.0f / 3.0f);
+                        G.v[c] += -dA_da * inv1a * S - wc[c] / st;
+                        G.v[3 + c] += -dA_ds * inv1a * S;
+                        if (md.sigma_s[c] > 0.0f) G.v[3 + c] += wc[c] * (1.0f / md.sigma_s[c] - 1.0f / st);
+                    }
+                }
+            }
             if (ev == kScatter) {
                 const V3 p = o + d * (tn < 0.0f ? s : tn + s);
                 // HomogeneousVolume::DirectLighting (homogeneousvolume.cpp:231-250)
@@ -616,18 +726,21 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
                 V3 wi;
                 float dist;
                 if (sample_light(sv, p, g, ls, wi, dist)) {
-                    const float tr = shadow_tr(md, p, wi, dist, g);
-                    const float fp = md.g == 0.0f ? 1.0f / (4.0f * kPiF) : hg_eval(md.g, dot(d, wi));
+                    float smax_s;
+                    const float tr = shadow_tr(md, p, wi, dist, g, smax_s);
+                    const float cl = dot(d, wi);
+                    const float fp = md.g == 0.0f ? 1.0f / (4.0f * kPiF) : hg_eval(md.g, cl);
                     const V3 c = cmul(cmul(beta, ls.le), ld3(md.sigma_s)) *
                                  (tr * fp * light_g(p, ls, wi) / ls.pdf);
-                    if (tr > 0.0f) push_shadow(out, p, wi, dist, c, pixel);
+                    if (tr > 0.0f) push_shadow(out, p, wi, dist, c, pixel, smax_s, dlog_hg(md.g, cl));
                 }
                 if (last) return;
                 float pdf;
                 const V3 w = phase_sample(md.g, d, g, pdf);
                 // Fp / pdf = sigma_s (phase value equals its pdf)
                 beta = cmul(beta, ld3(md.sigma_s));
-                push_next(out, p, w, beta, mk_flags(false, false, depth + 1));
+                push_next(out, p, w, beta, mk_flags(false, false, depth + 1),
+                          MODE == kReplay ? dlog_hg(md.g, dot(d, w)) : 0.0f);
                 return;
             }
         }
@@ -657,9 +770,31 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
         n = v3(s.g0[3], s.g1[0], s.g1[1]);
     }
 
+    // The shape's texture multiplies the radiance leaving every surface
+    // vertex, emitters and subsurface interior events included
+    // (PathTracer::Trace, pathtracer.cpp:343-344).  In the replay, the texel
+    // gains dLoss/dL x L_rem / texel.
+    auto texture_at = [&](V3 at) -> V3 {
+        int index = -1;
+        const V3 tx = pattern(sv, s, at, bu, bv, index);
+        if (MODE == kReplay && adj.tex_grad && index >= 0) {
+            float* tg = adj.tex_grad + 3 * (size_t)index;
+            if (tx.x > 0.0f && w3.x != 0.0f) atomicAdd(tg + 0, w3.x / tx.x);
+            if (tx.y > 0.0f && w3.y != 0.0f) atomicAdd(tg + 1, w3.y / tx.y);
+            if (tx.z > 0.0f && w3.z != 0.0f) atomicAdd(tg + 2, w3.z / tx.z);
+        }
+        return tx;
+    };
+
     if (m.kind == RUMI_MAT_EMISSIVE) {
         // Emissive::DirectLighting (emissive.cpp:60-74)
-        if (!in_obj && le && -dot(d, n) > 0.0f) film_add(film, pixel, cmul(beta, ld3(m.color) * m.kd));
+        if (!in_obj && le && -dot(d, n) > 0.0f) {
+            V3 c = cmul(beta, ld3(m.color) * m.kd);
+            if (s.info[2] >= 0) c = cmul(c, texture_at(p));
+            if (MODE != kReplay) film_add(film, pixel, c);
+            if (MODE == kPrimal) add3(adj.L_total + 3 * (size_t)sample, c);
+            if (MODE == kReplay) add3(adj.L_sofar + 3 * (size_t)sample, c);
+        }
         return;
     }
 
@@ -672,18 +807,19 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
         if (ev == kAbsorb) return;
         if (ev == kScatter) {
             // vol * Fp / pdf_w with Fp = sigma_s / 4pi and pdf_w = 1 / 4pi
-            beta = cmul(cmul(beta, ratio), ld3(m.sigma_s));
-            if (last) return;
             const V3 ps = o + d * sd;
+            beta = cmul(cmul(beta, ratio), ld3(m.sigma_s));
+            if (s.info[2] >= 0) beta = cmul(beta, texture_at(ps));
+            if (last) return;
             float pdf;
             const V3 w = phase_sample(0.0f, d, g, pdf);
-            push_next(out, ps, w, beta, mk_flags(true, true, depth + 1));
+            push_next(out, ps, w, beta, mk_flags(true, true, depth + 1), 0.0f);
             return;
         }
         vol = ratio;
     }
 
-    const V3 tex = pattern(sv, s, p, bu, bv);
+    const V3 tex = s.info[2] >= 0 ? texture_at(p) : v3(1, 1, 1);
     const V3 wo = -d;
     float xi = 1.0f, cost = 1.0f, pdf_w = 1.0f, pdf_path = 1.0f;
     bool le2 = false, in2 = false;
@@ -766,11 +902,12 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
     // Next-event estimation at this vertex (Material::DirectLighting).
     if (m.kind == RUMI_MAT_MATTE || (m.kind == RUMI_MAT_GLOSSY && m.kd != 0.0f && xi <= m.kd_pdf)) {
         V3 lo, wi;
-        float dist;
-        if (matte_direct(sv, m, p, n, g, lo, wi, dist)) {
+        float dist, smax_s;
+        if (matte_direct(sv, m, p, n, g, lo, wi, dist, smax_s)) {
             if (m.kind == RUMI_MAT_GLOSSY) lo = lo * (1.0f / m.kd_pdf);
             const V3 c = cmul(cmul(beta, tex), lo);
-            if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) push_shadow(out, p, wi, dist, c, pixel);
+            if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f)
+                push_shadow(out, p, wi, dist, c, pixel, smax_s, 0.0f);
         }
     }
     if (last) return;
@@ -816,18 +953,23 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
     }
     const float pdf = pdf_w * pdf_path;
     beta = cmul(cmul(beta, tex), fr * (cost / pdf));
-    push_next(out, p, w, beta, mk_flags(le2, in2, depth + 1));
+    push_next(out, p, w, beta, mk_flags(le2, in2, depth + 1), 0.0f);
 }
 
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const uint32_t* lists,
                                                   uint32_t cap, int4 off_lo, int4 off_hi,
                                                   uint32_t total, const int2* hits,
                                                   const float2* uvs, SceneView sv, Params prm,
-                                                  float* film, ShadowBuf sh, uint32_t* counters) {
+                                                  float* film, ShadowBuf sh, uint32_t* counters,
+                                                  AdjCtx adj) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     ShadeOut so;
     so.alive = false;
     so.shadow = false;
+    Grad7 G;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) G.v[k] = 0.0f;
     if (j < total) {
         const int offs[kKeys + 1] = {0, off_lo.x, off_lo.y, off_lo.z, off_lo.w,
                                      off_hi.x, off_hi.y, off_hi.z, off_hi.w};
@@ -838,16 +980,20 @@ __global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const
         const float4 r1 = in.ray[2 * (size_t)i + 1];
         const float4 bt = in.beta[i];
         const uint32_t pixel = in.pixel[i];
+        const uint32_t sample = in.sample[i];
         Rng g;
         g.state = in.rng[i];
         g.inc = ((uint64_t)pixel << 1u) | 1u;
         const int2 h = hits[i];
         const float2 uv = uvs[i];
-        shade(sv, prm, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), in.flags[i],
-              v3(bt.x, bt.y, bt.z), g, pixel, h.x, __int_as_float(h.y), uv.x, uv.y, film, so);
+        shade<MODE>(sv, prm, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), in.flags[i],
+                    v3(bt.x, bt.y, bt.z), g, pixel, sample, MODE == kReplay ? in.pend[i] : 0.0f,
+                    h.x, __int_as_float(h.y), uv.x, uv.y, film, adj, G, so);
         so.rng = g.state;
         so.pixel = pixel;
+        so.sample = sample;
     }
+    if (MODE == kReplay) flush_grad(G, adj.gstripes);
     const uint32_t ns = wave_append(&counters[0], so.alive);
     if (so.alive) {
         out.ray[2 * (size_t)ns + 0] = so.r0;
@@ -855,36 +1001,59 @@ __global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const
         out.beta[ns] = make_float4(so.beta.x, so.beta.y, so.beta.z, 0.0f);
         out.rng[ns] = so.rng;
         out.pixel[ns] = so.pixel;
+        out.sample[ns] = so.sample;
         out.flags[ns] = so.flags;
+        if (MODE == kReplay) out.pend[ns] = so.pend;
     }
     const uint32_t ss = wave_append(&counters[1], so.shadow);
     if (so.shadow) {
         sh.ray[2 * (size_t)ss + 0] = so.s0;
         sh.ray[2 * (size_t)ss + 1] = so.s1;
         sh.contrib[ss] = so.contrib;
+        if (MODE != kRender) sh.sample[ss] = so.sample;
+        if (MODE == kReplay) sh.score[ss] = so.score;
     }
 }
 
-template <bool SPILL>
+template <bool SPILL, int MODE>
 __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, uint32_t n, SceneView sv,
-                                                   float* film) {
+                                                   float* film, AdjCtx adj) {
     __shared__ int32_t s_ref[kStack * kBlock];
     __shared__ float s_tn[kStack * kBlock];
     const uint32_t tid = threadIdx.x;
     const uint32_t i = blockIdx.x * kBlock + tid;
-    if (i >= n) return;
-    const float4 r0 = sh.ray[2 * (size_t)i + 0];
-    const float4 r1 = sh.ray[2 * (size_t)i + 1];
-    Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, 0.0f};
-    float hu, hv;
-    int32_t nn = 0, np = 0;
-    traverse<kBlock, kStack, true, true, true, false, SPILL>(sv.nodes, sv.prims, sv.rects, sv.root,
-                                                             r, false, hu, hv, nn, np, s_ref, s_tn,
-                                                             tid);
-    // Scene::VisibilityTest: in shadow iff the any-hit query found a hit.
-    if (r.tmin > 0.0f) return;
-    const float4 c = sh.contrib[i];
-    film_add(film, (uint32_t)__float_as_int(c.w), v3(c.x, c.y, c.z));
+    Grad7 G;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) G.v[k] = 0.0f;
+    if (i < n) {
+        const float4 r0 = sh.ray[2 * (size_t)i + 0];
+        const float4 r1 = sh.ray[2 * (size_t)i + 1];
+        Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, 0.0f};
+        float hu, hv;
+        int32_t nn = 0, np = 0;
+        traverse<kBlock, kStack, true, true, true, false, SPILL>(sv.nodes, sv.prims, sv.rects,
+                                                                 sv.root, r, false, hu, hv, nn, np,
+                                                                 s_ref, s_tn, tid);
+        // Scene::VisibilityTest: in shadow iff the any-hit query found a hit.
+        if (!(r.tmin > 0.0f)) {
+            const float4 c4 = sh.contrib[i];
+            const V3 c = v3(c4.x, c4.y, c4.z);
+            const uint32_t pixel = (uint32_t)__float_as_int(c4.w);
+            if (MODE != kReplay) film_add(film, pixel, c);
+            if (MODE == kPrimal) add3(adj.L_total + 3 * (size_t)sh.sample[i], c);
+            if (MODE == kReplay && !is_nan(c)) {
+                add3(adj.L_sofar + 3 * (size_t)sh.sample[i], c);
+                // factors of this connection alone: the shadow segment's
+                // Bernoulli transmittance and the phase value toward the light
+                const V3 w3 = cmul(ld3(adj.dimg + 3 * (size_t)pixel) * adj.inv_spp, c);
+                const float Ss = w3.x + w3.y + w3.z;
+                const float2 sc = sh.score[i];
+                if (sc.x > 0.0f) grad_grey(G, -sc.x, Ss);
+                G.v[6] += sc.y * Ss;
+            }
+        }
+    }
+    if (MODE == kReplay) flush_grad(G, adj.gstripes);
 }
 
 // ============================================================ BDPT
@@ -1002,7 +1171,8 @@ __device__ inline void surface_at(const SceneView& sv, int32_t prim, V3 o, V3 d,
     } else {
         n = v3(s.g0[3], s.g1[0], s.g1[1]);
     }
-    tex = pattern(sv, s, p, bu, bv);
+    int tex_index;
+    tex = pattern(sv, s, p, bu, bv, tex_index);
 }
 
 __device__ inline void store_vert(BVert* arr, uint32_t idx, V3 p, float pdf_fwd, V3 n, V3 beta,
@@ -1503,26 +1673,48 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
 }
 
 
-}  // namespace
-
-extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_render_desc* desc,
-                           float* film, rumi_render_stats* stats, void* stream_v) {
-    clear_error();
-    rumi_scene* sc = const_cast<rumi_scene*>(scene_c);
-    if (!ctx) return fail(RUMI_ERR_NO_DEVICE, "rumi_render: null context");
-    if (!sc || !desc || !film) return fail(RUMI_ERR_INVALID, "rumi_render: null argument");
-    if (sc->device != ctx->device) return fail(RUMI_ERR_INVALID, "rumi_render: scene is on another device");
+// Shared checks of rumi_render and rumi_adjoint.
+int check_render_args(rumi_ctx* ctx, const rumi_scene* sc, const rumi_render_desc* desc, const char* fn) {
+    if (!ctx) return fail(RUMI_ERR_NO_DEVICE, "%s: null context", fn);
+    if (!sc || !desc) return fail(RUMI_ERR_INVALID, "%s: null argument", fn);
+    if (sc->device != ctx->device) return fail(RUMI_ERR_INVALID, "%s: scene is on another device", fn);
     if (desc->integrator != RUMI_INTEGRATOR_PT && desc->integrator != RUMI_INTEGRATOR_BDPT)
-        return fail(RUMI_ERR_UNSUPPORTED, "rumi_render: unknown integrator %d", desc->integrator);
+        return fail(RUMI_ERR_UNSUPPORTED, "%s: unknown integrator %d", fn, desc->integrator);
     if (desc->width <= 0 || desc->height <= 0 || desc->spp <= 0 || desc->max_depth <= 0)
-        return fail(RUMI_ERR_INVALID, "rumi_render: width, height, spp and max_depth must be > 0");
-    if (desc->max_depth > 1000) return fail(RUMI_ERR_INVALID, "rumi_render: max_depth > 1000");
+        return fail(RUMI_ERR_INVALID, "%s: width, height, spp and max_depth must be > 0", fn);
+    if (desc->max_depth > 1000) return fail(RUMI_ERR_INVALID, "%s: max_depth > 1000", fn);
     if ((uint64_t)desc->width * desc->height > 0xffffffffull / 4)
-        return fail(RUMI_ERR_INVALID, "rumi_render: image too large");
-    if (hipSetDevice(ctx->device) != hipSuccess) return fail(RUMI_ERR_HIP, "hipSetDevice failed");
-    hipStream_t stream = (hipStream_t)stream_v;
-    if (desc->integrator == RUMI_INTEGRATOR_BDPT) return render_bdpt(ctx, sc, desc, film, stats, stream);
+        return fail(RUMI_ERR_INVALID, "%s: image too large", fn);
+    if ((uint64_t)desc->width * desc->height * (uint64_t)desc->spp > 0xffffffffull)
+        return fail(RUMI_ERR_INVALID, "%s: more than 2^32 - 1 samples in one frame", fn);
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(RUMI_ERR_HIP, "%s: hipSetDevice failed", fn);
+    return RUMI_OK;
+}
 
+template <int MODE>
+void launch_shade(uint32_t grid, hipStream_t stream, const PathBuf& in, const PathBuf& out,
+                  const uint32_t* lists, uint32_t cap, int4 off_lo, int4 off_hi, uint32_t n,
+                  const int2* hits, const float2* uvs, const SceneView& sv, const Params& prm,
+                  float* film, const ShadowBuf& sh, uint32_t* counters, const AdjCtx& adj) {
+    hipLaunchKernelGGL(k_shade<MODE>, dim3(grid), dim3(kBlock), 0, stream, in, out, lists, cap, off_lo,
+                       off_hi, n, hits, uvs, sv, prm, film, sh, counters, adj);
+}
+
+template <int MODE>
+void launch_shadow(bool spill, uint32_t grid, hipStream_t stream, const ShadowBuf& sh, uint32_t n,
+                   const SceneView& sv, float* film, const AdjCtx& adj) {
+    if (spill)
+        hipLaunchKernelGGL((k_shadow<true, MODE>), dim3(grid), dim3(kBlock), 0, stream, sh, n, sv, film, adj);
+    else
+        hipLaunchKernelGGL((k_shadow<false, MODE>), dim3(grid), dim3(kBlock), 0, stream, sh, n, sv, film, adj);
+}
+
+// The wavefront path tracer: regeneration, extension, material buckets,
+// shading, shadow rays, until every sample of the frame has terminated.
+// MODE selects plain rendering, the adjoint's primal pass or its replay.
+int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
+              rumi_render_stats* stats, hipStream_t stream, int mode, const AdjCtx& adj) {
+    (void)ctx;
     const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
     const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
     uint32_t cap = desc->queue_capacity ? desc->queue_capacity : (1u << 21);
@@ -1531,8 +1723,9 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
     uint32_t regen = desc->regen_threshold ? desc->regen_threshold : cap;
 
     // Work buffers: 2 path sets, hits, bucket lists, shadow queue, counters.
-    const size_t per_path = 2 * (32 + 16 + 8 + 4 + 4) + 8 + 8 + 4 * kKeys + 32 + 16;
-    const size_t need = per_path * (size_t)cap + 256 * sizeof(uint32_t) + 1024;
+    const size_t per_path = 2 * (32 + 16 + 8 + 4 + 4 + 4 + 4) + 8 + 8 + 4 * kKeys + 32 + 16 + 4 + 8;
+    // + 256 B of alignment slack for each of the (fewer than 32) carved arrays
+    const size_t need = per_path * (size_t)cap + 256 * sizeof(uint32_t) + 32 * 256;
     if (sc->work_bytes < need) {
         if (sc->work) (void)hipFree(sc->work);
         sc->work = nullptr;
@@ -1553,7 +1746,9 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
         pb[k].beta = (float4*)take(16 * (size_t)cap);
         pb[k].rng = (uint64_t*)take(8 * (size_t)cap);
         pb[k].pixel = (uint32_t*)take(4 * (size_t)cap);
+        pb[k].sample = (uint32_t*)take(4 * (size_t)cap);
         pb[k].flags = (uint32_t*)take(4 * (size_t)cap);
+        pb[k].pend = (float*)take(4 * (size_t)cap);
     }
     int2* hits = (int2*)take(8 * (size_t)cap);
     float2* uvs = (float2*)take(8 * (size_t)cap);
@@ -1561,6 +1756,8 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
     ShadowBuf sh;
     sh.ray = (float4*)take(32 * (size_t)cap);
     sh.contrib = (float4*)take(16 * (size_t)cap);
+    sh.sample = (uint32_t*)take(4 * (size_t)cap);
+    sh.score = (float2*)take(8 * (size_t)cap);
     uint32_t* counters = (uint32_t*)take(256 * sizeof(uint32_t));  // [0] next [1] shadow [8..16) buckets
     if ((size_t)(w - (char*)sc->work) > sc->work_bytes)
         return fail(RUMI_ERR_INVALID, "rumi_render: internal work-buffer layout overflow");
@@ -1579,7 +1776,8 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
     (void)hipEventCreate(&ev1);
     (void)hipEventRecord(ev0, stream);
     const size_t film_n = 3 * (size_t)pixels;
-    hipLaunchKernelGGL(k_zero, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n);
+    if (mode != kReplay)
+        hipLaunchKernelGGL(k_zero, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n);
 
     uint64_t issued = 0, ext_rays = 0, shadow_rays = 0, iters = 0;
     uint32_t n_cur = 0;
@@ -1618,8 +1816,15 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
         for (int k = 0; k < kKeys; ++k) offs[k + 1] = offs[k] + (int)host_cnt[8 + k];
         const int4 off_lo = make_int4(offs[1], offs[2], offs[3], offs[4]);
         const int4 off_hi = make_int4(offs[5], offs[6], offs[7], offs[8]);
-        hipLaunchKernelGGL(k_shade, dim3(grid_of(n_cur)), dim3(kBlock), 0, stream, pb[cur], pb[cur ^ 1],
-                           lists, cap, off_lo, off_hi, n_cur, hits, uvs, sv, prm, film, sh, counters);
+        if (mode == kRender)
+            launch_shade<kRender>(grid_of(n_cur), stream, pb[cur], pb[cur ^ 1], lists, cap, off_lo, off_hi,
+                                  n_cur, hits, uvs, sv, prm, film, sh, counters, adj);
+        else if (mode == kPrimal)
+            launch_shade<kPrimal>(grid_of(n_cur), stream, pb[cur], pb[cur ^ 1], lists, cap, off_lo, off_hi,
+                                  n_cur, hits, uvs, sv, prm, film, sh, counters, adj);
+        else
+            launch_shade<kReplay>(grid_of(n_cur), stream, pb[cur], pb[cur ^ 1], lists, cap, off_lo, off_hi,
+                                  n_cur, hits, uvs, sv, prm, film, sh, counters, adj);
         if (hipMemcpyAsync(host_cnt, counters, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream) !=
                 hipSuccess ||
             hipStreamSynchronize(stream) != hipSuccess) {
@@ -1630,19 +1835,20 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
         const uint32_t n_next = host_cnt[0], n_shadow = host_cnt[1];
         if (n_shadow) {
             shadow_rays += n_shadow;
-            if (spill)
-                hipLaunchKernelGGL(k_shadow<true>, dim3(grid_of(n_shadow)), dim3(kBlock), 0, stream, sh,
-                                   n_shadow, sv, film);
+            if (mode == kRender)
+                launch_shadow<kRender>(spill, grid_of(n_shadow), stream, sh, n_shadow, sv, film, adj);
+            else if (mode == kPrimal)
+                launch_shadow<kPrimal>(spill, grid_of(n_shadow), stream, sh, n_shadow, sv, film, adj);
             else
-                hipLaunchKernelGGL(k_shadow<false>, dim3(grid_of(n_shadow)), dim3(kBlock), 0, stream,
-                                   sh, n_shadow, sv, film);
+                launch_shadow<kReplay>(spill, grid_of(n_shadow), stream, sh, n_shadow, sv, film, adj);
         }
         cur ^= 1;
         n_cur = n_next;
     }
     if (rc == RUMI_OK) {
-        hipLaunchKernelGGL(k_scale, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film,
-                           film_n, 1.0f / (float)desc->spp);
+        if (mode != kReplay)
+            hipLaunchKernelGGL(k_scale, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film,
+                               film_n, 1.0f / (float)desc->spp);
         (void)hipEventRecord(ev1, stream);
         if (hipStreamSynchronize(stream) != hipSuccess)
             rc = fail(RUMI_ERR_HIP, "rumi_render: %s", hipGetErrorString(hipGetLastError()));
@@ -1659,6 +1865,101 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
         stats->shadow_rays = shadow_rays;
         stats->iterations = iters;
         stats->milliseconds = ms;
+    }
+    return rc;
+}
+
+__global__ void k_reduce_stripes(const float* stripes, float* out) {
+    // one wave: lane k < 7 sums parameter k over the stripes in a fixed order
+    const int k = threadIdx.x;
+    if (k < 7) {
+        float a = 0.0f;
+        for (int j = 0; j < kStripes; ++j) a += stripes[8 * j + k];
+        out[k] = a;
+    }
+}
+
+}  // namespace
+
+extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_render_desc* desc,
+                           float* film, rumi_render_stats* stats, void* stream_v) {
+    clear_error();
+    rumi_scene* sc = const_cast<rumi_scene*>(scene_c);
+    int rc = check_render_args(ctx, sc, desc, "rumi_render");
+    if (rc != RUMI_OK) return rc;
+    if (!film) return fail(RUMI_ERR_INVALID, "rumi_render: null film");
+    hipStream_t stream = (hipStream_t)stream_v;
+    if (desc->integrator == RUMI_INTEGRATOR_BDPT) return render_bdpt(ctx, sc, desc, film, stats, stream);
+    AdjCtx none{};
+    return render_pt(ctx, sc, desc, film, stats, stream, kRender, none);
+}
+
+extern "C" int rumi_adjoint(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_render_desc* desc,
+                            const float* dloss_dimage, const rumi_grad_buffers* grads,
+                            rumi_render_stats* stats, void* stream_v) {
+    clear_error();
+    rumi_scene* sc = const_cast<rumi_scene*>(scene_c);
+    int rc = check_render_args(ctx, sc, desc, "rumi_adjoint");
+    if (rc != RUMI_OK) return rc;
+    if (!dloss_dimage || !grads) return fail(RUMI_ERR_INVALID, "rumi_adjoint: null argument");
+    if (desc->integrator != RUMI_INTEGRATOR_PT)
+        return fail(RUMI_ERR_UNSUPPORTED, "rumi_adjoint: only the path tracer is differentiable");
+    if (grads->d_textures && sc->texel_count == 0)
+        return fail(RUMI_ERR_INVALID, "rumi_adjoint: texture gradients asked of a scene without textures");
+    hipStream_t stream = (hipStream_t)stream_v;
+    const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
+    const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
+
+    // Per-sample radiance of the primal pass and of the replay so far,
+    // the gradient stripes, and the primal image when none is asked for.
+    const size_t l_bytes = 12 * (size_t)total;
+    const size_t need = 2 * ((l_bytes + 255) & ~(size_t)255) + 8 * kStripes * sizeof(float) + 256 +
+                        (grads->d_image ? 0 : ((12 * (size_t)pixels + 255) & ~(size_t)255));
+    if (sc->adj_bytes < need) {
+        if (sc->adj) (void)hipFree(sc->adj);
+        sc->adj = nullptr;
+        sc->adj_bytes = 0;
+        if (hipMalloc(&sc->adj, need) != hipSuccess)
+            return fail(RUMI_ERR_NO_MEMORY, "rumi_adjoint: hipMalloc of %zu bytes failed", need);
+        sc->adj_bytes = need;
+    }
+    char* w = (char*)sc->adj;
+    auto take = [&](size_t bytes) {
+        char* p = w;
+        w += (bytes + 255) & ~(size_t)255;
+        return (void*)p;
+    };
+    AdjCtx adj;
+    adj.L_total = (float*)take(l_bytes);
+    adj.L_sofar = (float*)take(l_bytes);
+    adj.gstripes = (float*)take(8 * kStripes * sizeof(float));
+    float* image = grads->d_image ? grads->d_image : (float*)take(12 * (size_t)pixels);
+    adj.dimg = dloss_dimage;
+    adj.inv_spp = 1.0f / (float)desc->spp;
+    adj.tex_grad = grads->d_textures;
+
+    hipLaunchKernelGGL(k_zero, dim3((uint32_t)((3 * total + 255) / 256)), dim3(256), 0, stream, adj.L_total,
+                       3 * (size_t)total);
+    hipLaunchKernelGGL(k_zero, dim3((uint32_t)((3 * total + 255) / 256)), dim3(256), 0, stream, adj.L_sofar,
+                       3 * (size_t)total);
+    hipLaunchKernelGGL(k_zero, dim3(2), dim3(256), 0, stream, adj.gstripes, (size_t)(8 * kStripes));
+    if (grads->d_textures)
+        hipLaunchKernelGGL(k_zero, dim3((uint32_t)((3 * sc->texel_count + 255) / 256)), dim3(256), 0, stream,
+                           grads->d_textures, 3 * (size_t)sc->texel_count);
+    rumi_render_stats st0{}, st1{};
+    rc = render_pt(ctx, sc, desc, image, &st0, stream, kPrimal, adj);
+    if (rc == RUMI_OK) rc = render_pt(ctx, sc, desc, nullptr, &st1, stream, kReplay, adj);
+    if (rc == RUMI_OK && grads->d_medium) {
+        hipLaunchKernelGGL(k_reduce_stripes, dim3(1), dim3(64), 0, stream, adj.gstripes, grads->d_medium);
+        if (hipStreamSynchronize(stream) != hipSuccess)
+            rc = fail(RUMI_ERR_HIP, "rumi_adjoint: %s", hipGetErrorString(hipGetLastError()));
+    }
+    if (stats) {
+        stats->samples = total;
+        stats->extension_rays = st0.extension_rays + st1.extension_rays;
+        stats->shadow_rays = st0.shadow_rays + st1.shadow_rays;
+        stats->iterations = st0.iterations + st1.iterations;
+        stats->milliseconds = st0.milliseconds + st1.milliseconds;
     }
     return rc;
 }

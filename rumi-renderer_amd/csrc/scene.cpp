@@ -57,7 +57,7 @@ void release(rumi_scene* s) {
         (void)hipGetDevice(&prev);
         (void)hipSetDevice(s->device);
         void* ptrs[] = {s->nodes, s->prims, s->rects, s->shapes, s->materials,
-                        s->lights, s->textures, s->texels, s->work};
+                        s->lights, s->textures, s->texels, s->work, s->adj};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         (void)hipSetDevice(prev);
@@ -413,6 +413,7 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
         if ((rc = upload(&sc->lights, lights))) break;
         if ((rc = upload(&sc->textures, texs))) break;
         if ((rc = upload(&sc->texels, texels))) break;
+        sc->texel_count = (uint32_t)(texels.size() / 3);
     } while (0);
     if (rc != RUMI_OK) {
         release(sc);

@@ -86,10 +86,14 @@ struct rumi_scene {
     void* lights = nullptr;     // DevLight[]
     void* textures = nullptr;   // DevTexture[]
     void* texels = nullptr;     // float RGB pool
+    uint32_t texel_count = 0;   // texels in the pool (3 floats each)
     // wavefront work buffers, grown on demand
     uint32_t capacity = 0;
     void* work = nullptr;
     size_t work_bytes = 0;
+    // adjoint buffers (per-sample radiance, gradient stripes), grown on demand
+    void* adj = nullptr;
+    size_t adj_bytes = 0;
     // host-side copies kept for inspection
     std::vector<int32_t> leaf_prims;
 };
