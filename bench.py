@@ -266,18 +266,84 @@ def render_legs(rumi, ctx, args, log):
         log(f"{name}: {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.1f} ms), "
             f"rel-MSE ratio {out[name]['rel_mse_64px']['ratio']:.3f}, CPU {cpu_sps:.0f} samples/s")
         scene.close()
+    out["media_gradient"] = gradient_leg(rumi, ctx, args, log, scenes)
     out["cornell_bdpt"] = bdpt_leg(rumi, ctx, args, log, scenes)
     return out
+
+
+def gradient_leg(rumi, ctx, args, log, scenes):
+    """Differentiable media: 512^2 x 32 spp, depth 16, L2 loss against the
+    image rendered at every medium parameter x 1.10.  One gradient = the
+    forward render, dLoss/dImage = 2 (I - T) on the device, and rumi_adjoint
+    (primal pass + path replay) for all 7 parameters (sigma_a rgb, sigma_s
+    rgb, g).  CPU baseline: the oracle's FP64 render + adjoint on one thread
+    over a 64^2 x 32 spp sample, scaled by the sample ratio."""
+    import numpy as np
+    import torch
+    w = h = 512
+    spp, depth = 32, 16
+    theta = dict(sigma_a=(0.05, 0.2, 0.4), sigma_s=(1.0, 1.0, 1.0), g=0.7)
+    target = {k: tuple(1.1 * x for x in v) if isinstance(v, tuple) else 1.1 * v
+              for k, v in theta.items()}
+    st_t = rumi.Scene(ctx, scenes.cornell_media(w, h, **target))
+    tgt, _ = st_t.render(w, h, spp, depth, seed=7)
+    torch.cuda.synchronize()
+    st_t.close()
+    scene = rumi.Scene(ctx, scenes.cornell_media(w, h, **theta))
+    dimg = torch.empty_like(tgt)
+
+    def one_grad(seed):
+        img, _ = scene.render(w, h, spp, depth, seed=seed)
+        torch.sub(img, tgt, out=dimg)
+        dimg.mul_(2.0)
+        return scene.adjoint(dimg, spp, depth, seed=seed + 1000)
+
+    one_grad(1)
+    torch.cuda.synchronize()
+    times, grads = [], []
+    for k in range(max(1, args.render_runs)):
+        t0 = time.perf_counter()
+        _, dmed, _, st = one_grad(20 + k)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        grads.append(dmed.cpu().tolist())
+    sec = min(times)
+    loss = float(((scene.render(w, h, spp, depth, seed=99)[0] - tgt) ** 2).sum())
+    scene.close()
+    import oracle
+    sc64 = scenes.cornell_media(64, 64, **theta)
+    t64 = oracle.render_pt(scenes.cornell_media(64, 64, **target), 64, 64, spp, depth, seed=5,
+                           nthreads=8)[0]
+    t0 = time.perf_counter()
+    i64 = oracle.render_pt(sc64, 64, 64, spp, depth, seed=6, nthreads=1)[0]
+    oracle.adjoint_pt(sc64, 2.0 * (i64 - t64), spp, depth, seed=7, nthreads=1)
+    cpu_sec = time.perf_counter() - t0
+    scale = (w * h) / (64 * 64)
+    res = {
+        "grads_per_s": round(1.0 / sec, 3), "unit": "grad/s", "seconds": round(sec, 4),
+        "width": w, "height": h, "spp": spp, "max_depth": depth,
+        "params": ["sigma_a.r", "sigma_a.g", "sigma_a.b", "sigma_s.r", "sigma_s.g", "sigma_s.b", "g"],
+        "gradient": [round(float(x), 6) for x in np.mean(np.array(grads), 0)],
+        "loss": loss, "samples_per_grad": 3 * w * h * spp,
+        "cpu_baseline": {"value": round(1.0 / (cpu_sec * scale), 6), "unit": "grad/s", "cores": 1,
+                         "kind": "port",
+                         "sample": f"64x64 at {spp} spp (1/{scale:.0f} of the image): the oracle's "
+                                   f"FP64 render + forward-mode adjoint, {cpu_sec:.2f}s, scaled"},
+    }
+    res["speedup_vs_cpu"] = round(res["grads_per_s"] / res["cpu_baseline"]["value"], 1)
+    log(f"media_gradient: {res['grads_per_s']:.2f} grad/s ({sec * 1e3:.1f} ms), "
+        f"CPU {res['cpu_baseline']['value']:.5f} grad/s")
+    return res
 
 
 def bdpt_leg(rumi, ctx, args, log, scenes):
     """Bidirectional path tracing on the 1024^2 Cornell box with the
     200k-triangle GGX icosphere and the glass sphere, path length <= 8,
-    power heuristic; samples/s at 16 spp (the full 64 spp is 4x the work)
-    and a CPU baseline of the oracle's FP64 BDPT on one thread."""
+    power heuristic, 64 spp; samples/s and a CPU baseline of the oracle's
+    FP64 BDPT on one thread over a bounded sample."""
     import torch
     w = h = 1024
-    spp, max_len = 16, 8
+    spp, max_len = 64, 8
     sc = scenes.cornell_bdpt(w, h)
     scene = rumi.Scene(ctx, sc)
     kw = dict(integrator=rumi.RUMI_INTEGRATOR_BDPT, mis_beta=2.0, n_eye=9, n_light=9)

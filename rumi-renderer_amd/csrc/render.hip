@@ -1685,8 +1685,6 @@ int check_render_args(rumi_ctx* ctx, const rumi_scene* sc, const rumi_render_des
     if (desc->max_depth > 1000) return fail(RUMI_ERR_INVALID, "%s: max_depth > 1000", fn);
     if ((uint64_t)desc->width * desc->height > 0xffffffffull / 4)
         return fail(RUMI_ERR_INVALID, "%s: image too large", fn);
-    if ((uint64_t)desc->width * desc->height * (uint64_t)desc->spp > 0xffffffffull)
-        return fail(RUMI_ERR_INVALID, "%s: more than 2^32 - 1 samples in one frame", fn);
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(RUMI_ERR_HIP, "%s: hipSetDevice failed", fn);
     return RUMI_OK;
 }
@@ -1902,6 +1900,8 @@ extern "C" int rumi_adjoint(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi
     int rc = check_render_args(ctx, sc, desc, "rumi_adjoint");
     if (rc != RUMI_OK) return rc;
     if (!dloss_dimage || !grads) return fail(RUMI_ERR_INVALID, "rumi_adjoint: null argument");
+    if ((uint64_t)desc->width * desc->height * (uint64_t)desc->spp > 0xffffffffull)
+        return fail(RUMI_ERR_INVALID, "rumi_adjoint: more than 2^32 - 1 samples in one frame");
     if (desc->integrator != RUMI_INTEGRATOR_PT)
         return fail(RUMI_ERR_UNSUPPORTED, "rumi_adjoint: only the path tracer is differentiable");
     if (grads->d_textures && sc->texel_count == 0)

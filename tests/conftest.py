@@ -14,6 +14,7 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+    config.addinivalue_line("markers", "slow: CPU statistical test of tens of seconds")
 
 
 @pytest.fixture(scope="session")
