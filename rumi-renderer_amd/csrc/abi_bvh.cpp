@@ -32,11 +32,22 @@ extern "C" int rumi_ctx_create(int device, rumi_ctx** out) {
     if (!c) return fail(RUMI_ERR_NO_MEMORY, "rumi_ctx_create: out of host memory");
     c->device = device;
     c->cu_count = prop.multiProcessorCount;
+    if (hipMalloc(&c->dev_counters, sizeof(uint32_t) * rumi_ctx::kCounterSlots) != hipSuccess) {
+        delete c;
+        return fail(RUMI_ERR_NO_MEMORY, "rumi_ctx_create: hipMalloc of the counter slots failed");
+    }
     *out = c;
     return RUMI_OK;
 }
 
 extern "C" int rumi_ctx_destroy(rumi_ctx* ctx) {
+    if (ctx && ctx->dev_counters) {
+        int prev = 0;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(ctx->device);
+        (void)hipFree(ctx->dev_counters);
+        (void)hipSetDevice(prev);
+    }
     delete ctx;
     return RUMI_OK;
 }

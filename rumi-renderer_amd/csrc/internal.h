@@ -2,6 +2,8 @@
 // librumi.  Nothing here is part of the C ABI (include/rumi.h).
 #pragma once
 
+#include <atomic>
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -108,6 +110,14 @@ double sah_cost_of(const BuildResult& br);
 struct rumi_ctx {
     int device = 0;
     int cu_count = 0;
+    // Work counters of the persistent kernels: one slot per launch, taken
+    // round robin, so launches on one stream never share a live slot and
+    // launches on different streams share one only after kCounterSlots
+    // further launches.
+    static constexpr uint32_t kCounterSlots = 1024;
+    uint32_t* dev_counters = nullptr;
+    std::atomic<uint32_t> next_counter{0};
+    uint32_t* take_counter() { return dev_counters + (next_counter.fetch_add(1) % kCounterSlots); }
 };
 
 struct rumi_bvh {

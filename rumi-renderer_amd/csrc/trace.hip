@@ -38,6 +38,76 @@ __global__ __launch_bounds__(BLOCK) void k_trace(
     }
 }
 
+__device__ __forceinline__ uint32_t lane_below(uint64_t mask) {
+    // set bits of mask in the lanes below this one
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Persistent-lane traversal.  Rays differ in cost by two orders of magnitude
+// (half of a uniform batch misses the root box; the 99th percentile visits
+// hundreds of nodes), so one ray per lane leaves a wave waiting on its
+// slowest lane.  Here every lane steps its own ray one node or leaf at a
+// time and, once enough lanes of the wave are idle, the wave takes that many
+// new ray indices from a global counter with one atomic; a lane whose new
+// ray misses the root box writes the miss at once and stays idle.  The wave
+// leaves when the batch is drained and its last ray is done.  Each ray's
+// arithmetic is the same as in k_trace, so the results are bit-identical.
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL>
+__global__ __launch_bounds__(BLOCK) void k_trace_lanes(
+    const float4* __restrict__ rays, uint32_t n, const float4* __restrict__ nodes,
+    const float4* __restrict__ tris, RootInfo root, int2* __restrict__ hits,
+    float2* __restrict__ uv, int32_t* __restrict__ any_out, uint32_t* __restrict__ counter,
+    int refill) {
+    __shared__ int32_t s_ref[STACK * BLOCK];
+    __shared__ float s_tn[STACK * BLOCK];
+    constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
+    int32_t p_ref[kSpill];
+    float p_tn[kSpill];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    TravState s;
+    uint32_t id = 0;
+    bool active = false, drained = false;
+    int32_t nn = 0, np = 0;
+    auto finish_ray = [&](uint32_t k) {
+        if (ANY && any_out) any_out[k] = s.hit;
+        if (hits) hits[k] = make_int2(s.hit, __float_as_int(s.r.tmin));
+        if (!ANY && uv) uv[k] = make_float2(s.hu, s.hv);
+    };
+    while (true) {
+        const uint64_t want = __ballot(!active && !drained);
+        const uint64_t busy = __ballot(active);
+        if (want == 0 && busy == 0) break;
+        if (want != 0 && (busy == 0 || __popcll(want) >= (uint32_t)refill)) {
+            const uint32_t leader = (uint32_t)__ffsll((long long)want) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(want));
+            base = __shfl(base, (int)leader);
+            if (!active && !drained) {
+                const uint32_t k = base + lane_below(want);
+                if (k >= n) {
+                    drained = true;
+                } else {
+                    const float4 r0 = rays[2 * (size_t)k + 0];
+                    const float4 r1 = rays[2 * (size_t)k + 1];
+                    s.r = Ray{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, r0.w};
+                    id = k;
+                    active = trav_begin(root, s);
+                    if (!active) finish_ray(k);
+                }
+            }
+        }
+        if (active) {
+            active = trav_step<BLOCK, STACK, ANY, ORDERED, false, false, SPILL>(
+                nodes, tris, nullptr, s, false, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+            if (!active) finish_ray(id);
+        }
+    }
+}
+
+__global__ void k_zero_counter(uint32_t* c) { *c = 0u; }
+
 struct TraceArgs {
     const float4* rays;
     uint32_t n;
@@ -49,6 +119,9 @@ struct TraceArgs {
     int32_t* any_out;
     int32_t* cnt_nodes;
     int32_t* cnt_prims;
+    uint32_t* counter;  // persistent launches: the work counter slot
+    int grid_cap;       // persistent launches: resident blocks to start
+    int refill;         // persistent launches: idle lanes that trigger a refill
 };
 
 constexpr int kBlock = 128;
@@ -56,15 +129,32 @@ constexpr int kBlock = 128;
 template <int STACK, bool ANY, bool ORDERED, bool COUNT, bool SPILL>
 hipError_t launch(const TraceArgs& a, hipStream_t s) {
     const uint32_t grid = (a.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((k_trace<kBlock, STACK, ANY, ORDERED, COUNT, SPILL>), dim3(grid),
-                       dim3(kBlock), 0, s, a.rays, a.n, a.nodes, a.tris, a.root, a.hits, a.uv,
-                       a.any_out, a.cnt_nodes, a.cnt_prims);
+    if (COUNT || !a.counter) {
+        hipLaunchKernelGGL((k_trace<kBlock, STACK, ANY, ORDERED, COUNT, SPILL>), dim3(grid),
+                           dim3(kBlock), 0, s, a.rays, a.n, a.nodes, a.tris, a.root, a.hits, a.uv,
+                           a.any_out, a.cnt_nodes, a.cnt_prims);
+        return hipGetLastError();
+    }
+    auto kern = k_trace_lanes<kBlock, STACK, ANY, ORDERED, SPILL>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const uint32_t resident = (uint32_t)per_cu * (uint32_t)a.grid_cap;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, a.counter);
+    hipLaunchKernelGGL(kern, dim3(grid < resident ? grid : resident), dim3(kBlock), 0, s, a.rays,
+                       a.n, a.nodes, a.tris, a.root, a.hits, a.uv, a.any_out, a.counter, a.refill);
     return hipGetLastError();
 }
 
 template <bool ANY, bool ORDERED, bool COUNT>
 hipError_t launch_depth(const TraceArgs& a, int depth, hipStream_t s) {
     if (depth <= 16) return launch<16, ANY, ORDERED, COUNT, false>(a, s);
+    if (!COUNT && a.counter) {
+        // persistent lanes: a 16-entry LDS stack keeps 5 waves per SIMD
+        // resident; deeper entries (rare) go to the private spill array
+        return launch<16, ANY, ORDERED, COUNT, true>(a, s);
+    }
     if (depth <= 32) return launch<32, ANY, ORDERED, COUNT, false>(a, s);
     return launch<32, ANY, ORDERED, COUNT, true>(a, s);
 }
@@ -85,6 +175,7 @@ int check_common(rumi_ctx* ctx, const rumi_bvh* bvh, const void* rays, uint32_t 
     if (n && !rays) return fail(RUMI_ERR_INVALID, "trace: null ray buffer");
     if (((uintptr_t)rays & 15u) != 0) return fail(RUMI_ERR_INVALID, "trace: rays must be 16-byte aligned");
     if (flags & ~3u) return fail(RUMI_ERR_INVALID, "trace: unknown flags 0x%x", flags);
+    if (n > 0x7fffffffu) return fail(RUMI_ERR_INVALID, "trace: more than 2^31 - 1 rays");
     if ((flags & RUMI_TRACE_REF_ORDER) && (flags & RUMI_TRACE_ORDERED))
         return fail(RUMI_ERR_INVALID, "trace: REF_ORDER and ORDERED are exclusive");
     if (flags & RUMI_TRACE_ORDERED)
@@ -98,8 +189,11 @@ int check_common(rumi_ctx* ctx, const rumi_bvh* bvh, const void* rays, uint32_t 
     return RUMI_OK;
 }
 
-TraceArgs base_args(const rumi_bvh* bvh, const rumi_ray* rays, uint32_t n) {
+TraceArgs base_args(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* rays, uint32_t n) {
     TraceArgs a{};
+    a.counter = ctx->take_counter();
+    a.grid_cap = ctx->cu_count;
+    a.refill = 16;
     a.rays = reinterpret_cast<const float4*>(rays);
     a.n = n;
     a.nodes = reinterpret_cast<const float4*>(bvh->dev_nodes);
@@ -129,7 +223,7 @@ extern "C" int rumi_trace_closest(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi
     if (((uintptr_t)hits & 7u) || ((uintptr_t)uv & 7u))
         return fail(RUMI_ERR_INVALID, "rumi_trace_closest: hits and uv must be 8-byte aligned");
     if (n == 0) return RUMI_OK;
-    TraceArgs a = base_args(bvh, rays, n);
+    TraceArgs a = base_args(ctx, bvh, rays, n);
     a.hits = reinterpret_cast<int2*>(hits);
     a.uv = reinterpret_cast<float2*>(uv);
     return finish(launch_order<false, false>(a, ordered, bvh->flat.max_depth, (hipStream_t)stream),
@@ -143,7 +237,7 @@ extern "C" int rumi_trace_any(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray
     if (int rc = check_common(ctx, bvh, rays, n, flags, &ordered)) return rc;
     if (n && !prim) return fail(RUMI_ERR_INVALID, "rumi_trace_any: null output");
     if (n == 0) return RUMI_OK;
-    TraceArgs a = base_args(bvh, rays, n);
+    TraceArgs a = base_args(ctx, bvh, rays, n);
     a.any_out = prim;
     return finish(launch_order<true, false>(a, ordered, bvh->flat.max_depth, (hipStream_t)stream),
                   "rumi_trace_any");
@@ -159,7 +253,7 @@ extern "C" int rumi_trace_count(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_r
         return fail(RUMI_ERR_INVALID, "rumi_trace_count: null output");
     if ((uintptr_t)hits & 7u) return fail(RUMI_ERR_INVALID, "rumi_trace_count: hits must be 8-byte aligned");
     if (n == 0) return RUMI_OK;
-    TraceArgs a = base_args(bvh, rays, n);
+    TraceArgs a = base_args(ctx, bvh, rays, n);
     a.cnt_nodes = node_count;
     a.cnt_prims = prim_count;
     a.hits = reinterpret_cast<int2*>(hits);

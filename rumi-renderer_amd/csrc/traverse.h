@@ -163,9 +163,159 @@ __device__ __forceinline__ bool hit_rect(const float4* rect, bool emissive, Ray&
     return true;
 }
 
-// One traversal.  Returns the hit primitive's input index or -1; r.tmin
-// holds the hit distance.  MIXED enables sphere/rect leaf records (tag in
-// c.z, rect side table index in c.x, emissive flag in c.w).
+// Traversal state of one ray between steps: the ray (r.tmin = closest hit
+// so far), its reciprocal direction, the node or leaf to visit next, the
+// stack depth and the hit.
+struct TravState {
+    Ray r;
+    float ix, iy, iz;
+    int32_t cur;
+    int32_t sp;
+    int32_t hit;
+    float hu, hv;
+};
+
+// Starts a traversal at the root: false when the ray has a zero direction or
+// misses the root box (BVHAccel::Travel's first node test, bvh.cpp:164-169).
+__device__ __forceinline__ bool trav_begin(const RootInfo& root, TravState& s) {
+    s.hit = -1;
+    s.sp = 0;
+    s.cur = root.ref;
+    s.hu = 0.0f;
+    s.hv = 0.0f;
+    if (s.r.dx == 0.0f && s.r.dy == 0.0f && s.r.dz == 0.0f) return false;
+    s.ix = 1.0f / s.r.dx;
+    s.iy = 1.0f / s.r.dy;
+    s.iz = 1.0f / s.r.dz;
+    float tn;
+    return box_static(root.lo[0], root.lo[1], root.lo[2], root.hi[0], root.hi[1], root.hi[2],
+                      s.r.ox, s.r.oy, s.r.oz, s.ix, s.iy, s.iz, s.r.tmax, tn) &&
+           closer(s.r.tmin, tn);
+}
+
+// One traversal step: visits node or leaf s.cur; after a leaf, or a node with
+// no child to enter, pops entries until one is closer than the current hit.
+// Returns false when the traversal is over (stack empty, or an any-hit query
+// found its hit).  MIXED enables sphere/rect leaf records (tag in c.z, rect
+// side table index in c.x, emissive flag in c.w).  Stack entries beyond
+// STACK live in p_ref/p_tn (SPILL only).
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool MIXED, bool COUNT, bool SPILL>
+__device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes,
+                                          const float4* __restrict__ prims,
+                                          const float4* __restrict__ rects, TravState& s,
+                                          bool in_object, int32_t& nn, int32_t& np,
+                                          int32_t* s_ref, float* s_tn, uint32_t tid,
+                                          int32_t* p_ref, float* p_tn) {
+    Ray& r = s.r;
+    if (s.cur >= 0) {
+        const float4* np4 = nodes + 4 * (size_t)s.cur;
+        const float4 a = np4[0];
+        const float4 b = np4[1];
+        const float4 c = np4[2];
+        const float4 d = np4[3];
+        if (COUNT) ++nn;
+        const int32_t ref0 = __float_as_int(d.x);
+        const int32_t ref1 = __float_as_int(d.y);
+        float tn0, tn1;
+        const bool h0 = box_static(a.x, a.y, a.z, a.w, b.x, b.y, r.ox, r.oy, r.oz, s.ix, s.iy, s.iz,
+                                   r.tmax, tn0);
+        const bool h1 = box_static(b.z, b.w, c.x, c.y, c.z, c.w, r.ox, r.oy, r.oz, s.ix, s.iy, s.iz,
+                                   r.tmax, tn1);
+        const bool a0 = h0 && closer(r.tmin, tn0);
+        const bool a1 = h1 && closer(r.tmin, tn1);
+        int32_t push_ref = 0;
+        float push_tn = 0.0f;
+        bool do_push = false;
+        bool descend = true;
+        if (ORDERED) {
+            if (a0 && a1) {
+                const bool swap = tn1 < tn0;
+                s.cur = swap ? ref1 : ref0;
+                push_ref = swap ? ref0 : ref1;
+                push_tn = swap ? tn0 : tn1;
+                do_push = true;
+            } else if (a0) {
+                s.cur = ref0;
+            } else if (a1) {
+                s.cur = ref1;
+            } else {
+                descend = false;
+            }
+        } else {
+            // Reference order: the right child waits on the stack, the left
+            // child is visited next.
+            do_push = a1;
+            push_ref = ref1;
+            push_tn = tn1;
+            if (a0)
+                s.cur = ref0;
+            else
+                descend = false;
+        }
+        if (do_push) {
+            if (!SPILL || s.sp < STACK) {
+                s_ref[s.sp * BLOCK + tid] = push_ref;
+                s_tn[s.sp * BLOCK + tid] = push_tn;
+            } else {
+                p_ref[s.sp - STACK] = push_ref;
+                p_tn[s.sp - STACK] = push_tn;
+            }
+            ++s.sp;
+        }
+        if (descend) return true;
+    } else {
+        const int32_t enc = ~s.cur;
+        const int32_t first = enc >> 4;
+        const int32_t count = enc & 15;
+        for (int32_t k = 0; k < count; ++k) {
+            const float4* t4 = prims + 3 * (size_t)(first + k);
+            const float4 ta = t4[0];
+            const float4 tb = t4[1];
+            const float4 tc = t4[2];
+            if (COUNT) ++np;
+            float u = 0.0f, v = 0.0f;
+            bool ok;
+            if (!MIXED) {
+                ok = hit_triangle(ta, tb, tc, r, u, v);
+            } else {
+                const int32_t tag = __float_as_int(tc.z);
+                if (tag == kTagTri)
+                    ok = hit_triangle(ta, tb, tc, r, u, v);
+                else if (tag == kTagSphere)
+                    ok = hit_sphere(ta, r, in_object);
+                else
+                    ok = hit_rect(rects + 4 * (size_t)__float_as_int(tc.x), __float_as_int(tc.w) != 0,
+                                  r);
+            }
+            if (ok) {
+                s.hit = __float_as_int(tc.y);
+                s.hu = u;
+                s.hv = v;
+            }
+        }
+        if (ANY && r.tmin > 0.0f) return false;
+    }
+    while (s.sp > 0) {
+        --s.sp;
+        int32_t rr;
+        float t;
+        if (!SPILL || s.sp < STACK) {
+            rr = s_ref[s.sp * BLOCK + tid];
+            t = s_tn[s.sp * BLOCK + tid];
+        } else {
+            rr = p_ref[s.sp - STACK];
+            t = p_tn[s.sp - STACK];
+        }
+        if (closer(r.tmin, t)) {
+            s.cur = rr;
+            return true;
+        }
+    }
+    return false;
+}
+
+// One whole traversal.  Returns the hit primitive's input index or -1;
+// r.tmin holds the hit distance.
 template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool MIXED, bool COUNT, bool SPILL>
 __device__ __forceinline__ int32_t traverse(const float4* __restrict__ nodes,
                                             const float4* __restrict__ prims,
@@ -176,126 +326,16 @@ __device__ __forceinline__ int32_t traverse(const float4* __restrict__ nodes,
     constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
     int32_t p_ref[kSpill];
     float p_tn[kSpill];
-    int32_t hit = -1;
-    if (r.dx == 0.0f && r.dy == 0.0f && r.dz == 0.0f) return -1;
-    const float ix = 1.0f / r.dx;
-    const float iy = 1.0f / r.dy;
-    const float iz = 1.0f / r.dz;
-    float tn;
-    bool go = box_static(root.lo[0], root.lo[1], root.lo[2], root.hi[0], root.hi[1], root.hi[2],
-                         r.ox, r.oy, r.oz, ix, iy, iz, r.tmax, tn) &&
-              closer(r.tmin, tn);
-    int32_t cur = root.ref;
-    int sp = 0;
-    while (go) {
-        if (cur >= 0) {
-            const float4* np4 = nodes + 4 * (size_t)cur;
-            const float4 a = np4[0];
-            const float4 b = np4[1];
-            const float4 c = np4[2];
-            const float4 d = np4[3];
-            if (COUNT) ++nn;
-            const int32_t ref0 = __float_as_int(d.x);
-            const int32_t ref1 = __float_as_int(d.y);
-            float tn0, tn1;
-            const bool h0 = box_static(a.x, a.y, a.z, a.w, b.x, b.y, r.ox, r.oy, r.oz, ix, iy, iz,
-                                       r.tmax, tn0);
-            const bool h1 = box_static(b.z, b.w, c.x, c.y, c.z, c.w, r.ox, r.oy, r.oz, ix, iy, iz,
-                                       r.tmax, tn1);
-            const bool a0 = h0 && closer(r.tmin, tn0);
-            const bool a1 = h1 && closer(r.tmin, tn1);
-            int32_t push_ref = 0;
-            float push_tn = 0.0f;
-            bool do_push = false;
-            bool descend = true;
-            if (ORDERED) {
-                if (a0 && a1) {
-                    const bool swap = tn1 < tn0;
-                    cur = swap ? ref1 : ref0;
-                    push_ref = swap ? ref0 : ref1;
-                    push_tn = swap ? tn0 : tn1;
-                    do_push = true;
-                } else if (a0) {
-                    cur = ref0;
-                } else if (a1) {
-                    cur = ref1;
-                } else {
-                    descend = false;
-                }
-            } else {
-                // Reference order: the right child waits on the stack, the
-                // left child is visited next.
-                do_push = a1;
-                push_ref = ref1;
-                push_tn = tn1;
-                if (a0)
-                    cur = ref0;
-                else
-                    descend = false;
-            }
-            if (do_push) {
-                if (!SPILL || sp < STACK) {
-                    s_ref[sp * BLOCK + tid] = push_ref;
-                    s_tn[sp * BLOCK + tid] = push_tn;
-                } else {
-                    p_ref[sp - STACK] = push_ref;
-                    p_tn[sp - STACK] = push_tn;
-                }
-                ++sp;
-            }
-            if (descend) continue;
-        } else {
-            const int32_t enc = ~cur;
-            const int32_t first = enc >> 4;
-            const int32_t count = enc & 15;
-            for (int32_t k = 0; k < count; ++k) {
-                const float4* t4 = prims + 3 * (size_t)(first + k);
-                const float4 ta = t4[0];
-                const float4 tb = t4[1];
-                const float4 tc = t4[2];
-                if (COUNT) ++np;
-                float u = 0.0f, v = 0.0f;
-                bool ok;
-                if (!MIXED) {
-                    ok = hit_triangle(ta, tb, tc, r, u, v);
-                } else {
-                    const int32_t tag = __float_as_int(tc.z);
-                    if (tag == kTagTri)
-                        ok = hit_triangle(ta, tb, tc, r, u, v);
-                    else if (tag == kTagSphere)
-                        ok = hit_sphere(ta, r, in_object);
-                    else
-                        ok = hit_rect(rects + 4 * (size_t)__float_as_int(tc.x),
-                                      __float_as_int(tc.w) != 0, r);
-                }
-                if (ok) {
-                    hit = __float_as_int(tc.y);
-                    hu = u;
-                    hv = v;
-                }
-            }
-            if (ANY && r.tmin > 0.0f) break;
+    TravState s;
+    s.r = r;
+    if (trav_begin(root, s))
+        while (trav_step<BLOCK, STACK, ANY, ORDERED, MIXED, COUNT, SPILL>(
+            nodes, prims, rects, s, in_object, nn, np, s_ref, s_tn, tid, p_ref, p_tn)) {
         }
-        go = false;
-        while (sp > 0) {
-            --sp;
-            int32_t rr;
-            float t;
-            if (!SPILL || sp < STACK) {
-                rr = s_ref[sp * BLOCK + tid];
-                t = s_tn[sp * BLOCK + tid];
-            } else {
-                rr = p_ref[sp - STACK];
-                t = p_tn[sp - STACK];
-            }
-            if (closer(r.tmin, t)) {
-                cur = rr;
-                go = true;
-                break;
-            }
-        }
-    }
-    return hit;
+    r = s.r;
+    hu = s.hu;
+    hv = s.hv;
+    return s.hit;
 }
 
 }  // namespace dev
