@@ -42,6 +42,10 @@ def main():
     ap.add_argument("--cpu-rays", type=int, default=1 << 20,
                     help="rays in the bounded CPU-baseline sample (0 = skip)")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing legs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-scene leg")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default traversal launch shape instead of autotuning it")
     ap.add_argument("--render-runs", type=int, default=3)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the "
@@ -116,9 +120,26 @@ def main():
         torch.cuda.synchronize()
         return start.elapsed_time(end) / steps  # ms per launch
 
+    tuning = None
+    if not args.no_tune:
+        # launch shape chosen by the autotuner on this workload (untimed setup;
+        # every candidate returns the same bits)
+        sys.path.insert(0, os.path.join(ROOT, "rumi-renderer_amd"))
+        import autotune
+        best, table = autotune.tune_trace(
+            ctx, lambda: rumi.trace_closest(ctx, bvh, rays, hits, None, 0, stream))
+        tuning = {"closest": best,
+                  "candidates_us": [[c["block"], c["stack"], c["refill"], c["persistent"],
+                                     round(t * 1e6, 1)] for c, t in table]}
+        log(f"autotuned closest-hit: {best}")
+
     ms_c = timed(lambda: rumi.trace_closest(ctx, bvh, rays, hits, None, 0, stream),
                  args.steps, args.warmup)
     log(f"closest-hit: {ms_c:.3f} ms/step, {n / ms_c / 1e3:.1f} Mrays/s")
+    if tuning is not None:
+        best_a, _ = autotune.tune_trace(ctx, lambda: rumi.trace_any(ctx, bvh, rays, anyp, 0, stream))
+        tuning["any"] = best_a
+        log(f"autotuned any-hit: {best_a}")
     ms_a = timed(lambda: rumi.trace_any(ctx, bvh, rays, anyp, 0, stream), args.steps, args.warmup)
     log(f"any-hit: {ms_a:.3f} ms/step, {n / ms_a / 1e3:.1f} Mrays/s")
 
@@ -169,6 +190,7 @@ def main():
             "bytes_per_ray": round(b_ray_c, 1), "nodes_per_ray": round(nodes_c, 3),
             "tris_per_ray": round(tris_c, 3),
         },
+        "tuning": tuning,
         "cpu_baseline": cpu,
         "any_hit": {
             "value": round(mrays_a, 2), "unit": "Mrays/s", "ms_per_step": round(ms_a, 4),
@@ -268,7 +290,118 @@ def render_legs(rumi, ctx, args, log):
         scene.close()
     out["media_gradient"] = gradient_leg(rumi, ctx, args, log, scenes)
     out["cornell_bdpt"] = bdpt_leg(rumi, ctx, args, log, scenes)
+    if not args.no_batch:
+        out["batch"] = batch_leg(rumi, ctx, args, log, scenes)
     return out
+
+
+def batch_leg(rumi, ctx, args, log, scenes):
+    """256 scenes (noisy icospheres of 1k-200k triangles, log-uniform, each
+    with a 128^2 albedo texture, a floor and an area light; image sides
+    log-uniform in 128..512) at 8 spp, depth 6, as one wavefront: the
+    forward render, dLoss/dImage of the L2 loss against targets (the same
+    scenes with textures x 0.8) and the adjoint with respect to every texel,
+    captured as one hipGraph.  The generation size is autotuned.  Reports
+    samples/s of the forward pass alone, grad/s of the whole graph and the
+    per-scene completion spread of the forward pass inside it."""
+    import numpy as np
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "rumi-renderer_amd"))
+    import autotune
+    spp, depth = 8, 6
+    params = scenes.batch_params(args.batch_scenes)
+    t0 = time.perf_counter()
+
+    def make(i, scale):
+        _, freq, side = params[i]
+        return rumi.Scene(ctx, scenes.batch_scene(i, freq, side, tex_scale=scale))
+
+    with ThreadPoolExecutor(16) as ex:
+        objs = list(ex.map(lambda i: make(i, 1.0), range(len(params))))
+        tobjs = list(ex.map(lambda i: make(i, 0.8), range(len(params))))
+    items = [(o, side, side) for o, (_, _, side) in zip(objs, params)]
+    tris = int(sum(20 * f * f + 2 for _, f, _ in params))
+    log(f"batch: {len(items)} scenes, {tris} triangles, setup {time.perf_counter() - t0:.1f}s")
+    tb = rumi.Batch(ctx, [(o, s_, s_) for o, (_, _, s_) in zip(tobjs, params)], spp, depth, seed=77)
+    targets = tb.run(use_graph=False, item_ms=False)[0].clone()
+    torch.cuda.synchronize()
+    tb.close()
+    for o in tobjs:
+        o.close()
+
+    def timed_graph(b, **kw):
+        b.run(use_graph=True, item_ms=False, **kw)  # capture + warm-up
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(max(1, args.render_runs - 1)):
+            r = b.run(use_graph=True, **kw)
+            torch.cuda.synchronize()
+            best = r if best is None or r[3].milliseconds < best[3].milliseconds else best
+        return best
+
+    # generation size: autotuned on the full graph (one replay after capture)
+    results = {}
+
+    def build(cap):
+        b = rumi.Batch(ctx, items, spp, depth, seed=5, queue_capacity=cap, adjoint=True)
+        b.capacity = cap
+        return b
+
+    def measure(b):
+        results[b.capacity] = timed_graph(b, targets=targets)
+        return results[b.capacity][3].milliseconds * 1e-3
+
+    best_cap, table = autotune.tune_batch(build, measure, release=lambda b: b.close())
+    best_run = results[best_cap]
+    best_ms = best_run[3].milliseconds
+    table = [[c, round(t * 1e3, 2)] for c, t in table]
+    log(f"batch: generation sizes {table} -> {best_cap}")
+    _, tg, loss, st, item_ms = best_run
+    grad_s = 1000.0 / best_ms
+    fb = rumi.Batch(ctx, items, spp, depth, seed=5, queue_capacity=best_cap)
+    fr = timed_graph(fb)
+    fwd_ms = fr[3].milliseconds
+    fb.close()
+    lat = np.array(item_ms)
+    res = {
+        "scenes": len(items), "triangles": tris, "pixels": int(st.total_pixels),
+        "texels": int(st.total_texels), "spp": spp, "max_depth": depth,
+        "samples": int(st.total_samples), "generation_size": best_cap,
+        "generations": int(st.generations), "generation_sizes_ms": table,
+        "forward_samples_per_s": round(st.total_samples / (fwd_ms * 1e-3), 1),
+        "forward_ms": round(fwd_ms, 2),
+        "graph_ms": round(best_ms, 2), "grads_per_s": round(grad_s, 4),
+        "unit": "grad/s (forward + dLoss/dImage + adjoint for all texels of all scenes)",
+        "graph_samples_per_s": round(3 * st.total_samples / (best_ms * 1e-3), 1),
+        "scene_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                             "p95": round(float(np.percentile(lat, 95)), 2),
+                             "max": round(float(lat.max()), 2),
+                             "min": round(float(lat.min()), 2)},
+        "loss_total": float(loss.double().sum()),
+        "texel_grad_l1": float(tg.double().abs().sum()),
+    }
+    # CPU baseline: the oracle's FP64 render + adjoint of scene 0 at 64^2, one thread
+    import oracle
+    _, f0, _ = params[0]
+    sc0 = scenes.batch_scene(0, f0, 64)
+    t64 = oracle.render_pt(scenes.batch_scene(0, f0, 64, tex_scale=0.8), 64, 64, spp, depth,
+                           seed=3, nthreads=8)[0]
+    t0 = time.perf_counter()
+    i64 = oracle.render_pt(sc0, 64, 64, spp, depth, seed=4, nthreads=1)[0]
+    oracle.adjoint_pt(sc0, 2.0 * (i64 - t64), spp, depth, seed=5, nthreads=1, textures=True)
+    cpu_sec = time.perf_counter() - t0
+    cpu_sps = 3 * 64 * 64 * spp / cpu_sec
+    res["cpu_baseline"] = {"value": round(cpu_sps, 1), "unit": "samples/s (render + adjoint)",
+                           "cores": 1, "kind": "port",
+                           "sample": f"scene 0 at 64x64, {spp} spp: the oracle's FP64 render and "
+                                     f"forward-mode texel adjoint ({cpu_sec:.2f}s)"}
+    res["speedup_vs_cpu"] = round(res["graph_samples_per_s"] / cpu_sps, 1)
+    log(f"batch: forward {res['forward_samples_per_s'] / 1e6:.1f} Msamples/s, graph {best_ms:.0f} ms "
+        f"= {grad_s:.3f} grad/s, latency p50/p95/max {res['scene_latency_ms']}")
+    for o in objs:
+        o.close()
+    return res
 
 
 def gradient_leg(rumi, ctx, args, log, scenes):

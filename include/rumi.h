@@ -151,6 +151,24 @@ int rumi_trace_count(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* dev_ray
                      uint32_t n, int32_t any, rumi_hit* dev_hits, int32_t* dev_node_count,
                      int32_t* dev_prim_count, uint32_t flags, void* stream);
 
+/* Launch shape of rumi_trace_closest / rumi_trace_any on this context (no
+ * counterpart in the reference, whose traversal is one call per ray).  By
+ * default every lane steps its own ray and swaps in a ray it has already
+ * prefetched when its ray ends (persistent 2); persistent 1 instead refills
+ * a wave from a shared counter once `refill` of its lanes are idle; -1 is one
+ * ray per lane.  Zero fields keep the defaults: block 128 threads, stack 16
+ * LDS entries per lane (deeper entries spill to private memory), refill 16,
+ * persistent 2.  Results are the same bits for every setting. */
+typedef struct {
+    int32_t block;       /* 64, 128 or 256 */
+    int32_t stack;       /* 8 or 16 */
+    int32_t refill;      /* 1..64 (persistent 1) */
+    int32_t persistent;  /* 2 prefetching lanes, 1 threshold refill, -1 one ray per lane */
+} rumi_trace_tuning;
+
+int rumi_trace_set_tuning(rumi_ctx* ctx, const rumi_trace_tuning* tuning);
+int rumi_trace_get_tuning(const rumi_ctx* ctx, rumi_trace_tuning* tuning);
+
 /* ----------------------------------------------------------------- scenes */
 /* A scene replaces the reference's Scene object graph (src/Scene/scene.h:
  * 16-53): shapes with materials, area lights, an optional homogeneous medium
@@ -299,6 +317,55 @@ typedef struct {
 int rumi_adjoint(rumi_ctx* ctx, const rumi_scene* scene, const rumi_render_desc* desc,
                  const float* dev_dloss_dimage, const rumi_grad_buffers* grads,
                  rumi_render_stats* stats, void* stream);
+
+/* ---------------------------------------------------------------- batches */
+/* Many scenes rendered (and differentiated) as one wavefront.  No reference
+ * counterpart: the reference renders one hard-coded scene per process
+ * (Scene::Build + Scene::Rendering, src/Scene/scene.cpp:135-258).  Samples
+ * are numbered item after item; each item's image equals the image
+ * rumi_render gives its scene alone with the same seed (up to the order of
+ * float atomics).  The launch schedule is fixed at creation (generations of
+ * queue_capacity samples, max_depth iterations each), so a run needs no host
+ * round trip; rumi_batch_run with use_graph captures it once as a hipGraph
+ * and replays it. */
+typedef struct rumi_batch rumi_batch;
+
+typedef struct {
+    const rumi_scene* scene;  /* the item's scene and camera */
+    int32_t width, height;
+} rumi_batch_item;
+
+typedef struct {
+    int32_t spp;              /* samples per pixel of every item */
+    int32_t max_depth;        /* PathTracer max depth of every item */
+    uint64_t seed;            /* forward pass; the adjoint passes use seed + 1 */
+    uint32_t queue_capacity;  /* samples per generation (0 = 2^22) */
+    int32_t adjoint;          /* 1: the run also forms dLoss/dImage = 2 (I - target)
+                                 and runs the path-replay adjoint */
+} rumi_batch_desc;
+
+typedef struct {
+    float* dev_images;         /* out: every item's image, concatenated in item order */
+    const float* dev_targets;  /* adjoint: the target images, same layout */
+    float* dev_texel_grads;    /* adjoint, out: dLoss/dtexel, items' texel pools concatenated */
+    float* dev_loss;           /* adjoint, out (may be NULL): sum (I - T)^2 per item */
+} rumi_batch_io;
+
+typedef struct {
+    uint64_t total_samples, total_pixels, total_texels;
+    uint32_t n_items, generations;
+    double milliseconds;   /* device time of the run */
+    double* host_item_ms;  /* caller array (n_items) or NULL: when each item's last
+                              forward path ended, ms after the run began */
+} rumi_batch_stats;
+
+int rumi_batch_create(rumi_ctx* ctx, const rumi_batch_item* items, uint32_t n,
+                      const rumi_batch_desc* desc, rumi_batch** out);
+/* Runs the batch on stream (use_graph: replay the captured hipGraph,
+ * capturing it first if the io pointers changed).  Synchronises the stream. */
+int rumi_batch_run(rumi_batch* batch, const rumi_batch_io* io, int32_t use_graph,
+                   rumi_batch_stats* stats, void* stream);
+int rumi_batch_destroy(rumi_batch* batch);
 
 /* ------------------------------------------------------------ workloads */
 /* Seeded geodesic icosphere of frequency f (20 f^2 triangles, 10 f^2 + 2

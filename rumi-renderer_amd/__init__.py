@@ -90,6 +90,31 @@ class RenderStats(Structure):
                 ("iterations", c_uint64), ("milliseconds", c_double)]
 
 
+class TraceTuning(Structure):
+    _fields_ = [("block", c_int32), ("stack", c_int32), ("refill", c_int32),
+                ("persistent", c_int32)]
+
+
+class BatchItem(Structure):
+    _fields_ = [("scene", c_void_p), ("width", c_int32), ("height", c_int32)]
+
+
+class BatchDesc(Structure):
+    _fields_ = [("spp", c_int32), ("max_depth", c_int32), ("seed", c_uint64),
+                ("queue_capacity", c_uint32), ("adjoint", c_int32)]
+
+
+class BatchIO(Structure):
+    _fields_ = [("dev_images", c_void_p), ("dev_targets", c_void_p),
+                ("dev_texel_grads", c_void_p), ("dev_loss", c_void_p)]
+
+
+class BatchStats(Structure):
+    _fields_ = [("total_samples", c_uint64), ("total_pixels", c_uint64),
+                ("total_texels", c_uint64), ("n_items", c_uint32), ("generations", c_uint32),
+                ("milliseconds", c_double), ("host_item_ms", c_void_p)]
+
+
 class GradBuffers(Structure):
     _fields_ = [("d_medium", c_void_p), ("d_textures", c_void_p), ("d_image", c_void_p)]
 
@@ -116,12 +141,19 @@ _SIGNATURES = {
                                c_void_p]),
     "rumi_trace_count": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_int32, c_void_p,
                                  c_void_p, c_void_p, c_uint32, c_void_p]),
+    "rumi_trace_set_tuning": (c_int, [c_void_p, POINTER(TraceTuning)]),
+    "rumi_trace_get_tuning": (c_int, [c_void_p, POINTER(TraceTuning)]),
     "rumi_scene_create": (c_int, [c_void_p, POINTER(SceneDesc), POINTER(c_void_p)]),
     "rumi_scene_destroy": (c_int, [c_void_p]),
     "rumi_render": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
                             POINTER(RenderStats), c_void_p]),
     "rumi_adjoint": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
                              POINTER(GradBuffers), POINTER(RenderStats), c_void_p]),
+    "rumi_batch_create": (c_int, [c_void_p, POINTER(BatchItem), c_uint32, POINTER(BatchDesc),
+                                  POINTER(c_void_p)]),
+    "rumi_batch_run": (c_int, [c_void_p, POINTER(BatchIO), c_int32, POINTER(BatchStats),
+                               c_void_p]),
+    "rumi_batch_destroy": (c_int, [c_void_p]),
     "rumi_make_icosphere": (c_int, [c_uint32, c_uint64, c_float, c_void_p, c_void_p]),
     "rumi_make_rays_uniform": (c_int, [c_uint64, c_uint64, c_float, c_void_p]),
     "rumi_pcg32_sequence": (c_int, [c_uint64, c_uint64, c_uint32, c_void_p]),
@@ -204,6 +236,16 @@ class Context:
         h = c_void_p()
         check(lib().rumi_ctx_create(device, byref(h)))
         self.handle = h
+
+    def set_trace_tuning(self, block=0, stack=0, refill=0, persistent=0):
+        """Launch shape of trace_closest / trace_any (zero = default)."""
+        check(lib().rumi_trace_set_tuning(self.handle, byref(TraceTuning(block, stack, refill,
+                                                                         persistent))))
+
+    def trace_tuning(self) -> dict:
+        t = TraceTuning()
+        check(lib().rumi_trace_get_tuning(self.handle, byref(t)))
+        return dict(block=t.block, stack=t.stack, refill=t.refill, persistent=t.persistent)
 
     def close(self):
         if self.handle:
@@ -421,6 +463,70 @@ class Scene:
     def close(self):
         if getattr(self, "handle", None):
             lib().rumi_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many scenes rendered as one wavefront (rumi_batch_*).  items: list of
+    (Scene, width, height)."""
+
+    def __init__(self, ctx: Context, items, spp: int, max_depth: int, seed: int = 0,
+                 queue_capacity: int = 0, adjoint: bool = False):
+        self.ctx = ctx
+        self.items = list(items)
+        arr = (BatchItem * len(self.items))()
+        for i, (scene, w, h) in enumerate(self.items):
+            arr[i].scene, arr[i].width, arr[i].height = scene.handle, w, h
+        d = BatchDesc(spp, max_depth, seed, queue_capacity, 1 if adjoint else 0)
+        h = c_void_p()
+        check(lib().rumi_batch_create(ctx.handle, arr, len(self.items), byref(d), byref(h)))
+        self.handle = h
+        self.adjoint = adjoint
+        self.sizes = [(w, h) for _, w, h in self.items]
+        self.total_pixels = sum(w * h for w, h in self.sizes)
+        self.n_texels = sum(scene.n_texels for scene, _, _ in self.items)
+
+    def split(self, flat):
+        """Per-item (height, width, 3) views of a concatenated image tensor."""
+        out, o = [], 0
+        for w, h in self.sizes:
+            out.append(flat[o:o + w * h].view(h, w, 3))
+            o += w * h
+        return out
+
+    def run(self, images=None, targets=None, texel_grads=None, loss=None, use_graph=True,
+            item_ms=True, stream=None):
+        """Returns (images (total_pixels, 3), texel_grads or None, loss or None,
+        BatchStats, per-item completion ms or None)."""
+        import torch
+        dev = "cuda"
+        if images is None:
+            images = torch.empty((self.total_pixels, 3), dtype=torch.float32, device=dev)
+        if self.adjoint:
+            if texel_grads is None:
+                texel_grads = torch.empty((max(1, self.n_texels), 3), dtype=torch.float32,
+                                          device=dev)
+            if loss is None:
+                loss = torch.empty(len(self.items), dtype=torch.float32, device=dev)
+        io = BatchIO(_ptr(images), _ptr(targets), _ptr(texel_grads) if self.adjoint else None,
+                     _ptr(loss) if self.adjoint else None)
+        ms = (c_double * len(self.items))() if item_ms else None
+        st = BatchStats()
+        st.host_item_ms = ctypes.cast(ms, c_void_p) if item_ms else None
+        check(lib().rumi_batch_run(self.handle, byref(io), 1 if use_graph else 0, byref(st),
+                                   _stream(stream)))
+        return (images, texel_grads if self.adjoint else None, loss if self.adjoint else None, st,
+                list(ms) if item_ms else None)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().rumi_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -55,11 +55,11 @@ extern "C" int rumi_ctx_destroy(rumi_ctx* ctx) {
 static void release_device(rumi_bvh* b) {
     if (b->device >= 0) {
         int prev = 0;
-        hipGetDevice(&prev);
-        hipSetDevice(b->device);
-        if (b->dev_nodes) hipFree(b->dev_nodes);
-        if (b->dev_tris) hipFree(b->dev_tris);
-        hipSetDevice(prev);
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(b->device);
+        if (b->dev_nodes) (void)hipFree(b->dev_nodes);
+        if (b->dev_tris) (void)hipFree(b->dev_tris);
+        (void)hipSetDevice(prev);
     }
     b->dev_nodes = b->dev_tris = nullptr;
     b->device = -1;

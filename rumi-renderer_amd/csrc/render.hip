@@ -120,23 +120,48 @@ struct PathBuf {
     float4* ray;       // 2 per path: o.xyz tmin | d.xyz tmax
     float4* beta;      // throughput rgb
     uint64_t* rng;     // PCG32 state (increment derived from the pixel)
-    uint32_t* pixel;
-    uint32_t* sample;  // sample index within the frame
+    uint32_t* pixel;   // pixel within its scene (selects the PCG stream)
+    uint32_t* sample;  // sample index within the frame (the batch, for batches)
     uint32_t* flags;   // bit 0 le, bit 1 in_object, bits 8.. depth
     float* pend;       // adjoint replay: score of the sampling that produced this ray
+    uint32_t* scene;   // batches: the path's scene
 };
 
 struct ShadowBuf {
     float4* ray;       // 2 per entry
-    float4* contrib;   // rgb, pixel (int bits) in w
+    float4* contrib;   // rgb, film pixel (int bits) in w
     uint32_t* sample;  // adjoint passes: the sample the contribution belongs to
     float2* score;     // adjoint replay: the connection's own score terms
+    uint32_t* scene;   // batches: the entry's scene
 };
 
 struct Params {
     int32_t width, height, max_depth;
     uint32_t pixels;
     uint64_t seed;
+    uint32_t n_scenes;  // batches
+};
+
+// One scene of a batch: its tables, image and offsets into the batch's
+// concatenated sample, film and texel-gradient arrays.
+struct SceneTab {
+    SceneView sv;
+    int32_t width, height, spp;
+    uint32_t pixels;
+    uint32_t sample_base;
+    uint32_t pixel_base;
+    uint32_t texel_base;
+    uint32_t pad;
+};
+
+// Device-side control block of the wavefront loop: queue sizes, bucket
+// counts and offsets live here so that no stage needs the host to read them.
+struct Ctl {
+    uint32_t queue[2];      // paths in path buffer 0 / 1
+    uint32_t shadow;        // entries in the shadow queue
+    uint32_t pad0[5];
+    uint32_t bucket[8];     // paths per material key (k_bucket)
+    int32_t offset[12];     // exclusive prefix of bucket[] (k_offsets), offset[8] = total
 };
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
@@ -164,8 +189,9 @@ __device__ inline void film_add(float* film, uint32_t pixel, V3 c) {
 
 // ------------------------------------------------------------------ kernels
 __global__ void k_zero(float* p, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -178,72 +204,146 @@ __global__ void k_scale(float* p, size_t n, float s) {
     if (i < n) p[i] *= s;
 }
 
-// Pinhole::CastViewRay (src/Camera/pinhole.cpp:97-136) for sample k:
-// pixel = k mod P, per-pixel PCG32 stream.
-__global__ __launch_bounds__(kBlock) void k_regen(PathBuf buf, uint32_t start, uint32_t count,
-                                                  uint64_t sample_base, Params prm, DevCamera cam) {
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= count) return;
-    const uint32_t slot = start + j;
-    const uint64_t k = sample_base + j;
-    const uint32_t pixel = (uint32_t)(k % prm.pixels);
-    const uint64_t s = k / prm.pixels;
-    Rng g = rng_seed(prm.seed + s * 0x9E3779B97F4A7C15ull, pixel);
-    const int x = (int)(pixel % (uint32_t)prm.width);
-    const int y = (int)(pixel / (uint32_t)prm.width);
-    const float sx = g.uniform(), sy = g.uniform();
-    const float hw = prm.width * 0.5f, hh = prm.height * 0.5f;
-    const float px = -(((float)x - hw) + sx) / hw * cam.tanfov;
-    const float py = -(((float)y - hh) + sy) / hh * cam.tanfov * cam.y_ratio;
-    const V3 l = normalize(v3(px, py, 1.0f));
-    const V3 d = v3(cam.R[0] * l.x + cam.R[1] * l.y + cam.R[2] * l.z,
-                    cam.R[3] * l.x + cam.R[4] * l.y + cam.R[5] * l.z,
-                    cam.R[6] * l.x + cam.R[7] * l.y + cam.R[8] * l.z);
-    buf.ray[2 * (size_t)slot + 0] = make_float4(cam.p[0], cam.p[1], cam.p[2], 0.0f);
-    buf.ray[2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, FLT_MAX);
-    buf.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    buf.rng[slot] = g.state;
-    buf.pixel[slot] = pixel;
-    buf.sample[slot] = (uint32_t)k;
-    buf.pend[slot] = 0.0f;
-    buf.flags[slot] = 1u;  // le = true, outside objects, depth 0
+// The scene a path belongs to: the only one, or its entry in the batch table.
+template <bool MULTI>
+__device__ __forceinline__ const SceneView& view_of(const SceneView& sv, const SceneTab* tab,
+                                                    uint32_t scene) {
+    if (MULTI) return tab[scene].sv;
+    return sv;
 }
 
-template <bool SPILL>
-__global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, uint32_t n, SceneView sv,
-                                                   int2* hits, float2* uvs) {
+// Grid-stride loop over n items in wave-sized steps: every lane of a wave
+// runs the same iterations (so wave-wide ballots are safe); i = base + lane.
+#define RUMI_WAVE_LOOP(n)                                                                      \
+    for (uint32_t base_ = blockIdx.x * kBlock + (threadIdx.x & ~63u); base_ < (n);            \
+         base_ += gridDim.x * kBlock)
+
+// Pinhole::CastViewRay (src/Camera/pinhole.cpp:97-136) for sample k of the
+// frame: pixel = k mod P, s = k / P, PCG32 stream (seed + s * golden, pixel).
+// Batches number samples scene after scene; a scene's samples and streams are
+// the ones it has when rendered alone.  Writes count paths from slot start
+// and sets the queue size to start + count.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_regen(PathBuf buf, uint32_t start, uint32_t count,
+                                                  uint64_t sample_base, Params prm, DevCamera cam,
+                                                  const SceneTab* tab, uint32_t* queue_size) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *queue_size = start + count;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < count; j += gridDim.x * kBlock) {
+        const uint32_t slot = start + j;
+        const uint64_t k = sample_base + j;
+        uint32_t scene = 0, pixels = prm.pixels, width = (uint32_t)prm.width,
+                 height = (uint32_t)prm.height;
+        uint64_t local = k;
+        const DevCamera* cp = &cam;
+        if (MULTI) {
+            uint32_t lo = 0, hi = prm.n_scenes;  // last scene with sample_base <= k
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if ((uint64_t)tab[mid].sample_base <= k) lo = mid; else hi = mid;
+            }
+            scene = lo;
+            const SceneTab& t = tab[scene];
+            local = k - t.sample_base;
+            pixels = t.pixels;
+            width = (uint32_t)t.width;
+            height = (uint32_t)t.height;
+            cp = &t.sv.cam;
+        }
+        const DevCamera& c = *cp;
+        const uint32_t pixel = (uint32_t)(local % pixels);
+        const uint64_t s = local / pixels;
+        Rng g = rng_seed(prm.seed + s * 0x9E3779B97F4A7C15ull, pixel);
+        const int x = (int)(pixel % width);
+        const int y = (int)(pixel / width);
+        const float sx = g.uniform(), sy = g.uniform();
+        const float hw = (float)(int)width * 0.5f, hh = (float)(int)height * 0.5f;
+        const float px = -(((float)x - hw) + sx) / hw * c.tanfov;
+        const float py = -(((float)y - hh) + sy) / hh * c.tanfov * c.y_ratio;
+        const V3 l = normalize(v3(px, py, 1.0f));
+        const V3 d = v3(c.R[0] * l.x + c.R[1] * l.y + c.R[2] * l.z,
+                        c.R[3] * l.x + c.R[4] * l.y + c.R[5] * l.z,
+                        c.R[6] * l.x + c.R[7] * l.y + c.R[8] * l.z);
+        buf.ray[2 * (size_t)slot + 0] = make_float4(c.p[0], c.p[1], c.p[2], 0.0f);
+        buf.ray[2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, FLT_MAX);
+        buf.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        buf.rng[slot] = g.state;
+        buf.pixel[slot] = pixel;
+        buf.sample[slot] = (uint32_t)k;
+        buf.pend[slot] = 0.0f;
+        buf.flags[slot] = 1u;  // le = true, outside objects, depth 0
+        if (MULTI) buf.scene[slot] = scene;
+    }
+}
+
+// Closest hit of every queued path (BVHAccel::Travel with the path's
+// in_object flag, bvh.cpp:150-213).
+template <bool SPILL, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* n_ptr,
+                                                   SceneView sv, const SceneTab* tab, int2* hits,
+                                                   float2* uvs) {
     __shared__ int32_t s_ref[kStack * kBlock];
     __shared__ float s_tn[kStack * kBlock];
     const uint32_t tid = threadIdx.x;
-    const uint32_t i = blockIdx.x * kBlock + tid;
-    if (i >= n) return;
-    const float4 r0 = buf.ray[2 * (size_t)i + 0];
-    const float4 r1 = buf.ray[2 * (size_t)i + 1];
-    Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, r0.w};
-    const bool in_object = (buf.flags[i] & 2u) != 0;
-    float hu = 0.0f, hv = 0.0f;
-    int32_t nn = 0, np = 0;
-    const int32_t hit = traverse<kBlock, kStack, false, true, true, false, SPILL>(
-        sv.nodes, sv.prims, sv.rects, sv.root, r, in_object, hu, hv, nn, np, s_ref, s_tn, tid);
-    hits[i] = make_int2(hit, __float_as_int(r.tmin));
-    uvs[i] = make_float2(hu, hv);
+    const uint32_t n = *n_ptr;
+    for (uint32_t i = blockIdx.x * kBlock + tid; i < n; i += gridDim.x * kBlock) {
+        const SceneView& v = view_of<MULTI>(sv, tab, MULTI ? buf.scene[i] : 0u);
+        const float4 r0 = buf.ray[2 * (size_t)i + 0];
+        const float4 r1 = buf.ray[2 * (size_t)i + 1];
+        Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, r0.w};
+        const bool in_object = (buf.flags[i] & 2u) != 0;
+        float hu = 0.0f, hv = 0.0f;
+        int32_t nn = 0, np = 0;
+        const int32_t hit = traverse<kBlock, kStack, false, true, true, false, SPILL>(
+            v.nodes, v.prims, v.rects, v.root, r, in_object, hu, hv, nn, np, s_ref, s_tn, tid);
+        hits[i] = make_int2(hit, __float_as_int(r.tmin));
+        uvs[i] = make_float2(hu, hv);
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bucket(uint32_t n, const int2* hits, SceneView sv,
-                                                   uint32_t* lists, uint32_t* counts,
-                                                   uint32_t cap) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = i < n;
-    int key = kKeyMiss;
-    if (active) {
-        const int32_t prim = hits[i].x;
-        if (prim >= 0) key = sv.mats[sv.shapes[prim].info[1]].kind;
+// Material key of every hit (kind 0..6, 7 = miss) into 8 bucket lists.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_bucket(const uint32_t* n_ptr, const int2* hits,
+                                                   SceneView sv, const SceneTab* tab,
+                                                   const uint32_t* scenes, uint32_t* lists,
+                                                   uint32_t* counts, uint32_t cap) {
+    const uint32_t n = *n_ptr;
+    RUMI_WAVE_LOOP(n) {
+        const uint32_t i = base_ + lane_id();
+        const bool active = i < n;
+        int key = kKeyMiss;
+        if (active) {
+            const int32_t prim = hits[i].x;
+            if (prim >= 0) {
+                const SceneView& v = view_of<MULTI>(sv, tab, MULTI ? scenes[i] : 0u);
+                key = v.mats[v.shapes[prim].info[1]].kind;
+            }
+        }
+        for (int k = 0; k < kKeys; ++k) {
+            const bool mine = active && key == k;
+            const uint32_t slot = wave_append(&counts[k], mine);
+            if (mine) lists[(size_t)k * cap + slot] = i;
+        }
     }
-    for (int k = 0; k < kKeys; ++k) {
-        const bool mine = active && key == k;
-        const uint32_t slot = wave_append(&counts[k], mine);
-        if (mine) lists[(size_t)k * cap + slot] = i;
+}
+
+// Bucket offsets (exclusive prefix of the counts) for k_shade.
+__global__ void k_offsets(Ctl* ctl) {
+    if (threadIdx.x == 0) {
+        int32_t acc = 0;
+        for (int k = 0; k < kKeys; ++k) {
+            ctl->offset[k] = acc;
+            acc += (int32_t)ctl->bucket[k];
+        }
+        ctl->offset[kKeys] = acc;
     }
+}
+
+// Clears the counters the next wavefront iteration appends to.
+__global__ void k_clear_ctl(Ctl* ctl, int next_queue) {
+    const uint32_t t = threadIdx.x;
+    if (t == 0) ctl->queue[next_queue] = 0u;
+    if (t == 1) ctl->shadow = 0u;
+    if (t < kKeys) ctl->bucket[t] = 0u;
 }
 
 // ------------------------------------------------------------ media helpers
@@ -667,9 +767,12 @@ __device__ inline bool matte_direct(const SceneView& sv, const DevMaterial& m, V
 }
 
 template <int MODE>
+// pixel: the film pixel (the batch's, for batches); tex_base: the scene's
+// first texel in the batch's texel-gradient pool.
 __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32_t flags, V3 beta,
-                      Rng& g, uint32_t pixel, uint32_t sample, float pend, int32_t prim, float t,
-                      float bu, float bv, float* film, const AdjCtx& adj, Grad7& G, ShadeOut& out) {
+                      Rng& g, uint32_t pixel, uint32_t sample, uint32_t tex_base, float pend,
+                      int32_t prim, float t, float bu, float bv, float* film, const AdjCtx& adj,
+                      Grad7& G, ShadeOut& out) {
     const bool le = (flags & 1u) != 0;
     const bool in_obj = (flags & 2u) != 0;
     const uint32_t depth = flags >> 8;
@@ -778,7 +881,7 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
         int index = -1;
         const V3 tx = pattern(sv, s, at, bu, bv, index);
         if (MODE == kReplay && adj.tex_grad && index >= 0) {
-            float* tg = adj.tex_grad + 3 * (size_t)index;
+            float* tg = adj.tex_grad + 3 * ((size_t)tex_base + (size_t)index);
             if (tx.x > 0.0f && w3.x != 0.0f) atomicAdd(tg + 0, w3.x / tx.x);
             if (tx.y > 0.0f && w3.y != 0.0f) atomicAdd(tg + 1, w3.y / tx.y);
             if (tx.z > 0.0f && w3.z != 0.0f) atomicAdd(tg + 2, w3.z / tx.z);
@@ -956,104 +1059,150 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
     push_next(out, p, w, beta, mk_flags(le2, in2, depth + 1), 0.0f);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const uint32_t* lists,
-                                                  uint32_t cap, int4 off_lo, int4 off_hi,
-                                                  uint32_t total, const int2* hits,
-                                                  const float2* uvs, SceneView sv, Params prm,
-                                                  float* film, ShadowBuf sh, uint32_t* counters,
-                                                  AdjCtx adj) {
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    ShadeOut so;
-    so.alive = false;
-    so.shadow = false;
-    Grad7 G;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) G.v[k] = 0.0f;
-    if (j < total) {
-        const int offs[kKeys + 1] = {0, off_lo.x, off_lo.y, off_lo.z, off_lo.w,
-                                     off_hi.x, off_hi.y, off_hi.z, off_hi.w};
-        int b = 0;
-        while (b + 1 < kKeys && (int)j >= offs[b + 1]) ++b;
-        const uint32_t i = lists[(size_t)b * cap + (j - (uint32_t)offs[b])];
-        const float4 r0 = in.ray[2 * (size_t)i + 0];
-        const float4 r1 = in.ray[2 * (size_t)i + 1];
-        const float4 bt = in.beta[i];
-        const uint32_t pixel = in.pixel[i];
-        const uint32_t sample = in.sample[i];
-        Rng g;
-        g.state = in.rng[i];
-        g.inc = ((uint64_t)pixel << 1u) | 1u;
-        const int2 h = hits[i];
-        const float2 uv = uvs[i];
-        shade<MODE>(sv, prm, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), in.flags[i],
-                    v3(bt.x, bt.y, bt.z), g, pixel, sample, MODE == kReplay ? in.pend[i] : 0.0f,
-                    h.x, __int_as_float(h.y), uv.x, uv.y, film, adj, G, so);
-        so.rng = g.state;
-        so.pixel = pixel;
-        so.sample = sample;
-    }
-    if (MODE == kReplay) flush_grad(G, adj.gstripes);
-    const uint32_t ns = wave_append(&counters[0], so.alive);
-    if (so.alive) {
-        out.ray[2 * (size_t)ns + 0] = so.r0;
-        out.ray[2 * (size_t)ns + 1] = so.r1;
-        out.beta[ns] = make_float4(so.beta.x, so.beta.y, so.beta.z, 0.0f);
-        out.rng[ns] = so.rng;
-        out.pixel[ns] = so.pixel;
-        out.sample[ns] = so.sample;
-        out.flags[ns] = so.flags;
-        if (MODE == kReplay) out.pend[ns] = so.pend;
-    }
-    const uint32_t ss = wave_append(&counters[1], so.shadow);
-    if (so.shadow) {
-        sh.ray[2 * (size_t)ss + 0] = so.s0;
-        sh.ray[2 * (size_t)ss + 1] = so.s1;
-        sh.contrib[ss] = so.contrib;
-        if (MODE != kRender) sh.sample[ss] = so.sample;
-        if (MODE == kReplay) sh.score[ss] = so.score;
+// Per-scene completion of a batch: samples still running per scene, and the
+// device clock when a scene's last path ended.
+struct Track {
+    uint32_t* left;
+    uint64_t* done;
+};
+
+// Counts this wave's path terminations against their scenes: one atomic per
+// distinct scene in the wave (a wave holds consecutive samples, so usually
+// one); the lane that retires a scene's last sample stamps the clock.
+__device__ inline void track_end(const Track& tr, bool ended, uint32_t scene) {
+    uint64_t todo = __ballot(ended);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t sc = __shfl(scene, leader);
+        const uint64_t same = __ballot(ended && scene == sc) & todo;
+        if ((int)lane_id() == leader) {
+            const uint32_t cnt = (uint32_t)__popcll(same);
+            if (atomicSub(&tr.left[sc], cnt) == cnt) tr.done[sc] = wall_clock64();
+        }
+        todo &= ~same;
     }
 }
 
-template <bool SPILL, int MODE>
-__global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, uint32_t n, SceneView sv,
-                                                   float* film, AdjCtx adj) {
+// Shading of every queued path, bucket by bucket (paths of one material kind
+// are adjacent, so a wave mostly runs one BSDF).  Appends continuing paths to
+// the other path buffer and shadow rays to the shadow queue.
+template <int MODE, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const uint32_t* lists,
+                                                  uint32_t cap, const Ctl* ctl, const int2* hits,
+                                                  const float2* uvs, SceneView sv,
+                                                  const SceneTab* tab, Params prm, float* film,
+                                                  ShadowBuf sh, uint32_t* next_count,
+                                                  uint32_t* shadow_count, AdjCtx adj, Track tr) {
+    const uint32_t total = (uint32_t)ctl->offset[kKeys];
+    int offs[kKeys + 1];
+#pragma unroll
+    for (int k = 0; k <= kKeys; ++k) offs[k] = ctl->offset[k];
+    RUMI_WAVE_LOOP(total) {
+        const uint32_t j = base_ + lane_id();
+        ShadeOut so;
+        so.alive = false;
+        so.shadow = false;
+        Grad7 G;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) G.v[k] = 0.0f;
+        uint32_t scene = 0;
+        if (j < total) {
+            int b = 0;
+            while (b + 1 < kKeys && (int)j >= offs[b + 1]) ++b;
+            const uint32_t i = lists[(size_t)b * cap + (j - (uint32_t)offs[b])];
+            if (MULTI) scene = in.scene[i];
+            const float4 r0 = in.ray[2 * (size_t)i + 0];
+            const float4 r1 = in.ray[2 * (size_t)i + 1];
+            const float4 bt = in.beta[i];
+            const uint32_t pixel = in.pixel[i];
+            const uint32_t sample = in.sample[i];
+            Rng g;
+            g.state = in.rng[i];
+            g.inc = ((uint64_t)pixel << 1u) | 1u;
+            const int2 h = hits[i];
+            const float2 uv = uvs[i];
+            const uint32_t fpix = MULTI ? tab[scene].pixel_base + pixel : pixel;
+            const uint32_t tbase = MULTI ? tab[scene].texel_base : 0u;
+            shade<MODE>(view_of<MULTI>(sv, tab, scene), prm, v3(r0.x, r0.y, r0.z),
+                        v3(r1.x, r1.y, r1.z), in.flags[i], v3(bt.x, bt.y, bt.z), g, fpix, sample,
+                        tbase, MODE == kReplay ? in.pend[i] : 0.0f, h.x, __int_as_float(h.y),
+                        uv.x, uv.y, film, adj, G, so);
+            so.rng = g.state;
+            so.pixel = pixel;
+            so.sample = sample;
+        }
+        if (MODE == kReplay) flush_grad(G, adj.gstripes);
+        if (MULTI && tr.left) track_end(tr, j < total && !so.alive, scene);
+        const uint32_t ns = wave_append(next_count, so.alive);
+        if (so.alive) {
+            out.ray[2 * (size_t)ns + 0] = so.r0;
+            out.ray[2 * (size_t)ns + 1] = so.r1;
+            out.beta[ns] = make_float4(so.beta.x, so.beta.y, so.beta.z, 0.0f);
+            out.rng[ns] = so.rng;
+            out.pixel[ns] = so.pixel;
+            out.sample[ns] = so.sample;
+            out.flags[ns] = so.flags;
+            if (MODE == kReplay) out.pend[ns] = so.pend;
+            if (MULTI) out.scene[ns] = scene;
+        }
+        const uint32_t ss = wave_append(shadow_count, so.shadow);
+        if (so.shadow) {
+            sh.ray[2 * (size_t)ss + 0] = so.s0;
+            sh.ray[2 * (size_t)ss + 1] = so.s1;
+            sh.contrib[ss] = so.contrib;
+            if (MODE != kRender) sh.sample[ss] = so.sample;
+            if (MODE == kReplay) sh.score[ss] = so.score;
+            if (MULTI) sh.scene[ss] = scene;
+        }
+    }
+}
+
+// Any-hit test of every shadow ray (Scene::VisibilityTest, scene.cpp:91-123);
+// unoccluded contributions go to the film (the adjoint passes: to the
+// per-sample sums and the connection's own gradient terms).
+template <bool SPILL, int MODE, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t* n_ptr,
+                                                   SceneView sv, const SceneTab* tab, float* film,
+                                                   AdjCtx adj) {
     __shared__ int32_t s_ref[kStack * kBlock];
     __shared__ float s_tn[kStack * kBlock];
     const uint32_t tid = threadIdx.x;
-    const uint32_t i = blockIdx.x * kBlock + tid;
-    Grad7 G;
+    const uint32_t n = *n_ptr;
+    RUMI_WAVE_LOOP(n) {
+        const uint32_t i = base_ + lane_id();
+        Grad7 G;
 #pragma unroll
-    for (int k = 0; k < 7; ++k) G.v[k] = 0.0f;
-    if (i < n) {
-        const float4 r0 = sh.ray[2 * (size_t)i + 0];
-        const float4 r1 = sh.ray[2 * (size_t)i + 1];
-        Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, 0.0f};
-        float hu, hv;
-        int32_t nn = 0, np = 0;
-        traverse<kBlock, kStack, true, true, true, false, SPILL>(sv.nodes, sv.prims, sv.rects,
-                                                                 sv.root, r, false, hu, hv, nn, np,
-                                                                 s_ref, s_tn, tid);
-        // Scene::VisibilityTest: in shadow iff the any-hit query found a hit.
-        if (!(r.tmin > 0.0f)) {
-            const float4 c4 = sh.contrib[i];
-            const V3 c = v3(c4.x, c4.y, c4.z);
-            const uint32_t pixel = (uint32_t)__float_as_int(c4.w);
-            if (MODE != kReplay) film_add(film, pixel, c);
-            if (MODE == kPrimal) add3(adj.L_total + 3 * (size_t)sh.sample[i], c);
-            if (MODE == kReplay && !is_nan(c)) {
-                add3(adj.L_sofar + 3 * (size_t)sh.sample[i], c);
-                // factors of this connection alone: the shadow segment's
-                // Bernoulli transmittance and the phase value toward the light
-                const V3 w3 = cmul(ld3(adj.dimg + 3 * (size_t)pixel) * adj.inv_spp, c);
-                const float Ss = w3.x + w3.y + w3.z;
-                const float2 sc = sh.score[i];
-                if (sc.x > 0.0f) grad_grey(G, -sc.x, Ss);
-                G.v[6] += sc.y * Ss;
+        for (int k = 0; k < 7; ++k) G.v[k] = 0.0f;
+        if (i < n) {
+            const SceneView& v = view_of<MULTI>(sv, tab, MULTI ? sh.scene[i] : 0u);
+            const float4 r0 = sh.ray[2 * (size_t)i + 0];
+            const float4 r1 = sh.ray[2 * (size_t)i + 1];
+            Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, 0.0f};
+            float hu, hv;
+            int32_t nn = 0, np = 0;
+            traverse<kBlock, kStack, true, true, true, false, SPILL>(
+                v.nodes, v.prims, v.rects, v.root, r, false, hu, hv, nn, np, s_ref, s_tn, tid);
+            // Scene::VisibilityTest: in shadow iff the any-hit query found a hit.
+            if (!(r.tmin > 0.0f)) {
+                const float4 c4 = sh.contrib[i];
+                const V3 c = v3(c4.x, c4.y, c4.z);
+                const uint32_t pixel = (uint32_t)__float_as_int(c4.w);
+                if (MODE != kReplay) film_add(film, pixel, c);
+                if (MODE == kPrimal) add3(adj.L_total + 3 * (size_t)sh.sample[i], c);
+                if (MODE == kReplay && !is_nan(c)) {
+                    add3(adj.L_sofar + 3 * (size_t)sh.sample[i], c);
+                    // factors of this connection alone: the shadow segment's
+                    // Bernoulli transmittance and the phase value toward the light
+                    const V3 w3 = cmul(ld3(adj.dimg + 3 * (size_t)pixel) * adj.inv_spp, c);
+                    const float Ss = w3.x + w3.y + w3.z;
+                    const float2 sc = sh.score[i];
+                    if (sc.x > 0.0f) grad_grey(G, -sc.x, Ss);
+                    G.v[6] += sc.y * Ss;
+                }
             }
         }
+        if (MODE == kReplay) flush_grad(G, adj.gstripes);
     }
-    if (MODE == kReplay) flush_grad(G, adj.gstripes);
 }
 
 // ============================================================ BDPT
@@ -1689,41 +1838,134 @@ int check_render_args(rumi_ctx* ctx, const rumi_scene* sc, const rumi_render_des
     return RUMI_OK;
 }
 
-template <int MODE>
-void launch_shade(uint32_t grid, hipStream_t stream, const PathBuf& in, const PathBuf& out,
-                  const uint32_t* lists, uint32_t cap, int4 off_lo, int4 off_hi, uint32_t n,
-                  const int2* hits, const float2* uvs, const SceneView& sv, const Params& prm,
-                  float* film, const ShadowBuf& sh, uint32_t* counters, const AdjCtx& adj) {
-    hipLaunchKernelGGL(k_shade<MODE>, dim3(grid), dim3(kBlock), 0, stream, in, out, lists, cap, off_lo,
-                       off_hi, n, hits, uvs, sv, prm, film, sh, counters, adj);
+// ------------------------------------------------------------ wavefront host
+// Device buffers of the wavefront path tracer, carved from one allocation.
+struct Wave {
+    PathBuf pb[2];
+    int2* hits;
+    float2* uvs;
+    uint32_t* lists;
+    ShadowBuf sh;
+    Ctl* ctl;
+    uint32_t cap;
+};
+
+// Bytes per queued path: 2 path sets (ray 32, beta 16, rng 8, pixel, sample,
+// flags, pend, scene 4 each), hit 8, barycentrics 8, 8 bucket lists 4 each,
+// shadow entry (ray 32, contrib 16, sample 4, score 8, scene 4).
+constexpr size_t kBytesPerPath = 2 * (32 + 16 + 8 + 4 * 5) + 8 + 8 + 4 * kKeys + (32 + 16 + 4 + 8 + 4);
+constexpr int kCarved = 24;  // arrays carved below, each aligned to 256 B
+
+size_t wave_bytes(uint32_t cap) { return kBytesPerPath * (size_t)cap + sizeof(Ctl) + 256 * kCarved; }
+
+bool wave_carve(Wave& w, void* mem, size_t bytes, uint32_t cap) {
+    char* p = (char*)mem;
+    auto take = [&](size_t n) {
+        char* q = p;
+        p += (n + 255) & ~(size_t)255;
+        return (void*)q;
+    };
+    for (int k = 0; k < 2; ++k) {
+        w.pb[k].ray = (float4*)take(32 * (size_t)cap);
+        w.pb[k].beta = (float4*)take(16 * (size_t)cap);
+        w.pb[k].rng = (uint64_t*)take(8 * (size_t)cap);
+        w.pb[k].pixel = (uint32_t*)take(4 * (size_t)cap);
+        w.pb[k].sample = (uint32_t*)take(4 * (size_t)cap);
+        w.pb[k].flags = (uint32_t*)take(4 * (size_t)cap);
+        w.pb[k].pend = (float*)take(4 * (size_t)cap);
+        w.pb[k].scene = (uint32_t*)take(4 * (size_t)cap);
+    }
+    w.hits = (int2*)take(8 * (size_t)cap);
+    w.uvs = (float2*)take(8 * (size_t)cap);
+    w.lists = (uint32_t*)take(4 * (size_t)kKeys * cap);
+    w.sh.ray = (float4*)take(32 * (size_t)cap);
+    w.sh.contrib = (float4*)take(16 * (size_t)cap);
+    w.sh.sample = (uint32_t*)take(4 * (size_t)cap);
+    w.sh.score = (float2*)take(8 * (size_t)cap);
+    w.sh.scene = (uint32_t*)take(4 * (size_t)cap);
+    w.ctl = (Ctl*)take(sizeof(Ctl));
+    w.cap = cap;
+    return (size_t)(p - (char*)mem) <= bytes;
 }
 
-template <int MODE>
-void launch_shadow(bool spill, uint32_t grid, hipStream_t stream, const ShadowBuf& sh, uint32_t n,
-                   const SceneView& sv, float* film, const AdjCtx& adj) {
-    if (spill)
-        hipLaunchKernelGGL((k_shadow<true, MODE>), dim3(grid), dim3(kBlock), 0, stream, sh, n, sv, film, adj);
+// What the stages of one pass share.
+struct Pass {
+    SceneView sv;         // single scene
+    const SceneTab* tab;  // batches
+    Params prm;
+    float* film;
+    AdjCtx adj;
+    Track tr;
+    bool spill;
+    int mode;
+    uint32_t grid_cap;    // most blocks a grid-stride stage is given
+};
+
+uint32_t grid_for(uint64_t n, uint32_t cap_blocks) {
+    uint64_t g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    return (uint32_t)(g < cap_blocks ? g : cap_blocks);
+}
+
+template <int MODE, bool MULTI>
+void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t s) {
+    const uint32_t g = grid_for(bound, ps.grid_cap);
+    Ctl* ctl = w.ctl;
+    hipLaunchKernelGGL(k_clear_ctl, dim3(1), dim3(64), 0, s, ctl, cur ^ 1);
+    if (ps.spill)
+        hipLaunchKernelGGL((k_extend<true, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
+                           &ctl->queue[cur], ps.sv, ps.tab, w.hits, w.uvs);
     else
-        hipLaunchKernelGGL((k_shadow<false, MODE>), dim3(grid), dim3(kBlock), 0, stream, sh, n, sv, film, adj);
+        hipLaunchKernelGGL((k_extend<false, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
+                           &ctl->queue[cur], ps.sv, ps.tab, w.hits, w.uvs);
+    hipLaunchKernelGGL((k_bucket<MULTI>), dim3(g), dim3(kBlock), 0, s, &ctl->queue[cur], w.hits, ps.sv,
+                       ps.tab, w.pb[cur].scene, w.lists, ctl->bucket, w.cap);
+    hipLaunchKernelGGL(k_offsets, dim3(1), dim3(64), 0, s, ctl);
+    hipLaunchKernelGGL((k_shade<MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur], w.pb[cur ^ 1],
+                       w.lists, w.cap, ctl, w.hits, w.uvs, ps.sv, ps.tab, ps.prm, ps.film, w.sh,
+                       &ctl->queue[cur ^ 1], &ctl->shadow, ps.adj, ps.tr);
+    if (ps.spill)
+        hipLaunchKernelGGL((k_shadow<true, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh, &ctl->shadow,
+                           ps.sv, ps.tab, ps.film, ps.adj);
+    else
+        hipLaunchKernelGGL((k_shadow<false, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh,
+                           &ctl->shadow, ps.sv, ps.tab, ps.film, ps.adj);
+}
+
+// One wavefront iteration over queue `cur` (at most `bound` paths).
+template <bool MULTI>
+void iterate(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t s) {
+    if (ps.mode == kRender)
+        stages<kRender, MULTI>(w, cur, bound, ps, s);
+    else if (ps.mode == kPrimal)
+        stages<kPrimal, MULTI>(w, cur, bound, ps, s);
+    else
+        stages<kReplay, MULTI>(w, cur, bound, ps, s);
+}
+
+template <bool MULTI>
+void regen(const Wave& w, int cur, uint32_t start, uint32_t count, uint64_t sample_base,
+           const Pass& ps, hipStream_t s) {
+    hipLaunchKernelGGL((k_regen<MULTI>), dim3(grid_for(count, ps.grid_cap)), dim3(kBlock), 0, s,
+                       w.pb[cur], start, count, sample_base, ps.prm, ps.sv.cam, ps.tab,
+                       &w.ctl->queue[cur]);
 }
 
 // The wavefront path tracer: regeneration, extension, material buckets,
 // shading, shadow rays, until every sample of the frame has terminated.
 // MODE selects plain rendering, the adjoint's primal pass or its replay.
+// The stages read their sizes from the device control block; the host reads
+// one word per iteration to decide on regeneration and the end.
 int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
               rumi_render_stats* stats, hipStream_t stream, int mode, const AdjCtx& adj) {
-    (void)ctx;
     const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
     const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
     uint32_t cap = desc->queue_capacity ? desc->queue_capacity : (1u << 21);
     if ((uint64_t)cap > total) cap = (uint32_t)total;
     if (cap < 64) cap = 64;
-    uint32_t regen = desc->regen_threshold ? desc->regen_threshold : cap;
+    uint32_t regen_at = desc->regen_threshold ? desc->regen_threshold : cap;
 
-    // Work buffers: 2 path sets, hits, bucket lists, shadow queue, counters.
-    const size_t per_path = 2 * (32 + 16 + 8 + 4 + 4 + 4 + 4) + 8 + 8 + 4 * kKeys + 32 + 16 + 4 + 8;
-    // + 256 B of alignment slack for each of the (fewer than 32) carved arrays
-    const size_t need = per_path * (size_t)cap + 256 * sizeof(uint32_t) + 32 * 256;
+    const size_t need = wave_bytes(cap);
     if (sc->work_bytes < need) {
         if (sc->work) (void)hipFree(sc->work);
         sc->work = nullptr;
@@ -1732,42 +1974,25 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
             return fail(RUMI_ERR_NO_MEMORY, "rumi_render: hipMalloc of %zu work bytes failed", need);
         sc->work_bytes = need;
     }
-    char* w = (char*)sc->work;
-    auto take = [&](size_t bytes) {
-        char* p = w;
-        w += (bytes + 255) & ~(size_t)255;
-        return (void*)p;
-    };
-    PathBuf pb[2];
-    for (int k = 0; k < 2; ++k) {
-        pb[k].ray = (float4*)take(32 * (size_t)cap);
-        pb[k].beta = (float4*)take(16 * (size_t)cap);
-        pb[k].rng = (uint64_t*)take(8 * (size_t)cap);
-        pb[k].pixel = (uint32_t*)take(4 * (size_t)cap);
-        pb[k].sample = (uint32_t*)take(4 * (size_t)cap);
-        pb[k].flags = (uint32_t*)take(4 * (size_t)cap);
-        pb[k].pend = (float*)take(4 * (size_t)cap);
-    }
-    int2* hits = (int2*)take(8 * (size_t)cap);
-    float2* uvs = (float2*)take(8 * (size_t)cap);
-    uint32_t* lists = (uint32_t*)take(4 * (size_t)kKeys * cap);
-    ShadowBuf sh;
-    sh.ray = (float4*)take(32 * (size_t)cap);
-    sh.contrib = (float4*)take(16 * (size_t)cap);
-    sh.sample = (uint32_t*)take(4 * (size_t)cap);
-    sh.score = (float2*)take(8 * (size_t)cap);
-    uint32_t* counters = (uint32_t*)take(256 * sizeof(uint32_t));  // [0] next [1] shadow [8..16) buckets
-    if ((size_t)(w - (char*)sc->work) > sc->work_bytes)
+    Wave w;
+    if (!wave_carve(w, sc->work, sc->work_bytes, cap))
         return fail(RUMI_ERR_INVALID, "rumi_render: internal work-buffer layout overflow");
 
-    SceneView sv = scene_view(sc);
-    Params prm;
-    prm.width = desc->width;
-    prm.height = desc->height;
-    prm.max_depth = desc->max_depth;
-    prm.pixels = pixels;
-    prm.seed = desc->seed;
-    const bool spill = sc->max_depth > kStack;
+    Pass ps{};
+    ps.sv = scene_view(sc);
+    ps.tab = nullptr;
+    ps.prm.width = desc->width;
+    ps.prm.height = desc->height;
+    ps.prm.max_depth = desc->max_depth;
+    ps.prm.pixels = pixels;
+    ps.prm.seed = desc->seed;
+    ps.prm.n_scenes = 1;
+    ps.film = film;
+    ps.adj = adj;
+    ps.tr = Track{nullptr, nullptr};
+    ps.spill = sc->max_depth > kStack;
+    ps.mode = mode;
+    ps.grid_cap = (uint32_t)ctx->cu_count * 16u;
 
     hipEvent_t ev0, ev1;
     (void)hipEventCreate(&ev0);
@@ -1780,68 +2005,28 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     uint64_t issued = 0, ext_rays = 0, shadow_rays = 0, iters = 0;
     uint32_t n_cur = 0;
     int cur = 0;
-    uint32_t host_cnt[16];
+    uint32_t host_ctl[3];
     int rc = RUMI_OK;
     while (true) {
-        if (issued < total && n_cur < cap && n_cur <= regen) {
+        if (issued < total && n_cur < cap && n_cur <= regen_at) {
             const uint32_t cnt = (uint32_t)std::min<uint64_t>(cap - n_cur, total - issued);
-            hipLaunchKernelGGL(k_regen, dim3(grid_of(cnt)), dim3(kBlock), 0, stream, pb[cur], n_cur,
-                               cnt, issued, prm, sc->camera);
+            regen<false>(w, cur, n_cur, cnt, issued, ps, stream);
             n_cur += cnt;
             issued += cnt;
         }
         if (n_cur == 0) break;
         ++iters;
         ext_rays += n_cur;
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, counters, 16u);
-        if (spill)
-            hipLaunchKernelGGL(k_extend<true>, dim3(grid_of(n_cur)), dim3(kBlock), 0, stream, pb[cur],
-                               n_cur, sv, hits, uvs);
-        else
-            hipLaunchKernelGGL(k_extend<false>, dim3(grid_of(n_cur)), dim3(kBlock), 0, stream,
-                               pb[cur], n_cur, sv, hits, uvs);
-        hipLaunchKernelGGL(k_bucket, dim3(grid_of(n_cur)), dim3(kBlock), 0, stream, n_cur, hits, sv,
-                           lists, counters + 8, cap);
-        if (hipMemcpyAsync(host_cnt + 8, counters + 8, kKeys * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           stream) != hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess) {
-            rc = fail(RUMI_ERR_HIP, "rumi_render: bucket count read failed: %s",
-                      hipGetErrorString(hipGetLastError()));
-            break;
-        }
-        int offs[kKeys + 1];
-        offs[0] = 0;
-        for (int k = 0; k < kKeys; ++k) offs[k + 1] = offs[k] + (int)host_cnt[8 + k];
-        const int4 off_lo = make_int4(offs[1], offs[2], offs[3], offs[4]);
-        const int4 off_hi = make_int4(offs[5], offs[6], offs[7], offs[8]);
-        if (mode == kRender)
-            launch_shade<kRender>(grid_of(n_cur), stream, pb[cur], pb[cur ^ 1], lists, cap, off_lo, off_hi,
-                                  n_cur, hits, uvs, sv, prm, film, sh, counters, adj);
-        else if (mode == kPrimal)
-            launch_shade<kPrimal>(grid_of(n_cur), stream, pb[cur], pb[cur ^ 1], lists, cap, off_lo, off_hi,
-                                  n_cur, hits, uvs, sv, prm, film, sh, counters, adj);
-        else
-            launch_shade<kReplay>(grid_of(n_cur), stream, pb[cur], pb[cur ^ 1], lists, cap, off_lo, off_hi,
-                                  n_cur, hits, uvs, sv, prm, film, sh, counters, adj);
-        if (hipMemcpyAsync(host_cnt, counters, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream) !=
-                hipSuccess ||
+        iterate<false>(w, cur, n_cur, ps, stream);
+        if (hipMemcpyAsync(host_ctl, w.ctl, sizeof(host_ctl), hipMemcpyDeviceToHost, stream) != hipSuccess ||
             hipStreamSynchronize(stream) != hipSuccess) {
             rc = fail(RUMI_ERR_HIP, "rumi_render: counter read failed: %s",
                       hipGetErrorString(hipGetLastError()));
             break;
         }
-        const uint32_t n_next = host_cnt[0], n_shadow = host_cnt[1];
-        if (n_shadow) {
-            shadow_rays += n_shadow;
-            if (mode == kRender)
-                launch_shadow<kRender>(spill, grid_of(n_shadow), stream, sh, n_shadow, sv, film, adj);
-            else if (mode == kPrimal)
-                launch_shadow<kPrimal>(spill, grid_of(n_shadow), stream, sh, n_shadow, sv, film, adj);
-            else
-                launch_shadow<kReplay>(spill, grid_of(n_shadow), stream, sh, n_shadow, sv, film, adj);
-        }
+        shadow_rays += host_ctl[2];
         cur ^= 1;
-        n_cur = n_next;
+        n_cur = host_ctl[cur];
     }
     if (rc == RUMI_OK) {
         if (mode != kReplay)
@@ -1875,6 +2060,160 @@ __global__ void k_reduce_stripes(const float* stripes, float* out) {
         for (int j = 0; j < kStripes; ++j) a += stripes[8 * j + k];
         out[k] = a;
     }
+}
+
+// ============================================================ batches
+// Many scenes in one wavefront: samples are numbered scene after scene, every
+// path carries its scene, and the stages index a device table of the scenes'
+// BVHs, materials, lights, textures and cameras.  The schedule is fixed when
+// the batch is built -- generations of at most `cap` samples, each drained by
+// exactly max_depth iterations (every path ends by then) -- so one run is a
+// straight sequence of launches with no host round trip, which is captured
+// once as a hipGraph and replayed.
+
+__global__ void k_batch_begin(uint32_t* left, const uint32_t* samples, uint64_t* done, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        left[i] = samples[i];
+        done[i] = 0ull;
+    }
+    if (i == 0) done[n] = wall_clock64();
+}
+
+// dLoss/dImage of the L2 loss sum (I - T)^2, and the loss per item: grid
+// (blocks, items), each block striding over its item's pixels, one atomic per
+// wave.
+__global__ void k_l2_grad(const float* img, const float* target, float* dimg,
+                          const uint32_t* pixel_base, uint32_t n_items, uint64_t total_pixels,
+                          float* loss) {
+    const uint32_t item = blockIdx.y;
+    const size_t lo = 3 * (size_t)pixel_base[item];
+    const size_t hi = 3 * (item + 1 < n_items ? (size_t)pixel_base[item + 1] : (size_t)total_pixels);
+    float acc = 0.0f;
+    for (size_t i = lo + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < hi;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const float d = img[i] - target[i];
+        dimg[i] = 2.0f * d;
+        acc += d * d;
+    }
+    if (loss) {
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if ((threadIdx.x & 63u) == 0 && acc != 0.0f) atomicAdd(loss + item, acc);
+    }
+}
+
+}  // namespace
+
+struct rumi_batch {
+    int device = -1;
+    int cu_count = 0;
+    uint32_t n = 0;
+    rumi_batch_desc desc{};
+    uint32_t cap = 0, generations = 0;
+    uint64_t total_samples = 0, total_pixels = 0, total_texels = 0;
+    bool spill = false;
+    std::vector<uint32_t> item_samples, item_pixels;
+    // device
+    void* mem = nullptr;
+    size_t mem_bytes = 0;
+    SceneTab* tab = nullptr;
+    uint32_t* samples = nullptr;     // per item
+    uint32_t* pixel_base = nullptr;  // per item
+    uint32_t* left = nullptr;        // per item
+    uint64_t* done = nullptr;        // per item + start stamp
+    float* scratch_film = nullptr;   // the adjoint's primal image
+    float* dimg = nullptr;
+    float* L_total = nullptr;
+    float* L_sofar = nullptr;
+    float* stripes = nullptr;
+    Wave w{};
+    // captured graph and the io it was captured with
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    rumi_batch_io graph_io{};
+    double clock_khz = 0.0;
+};
+
+namespace {
+
+// Every launch of one batch run, in stream order (eagerly or under capture).
+void batch_enqueue(rumi_batch* b, const rumi_batch_io& io, hipStream_t s) {
+    const uint32_t n = b->n;
+    const size_t film_n = 3 * (size_t)b->total_pixels;
+    auto zero = [&](float* p, size_t cnt) {
+        if (!p || cnt == 0) return;
+        const size_t blocks = (cnt + 255) / 256;
+        const size_t cap_blocks = (size_t)b->cu_count * 64;
+        hipLaunchKernelGGL(k_zero, dim3((uint32_t)(blocks < cap_blocks ? blocks : cap_blocks)), dim3(256), 0,
+                           s, p, cnt);
+    };
+    hipLaunchKernelGGL(k_batch_begin, dim3((n + 255) / 256), dim3(256), 0, s, b->left, b->samples, b->done, n);
+    zero(io.dev_images, film_n);
+
+    Pass ps{};
+    ps.sv = SceneView{};
+    ps.tab = b->tab;
+    ps.prm.width = 0;
+    ps.prm.height = 0;
+    ps.prm.max_depth = b->desc.max_depth;
+    ps.prm.pixels = 0;
+    ps.prm.n_scenes = n;
+    ps.spill = b->spill;
+    ps.grid_cap = (uint32_t)b->cu_count * 16u;
+    ps.adj = AdjCtx{};
+    ps.adj.gstripes = b->stripes;
+    ps.adj.inv_spp = 1.0f / (float)b->desc.spp;
+
+    auto pass = [&](int mode, uint64_t seed, float* film, Track tr) {
+        ps.mode = mode;
+        ps.prm.seed = seed;
+        ps.film = film;
+        ps.tr = tr;
+        for (uint32_t g = 0; g < b->generations; ++g) {
+            const uint64_t first = (uint64_t)g * b->cap;
+            const uint32_t cnt = (uint32_t)std::min<uint64_t>(b->cap, b->total_samples - first);
+            regen<true>(b->w, 0, 0, cnt, first, ps, s);
+            int cur = 0;
+            for (int d = 0; d < b->desc.max_depth; ++d) {
+                iterate<true>(b->w, cur, cnt, ps, s);
+                cur ^= 1;
+            }
+        }
+    };
+    pass(kRender, b->desc.seed, io.dev_images, Track{b->left, b->done});
+    hipLaunchKernelGGL(k_scale, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, s, io.dev_images,
+                       film_n, 1.0f / (float)b->desc.spp);
+    if (b->desc.adjoint) {
+        if (io.dev_loss) zero(io.dev_loss, n);
+        hipLaunchKernelGGL(k_l2_grad, dim3(64, n), dim3(256), 0, s, io.dev_images, io.dev_targets,
+                           b->dimg, b->pixel_base, n, b->total_pixels, io.dev_loss);
+        zero(b->L_total, 3 * (size_t)b->total_samples);
+        zero(b->L_sofar, 3 * (size_t)b->total_samples);
+        zero(b->stripes, 8 * kStripes);
+        zero(io.dev_texel_grads, 3 * (size_t)b->total_texels);
+        ps.adj.L_total = b->L_total;
+        ps.adj.L_sofar = b->L_sofar;
+        ps.adj.dimg = b->dimg;
+        ps.adj.tex_grad = io.dev_texel_grads;
+        zero(b->scratch_film, film_n);
+        pass(kPrimal, b->desc.seed + 1, b->scratch_film, Track{nullptr, nullptr});
+        pass(kReplay, b->desc.seed + 1, nullptr, Track{nullptr, nullptr});
+    }
+}
+
+void batch_free(rumi_batch* b) {
+    if (b->device >= 0) {
+        int prev = 0;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(b->device);
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->mem) (void)hipFree(b->mem);
+        (void)hipSetDevice(prev);
+    }
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->mem = nullptr;
 }
 
 }  // namespace
@@ -1962,4 +2301,194 @@ extern "C" int rumi_adjoint(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi
         stats->milliseconds = st0.milliseconds + st1.milliseconds;
     }
     return rc;
+}
+
+extern "C" int rumi_batch_create(rumi_ctx* ctx, const rumi_batch_item* items, uint32_t n,
+                                 const rumi_batch_desc* desc, rumi_batch** out) {
+    clear_error();
+    if (!out) return fail(RUMI_ERR_INVALID, "rumi_batch_create: null output");
+    *out = nullptr;
+    if (!ctx) return fail(RUMI_ERR_NO_DEVICE, "rumi_batch_create: null context");
+    if (!items || n == 0 || !desc) return fail(RUMI_ERR_INVALID, "rumi_batch_create: no items");
+    if (desc->spp <= 0 || desc->max_depth <= 0 || desc->max_depth > 1000)
+        return fail(RUMI_ERR_INVALID, "rumi_batch_create: spp > 0 and max_depth in [1, 1000] required");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(RUMI_ERR_HIP, "rumi_batch_create: hipSetDevice failed");
+    rumi_batch* b = new (std::nothrow) rumi_batch;
+    if (!b) return fail(RUMI_ERR_NO_MEMORY, "rumi_batch_create: out of host memory");
+    b->n = n;
+    b->desc = *desc;
+    b->cu_count = ctx->cu_count;
+    std::vector<SceneTab> tab(n);
+    std::vector<uint32_t> pbase(n);
+    uint64_t samples = 0, pixels = 0, texels = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const rumi_batch_item& it = items[i];
+        const rumi_scene* sc = it.scene;
+        if (!sc || sc->device != ctx->device) {
+            delete b;
+            return fail(RUMI_ERR_INVALID, "rumi_batch_create: item %u has no scene on the context's device", i);
+        }
+        if (it.width <= 0 || it.height <= 0) {
+            delete b;
+            return fail(RUMI_ERR_INVALID, "rumi_batch_create: item %u has an empty image", i);
+        }
+        SceneTab& t = tab[i];
+        t.sv = scene_view(sc);
+        t.width = it.width;
+        t.height = it.height;
+        t.spp = desc->spp;
+        t.pixels = (uint32_t)it.width * (uint32_t)it.height;
+        t.sample_base = (uint32_t)samples;
+        t.pixel_base = (uint32_t)pixels;
+        t.texel_base = (uint32_t)texels;
+        t.pad = 0;
+        pbase[i] = (uint32_t)pixels;
+        b->item_samples.push_back(t.pixels * (uint32_t)desc->spp);
+        b->item_pixels.push_back(t.pixels);
+        samples += (uint64_t)t.pixels * (uint64_t)desc->spp;
+        pixels += t.pixels;
+        texels += sc->texel_count;
+        if (sc->max_depth > kStack) b->spill = true;
+        if (samples > 0xffffffffull || pixels > 0xffffffffull / 4) {
+            delete b;
+            return fail(RUMI_ERR_INVALID, "rumi_batch_create: more than 2^32 - 1 samples in the batch");
+        }
+    }
+    b->total_samples = samples;
+    b->total_pixels = pixels;
+    b->total_texels = texels;
+    uint32_t cap = desc->queue_capacity ? desc->queue_capacity : (1u << 22);
+    if ((uint64_t)cap > samples) cap = (uint32_t)samples;
+    if (cap < 64) cap = 64;
+    b->cap = cap;
+    b->generations = (uint32_t)((samples + cap - 1) / cap);
+
+    // one allocation: table, per-item arrays, wave buffers, adjoint buffers
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t wbytes = wave_bytes(cap);
+    size_t bytes = al(sizeof(SceneTab) * n) + 3 * al(4 * (size_t)n) + al(8 * ((size_t)n + 1)) + al(wbytes) +
+                   al(4 * 8 * kStripes);
+    if (desc->adjoint)
+        bytes += 2 * al(12 * (size_t)pixels) + 2 * al(12 * (size_t)samples);
+    if (hipMalloc(&b->mem, bytes) != hipSuccess) {
+        delete b;
+        return fail(RUMI_ERR_NO_MEMORY, "rumi_batch_create: hipMalloc of %zu bytes failed", bytes);
+    }
+    b->device = ctx->device;
+    b->mem_bytes = bytes;
+    char* p = (char*)b->mem;
+    auto take = [&](size_t x) {
+        char* q = p;
+        p += al(x);
+        return (void*)q;
+    };
+    b->tab = (SceneTab*)take(sizeof(SceneTab) * n);
+    b->samples = (uint32_t*)take(4 * (size_t)n);
+    b->pixel_base = (uint32_t*)take(4 * (size_t)n);
+    b->left = (uint32_t*)take(4 * (size_t)n);
+    b->done = (uint64_t*)take(8 * ((size_t)n + 1));
+    void* wmem = take(wbytes);
+    b->stripes = (float*)take(4 * 8 * kStripes);
+    if (desc->adjoint) {
+        b->scratch_film = (float*)take(12 * (size_t)pixels);
+        b->dimg = (float*)take(12 * (size_t)pixels);
+        b->L_total = (float*)take(12 * (size_t)samples);
+        b->L_sofar = (float*)take(12 * (size_t)samples);
+    }
+    if ((size_t)(p - (char*)b->mem) > bytes || !wave_carve(b->w, wmem, wbytes, cap)) {
+        batch_free(b);
+        delete b;
+        return fail(RUMI_ERR_INVALID, "rumi_batch_create: internal layout overflow");
+    }
+    if (hipMemcpy(b->tab, tab.data(), sizeof(SceneTab) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->samples, b->item_samples.data(), 4 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->pixel_base, pbase.data(), 4 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return fail(RUMI_ERR_HIP, "rumi_batch_create: table upload failed");
+    }
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || khz <= 0)
+        khz = 100000;
+    b->clock_khz = (double)khz;
+    *out = b;
+    return RUMI_OK;
+}
+
+extern "C" int rumi_batch_run(rumi_batch* b, const rumi_batch_io* io, int32_t use_graph,
+                              rumi_batch_stats* stats, void* stream_v) {
+    clear_error();
+    if (!b || !io || !io->dev_images) return fail(RUMI_ERR_INVALID, "rumi_batch_run: null argument");
+    if (b->desc.adjoint && (!io->dev_targets || (b->total_texels && !io->dev_texel_grads)))
+        return fail(RUMI_ERR_INVALID, "rumi_batch_run: the adjoint needs targets and a texel-gradient buffer");
+    if (hipSetDevice(b->device) != hipSuccess) return fail(RUMI_ERR_HIP, "rumi_batch_run: hipSetDevice failed");
+    hipStream_t s = (hipStream_t)stream_v;
+    hipEvent_t ev0, ev1;
+    (void)hipEventCreate(&ev0);
+    (void)hipEventCreate(&ev1);
+    (void)hipEventRecord(ev0, s);
+    int rc = RUMI_OK;
+    if (use_graph) {
+        const bool same = b->exec && b->graph_io.dev_images == io->dev_images &&
+                          b->graph_io.dev_targets == io->dev_targets &&
+                          b->graph_io.dev_texel_grads == io->dev_texel_grads &&
+                          b->graph_io.dev_loss == io->dev_loss;
+        if (!same) {
+            if (b->exec) (void)hipGraphExecDestroy(b->exec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+            hipStream_t cs;
+            if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+                rc = fail(RUMI_ERR_HIP, "rumi_batch_run: capture stream creation failed");
+            if (rc == RUMI_OK && hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                rc = fail(RUMI_ERR_HIP, "rumi_batch_run: hipStreamBeginCapture failed");
+            if (rc == RUMI_OK) {
+                batch_enqueue(b, *io, cs);
+                if (hipStreamEndCapture(cs, &b->graph) != hipSuccess || !b->graph)
+                    rc = fail(RUMI_ERR_HIP, "rumi_batch_run: capture failed: %s", hipGetErrorString(hipGetLastError()));
+                else if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+                    rc = fail(RUMI_ERR_HIP, "rumi_batch_run: hipGraphInstantiate failed");
+            }
+            (void)hipStreamDestroy(cs);
+            if (rc == RUMI_OK) b->graph_io = *io;
+        }
+        if (rc == RUMI_OK && hipGraphLaunch(b->exec, s) != hipSuccess)
+            rc = fail(RUMI_ERR_HIP, "rumi_batch_run: hipGraphLaunch failed");
+    } else {
+        batch_enqueue(b, *io, s);
+    }
+    (void)hipEventRecord(ev1, s);
+    if (hipStreamSynchronize(s) != hipSuccess)
+        rc = fail(RUMI_ERR_HIP, "rumi_batch_run: %s", hipGetErrorString(hipGetLastError()));
+    else if (rc == RUMI_OK && hipGetLastError() != hipSuccess)
+        rc = fail(RUMI_ERR_HIP, "rumi_batch_run: a kernel launch failed");
+    float ms = 0.0f;
+    if (rc == RUMI_OK) (void)hipEventElapsedTime(&ms, ev0, ev1);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (stats) {
+        stats->total_samples = b->total_samples;
+        stats->total_pixels = b->total_pixels;
+        stats->total_texels = b->total_texels;
+        stats->n_items = b->n;
+        stats->generations = b->generations;
+        stats->milliseconds = ms;
+        if (rc == RUMI_OK && stats->host_item_ms) {
+            std::vector<uint64_t> done(b->n + 1);
+            if (hipMemcpy(done.data(), b->done, 8 * ((size_t)b->n + 1), hipMemcpyDeviceToHost) != hipSuccess)
+                return fail(RUMI_ERR_HIP, "rumi_batch_run: completion stamps unreadable");
+            for (uint32_t i = 0; i < b->n; ++i)
+                stats->host_item_ms[i] =
+                    done[i] >= done[b->n] ? (double)(done[i] - done[b->n]) / b->clock_khz : -1.0;
+        }
+    }
+    return rc;
+}
+
+extern "C" int rumi_batch_destroy(rumi_batch* b) {
+    if (!b) return RUMI_OK;
+    batch_free(b);
+    delete b;
+    return RUMI_OK;
 }

@@ -106,6 +106,88 @@ __global__ __launch_bounds__(BLOCK) void k_trace_lanes(
     }
 }
 
+// Persistent lanes with a prefetched next ray per lane.  A lane whose ray
+// ends swaps in the ray it has already loaded and, in the same iteration,
+// issues the loads of its next one; those complete while the wave waits on
+// its node fetches anyway, so taking a new ray costs no extra memory round
+// trip.  Ray indices come from the global counter in wave chunks of 64,
+// requested half a chunk ahead.  Each ray's arithmetic is the same as in
+// k_trace, so the results are bit-identical.
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL>
+__global__ __launch_bounds__(BLOCK) void k_trace_prefetch(
+    const float4* __restrict__ rays, uint32_t n, const float4* __restrict__ nodes,
+    const float4* __restrict__ tris, RootInfo root, int2* __restrict__ hits,
+    float2* __restrict__ uv, int32_t* __restrict__ any_out, uint32_t* __restrict__ counter) {
+    constexpr uint32_t kChunk = 64;
+    __shared__ int32_t s_ref[STACK * BLOCK];
+    __shared__ float s_tn[STACK * BLOCK];
+    constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
+    int32_t p_ref[kSpill];
+    float p_tn[kSpill];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    TravState s;
+    uint32_t id = 0, nid = 0;
+    float4 n0 = make_float4(0, 0, 0, 0), n1 = n0;
+    bool active = false, has_next = false, drained = false;
+    // wave-uniform chunk state; the pending chunk's base sits in lane 0
+    uint32_t cbase = 0, cleft = 0, pend_val = 0;
+    bool pend = false;
+    int32_t nn = 0, np = 0;
+    auto finish_ray = [&](uint32_t k) {
+        if (ANY && any_out) any_out[k] = s.hit;
+        if (hits) hits[k] = make_int2(s.hit, __float_as_int(s.r.tmin));
+        if (!ANY && uv) uv[k] = make_float2(s.hu, s.hv);
+    };
+    while (true) {
+        // 1. lanes whose ray ended take their prefetched one
+        if (!active && has_next) {
+            s.r = Ray{n0.x, n0.y, n0.z, n1.x, n1.y, n1.z, n1.w, n0.w};
+            id = nid;
+            has_next = false;
+            active = trav_begin(root, s);
+            if (!active) finish_ray(id);
+        }
+        // 2. ask for the next chunk of indices half a chunk ahead
+        if (!pend && cleft < kChunk / 2) {
+            if (lane == 0) pend_val = atomicAdd(counter, kChunk);
+            pend = true;
+        }
+        // 3. lanes without a prefetched ray take indices and issue its loads
+        const uint64_t want = __ballot(!has_next && !drained);
+        if (want != 0) {
+            if (cleft == 0) {
+                cbase = __shfl(pend_val, 0);
+                cleft = kChunk;
+                pend = false;
+            }
+            const uint32_t need = (uint32_t)__popcll(want);
+            const uint32_t take = need < cleft ? need : cleft;
+            const uint32_t rank = lane_below(want);
+            if (!has_next && !drained && rank < take) {
+                const uint32_t k = cbase + rank;
+                if (k >= n) {
+                    drained = true;
+                } else {
+                    n0 = rays[2 * (size_t)k + 0];
+                    n1 = rays[2 * (size_t)k + 1];
+                    nid = k;
+                    has_next = true;
+                }
+            }
+            cbase += take;
+            cleft -= take;
+        }
+        // 4. one traversal step of every live ray
+        if (active) {
+            active = trav_step<BLOCK, STACK, ANY, ORDERED, false, false, SPILL>(
+                nodes, tris, nullptr, s, false, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+            if (!active) finish_ray(id);
+        }
+        if (__ballot(active || has_next || !drained) == 0) break;
+    }
+}
+
 __global__ void k_zero_counter(uint32_t* c) { *c = 0u; }
 
 struct TraceArgs {
@@ -120,43 +202,75 @@ struct TraceArgs {
     int32_t* cnt_nodes;
     int32_t* cnt_prims;
     uint32_t* counter;  // persistent launches: the work counter slot
-    int grid_cap;       // persistent launches: resident blocks to start
+    int grid_cap;       // persistent launches: compute units
     int refill;         // persistent launches: idle lanes that trigger a refill
+    int block;          // persistent launches: threads per block (64, 128, 256)
+    int stack;          // persistent launches: LDS stack entries per lane (8, 16)
+    bool persistent;
+    int persistent_kind;  // 1 refill on a threshold, 2 prefetched next ray
 };
 
 constexpr int kBlock = 128;
 
+// One ray per lane (the instrumented launches, and tuning.persistent < 0).
 template <int STACK, bool ANY, bool ORDERED, bool COUNT, bool SPILL>
-hipError_t launch(const TraceArgs& a, hipStream_t s) {
+hipError_t launch_rays(const TraceArgs& a, hipStream_t s) {
     const uint32_t grid = (a.n + kBlock - 1) / kBlock;
-    if (COUNT || !a.counter) {
-        hipLaunchKernelGGL((k_trace<kBlock, STACK, ANY, ORDERED, COUNT, SPILL>), dim3(grid),
-                           dim3(kBlock), 0, s, a.rays, a.n, a.nodes, a.tris, a.root, a.hits, a.uv,
-                           a.any_out, a.cnt_nodes, a.cnt_prims);
-        return hipGetLastError();
-    }
-    auto kern = k_trace_lanes<kBlock, STACK, ANY, ORDERED, SPILL>;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    const uint32_t resident = (uint32_t)per_cu * (uint32_t)a.grid_cap;
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, a.counter);
-    hipLaunchKernelGGL(kern, dim3(grid < resident ? grid : resident), dim3(kBlock), 0, s, a.rays,
-                       a.n, a.nodes, a.tris, a.root, a.hits, a.uv, a.any_out, a.counter, a.refill);
+    hipLaunchKernelGGL((k_trace<kBlock, STACK, ANY, ORDERED, COUNT, SPILL>), dim3(grid),
+                       dim3(kBlock), 0, s, a.rays, a.n, a.nodes, a.tris, a.root, a.hits, a.uv,
+                       a.any_out, a.cnt_nodes, a.cnt_prims);
     return hipGetLastError();
+}
+
+// Persistent lanes: as many blocks as stay resident, refilled from a counter.
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL>
+hipError_t launch_lanes(const TraceArgs& a, hipStream_t s) {
+    const uint32_t grid = (a.n + BLOCK - 1) / BLOCK;
+    int per_cu = 0;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, a.counter);
+    if (a.persistent_kind == 2) {
+        auto kern = k_trace_prefetch<BLOCK, STACK, ANY, ORDERED, SPILL>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BLOCK, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        const uint32_t resident = (uint32_t)per_cu * (uint32_t)a.grid_cap;
+        hipLaunchKernelGGL(kern, dim3(grid < resident ? grid : resident), dim3(BLOCK), 0, s, a.rays,
+                           a.n, a.nodes, a.tris, a.root, a.hits, a.uv, a.any_out, a.counter);
+    } else {
+        auto kern = k_trace_lanes<BLOCK, STACK, ANY, ORDERED, SPILL>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BLOCK, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        const uint32_t resident = (uint32_t)per_cu * (uint32_t)a.grid_cap;
+        hipLaunchKernelGGL(kern, dim3(grid < resident ? grid : resident), dim3(BLOCK), 0, s, a.rays,
+                           a.n, a.nodes, a.tris, a.root, a.hits, a.uv, a.any_out, a.counter,
+                           a.refill);
+    }
+    return hipGetLastError();
+}
+
+template <int BLOCK, int STACK, bool ANY, bool ORDERED>
+hipError_t lanes_depth(const TraceArgs& a, int depth, hipStream_t s) {
+    return depth <= STACK ? launch_lanes<BLOCK, STACK, ANY, ORDERED, false>(a, s)
+                          : launch_lanes<BLOCK, STACK, ANY, ORDERED, true>(a, s);
+}
+
+template <int BLOCK, bool ANY, bool ORDERED>
+hipError_t lanes_stack(const TraceArgs& a, int depth, hipStream_t s) {
+    return a.stack == 8 ? lanes_depth<BLOCK, 8, ANY, ORDERED>(a, depth, s)
+                        : lanes_depth<BLOCK, 16, ANY, ORDERED>(a, depth, s);
 }
 
 template <bool ANY, bool ORDERED, bool COUNT>
 hipError_t launch_depth(const TraceArgs& a, int depth, hipStream_t s) {
-    if (depth <= 16) return launch<16, ANY, ORDERED, COUNT, false>(a, s);
-    if (!COUNT && a.counter) {
-        // persistent lanes: a 16-entry LDS stack keeps 5 waves per SIMD
-        // resident; deeper entries (rare) go to the private spill array
-        return launch<16, ANY, ORDERED, COUNT, true>(a, s);
+    if (!COUNT && a.persistent) {
+        if (a.block == 64) return lanes_stack<64, ANY, ORDERED>(a, depth, s);
+        if (a.block == 256) return lanes_stack<256, ANY, ORDERED>(a, depth, s);
+        return lanes_stack<128, ANY, ORDERED>(a, depth, s);
     }
-    if (depth <= 32) return launch<32, ANY, ORDERED, COUNT, false>(a, s);
-    return launch<32, ANY, ORDERED, COUNT, true>(a, s);
+    if (depth <= 16) return launch_rays<16, ANY, ORDERED, COUNT, false>(a, s);
+    if (depth <= 32) return launch_rays<32, ANY, ORDERED, COUNT, false>(a, s);
+    return launch_rays<32, ANY, ORDERED, COUNT, true>(a, s);
 }
 
 template <bool ANY, bool COUNT>
@@ -193,7 +307,11 @@ TraceArgs base_args(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* rays, ui
     TraceArgs a{};
     a.counter = ctx->take_counter();
     a.grid_cap = ctx->cu_count;
-    a.refill = 16;
+    a.refill = ctx->tuning.refill;
+    a.block = ctx->tuning.block;
+    a.stack = ctx->tuning.stack;
+    a.persistent = ctx->tuning.persistent >= 0;
+    a.persistent_kind = ctx->tuning.persistent;
     a.rays = reinterpret_cast<const float4*>(rays);
     a.n = n;
     a.nodes = reinterpret_cast<const float4*>(bvh->dev_nodes);
@@ -261,4 +379,34 @@ extern "C" int rumi_trace_count(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_r
         any ? launch_order<true, true>(a, ordered, bvh->flat.max_depth, (hipStream_t)stream)
             : launch_order<false, true>(a, ordered, bvh->flat.max_depth, (hipStream_t)stream);
     return finish(e, "rumi_trace_count");
+}
+
+extern "C" int rumi_trace_set_tuning(rumi_ctx* ctx, const rumi_trace_tuning* t) {
+    clear_error();
+    if (!ctx) return fail(RUMI_ERR_NO_DEVICE, "rumi_trace_set_tuning: null context");
+    rumi_trace_tuning v = {128, 16, 16, 2};
+    if (t) {
+        if (t->block) v.block = t->block;
+        if (t->stack) v.stack = t->stack;
+        if (t->refill) v.refill = t->refill;
+        if (t->persistent) v.persistent = t->persistent;
+    }
+    if (v.block != 64 && v.block != 128 && v.block != 256)
+        return fail(RUMI_ERR_INVALID, "rumi_trace_set_tuning: block %d not 64, 128 or 256", v.block);
+    if (v.stack != 8 && v.stack != 16)
+        return fail(RUMI_ERR_INVALID, "rumi_trace_set_tuning: stack %d not 8 or 16", v.stack);
+    if (v.refill < 1 || v.refill > 64)
+        return fail(RUMI_ERR_INVALID, "rumi_trace_set_tuning: refill %d not in [1, 64]", v.refill);
+    if (v.persistent != 1 && v.persistent != 2 && v.persistent != -1)
+        return fail(RUMI_ERR_INVALID, "rumi_trace_set_tuning: persistent %d not 1, 2 or -1",
+                    v.persistent);
+    ctx->tuning = v;
+    return RUMI_OK;
+}
+
+extern "C" int rumi_trace_get_tuning(const rumi_ctx* ctx, rumi_trace_tuning* t) {
+    clear_error();
+    if (!ctx || !t) return fail(RUMI_ERR_INVALID, "rumi_trace_get_tuning: null argument");
+    *t = ctx->tuning;
+    return RUMI_OK;
 }

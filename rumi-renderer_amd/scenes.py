@@ -103,3 +103,60 @@ def diffuse_box(width: int = 64, height: int = 64) -> dict:
     sc = cornell_box(width, height)
     sc["materials"][4] = dict(kind="matte", kd=1.0, color=(0.5, 0.5, 0.5))
     return sc
+
+
+def _pcg_uniform(seed: int, stream: int, n: int) -> np.ndarray:
+    """n uniforms in [0, 1) from PCG32(seed, stream): (x >> 8) * 2^-24."""
+    import importlib
+    rumi = importlib.import_module("rumi-renderer_amd")
+    return (rumi.pcg32_sequence(seed, stream, n) >> 8).astype(np.float64) * (1.0 / (1 << 24))
+
+
+def batch_params(n_scenes: int = 256, seed: int = 4):
+    """Per-scene (triangle target T, icosphere frequency, image side) of the
+    batched workload: T log-uniform in [1e3, 2e5], frequency round(sqrt(T/20)),
+    side log-uniform in [128, 512]; uniforms from PCG32(seed, stream = scene)."""
+    out = []
+    for i in range(n_scenes):
+        u = _pcg_uniform(seed, i, 2)
+        t = float(np.exp(np.log(1e3) + u[0] * (np.log(2e5) - np.log(1e3))))
+        freq = max(1, int(round(np.sqrt(t / 20.0))))
+        side = int(round(np.exp(np.log(128.0) + u[1] * np.log(4.0))))
+        out.append((t, freq, side))
+    return out
+
+
+def batch_scene(i: int, freq: int, side: int, seed: int = 4, tex_size: int = 128,
+                tex_scale: float = 1.0) -> dict:
+    """Scene i of the batched workload: a noisy icosphere (PCG seed 1 + i,
+    amplitude 0.05) of 20 freq^2 triangles in Matte material with a
+    tex_size^2 RGB albedo texture (uniform in [0.1, 0.9] from PCG32(seed + i),
+    times tex_scale), mapped by per-vertex spherical uv; a Lambertian floor;
+    a 2 x 2 area light of radiance 6 above; a 40 degree pinhole camera."""
+    xyz, idx = icosphere_mesh(freq, 1 + i, 0.05, center=(0.0, 0.0, 0.0), scale=1.0)
+    p = xyz.astype(np.float64)
+    r = np.linalg.norm(p, axis=1)
+    uv = np.stack([np.arctan2(p[:, 0], p[:, 2]) / (2 * np.pi) + 0.5,
+                   1.0 - np.arccos(np.clip(p[:, 1] / r, -1, 1)) / np.pi], axis=1).astype(np.float32)
+    tex = (0.1 + 0.8 * _pcg_uniform(seed + i, 0, tex_size * tex_size * 3)).astype(np.float32)
+    tex = (tex.reshape(tex_size, tex_size, 3) * np.float32(tex_scale)).astype(np.float32)
+    mats = [dict(kind="matte", kd=1.0, color=(1.0, 1.0, 1.0)),
+            dict(kind="matte", kd=1.0, color=(0.5, 0.5, 0.5)),
+            dict(kind="emissive", kd=1.0, color=(6.0, 6.0, 6.0))]
+    shapes = [dict(kind="mesh", xyz=xyz, idx=idx, uv=uv, material=0, texture=0)]
+    shapes += _quad((-6, -1.2, 6), (6, -1.2, 6), (6, -1.2, -6), (-6, -1.2, -6), 1, (0, 1, 0))
+    shapes.append(dict(kind="rect", p=(-1.0, 3.0, 1.0), l=(2.0, 0, 0), w=(0, 0, -2.0), material=2,
+                       light=True))
+    pos = (0.0, 0.6, 3.6)
+    cam = dict(position=pos, basis=_look_at(pos, (0.0, 0.0, 0.0)), fov_deg=40.0, y_ratio=1.0)
+    return dict(materials=mats, shapes=shapes, camera=cam, medium=None, textures=[tex])
+
+
+def _look_at(position, target):
+    """Row-major camera basis whose third column is the view direction."""
+    d = np.asarray(target, np.float64) - np.asarray(position, np.float64)
+    d /= np.linalg.norm(d)
+    cx = np.cross(np.array([0.0, 1.0, 0.0]), d)
+    cx /= np.linalg.norm(cx)
+    cy = np.cross(d, cx)
+    return np.stack([cx, cy, d], axis=1).tolist()

@@ -70,7 +70,9 @@ def test_primal_image_equals_render(rumi, ctx, scenes):
 def test_texel_gradients_equal_exact_differences(rumi, ctx):
     """No sampling decision depends on a texel, so two renders with one seed
     differ only through it: central differences are exact up to float
-    rounding and the h^2 term."""
+    rounding, the order of the film's float atomics and the h^2 term.  The
+    texels tested are each texture's most influential ones; the tolerance
+    includes the difference two identical renders show."""
     sc = rc.textured_scene(16, 16)
     dimg = np.random.default_rng(2).uniform(0.5, 1.5, (16, 16, 3))
     spp, depth, seed = 64, 5, 31
@@ -78,8 +80,18 @@ def test_texel_gradients_equal_exact_differences(rumi, ctx):
     sizes = [t.shape[0] * t.shape[1] for t in sc["textures"]]
     offs = np.cumsum([0] + sizes)
     assert gt.shape == (offs[-1], 3)
-    for t, y, x, c in [(0, 1, 2, 0), (0, 3, 0, 1), (1, 4, 1, 2), (1, 0, 2, 0), (2, 1, 1, 1)]:
-        idx = offs[t] + y * sc["textures"][t].shape[1] + x
+    base = render(rumi, ctx, sc, 16, 16, spp, depth, seed)
+    again = render(rumi, ctx, sc, 16, 16, spp, depth, seed)
+    atomics = abs(((base - again) * dimg).sum())
+    picks = []
+    for t in range(3):
+        g = np.abs(gt[offs[t]:offs[t + 1]])
+        for rank in (0, 1):
+            flat = np.argsort(g.reshape(-1))[::-1][rank]
+            picks.append((t, int(flat // 3), int(flat % 3)))
+    for t, texel, c in picks:
+        w = sc["textures"][t].shape[1]
+        y, x = texel // w, texel % w
         v = float(sc["textures"][t][y, x, c])
         res = []
         for sgn in (1, -1):
@@ -90,7 +102,10 @@ def test_texel_gradients_equal_exact_differences(rumi, ctx):
                         float(s2["textures"][t][y, x, c])))
         (ip, vp), (im, vm) = res
         fd = ((ip - im) * dimg).sum() / (vp - vm)
-        np.testing.assert_allclose(gt[idx, c], fd, rtol=1e-2, atol=1e-6, err_msg=str((t, y, x, c)))
+        got = gt[offs[t] + texel, c]
+        assert got != 0.0
+        tol = 1e-2 * abs(fd) + 4 * atomics / (vp - vm) + 1e-7
+        assert abs(got - fd) <= tol, (t, y, x, c, got, fd, tol)
 
 
 def test_texel_gradients_match_oracle(rumi, oracle, ctx):

@@ -245,3 +245,38 @@ def test_full_config_icosphere_1p3m_8m_rays(rumi, oracle, ctx):
     assert ((p[:128] == bp) | tie_ok(t[:128], bt)).all()
     del b
     torch.cuda.synchronize()
+
+
+def test_every_tuning_gives_identical_hits(rumi, ctx, small_mesh):
+    """Block size, LDS stack depth (with spill), refill threshold and the
+    one-ray-per-lane launch change the schedule, never the result."""
+    import importlib
+    import os
+    import sys
+    import torch
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "rumi-renderer_amd"))
+    autotune = importlib.import_module("autotune")
+    xyz, idx = rumi.make_icosphere(48, 1, 0.05)
+    bvh = rumi.Bvh(ctx, xyz, idx)
+    rays = torch.from_numpy(rumi.make_rays_uniform(1 << 16, 9, 2.1)).cuda()
+    ref_h = torch.empty((1 << 16, 2), dtype=torch.int32, device="cuda")
+    ref_a = torch.empty(1 << 16, dtype=torch.int32, device="cuda")
+    ctx.set_trace_tuning(persistent=-1)
+    rumi.trace_closest(ctx, bvh, rays, ref_h)
+    rumi.trace_any(ctx, bvh, rays, ref_a)
+    torch.cuda.synchronize()
+    for t in autotune.trace_candidates():
+        ctx.set_trace_tuning(**t)
+        assert ctx.trace_tuning() == t
+        h = torch.full_like(ref_h, 7)
+        a = torch.full_like(ref_a, 7)
+        rumi.trace_closest(ctx, bvh, rays, h)
+        rumi.trace_any(ctx, bvh, rays, a)
+        torch.cuda.synchronize()
+        assert torch.equal(h, ref_h), t
+        assert torch.equal(a, ref_a), t
+    best, table = autotune.tune_trace(ctx, lambda: rumi.trace_closest(ctx, bvh, rays, ref_h))
+    print("tuning", best, [(c, round(s * 1e6, 1)) for c, s in table])
+    ctx.set_trace_tuning()
+    assert ctx.trace_tuning() == dict(block=128, stack=16, refill=16, persistent=2)
