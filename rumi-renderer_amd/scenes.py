@@ -145,7 +145,8 @@ def batch_scene(i: int, freq: int, side: int, seed: int = 4, tex_size: int = 128
             dict(kind="emissive", kd=1.0, color=(6.0, 6.0, 6.0))]
     shapes = [dict(kind="mesh", xyz=xyz, idx=idx, uv=uv, material=0, texture=0)]
     shapes += _quad((-6, -1.2, 6), (6, -1.2, 6), (6, -1.2, -6), (-6, -1.2, -6), 1, (0, 1, 0))
-    shapes.append(dict(kind="rect", p=(-1.0, 3.0, 1.0), l=(2.0, 0, 0), w=(0, 0, -2.0), material=2,
+    # l x w points down (-y): the light faces the scene
+    shapes.append(dict(kind="rect", p=(-1.0, 3.0, -1.0), l=(2.0, 0, 0), w=(0, 0, 2.0), material=2,
                        light=True))
     pos = (0.0, 0.6, 3.6)
     cam = dict(position=pos, basis=_look_at(pos, (0.0, 0.0, 0.0)), fov_deg=40.0, y_ratio=1.0)

@@ -54,6 +54,7 @@ def test_items_equal_single_renders_and_graph_replays(rumi, ctx, scenes):
     scale = float(eager.abs().max())
     parts = batch.split(eager)
     for o, (sc, w, h), part in zip(objs, items, parts):
+        assert float(part.mean()) > 1e-3  # every item is lit
         single, _ = o.render(w, h, spp, depth, seed=seed)
         torch.cuda.synchronize()
         _close(part.cpu().numpy(), single.cpu().numpy(), scale)
@@ -93,12 +94,14 @@ def test_batched_adjoint_equals_per_scene_adjoint(rumi, ctx, scenes):
         for i, (o, (sc, w, h)) in enumerate(zip(objs, items)):
             I = img_h[po:po + w * h]
             T = t_h[po:po + w * h]
+            assert I.mean() > 1e-3
             np.testing.assert_allclose(loss_h[i], ((I.astype(np.float64) - T) ** 2).sum(), rtol=1e-4)
             dimg = torch.from_numpy((2.0 * (I - T)).reshape(h, w, 3).astype(np.float32)).cuda()
             _, _, dtex, _ = o.adjoint(dimg.contiguous(), spp, depth, seed=seed + 1, textures=True)
             torch.cuda.synchronize()
             ref = dtex.cpu().numpy()[:o.n_texels]
             got = tg_h[to:to + o.n_texels]
+            assert np.abs(ref).max() > 0
             np.testing.assert_allclose(got, ref, rtol=2e-3, atol=2e-4 * float(np.abs(ref).max()))
             po += w * h
             to += o.n_texels

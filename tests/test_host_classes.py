@@ -1,0 +1,110 @@
+"""The C++ drop-in classes (rumi-renderer_amd/host/rumi_renderer.hpp): the
+Cornell box written as the reference's Scene::Build writes a scene describes
+the same scene as the Python description, and renders the same image."""
+import importlib
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "rumi-renderer_amd")
+EXE = os.path.join(PKG, "cornell_example")
+
+
+def _exe():
+    if not os.path.exists(EXE):
+        subprocess.run(["make", "-s", "-C", PKG, "cornell_example"], check=True)
+    return EXE
+
+
+def _describe():
+    out = subprocess.run([_exe(), "--describe"], check=True, capture_output=True, text=True).stdout
+    shapes, mats, cam = [], [], None
+    for line in out.splitlines():
+        f = line.split()
+        if f[0] == "shape":
+            shapes.append([float(x) for x in f[1:]])
+        elif f[0] == "material":
+            mats.append([float(x) for x in f[1:]])
+        elif f[0] == "camera":
+            cam = [float(x) for x in f[1:]]
+    return np.array(shapes), np.array(mats), np.array(cam)
+
+
+def _scene_from_description(shapes, mats, cam, w, h):
+    kinds = {0: "matte", 1: "glossy", 2: "specular", 3: "emissive", 4: "dielectric"}
+    materials = []
+    for m in mats:
+        kind = kinds[int(m[0])]
+        d = dict(kind=kind, kd=m[1], ks=m[2], exponent=m[3], color=tuple(m[4:7]))
+        if kind == "dielectric":
+            d.update(eta_in=m[7], eta_out=m[8], absorption=(0, 0, 0))
+        materials.append(d)
+    out = []
+    for s in shapes:
+        kind, mat, light = int(s[0]), int(s[1]), bool(s[2])
+        v, n, r = s[4:13], s[13:22], s[22]
+        if kind == 0:
+            out.append(dict(kind="tri", v=v.reshape(3, 3), n=n.reshape(3, 3), material=mat))
+        elif kind == 1:
+            out.append(dict(kind="sphere", center=tuple(v[:3]), radius=r, material=mat))
+        else:
+            out.append(dict(kind="rect", p=tuple(v[:3]), l=tuple(v[3:6]), w=tuple(v[6:9]),
+                            material=mat, light=light))
+    camera = dict(position=tuple(cam[:3]), basis=cam[3:12].reshape(3, 3).tolist(),
+                  fov_deg=cam[12], y_ratio=h / w)
+    return dict(materials=materials, shapes=out, camera=camera, medium=None)
+
+
+def test_classes_describe_the_cornell_box():
+    sys.path.insert(0, PKG)
+    scenes = importlib.import_module("scenes")
+    ref = scenes.cornell_box(64, 64)
+    shapes, mats, cam = _describe()
+    assert len(shapes) == len(ref["shapes"]) and len(mats) == len(ref["materials"])
+    for s, r in zip(shapes, ref["shapes"]):
+        kind = {"tri": 0, "sphere": 1, "rect": 2}[r["kind"]]
+        assert int(s[0]) == kind and int(s[1]) == r["material"]
+        assert bool(s[2]) == bool(r.get("light"))
+        if kind == 0:
+            np.testing.assert_allclose(s[4:13], np.ravel(r["v"]), rtol=0, atol=1e-7)
+            np.testing.assert_allclose(s[13:22], np.ravel(r["n"]), rtol=0, atol=1e-7)
+        elif kind == 1:
+            np.testing.assert_allclose(s[4:7], r["center"], atol=1e-7)
+            assert s[22] == pytest.approx(r["radius"], rel=1e-6)
+        else:
+            np.testing.assert_allclose(s[4:13], np.ravel([r["p"], r["l"], r["w"]]), atol=1e-7)
+    for m, r in zip(mats, ref["materials"]):
+        kind = {"matte": 0, "emissive": 3, "dielectric": 4}[r["kind"]]
+        assert int(m[0]) == kind
+        if kind == 4:  # Dielectric(eta_in, eta_out, absorption): no colour
+            assert (m[7], m[8]) == (r["eta_in"], r["eta_out"])
+        else:
+            assert m[1] == r["kd"]
+            np.testing.assert_allclose(m[4:7], r["color"], rtol=1e-6)
+    np.testing.assert_allclose(cam[:3], ref["camera"]["position"], atol=1e-7)
+    # RotationY(pi) in float: the Python basis up to sin(float(pi))
+    np.testing.assert_allclose(cam[3:12], np.ravel(ref["camera"]["basis"]), atol=1e-6)
+    assert cam[12] == 40.0 and cam[13] == 1.0
+
+
+@pytest.mark.gpu
+def test_classes_render_the_python_image(rumi, ctx, tmp_path):
+    import torch
+    w, h, spp, seed = 48, 40, 16, 9
+    out = tmp_path / "img.f32"
+    r = subprocess.run([_exe(), str(w), str(h), str(spp), str(seed), str(out)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    img = np.fromfile(out, np.float32).reshape(h, w, 3)
+    shapes, mats, cam = _describe()
+    sc = rumi.Scene(ctx, _scene_from_description(shapes, mats, cam, w, h))
+    ref, _ = sc.render(w, h, spp, 5, seed=seed)
+    torch.cuda.synchronize()
+    ref = ref.cpu().numpy()
+    sc.close()
+    assert img.mean() > 0.01
+    np.testing.assert_allclose(img, ref, rtol=2e-4, atol=2e-6 * float(np.abs(ref).max()))
