@@ -8,10 +8,11 @@
 // pdf_k, which needs no per-vertex storage.  Each wavefront iteration runs
 //   regen   -- new camera paths into free queue slots (per-pixel PCG32),
 //   extend  -- closest hit of every queued ray (traverse.h),
-//   bucket  -- per-material index lists (wave ballot + one atomic per wave),
-//   shade   -- media, emission, next-event estimation, BSDF sampling; the
-//              surviving paths and the shadow rays are appended compactly
-//              to the next queue and the shadow queue,
+//   shade   -- per tile of 256 paths, a counting sort by material key in LDS
+//              (so a wave mostly runs one BSDF), then media, emission,
+//              next-event estimation, BSDF sampling; the surviving paths and
+//              the shadow rays are appended compactly to the next queue and
+//              the shadow queue (one atomic per tile and queue),
 //   shadow  -- any-hit of the shadow queue; unoccluded contributions are
 //              added to the film with float atomics.
 // Path state lives in HBM as structure-of-arrays buffers sized by the
@@ -159,9 +160,9 @@ struct SceneTab {
 struct Ctl {
     uint32_t queue[2];      // paths in path buffer 0 / 1
     uint32_t shadow;        // entries in the shadow queue
-    uint32_t pad0[5];
-    uint32_t bucket[8];     // paths per material key (k_bucket)
-    int32_t offset[12];     // exclusive prefix of bucket[] (k_offsets), offset[8] = total
+    uint32_t fetch[2];      // ray counters of the persistent extension / shadow stages
+    uint32_t pad0[3];
+    uint32_t pad1[20];
 };
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
@@ -276,66 +277,79 @@ __global__ __launch_bounds__(kBlock) void k_regen(PathBuf buf, uint32_t start, u
 }
 
 // Closest hit of every queued path (BVHAccel::Travel with the path's
-// in_object flag, bvh.cpp:150-213).
+// in_object flag, bvh.cpp:150-213), persistent lanes (run_lanes).
+template <bool SPILL, bool MULTI>
+struct ExtendItem {
+    PathBuf buf;
+    const SceneTab* tab;
+    const float4 *nodes, *prims, *rects;
+    RootInfo root;
+    int2* hits;
+    float2* uvs;
+    int32_t* s_ref;
+    float* s_tn;
+    int32_t* p_ref;
+    float* p_tn;
+    uint32_t tid, id, nid, nflags, nscene;
+    bool in_obj;
+    float4 n0, n1;
+    __device__ void prefetch(uint32_t k) {
+        n0 = buf.ray[2 * (size_t)k + 0];
+        n1 = buf.ray[2 * (size_t)k + 1];
+        nflags = buf.flags[k];
+        if (MULTI) nscene = buf.scene[k];
+        nid = k;
+    }
+    __device__ bool start(TravState& s) {
+        s.r = Ray{n0.x, n0.y, n0.z, n1.x, n1.y, n1.z, n1.w, n0.w};
+        id = nid;
+        in_obj = (nflags & 2u) != 0;
+        if (MULTI) {
+            const SceneView& v = tab[nscene].sv;
+            nodes = v.nodes;
+            prims = v.prims;
+            rects = v.rects;
+            return trav_begin(v.root, s);
+        }
+        return trav_begin(root, s);
+    }
+    __device__ bool step(TravState& s) {
+        int32_t nn = 0, np = 0;
+        return trav_step<kBlock, kStack, false, true, true, false, SPILL>(
+            nodes, prims, rects, s, in_obj, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+    }
+    __device__ void finish(const TravState& s) {
+        hits[id] = make_int2(s.hit, __float_as_int(s.r.tmin));
+        uvs[id] = make_float2(s.hu, s.hv);
+    }
+};
+
 template <bool SPILL, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* n_ptr,
-                                                   SceneView sv, const SceneTab* tab, int2* hits,
-                                                   float2* uvs) {
+                                                   uint32_t* counter, SceneView sv,
+                                                   const SceneTab* tab, int2* hits, float2* uvs) {
     __shared__ int32_t s_ref[kStack * kBlock];
     __shared__ float s_tn[kStack * kBlock];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t n = *n_ptr;
-    for (uint32_t i = blockIdx.x * kBlock + tid; i < n; i += gridDim.x * kBlock) {
-        const SceneView& v = view_of<MULTI>(sv, tab, MULTI ? buf.scene[i] : 0u);
-        const float4 r0 = buf.ray[2 * (size_t)i + 0];
-        const float4 r1 = buf.ray[2 * (size_t)i + 1];
-        Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, r0.w};
-        const bool in_object = (buf.flags[i] & 2u) != 0;
-        float hu = 0.0f, hv = 0.0f;
-        int32_t nn = 0, np = 0;
-        const int32_t hit = traverse<kBlock, kStack, false, true, true, false, SPILL>(
-            v.nodes, v.prims, v.rects, v.root, r, in_object, hu, hv, nn, np, s_ref, s_tn, tid);
-        hits[i] = make_int2(hit, __float_as_int(r.tmin));
-        uvs[i] = make_float2(hu, hv);
-    }
-}
-
-// Material key of every hit (kind 0..6, 7 = miss) into 8 bucket lists.
-template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_bucket(const uint32_t* n_ptr, const int2* hits,
-                                                   SceneView sv, const SceneTab* tab,
-                                                   const uint32_t* scenes, uint32_t* lists,
-                                                   uint32_t* counts, uint32_t cap) {
-    const uint32_t n = *n_ptr;
-    RUMI_WAVE_LOOP(n) {
-        const uint32_t i = base_ + lane_id();
-        const bool active = i < n;
-        int key = kKeyMiss;
-        if (active) {
-            const int32_t prim = hits[i].x;
-            if (prim >= 0) {
-                const SceneView& v = view_of<MULTI>(sv, tab, MULTI ? scenes[i] : 0u);
-                key = v.mats[v.shapes[prim].info[1]].kind;
-            }
-        }
-        for (int k = 0; k < kKeys; ++k) {
-            const bool mine = active && key == k;
-            const uint32_t slot = wave_append(&counts[k], mine);
-            if (mine) lists[(size_t)k * cap + slot] = i;
-        }
-    }
-}
-
-// Bucket offsets (exclusive prefix of the counts) for k_shade.
-__global__ void k_offsets(Ctl* ctl) {
-    if (threadIdx.x == 0) {
-        int32_t acc = 0;
-        for (int k = 0; k < kKeys; ++k) {
-            ctl->offset[k] = acc;
-            acc += (int32_t)ctl->bucket[k];
-        }
-        ctl->offset[kKeys] = acc;
-    }
+    constexpr int kSpill = SPILL ? kMaxDepth - kStack : 1;
+    int32_t p_ref[kSpill];
+    float p_tn[kSpill];
+    ExtendItem<SPILL, MULTI> it;
+    it.buf = buf;
+    it.tab = tab;
+    it.nodes = sv.nodes;
+    it.prims = sv.prims;
+    it.rects = sv.rects;
+    it.root = sv.root;
+    it.hits = hits;
+    it.uvs = uvs;
+    it.s_ref = s_ref;
+    it.s_tn = s_tn;
+    it.p_ref = p_ref;
+    it.p_tn = p_tn;
+    it.tid = threadIdx.x;
+    it.id = it.nid = it.nflags = it.nscene = 0;
+    it.in_obj = false;
+    run_lanes(*n_ptr, counter, it);
 }
 
 // Clears the counters the next wavefront iteration appends to.
@@ -343,7 +357,8 @@ __global__ void k_clear_ctl(Ctl* ctl, int next_queue) {
     const uint32_t t = threadIdx.x;
     if (t == 0) ctl->queue[next_queue] = 0u;
     if (t == 1) ctl->shadow = 0u;
-    if (t < kKeys) ctl->bucket[t] = 0u;
+    if (t == 2) ctl->fetch[0] = 0u;
+    if (t == 3) ctl->fetch[1] = 0u;
 }
 
 // ------------------------------------------------------------ media helpers
@@ -1083,22 +1098,93 @@ __device__ inline void track_end(const Track& tr, bool ended, uint32_t scene) {
     }
 }
 
-// Shading of every queued path, bucket by bucket (paths of one material kind
-// are adjacent, so a wave mostly runs one BSDF).  Appends continuing paths to
-// the other path buffer and shadow rays to the shadow queue.
+constexpr int kShadeBlock = 256;
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+    // set bits of mask in the lanes below this one
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Block-aggregated append: wave counts from ballots, one global atomic per
+// block, consecutive slots in lane order.  Every thread of the block calls it.
+template <int BLOCK>
+__device__ inline uint32_t block_append(uint32_t* counter, bool pred, uint32_t* s_tmp) {
+    constexpr int W = BLOCK / 64;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t mask = __ballot(pred);
+    const uint32_t rank = lane_rank(mask);
+    if (lane_id() == 0) s_tmp[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < W; ++w) {
+            const uint32_t c = s_tmp[w];
+            s_tmp[w] = total;
+            total += c;
+        }
+        s_tmp[W] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = s_tmp[W] + s_tmp[wave] + rank;
+    __syncthreads();
+    return slot;
+}
+
+// Shading of every queued path.  Each block takes a tile of 256 paths and
+// orders them by material key (kind 0..6, 7 = miss) with a counting sort in
+// LDS, so the waves of a tile mostly run one BSDF; continuing paths go to the
+// other path buffer and shadow rays to the shadow queue.
 template <int MODE, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const uint32_t* lists,
-                                                  uint32_t cap, const Ctl* ctl, const int2* hits,
-                                                  const float2* uvs, SceneView sv,
-                                                  const SceneTab* tab, Params prm, float* film,
-                                                  ShadowBuf sh, uint32_t* next_count,
-                                                  uint32_t* shadow_count, AdjCtx adj, Track tr) {
-    const uint32_t total = (uint32_t)ctl->offset[kKeys];
-    int offs[kKeys + 1];
-#pragma unroll
-    for (int k = 0; k <= kKeys; ++k) offs[k] = ctl->offset[k];
-    RUMI_WAVE_LOOP(total) {
-        const uint32_t j = base_ + lane_id();
+__global__ __launch_bounds__(kShadeBlock) void k_shade(PathBuf in, PathBuf out, const uint32_t* n_ptr,
+                                                       const int2* hits, const float2* uvs,
+                                                       SceneView sv, const SceneTab* tab,
+                                                       Params prm, float* film, ShadowBuf sh,
+                                                       uint32_t* next_count,
+                                                       uint32_t* shadow_count, AdjCtx adj,
+                                                       Track tr) {
+    constexpr int W = kShadeBlock / 64;
+    constexpr int kSortKeys = kKeys + 1;  // + 1: slots past the end of the queue
+    __shared__ uint32_t s_perm[kShadeBlock];
+    __shared__ uint32_t s_hist[W][kSortKeys];
+    __shared__ uint32_t s_tmp[W + 1];
+    const uint32_t n = *n_ptr;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
+    const uint32_t base = blockIdx.x * kShadeBlock;  // one tile per block
+    if (base >= n) return;
+    {
+        // counting sort of the tile by material key
+        const uint32_t i0 = base + tid;
+        int key = kKeys;
+        if (i0 < n) {
+            key = kKeyMiss;
+            const int32_t prim = hits[i0].x;
+            if (prim >= 0) {
+                const SceneView& v = view_of<MULTI>(sv, tab, MULTI ? in.scene[i0] : 0u);
+                key = v.mats[v.shapes[prim].info[1]].kind;
+            }
+        }
+        uint32_t rank = 0;
+        for (int k = 0; k < kSortKeys; ++k) {
+            const uint64_t m = __ballot(key == k);
+            if (key == k) rank = lane_rank(m);
+            if (lane_id() == 0) s_hist[wave][k] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < kSortKeys; ++k)
+                for (int w = 0; w < W; ++w) {
+                    const uint32_t c = s_hist[w][k];
+                    s_hist[w][k] = run;
+                    run += c;
+                }
+        }
+        __syncthreads();
+        s_perm[s_hist[wave][key] + rank] = i0;
+        __syncthreads();
+        const uint32_t i = s_perm[tid];
+
         ShadeOut so;
         so.alive = false;
         so.shadow = false;
@@ -1106,10 +1192,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const
 #pragma unroll
         for (int k = 0; k < 7; ++k) G.v[k] = 0.0f;
         uint32_t scene = 0;
-        if (j < total) {
-            int b = 0;
-            while (b + 1 < kKeys && (int)j >= offs[b + 1]) ++b;
-            const uint32_t i = lists[(size_t)b * cap + (j - (uint32_t)offs[b])];
+        const bool valid = i < n;
+        if (valid) {
             if (MULTI) scene = in.scene[i];
             const float4 r0 = in.ray[2 * (size_t)i + 0];
             const float4 r1 = in.ray[2 * (size_t)i + 1];
@@ -1132,8 +1216,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const
             so.sample = sample;
         }
         if (MODE == kReplay) flush_grad(G, adj.gstripes);
-        if (MULTI && tr.left) track_end(tr, j < total && !so.alive, scene);
-        const uint32_t ns = wave_append(next_count, so.alive);
+        if (MULTI && tr.left) track_end(tr, valid && !so.alive, scene);
+        const uint32_t ns = block_append<kShadeBlock>(next_count, so.alive, s_tmp);
         if (so.alive) {
             out.ray[2 * (size_t)ns + 0] = so.r0;
             out.ray[2 * (size_t)ns + 1] = so.r1;
@@ -1145,7 +1229,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const
             if (MODE == kReplay) out.pend[ns] = so.pend;
             if (MULTI) out.scene[ns] = scene;
         }
-        const uint32_t ss = wave_append(shadow_count, so.shadow);
+        const uint32_t ss = block_append<kShadeBlock>(shadow_count, so.shadow, s_tmp);
         if (so.shadow) {
             sh.ray[2 * (size_t)ss + 0] = so.s0;
             sh.ray[2 * (size_t)ss + 1] = so.s1;
@@ -1157,52 +1241,96 @@ __global__ __launch_bounds__(kBlock) void k_shade(PathBuf in, PathBuf out, const
     }
 }
 
-// Any-hit test of every shadow ray (Scene::VisibilityTest, scene.cpp:91-123);
-// unoccluded contributions go to the film (the adjoint passes: to the
-// per-sample sums and the connection's own gradient terms).
+// Any-hit test of every shadow ray (Scene::VisibilityTest, scene.cpp:91-123),
+// persistent lanes; unoccluded contributions go to the film (the adjoint
+// passes: to the per-sample sums and the connection's own gradient terms).
+template <bool SPILL, int MODE, bool MULTI>
+struct ShadowItem {
+    ShadowBuf sh;
+    const SceneTab* tab;
+    const float4 *nodes, *prims, *rects;
+    RootInfo root;
+    float* film;
+    AdjCtx adj;
+    Grad7 G;
+    int32_t* s_ref;
+    float* s_tn;
+    int32_t* p_ref;
+    float* p_tn;
+    uint32_t tid, id, nid, nscene;
+    float4 n0, n1;
+    __device__ void prefetch(uint32_t k) {
+        n0 = sh.ray[2 * (size_t)k + 0];
+        n1 = sh.ray[2 * (size_t)k + 1];
+        if (MULTI) nscene = sh.scene[k];
+        nid = k;
+    }
+    __device__ bool start(TravState& s) {
+        s.r = Ray{n0.x, n0.y, n0.z, n1.x, n1.y, n1.z, n1.w, 0.0f};
+        id = nid;
+        if (MULTI) {
+            const SceneView& v = tab[nscene].sv;
+            nodes = v.nodes;
+            prims = v.prims;
+            rects = v.rects;
+            return trav_begin(v.root, s);
+        }
+        return trav_begin(root, s);
+    }
+    __device__ bool step(TravState& s) {
+        int32_t nn = 0, np = 0;
+        return trav_step<kBlock, kStack, true, true, true, false, SPILL>(
+            nodes, prims, rects, s, false, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+    }
+    __device__ void finish(const TravState& s) {
+        // Scene::VisibilityTest: in shadow iff the any-hit query found a hit.
+        if (s.r.tmin > 0.0f) return;
+        const float4 c4 = sh.contrib[id];
+        const V3 c = v3(c4.x, c4.y, c4.z);
+        const uint32_t pixel = (uint32_t)__float_as_int(c4.w);
+        if (MODE != kReplay) film_add(film, pixel, c);
+        if (MODE == kPrimal) add3(adj.L_total + 3 * (size_t)sh.sample[id], c);
+        if (MODE == kReplay && !is_nan(c)) {
+            add3(adj.L_sofar + 3 * (size_t)sh.sample[id], c);
+            // factors of this connection alone: the shadow segment's
+            // Bernoulli transmittance and the phase value toward the light
+            const V3 w3 = cmul(ld3(adj.dimg + 3 * (size_t)pixel) * adj.inv_spp, c);
+            const float Ss = w3.x + w3.y + w3.z;
+            const float2 sc = sh.score[id];
+            if (sc.x > 0.0f) grad_grey(G, -sc.x, Ss);
+            G.v[6] += sc.y * Ss;
+        }
+    }
+};
+
 template <bool SPILL, int MODE, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t* n_ptr,
-                                                   SceneView sv, const SceneTab* tab, float* film,
-                                                   AdjCtx adj) {
+                                                   uint32_t* counter, SceneView sv,
+                                                   const SceneTab* tab, float* film, AdjCtx adj) {
     __shared__ int32_t s_ref[kStack * kBlock];
     __shared__ float s_tn[kStack * kBlock];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t n = *n_ptr;
-    RUMI_WAVE_LOOP(n) {
-        const uint32_t i = base_ + lane_id();
-        Grad7 G;
+    constexpr int kSpill = SPILL ? kMaxDepth - kStack : 1;
+    int32_t p_ref[kSpill];
+    float p_tn[kSpill];
+    ShadowItem<SPILL, MODE, MULTI> it;
+    it.sh = sh;
+    it.tab = tab;
+    it.nodes = sv.nodes;
+    it.prims = sv.prims;
+    it.rects = sv.rects;
+    it.root = sv.root;
+    it.film = film;
+    it.adj = adj;
 #pragma unroll
-        for (int k = 0; k < 7; ++k) G.v[k] = 0.0f;
-        if (i < n) {
-            const SceneView& v = view_of<MULTI>(sv, tab, MULTI ? sh.scene[i] : 0u);
-            const float4 r0 = sh.ray[2 * (size_t)i + 0];
-            const float4 r1 = sh.ray[2 * (size_t)i + 1];
-            Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, 0.0f};
-            float hu, hv;
-            int32_t nn = 0, np = 0;
-            traverse<kBlock, kStack, true, true, true, false, SPILL>(
-                v.nodes, v.prims, v.rects, v.root, r, false, hu, hv, nn, np, s_ref, s_tn, tid);
-            // Scene::VisibilityTest: in shadow iff the any-hit query found a hit.
-            if (!(r.tmin > 0.0f)) {
-                const float4 c4 = sh.contrib[i];
-                const V3 c = v3(c4.x, c4.y, c4.z);
-                const uint32_t pixel = (uint32_t)__float_as_int(c4.w);
-                if (MODE != kReplay) film_add(film, pixel, c);
-                if (MODE == kPrimal) add3(adj.L_total + 3 * (size_t)sh.sample[i], c);
-                if (MODE == kReplay && !is_nan(c)) {
-                    add3(adj.L_sofar + 3 * (size_t)sh.sample[i], c);
-                    // factors of this connection alone: the shadow segment's
-                    // Bernoulli transmittance and the phase value toward the light
-                    const V3 w3 = cmul(ld3(adj.dimg + 3 * (size_t)pixel) * adj.inv_spp, c);
-                    const float Ss = w3.x + w3.y + w3.z;
-                    const float2 sc = sh.score[i];
-                    if (sc.x > 0.0f) grad_grey(G, -sc.x, Ss);
-                    G.v[6] += sc.y * Ss;
-                }
-            }
-        }
-        if (MODE == kReplay) flush_grad(G, adj.gstripes);
-    }
+    for (int k = 0; k < 7; ++k) it.G.v[k] = 0.0f;
+    it.s_ref = s_ref;
+    it.s_tn = s_tn;
+    it.p_ref = p_ref;
+    it.p_tn = p_tn;
+    it.tid = threadIdx.x;
+    it.id = it.nid = it.nscene = 0;
+    run_lanes(*n_ptr, counter, it);
+    if (MODE == kReplay) flush_grad(it.G, adj.gstripes);
 }
 
 // ============================================================ BDPT
@@ -1844,16 +1972,15 @@ struct Wave {
     PathBuf pb[2];
     int2* hits;
     float2* uvs;
-    uint32_t* lists;
     ShadowBuf sh;
     Ctl* ctl;
     uint32_t cap;
 };
 
 // Bytes per queued path: 2 path sets (ray 32, beta 16, rng 8, pixel, sample,
-// flags, pend, scene 4 each), hit 8, barycentrics 8, 8 bucket lists 4 each,
-// shadow entry (ray 32, contrib 16, sample 4, score 8, scene 4).
-constexpr size_t kBytesPerPath = 2 * (32 + 16 + 8 + 4 * 5) + 8 + 8 + 4 * kKeys + (32 + 16 + 4 + 8 + 4);
+// flags, pend, scene 4 each), hit 8, barycentrics 8, shadow entry (ray 32,
+// contrib 16, sample 4, score 8, scene 4).
+constexpr size_t kBytesPerPath = 2 * (32 + 16 + 8 + 4 * 5) + 8 + 8 + (32 + 16 + 4 + 8 + 4);
 constexpr int kCarved = 24;  // arrays carved below, each aligned to 256 B
 
 size_t wave_bytes(uint32_t cap) { return kBytesPerPath * (size_t)cap + sizeof(Ctl) + 256 * kCarved; }
@@ -1877,7 +2004,6 @@ bool wave_carve(Wave& w, void* mem, size_t bytes, uint32_t cap) {
     }
     w.hits = (int2*)take(8 * (size_t)cap);
     w.uvs = (float2*)take(8 * (size_t)cap);
-    w.lists = (uint32_t*)take(4 * (size_t)kKeys * cap);
     w.sh.ray = (float4*)take(32 * (size_t)cap);
     w.sh.contrib = (float4*)take(16 * (size_t)cap);
     w.sh.sample = (uint32_t*)take(4 * (size_t)cap);
@@ -1914,22 +2040,21 @@ void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t 
     hipLaunchKernelGGL(k_clear_ctl, dim3(1), dim3(64), 0, s, ctl, cur ^ 1);
     if (ps.spill)
         hipLaunchKernelGGL((k_extend<true, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
-                           &ctl->queue[cur], ps.sv, ps.tab, w.hits, w.uvs);
+                           &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs);
     else
         hipLaunchKernelGGL((k_extend<false, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
-                           &ctl->queue[cur], ps.sv, ps.tab, w.hits, w.uvs);
-    hipLaunchKernelGGL((k_bucket<MULTI>), dim3(g), dim3(kBlock), 0, s, &ctl->queue[cur], w.hits, ps.sv,
-                       ps.tab, w.pb[cur].scene, w.lists, ctl->bucket, w.cap);
-    hipLaunchKernelGGL(k_offsets, dim3(1), dim3(64), 0, s, ctl);
-    hipLaunchKernelGGL((k_shade<MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur], w.pb[cur ^ 1],
-                       w.lists, w.cap, ctl, w.hits, w.uvs, ps.sv, ps.tab, ps.prm, ps.film, w.sh,
-                       &ctl->queue[cur ^ 1], &ctl->shadow, ps.adj, ps.tr);
+                           &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs);
+    // one 256-path tile per block; blocks past the queue's size return at once
+    const uint32_t gs = (uint32_t)((bound + kShadeBlock - 1) / kShadeBlock);
+    hipLaunchKernelGGL((k_shade<MODE, MULTI>), dim3(gs < 1 ? 1 : gs), dim3(kShadeBlock), 0, s, w.pb[cur],
+                       w.pb[cur ^ 1], &ctl->queue[cur], w.hits, w.uvs, ps.sv, ps.tab, ps.prm, ps.film,
+                       w.sh, &ctl->queue[cur ^ 1], &ctl->shadow, ps.adj, ps.tr);
     if (ps.spill)
         hipLaunchKernelGGL((k_shadow<true, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh, &ctl->shadow,
-                           ps.sv, ps.tab, ps.film, ps.adj);
+                           &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj);
     else
         hipLaunchKernelGGL((k_shadow<false, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh,
-                           &ctl->shadow, ps.sv, ps.tab, ps.film, ps.adj);
+                           &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj);
 }
 
 // One wavefront iteration over queue `cur` (at most `bound` paths).

@@ -106,86 +106,69 @@ __global__ __launch_bounds__(BLOCK) void k_trace_lanes(
     }
 }
 
-// Persistent lanes with a prefetched next ray per lane.  A lane whose ray
-// ends swaps in the ray it has already loaded and, in the same iteration,
-// issues the loads of its next one; those complete while the wave waits on
-// its node fetches anyway, so taking a new ray costs no extra memory round
-// trip.  Ray indices come from the global counter in wave chunks of 64,
-// requested half a chunk ahead.  Each ray's arithmetic is the same as in
-// k_trace, so the results are bit-identical.
+// Persistent lanes with a prefetched next ray per lane (run_lanes).
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL>
+struct QueryItem {
+    const float4* rays;
+    const float4* nodes;
+    const float4* tris;
+    RootInfo root;
+    int2* hits;
+    float2* uv;
+    int32_t* any_out;
+    int32_t* s_ref;
+    float* s_tn;
+    int32_t* p_ref;
+    float* p_tn;
+    uint32_t tid, id, nid;
+    float4 n0, n1;
+    __device__ void prefetch(uint32_t k) {
+        n0 = rays[2 * (size_t)k + 0];
+        n1 = rays[2 * (size_t)k + 1];
+        nid = k;
+    }
+    __device__ bool start(TravState& s) {
+        s.r = Ray{n0.x, n0.y, n0.z, n1.x, n1.y, n1.z, n1.w, n0.w};
+        id = nid;
+        return trav_begin(root, s);
+    }
+    __device__ bool step(TravState& s) {
+        int32_t nn = 0, np = 0;
+        return trav_step<BLOCK, STACK, ANY, ORDERED, false, false, SPILL>(
+            nodes, tris, nullptr, s, false, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+    }
+    __device__ void finish(const TravState& s) {
+        if (ANY && any_out) any_out[id] = s.hit;
+        if (hits) hits[id] = make_int2(s.hit, __float_as_int(s.r.tmin));
+        if (!ANY && uv) uv[id] = make_float2(s.hu, s.hv);
+    }
+};
+
 template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL>
 __global__ __launch_bounds__(BLOCK) void k_trace_prefetch(
     const float4* __restrict__ rays, uint32_t n, const float4* __restrict__ nodes,
     const float4* __restrict__ tris, RootInfo root, int2* __restrict__ hits,
     float2* __restrict__ uv, int32_t* __restrict__ any_out, uint32_t* __restrict__ counter) {
-    constexpr uint32_t kChunk = 64;
     __shared__ int32_t s_ref[STACK * BLOCK];
     __shared__ float s_tn[STACK * BLOCK];
     constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
     int32_t p_ref[kSpill];
     float p_tn[kSpill];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63u;
-    TravState s;
-    uint32_t id = 0, nid = 0;
-    float4 n0 = make_float4(0, 0, 0, 0), n1 = n0;
-    bool active = false, has_next = false, drained = false;
-    // wave-uniform chunk state; the pending chunk's base sits in lane 0
-    uint32_t cbase = 0, cleft = 0, pend_val = 0;
-    bool pend = false;
-    int32_t nn = 0, np = 0;
-    auto finish_ray = [&](uint32_t k) {
-        if (ANY && any_out) any_out[k] = s.hit;
-        if (hits) hits[k] = make_int2(s.hit, __float_as_int(s.r.tmin));
-        if (!ANY && uv) uv[k] = make_float2(s.hu, s.hv);
-    };
-    while (true) {
-        // 1. lanes whose ray ended take their prefetched one
-        if (!active && has_next) {
-            s.r = Ray{n0.x, n0.y, n0.z, n1.x, n1.y, n1.z, n1.w, n0.w};
-            id = nid;
-            has_next = false;
-            active = trav_begin(root, s);
-            if (!active) finish_ray(id);
-        }
-        // 2. ask for the next chunk of indices half a chunk ahead
-        if (!pend && cleft < kChunk / 2) {
-            if (lane == 0) pend_val = atomicAdd(counter, kChunk);
-            pend = true;
-        }
-        // 3. lanes without a prefetched ray take indices and issue its loads
-        const uint64_t want = __ballot(!has_next && !drained);
-        if (want != 0) {
-            if (cleft == 0) {
-                cbase = __shfl(pend_val, 0);
-                cleft = kChunk;
-                pend = false;
-            }
-            const uint32_t need = (uint32_t)__popcll(want);
-            const uint32_t take = need < cleft ? need : cleft;
-            const uint32_t rank = lane_below(want);
-            if (!has_next && !drained && rank < take) {
-                const uint32_t k = cbase + rank;
-                if (k >= n) {
-                    drained = true;
-                } else {
-                    n0 = rays[2 * (size_t)k + 0];
-                    n1 = rays[2 * (size_t)k + 1];
-                    nid = k;
-                    has_next = true;
-                }
-            }
-            cbase += take;
-            cleft -= take;
-        }
-        // 4. one traversal step of every live ray
-        if (active) {
-            active = trav_step<BLOCK, STACK, ANY, ORDERED, false, false, SPILL>(
-                nodes, tris, nullptr, s, false, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
-            if (!active) finish_ray(id);
-        }
-        if (__ballot(active || has_next || !drained) == 0) break;
-    }
+    QueryItem<BLOCK, STACK, ANY, ORDERED, SPILL> item;
+    item.rays = rays;
+    item.nodes = nodes;
+    item.tris = tris;
+    item.root = root;
+    item.hits = hits;
+    item.uv = uv;
+    item.any_out = any_out;
+    item.s_ref = s_ref;
+    item.s_tn = s_tn;
+    item.p_ref = p_ref;
+    item.p_tn = p_tn;
+    item.tid = threadIdx.x;
+    item.id = item.nid = 0;
+    run_lanes(n, counter, item);
 }
 
 __global__ void k_zero_counter(uint32_t* c) { *c = 0u; }

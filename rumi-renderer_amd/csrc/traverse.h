@@ -338,5 +338,75 @@ __device__ __forceinline__ int32_t traverse(const float4* __restrict__ nodes,
     return s.hit;
 }
 
+// Set bits of mask in the lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Persistent-lane driver over a queue of n rays.  Rays differ in cost by two
+// orders of magnitude, so one ray per lane leaves a wave waiting on its
+// slowest lane.  Here every lane steps its own ray one node or leaf at a
+// time; a lane whose ray ends swaps in the ray it has already prefetched and,
+// in the same iteration, issues the loads of its next one, which complete
+// while the wave waits on its node fetches anyway.  Ray indices come from
+// *counter (zero before the launch) in wave chunks of 64, requested half a
+// chunk ahead.  Each ray's arithmetic is the one-ray-per-lane arithmetic, so
+// results are bit-identical.
+//
+// Item (per lane, all inlined):
+//   void prefetch(uint32_t k)      issue the loads of ray k
+//   bool start(TravState& s)       take the prefetched ray: fill s.r, select
+//                                  the BVH, return trav_begin(...)
+//   bool step(TravState& s)        one trav_step on the selected BVH
+//   void finish(const TravState&)  consume the result of the current ray
+template <class Item>
+__device__ __forceinline__ void run_lanes(uint32_t n, uint32_t* counter, Item& item) {
+    constexpr uint32_t kChunk = 64;
+    const uint32_t lane = threadIdx.x & 63u;
+    TravState s;
+    bool active = false, has_next = false, drained = false;
+    uint32_t cbase = 0, cleft = 0, pend_val = 0;  // wave-uniform chunk state
+    bool pend = false;
+    while (true) {
+        if (!active && has_next) {
+            has_next = false;
+            active = item.start(s);
+            if (!active) item.finish(s);
+        }
+        if (!pend && cleft < kChunk / 2) {
+            if (lane == 0) pend_val = atomicAdd(counter, kChunk);
+            pend = true;
+        }
+        const uint64_t want = __ballot(!has_next && !drained);
+        if (want != 0) {
+            if (cleft == 0) {
+                cbase = __shfl(pend_val, 0);
+                cleft = kChunk;
+                pend = false;
+            }
+            const uint32_t need = (uint32_t)__popcll(want);
+            const uint32_t take = need < cleft ? need : cleft;
+            const uint32_t rank = lanes_below(want);
+            if (!has_next && !drained && rank < take) {
+                const uint32_t k = cbase + rank;
+                if (k >= n) {
+                    drained = true;
+                } else {
+                    item.prefetch(k);
+                    has_next = true;
+                }
+            }
+            cbase += take;
+            cleft -= take;
+        }
+        if (active) {
+            active = item.step(s);
+            if (!active) item.finish(s);
+        }
+        if (__ballot(active || has_next || !drained) == 0) break;
+    }
+}
+
 }  // namespace dev
 }  // namespace rumi

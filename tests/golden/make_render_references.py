@@ -1,10 +1,12 @@
 """Writes the FP64 oracle reference images the statistical render tests
-compare against: per-pixel mean radiance and mean of squares at 4096 spp.
+compare against: per-pixel mean radiance (and, for the path tracer, mean of
+squares) at 4096 spp.
 
-    python tests/golden/make_render_references.py
+    python tests/golden/make_render_references.py [name ...]
 
-Runs the oracle (oracle/oracle_render.c, the restatement of
-PathTracer::Trace) with 8 threads; about a minute on 8 cores.
+Runs the oracle (oracle/oracle_render.c: the restatements of
+PathTracer::Trace and BidirectionalPathTracer::Trace) with 8 threads; a few
+minutes on 8 cores.
 """
 import importlib
 import os
@@ -21,21 +23,32 @@ import oracle  # noqa: E402
 import scenes  # noqa: E402
 
 CASES = {
-    # name: (scene factory, width, height, spp, max_depth, seed)
-    "cornell64_pt_d5": (scenes.cornell_box, 64, 64, 4096, 5, 1001),
-    "media64_pt_d16": (scenes.cornell_media, 64, 64, 4096, 16, 1002),
+    # name: (integrator, scene factory, width, height, spp, max_depth or path length, seed)
+    "cornell64_pt_d5": ("pt", scenes.cornell_box, 64, 64, 4096, 5, 1001),
+    "media64_pt_d16": ("pt", scenes.cornell_media, 64, 64, 4096, 16, 1002),
+    # the BDPT workload's scene (200k-triangle GGX icosphere + glass sphere),
+    # path length 8, 9 + 9 vertex caps, power heuristic
+    "bdpt64_l8": ("bdpt", scenes.cornell_bdpt, 64, 64, 4096, 8, 1003),
 }
 
 
-def main():
-    for name, (factory, w, h, spp, depth, seed) in CASES.items():
+def main(names):
+    for name, (kind, factory, w, h, spp, depth, seed) in CASES.items():
+        if names and name not in names:
+            continue
         sc = factory(w, h)
-        mean, m2 = oracle.render_pt(sc, w, h, spp, depth, seed=seed, nthreads=8)
         path = os.path.join(HERE, name + ".npz")
-        np.savez_compressed(path, mean=mean.astype(np.float32), m2=m2.astype(np.float32),
-                            spp=spp, max_depth=depth, seed=seed)
+        if kind == "pt":
+            mean, m2 = oracle.render_pt(sc, w, h, spp, depth, seed=seed, nthreads=8)
+            np.savez_compressed(path, mean=mean.astype(np.float32), m2=m2.astype(np.float32),
+                                spp=spp, max_depth=depth, seed=seed)
+        else:
+            mean = oracle.render_bdpt(sc, w, h, spp, depth, n_eye=depth + 1, n_light=depth + 1,
+                                      mis_beta=2.0, seed=seed, nthreads=8)
+            np.savez_compressed(path, mean=mean.astype(np.float32), spp=spp, max_len=depth,
+                                seed=seed, mis_beta=2.0)
         print(name, "mean", mean.mean(axis=(0, 1)), "->", path)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
