@@ -491,17 +491,38 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     sec = min(times)
     sps = st.samples / sec
     scene.close()
-    # CPU baseline leg: the oracle's FP64 bidirectional tracer, one thread
+    # rel-MSE at equal spp against the committed 64^2 4096-spp oracle
+    # reference: the GPU's renders here, the CPU's from the baseline leg
+    import numpy as np
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "bdpt64_l8.npz"))
+    sc64 = scenes.cornell_bdpt(64, 64)
+    s64 = rumi.Scene(ctx, sc64)
+    g_vals = []
+    for k in range(4):
+        gi, _ = s64.render(64, 64, spp, max_len, seed=300 + k, **kw)
+        torch.cuda.synchronize()
+        g_vals.append(rel_mse(gi.cpu().numpy(), ref["mean"]))
+    s64.close()
+    # CPU baseline leg: the oracle's FP64 bidirectional tracer timed on one
+    # thread over a 32^2 render at the same spp; its rel-MSE at equal spp from
+    # two 64^2 renders on all host threads
     import oracle
     t0 = time.perf_counter()
     oracle.render_bdpt(scenes.cornell_bdpt(32, 32), 32, 32, spp, max_len, n_eye=9, n_light=9,
                        mis_beta=2.0, seed=5, nthreads=1)
     cpu_sec = time.perf_counter() - t0
     cpu_sps = 32 * 32 * spp / cpu_sec
-    log(f"cornell_bdpt: {sps / 1e6:.3f} Msamples/s ({sec * 1e3:.1f} ms), CPU {cpu_sps:.0f} samples/s")
+    c_vals = [rel_mse(oracle.render_bdpt(sc64, 64, 64, spp, max_len, n_eye=9, n_light=9,
+                                         mis_beta=2.0, seed=400 + k, nthreads=os.cpu_count() or 1),
+                      ref["mean"]) for k in range(2)]
+    log(f"cornell_bdpt: {sps / 1e6:.3f} Msamples/s ({sec * 1e3:.1f} ms), CPU {cpu_sps:.0f} samples/s, "
+        f"rel-MSE ratio {np.mean(g_vals) / np.mean(c_vals):.3f}")
     return {"samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4),
             "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
             "max_path_length": max_len, "mis_beta": 2.0, "triangles": 200000 + 10,
+            "rel_mse_64px": {"gpu": float(np.mean(g_vals)), "cpu_oracle": float(np.mean(c_vals)),
+                             "ratio": float(np.mean(g_vals) / np.mean(c_vals)), "spp": spp,
+                             "renders": [4, 2]},
             "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
                              "kind": "port",
                              "sample": f"32x32 at {spp} spp, the oracle's FP64 bidirectional "

@@ -1359,6 +1359,8 @@ struct BVert {
 struct BdptBufs {
     BVert* eye;       // [vertex][sample]
     BVert* light;
+    float4* eye_mis;  // [vertex][sample]: pdf_fwd, pdf_rev, flags, 0 -- what the MIS walk reads
+    float4* light_mis;
     uint8_t* n_eye;
     uint8_t* n_light;
     uint32_t nb;      // samples in the batch (row stride)
@@ -1452,20 +1454,21 @@ __device__ inline void surface_at(const SceneView& sv, int32_t prim, V3 o, V3 d,
     tex = pattern(sv, s, p, bu, bv, tex_index);
 }
 
-__device__ inline void store_vert(BVert* arr, uint32_t idx, V3 p, float pdf_fwd, V3 n, V3 beta,
-                                  uint32_t flags, V3 tex, int mat, float bw = 0.0f) {
+__device__ inline void store_vert(BVert* arr, float4* mis, uint32_t idx, V3 p, float pdf_fwd, V3 n,
+                                  V3 beta, uint32_t flags, V3 tex, int mat, float bw = 0.0f) {
     BVert v;
     v.a = make_float4(p.x, p.y, p.z, pdf_fwd);
     v.b = make_float4(n.x, n.y, n.z, bw);
     v.c = make_float4(beta.x, beta.y, beta.z, __int_as_float((int)flags));
     v.d = make_float4(tex.x, tex.y, tex.z, __int_as_float(mat));
     arr[idx] = v;
+    mis[idx] = make_float4(pdf_fwd, bw, __int_as_float((int)flags), 0.0f);
 }
 
 // One random walk (eye: radiance transport; light: importance transport,
 // no 1/eta^2 on refraction) writing vertices start..; returns the count.
 template <bool EYE>
-__device__ int random_walk(const SceneView& sv, BVert* arr, uint32_t nb, uint32_t j, int start,
+__device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_t nb, uint32_t j, int start,
                            int max_verts, V3 o, V3 d, V3 beta, float pdf_w, Rng& g,
                            int32_t* s_ref, float* s_tn, uint32_t tid, bool spill) {
     int nv = start;
@@ -1490,7 +1493,7 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, uint32_t nb, uint32_
             // The walk ends on an emitter (bidirectionalpathtracer.cpp:693-699).
             // an emitter vertex is never scattered from, so its reverse-pdf
             // word carries the primitive id instead
-            store_vert(arr, (uint32_t)nv * nb + j, p, pdf_fwd, n, beta, kVEmitter, tex, mi,
+            store_vert(arr, mis, (uint32_t)nv * nb + j, p, pdf_fwd, n, beta, kVEmitter, tex, mi,
                        __int_as_float(prim));
             ++nv;
             break;
@@ -1515,7 +1518,7 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, uint32_t nb, uint32_
             const float cost = dot(n, w);
             pdf_next = ggx_pdf(m, n, w, wo);
             if (!(cost > 0.0f && pdf_next > 0.0f)) {
-                store_vert(arr, slot, p, pdf_fwd, n, beta, 0u, tex, mi);
+                store_vert(arr, mis, slot, p, pdf_fwd, n, beta, 0u, tex, mi);
                 break;
             }
             weight = cmul(ggx_f(m, n, w, wo), tex) * (cost / pdf_next);
@@ -1569,11 +1572,13 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, uint32_t nb, uint32_
                 weight = cmul(weight, v3(expf(-m.absorption[0] * r.tmin), expf(-m.absorption[1] * r.tmin),
                                          expf(-m.absorption[2] * r.tmin)));
         }
-        store_vert(arr, slot, p, pdf_fwd, n, beta, delta ? kVDelta : 0u, tex, mi);
+        store_vert(arr, mis, slot, p, pdf_fwd, n, beta, delta ? kVDelta : 0u, tex, mi);
         // reverse pdf of the previous vertex (area measure at prev)
         if (nv >= 2) {
-            BVert& pv = arr[(uint32_t)(nv - 2) * nb + j];
+            const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
+            BVert& pv = arr[pi];
             pv.b.w = delta ? 0.0f : to_area(pdf_rev, p, prev_p, v3(pv.b.x, pv.b.y, pv.b.z));
+            mis[pi].y = pv.b.w;
         }
         beta = cmul(beta, weight);
         pdf_w = delta ? 0.0f : pdf_next;
@@ -1610,10 +1615,10 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_eye(BdptBufs bb, uint64_t base,
                     cam.R[6] * l.x + cam.R[7] * l.y + cam.R[8] * l.z);
     const V3 cp = v3(cam.p[0], cam.p[1], cam.p[2]);
     const V3 cdir = v3(cam.R[2], cam.R[5], cam.R[8]);
-    store_vert(bb.eye, j, cp, 1.0f, cdir, v3(1, 1, 1), kVCamera, v3(1, 1, 1), -1);
+    store_vert(bb.eye, bb.eye_mis, j, cp, 1.0f, cdir, v3(1, 1, 1), kVCamera, v3(1, 1, 1), -1);
     // Pinhole::Pdfw: 1 / (cos^3 film_area) at plane distance 1
     const float pdf_dir = 1.0f / (l.z * l.z * l.z * bp.film_area);
-    const int nv = random_walk<true>(sv, bb.eye, bb.nb, j, 1, bp.n_eye, cp, d, v3(1, 1, 1), pdf_dir,
+    const int nv = random_walk<true>(sv, bb.eye, bb.eye_mis, bb.nb, j, 1, bp.n_eye, cp, d, v3(1, 1, 1), pdf_dir,
                                      g, s_ref, s_tn, tid, spill);
     bb.n_eye[j] = (uint8_t)nv;
 }
@@ -1652,11 +1657,11 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_light(BdptBufs bb, uint64_t bas
     }
     const float pdf_pos = 1.0f / (float)sv.n_lights * L.p[3];
     const V3 le = ld3(L.le);
-    store_vert(bb.light, j, p0, pdf_pos, n0, le * (1.0f / pdf_pos), kVLight, v3(1, 1, 1), -1);
+    store_vert(bb.light, bb.light_mis, j, p0, pdf_pos, n0, le * (1.0f / pdf_pos), kVLight, v3(1, 1, 1), -1);
     float cost, pdf_dir;
     const V3 w = lambert_sample(n0, g, cost, pdf_dir);
     const V3 beta = le * (cost / (pdf_pos * pdf_dir));
-    const int nv = random_walk<false>(sv, bb.light, bb.nb, j, 1, bp.n_light, p0, w, beta, pdf_dir, g,
+    const int nv = random_walk<false>(sv, bb.light, bb.light_mis, bb.nb, j, 1, bp.n_light, p0, w, beta, pdf_dir, g,
                                       s_ref, s_tn, tid, spill);
     bb.n_light[j] = (uint8_t)nv;
 }
@@ -1778,23 +1783,33 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
     // MIS weight over all strategies of this path (power heuristic with exponent beta)
     float w = 1.0f;
     if (s + t > 2 && !mis_trivial) {
+        // the walks read only the compact (pdf_fwd, pdf_rev, flags) records,
+        // each vertex once
         float sum = 0.0f, ri = 1.0f;
-        for (int i = t - 1; i > 0; --i) {
-            const BVert vi = bb.eye[(uint32_t)i * nb + j];
-            const float pr = i == t - 1 ? pt_rev : (i == t - 2 ? ptm_rev : vi.b.w);
-            ri *= remap0(pr) / remap0(vi.a.w);
-            const bool di = i == t - 1 ? false : (bv_flags(vi) & kVDelta) != 0;
-            const bool dprev = (bv_flags(bb.eye[(uint32_t)(i - 1) * nb + j]) & kVDelta) != 0;
-            if (!di && !dprev) sum += mis_pow(ri, bp.beta);
+        if (t - 1 > 0) {
+            float4 mi = bb.eye_mis[(uint32_t)(t - 1) * nb + j];
+            for (int i = t - 1; i > 0; --i) {
+                const float4 mp = bb.eye_mis[(uint32_t)(i - 1) * nb + j];
+                const float pr = i == t - 1 ? pt_rev : (i == t - 2 ? ptm_rev : mi.y);
+                ri *= remap0(pr) / remap0(mi.x);
+                const bool di = i == t - 1 ? false : ((uint32_t)__float_as_int(mi.z) & kVDelta) != 0;
+                const bool dprev = ((uint32_t)__float_as_int(mp.z) & kVDelta) != 0;
+                if (!di && !dprev) sum += mis_pow(ri, bp.beta);
+                mi = mp;
+            }
         }
         ri = 1.0f;
-        for (int i = s - 1; i >= 0; --i) {
-            const BVert vi = bb.light[(uint32_t)i * nb + j];
-            const float pr = i == s - 1 ? qs_rev : (i == s - 2 ? qsm_rev : vi.b.w);
-            ri *= remap0(pr) / remap0(vi.a.w);
-            const bool di = i == s - 1 ? false : (bv_flags(vi) & kVDelta) != 0;
-            const bool dprev = i > 0 ? (bv_flags(bb.light[(uint32_t)(i - 1) * nb + j]) & kVDelta) != 0 : false;
-            if (!di && !dprev) sum += mis_pow(ri, bp.beta);
+        if (s - 1 >= 0) {
+            float4 mi = bb.light_mis[(uint32_t)(s - 1) * nb + j];
+            for (int i = s - 1; i >= 0; --i) {
+                const float4 mp = i > 0 ? bb.light_mis[(uint32_t)(i - 1) * nb + j] : make_float4(0, 0, 0, 0);
+                const float pr = i == s - 1 ? qs_rev : (i == s - 2 ? qsm_rev : mi.y);
+                ri *= remap0(pr) / remap0(mi.x);
+                const bool di = i == s - 1 ? false : ((uint32_t)__float_as_int(mi.z) & kVDelta) != 0;
+                const bool dprev = i > 0 ? ((uint32_t)__float_as_int(mp.z) & kVDelta) != 0 : false;
+                if (!di && !dprev) sum += mis_pow(ri, bp.beta);
+                mi = mp;
+            }
         }
         w = 1.0f / (1.0f + sum);
     }
@@ -1868,7 +1883,9 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     uint32_t nb = desc->queue_capacity ? desc->queue_capacity : (1u << 19);
     if ((uint64_t)nb > total) nb = (uint32_t)total;
     const size_t vbytes = sizeof(BVert) * (size_t)nb;
-    const size_t need = vbytes * (size_t)(nE + nL) + 2 * (size_t)nb + sizeof(int2) * strat.size() + 4096;
+    const size_t mbytes = sizeof(float4) * (size_t)nb;
+    const size_t need = (vbytes + mbytes) * (size_t)(nE + nL) + 2 * (size_t)nb + sizeof(int2) * strat.size() +
+                        8192;
     if (sc->work_bytes < need) {
         if (sc->work) (void)hipFree(sc->work);
         sc->work = nullptr;
@@ -1886,6 +1903,8 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     BdptBufs bb;
     bb.eye = (BVert*)take(vbytes * (size_t)nE);
     bb.light = (BVert*)take(vbytes * (size_t)nL);
+    bb.eye_mis = (float4*)take(mbytes * (size_t)nE);
+    bb.light_mis = (float4*)take(mbytes * (size_t)nL);
     bb.n_eye = (uint8_t*)take(nb);
     bb.n_light = (uint8_t*)take(nb);
     int2* d_strat = (int2*)take(sizeof(int2) * strat.size());

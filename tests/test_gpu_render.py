@@ -180,3 +180,26 @@ def test_bdpt_rejects_media_and_glossy(rumi, ctx, scenes):
     with pytest.raises(rumi.RumiError):
         scene.render(8, 8, 1, 4, integrator=rumi.RUMI_INTEGRATOR_BDPT)
     scene.close()
+
+
+def test_bdpt_workload_matches_oracle_reference(rumi, oracle, ctx, scenes):
+    """The BDPT workload's scene (200k-triangle GGX icosphere, glass sphere,
+    path length 8, power heuristic) at 64^2: channel means of the GPU within
+    3.5 sigma of the oracle's 4096-spp reference, and mean rel-MSE at equal
+    spp within 3 standard errors (plus 5%) of the oracle's."""
+    ref = np.load(os.path.join(HERE, "golden", "bdpt64_l8.npz"))
+    sc = scenes.cornell_bdpt(64, 64)
+    spp, runs = 64, 6
+    kw = dict(integrator=rumi.RUMI_INTEGRATOR_BDPT, mis_beta=2.0, n_eye=9, n_light=9)
+    g = [render(rumi, ctx, sc, 64, 64, spp, 8, seed=700 + k, **kw)[0] for k in range(runs)]
+    c = [oracle.render_bdpt(sc, 64, 64, spp, 8, n_eye=9, n_light=9, mis_beta=2.0, seed=800 + k,
+                            nthreads=8) for k in range(runs)]
+    gm = np.array([i.mean(axis=(0, 1)) for i in g])
+    sd = gm.std(0, ddof=1)
+    z = (gm.mean(0) - ref["mean"].mean(axis=(0, 1))) / np.sqrt(sd ** 2 / runs + sd ** 2 * spp / 4096)
+    gv = np.array([rc.rel_mse(i, ref["mean"]) for i in g])
+    cv = np.array([rc.rel_mse(i, ref["mean"]) for i in c])
+    se = np.sqrt(gv.var(ddof=1) / runs + cv.var(ddof=1) / runs)
+    print("BDPT z", z, "rel-MSE GPU", gv.mean(), "CPU", cv.mean())
+    assert np.all(np.abs(z) < 3.5), z
+    assert abs(gv.mean() - cv.mean()) < 3 * se + 0.05 * cv.mean(), (gv, cv)
