@@ -43,6 +43,8 @@ def main():
                     help="rays in the bounded CPU-baseline sample (0 = skip)")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-scene leg")
+    ap.add_argument("--grad-check", type=float, default=0.0, metavar="SECONDS",
+                    help="only compare the media adjoint with central differences for about SECONDS")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default traversal launch shape instead of autotuning it")
@@ -65,7 +67,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by rays; see DESIGN.md)", file=sys.stderr)
+              "(the work shards by ray and sample ranges; this build measures one GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -73,6 +76,10 @@ def main():
 
     rumi = importlib.import_module("rumi-renderer_amd")
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+
+    if args.grad_check > 0:
+        grad_check(rumi, rumi.Context(0), args, log)
+        return
 
     t0 = time.time()
     xyz, idx = rumi.make_icosphere(args.freq, 1, 0.05)
@@ -404,6 +411,69 @@ def batch_leg(rumi, ctx, args, log, scenes):
     return res
 
 
+def grad_check(rumi, ctx, args, log):
+    """Adjoint vs central finite differences on the differentiable-media
+    workload (512^2, depth 16), step eps = 1e-3 relative per parameter, common
+    random numbers (one seed per +/- pair).  The loss is linearised at a fixed
+    weight image w = 2 (I0 - T) (I0 at theta, T at theta x 1.10, both 256 spp)
+    so that the adjoint and the differences estimate the same quantity,
+    sum w dE[I]/dtheta.  Runs each side for the given seconds; prints one JSON
+    line with the per-parameter relative difference and its standard error."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "rumi-renderer_amd"))
+    import scenes
+    w = h = 512
+    spp, depth, eps = 32, 16, 1e-3
+    theta = np.array([0.05, 0.2, 0.4, 1.0, 1.0, 1.0, 0.7])
+
+    def scene(p):
+        return rumi.Scene(ctx, scenes.cornell_media(w, h, sigma_a=tuple(p[:3]), sigma_s=tuple(p[3:6]),
+                                                    g=float(p[6])))
+
+    s0, st = scene(theta), scene(theta * 1.10)
+    i0 = s0.render(w, h, 256, depth, seed=11)[0].clone()
+    tg = st.render(w, h, 256, depth, seed=12)[0].clone()
+    st.close()
+    wgt = (2.0 * (i0 - tg)).contiguous()
+    torch.cuda.synchronize()
+    adj = []
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.grad_check * 0.2 or len(adj) < 8:
+        adj.append(s0.adjoint(wgt, spp, depth, seed=1000 + len(adj))[1].cpu().double().numpy())
+    adj = np.array(adj)
+    fd = [[] for _ in range(7)]
+    t0 = time.perf_counter()
+    r = 0
+    budget = args.grad_check * 0.8
+    while time.perf_counter() - t0 < budget or r < 4:
+        for k in range(7):
+            hk = eps * abs(theta[k])
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += hk
+            tm[k] -= hk
+            sp, sm = scene(tp), scene(tm)
+            ip = sp.render(w, h, spp, depth, seed=5000 + r)[0]
+            im = sm.render(w, h, spp, depth, seed=5000 + r)[0]
+            fd[k].append(float(((ip - im).double() * wgt.double()).sum()) / (2 * hk))
+            sp.close()
+            sm.close()
+        r += 1
+    s0.close()
+    fd = np.array(fd).T
+    a_m, a_se = adj.mean(0), adj.std(0, ddof=1) / np.sqrt(len(adj))
+    f_m, f_se = fd.mean(0), fd.std(0, ddof=1) / np.sqrt(len(fd))
+    rel = np.abs(a_m - f_m) / np.abs(f_m)
+    rel_se = np.sqrt(a_se ** 2 + f_se ** 2) / np.abs(f_m)
+    out = {"metric": "adjoint vs central FD (eps 1e-3), media 512^2 depth 16",
+           "params": ["sigma_a.r", "sigma_a.g", "sigma_a.b", "sigma_s.r", "sigma_s.g", "sigma_s.b", "g"],
+           "adjoint": a_m.round(5).tolist(), "adjoint_se": a_se.round(5).tolist(),
+           "fd": f_m.round(5).tolist(), "fd_se": f_se.round(5).tolist(),
+           "rel_diff": rel.round(4).tolist(), "rel_diff_se": rel_se.round(4).tolist(),
+           "adjoint_runs": len(adj), "fd_pairs_per_param": len(fd), "spp": spp}
+    print(json.dumps(out), flush=True)
+
+
 def gradient_leg(rumi, ctx, args, log, scenes):
     """Differentiable media: 512^2 x 32 spp, depth 16, L2 loss against the
     image rendered at every medium parameter x 1.10.  One gradient = the
@@ -498,14 +568,14 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     sc64 = scenes.cornell_bdpt(64, 64)
     s64 = rumi.Scene(ctx, sc64)
     g_vals = []
-    for k in range(4):
+    for k in range(6):
         gi, _ = s64.render(64, 64, spp, max_len, seed=300 + k, **kw)
         torch.cuda.synchronize()
         g_vals.append(rel_mse(gi.cpu().numpy(), ref["mean"]))
     s64.close()
     # CPU baseline leg: the oracle's FP64 bidirectional tracer timed on one
     # thread over a 32^2 render at the same spp; its rel-MSE at equal spp from
-    # two 64^2 renders on all host threads
+    # six 64^2 renders on all host threads (single renders scatter by 2x)
     import oracle
     t0 = time.perf_counter()
     oracle.render_bdpt(scenes.cornell_bdpt(32, 32), 32, 32, spp, max_len, n_eye=9, n_light=9,
@@ -514,7 +584,7 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     cpu_sps = 32 * 32 * spp / cpu_sec
     c_vals = [rel_mse(oracle.render_bdpt(sc64, 64, 64, spp, max_len, n_eye=9, n_light=9,
                                          mis_beta=2.0, seed=400 + k, nthreads=os.cpu_count() or 1),
-                      ref["mean"]) for k in range(2)]
+                      ref["mean"]) for k in range(6)]
     log(f"cornell_bdpt: {sps / 1e6:.3f} Msamples/s ({sec * 1e3:.1f} ms), CPU {cpu_sps:.0f} samples/s, "
         f"rel-MSE ratio {np.mean(g_vals) / np.mean(c_vals):.3f}")
     return {"samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4),
@@ -522,7 +592,7 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
             "max_path_length": max_len, "mis_beta": 2.0, "triangles": 200000 + 10,
             "rel_mse_64px": {"gpu": float(np.mean(g_vals)), "cpu_oracle": float(np.mean(c_vals)),
                              "ratio": float(np.mean(g_vals) / np.mean(c_vals)), "spp": spp,
-                             "renders": [4, 2]},
+                             "renders": [6, 6]},
             "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
                              "kind": "port",
                              "sample": f"32x32 at {spp} spp, the oracle's FP64 bidirectional "
