@@ -441,6 +441,7 @@ def grad_check(rumi, ctx, args, log):
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < args.grad_check * 0.2 or len(adj) < 8:
         adj.append(s0.adjoint(wgt, spp, depth, seed=1000 + len(adj))[1].cpu().double().numpy())
+    log(f"grad check: {len(adj)} adjoint runs")
     adj = np.array(adj)
     fd = [[] for _ in range(7)]
     t0 = time.perf_counter()
@@ -459,6 +460,7 @@ def grad_check(rumi, ctx, args, log):
             sp.close()
             sm.close()
         r += 1
+        log(f"grad check: {r} difference rounds, {time.perf_counter() - t0:.0f}s")
     s0.close()
     fd = np.array(fd).T
     a_m, a_se = adj.mean(0), adj.std(0, ddof=1) / np.sqrt(len(adj))
