@@ -1462,12 +1462,25 @@ int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
  * (Pinhole::FilmPlaneIntersection, pinhole.cpp:138-167) -- weighted by MIS
  * over all strategies with exponent beta (1 = the reference's balance
  * heuristic, MISWeight :1096-1586).  Mirror and dielectric vertices are
- * never connected.  Materials: matte, GGX, specular, dielectric, emissive. */
+ * never connected.  Materials: matte, glossy, GGX, specular, dielectric,
+ * emissive.
+ *
+ * Glossy picks one lobe per vertex as Glossy::BSDFSampling does
+ * (src/Material/glossy.cpp:56-78): the Lambertian lobe with probability
+ * kd_pdf (then the vertex is a matte vertex whose f and throughput carry
+ * 1/kd_pdf, Glossy::Fr :80-90 over _pdf_path), else the Phong lobe, which
+ * Glossy::IsDirac (:35-41) marks as never connected.  The Phong lobe is
+ * sampled with Phong::BSDFSampling's rules (bsdf.cpp:127-170: a mirrored
+ * below-horizon sample, a returned cosine of 1), so its integrand is the path
+ * tracer's and is only reproduced from the eye side: light subpaths end at a
+ * Phong-lobe vertex, and the MIS sums stop at one (BV_PHONG), since every
+ * strategy beyond it would need a light subpath through it. */
 
 #define BV_DELTA 1
 #define BV_EMIT 2
 #define BV_CAM 4
 #define BV_LIGHT 8
+#define BV_PHONG 16
 
 typedef struct {
     v3 p, n;
@@ -1487,6 +1500,8 @@ static col bsdf_f_v(const scene_t* sc, const bvert_t* v, v3 wi, v3 wo)
 {
     const mat_t* m = &sc->mats[v->mat];
     if (m->kind == MAT_MATTE) return cscale(m->color, m->kd * INV_PI_D);
+    /* only Lambertian-lobe glossy vertices are connected */
+    if (m->kind == MAT_GLOSSY) return cscale(m->color, m->kd * INV_PI_D / m->kd_pdf);
     if (m->kind == MAT_GGX) return ggx_f(m, v->n, wi, wo);
     return cs(0.0);
 }
@@ -1494,7 +1509,7 @@ static col bsdf_f_v(const scene_t* sc, const bvert_t* v, v3 wi, v3 wo)
 static double bsdf_pdf_v(const scene_t* sc, const bvert_t* v, v3 wi, v3 wo)
 {
     const mat_t* m = &sc->mats[v->mat];
-    if (m->kind == MAT_MATTE) {
+    if (m->kind == MAT_MATTE || m->kind == MAT_GLOSSY) {
         double p = INV_PI_D * dot(v->n, wi); /* Lambertian::Pdfw (bsdf.cpp:34-41) */
         return p <= 0.0 ? 1e-4 : p;
     }
@@ -1541,6 +1556,22 @@ static int bd_walk(const scene_t* sc, bvert_t* arr, int start, int maxv, v3 o, v
             pdf_next = pdf;
             double pr = INV_PI_D * dot(n, wo);
             pdf_rev = pr <= 0.0 ? 1e-4 : pr;
+        } else if (m->kind == MAT_GLOSSY) {
+            if (m->kd != 0.0 && rnd(g) <= m->kd_pdf) {
+                double cost, pdf;
+                w = lambert_sample(n, g, &cost, &pdf);
+                weight = cscale(m->color, m->kd * INV_PI_D * cost / (pdf * m->kd_pdf));
+                pdf_next = pdf;
+                double pr = INV_PI_D * dot(n, wo);
+                pdf_rev = pr <= 0.0 ? 1e-4 : pr;
+            } else {
+                v->flags = BV_DELTA | BV_PHONG;
+                if (!eye) break;
+                double cost, pdf;
+                w = phong_sample(m, wo, n, g, &cost, &pdf);
+                weight = cs(phong_f(m, n, w, wo) * m->ks * cost / (pdf * (1.0 - m->kd_pdf)));
+                delta = 1;
+            }
         } else if (m->kind == MAT_GGX) {
             w = ggx_sample(m, n, wo, g);
             double cost = dot(n, w);
@@ -1570,7 +1601,7 @@ static int bd_walk(const scene_t* sc, bvert_t* arr, int start, int maxv, v3 o, v
             }
             if (was_in) weight = cmul(weight, exp_tau(m->absorption, r.tmin));
         } else {
-            break; /* glossy / subsurface: not handled by the bidirectional tracer */
+            break; /* subsurface: not handled by the bidirectional tracer */
         }
         if (delta) v->flags |= BV_DELTA;
         if (nv >= 2) {
@@ -1765,6 +1796,7 @@ static void* bdpt_rows(void* arg)
                                 int di = i == t - 1 ? 0 : (E[i].flags & BV_DELTA) != 0;
                                 int dp = (E[i - 1].flags & BV_DELTA) != 0;
                                 if (!di && !dp) sum += pow(ri, j->beta);
+                                if (E[i].flags & BV_PHONG) break;
                             }
                             ri = 1.0;
                             for (i = s - 1; i >= 0; --i) {

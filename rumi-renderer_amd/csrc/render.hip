@@ -1346,8 +1346,18 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
 // dielectric) are never connected and enter the MIS sums with remapped
 // pdfs.  Stages: k_bdpt_eye, k_bdpt_light, k_bdpt_connect (one thread per
 // sample and strategy, shadow ray inline).
+//
+// A glossy vertex keeps the one lobe Glossy::BSDFSampling picks
+// (src/Material/glossy.cpp:56-78).  Lambertian lobe (probability kd_pdf): a
+// matte vertex whose f and throughput carry 1/kd_pdf (Glossy::Fr :80-90 over
+// _pdf_path).  Phong lobe: Dirac per Glossy::IsDirac (:35-41), sampled with
+// Phong::BSDFSampling's rules (bsdf.cpp:127-170: mirrored below-horizon
+// samples, a returned cosine of 1) -- the path tracer's integrand, which only
+// the eye side reproduces: a light walk ends at a Phong vertex (kVPhong) and
+// the MIS sums stop there, every strategy beyond needing a light walk
+// through it.
 
-constexpr uint32_t kVDelta = 1u, kVEmitter = 2u, kVCamera = 4u, kVLight = 8u;
+constexpr uint32_t kVDelta = 1u, kVEmitter = 2u, kVCamera = 4u, kVLight = 8u, kVPhong = 16u;
 
 struct BVert {
     float4 a;  // p.xyz, pdf_fwd (area)
@@ -1379,11 +1389,13 @@ __device__ inline V3 bv_n(const BVert& v) { return v3(v.b.x, v.b.y, v.b.z); }
 __device__ inline uint32_t bv_flags(const BVert& v) { return (uint32_t)__float_as_int(v.c.w); }
 __device__ inline int bv_mat(const BVert& v) { return __float_as_int(v.d.w); }
 
-// Non-delta BSDF value f(wi, wo) at a stored vertex (matte, GGX).
+// Non-delta BSDF value f(wi, wo) at a stored vertex (matte, the Lambertian
+// lobe of glossy, GGX).
 __device__ inline V3 bsdf_f(const SceneView& sv, const BVert& v, V3 wi, V3 wo) {
     const DevMaterial& m = sv.mats[bv_mat(v)];
     const V3 tex = v3(v.d.x, v.d.y, v.d.z);
     if (m.kind == RUMI_MAT_MATTE) return cmul(ld3(m.color), tex) * (kInvPiF * m.kd);
+    if (m.kind == RUMI_MAT_GLOSSY) return cmul(ld3(m.color), tex) * (kInvPiF * m.kd / m.kd_pdf);
     if (m.kind == RUMI_MAT_GGX) return cmul(ggx_f(m, bv_n(v), wi, wo), tex);
     return v3(0, 0, 0);
 }
@@ -1391,7 +1403,7 @@ __device__ inline V3 bsdf_f(const SceneView& sv, const BVert& v, V3 wi, V3 wo) {
 // Solid-angle pdf of sampling wi given wo at a stored vertex.
 __device__ inline float bsdf_pdf(const SceneView& sv, const BVert& v, V3 wi, V3 wo) {
     const DevMaterial& m = sv.mats[bv_mat(v)];
-    if (m.kind == RUMI_MAT_MATTE) {
+    if (m.kind == RUMI_MAT_MATTE || m.kind == RUMI_MAT_GLOSSY) {
         // Lambertian::Pdfw (bsdf.cpp:34-41)
         const float p = kInvPiF * dot(bv_n(v), wi);
         return p <= 0.0f ? 1e-4f : p;
@@ -1505,6 +1517,7 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
         V3 w;
         float pdf_next = 0.0f, pdf_rev = 0.0f;
         bool delta = false;
+        uint32_t vflags = 0u;
         V3 weight;  // f |cos| / pdf of this bounce
         if (m.kind == RUMI_MAT_MATTE) {
             float cost, pdf;
@@ -1513,6 +1526,25 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
             pdf_next = pdf;
             const float pr = kInvPiF * dot(n, wo);
             pdf_rev = pr <= 0.0f ? 1e-4f : pr;
+        } else if (m.kind == RUMI_MAT_GLOSSY) {
+            if (m.kd != 0.0f && g.uniform() <= m.kd_pdf) {
+                float cost, pdf;
+                w = lambert_sample(n, g, cost, pdf);
+                weight = cmul(ld3(m.color), tex) * (m.kd * kInvPiF * cost / (pdf * m.kd_pdf));
+                pdf_next = pdf;
+                const float pr = kInvPiF * dot(n, wo);
+                pdf_rev = pr <= 0.0f ? 1e-4f : pr;
+            } else {
+                if (!EYE) {  // a light walk ends at a Phong-lobe vertex
+                    store_vert(arr, mis, slot, p, pdf_fwd, n, beta, kVDelta | kVPhong, tex, mi);
+                    break;
+                }
+                float cost, pdf;
+                w = phong_sample(m, wo, n, g, cost, pdf);
+                weight = tex * (phong_f(m, n, w, wo) * m.ks * cost / (pdf * (1.0f - m.kd_pdf)));
+                delta = true;
+                vflags = kVPhong;
+            }
         } else if (m.kind == RUMI_MAT_GGX) {
             w = ggx_sample(m, n, wo, g);
             const float cost = dot(n, w);
@@ -1572,7 +1604,7 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
                 weight = cmul(weight, v3(expf(-m.absorption[0] * r.tmin), expf(-m.absorption[1] * r.tmin),
                                          expf(-m.absorption[2] * r.tmin)));
         }
-        store_vert(arr, mis, slot, p, pdf_fwd, n, beta, delta ? kVDelta : 0u, tex, mi);
+        store_vert(arr, mis, slot, p, pdf_fwd, n, beta, vflags | (delta ? kVDelta : 0u), tex, mi);
         // reverse pdf of the previous vertex (area measure at prev)
         if (nv >= 2) {
             const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
@@ -1795,6 +1827,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
                 const bool di = i == t - 1 ? false : ((uint32_t)__float_as_int(mi.z) & kVDelta) != 0;
                 const bool dprev = ((uint32_t)__float_as_int(mp.z) & kVDelta) != 0;
                 if (!di && !dprev) sum += mis_pow(ri, bp.beta);
+                if ((uint32_t)__float_as_int(mi.z) & kVPhong) break;
                 mi = mp;
             }
         }
@@ -1859,11 +1892,10 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     (void)ctx;
     if (sc->medium.bound != RUMI_MEDIUM_NONE)
         return fail(RUMI_ERR_UNSUPPORTED, "rumi_render: the BDPT integrator does not handle media");
-    const uint32_t bad = (1u << RUMI_MAT_GLOSSY) | (1u << RUMI_MAT_SUBSURFACE);
-    if (sc->material_kinds & bad)
+    if (sc->material_kinds & (1u << RUMI_MAT_SUBSURFACE))
         return fail(RUMI_ERR_UNSUPPORTED,
-                    "rumi_render: the BDPT integrator handles matte, GGX, specular, dielectric and "
-                    "emissive materials");
+                    "rumi_render: the BDPT integrator handles matte, glossy, GGX, specular, "
+                    "dielectric and emissive materials");
     const int nE = desc->n_eye > 0 ? desc->n_eye : desc->max_depth + 1;
     const int nL = desc->n_light > 0 ? desc->n_light : desc->max_depth + 1;
     if (nE < 2 || nE > 64 || nL < 1 || nL > 64)

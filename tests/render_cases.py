@@ -129,3 +129,21 @@ def textured_scene(width=16, height=16, seed=7):
         if s.get("light"):
             s["texture"] = 2
     return sc
+
+
+def glossy_box(width=16, height=16):
+    """The Cornell box without glass, its floor and large sphere Glossy
+    (Lambertian + Phong lobes, glossy.cpp:27-33) with different exponents."""
+    import importlib
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "rumi-renderer_amd"))
+    sc = importlib.import_module("scenes").diffuse_box(width, height)
+    sc["materials"].append(dict(kind="glossy", kd=0.15, ks=0.85, e=40.0, color=(0.8, 0.55, 0.3)))
+    sc["materials"].append(dict(kind="glossy", kd=0.4, ks=0.6, e=8.0, color=(0.7, 0.7, 0.7)))
+    shapes = [dict(s) for s in sc["shapes"]]
+    shapes[0]["material"] = shapes[1]["material"] = 6  # floor
+    shapes[-2]["material"] = 5                          # the large sphere
+    sc["shapes"] = shapes
+    return sc

@@ -175,11 +175,38 @@ def test_bdpt_config_scene_runs(rumi, ctx, scenes):
     assert img.mean() > 0.01 and st.samples == 64 * 64 * 16
 
 
-def test_bdpt_rejects_media_and_glossy(rumi, ctx, scenes):
+def test_bdpt_rejects_media_and_subsurface(rumi, ctx, scenes):
     scene = rumi.Scene(ctx, scenes.cornell_media(8, 8))
     with pytest.raises(rumi.RumiError):
         scene.render(8, 8, 1, 4, integrator=rumi.RUMI_INTEGRATOR_BDPT)
     scene.close()
+    sc = scenes.diffuse_box(8, 8)
+    sc["materials"][4] = dict(kind="subsurface", eta_in=1.3, eta_out=1.0, sigma_s=(1, 1, 1),
+                              sigma_a=(0.1, 0.1, 0.1), density=1.0)
+    scene = rumi.Scene(ctx, sc)
+    with pytest.raises(rumi.RumiError):
+        scene.render(8, 8, 1, 4, integrator=rumi.RUMI_INTEGRATOR_BDPT)
+    scene.close()
+
+
+def _blocks(imgs, n=4):
+    h, w = imgs[0].shape[:2]
+    return np.array([i.reshape(n, h // n, n, w // n, 3).mean(axis=(1, 3)) for i in imgs])
+
+
+def test_bdpt_glossy_equals_path_tracer_and_oracle(rumi, oracle, ctx):
+    """Glossy floor and sphere (Lambertian lobe connected, Phong lobe sampled
+    from the eye side only): GPU BDPT against the GPU path tracer and against
+    the oracle's BDPT, on 4 x 4 block means of the image."""
+    sc = rc.glossy_box(16, 16)
+    pt = _blocks([render(rumi, ctx, sc, 16, 16, 8192, 4, seed=50 + k)[0] for k in range(6)])
+    bd = _blocks([_bdpt(rumi, ctx, sc, 16, 16, 2048, 5, seed=60 + k) for k in range(6)])
+    orc = _blocks([oracle.render_bdpt(sc, 16, 16, 2048, 5, seed=70 + k, nthreads=8)
+                   for k in range(6)])
+    for other in (pt, orc):
+        z = (bd.mean(0) - other.mean(0)) / np.sqrt(bd.var(0, ddof=1) / 6 + other.var(0, ddof=1) / 6)
+        print("max |z|", np.abs(z).max())
+        assert np.all(np.abs(z) < 4.5), z
 
 
 def test_bdpt_workload_matches_oracle_reference(rumi, oracle, ctx, scenes):

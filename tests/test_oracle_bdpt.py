@@ -56,3 +56,25 @@ def test_bdpt_direct_lighting_form_factor(oracle):
     img = oracle.render_bdpt(sc, 1, 1, 200000, 2, seed=5, nthreads=1)
     want = rc.square_light_radiance(0.5, (1.0, 2.0, 3.0), 0.5, 1.0)
     np.testing.assert_allclose(img[0, 0], want, rtol=0.01)
+
+
+
+def block_means(imgs, n=4):
+    """Per-run means over an n x n grid of image blocks: (runs, n, n, 3)."""
+    h, w = imgs[0].shape[:2]
+    return np.array([i.reshape(n, h // n, n, w // n, 3).mean(axis=(1, 3)) for i in imgs])
+
+
+def test_bdpt_matches_pt_with_glossy(oracle):
+    """Glossy floor and sphere: Lambertian-lobe vertices are connected, Phong
+    vertices only sampled from the eye side, as the path tracer does.  Block
+    means resolve the glossy floor's highlight, which the image mean hides."""
+    import render_cases as rc
+    sc = rc.glossy_box(16, 16)
+    pt = block_means([oracle.render_pt(sc, 16, 16, 4096, 4, seed=500 + k, nthreads=8)[0]
+                      for k in range(6)])
+    bd = block_means([oracle.render_bdpt(sc, 16, 16, 2048, 5, seed=600 + k, nthreads=8)
+                      for k in range(6)])
+    z = (bd.mean(0) - pt.mean(0)) / np.sqrt(pt.var(0, ddof=1) / 6 + bd.var(0, ddof=1) / 6)
+    print("max |z|", np.abs(z).max())
+    assert np.all(np.abs(z) < 4.5), z
