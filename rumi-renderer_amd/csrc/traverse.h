@@ -193,8 +193,9 @@ __device__ __forceinline__ bool trav_begin(const RootInfo& root, TravState& s) {
            closer(s.r.tmin, tn);
 }
 
-// One traversal step: visits node or leaf s.cur; after a leaf, or a node with
-// no child to enter, pops entries until one is closer than the current hit.
+// One traversal step: visits node s.cur or the next primitive of leaf s.cur;
+// after a leaf's last primitive, or a node with no child to enter, pops
+// entries until one is closer than the current hit.
 // Returns false when the traversal is over (stack empty, or an any-hit query
 // found its hit).  MIXED enables sphere/rect leaf records (tag in c.z, rect
 // side table index in c.x, emissive flag in c.w).  Stack entries beyond
@@ -264,11 +265,16 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes,
         }
         if (descend) return true;
     } else {
+        // One primitive per step: a wave then waits on one fetch per step
+        // whether its lanes are at nodes or in leaves (a whole leaf per step
+        // would cost its count of fetch round trips for the entire wave).
+        // The leaf's remaining primitives stay encoded in s.cur, so the
+        // primitives are tested in the same order as a loop over the leaf.
         const int32_t enc = ~s.cur;
         const int32_t first = enc >> 4;
         const int32_t count = enc & 15;
-        for (int32_t k = 0; k < count; ++k) {
-            const float4* t4 = prims + 3 * (size_t)(first + k);
+        if (count > 0) {
+            const float4* t4 = prims + 3 * (size_t)first;
             const float4 ta = t4[0];
             const float4 tb = t4[1];
             const float4 tc = t4[2];
@@ -292,7 +298,12 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes,
                 s.hu = u;
                 s.hv = v;
             }
+            if (count > 1) {
+                s.cur = ~(((first + 1) << 4) | (count - 1));
+                return true;
+            }
         }
+        // an any-hit query ends after the leaf that found a hit (bvh.cpp:181-182)
         if (ANY && r.tmin > 0.0f) return false;
     }
     while (s.sp > 0) {
@@ -338,6 +349,17 @@ __device__ __forceinline__ int32_t traverse(const float4* __restrict__ nodes,
     return s.hit;
 }
 
+// Rays a wave takes from the work counter per atomic (before the tail).
+#ifndef RUMI_RUN_CHUNK
+#define RUMI_RUN_CHUNK 256
+#endif
+constexpr uint32_t kRunChunk = RUMI_RUN_CHUNK;
+// Lanes with a fetched ray that start together (see run_lanes).
+#ifndef RUMI_START_BATCH
+#define RUMI_START_BATCH 16
+#endif
+constexpr uint32_t kStartBatch = RUMI_START_BATCH;
+
 // Set bits of mask in the lanes below this one.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
@@ -346,12 +368,13 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 
 // Persistent-lane driver over a queue of n rays.  Rays differ in cost by two
 // orders of magnitude, so one ray per lane leaves a wave waiting on its
-// slowest lane.  Here every lane steps its own ray one node or leaf at a
-// time; a lane whose ray ends swaps in the ray it has already prefetched and,
-// in the same iteration, issues the loads of its next one, which complete
-// while the wave waits on its node fetches anyway.  Ray indices come from
-// *counter (zero before the launch) in wave chunks of 64, requested half a
-// chunk ahead.  Each ray's arithmetic is the one-ray-per-lane arithmetic, so
+// slowest lane.  Here every lane steps its own ray one node or primitive at
+// a time; a lane whose ray ends swaps in the ray it has already prefetched
+// and, in the same iteration, issues the loads of its next one, which
+// complete while the wave waits on its node fetches anyway.  Ray indices come
+// from *counter (zero before the launch) in wave chunks of kRunChunk (64 in
+// the tail), the next requested while the current one still holds up to
+// kRunChunk.  Each ray's arithmetic is the one-ray-per-lane arithmetic, so
 // results are bit-identical.
 //
 // Item (per lane, all inlined):
@@ -362,27 +385,44 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 //   void finish(const TravState&)  consume the result of the current ray
 template <class Item>
 __device__ __forceinline__ void run_lanes(uint32_t n, uint32_t* counter, Item& item) {
-    constexpr uint32_t kChunk = 64;
     const uint32_t lane = threadIdx.x & 63u;
+    // Every wave draws from one counter, whose atomics serialise; large
+    // chunks keep them rare, and chunks shrink to a wave's width once fewer
+    // than two rays per lane of the grid remain, to balance the tail.
+    const uint32_t tail = gridDim.x * blockDim.x * 2u;
     TravState s;
-    bool active = false, has_next = false, drained = false;
-    uint32_t cbase = 0, cleft = 0, pend_val = 0;  // wave-uniform chunk state
+    bool active = false, has_next = false, drained = false, unfinished = false;
+    uint32_t cbase = 0, cleft = 0, pend_val = 0, pend_sz = 0;  // wave-uniform chunk state
     bool pend = false;
     while (true) {
-        if (!active && has_next) {
-            has_next = false;
-            active = item.start(s);
-            if (!active) item.finish(s);
+        // Ending a ray and starting the next one run for few lanes at a time;
+        // they wait until kStartBatch lanes are ready (or nothing else is
+        // left to do), so the wave pays for those instructions less often.
+        const uint64_t ready = __ballot(!active && has_next);
+        if (ready != 0 && ((uint32_t)__popcll(ready) >= kStartBatch || __ballot(active) == 0 ||
+                           __ballot(drained) != 0)) {
+            if (!active) {
+                if (unfinished) {
+                    item.finish(s);
+                    unfinished = false;
+                }
+                if (has_next) {
+                    has_next = false;
+                    active = item.start(s);
+                    unfinished = !active;
+                }
+            }
         }
-        if (!pend && cleft < kChunk / 2) {
-            if (lane == 0) pend_val = atomicAdd(counter, kChunk);
+        if (!pend && cleft < kRunChunk) {
+            pend_sz = cbase + tail < n ? kRunChunk : 64u;
+            if (lane == 0) pend_val = atomicAdd(counter, pend_sz);
             pend = true;
         }
         const uint64_t want = __ballot(!has_next && !drained);
         if (want != 0) {
             if (cleft == 0) {
                 cbase = __shfl(pend_val, 0);
-                cleft = kChunk;
+                cleft = pend_sz;
                 pend = false;
             }
             const uint32_t need = (uint32_t)__popcll(want);
@@ -402,10 +442,11 @@ __device__ __forceinline__ void run_lanes(uint32_t n, uint32_t* counter, Item& i
         }
         if (active) {
             active = item.step(s);
-            if (!active) item.finish(s);
+            unfinished = !active;
         }
         if (__ballot(active || has_next || !drained) == 0) break;
     }
+    if (unfinished) item.finish(s);
 }
 
 }  // namespace dev
