@@ -112,8 +112,10 @@ typedef struct {
  *   any      = 0: closest hit (isect != NULL); 1: shadow ray (isect == NULL),
  *              return after the first leaf that produced a hit.
  *   ordered  = 0: reference order (push right, descend left).
- *            = 1: near child first by box entry distance (the GPU's order
- *              for SAH trees); same node and primitive tests.
+ *            = 1: near child first by box entry distance with postponed
+ *              leaves (the GPU's order for SAH trees): the same closest
+ *              distance; ids differ only between primitives at exactly
+ *              equal distance.
  * Returns the primitive index (scene order) or -1; ray->tmin holds t. */
 int or_travel(const or_bvh* bvh, const or_prim* prims, or_ray* ray, int any,
               int ordered, int in_object, or_stats* st);

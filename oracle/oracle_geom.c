@@ -495,6 +495,74 @@ static inline int tmin_accept(const or_ray* r, float tn)
     return r->tmin == 0.0f || r->tmin > tn;
 }
 
+/* Near-first order with postponed leaves (the GPU's order for SAH trees).
+ * One node to visit (cur) and one leaf under test advance together per
+ * step: the leaf's next primitive is tested first, then cur's children are
+ * tested against the updated tmin (nearer accepted child next, farther one
+ * stacked); a leaf reached while the leaf slot is free takes the slot and
+ * the next stack entry is popped, otherwise it waits in cur.  Restates the
+ * product's traversal order so hits and work counts compare exactly; the
+ * closest hit distance equals the reference order's (parity pinned against
+ * brute force). */
+static int travel_near(const or_bvh* b, const or_prim* prims, or_ray* ray, int any,
+                       int in_object, or_stats* st)
+{
+    stk_t stack[256];
+    int sp = 0, hit = -1;
+    int cur = b->root, leaf = -1, lpos = 0, lend = 0;
+    for (;;) {
+        int at_node = cur >= 0 && b->left[cur] >= 0;
+        if (leaf >= 0) {
+            int p = b->order[lpos];
+            if (st) st->prims++;
+            if (or_prim_intersect(&prims[p], ray, in_object))
+                hit = p;
+            if (++lpos == lend) {
+                leaf = -1;
+                if (any && ray->tmin > 0.0f)
+                    return hit;
+            }
+        }
+        if (at_node) {
+            if (st) st->nodes++;
+            int l = b->left[cur], r = b->right[cur];
+            float tl, tr;
+            int al = static_accept(b, l, ray, &tl) && tmin_accept(ray, tl);
+            int ar = static_accept(b, r, ray, &tr) && tmin_accept(ray, tr);
+            if (al && ar) {
+                int swap = tr < tl;
+                stack[sp].node = swap ? l : r;
+                stack[sp].tnear = swap ? tl : tr;
+                ++sp;
+                cur = swap ? r : l;
+            } else {
+                cur = al ? l : (ar ? r : -1);
+            }
+        }
+        for (;;) {
+            if (cur >= 0 && b->left[cur] < 0) {
+                if (b->count[cur] == 0) {
+                    cur = -1;
+                } else if (leaf >= 0) {
+                    break;
+                } else {
+                    leaf = cur;
+                    lpos = b->first[cur];
+                    lend = lpos + b->count[cur];
+                    cur = -1;
+                }
+            }
+            if (cur >= 0 || sp == 0)
+                break;
+            --sp;
+            if (tmin_accept(ray, stack[sp].tnear))
+                cur = stack[sp].node;
+        }
+        if (cur < 0 && leaf < 0)
+            return hit;
+    }
+}
+
 int or_travel(const or_bvh* b, const or_prim* prims, or_ray* ray, int any,
               int ordered, int in_object, or_stats* st)
 {
@@ -508,6 +576,8 @@ int or_travel(const or_bvh* b, const or_prim* prims, or_ray* ray, int any,
     float tn;
     if (!(static_accept(b, b->root, ray, &tn) && tmin_accept(ray, tn)))
         return -1;
+    if (ordered)
+        return travel_near(b, prims, ray, any, in_object, st);
     int cur = b->root;
     for (;;) {
         if (b->left[cur] < 0) {

@@ -173,7 +173,13 @@ struct TravState {
     int32_t sp;
     int32_t hit;
     float hu, hv;
+    int32_t leaf;  // near-first order: the leaf whose primitives are being tested (0 = none)
 };
+
+// "No entry" in TravState::cur (near-first order).  A leaf reference equals
+// it only for a leaf at primitive 2^27 - 16 holding 15 primitives, beyond
+// the 2^27 primitives a leaf reference can address.
+constexpr int32_t kNoRef = -2147483647 - 1;
 
 // Starts a traversal at the root: false when the ray has a zero direction or
 // misses the root box (BVHAccel::Travel's first node test, bvh.cpp:164-169).
@@ -181,6 +187,7 @@ __device__ __forceinline__ bool trav_begin(const RootInfo& root, TravState& s) {
     s.hit = -1;
     s.sp = 0;
     s.cur = root.ref;
+    s.leaf = 0;
     s.hu = 0.0f;
     s.hv = 0.0f;
     if (s.r.dx == 0.0f && s.r.dy == 0.0f && s.r.dz == 0.0f) return false;
@@ -191,6 +198,140 @@ __device__ __forceinline__ bool trav_begin(const RootInfo& root, TravState& s) {
     return box_static(root.lo[0], root.lo[1], root.lo[2], root.hi[0], root.hi[1], root.hi[2],
                       s.r.ox, s.r.oy, s.r.oz, s.ix, s.iy, s.iz, s.r.tmax, tn) &&
            closer(s.r.tmin, tn);
+}
+
+// Stack push/pop of (reference, entry distance): LDS for the first STACK
+// entries, private memory beyond (SPILL only).
+template <int BLOCK, int STACK, bool SPILL>
+__device__ __forceinline__ void stack_push(TravState& s, int32_t ref, float tn, int32_t* s_ref,
+                                           float* s_tn, uint32_t tid, int32_t* p_ref, float* p_tn) {
+    if (!SPILL || s.sp < STACK) {
+        s_ref[s.sp * BLOCK + tid] = ref;
+        s_tn[s.sp * BLOCK + tid] = tn;
+    } else {
+        p_ref[s.sp - STACK] = ref;
+        p_tn[s.sp - STACK] = tn;
+    }
+    ++s.sp;
+}
+
+template <int BLOCK, int STACK, bool SPILL>
+__device__ __forceinline__ void stack_pop(TravState& s, int32_t& ref, float& tn, const int32_t* s_ref,
+                                          const float* s_tn, uint32_t tid, const int32_t* p_ref,
+                                          const float* p_tn) {
+    --s.sp;
+    if (!SPILL || s.sp < STACK) {
+        ref = s_ref[s.sp * BLOCK + tid];
+        tn = s_tn[s.sp * BLOCK + tid];
+    } else {
+        ref = p_ref[s.sp - STACK];
+        tn = p_tn[s.sp - STACK];
+    }
+}
+
+// Near-first traversal with postponed leaves.  A lane holds one node to
+// visit (s.cur) and one leaf being tested (s.leaf), and each step advances
+// both: the next primitive of s.leaf is tested first (it may lower
+// r.tmin), then node s.cur's children are tested against r.tmin, the nearer
+// accepted child becoming s.cur and the farther pushed.  A leaf reached
+// while s.leaf is free moves there at once and traversal goes on with the
+// next stack entry; otherwise it waits in s.cur.  Node and primitive fetches
+// of a step are independent, so a wave waits on one fetch round trip per
+// step while its lanes do node and primitive work together.  The order
+// depends only on the ray (not on other lanes), so hits and work counts are
+// reproducible; the closest hit distance is that of every other order, and
+// only which of two primitives at exactly equal distance is reported can
+// differ from them.  Any-hit queries end after the leaf that found a hit.
+template <int BLOCK, int STACK, bool ANY, bool MIXED, bool COUNT, bool SPILL>
+__device__ __forceinline__ bool trav_step_near(const float4* __restrict__ nodes,
+                                               const float4* __restrict__ prims,
+                                               const float4* __restrict__ rects, TravState& s,
+                                               bool in_object, int32_t& nn, int32_t& np,
+                                               int32_t* s_ref, float* s_tn, uint32_t tid,
+                                               int32_t* p_ref, float* p_tn) {
+    Ray& r = s.r;
+    const bool at_node = s.cur >= 0;
+    const bool at_leaf = s.leaf != 0;
+    float4 a, b, c, d, ta, tb, tc;
+    if (at_node) {
+        const float4* np4 = nodes + 4 * (size_t)s.cur;
+        a = np4[0];
+        b = np4[1];
+        c = np4[2];
+        d = np4[3];
+    }
+    const int32_t enc = ~s.leaf;
+    if (at_leaf) {
+        const float4* t4 = prims + 3 * (size_t)(enc >> 4);
+        ta = t4[0];
+        tb = t4[1];
+        tc = t4[2];
+    }
+    if (at_leaf) {
+        if (COUNT) ++np;
+        float u = 0.0f, v = 0.0f;
+        bool ok;
+        if (!MIXED) {
+            ok = hit_triangle(ta, tb, tc, r, u, v);
+        } else {
+            const int32_t tag = __float_as_int(tc.z);
+            if (tag == kTagTri)
+                ok = hit_triangle(ta, tb, tc, r, u, v);
+            else if (tag == kTagSphere)
+                ok = hit_sphere(ta, r, in_object);
+            else
+                ok = hit_rect(rects + 4 * (size_t)__float_as_int(tc.x), __float_as_int(tc.w) != 0, r);
+        }
+        if (ok) {
+            s.hit = __float_as_int(tc.y);
+            s.hu = u;
+            s.hv = v;
+        }
+        const int32_t count = enc & 15;
+        if (count > 1) {
+            s.leaf = ~((((enc >> 4) + 1) << 4) | (count - 1));
+        } else {
+            s.leaf = 0;
+            if (ANY && r.tmin > 0.0f) return false;
+        }
+    }
+    if (at_node) {
+        if (COUNT) ++nn;
+        const int32_t ref0 = __float_as_int(d.x);
+        const int32_t ref1 = __float_as_int(d.y);
+        float tn0, tn1;
+        const bool a0 = box_static(a.x, a.y, a.z, a.w, b.x, b.y, r.ox, r.oy, r.oz, s.ix, s.iy, s.iz,
+                                   r.tmax, tn0) && closer(r.tmin, tn0);
+        const bool a1 = box_static(b.z, b.w, c.x, c.y, c.z, c.w, r.ox, r.oy, r.oz, s.ix, s.iy, s.iz,
+                                   r.tmax, tn1) && closer(r.tmin, tn1);
+        if (a0 && a1) {
+            const bool swap = tn1 < tn0;
+            s.cur = swap ? ref1 : ref0;
+            stack_push<BLOCK, STACK, SPILL>(s, swap ? ref0 : ref1, swap ? tn0 : tn1, s_ref, s_tn, tid,
+                                            p_ref, p_tn);
+        } else {
+            s.cur = a0 ? ref0 : (a1 ? ref1 : kNoRef);
+        }
+    }
+    // next node to visit; a leaf moves to the free leaf slot
+    while (true) {
+        if (s.cur != kNoRef && s.cur < 0) {
+            if ((~s.cur & 15) == 0) {  // an empty leaf
+                s.cur = kNoRef;
+            } else if (s.leaf != 0) {
+                break;
+            } else {
+                s.leaf = s.cur;
+                s.cur = kNoRef;
+            }
+        }
+        if (s.cur != kNoRef || s.sp == 0) break;
+        int32_t rr;
+        float t;
+        stack_pop<BLOCK, STACK, SPILL>(s, rr, t, s_ref, s_tn, tid, p_ref, p_tn);
+        if (closer(r.tmin, t)) s.cur = rr;
+    }
+    return s.cur != kNoRef || s.leaf != 0;
 }
 
 // One traversal step: visits node s.cur or the next primitive of leaf s.cur;
@@ -207,6 +348,9 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes,
                                           bool in_object, int32_t& nn, int32_t& np,
                                           int32_t* s_ref, float* s_tn, uint32_t tid,
                                           int32_t* p_ref, float* p_tn) {
+    if (ORDERED)
+        return trav_step_near<BLOCK, STACK, ANY, MIXED, COUNT, SPILL>(
+            nodes, prims, rects, s, in_object, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
     Ray& r = s.r;
     if (s.cur >= 0) {
         const float4* np4 = nodes + 4 * (size_t)s.cur;
