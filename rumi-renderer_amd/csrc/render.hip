@@ -286,10 +286,8 @@ struct ExtendItem {
     RootInfo root;
     int2* hits;
     float2* uvs;
-    int32_t* s_ref;
-    float* s_tn;
-    int32_t* p_ref;
-    float* p_tn;
+    int2* s_stk;
+    int2* p_stk;
     uint32_t tid, id, nid, nflags, nscene;
     bool in_obj;
     float4 n0, n1;
@@ -316,7 +314,7 @@ struct ExtendItem {
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
         return trav_step<kBlock, kStack, false, true, true, false, SPILL>(
-            nodes, prims, rects, s, in_obj, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+            nodes, prims, rects, s, in_obj, nn, np, s_stk, tid, p_stk);
     }
     __device__ void finish(const TravState& s) {
         hits[id] = make_int2(s.hit, __float_as_int(s.r.tmin));
@@ -328,11 +326,9 @@ template <bool SPILL, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* n_ptr,
                                                    uint32_t* counter, SceneView sv,
                                                    const SceneTab* tab, int2* hits, float2* uvs) {
-    __shared__ int32_t s_ref[kStack * kBlock];
-    __shared__ float s_tn[kStack * kBlock];
+    __shared__ int2 s_stk[kStack * kBlock];
     constexpr int kSpill = SPILL ? kMaxDepth - kStack : 1;
-    int32_t p_ref[kSpill];
-    float p_tn[kSpill];
+    int2 p_stk[kSpill];
     ExtendItem<SPILL, MULTI> it;
     it.buf = buf;
     it.tab = tab;
@@ -342,10 +338,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* 
     it.root = sv.root;
     it.hits = hits;
     it.uvs = uvs;
-    it.s_ref = s_ref;
-    it.s_tn = s_tn;
-    it.p_ref = p_ref;
-    it.p_tn = p_tn;
+    it.s_stk = s_stk;
+    it.p_stk = p_stk;
     it.tid = threadIdx.x;
     it.id = it.nid = it.nflags = it.nscene = 0;
     it.in_obj = false;
@@ -1253,10 +1247,8 @@ struct ShadowItem {
     float* film;
     AdjCtx adj;
     Grad7 G;
-    int32_t* s_ref;
-    float* s_tn;
-    int32_t* p_ref;
-    float* p_tn;
+    int2* s_stk;
+    int2* p_stk;
     uint32_t tid, id, nid, nscene;
     float4 n0, n1;
     __device__ void prefetch(uint32_t k) {
@@ -1280,7 +1272,7 @@ struct ShadowItem {
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
         return trav_step<kBlock, kStack, true, true, true, false, SPILL>(
-            nodes, prims, rects, s, false, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+            nodes, prims, rects, s, false, nn, np, s_stk, tid, p_stk);
     }
     __device__ void finish(const TravState& s) {
         // Scene::VisibilityTest: in shadow iff the any-hit query found a hit.
@@ -1307,11 +1299,9 @@ template <bool SPILL, int MODE, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t* n_ptr,
                                                    uint32_t* counter, SceneView sv,
                                                    const SceneTab* tab, float* film, AdjCtx adj) {
-    __shared__ int32_t s_ref[kStack * kBlock];
-    __shared__ float s_tn[kStack * kBlock];
+    __shared__ int2 s_stk[kStack * kBlock];
     constexpr int kSpill = SPILL ? kMaxDepth - kStack : 1;
-    int32_t p_ref[kSpill];
-    float p_tn[kSpill];
+    int2 p_stk[kSpill];
     ShadowItem<SPILL, MODE, MULTI> it;
     it.sh = sh;
     it.tab = tab;
@@ -1323,10 +1313,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
     it.adj = adj;
 #pragma unroll
     for (int k = 0; k < 7; ++k) it.G.v[k] = 0.0f;
-    it.s_ref = s_ref;
-    it.s_tn = s_tn;
-    it.p_ref = p_ref;
-    it.p_tn = p_tn;
+    it.s_stk = s_stk;
+    it.p_stk = p_stk;
     it.tid = threadIdx.x;
     it.id = it.nid = it.nscene = 0;
     run_lanes(*n_ptr, counter, it);
@@ -1482,7 +1470,7 @@ __device__ inline void store_vert(BVert* arr, float4* mis, uint32_t idx, V3 p, f
 template <bool EYE>
 __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_t nb, uint32_t j, int start,
                            int max_verts, V3 o, V3 d, V3 beta, float pdf_w, Rng& g,
-                           int32_t* s_ref, float* s_tn, uint32_t tid, bool spill) {
+                           int2* s_stk, uint32_t tid, bool spill) {
     int nv = start;
     bool in_obj = false;
     V3 prev_p = o;
@@ -1492,9 +1480,9 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
         int32_t nn = 0, np = 0;
         const int32_t prim =
             spill ? traverse<kBlock, kStack, false, true, true, false, true>(
-                        sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_ref, s_tn, tid)
+                        sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_stk, tid)
                   : traverse<kBlock, kStack, false, true, true, false, false>(
-                        sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_ref, s_tn, tid);
+                        sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_stk, tid);
         if (prim < 0) break;
         V3 p, n, tex;
         surface_at(sv, prim, o, d, r.tmin, bu, bv, p, n, tex);
@@ -1624,8 +1612,7 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
 
 __global__ __launch_bounds__(kBlock) void k_bdpt_eye(BdptBufs bb, uint64_t base, BdptParams bp,
                                                      SceneView sv, bool spill) {
-    __shared__ int32_t s_ref[kStack * kBlock];
-    __shared__ float s_tn[kStack * kBlock];
+    __shared__ int2 s_stk[kStack * kBlock];
     const uint32_t tid = threadIdx.x;
     const uint32_t j = blockIdx.x * kBlock + tid;
     if (j >= bb.nb) return;
@@ -1651,14 +1638,13 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_eye(BdptBufs bb, uint64_t base,
     // Pinhole::Pdfw: 1 / (cos^3 film_area) at plane distance 1
     const float pdf_dir = 1.0f / (l.z * l.z * l.z * bp.film_area);
     const int nv = random_walk<true>(sv, bb.eye, bb.eye_mis, bb.nb, j, 1, bp.n_eye, cp, d, v3(1, 1, 1), pdf_dir,
-                                     g, s_ref, s_tn, tid, spill);
+                                     g, s_stk, tid, spill);
     bb.n_eye[j] = (uint8_t)nv;
 }
 
 __global__ __launch_bounds__(kBlock) void k_bdpt_light(BdptBufs bb, uint64_t base, BdptParams bp,
                                                        SceneView sv, bool spill) {
-    __shared__ int32_t s_ref[kStack * kBlock];
-    __shared__ float s_tn[kStack * kBlock];
+    __shared__ int2 s_stk[kStack * kBlock];
     const uint32_t tid = threadIdx.x;
     const uint32_t j = blockIdx.x * kBlock + tid;
     if (j >= bb.nb) return;
@@ -1694,7 +1680,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_light(BdptBufs bb, uint64_t bas
     const V3 w = lambert_sample(n0, g, cost, pdf_dir);
     const V3 beta = le * (cost / (pdf_pos * pdf_dir));
     const int nv = random_walk<false>(sv, bb.light, bb.light_mis, bb.nb, j, 1, bp.n_light, p0, w, beta, pdf_dir, g,
-                                      s_ref, s_tn, tid, spill);
+                                      s_stk, tid, spill);
     bb.n_light[j] = (uint8_t)nv;
 }
 
@@ -1705,8 +1691,7 @@ __device__ inline float mis_pow(float r, float beta) { return beta == 1.0f ? r :
 __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t base, BdptParams bp,
                                                          SceneView sv, const int2* strategies,
                                                          uint32_t n_strat, float* film, bool spill) {
-    __shared__ int32_t s_ref[kStack * kBlock];
-    __shared__ float s_tn[kStack * kBlock];
+    __shared__ int2 s_stk[kStack * kBlock];
     const uint32_t tid = threadIdx.x;
     const uint64_t gid = (uint64_t)blockIdx.x * kBlock + tid;
     if (gid >= (uint64_t)n_strat * bb.nb) return;
@@ -1855,10 +1840,10 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
         int32_t nn = 0, np = 0;
         if (spill)
             traverse<kBlock, kStack, true, true, true, false, true>(sv.nodes, sv.prims, sv.rects, sv.root,
-                                                                    r, false, hu, hv, nn, np, s_ref, s_tn, tid);
+                                                                    r, false, hu, hv, nn, np, s_stk, tid);
         else
             traverse<kBlock, kStack, true, true, true, false, false>(sv.nodes, sv.prims, sv.rects, sv.root,
-                                                                     r, false, hu, hv, nn, np, s_ref, s_tn, tid);
+                                                                     r, false, hu, hv, nn, np, s_stk, tid);
         if (r.tmin > 0.0f) return;
     }
     film_add(film, pixel, c * w);

@@ -17,8 +17,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(
     const float4* __restrict__ tris, RootInfo root, int2* __restrict__ hits,
     float2* __restrict__ uv, int32_t* __restrict__ any_out, int32_t* __restrict__ cnt_nodes,
     int32_t* __restrict__ cnt_prims) {
-    __shared__ int32_t s_ref[STACK * BLOCK];
-    __shared__ float s_tn[STACK * BLOCK];
+    __shared__ int2 s_stk[STACK * BLOCK];
     const uint32_t tid = threadIdx.x;
     const uint32_t i = blockIdx.x * BLOCK + tid;
     if (i >= n) return;
@@ -28,7 +27,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(
     float hu = 0.0f, hv = 0.0f;
     int32_t nn = 0, np = 0;
     const int32_t hit = traverse<BLOCK, STACK, ANY, ORDERED, false, COUNT, SPILL>(
-        nodes, tris, nullptr, root, r, false, hu, hv, nn, np, s_ref, s_tn, tid);
+        nodes, tris, nullptr, root, r, false, hu, hv, nn, np, s_stk, tid);
     if (ANY && any_out) any_out[i] = hit;
     if (hits) hits[i] = make_int2(hit, __float_as_int(r.tmin));
     if (!ANY && uv) uv[i] = make_float2(hu, hv);
@@ -59,11 +58,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_lanes(
     const float4* __restrict__ tris, RootInfo root, int2* __restrict__ hits,
     float2* __restrict__ uv, int32_t* __restrict__ any_out, uint32_t* __restrict__ counter,
     int refill) {
-    __shared__ int32_t s_ref[STACK * BLOCK];
-    __shared__ float s_tn[STACK * BLOCK];
+    __shared__ int2 s_stk[STACK * BLOCK];
     constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
-    int32_t p_ref[kSpill];
-    float p_tn[kSpill];
+    int2 p_stk[kSpill];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     TravState s;
@@ -100,7 +97,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_lanes(
         }
         if (active) {
             active = trav_step<BLOCK, STACK, ANY, ORDERED, false, false, SPILL>(
-                nodes, tris, nullptr, s, false, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+                nodes, tris, nullptr, s, false, nn, np, s_stk, tid, p_stk);
             if (!active) finish_ray(id);
         }
     }
@@ -116,10 +113,8 @@ struct QueryItem {
     int2* hits;
     float2* uv;
     int32_t* any_out;
-    int32_t* s_ref;
-    float* s_tn;
-    int32_t* p_ref;
-    float* p_tn;
+    int2* s_stk;
+    int2* p_stk;
     uint32_t tid, id, nid;
     float4 n0, n1;
     __device__ void prefetch(uint32_t k) {
@@ -135,7 +130,7 @@ struct QueryItem {
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
         return trav_step<BLOCK, STACK, ANY, ORDERED, false, false, SPILL>(
-            nodes, tris, nullptr, s, false, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+            nodes, tris, nullptr, s, false, nn, np, s_stk, tid, p_stk);
     }
     __device__ void finish(const TravState& s) {
         if (ANY && any_out) any_out[id] = s.hit;
@@ -149,11 +144,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_prefetch(
     const float4* __restrict__ rays, uint32_t n, const float4* __restrict__ nodes,
     const float4* __restrict__ tris, RootInfo root, int2* __restrict__ hits,
     float2* __restrict__ uv, int32_t* __restrict__ any_out, uint32_t* __restrict__ counter) {
-    __shared__ int32_t s_ref[STACK * BLOCK];
-    __shared__ float s_tn[STACK * BLOCK];
+    __shared__ int2 s_stk[STACK * BLOCK];
     constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
-    int32_t p_ref[kSpill];
-    float p_tn[kSpill];
+    int2 p_stk[kSpill];
     QueryItem<BLOCK, STACK, ANY, ORDERED, SPILL> item;
     item.rays = rays;
     item.nodes = nodes;
@@ -162,10 +155,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_prefetch(
     item.hits = hits;
     item.uv = uv;
     item.any_out = any_out;
-    item.s_ref = s_ref;
-    item.s_tn = s_tn;
-    item.p_ref = p_ref;
-    item.p_tn = p_tn;
+    item.s_stk = s_stk;
+    item.p_stk = p_stk;
     item.tid = threadIdx.x;
     item.id = item.nid = 0;
     run_lanes(n, counter, item);

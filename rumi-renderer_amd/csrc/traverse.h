@@ -203,30 +203,33 @@ __device__ __forceinline__ bool trav_begin(const RootInfo& root, TravState& s) {
 // Stack push/pop of (reference, entry distance): LDS for the first STACK
 // entries, private memory beyond (SPILL only).
 template <int BLOCK, int STACK, bool SPILL>
-__device__ __forceinline__ void stack_push(TravState& s, int32_t ref, float tn, int32_t* s_ref,
-                                           float* s_tn, uint32_t tid, int32_t* p_ref, float* p_tn) {
+__device__ __forceinline__ void stack_push(TravState& s, int32_t ref, float tn, int2* s_stk,
+                                           uint32_t tid, int2* p_stk) {
+    const int2 e = make_int2(ref, __float_as_int(tn));
     if (!SPILL || s.sp < STACK) {
-        s_ref[s.sp * BLOCK + tid] = ref;
-        s_tn[s.sp * BLOCK + tid] = tn;
+        s_stk[s.sp * BLOCK + tid] = e;
     } else {
-        p_ref[s.sp - STACK] = ref;
-        p_tn[s.sp - STACK] = tn;
+        p_stk[s.sp - STACK] = e;
+        // keeps this private-memory access apart from the LDS one: merged,
+        // they become generic (flat) accesses on every push
+        __asm__ volatile("" ::: "memory");
     }
     ++s.sp;
 }
 
 template <int BLOCK, int STACK, bool SPILL>
-__device__ __forceinline__ void stack_pop(TravState& s, int32_t& ref, float& tn, const int32_t* s_ref,
-                                          const float* s_tn, uint32_t tid, const int32_t* p_ref,
-                                          const float* p_tn) {
+__device__ __forceinline__ void stack_pop(TravState& s, int32_t& ref, float& tn, const int2* s_stk,
+                                          uint32_t tid, const int2* p_stk) {
     --s.sp;
+    int2 e;
     if (!SPILL || s.sp < STACK) {
-        ref = s_ref[s.sp * BLOCK + tid];
-        tn = s_tn[s.sp * BLOCK + tid];
+        e = s_stk[s.sp * BLOCK + tid];
     } else {
-        ref = p_ref[s.sp - STACK];
-        tn = p_tn[s.sp - STACK];
+        e = p_stk[s.sp - STACK];
+        __asm__ volatile("" : "+v"(e.x), "+v"(e.y));  // as in stack_push
     }
+    ref = e.x;
+    tn = __int_as_float(e.y);
 }
 
 // Near-first traversal with postponed leaves.  A lane holds one node to
@@ -247,8 +250,7 @@ __device__ __forceinline__ bool trav_step_near(const float4* __restrict__ nodes,
                                                const float4* __restrict__ prims,
                                                const float4* __restrict__ rects, TravState& s,
                                                bool in_object, int32_t& nn, int32_t& np,
-                                               int32_t* s_ref, float* s_tn, uint32_t tid,
-                                               int32_t* p_ref, float* p_tn) {
+                                               int2* s_stk, uint32_t tid, int2* p_stk) {
     Ray& r = s.r;
     const bool at_node = s.cur >= 0;
     const bool at_leaf = s.leaf != 0;
@@ -307,8 +309,7 @@ __device__ __forceinline__ bool trav_step_near(const float4* __restrict__ nodes,
         if (a0 && a1) {
             const bool swap = tn1 < tn0;
             s.cur = swap ? ref1 : ref0;
-            stack_push<BLOCK, STACK, SPILL>(s, swap ? ref0 : ref1, swap ? tn0 : tn1, s_ref, s_tn, tid,
-                                            p_ref, p_tn);
+            stack_push<BLOCK, STACK, SPILL>(s, swap ? ref0 : ref1, swap ? tn0 : tn1, s_stk, tid, p_stk);
         } else {
             s.cur = a0 ? ref0 : (a1 ? ref1 : kNoRef);
         }
@@ -328,7 +329,7 @@ __device__ __forceinline__ bool trav_step_near(const float4* __restrict__ nodes,
         if (s.cur != kNoRef || s.sp == 0) break;
         int32_t rr;
         float t;
-        stack_pop<BLOCK, STACK, SPILL>(s, rr, t, s_ref, s_tn, tid, p_ref, p_tn);
+        stack_pop<BLOCK, STACK, SPILL>(s, rr, t, s_stk, tid, p_stk);
         if (closer(r.tmin, t)) s.cur = rr;
     }
     return s.cur != kNoRef || s.leaf != 0;
@@ -340,17 +341,16 @@ __device__ __forceinline__ bool trav_step_near(const float4* __restrict__ nodes,
 // Returns false when the traversal is over (stack empty, or an any-hit query
 // found its hit).  MIXED enables sphere/rect leaf records (tag in c.z, rect
 // side table index in c.x, emissive flag in c.w).  Stack entries beyond
-// STACK live in p_ref/p_tn (SPILL only).
+// STACK live in p_stk (SPILL only).
 template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool MIXED, bool COUNT, bool SPILL>
 __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes,
                                           const float4* __restrict__ prims,
                                           const float4* __restrict__ rects, TravState& s,
                                           bool in_object, int32_t& nn, int32_t& np,
-                                          int32_t* s_ref, float* s_tn, uint32_t tid,
-                                          int32_t* p_ref, float* p_tn) {
+                                          int2* s_stk, uint32_t tid, int2* p_stk) {
     if (ORDERED)
         return trav_step_near<BLOCK, STACK, ANY, MIXED, COUNT, SPILL>(
-            nodes, prims, rects, s, in_object, nn, np, s_ref, s_tn, tid, p_ref, p_tn);
+            nodes, prims, rects, s, in_object, nn, np, s_stk, tid, p_stk);
     Ray& r = s.r;
     if (s.cur >= 0) {
         const float4* np4 = nodes + 4 * (size_t)s.cur;
@@ -398,14 +398,7 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes,
                 descend = false;
         }
         if (do_push) {
-            if (!SPILL || s.sp < STACK) {
-                s_ref[s.sp * BLOCK + tid] = push_ref;
-                s_tn[s.sp * BLOCK + tid] = push_tn;
-            } else {
-                p_ref[s.sp - STACK] = push_ref;
-                p_tn[s.sp - STACK] = push_tn;
-            }
-            ++s.sp;
+            stack_push<BLOCK, STACK, SPILL>(s, push_ref, push_tn, s_stk, tid, p_stk);
         }
         if (descend) return true;
     } else {
@@ -451,16 +444,9 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes,
         if (ANY && r.tmin > 0.0f) return false;
     }
     while (s.sp > 0) {
-        --s.sp;
         int32_t rr;
         float t;
-        if (!SPILL || s.sp < STACK) {
-            rr = s_ref[s.sp * BLOCK + tid];
-            t = s_tn[s.sp * BLOCK + tid];
-        } else {
-            rr = p_ref[s.sp - STACK];
-            t = p_tn[s.sp - STACK];
-        }
+        stack_pop<BLOCK, STACK, SPILL>(s, rr, t, s_stk, tid, p_stk);
         if (closer(r.tmin, t)) {
             s.cur = rr;
             return true;
@@ -476,16 +462,15 @@ __device__ __forceinline__ int32_t traverse(const float4* __restrict__ nodes,
                                             const float4* __restrict__ prims,
                                             const float4* __restrict__ rects, const RootInfo& root,
                                             Ray& r, bool in_object, float& hu, float& hv,
-                                            int32_t& nn, int32_t& np, int32_t* s_ref, float* s_tn,
+                                            int32_t& nn, int32_t& np, int2* s_stk,
                                             uint32_t tid) {
     constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
-    int32_t p_ref[kSpill];
-    float p_tn[kSpill];
+    int2 p_stk[kSpill];
     TravState s;
     s.r = r;
     if (trav_begin(root, s))
         while (trav_step<BLOCK, STACK, ANY, ORDERED, MIXED, COUNT, SPILL>(
-            nodes, prims, rects, s, in_object, nn, np, s_ref, s_tn, tid, p_ref, p_tn)) {
+            nodes, prims, rects, s, in_object, nn, np, s_stk, tid, p_stk)) {
         }
     r = s.r;
     hu = s.hu;
