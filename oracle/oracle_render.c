@@ -373,12 +373,16 @@ static void build_bvh(scene_t* sc)
 }
 
 /* ------------------------------------------------------- shading helpers */
-/* OrthonormalBasis (src/Kernel/vector.h:70-95). */
+/* OrthonormalBasis (src/Kernel/vector.h:70-95), its +-Y special case (tangents X and
+ * -Z) taken within 1e-6 as the GPU renderer does: interpolated normals of
+ * axis-aligned surfaces land on or beside +-Y by rounding, which would
+ * otherwise decide the frame's orientation (a choice no distribution sampled
+ * in the frame depends on). */
 static void basis(v3 v0, v3* v1, v3* v2)
 {
-    if (v0.x == 0.0 && v0.z == 0.0 && (v0.y == 1.0 || v0.y == -1.0)) {
+    if (fabs(v0.x) <= 1e-6 && fabs(v0.z) <= 1e-6 && fabs(fabs(v0.y) - 1.0) <= 1e-6) {
         *v1 = mk(1, 0, 0);
-        *v2 = v0.y == 0.0 ? mk(0, 0, 1) : mk(0, 0, -1);
+        *v2 = mk(0, 0, -1);
     } else {
         v3 c = cross(v0, mk(0, 1, 0));
         *v1 = norm(c);
@@ -646,6 +650,21 @@ typedef struct {
     col le;
 } lsample_t;
 
+/* The stream of a vertex's light sample: a second PCG32 stream seeded from
+ * the path stream's state on arrival at the vertex (recorded in the forward
+ * loop), so the numbers the path itself draws do not depend on how many its
+ * light samples take -- the same numbering as the GPU renderer, which takes
+ * each light sample at its vertex rather than after the whole path. */
+#define NEE_MIX 0xDA3E39CB94B95BDBull
+static void nee_stream(pcg_t* n, const pcg_t* path, uint64_t key)
+{
+    n->state = 0;
+    n->inc = path->inc ^ 2u;
+    pcg_next(n);
+    n->state += key ^ NEE_MIX;
+    pcg_next(n);
+}
+
 static void light_sample(const scene_t* sc, pcg_t* g, lsample_t* ls)
 {
     int li = (int)(rnd(g) * sc->n_lights);
@@ -817,6 +836,7 @@ typedef struct {
     double vs, vsmax;
     col vtau, vpdf;
     double bu, bv;   /* triangle barycentrics of the hit */
+    uint64_t nee;    /* path-stream state on arrival: seeds the light sample */
 } pv_t;
 
 static void pv_clean(pv_t* v)
@@ -880,6 +900,7 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
     cur->in_object = in_object;
     for (di = 0; di < max_depth; ++di) {
         cur = &P[di];
+        cur->nee = g->state;
         ray_t r = {cur->o, cur->d, 0.0, 1e300};
         hitrec_t h = {-1, 0, 0, 0};
         int hit = travel(sc, &r, 0, cur->in_object, &h);
@@ -1024,7 +1045,9 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
             if (cur->vs == -1.0) continue;
             if (cur->vs > 0.0) {
                 double smax_s, cos_l;
-                col dir0 = medium_direct0(sc, cur->p, cur->d, g, &smax_s, &cos_l);
+                pcg_t gn;
+                nee_stream(&gn, g, cur->nee);
+                col dir0 = medium_direct0(sc, cur->p, cur->d, &gn, &smax_s, &cos_l);
                 direct = cmul(dir0, md->sigma_s);
                 double fp = md->g == 0.0 ? 1.0 / (4.0 * PI_D) : hg_eval(md->g, dot(cur->d, w));
                 col L_in = L;
@@ -1075,15 +1098,17 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
             const mat_t* m = &sc->mats[s->mat];
             int rinfo_in = cur->in_object;
             double smax_s = 0.0;
+            pcg_t gn;
+            nee_stream(&gn, g, cur->nee);
             switch (m->kind) {
             case MAT_MATTE:
                 fr = cscale(m->color, INV_PI_D * m->kd);
-                direct = matte_direct(sc, m, cur->p, n, g, &smax_s);
+                direct = matte_direct(sc, m, cur->p, n, &gn, &smax_s);
                 break;
             case MAT_GLOSSY:
                 if (m->kd != 0.0 && cur->rand <= m->kd_pdf) {
                     fr = cscale(m->color, INV_PI_D * m->kd);
-                    direct = cscale(matte_direct(sc, m, cur->p, n, g, &smax_s), 1.0 / m->kd_pdf);
+                    direct = cscale(matte_direct(sc, m, cur->p, n, &gn, &smax_s), 1.0 / m->kd_pdf);
                 } else {
                     fr = cs(phong_f(m, n, w, wo) * m->ks);
                 }

@@ -62,6 +62,22 @@ __device__ inline Rng rng_seed(uint64_t initstate, uint64_t initseq) {
     return g;
 }
 
+// The stream of a vertex's light sample: a second PCG32 stream seeded from
+// the path stream's state on arrival at the vertex, so the numbers the path
+// itself draws (distances, lobes, directions) do not depend on how many its
+// light samples take.  Both renderers of this path number their draws this
+// way, whatever order they evaluate light samples in.
+constexpr uint64_t kNeeMix = 0xDA3E39CB94B95BDBull;
+__device__ inline Rng nee_stream(const Rng& path, uint64_t key) {
+    Rng n;
+    n.state = 0;
+    n.inc = path.inc ^ 2u;
+    n.next();
+    n.state += key ^ kNeeMix;
+    n.next();
+    return n;
+}
+
 // ------------------------------------------------------------------ vectors
 struct V3 {
     float x, y, z;
@@ -85,11 +101,17 @@ __device__ inline V3 cmul(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.
 __device__ inline V3 cdiv(V3 a, V3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
 __device__ inline V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
 
-// OrthonormalBasis (src/Kernel/vector.h:70-95).
+// OrthonormalBasis (src/Kernel/vector.h:70-95).  Its special case for an axis of
+// exactly +-Y (tangents X and -Z) is taken within 1e-6 here: an interpolated
+// normal of an axis-aligned surface lands on or beside +-Y by rounding, and
+// beside it the general case turns rounding noise into the frame's
+// orientation.  The orientation about the axis changes no distribution
+// sampled in the frame (every sampler here is symmetric about its axis), only
+// which direction a given pair of random numbers maps to.
 __device__ inline void basis(V3 v0, V3& v1, V3& v2) {
-    if (v0.x == 0.0f && v0.z == 0.0f && (v0.y == 1.0f || v0.y == -1.0f)) {
+    if (fabsf(v0.x) <= 1e-6f && fabsf(v0.z) <= 1e-6f && fabsf(fabsf(v0.y) - 1.0f) <= 1e-6f) {
         v1 = v3(1, 0, 0);
-        v2 = v0.y == 0.0f ? v3(0, 0, 1) : v3(0, 0, -1);
+        v2 = v3(0, 0, -1);
     } else {
         v1 = normalize(cross(v0, v3(0, 1, 0)));
         if (is_nan(v1)) v1 = v3(1, 0, 0);
@@ -787,6 +809,7 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
     const uint32_t depth = flags >> 8;
     const bool last = (int)depth + 1 > prm.max_depth - 1;
     const DevMedium& md = sv.med;
+    const uint64_t nee_key = g.state;  // this vertex's light-sample stream
 
     // Replay: the radiance this sample still collects from here on.
     V3 w3 = v3(0, 0, 0);
@@ -837,9 +860,10 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
                 LightSample ls;
                 V3 wi;
                 float dist;
-                if (sample_light(sv, p, g, ls, wi, dist)) {
+                Rng gn = nee_stream(g, nee_key);
+                if (sample_light(sv, p, gn, ls, wi, dist)) {
                     float smax_s;
-                    const float tr = shadow_tr(md, p, wi, dist, g, smax_s);
+                    const float tr = shadow_tr(md, p, wi, dist, gn, smax_s);
                     const float cl = dot(d, wi);
                     const float fp = md.g == 0.0f ? 1.0f / (4.0f * kPiF) : hg_eval(md.g, cl);
                     const V3 c = cmul(cmul(beta, ls.le), ld3(md.sigma_s)) *
@@ -1015,7 +1039,8 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
     if (m.kind == RUMI_MAT_MATTE || (m.kind == RUMI_MAT_GLOSSY && m.kd != 0.0f && xi <= m.kd_pdf)) {
         V3 lo, wi;
         float dist, smax_s;
-        if (matte_direct(sv, m, p, n, g, lo, wi, dist, smax_s)) {
+        Rng gn = nee_stream(g, nee_key);
+        if (matte_direct(sv, m, p, n, gn, lo, wi, dist, smax_s)) {
             if (m.kind == RUMI_MAT_GLOSSY) lo = lo * (1.0f / m.kd_pdf);
             const V3 c = cmul(cmul(beta, tex), lo);
             if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f)

@@ -148,6 +148,28 @@ def test_medium_gradient_matches_oracle_adjoint(rumi, oracle, ctx, scenes):
     assert np.all(np.abs(z) < 4.0), z
 
 
+def test_gradients_equal_oracle_on_common_random_numbers(rumi, oracle, ctx, scenes):
+    """With one seed the GPU adjoint and the oracle's FP64 forward-mode
+    derivative follow the same paths (the renderers share their random-number
+    numbering), so their gradients agree to rounding: the 7 medium parameters
+    of the media workload at 512 x 512, 32 spp, depth 16 (the benchmark's
+    gradient) within 0.1% each, and the texel gradients of a textured scene
+    within 0.1% of the largest."""
+    w, spp, depth = 512, 32, 16
+    sc = _media_params(scenes, THETA, w)
+    dimg = np.random.default_rng(4).uniform(0.5, 1.5, (w, w, 3))
+    g = adjoint(rumi, ctx, sc, dimg, spp, depth, seed=7)[1]
+    c = oracle.adjoint_pt(sc, dimg, spp, depth, seed=7, nthreads=16)[1]
+    print("GPU", g, "\noracle", c)
+    np.testing.assert_allclose(g, c, rtol=1e-3)
+    sc = rc.textured_scene(16, 16)
+    dimg = np.random.default_rng(5).uniform(0.5, 1.5, (16, 16, 3))
+    gt = adjoint(rumi, ctx, sc, dimg, 256, 5, seed=9, textures=True)[2]
+    ct = oracle.adjoint_pt(sc, dimg, 256, 5, seed=9, nthreads=8, textures=True)[3]
+    assert np.abs(ct).max() > 0
+    np.testing.assert_allclose(gt, ct, rtol=0, atol=1e-3 * np.abs(ct).max())
+
+
 def test_medium_gradient_matches_finite_differences(rumi, ctx, scenes):
     """Central differences of the GPU renderer (step 10% of each parameter,
     common random numbers) against the adjoint, over independent seeds."""
