@@ -256,7 +256,9 @@ def render_legs(rumi, ctx, args, log):
         sec = min(times)
         sps = st.samples / sec
         # rel-MSE at equal spp against the committed 64^2 4096-spp reference:
-        # the GPU's renders here, the CPU's from the baseline leg below.
+        # the GPU's renders here, the CPU's from the baseline leg below, with
+        # the same seeds (the two number their random draws alike, so each
+        # pair of renders holds the same samples up to rounding).
         ref = np.load(os.path.join(ROOT, "tests", "golden", golden + ".npz"))
         sc64 = factory(64, 64)
         s64 = rumi.Scene(ctx, sc64)
@@ -273,7 +275,7 @@ def render_legs(rumi, ctx, args, log):
         c_vals = []
         t0 = time.perf_counter()
         for k in range(4):
-            ci, _ = oracle.render_pt(sc64, 64, 64, spp, depth, seed=300 + k, nthreads=1)
+            ci, _ = oracle.render_pt(sc64, 64, 64, spp, depth, seed=200 + k, nthreads=1)
             c_vals.append(rel_mse(ci, ref["mean"]))
         cpu_sec = time.perf_counter() - t0
         cpu_sps = 4 * 64 * 64 * spp / cpu_sec
@@ -285,7 +287,7 @@ def render_legs(rumi, ctx, args, log):
             "rays_per_s": round((st.extension_rays + st.shadow_rays) / sec, 1),
             "rel_mse_64px": {"gpu": float(np.mean(g_vals)), "cpu_oracle": float(np.mean(c_vals)),
                              "ratio": float(np.mean(g_vals) / np.mean(c_vals)),
-                             "spp": spp, "renders": 4},
+                             "spp": spp, "renders": 4, "same_seeds": True},
             "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
                              "kind": "port",
                              "sample": f"4 renders of 64x64 at {spp} spp, the oracle's FP64 "
@@ -564,7 +566,8 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     sps = st.samples / sec
     scene.close()
     # rel-MSE at equal spp against the committed 64^2 4096-spp oracle
-    # reference: the GPU's renders here, the CPU's from the baseline leg
+    # reference: the GPU's renders here, the CPU's from the baseline leg, with
+    # the same seeds (equal samples up to rounding)
     import numpy as np
     ref = np.load(os.path.join(ROOT, "tests", "golden", "bdpt64_l8.npz"))
     sc64 = scenes.cornell_bdpt(64, 64)
@@ -577,7 +580,7 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     s64.close()
     # CPU baseline leg: the oracle's FP64 bidirectional tracer timed on one
     # thread over a 32^2 render at the same spp; its rel-MSE at equal spp from
-    # six 64^2 renders on all host threads (single renders scatter by 2x)
+    # the six 64^2 renders on all host threads
     import oracle
     t0 = time.perf_counter()
     oracle.render_bdpt(scenes.cornell_bdpt(32, 32), 32, 32, spp, max_len, n_eye=9, n_light=9,
@@ -585,7 +588,7 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     cpu_sec = time.perf_counter() - t0
     cpu_sps = 32 * 32 * spp / cpu_sec
     c_vals = [rel_mse(oracle.render_bdpt(sc64, 64, 64, spp, max_len, n_eye=9, n_light=9,
-                                         mis_beta=2.0, seed=400 + k, nthreads=os.cpu_count() or 1),
+                                         mis_beta=2.0, seed=300 + k, nthreads=min(16, os.cpu_count() or 1)),
                       ref["mean"]) for k in range(6)]
     log(f"cornell_bdpt: {sps / 1e6:.3f} Msamples/s ({sec * 1e3:.1f} ms), CPU {cpu_sps:.0f} samples/s, "
         f"rel-MSE ratio {np.mean(g_vals) / np.mean(c_vals):.3f}")
@@ -594,7 +597,7 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
             "max_path_length": max_len, "mis_beta": 2.0, "triangles": 200000 + 10,
             "rel_mse_64px": {"gpu": float(np.mean(g_vals)), "cpu_oracle": float(np.mean(c_vals)),
                              "ratio": float(np.mean(g_vals) / np.mean(c_vals)), "spp": spp,
-                             "renders": [6, 6]},
+                             "renders": [6, 6], "same_seeds": True},
             "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
                              "kind": "port",
                              "sample": f"32x32 at {spp} spp, the oracle's FP64 bidirectional "

@@ -1681,6 +1681,8 @@ typedef struct {
     double* film;  /* per-thread accumulation, W*H*3 */
 } bjob_t;
 
+#define BDPT_LIGHT_MIX 0x2545F4914F6CDD1Dull
+
 static void* bdpt_rows(void* arg)
 {
     bjob_t* j = (bjob_t*)arg;
@@ -1693,9 +1695,12 @@ static void* bdpt_rows(void* arg)
     for (y = j->row0; y < j->row1; ++y)
         for (x = 0; x < j->W; ++x) {
             uint64_t pixel = (uint64_t)y * j->W + x;
-            pcg_t g;
-            pcg_seed(&g, j->seed, pixel);
+            pcg_t g, gl;
             for (smp = 0; smp < j->spp; ++smp) {
+                /* every sample's eye and light subpaths draw from their own
+                 * PCG32 streams, numbered as the GPU tracer numbers them */
+                pcg_seed(&g, j->seed + (uint64_t)smp * 0x9E3779B97F4A7C15ull, pixel);
+                pcg_seed(&gl, (j->seed ^ BDPT_LIGHT_MIX) + (uint64_t)smp * 0x9E3779B97F4A7C15ull, pixel);
                 /* eye subpath */
                 double sx = rnd(&g), sy = rnd(&g);
                 ray_t cr = camera_ray(cam, j->W, j->H, x, y, sx, sy);
@@ -1715,7 +1720,7 @@ static void* bdpt_rows(void* arg)
                 int nL = 0;
                 if (sc->n_lights > 0) {
                     lsample_t ls;
-                    light_sample(sc, &g, &ls);
+                    light_sample(sc, &gl, &ls);
                     Lv[0].p = ls.p;
                     Lv[0].n = ls.n;
                     Lv[0].pdf_fwd = ls.pdf;
@@ -1724,9 +1729,9 @@ static void* bdpt_rows(void* arg)
                     Lv[0].flags = BV_LIGHT;
                     Lv[0].mat = -1;
                     double cost, pdfd;
-                    v3 w = lambert_sample(ls.n, &g, &cost, &pdfd);
+                    v3 w = lambert_sample(ls.n, &gl, &cost, &pdfd);
                     col b = cscale(ls.le, cost / (ls.pdf * pdfd));
-                    nL = bd_walk(sc, Lv, 1, j->nL, ls.p, w, b, pdfd, &g, 0);
+                    nL = bd_walk(sc, Lv, 1, j->nL, ls.p, w, b, pdfd, &gl, 0);
                 }
                 col own = cs(0.0);
                 for (t = 1; t <= nE; ++t)

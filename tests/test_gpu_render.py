@@ -177,6 +177,25 @@ def test_bdpt_equals_path_tracer(rumi, ctx, scenes, beta):
     assert np.all(np.abs(z) < 3.5), z
 
 
+@pytest.mark.parametrize("case", ["cornell", "glossy", "ggx"])
+def test_bdpt_same_seed_matches_oracle_sample_for_sample(rumi, oracle, ctx, scenes, case):
+    """The GPU and oracle bidirectional tracers draw each sample's eye and
+    light subpaths from the same PCG32 streams, so with one seed they build
+    the same subpaths and connections: equal images up to FP32-vs-FP64
+    rounding (MIS weights included), far below a different seed's noise."""
+    w = 32
+    sc = {"cornell": lambda: scenes.cornell_box(w, w), "glossy": lambda: rc.glossy_box(w, w),
+          "ggx": lambda: scenes.cornell_bdpt(w, w, freq=6)}[case]()
+    kw = dict(mis_beta=2.0) if case == "ggx" else {}
+    c = oracle.render_bdpt(sc, w, w, 16, 6, seed=5, nthreads=8, **kw)
+    same = _bdpt(rumi, ctx, sc, w, w, 16, 6, seed=5, beta=kw.get("mis_beta", 1.0))
+    other = _bdpt(rumi, ctx, sc, w, w, 16, 6, seed=6, beta=kw.get("mis_beta", 1.0))
+    d_same = np.abs(same - c).sum() / np.abs(c).sum()
+    d_other = np.abs(other - c).sum() / np.abs(c).sum()
+    print(case, "relative L1, same seed", d_same, "other seed", d_other)
+    assert d_same < 1e-3 and d_other > 0.01, (d_same, d_other)
+
+
 def test_bdpt_direct_lighting_form_factor(rumi, ctx):
     sc = rc.square_light_scene()
     img = _bdpt(rumi, ctx, sc, 1, 1, 1 << 18, 2, seed=3)
