@@ -404,17 +404,17 @@ def render_bdpt(scene: dict, width: int, height: int, spp: int, max_len: int, n_
     """FP64 bidirectional path tracer (or_render_bdpt): mean radiance image."""
     L = lib()
     if not hasattr(L, "_bdpt_bound"):
-        L.or_render_bdpt.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_int,
-                                     c_int, c_int, c_int, c_double, ctypes.c_uint64, c_int, c_void_p]
+        L.or_render_bdpt.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,
+                                     c_int, c_int, c_int, c_int, c_int, c_double, ctypes.c_uint64,
+                                     c_int, c_void_p]
         L._bdpt_bound = True
     shapes, mats, cam, med = scene_arrays(scene)
-    if med is not None:
-        raise ValueError("the bidirectional tracer has no media")
     n_eye = n_eye or max_len + 1
     n_light = n_light or max_len + 1
     mean = np.zeros((height, width, 3), np.float64)
-    rc = L.or_render_bdpt(_p(shapes), shapes.shape[0], _p(mats), mats.shape[0], _p(cam), width,
-                          height, spp, n_eye, n_light, max_len, mis_beta, seed, nthreads, _p(mean))
+    rc = L.or_render_bdpt(_p(shapes), shapes.shape[0], _p(mats), mats.shape[0], _p(cam), _p(med),
+                          width, height, spp, n_eye, n_light, max_len, mis_beta, seed, nthreads,
+                          _p(mean))
     if rc != 0:
         raise RuntimeError(f"or_render_bdpt failed ({rc})")
     return mean

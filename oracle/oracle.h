@@ -195,8 +195,9 @@ int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
  * of at most n_light vertices, paths of at most max_len edges, MIS exponent
  * mis_beta.  Writes the per-pixel mean radiance. */
 int or_render_bdpt(const float* shapes, int n_shapes, const float* mats, int n_mats,
-                   const float* camera, int width, int height, int spp, int n_eye, int n_light,
-                   int max_len, double mis_beta, uint64_t seed, int nthreads, double* mean_rgb);
+                   const float* camera, const float* medium, int width, int height, int spp,
+                   int n_eye, int n_light, int max_len, double mis_beta, uint64_t seed,
+                   int nthreads, double* mean_rgb);
 
 #ifdef __cplusplus
 }

@@ -1506,6 +1506,7 @@ int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
 #define BV_CAM 4
 #define BV_LIGHT 8
 #define BV_PHONG 16
+#define BV_MEDIUM 32
 
 typedef struct {
     v3 p, n;
@@ -1514,15 +1515,57 @@ typedef struct {
     int flags, mat, prim;
 } bvert_t;
 
-static double to_area(double pdf_w, v3 from, v3 to, v3 n_to)
+/* Medium length of the segment a -> b (0 without a medium). */
+static double seg_medium(const scene_t* sc, v3 a, v3 b)
 {
-    v3 d = sub(to, from);
+    const medium_t* md = &sc->med;
+    if (!md->kind) return 0.0;
+    v3 dv = sub(b, a);
+    double dist = len(dv);
+    if (!(dist > 0.0)) return 0.0;
+    ray_t r = {a, mul(dv, 1.0 / dist), dist, 1e300};
+    double tn, tf;
+    if (!med_intersect(md, &r, &tn, &tf)) return 0.0;
+    double L = med_maxdist(dist, tn, tf);
+    return L > 0.0 ? L : 0.0;
+}
+
+/* Density of reaching a vertex across a segment of medium length L, as the
+ * path tracer samples segments (distance_sample): a surface vertex passes the
+ * medium (probability exp(-sbar L)), a medium vertex is a sampled distance
+ * that scattered (sbar (1 - A) exp(-sbar L)). */
+static double seg_density(const scene_t* sc, double L, int medium_vertex)
+{
+    const medium_t* md = &sc->med;
+    if (!md->kind) return 1.0;
+    double tr = exp(-md->avg_sigma_t * L);
+    return medium_vertex ? md->avg_sigma_t * (1.0 - md->absorb) * tr : tr;
+}
+
+/* Area (surface vertex) or volume (medium vertex) density at v of a direction
+ * sampled at `from` with solid-angle pdf pdf_w, the segment holding L of
+ * medium. */
+static double to_meas(const scene_t* sc, double pdf_w, v3 from, const bvert_t* v, double L)
+{
+    v3 d = sub(v->p, from);
     double d2 = dot(d, d);
-    return pdf_w * fabs(dot(n_to, d)) / (d2 * sqrt(d2));
+    int med = (v->flags & BV_MEDIUM) != 0;
+    double c = med ? 1.0 : fabs(dot(v->n, d)) / sqrt(d2);
+    return pdf_w * c / d2 * seg_density(sc, L, med);
+}
+
+/* Phase value (HomogeneousVolume::Fp without sigma_s) between the two
+ * directions leaving a medium vertex toward its neighbours: light arrives
+ * along -wi and leaves along wo. */
+static double phase_v(const scene_t* sc, v3 wi, v3 wo)
+{
+    double g = sc->med.g;
+    return g == 0.0 ? 1.0 / (4.0 * PI_D) : hg_eval(g, -dot(wi, wo));
 }
 
 static col bsdf_f_v(const scene_t* sc, const bvert_t* v, v3 wi, v3 wo)
 {
+    if (v->flags & BV_MEDIUM) return cscale(sc->med.sigma_s, phase_v(sc, wi, wo));
     const mat_t* m = &sc->mats[v->mat];
     if (m->kind == MAT_MATTE) return cscale(m->color, m->kd * INV_PI_D);
     /* only Lambertian-lobe glossy vertices are connected */
@@ -1533,6 +1576,7 @@ static col bsdf_f_v(const scene_t* sc, const bvert_t* v, v3 wi, v3 wo)
 
 static double bsdf_pdf_v(const scene_t* sc, const bvert_t* v, v3 wi, v3 wo)
 {
+    if (v->flags & BV_MEDIUM) return phase_v(sc, wi, wo);
     const mat_t* m = &sc->mats[v->mat];
     if (m->kind == MAT_MATTE || m->kind == MAT_GLOSSY) {
         double p = INV_PI_D * dot(v->n, wi); /* Lambertian::Pdfw (bsdf.cpp:34-41) */
@@ -1545,25 +1589,74 @@ static double bsdf_pdf_v(const scene_t* sc, const bvert_t* v, v3 wi, v3 wo)
 static int bd_walk(const scene_t* sc, bvert_t* arr, int start, int maxv, v3 o, v3 d, col beta,
                    double pdf_w, pcg_t* g, int eye)
 {
+    const medium_t* md = &sc->med;
     int nv = start, in_obj = 0;
     v3 prev_p = o;
     while (nv < maxv) {
         ray_t r = {o, d, 0.0, 1e300};
         hitrec_t h = {-1, 0, 0, 0};
         int hit = travel(sc, &r, 0, in_obj, &h);
-        if (hit < 0) break;
+        /* the medium along the segment, sampled as the path tracer does
+         * (pathtracer.cpp:156-178; bidirectionalpathtracer.cpp:452-473) */
+        double seg_L = 0.0;
+        int scatter = 0;
         v3 p, n;
+        if (md->kind && !in_obj) {
+            double tn, tf;
+            if (med_intersect(md, &r, &tn, &tf)) {
+                double smax = med_maxdist(r.tmin, tn, tf), sd = 0.0;
+                int absorbed = 0;
+                col tau, pdf;
+                scatter = distance_sample(md->sigma_t, md->avg_sigma_t, md->absorb, smax, g, &sd,
+                                          &absorbed, &tau, &pdf);
+                if (absorbed) break;
+                beta = cmul(beta, cdiv(tau, pdf));
+                if (scatter) {
+                    seg_L = sd;
+                    p = add(r.o, mul(r.d, tn < 0.0 ? sd : tn + sd));
+                } else {
+                    seg_L = smax;
+                }
+            }
+        }
+        if (!scatter && hit < 0) break;
+        if (scatter) {
+            /* a medium vertex: HomogeneousVolume::PhaseSampling, Fp / pdf = sigma_s */
+            bvert_t* v = &arr[nv];
+            v->p = p;
+            v->n = d; /* no surface: the arrival direction */
+            v->pdf_rev = 0.0;
+            v->beta = beta;
+            v->flags = BV_MEDIUM;
+            v->mat = -1;
+            v->prim = -1;
+            v->pdf_fwd = to_meas(sc, pdf_w, prev_p, v, seg_L);
+            ++nv;
+            double pdf_ph;
+            v3 w = phase_sample(md->g, d, g, &pdf_ph);
+            if (nv >= 2) {
+                bvert_t* pv = &arr[nv - 2];
+                pv->pdf_rev = to_meas(sc, pdf_ph, p, pv, seg_L);
+            }
+            beta = cmul(beta, md->sigma_s);
+            pdf_w = pdf_ph;
+            prev_p = p;
+            o = p;
+            d = w;
+            in_obj = 0;
+            continue;
+        }
         surface(sc, &h, &r, &p, &n);
         const mat_t* m = &sc->mats[sc->shapes[hit].mat];
         bvert_t* v = &arr[nv];
         v->p = p;
         v->n = n;
-        v->pdf_fwd = to_area(pdf_w, prev_p, p, n);
         v->pdf_rev = 0.0;
         v->beta = beta;
         v->flags = 0;
         v->mat = sc->shapes[hit].mat;
         v->prim = hit;
+        v->pdf_fwd = to_meas(sc, pdf_w, prev_p, v, seg_L);
         ++nv;
         if (m->kind == MAT_EMISSIVE) {
             v->flags = BV_EMIT;
@@ -1631,7 +1724,7 @@ static int bd_walk(const scene_t* sc, bvert_t* arr, int start, int maxv, v3 o, v
         if (delta) v->flags |= BV_DELTA;
         if (nv >= 2) {
             bvert_t* pv = &arr[nv - 2];
-            pv->pdf_rev = delta ? 0.0 : to_area(pdf_rev, p, prev_p, pv->n);
+            pv->pdf_rev = delta ? 0.0 : to_meas(sc, pdf_rev, p, pv, seg_L);
         }
         beta = cmul(beta, weight);
         pdf_w = delta ? 0.0 : pdf_next;
@@ -1682,6 +1775,7 @@ typedef struct {
 } bjob_t;
 
 #define BDPT_LIGHT_MIX 0x2545F4914F6CDD1Dull
+#define BDPT_CONN_MIX 0x9FB21C651E98DF25ull
 
 static void* bdpt_rows(void* arg)
 {
@@ -1743,6 +1837,7 @@ static void* bdpt_rows(void* arg)
                         col c;
                         v3 va, vb;
                         int vis = 0;
+                        double medL = 0.0; /* medium length of the connecting segment */
                         if (s == 0) {
                             if (!(pt->flags & BV_EMIT) || t < 2) continue;
                             const bvert_t* pm = &E[t - 2];
@@ -1759,7 +1854,8 @@ static void* bdpt_rows(void* arg)
                                                                   : 4.0 * PI_D * sh->r * sh->r;
                                 pt_rev = (1.0 / sc->n_lights) * (1.0 / area);
                                 if (!(pm->flags & BV_CAM))
-                                    ptm_rev = to_area(INV_PI_D * dot(pt->n, to_prev), pt->p, pm->p, pm->n);
+                                    ptm_rev = to_meas(sc, INV_PI_D * dot(pt->n, to_prev), pt->p, pm,
+                                                      seg_medium(sc, pt->p, pm->p));
                             }
                         } else {
                             const bvert_t* qs = &Lv[s - 1];
@@ -1771,19 +1867,21 @@ static void* bdpt_rows(void* arg)
                                 v3 to_cam = neg(dc);
                                 v3 dd = sub(qs->p, cam->p);
                                 double d2 = dot(dd, dd);
-                                double cos_q = dot(qs->n, to_cam);
+                                double cos_q = (qs->flags & BV_MEDIUM) ? 1.0 : dot(qs->n, to_cam);
                                 if (!(cos_q > 0.0)) continue;
                                 col fL = cs(1.0);
                                 if (s >= 2) {
                                     const bvert_t* qm = &Lv[s - 2];
                                     v3 to_qm = norm(sub(qm->p, qs->p));
                                     fL = bsdf_f_v(sc, qs, to_cam, to_qm);
-                                    qsm_rev = to_area(bsdf_pdf_v(sc, qs, to_qm, to_cam), qs->p, qm->p, qm->n);
+                                    qsm_rev = to_meas(sc, bsdf_pdf_v(sc, qs, to_qm, to_cam), qs->p, qm,
+                                                      seg_medium(sc, qs->p, qm->p));
                                 }
                                 double We = 1.0 / (film_area * cos_c * cos_c * cos_c * cos_c);
                                 double G = cos_q * cos_c / d2;
                                 c = cscale(cmul(qs->beta, fL), We * G);
-                                qs_rev = to_area(1.0 / (cos_c * cos_c * cos_c * film_area), cam->p, qs->p, qs->n);
+                                medL = seg_medium(sc, qs->p, cam->p);
+                                qs_rev = to_meas(sc, 1.0 / (cos_c * cos_c * cos_c * film_area), cam->p, qs, medL);
                                 va = qs->p;
                                 vb = cam->p;
                                 vis = 1;
@@ -1793,24 +1891,28 @@ static void* bdpt_rows(void* arg)
                                 v3 dpq = sub(qs->p, pt->p);
                                 double d2 = dot(dpq, dpq);
                                 v3 wpq = mul(dpq, 1.0 / sqrt(d2));
-                                double cos_p = dot(pt->n, wpq), cos_q = -dot(qs->n, wpq);
+                                double cos_p = (pt->flags & BV_MEDIUM) ? 1.0 : dot(pt->n, wpq);
+                                double cos_q = (qs->flags & BV_MEDIUM) ? 1.0 : -dot(qs->n, wpq);
                                 if (!(cos_p > 0.0 && cos_q > 0.0)) continue;
                                 v3 to_pm = norm(sub(pm->p, pt->p));
                                 col fE = bsdf_f_v(sc, pt, wpq, to_pm);
                                 col fL = cs(1.0);
+                                medL = seg_medium(sc, pt->p, qs->p);
                                 if (s == 1) {
-                                    pt_rev = to_area(INV_PI_D * cos_q, qs->p, pt->p, pt->n);
+                                    pt_rev = to_meas(sc, INV_PI_D * cos_q, qs->p, pt, medL);
                                 } else {
                                     const bvert_t* qm = &Lv[s - 2];
                                     v3 to_qm = norm(sub(qm->p, qs->p));
                                     fL = bsdf_f_v(sc, qs, neg(wpq), to_qm);
-                                    pt_rev = to_area(bsdf_pdf_v(sc, qs, neg(wpq), to_qm), qs->p, pt->p, pt->n);
-                                    qsm_rev = to_area(bsdf_pdf_v(sc, qs, to_qm, neg(wpq)), qs->p, qm->p, qm->n);
+                                    pt_rev = to_meas(sc, bsdf_pdf_v(sc, qs, neg(wpq), to_qm), qs->p, pt, medL);
+                                    qsm_rev = to_meas(sc, bsdf_pdf_v(sc, qs, to_qm, neg(wpq)), qs->p, qm,
+                                                      seg_medium(sc, qs->p, qm->p));
                                 }
                                 c = cscale(cmul(cmul(cmul(pt->beta, fE), fL), qs->beta), cos_p * cos_q / d2);
-                                qs_rev = to_area(bsdf_pdf_v(sc, pt, wpq, to_pm), pt->p, qs->p, qs->n);
+                                qs_rev = to_meas(sc, bsdf_pdf_v(sc, pt, wpq, to_pm), pt->p, qs, medL);
                                 if (!(pm->flags & BV_CAM))
-                                    ptm_rev = to_area(bsdf_pdf_v(sc, pt, to_pm, wpq), pt->p, pm->p, pm->n);
+                                    ptm_rev = to_meas(sc, bsdf_pdf_v(sc, pt, to_pm, wpq), pt->p, pm,
+                                                      seg_medium(sc, pt->p, pm->p));
                                 va = qs->p;
                                 vb = pt->p;
                                 vis = 1;
@@ -1838,6 +1940,17 @@ static void* bdpt_rows(void* arg)
                             }
                             w = 1.0 / (1.0 + sum);
                         }
+                        /* the connection's medium: a grey Bernoulli survival, as
+                         * VolumetricTest (bidirectionalpathtracer.cpp:1050-1062)
+                         * and the path tracer's shadow rays; one number from the
+                         * connection's own stream */
+                        if (medL > 0.0) {
+                            pcg_t gc;
+                            uint64_t k = (uint64_t)smp * (uint64_t)j->W * (uint64_t)j->H + pixel;
+                            pcg_seed(&gc, (j->seed ^ BDPT_CONN_MIX) + k * 0x9E3779B97F4A7C15ull,
+                                     ((uint64_t)s << 8) | (uint64_t)t);
+                            if (rnd(&gc) < 1.0 - exp(-sc->med.avg_sigma_t * medL)) continue;
+                        }
                         if (vis && !visible(sc, va, vb)) continue;
                         c = cscale(c, w);
                         if (cnan(c)) continue;
@@ -1862,12 +1975,13 @@ static void* bdpt_rows(void* arg)
 }
 
 int or_render_bdpt(const float* shapes, int n_shapes, const float* mats, int n_mats,
-                   const float* camera, int width, int height, int spp, int n_eye, int n_light,
-                   int max_len, double mis_beta, uint64_t seed, int nthreads, double* mean_rgb)
+                   const float* camera, const float* medium, int width, int height, int spp,
+                   int n_eye, int n_light, int max_len, double mis_beta, uint64_t seed,
+                   int nthreads, double* mean_rgb)
 {
     if (width <= 0 || height <= 0 || spp <= 0 || n_eye < 2 || n_light < 1 || !mean_rgb) return 1;
     scene_t sc;
-    load_scene(&sc, shapes, n_shapes, mats, n_mats, camera, 0);
+    load_scene(&sc, shapes, n_shapes, mats, n_mats, camera, medium);
     int* light_of = (int*)malloc(sizeof(int) * (size_t)(n_shapes > 0 ? n_shapes : 1));
     int i, t;
     for (i = 0; i < n_shapes; ++i) light_of[i] = -1;

@@ -1370,7 +1370,9 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
 // the MIS sums stop there, every strategy beyond needing a light walk
 // through it.
 
-constexpr uint32_t kVDelta = 1u, kVEmitter = 2u, kVCamera = 4u, kVLight = 8u, kVPhong = 16u;
+constexpr uint32_t kVDelta = 1u, kVEmitter = 2u, kVCamera = 4u, kVLight = 8u, kVPhong = 16u,
+                   kVMedium = 32u;
+constexpr uint64_t kConnMix = 0x9FB21C651E98DF25ull;  // connection streams
 
 struct BVert {
     float4 a;  // p.xyz, pdf_fwd (area)
@@ -1402,9 +1404,52 @@ __device__ inline V3 bv_n(const BVert& v) { return v3(v.b.x, v.b.y, v.b.z); }
 __device__ inline uint32_t bv_flags(const BVert& v) { return (uint32_t)__float_as_int(v.c.w); }
 __device__ inline int bv_mat(const BVert& v) { return __float_as_int(v.d.w); }
 
+// Medium length of the segment a -> b (0 without a medium).
+__device__ inline float seg_medium(const DevMedium& md, V3 a, V3 b) {
+    if (!md.bound) return 0.0f;
+    const V3 dv = b - a;
+    const float dist = length(dv);
+    if (!(dist > 0.0f)) return 0.0f;
+    float tn, tf;
+    if (!med_intersect(md, a, dv * (1.0f / dist), dist, tn, tf)) return 0.0f;
+    return fmaxf(med_maxdist(dist, tn, tf), 0.0f);
+}
+
+// Density of reaching a vertex across a segment of medium length L, as the
+// path tracer samples segments (distance_sample): a surface vertex passes the
+// medium (probability exp(-sbar L)), a medium vertex is a sampled distance
+// that scattered (sbar (1 - A) exp(-sbar L)).
+__device__ inline float seg_density(const DevMedium& md, float L, bool medium_vertex) {
+    if (!md.bound) return 1.0f;
+    const float tr = expf(-md.avg_sigma_t * L);
+    return medium_vertex ? md.avg_sigma_t * (1.0f - md.absorb) * tr : tr;
+}
+
+// Area (surface vertex) or volume (medium vertex) density at p of a direction
+// sampled at `from` with solid-angle pdf pdf_w, the segment holding L of medium.
+__device__ inline float to_meas(const DevMedium& md, float pdf_w, V3 from, V3 p, V3 n, bool medium,
+                                float L) {
+    const V3 d = p - from;
+    const float d2 = dot(d, d);
+    const float c = medium ? 1.0f : fabsf(dot(n, d)) / sqrtf(d2);
+    return pdf_w * c / d2 * seg_density(md, L, medium);
+}
+
+__device__ inline float to_meas(const DevMedium& md, float pdf_w, V3 from, const BVert& v, float L) {
+    return to_meas(md, pdf_w, from, bv_p(v), bv_n(v), (bv_flags(v) & kVMedium) != 0, L);
+}
+
+// Phase value (HomogeneousVolume::Fp without sigma_s) between the directions
+// leaving a medium vertex toward its neighbours: light arrives along -wi and
+// leaves along wo.
+__device__ inline float phase_v(const DevMedium& md, V3 wi, V3 wo) {
+    return md.g == 0.0f ? 1.0f / (4.0f * kPiF) : hg_eval(md.g, -dot(wi, wo));
+}
+
 // Non-delta BSDF value f(wi, wo) at a stored vertex (matte, the Lambertian
-// lobe of glossy, GGX).
+// lobe of glossy, GGX; sigma_s times the phase value in a medium).
 __device__ inline V3 bsdf_f(const SceneView& sv, const BVert& v, V3 wi, V3 wo) {
+    if (bv_flags(v) & kVMedium) return ld3(sv.med.sigma_s) * phase_v(sv.med, wi, wo);
     const DevMaterial& m = sv.mats[bv_mat(v)];
     const V3 tex = v3(v.d.x, v.d.y, v.d.z);
     if (m.kind == RUMI_MAT_MATTE) return cmul(ld3(m.color), tex) * (kInvPiF * m.kd);
@@ -1415,6 +1460,7 @@ __device__ inline V3 bsdf_f(const SceneView& sv, const BVert& v, V3 wi, V3 wo) {
 
 // Solid-angle pdf of sampling wi given wo at a stored vertex.
 __device__ inline float bsdf_pdf(const SceneView& sv, const BVert& v, V3 wi, V3 wo) {
+    if (bv_flags(v) & kVMedium) return phase_v(sv.med, wi, wo);
     const DevMaterial& m = sv.mats[bv_mat(v)];
     if (m.kind == RUMI_MAT_MATTE || m.kind == RUMI_MAT_GLOSSY) {
         // Lambertian::Pdfw (bsdf.cpp:34-41)
@@ -1423,13 +1469,6 @@ __device__ inline float bsdf_pdf(const SceneView& sv, const BVert& v, V3 wi, V3 
     }
     if (m.kind == RUMI_MAT_GGX) return ggx_pdf(m, bv_n(v), wi, wo);
     return 0.0f;
-}
-
-// Area density at `to` of a direction sampled at `from` with solid-angle pdf.
-__device__ inline float to_area(float pdf_w, V3 from, V3 to, V3 n_to) {
-    const V3 d = to - from;
-    const float d2 = dot(d, d);
-    return pdf_w * fabsf(dot(n_to, d)) / (d2 * sqrtf(d2));
 }
 
 // Camera direction pdf (Pinhole::Pdfw, pinhole.cpp:58-62) and importance
@@ -1508,12 +1547,52 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
                         sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_stk, tid)
                   : traverse<kBlock, kStack, false, true, true, false, false>(
                         sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_stk, tid);
+        // the medium along the segment, sampled as the path tracer does
+        // (pathtracer.cpp:156-178; bidirectionalpathtracer.cpp:452-473)
+        const DevMedium& md = sv.med;
+        float seg_L = 0.0f;
+        if (md.bound && !in_obj) {
+            const float thit = prim >= 0 ? r.tmin : 0.0f;
+            float tn, tf;
+            if (med_intersect(md, o, d, thit, tn, tf)) {
+                const float smax = med_maxdist(thit, tn, tf);
+                float sd = 0.0f;
+                V3 ratio;
+                const int ev = distance_sample(md.sigma_t, md.avg_sigma_t, md.absorb, smax, g, sd, ratio);
+                if (ev == kAbsorb) break;
+                beta = cmul(beta, ratio);
+                if (ev == kScatter) {
+                    // a medium vertex: HomogeneousVolume::PhaseSampling, Fp / pdf = sigma_s
+                    const V3 mp = o + d * (tn < 0.0f ? sd : tn + sd);
+                    const uint32_t mslot = (uint32_t)nv * nb + j;
+                    store_vert(arr, mis, mslot, mp, to_meas(md, pdf_w, prev_p, mp, d, true, sd), d, beta,
+                               kVMedium, v3(1, 1, 1), -1);
+                    ++nv;
+                    float pdf_ph;
+                    const V3 w = phase_sample(md.g, d, g, pdf_ph);
+                    if (nv >= 2) {
+                        const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
+                        BVert& pv = arr[pi];
+                        pv.b.w = to_meas(md, pdf_ph, mp, pv, sd);
+                        mis[pi].y = pv.b.w;
+                    }
+                    beta = cmul(beta, ld3(md.sigma_s));
+                    pdf_w = pdf_ph;
+                    prev_p = mp;
+                    o = mp;
+                    d = w;
+                    in_obj = false;
+                    continue;
+                }
+                seg_L = smax;
+            }
+        }
         if (prim < 0) break;
         V3 p, n, tex;
         surface_at(sv, prim, o, d, r.tmin, bu, bv, p, n, tex);
         const int mi = sv.shapes[prim].info[1];
         const DevMaterial& m = sv.mats[mi];
-        const float pdf_fwd = to_area(pdf_w, prev_p, p, n);
+        const float pdf_fwd = to_meas(md, pdf_w, prev_p, p, n, false, seg_L);
         if (m.kind == RUMI_MAT_EMISSIVE) {
             // The walk ends on an emitter (bidirectionalpathtracer.cpp:693-699).
             // an emitter vertex is never scattered from, so its reverse-pdf
@@ -1622,7 +1701,7 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
         if (nv >= 2) {
             const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
             BVert& pv = arr[pi];
-            pv.b.w = delta ? 0.0f : to_area(pdf_rev, p, prev_p, v3(pv.b.x, pv.b.y, pv.b.z));
+            pv.b.w = delta ? 0.0f : to_meas(md, pdf_rev, p, pv, seg_L);
             mis[pi].y = pv.b.w;
         }
         beta = cmul(beta, weight);
@@ -1736,6 +1815,8 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
     float pt_rev = 0.0f, ptm_rev = 0.0f, qs_rev = 0.0f, qsm_rev = 0.0f;
     bool visible_test = false;
     V3 va, vb;
+    const DevMedium& md = sv.med;
+    float medL = 0.0f;  // medium length of the connecting segment
     if (s == 0) {
         if (!(ptf & kVEmitter) || t < 2) return;
         const BVert pm = bb.eye[(uint32_t)(t - 2) * nb + j];
@@ -1753,7 +1834,8 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
             const DevLight& L = sv.lights[li];
             pt_rev = 1.0f / (float)sv.n_lights * L.p[3];
             const float cl = fmaxf(dot(bv_n(pt), to_prev), 0.0f);
-            if (!(bv_flags(pm) & kVCamera)) ptm_rev = to_area(kInvPiF * cl, bv_p(pt), bv_p(pm), bv_n(pm));
+            if (!(bv_flags(pm) & kVCamera))
+                ptm_rev = to_meas(md, kInvPiF * cl, bv_p(pt), pm, seg_medium(md, bv_p(pt), bv_p(pm)));
         }
     } else {
         const BVert qs = bb.light[(uint32_t)(s - 1) * nb + j];
@@ -1768,24 +1850,22 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
             const V3 to_cam = -dc;
             const float d2 = dot(bv_p(qs) - cp, bv_p(qs) - cp);
             V3 fL;
-            float cos_q;
+            const float cos_q = (qsf & kVMedium) ? 1.0f : dot(bv_n(qs), to_cam);
+            if (!(cos_q > 0.0f)) return;
             if (s == 1) {
-                cos_q = dot(bv_n(qs), to_cam);
-                if (!(cos_q > 0.0f)) return;
                 fL = v3(1, 1, 1);
             } else {
                 const BVert qm = bb.light[(uint32_t)(s - 2) * nb + j];
-                cos_q = dot(bv_n(qs), to_cam);
-                if (!(cos_q > 0.0f)) return;
                 fL = bsdf_f(sv, qs, to_cam, normalize(bv_p(qm) - bv_p(qs)));
-                qsm_rev = to_area(bsdf_pdf(sv, qs, normalize(bv_p(qm) - bv_p(qs)), to_cam), bv_p(qs),
-                                  bv_p(qm), bv_n(qm));
+                qsm_rev = to_meas(md, bsdf_pdf(sv, qs, normalize(bv_p(qm) - bv_p(qs)), to_cam), bv_p(qs), qm,
+                                  seg_medium(md, bv_p(qs), bv_p(qm)));
             }
             // Pinhole::We = 1 / (A cos^4) (pinhole.cpp:84-87); G = cos_q cos_c / d^2
             const float We = 1.0f / (bp.film_area * cos_c * cos_c * cos_c * cos_c);
             const float G = cos_q * cos_c / d2;
             c = cmul(v3(qs.c.x, qs.c.y, qs.c.z), fL) * (We * G);
-            qs_rev = to_area(1.0f / (cos_c * cos_c * cos_c * bp.film_area), cp, bv_p(qs), bv_n(qs));
+            medL = seg_medium(md, bv_p(qs), cp);
+            qs_rev = to_meas(md, 1.0f / (cos_c * cos_c * cos_c * bp.film_area), cp, qs, medL);
             visible_test = true;
             va = bv_p(qs);
             vb = cp;
@@ -1795,27 +1875,30 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
             const V3 d_pq = bv_p(qs) - bv_p(pt);
             const float d2 = dot(d_pq, d_pq);
             const V3 w_pq = d_pq * (1.0f / sqrtf(d2));
-            const float cos_p = dot(bv_n(pt), w_pq);
-            const float cos_q = -dot(bv_n(qs), w_pq);
+            const float cos_p = (ptf & kVMedium) ? 1.0f : dot(bv_n(pt), w_pq);
+            const float cos_q = (qsf & kVMedium) ? 1.0f : -dot(bv_n(qs), w_pq);
             if (!(cos_p > 0.0f && cos_q > 0.0f)) return;
             const V3 to_pm = normalize(bv_p(pm) - bv_p(pt));
             const V3 fE = bsdf_f(sv, pt, w_pq, to_pm);
             V3 fL;
+            medL = seg_medium(md, bv_p(pt), bv_p(qs));
             if (s == 1) {
                 fL = v3(1, 1, 1);
-                pt_rev = to_area(kInvPiF * cos_q, bv_p(qs), bv_p(pt), bv_n(pt));
+                pt_rev = to_meas(md, kInvPiF * cos_q, bv_p(qs), pt, medL);
             } else {
                 const BVert qm = bb.light[(uint32_t)(s - 2) * nb + j];
                 const V3 to_qm = normalize(bv_p(qm) - bv_p(qs));
                 fL = bsdf_f(sv, qs, -w_pq, to_qm);
-                pt_rev = to_area(bsdf_pdf(sv, qs, -w_pq, to_qm), bv_p(qs), bv_p(pt), bv_n(pt));
-                qsm_rev = to_area(bsdf_pdf(sv, qs, to_qm, -w_pq), bv_p(qs), bv_p(qm), bv_n(qm));
+                pt_rev = to_meas(md, bsdf_pdf(sv, qs, -w_pq, to_qm), bv_p(qs), pt, medL);
+                qsm_rev = to_meas(md, bsdf_pdf(sv, qs, to_qm, -w_pq), bv_p(qs), qm,
+                                  seg_medium(md, bv_p(qs), bv_p(qm)));
             }
             const float G = cos_p * cos_q / d2;
             c = cmul(cmul(cmul(v3(pt.c.x, pt.c.y, pt.c.z), fE), fL), v3(qs.c.x, qs.c.y, qs.c.z)) * G;
-            qs_rev = to_area(bsdf_pdf(sv, pt, w_pq, to_pm), bv_p(pt), bv_p(qs), bv_n(qs));
+            qs_rev = to_meas(md, bsdf_pdf(sv, pt, w_pq, to_pm), bv_p(pt), qs, medL);
             if (!(bv_flags(pm) & kVCamera))
-                ptm_rev = to_area(bsdf_pdf(sv, pt, to_pm, w_pq), bv_p(pt), bv_p(pm), bv_n(pm));
+                ptm_rev = to_meas(md, bsdf_pdf(sv, pt, to_pm, w_pq), bv_p(pt), pm,
+                                  seg_medium(md, bv_p(pt), bv_p(pm)));
             visible_test = true;
             va = bv_p(qs);
             vb = bv_p(pt);
@@ -1855,6 +1938,14 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
             }
         }
         w = 1.0f / (1.0f + sum);
+    }
+    // the connection's medium: a grey Bernoulli survival, as VolumetricTest
+    // (bidirectionalpathtracer.cpp:1050-1062) and the path tracer's shadow
+    // rays; one number from the connection's own stream
+    if (medL > 0.0f) {
+        Rng gc = rng_seed((prm.seed ^ kConnMix) + (base + j) * 0x9E3779B97F4A7C15ull,
+                          ((uint64_t)s << 8) | (uint64_t)t);
+        if (gc.uniform() < 1.0f - expf(-md.avg_sigma_t * medL)) return;
     }
     if (visible_test) {
         const V3 dv = vb - va;
@@ -1900,12 +1991,10 @@ SceneView scene_view(const rumi_scene* sc) {
 int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
                 rumi_render_stats* stats, hipStream_t stream) {
     (void)ctx;
-    if (sc->medium.bound != RUMI_MEDIUM_NONE)
-        return fail(RUMI_ERR_UNSUPPORTED, "rumi_render: the BDPT integrator does not handle media");
     if (sc->material_kinds & (1u << RUMI_MAT_SUBSURFACE))
         return fail(RUMI_ERR_UNSUPPORTED,
                     "rumi_render: the BDPT integrator handles matte, glossy, GGX, specular, "
-                    "dielectric and emissive materials");
+                    "dielectric and emissive materials and homogeneous media");
     const int nE = desc->n_eye > 0 ? desc->n_eye : desc->max_depth + 1;
     const int nL = desc->n_light > 0 ? desc->n_light : desc->max_depth + 1;
     if (nE < 2 || nE > 64 || nL < 1 || nL > 64)
