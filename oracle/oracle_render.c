@@ -1486,9 +1486,11 @@ int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
  * connection -- t = 1 splats through the pinhole onto the film
  * (Pinhole::FilmPlaneIntersection, pinhole.cpp:138-167) -- weighted by MIS
  * over all strategies with exponent beta (1 = the reference's balance
- * heuristic, MISWeight :1096-1586).  Mirror and dielectric vertices are
- * never connected.  Materials: matte, glossy, GGX, specular, dielectric,
- * emissive.
+ * heuristic, MISWeight :1096-1586).  Mirror, dielectric and subsurface
+ * boundary vertices and subsurface interior events are never connected.
+ * Materials: matte, glossy, GGX, specular, dielectric, subsurface, emissive;
+ * the homogeneous medium (medium vertices on both subpaths, a grey Bernoulli
+ * survival on every connection, as VolumetricTest :1037-1077).
  *
  * Glossy picks one lobe per vertex as Glossy::BSDFSampling does
  * (src/Material/glossy.cpp:56-78): the Lambertian lobe with probability
@@ -1648,6 +1650,42 @@ static int bd_walk(const scene_t* sc, bvert_t* arr, int start, int maxv, v3 o, v
         }
         surface(sc, &h, &r, &p, &n);
         const mat_t* m = &sc->mats[sc->shapes[hit].mat];
+        col vol = cs(1.0); /* subsurface pass-through tau / pdf at the exit boundary */
+        if (m->kind == MAT_SUBSURFACE && in_obj) {
+            /* Subsurface::PhaseSampling (subsurface.cpp:135-155): the interior
+             * walk over the segment to the boundary.  Interior vertices are
+             * never connected (a connection from inside ends on the
+             * boundary), so they enter the MIS walk like Dirac vertices. */
+            double sd = 0.0;
+            int absorbed = 0;
+            col tau, pdf;
+            int sc_ev = distance_sample(m->sigma_t, m->avg_sigmat, m->absorb, r.tmin, g, &sd, &absorbed,
+                                        &tau, &pdf);
+            if (absorbed) break;
+            if (sc_ev) {
+                v3 ps = add(r.o, mul(r.d, sd));
+                bvert_t* v = &arr[nv];
+                v->p = ps;
+                v->n = d;
+                v->pdf_fwd = 0.0;
+                v->pdf_rev = 0.0;
+                v->beta = beta;
+                v->flags = BV_DELTA;
+                v->mat = sc->shapes[hit].mat;
+                v->prim = hit;
+                ++nv;
+                if (nv >= 2) arr[nv - 2].pdf_rev = 0.0;
+                beta = cmul(cmul(beta, cdiv(tau, pdf)), m->sigma_s);
+                double pdf_ph;
+                v3 w = phase_sample(0.0, d, g, &pdf_ph);
+                pdf_w = 0.0;
+                prev_p = ps;
+                o = ps;
+                d = w;
+                continue;
+            }
+            vol = cdiv(tau, pdf);
+        }
         bvert_t* v = &arr[nv];
         v->p = p;
         v->n = n;
@@ -1703,7 +1741,7 @@ static int bd_walk(const scene_t* sc, bvert_t* arr, int start, int maxv, v3 o, v
             w = add(neg(wo), mul(n, 2.0 * won));
             weight = cscale(m->color, m->kd);
             delta = 1;
-        } else if (m->kind == MAT_DIELECTRIC) {
+        } else if (m->kind == MAT_DIELECTRIC || m->kind == MAT_SUBSURFACE) {
             double kr = fresnel(neg(wo), n, m->eta_in, m->eta_out), c;
             delta = 1;
             if (kr != 0.0 && rnd(g) <= kr) {
@@ -1717,9 +1755,10 @@ static int bd_walk(const scene_t* sc, bvert_t* arr, int start, int maxv, v3 o, v
                 double er = was_in ? m->eta_in / m->eta_out : m->eta_out / m->eta_in;
                 weight = cs(eye ? 1.0 / (er * er) : 1.0);
             }
-            if (was_in) weight = cmul(weight, exp_tau(m->absorption, r.tmin));
+            if (was_in && m->kind == MAT_DIELECTRIC) weight = cmul(weight, exp_tau(m->absorption, r.tmin));
+            if (was_in && m->kind == MAT_SUBSURFACE) weight = cmul(weight, vol);
         } else {
-            break; /* subsurface: not handled by the bidirectional tracer */
+            break;
         }
         if (delta) v->flags |= BV_DELTA;
         if (nv >= 2) {
@@ -1731,7 +1770,7 @@ static int bd_walk(const scene_t* sc, bvert_t* arr, int start, int maxv, v3 o, v
         prev_p = p;
         o = p;
         d = w;
-        if (m->kind != MAT_DIELECTRIC) in_obj = 0;
+        if (m->kind != MAT_DIELECTRIC && m->kind != MAT_SUBSURFACE) in_obj = 0;
     }
     return nv;
 }

@@ -1592,6 +1592,40 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
         surface_at(sv, prim, o, d, r.tmin, bu, bv, p, n, tex);
         const int mi = sv.shapes[prim].info[1];
         const DevMaterial& m = sv.mats[mi];
+        V3 vol = v3(1, 1, 1);  // subsurface pass-through tau / pdf at the exit boundary
+        if (m.kind == RUMI_MAT_SUBSURFACE && in_obj) {
+            // Subsurface::PhaseSampling (subsurface.cpp:135-155): the interior
+            // walk over the segment to the boundary, as the path tracer takes
+            // it.  Interior vertices are never connected (a connection from
+            // inside ends on the boundary), so they enter the MIS walk like
+            // Dirac vertices.
+            float sd = 0.0f;
+            V3 ratio;
+            const int ev = distance_sample(m.sigma_t, m.avg_sigmat, m.absorb, r.tmin, g, sd, ratio);
+            if (ev == kAbsorb) break;
+            if (ev == kScatter) {
+                const V3 ps = o + d * sd;
+                int tix;
+                const V3 txs = sv.shapes[prim].info[2] >= 0 ? pattern(sv, sv.shapes[prim], ps, bu, bv, tix)
+                                                           : v3(1, 1, 1);
+                store_vert(arr, mis, (uint32_t)nv * nb + j, ps, 0.0f, d, beta, kVDelta, txs, mi);
+                ++nv;
+                if (nv >= 2) {
+                    const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
+                    arr[pi].b.w = 0.0f;
+                    mis[pi].y = 0.0f;
+                }
+                beta = cmul(cmul(cmul(beta, ratio), ld3(m.sigma_s)), txs);
+                float pdf_ph;
+                const V3 w = phase_sample(0.0f, d, g, pdf_ph);
+                pdf_w = 0.0f;
+                prev_p = ps;
+                o = ps;
+                d = w;
+                continue;
+            }
+            vol = ratio;
+        }
         const float pdf_fwd = to_meas(md, pdf_w, prev_p, p, n, false, seg_L);
         if (m.kind == RUMI_MAT_EMISSIVE) {
             // The walk ends on an emitter (bidirectionalpathtracer.cpp:693-699).
@@ -1651,9 +1685,9 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
             float won = dot(wo, n);
             if (won <= 0.0f) won = 1e-4f;
             w = -wo + n * (2.0f * won);
-            weight = ld3(m.color) * m.kd;
+            weight = cmul(ld3(m.color), tex) * m.kd;
             delta = true;
-        } else {  // dielectric
+        } else {  // dielectric, subsurface boundary
             const float kr = fresnel(-wo, n, m.eta_in, m.eta_out);
             const float c = dot(wo, n);
             delta = true;
@@ -1691,10 +1725,13 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
                 const float scale = EYE ? 1.0f / (er * er) : 1.0f;
                 weight = v3(scale, scale, scale);
             }
-            // Beer-Lambert along a segment travelled inside (dielectric.cpp:94-95)
-            if (was_in)
+            weight = cmul(weight, tex);
+            // Beer-Lambert along a segment travelled inside (dielectric.cpp:94-95);
+            // a subsurface interior passed through without scattering
+            if (was_in && m.kind == RUMI_MAT_DIELECTRIC)
                 weight = cmul(weight, v3(expf(-m.absorption[0] * r.tmin), expf(-m.absorption[1] * r.tmin),
                                          expf(-m.absorption[2] * r.tmin)));
+            if (was_in && m.kind == RUMI_MAT_SUBSURFACE) weight = cmul(weight, vol);
         }
         store_vert(arr, mis, slot, p, pdf_fwd, n, beta, vflags | (delta ? kVDelta : 0u), tex, mi);
         // reverse pdf of the previous vertex (area measure at prev)
@@ -1709,7 +1746,7 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
         prev_p = p;
         o = p;
         d = w;
-        if (m.kind != RUMI_MAT_DIELECTRIC) in_obj = false;
+        if (m.kind != RUMI_MAT_DIELECTRIC && m.kind != RUMI_MAT_SUBSURFACE) in_obj = false;
     }
     return nv;
 }
@@ -1991,10 +2028,6 @@ SceneView scene_view(const rumi_scene* sc) {
 int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
                 rumi_render_stats* stats, hipStream_t stream) {
     (void)ctx;
-    if (sc->material_kinds & (1u << RUMI_MAT_SUBSURFACE))
-        return fail(RUMI_ERR_UNSUPPORTED,
-                    "rumi_render: the BDPT integrator handles matte, glossy, GGX, specular, "
-                    "dielectric and emissive materials and homogeneous media");
     const int nE = desc->n_eye > 0 ? desc->n_eye : desc->max_depth + 1;
     const int nL = desc->n_light > 0 ? desc->n_light : desc->max_depth + 1;
     if (nE < 2 || nE > 64 || nL < 1 || nL > 64)

@@ -213,40 +213,25 @@ def test_bdpt_config_scene_runs(rumi, ctx, scenes):
     assert img.mean() > 0.01 and st.samples == 64 * 64 * 16
 
 
-def _media_no_sss(scenes, w, g=0.6):
-    """The media scene with its subsurface sphere made Lambertian."""
-    sc = scenes.cornell_media(w, w, g=g)
-    sc["materials"][-1] = dict(kind="matte", kd=1.0, color=(0.6, 0.6, 0.6))
-    return sc
-
-
 @pytest.mark.parametrize("g", [0.0, 0.6])
 def test_bdpt_media_same_seed_matches_oracle_and_equals_path_tracer(rumi, oracle, ctx, scenes, g):
-    """Homogeneous medium in the bidirectional tracer: with one seed the GPU
-    and the oracle build the same subpaths (medium vertices included) and draw
-    the same connection survivals, so their images agree to rounding; and the
-    GPU image equals the GPU path tracer's on 4 x 4 block means."""
-    sc = _media_no_sss(scenes, 16, g)
+    """The media scene (homogeneous-medium sphere, subsurface sphere) in the
+    bidirectional tracer: with one seed the GPU and the oracle build the same
+    subpaths (medium vertices and subsurface interior events included) and
+    draw the same connection survivals, so their images agree to rounding;
+    and the GPU image equals the GPU path tracer's on 4 x 4 block means at
+    depth 12 (path length 13)."""
+    sc = scenes.cornell_media(16, 16, g=g)
     c = oracle.render_bdpt(sc, 16, 16, 64, 5, seed=5, nthreads=8)
     same = _bdpt(rumi, ctx, sc, 16, 16, 64, 5, seed=5)
     d_same = np.abs(same - c).sum() / np.abs(c).sum()
     print("relative L1 to the oracle, same seed", d_same)
     assert d_same < 1e-3, d_same
-    pt = _blocks([render(rumi, ctx, sc, 16, 16, 4096, 4, seed=80 + k)[0] for k in range(12)])
-    bd = _blocks([_bdpt(rumi, ctx, sc, 16, 16, 2048, 5, seed=90 + k) for k in range(12)])
+    pt = _blocks([render(rumi, ctx, sc, 16, 16, 4096, 12, seed=80 + k)[0] for k in range(12)])
+    bd = _blocks([_bdpt(rumi, ctx, sc, 16, 16, 2048, 13, seed=90 + k) for k in range(12)])
     z = (bd.mean(0) - pt.mean(0)) / np.sqrt(bd.var(0, ddof=1) / 12 + pt.var(0, ddof=1) / 12)
     print("max |z| against the path tracer", np.abs(z).max())
     assert np.all(np.abs(z) < 5.0), z
-
-
-def test_bdpt_rejects_subsurface(rumi, ctx, scenes):
-    sc = scenes.diffuse_box(8, 8)
-    sc["materials"][4] = dict(kind="subsurface", eta_in=1.3, eta_out=1.0, sigma_s=(1, 1, 1),
-                              sigma_a=(0.1, 0.1, 0.1), density=1.0)
-    scene = rumi.Scene(ctx, sc)
-    with pytest.raises(rumi.RumiError):
-        scene.render(8, 8, 1, 4, integrator=rumi.RUMI_INTEGRATOR_BDPT)
-    scene.close()
 
 
 def _blocks(imgs, n=4):
@@ -257,11 +242,14 @@ def _blocks(imgs, n=4):
 def test_bdpt_glossy_equals_path_tracer_and_oracle(rumi, oracle, ctx):
     """Glossy floor and sphere (Lambertian lobe connected, Phong lobe sampled
     from the eye side only): GPU BDPT against the GPU path tracer and against
-    the oracle's BDPT, on 4 x 4 block means of the image."""
+    the oracle's BDPT, on 4 x 4 block means of the image, at depth 12 (path
+    length 13): a path tracer stops at its last vertex, so paths one edge
+    longer that reach the light right after a Phong vertex are the
+    bidirectional tracer's alone, a negligible share at this depth."""
     sc = rc.glossy_box(16, 16)
-    pt = _blocks([render(rumi, ctx, sc, 16, 16, 8192, 4, seed=50 + k)[0] for k in range(12)])
-    bd = _blocks([_bdpt(rumi, ctx, sc, 16, 16, 2048, 5, seed=60 + k) for k in range(12)])
-    orc = _blocks([oracle.render_bdpt(sc, 16, 16, 2048, 5, seed=70 + k, nthreads=8)
+    pt = _blocks([render(rumi, ctx, sc, 16, 16, 8192, 12, seed=50 + k)[0] for k in range(12)])
+    bd = _blocks([_bdpt(rumi, ctx, sc, 16, 16, 2048, 13, seed=60 + k) for k in range(12)])
+    orc = _blocks([oracle.render_bdpt(sc, 16, 16, 1024, 13, seed=70 + k, nthreads=8)
                    for k in range(12)])
     for other in (pt, orc):
         z = (bd.mean(0) - other.mean(0)) / np.sqrt(bd.var(0, ddof=1) / 12 + other.var(0, ddof=1) / 12)

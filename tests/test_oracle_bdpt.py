@@ -65,35 +65,36 @@ def block_means(imgs, n=4):
     return np.array([i.reshape(n, h // n, n, w // n, 3).mean(axis=(1, 3)) for i in imgs])
 
 
+def _match_pt(oracle, sc, pt_spp, bd_spp, depth, runs=12, seed=500):
+    """4 x 4 block z-scores of the bidirectional image against the path
+    tracer's.  A path tracer of depth D stops at its D-th vertex, so paths of
+    D + 1 edges that reach the light right after a Dirac vertex are the
+    bidirectional tracer's alone; at D = 12 they carry a negligible share."""
+    pt = block_means([oracle.render_pt(sc, 16, 16, pt_spp, depth, seed=seed + k, nthreads=8)[0]
+                      for k in range(runs)])
+    bd = block_means([oracle.render_bdpt(sc, 16, 16, bd_spp, depth + 1, seed=seed + 100 + k,
+                                         nthreads=8) for k in range(runs)])
+    z = (bd.mean(0) - pt.mean(0)) / np.sqrt(pt.var(0, ddof=1) / runs + bd.var(0, ddof=1) / runs)
+    print("max |z|", np.abs(z).max())
+    return z
+
+
 def test_bdpt_matches_pt_with_glossy(oracle):
     """Glossy floor and sphere: Lambertian-lobe vertices are connected, Phong
     vertices only sampled from the eye side, as the path tracer does.  Block
     means resolve the glossy floor's highlight, which the image mean hides."""
     import render_cases as rc
-    sc = rc.glossy_box(16, 16)
-    pt = block_means([oracle.render_pt(sc, 16, 16, 4096, 4, seed=500 + k, nthreads=8)[0]
-                      for k in range(12)])
-    bd = block_means([oracle.render_bdpt(sc, 16, 16, 2048, 5, seed=600 + k, nthreads=8)
-                      for k in range(12)])
-    z = (bd.mean(0) - pt.mean(0)) / np.sqrt(pt.var(0, ddof=1) / 12 + bd.var(0, ddof=1) / 12)
-    print("max |z|", np.abs(z).max())
+    z = _match_pt(oracle, rc.glossy_box(16, 16), 2048, 1024, 12)
     assert np.all(np.abs(z) < 5.0), z
 
 
 @pytest.mark.parametrize("g", [0.0, 0.6])
-def test_bdpt_matches_pt_with_media(oracle, scenes, g):
-    """A homogeneous-medium sphere (isotropic and Henyey-Greenstein): medium
-    vertices on both subpaths, connections through the medium with the path
-    tracer's grey Bernoulli transmittance, and volume-measure pdfs in the MIS
-    weights; the bidirectional image equals the path tracer's on 4 x 4 block
-    means.  (The scene's subsurface sphere is made Lambertian: subsurface
-    materials are not part of the bidirectional tracer.)"""
-    sc = scenes.cornell_media(16, 16, g=g)
-    sc["materials"][-1] = dict(kind="matte", kd=1.0, color=(0.6, 0.6, 0.6))
-    pt = block_means([oracle.render_pt(sc, 16, 16, 2048, 4, seed=500 + k, nthreads=8)[0]
-                      for k in range(12)])
-    bd = block_means([oracle.render_bdpt(sc, 16, 16, 1024, 5, seed=600 + k, nthreads=8)
-                      for k in range(12)])
-    z = (bd.mean(0) - pt.mean(0)) / np.sqrt(pt.var(0, ddof=1) / 12 + bd.var(0, ddof=1) / 12)
-    print("max |z|", np.abs(z).max())
+def test_bdpt_matches_pt_with_media_and_subsurface(oracle, scenes, g):
+    """The media scene: a homogeneous-medium sphere (isotropic and
+    Henyey-Greenstein phase) and a subsurface sphere.  Medium vertices on both
+    subpaths, connections through the medium with the path tracer's grey
+    Bernoulli transmittance, volume-measure pdfs in the MIS weights; subsurface
+    boundaries and interior events sampled by the walks and never connected.
+    The bidirectional image equals the path tracer's on 4 x 4 block means."""
+    z = _match_pt(oracle, scenes.cornell_media(16, 16, g=g), 1024, 512, 12)
     assert np.all(np.abs(z) < 5.0), z
