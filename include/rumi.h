@@ -286,7 +286,10 @@ typedef struct {
 
 /* Renders one frame: Scene::Rendering's pixel x sample loop
  * (src/Scene/scene.cpp:206-258) over Integrator::Trace(w, h, camera, film)
- * (PathTracer src/Integrator/pathtracer.cpp:117-354).  dev_rgb receives the
+ * (PathTracer src/Integrator/pathtracer.cpp:117-354; BidirectionalPathTracer
+ * src/Integrator/bidirectionalpathtracer.cpp:121-394, all materials, the
+ * homogeneous medium and textures; for BDPT max_depth is the path length in
+ * edges and n_light / n_eye default to max_depth + 1).  dev_rgb receives the
  * per-pixel mean radiance, height x width x 3 floats RGB, row 0 at the top of
  * the image as Film::Pixel indexes it (src/Camera/film.cpp:83-97).  Samples
  * whose radiance is NaN are dropped, as Scene::Rendering drops them.
