@@ -24,7 +24,8 @@ METRIC = "Mrays/s (closest-hit, incoherent) + samples/s at matched rel-MSE vs CP
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published peak
 RAY_IN_BYTES = 32      # rumi_ray
 HIT_OUT_BYTES = 8      # rumi_hit (closest); any-hit writes 4
-NODE_BYTES = 64        # interior node: two child boxes + refs
+NODE_BYTES = 64        # binary interior node: two child boxes + refs
+NODE_BYTES4 = 128      # 4-wide interior node: four child boxes + refs
 TRI_BYTES = 48         # leaf-ordered triangle: v0, e1, e2, id
 
 
@@ -96,21 +97,15 @@ def main():
         f"depth {info.max_depth}, SAH build {info.build_seconds:.2f}s")
 
     # Per-ray work counted by the instrumented kernel over the whole batch
-    # (the same traversal; outside the timed region).
-    nc = torch.empty(n, dtype=torch.int32, device="cuda")
-    pc = torch.empty(n, dtype=torch.int32, device="cuda")
-    rumi.trace_count(ctx, bvh, rays, False, hits, nc, pc)
-    torch.cuda.synchronize()
-    nodes_c = float(nc.double().mean())
-    tris_c = float(pc.double().mean())
-    hit_frac = float((hits[:, 0] >= 0).double().mean())
-    rumi.trace_count(ctx, bvh, rays, True, hits, nc, pc)
-    torch.cuda.synchronize()
-    nodes_a = float(nc.double().mean())
-    tris_a = float(pc.double().mean())
-    del nc, pc
-    b_ray_c = RAY_IN_BYTES + HIT_OUT_BYTES + NODE_BYTES * nodes_c + TRI_BYTES * tris_c
-    b_ray_a = RAY_IN_BYTES + 4 + NODE_BYTES * nodes_a + TRI_BYTES * tris_a
+    # (the same traversal as the timed launches, with the context's current
+    # tree width; outside the timed region).
+    def work(any_hit):
+        nc = torch.empty(n, dtype=torch.int32, device="cuda")
+        pc = torch.empty(n, dtype=torch.int32, device="cuda")
+        rumi.trace_count(ctx, bvh, rays, any_hit, hits, nc, pc)
+        torch.cuda.synchronize()
+        node_b = NODE_BYTES4 if ctx.trace_tuning()["width"] == 4 else NODE_BYTES
+        return float(nc.double().mean()), float(pc.double().mean()), node_b
 
     stream = torch.cuda.current_stream()
 
@@ -137,9 +132,11 @@ def main():
             ctx, lambda: rumi.trace_closest(ctx, bvh, rays, hits, None, 0, stream))
         tuning = {"closest": best,
                   "candidates_us": [[c["block"], c["stack"], c["refill"], c["persistent"],
-                                     round(t * 1e6, 1)] for c, t in table]}
+                                     c["width"], round(t * 1e6, 1)] for c, t in table]}
         log(f"autotuned closest-hit: {best}")
 
+    nodes_c, tris_c, node_bc = work(False)
+    hit_frac = float((hits[:, 0] >= 0).double().mean())
     ms_c = timed(lambda: rumi.trace_closest(ctx, bvh, rays, hits, None, 0, stream),
                  args.steps, args.warmup)
     log(f"closest-hit: {ms_c:.3f} ms/step, {n / ms_c / 1e3:.1f} Mrays/s")
@@ -147,8 +144,11 @@ def main():
         best_a, _ = autotune.tune_trace(ctx, lambda: rumi.trace_any(ctx, bvh, rays, anyp, 0, stream))
         tuning["any"] = best_a
         log(f"autotuned any-hit: {best_a}")
+    nodes_a, tris_a, node_ba = work(True)
     ms_a = timed(lambda: rumi.trace_any(ctx, bvh, rays, anyp, 0, stream), args.steps, args.warmup)
     log(f"any-hit: {ms_a:.3f} ms/step, {n / ms_a / 1e3:.1f} Mrays/s")
+    b_ray_c = RAY_IN_BYTES + HIT_OUT_BYTES + node_bc * nodes_c + TRI_BYTES * tris_c
+    b_ray_a = RAY_IN_BYTES + 4 + node_ba * nodes_a + TRI_BYTES * tris_a
 
     mrays_c = n / ms_c / 1e3
     mrays_a = n / ms_a / 1e3
@@ -195,7 +195,7 @@ def main():
             "bound": "hbm", "achieved": round(achieved_c, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved_c / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_ray": round(b_ray_c, 1), "nodes_per_ray": round(nodes_c, 3),
-            "tris_per_ray": round(tris_c, 3),
+            "tris_per_ray": round(tris_c, 3), "node_bytes": node_bc,
         },
         "tuning": tuning,
         "cpu_baseline": cpu,
@@ -204,7 +204,7 @@ def main():
             "roofline_achieved_GBs": round(achieved_a, 1),
             "roofline_frac": round(achieved_a / HBM_PEAK_GBS, 4),
             "bytes_per_ray": round(b_ray_a, 1), "nodes_per_ray": round(nodes_a, 3),
-            "tris_per_ray": round(tris_a, 3),
+            "tris_per_ray": round(tris_a, 3), "node_bytes": node_ba,
         },
         "path_tracing": pt,
     }

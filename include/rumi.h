@@ -156,14 +156,19 @@ int rumi_trace_count(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* dev_ray
  * default every lane steps its own ray and swaps in a ray it has already
  * prefetched when its ray ends (persistent 2); persistent 1 instead refills
  * a wave from a shared counter once `refill` of its lanes are idle; -1 is one
- * ray per lane.  Zero fields keep the defaults: block 128 threads, stack 16
- * LDS entries per lane (deeper entries spill to private memory), refill 16,
- * persistent 2.  Results are the same bits for every setting. */
+ * ray per lane.  width 4 traverses SAH trees in their 4-wide form (near-first
+ * order; persistent 1 then runs as 2).  Zero fields keep the defaults: block
+ * 128 threads, stack 16 LDS entries per lane (deeper entries spill to private
+ * memory), refill 16, persistent 2, width 2.  Every setting returns the same
+ * hit distances and, within one width, the same bits; between widths 2 and 4
+ * only which of two primitives at exactly equal distance is reported can
+ * differ (rumi_trace_count follows the width). */
 typedef struct {
     int32_t block;       /* 64, 128 or 256 */
     int32_t stack;       /* 8 or 16 */
     int32_t refill;      /* 1..64 (persistent 1) */
     int32_t persistent;  /* 2 prefetching lanes, 1 threshold refill, -1 one ray per lane */
+    int32_t width;       /* 2 binary nodes, 4 four-wide nodes (SAH trees) */
 } rumi_trace_tuning;
 
 int rumi_trace_set_tuning(rumi_ctx* ctx, const rumi_trace_tuning* tuning);

@@ -563,6 +563,119 @@ static int travel_near(const or_bvh* b, const or_prim* prims, or_ray* ray, int a
     }
 }
 
+/* The 4-wide form of an interior node, as the GPU builds it (widen,
+ * rumi-renderer_amd/csrc/bvh_build.cpp): its two children, the interior child
+ * of largest surface area (first on ties) repeatedly replaced in place by its
+ * own two children, until four. */
+static int wide_children(const or_bvh* b, int node, int out[4])
+{
+    int n = 2, k;
+    out[0] = b->left[node];
+    out[1] = b->right[node];
+    while (n < 4) {
+        int best = -1;
+        double best_a = -1.0;
+        for (k = 0; k < n; ++k) {
+            if (b->left[out[k]] < 0) continue;
+            const float* lo = b->lo + 3 * out[k];
+            const float* hi = b->hi + 3 * out[k];
+            double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+            double a = 2.0 * (dx * dy + dy * dz + dz * dx);
+            if (a > best_a) {
+                best_a = a;
+                best = k;
+            }
+        }
+        if (best < 0) break;
+        int c = out[best];
+        for (k = n; k > best + 1; --k) out[k] = out[k - 1];
+        out[best] = b->left[c];
+        out[best + 1] = b->right[c];
+        ++n;
+    }
+    return n;
+}
+
+/* travel_near over the 4-wide form: the accepted children are ordered by
+ * entry distance (ties by slot), the nearest is visited next and the others
+ * stacked farthest first (trav_step_near4 in csrc/traverse.h). */
+static int travel_near4(const or_bvh* b, const or_prim* prims, or_ray* ray, int any,
+                        int in_object, or_stats* st)
+{
+    stk_t stack[256];
+    int sp = 0, hit = -1;
+    int cur = b->root, leaf = -1, lpos = 0, lend = 0;
+    for (;;) {
+        int at_node = cur >= 0 && b->left[cur] >= 0;
+        if (leaf >= 0) {
+            int p = b->order[lpos];
+            if (st) st->prims++;
+            if (or_prim_intersect(&prims[p], ray, in_object))
+                hit = p;
+            if (++lpos == lend) {
+                leaf = -1;
+                if (any && ray->tmin > 0.0f)
+                    return hit;
+            }
+        }
+        if (at_node) {
+            int ch[4], acc[4], slot[4], i, n;
+            float tn[4];
+            if (st) st->nodes++;
+            n = wide_children(b, cur, ch);
+            for (i = 0; i < 4; ++i) {
+                slot[i] = i;
+                tn[i] = 0.0f;
+                acc[i] = 0;
+                if (i < n) acc[i] = static_accept(b, ch[i], ray, &tn[i]) && tmin_accept(ray, tn[i]);
+                else ch[i] = -1;
+            }
+            static const int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+            for (i = 0; i < 5; ++i) {
+                int x = net[i][0], y = net[i][1], before;
+                if (acc[y] != acc[x]) before = acc[y];
+                else if (tn[y] < tn[x]) before = 1;
+                else if (tn[x] < tn[y]) before = 0;
+                else before = slot[y] < slot[x];
+                if (before) {
+                    int ta = acc[x], tc = ch[x], ts = slot[x];
+                    float tt = tn[x];
+                    acc[x] = acc[y]; ch[x] = ch[y]; slot[x] = slot[y]; tn[x] = tn[y];
+                    acc[y] = ta; ch[y] = tc; slot[y] = ts; tn[y] = tt;
+                }
+            }
+            cur = acc[0] ? ch[0] : -1;
+            for (i = 3; i >= 1; --i)
+                if (acc[i]) {
+                    stack[sp].node = ch[i];
+                    stack[sp].tnear = tn[i];
+                    ++sp;
+                }
+        }
+        for (;;) {
+            if (cur >= 0 && b->left[cur] < 0) {
+                if (b->count[cur] == 0) {
+                    cur = -1;
+                } else if (leaf >= 0) {
+                    break;
+                } else {
+                    leaf = cur;
+                    lpos = b->first[cur];
+                    lend = lpos + b->count[cur];
+                    cur = -1;
+                }
+            }
+            if (cur >= 0 || sp == 0)
+                break;
+            --sp;
+            if (tmin_accept(ray, stack[sp].tnear))
+                cur = stack[sp].node;
+        }
+        if (cur < 0 && leaf < 0)
+            return hit;
+    }
+}
+
 int or_travel(const or_bvh* b, const or_prim* prims, or_ray* ray, int any,
               int ordered, int in_object, or_stats* st)
 {
@@ -576,6 +689,8 @@ int or_travel(const or_bvh* b, const or_prim* prims, or_ray* ray, int any,
     float tn;
     if (!(static_accept(b, b->root, ray, &tn) && tmin_accept(ray, tn)))
         return -1;
+    if (ordered == 2)
+        return travel_near4(b, prims, ray, any, in_object, st);
     if (ordered)
         return travel_near(b, prims, ray, any, in_object, st);
     int cur = b->root;

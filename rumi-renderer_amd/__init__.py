@@ -92,7 +92,7 @@ class RenderStats(Structure):
 
 class TraceTuning(Structure):
     _fields_ = [("block", c_int32), ("stack", c_int32), ("refill", c_int32),
-                ("persistent", c_int32)]
+                ("persistent", c_int32), ("width", c_int32)]
 
 
 class BatchItem(Structure):
@@ -237,15 +237,16 @@ class Context:
         check(lib().rumi_ctx_create(device, byref(h)))
         self.handle = h
 
-    def set_trace_tuning(self, block=0, stack=0, refill=0, persistent=0):
+    def set_trace_tuning(self, block=0, stack=0, refill=0, persistent=0, width=0):
         """Launch shape of trace_closest / trace_any (zero = default)."""
         check(lib().rumi_trace_set_tuning(self.handle, byref(TraceTuning(block, stack, refill,
-                                                                         persistent))))
+                                                                         persistent, width))))
 
     def trace_tuning(self) -> dict:
         t = TraceTuning()
         check(lib().rumi_trace_get_tuning(self.handle, byref(t)))
-        return dict(block=t.block, stack=t.stack, refill=t.refill, persistent=t.persistent)
+        return dict(block=t.block, stack=t.stack, refill=t.refill, persistent=t.persistent,
+                    width=t.width)
 
     def close(self):
         if self.handle:

@@ -1,10 +1,12 @@
 """Autotuner for the launch shapes of the traversal and the wavefront renderer.
 
-Traversal: threads per block, LDS stack entries per lane and the refill
-threshold of the persistent-lane kernels (rumi_trace_set_tuning).  Renderer:
-the wavefront queue capacity and regeneration threshold (rumi_render_desc).
-Every candidate gives the same results bit for bit (traversal) or the same
-image up to the order of float atomics (renderer), so the tuner only times.
+Traversal: threads per block, LDS stack entries per lane, the refill
+threshold of the persistent-lane kernels and the node width (binary or
+4-wide) of SAH trees (rumi_trace_set_tuning).  Renderer: the wavefront queue
+capacity and regeneration threshold (rumi_render_desc).  Every traversal
+candidate gives the same hit distances (and, within one node width, the same
+bits); renderer candidates give the same image up to the order of float
+atomics; so the tuner only times.
 
 The timer is pluggable: any callable that runs a zero-argument function and
 returns its duration in seconds.  The default times on the device with HIP
@@ -21,6 +23,7 @@ Timer = Callable[[Callable[[], object]], float]
 TRACE_BLOCKS = (64, 128, 256)
 TRACE_STACKS = (8, 16)
 TRACE_REFILLS = (16, 32, 48, 64)
+TRACE_WIDTHS = (2, 4)
 
 
 class HipEventTimer:
@@ -50,17 +53,19 @@ class HipEventTimer:
 
 
 def trace_candidates(blocks: Sequence[int] = TRACE_BLOCKS, stacks: Sequence[int] = TRACE_STACKS,
-                     refills: Sequence[int] = TRACE_REFILLS, include_one_ray_per_lane=True):
+                     refills: Sequence[int] = TRACE_REFILLS, widths: Sequence[int] = TRACE_WIDTHS,
+                     include_one_ray_per_lane=True):
     """Tuning dicts to try: the one-ray-per-lane launch first, then the
-    prefetching lanes (persistent 2) for every block and stack size, then the
-    threshold refill (persistent 1) for every block, stack and threshold."""
+    prefetching lanes (persistent 2) for every width, block and stack size,
+    then the threshold refill (persistent 1, binary nodes) for every block,
+    stack and threshold."""
     out = []
     if include_one_ray_per_lane:
-        out.append(dict(block=128, stack=16, refill=16, persistent=-1))
-    for b, s in itertools.product(blocks, stacks):
-        out.append(dict(block=b, stack=s, refill=16, persistent=2))
+        out.append(dict(block=128, stack=16, refill=16, persistent=-1, width=2))
+    for w, b, s in itertools.product(widths, blocks, stacks):
+        out.append(dict(block=b, stack=s, refill=16, persistent=2, width=w))
     for b, s, r in itertools.product(blocks, stacks, refills):
-        out.append(dict(block=b, stack=s, refill=r, persistent=1))
+        out.append(dict(block=b, stack=s, refill=r, persistent=1, width=2))
     return out
 
 

@@ -59,9 +59,10 @@ static void release_device(rumi_bvh* b) {
         (void)hipSetDevice(b->device);
         if (b->dev_nodes) (void)hipFree(b->dev_nodes);
         if (b->dev_tris) (void)hipFree(b->dev_tris);
+        if (b->dev_nodes4) (void)hipFree(b->dev_nodes4);
         (void)hipSetDevice(prev);
     }
-    b->dev_nodes = b->dev_tris = nullptr;
+    b->dev_nodes = b->dev_tris = b->dev_nodes4 = nullptr;
     b->device = -1;
 }
 
@@ -126,6 +127,10 @@ extern "C" int rumi_bvh_build_mesh(rumi_ctx* ctx, const float* xyz, uint32_t nv,
     b->mode = p.mode;
     b->n_prims = (int32_t)nt;
     b->sah_cost = br.sah_cost;
+    if (p.mode == RUMI_BVH_SAH_BINNED) {
+        widen(b->flat, &b->wide);
+        b->wide_ok = 3 * b->wide.max_depth <= kMaxDepth;
+    }
     b->host_tris.resize(nt);
     for (uint32_t s = 0; s < nt; ++s) {
         const int32_t prim = b->flat.leaf_prims[s];
@@ -172,6 +177,17 @@ extern "C" int rumi_bvh_build_mesh(rumi_ctx* ctx, const float* xyz, uint32_t nv,
                 }
                 if (hipMemcpy(b->dev_tris, b->host_tris.data(), tb, hipMemcpyHostToDevice) != hipSuccess) {
                     rc = fail(RUMI_ERR_HIP, "hipMemcpy of triangles failed");
+                    break;
+                }
+            }
+            const size_t wb = sizeof(DevNode4) * b->wide.nodes.size();
+            if (b->wide_ok && wb) {
+                if (hipMalloc(&b->dev_nodes4, wb) != hipSuccess) {
+                    rc = fail(RUMI_ERR_NO_MEMORY, "hipMalloc of %zu wide-node bytes failed", wb);
+                    break;
+                }
+                if (hipMemcpy(b->dev_nodes4, b->wide.nodes.data(), wb, hipMemcpyHostToDevice) != hipSuccess) {
+                    rc = fail(RUMI_ERR_HIP, "hipMemcpy of wide BVH nodes failed");
                     break;
                 }
             }

@@ -353,4 +353,96 @@ void flatten(const BuildResult& br, FlatBvh* out) {
     out->root_ref = f.emit(br.root, 0);
 }
 
+namespace {
+
+struct WideChild {
+    float lo[3], hi[3];
+    int32_t ref;
+};
+
+WideChild child_of(const DevNode& d, int c) {
+    WideChild w;
+    if (c == 0) {
+        w.lo[0] = d.a[0]; w.lo[1] = d.a[1]; w.lo[2] = d.a[2];
+        w.hi[0] = d.a[3]; w.hi[1] = d.b[0]; w.hi[2] = d.b[1];
+    } else {
+        w.lo[0] = d.b[2]; w.lo[1] = d.b[3]; w.lo[2] = d.c[0];
+        w.hi[0] = d.c[1]; w.hi[1] = d.c[2]; w.hi[2] = d.c[3];
+    }
+    w.ref = d.d[c];
+    return w;
+}
+
+double wide_area(const WideChild& c) {
+    const double dx = (double)c.hi[0] - c.lo[0], dy = (double)c.hi[1] - c.lo[1],
+                 dz = (double)c.hi[2] - c.lo[2];
+    return 2.0 * (dx * dy + dy * dz + dz * dx);
+}
+
+struct Widener {
+    const FlatBvh& b2;
+    FlatBvh4* out;
+
+    // Converts binary interior node `node`; returns its wide index.
+    int32_t emit(int32_t node, int depth) {
+        WideChild ch[4];
+        int n = 2;
+        ch[0] = child_of(b2.nodes[node], 0);
+        ch[1] = child_of(b2.nodes[node], 1);
+        while (n < 4) {
+            int best = -1;
+            double best_area = -1.0;
+            for (int k = 0; k < n; ++k)
+                if (ch[k].ref >= 0) {
+                    const double a = wide_area(ch[k]);
+                    if (a > best_area) {
+                        best_area = a;
+                        best = k;
+                    }
+                }
+            if (best < 0) break;
+            const DevNode& e = b2.nodes[ch[best].ref];
+            for (int k = n; k > best + 1; --k) ch[k] = ch[k - 1];
+            ch[best] = child_of(e, 0);
+            ch[best + 1] = child_of(e, 1);
+            ++n;
+        }
+        const int32_t idx = (int32_t)out->nodes.size();
+        out->nodes.emplace_back();
+        int32_t refs[4] = {kNoChild, kNoChild, kNoChild, kNoChild};
+        for (int k = 0; k < n; ++k) {
+            if (ch[k].ref >= 0) {
+                refs[k] = emit(ch[k].ref, depth + 1);
+            } else {
+                refs[k] = ch[k].ref;
+                if (depth + 1 > out->max_depth) out->max_depth = depth + 1;
+            }
+        }
+        DevNode4& d = out->nodes[idx];
+        for (int k = 0; k < 4; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                d.lo[a][k] = k < n ? ch[k].lo[a] : 0.0f;
+                d.hi[a][k] = k < n ? ch[k].hi[a] : 0.0f;
+            }
+            d.ref[k] = refs[k];
+            d.pad[k] = 0;
+        }
+        return idx;
+    }
+};
+
+}  // namespace
+
+void widen(const FlatBvh& b2, FlatBvh4* out) {
+    out->nodes.clear();
+    out->max_depth = 0;
+    if (b2.root_ref < 0) {
+        out->root_ref = b2.root_ref;
+        return;
+    }
+    out->nodes.reserve(b2.nodes.size() / 3 + 1);
+    Widener w{b2, out};
+    out->root_ref = w.emit(b2.root_ref, 0);
+}
+
 }  // namespace rumi

@@ -56,6 +56,21 @@ struct alignas(16) DevNode {
 };
 static_assert(sizeof(DevNode) == 64, "DevNode must be 64 bytes");
 
+// Interior node of the 4-wide GPU BVH, 128 B: up to four child boxes as one
+// float4 per bound and axis (lo.x of the four children, then lo.y, ...), the
+// four child refs (as in DevNode; kNoChild marks an unused slot) and a pad.
+// Built by collapsing the binary tree: a node takes its two children and
+// keeps replacing the interior child of largest surface area by that child's
+// two children until it holds four.
+struct alignas(16) DevNode4 {
+    float lo[3][4];
+    float hi[3][4];
+    int32_t ref[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(DevNode4) == 128, "DevNode4 must be 128 bytes");
+constexpr int32_t kNoChild = -2147483647 - 1;
+
 // Leaf-ordered triangle, 48 B: v0, e1 = v1 - v0, e2 = v2 - v0 (the same FP32
 // subtractions Triangle::Intersect performs per test), input index.
 //   a = v0.x v0.y v0.z e1.x ; b = e1.y e1.z e2.x e2.y ; c = e2.z id 0 0
@@ -102,6 +117,14 @@ struct FlatBvh {
 };
 void flatten(const BuildResult& br, FlatBvh* out);
 
+// The 4-wide form of a flattened tree (leaf refs and triangles unchanged).
+struct FlatBvh4 {
+    std::vector<DevNode4> nodes;
+    int32_t root_ref = 0;
+    int32_t max_depth = 0;  // wide levels to the deepest leaf
+};
+void widen(const FlatBvh& b2, FlatBvh4* out);
+
 double sah_cost_of(const BuildResult& br);
 
 }  // namespace rumi
@@ -118,7 +141,7 @@ struct rumi_ctx {
     uint32_t* dev_counters = nullptr;
     std::atomic<uint32_t> next_counter{0};
     uint32_t* take_counter() { return dev_counters + (next_counter.fetch_add(1) % kCounterSlots); }
-    rumi_trace_tuning tuning{128, 16, 16, 2};
+    rumi_trace_tuning tuning{128, 16, 16, 2, 2};
 };
 
 struct rumi_bvh {
@@ -131,4 +154,9 @@ struct rumi_bvh {
     void* dev_nodes = nullptr;  // DevNode[n_inner]
     void* dev_tris = nullptr;   // DevTri[n_prims]
     std::vector<rumi::DevTri> host_tris;
+    // SAH trees: the 4-wide form, usable when its stack bound (three entries
+    // per wide level) fits the traversal stack
+    rumi::FlatBvh4 wide;
+    bool wide_ok = false;
+    void* dev_nodes4 = nullptr;  // DevNode4[wide.nodes.size()]
 };

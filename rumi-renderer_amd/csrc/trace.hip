@@ -11,7 +11,7 @@ using namespace rumi::dev;
 
 namespace {
 
-template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool COUNT, bool SPILL>
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool COUNT, bool SPILL, bool WIDE = false>
 __global__ __launch_bounds__(BLOCK) void k_trace(
     const float4* __restrict__ rays, uint32_t n, const float4* __restrict__ nodes,
     const float4* __restrict__ tris, RootInfo root, int2* __restrict__ hits,
@@ -26,7 +26,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(
     Ray r{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, r0.w};
     float hu = 0.0f, hv = 0.0f;
     int32_t nn = 0, np = 0;
-    const int32_t hit = traverse<BLOCK, STACK, ANY, ORDERED, false, COUNT, SPILL>(
+    const int32_t hit = traverse<BLOCK, STACK, ANY, ORDERED, false, COUNT, SPILL, WIDE>(
         nodes, tris, nullptr, root, r, false, hu, hv, nn, np, s_stk, tid);
     if (ANY && any_out) any_out[i] = hit;
     if (hits) hits[i] = make_int2(hit, __float_as_int(r.tmin));
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_lanes(
 }
 
 // Persistent lanes with a prefetched next ray per lane (run_lanes).
-template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL>
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL, bool WIDE>
 struct QueryItem {
     const float4* rays;
     const float4* nodes;
@@ -129,7 +129,7 @@ struct QueryItem {
     }
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
-        return trav_step<BLOCK, STACK, ANY, ORDERED, false, false, SPILL>(
+        return trav_step<BLOCK, STACK, ANY, ORDERED, false, false, SPILL, WIDE>(
             nodes, tris, nullptr, s, false, nn, np, s_stk, tid, p_stk);
     }
     __device__ void finish(const TravState& s) {
@@ -139,7 +139,7 @@ struct QueryItem {
     }
 };
 
-template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL>
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL, bool WIDE = false>
 __global__ __launch_bounds__(BLOCK) void k_trace_prefetch(
     const float4* __restrict__ rays, uint32_t n, const float4* __restrict__ nodes,
     const float4* __restrict__ tris, RootInfo root, int2* __restrict__ hits,
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_prefetch(
     __shared__ int2 s_stk[STACK * BLOCK];
     constexpr int kSpill = SPILL ? kMaxDepth - STACK : 1;
     int2 p_stk[kSpill];
-    QueryItem<BLOCK, STACK, ANY, ORDERED, SPILL> item;
+    QueryItem<BLOCK, STACK, ANY, ORDERED, SPILL, WIDE> item;
     item.rays = rays;
     item.nodes = nodes;
     item.tris = tris;
@@ -182,35 +182,38 @@ struct TraceArgs {
     int stack;          // persistent launches: LDS stack entries per lane (8, 16)
     bool persistent;
     int persistent_kind;  // 1 refill on a threshold, 2 prefetched next ray
+    bool wide;            // nodes is the 4-wide tree (near-first order only)
+    int depth;            // stack entries a traversal can need
 };
 
 constexpr int kBlock = 128;
 
 // One ray per lane (the instrumented launches, and tuning.persistent < 0).
-template <int STACK, bool ANY, bool ORDERED, bool COUNT, bool SPILL>
+template <int STACK, bool ANY, bool ORDERED, bool COUNT, bool SPILL, bool WIDE = false>
 hipError_t launch_rays(const TraceArgs& a, hipStream_t s) {
     const uint32_t grid = (a.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((k_trace<kBlock, STACK, ANY, ORDERED, COUNT, SPILL>), dim3(grid),
+    hipLaunchKernelGGL((k_trace<kBlock, STACK, ANY, ORDERED, COUNT, SPILL, WIDE>), dim3(grid),
                        dim3(kBlock), 0, s, a.rays, a.n, a.nodes, a.tris, a.root, a.hits, a.uv,
                        a.any_out, a.cnt_nodes, a.cnt_prims);
     return hipGetLastError();
 }
 
 // Persistent lanes: as many blocks as stay resident, refilled from a counter.
-template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL>
+// The 4-wide tree runs with prefetching lanes only.
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool SPILL, bool WIDE = false>
 hipError_t launch_lanes(const TraceArgs& a, hipStream_t s) {
     const uint32_t grid = (a.n + BLOCK - 1) / BLOCK;
     int per_cu = 0;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, a.counter);
-    if (a.persistent_kind == 2) {
-        auto kern = k_trace_prefetch<BLOCK, STACK, ANY, ORDERED, SPILL>;
+    if (WIDE || a.persistent_kind == 2) {
+        auto kern = k_trace_prefetch<BLOCK, STACK, ANY, ORDERED, SPILL, WIDE>;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BLOCK, 0) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         const uint32_t resident = (uint32_t)per_cu * (uint32_t)a.grid_cap;
         hipLaunchKernelGGL(kern, dim3(grid < resident ? grid : resident), dim3(BLOCK), 0, s, a.rays,
                            a.n, a.nodes, a.tris, a.root, a.hits, a.uv, a.any_out, a.counter);
-    } else {
+    } else if constexpr (!WIDE) {
         auto kern = k_trace_lanes<BLOCK, STACK, ANY, ORDERED, SPILL>;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BLOCK, 0) != hipSuccess ||
             per_cu < 1)
@@ -223,20 +226,32 @@ hipError_t launch_lanes(const TraceArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int BLOCK, int STACK, bool ANY, bool ORDERED>
+template <int BLOCK, int STACK, bool ANY, bool ORDERED, bool WIDE = false>
 hipError_t lanes_depth(const TraceArgs& a, int depth, hipStream_t s) {
-    return depth <= STACK ? launch_lanes<BLOCK, STACK, ANY, ORDERED, false>(a, s)
-                          : launch_lanes<BLOCK, STACK, ANY, ORDERED, true>(a, s);
+    return depth <= STACK ? launch_lanes<BLOCK, STACK, ANY, ORDERED, false, WIDE>(a, s)
+                          : launch_lanes<BLOCK, STACK, ANY, ORDERED, true, WIDE>(a, s);
 }
 
-template <int BLOCK, bool ANY, bool ORDERED>
+template <int BLOCK, bool ANY, bool ORDERED, bool WIDE = false>
 hipError_t lanes_stack(const TraceArgs& a, int depth, hipStream_t s) {
-    return a.stack == 8 ? lanes_depth<BLOCK, 8, ANY, ORDERED>(a, depth, s)
-                        : lanes_depth<BLOCK, 16, ANY, ORDERED>(a, depth, s);
+    return a.stack == 8 ? lanes_depth<BLOCK, 8, ANY, ORDERED, WIDE>(a, depth, s)
+                        : lanes_depth<BLOCK, 16, ANY, ORDERED, WIDE>(a, depth, s);
 }
 
 template <bool ANY, bool ORDERED, bool COUNT>
 hipError_t launch_depth(const TraceArgs& a, int depth, hipStream_t s) {
+    if constexpr (ORDERED) {
+        if (a.wide) {
+            if (!COUNT && a.persistent) {
+                if (a.block == 64) return lanes_stack<64, ANY, true, true>(a, depth, s);
+                if (a.block == 256) return lanes_stack<256, ANY, true, true>(a, depth, s);
+                return lanes_stack<128, ANY, true, true>(a, depth, s);
+            }
+            if (depth <= 16) return launch_rays<16, ANY, true, COUNT, false, true>(a, s);
+            if (depth <= 32) return launch_rays<32, ANY, true, COUNT, false, true>(a, s);
+            return launch_rays<32, ANY, true, COUNT, true, true>(a, s);
+        }
+    }
     if (!COUNT && a.persistent) {
         if (a.block == 64) return lanes_stack<64, ANY, ORDERED>(a, depth, s);
         if (a.block == 256) return lanes_stack<256, ANY, ORDERED>(a, depth, s);
@@ -248,9 +263,9 @@ hipError_t launch_depth(const TraceArgs& a, int depth, hipStream_t s) {
 }
 
 template <bool ANY, bool COUNT>
-hipError_t launch_order(const TraceArgs& a, bool ordered, int depth, hipStream_t s) {
-    return ordered ? launch_depth<ANY, true, COUNT>(a, depth, s)
-                   : launch_depth<ANY, false, COUNT>(a, depth, s);
+hipError_t launch_order(const TraceArgs& a, bool ordered, hipStream_t s) {
+    return ordered ? launch_depth<ANY, true, COUNT>(a, a.depth, s)
+                   : launch_depth<ANY, false, COUNT>(a, a.depth, s);
 }
 
 int check_common(rumi_ctx* ctx, const rumi_bvh* bvh, const void* rays, uint32_t n,
@@ -277,7 +292,10 @@ int check_common(rumi_ctx* ctx, const rumi_bvh* bvh, const void* rays, uint32_t 
     return RUMI_OK;
 }
 
-TraceArgs base_args(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* rays, uint32_t n) {
+// ordered: near-first order (the 4-wide tree is used when the context asks
+// for it and the tree has one).
+TraceArgs base_args(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* rays, uint32_t n,
+                    bool ordered) {
     TraceArgs a{};
     a.counter = ctx->take_counter();
     a.grid_cap = ctx->cu_count;
@@ -295,6 +313,14 @@ TraceArgs base_args(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* rays, ui
         a.root.hi[k] = bvh->flat.root_hi[k];
     }
     a.root.ref = bvh->flat.root_ref;
+    a.depth = bvh->flat.max_depth;
+    a.wide = ordered && ctx->tuning.width == 4 && bvh->wide_ok &&
+             (bvh->dev_nodes4 != nullptr || bvh->wide.nodes.empty());
+    if (a.wide) {
+        a.nodes = reinterpret_cast<const float4*>(bvh->dev_nodes4);
+        a.root.ref = bvh->wide.root_ref;
+        a.depth = 3 * bvh->wide.max_depth;
+    }
     return a;
 }
 
@@ -315,10 +341,10 @@ extern "C" int rumi_trace_closest(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi
     if (((uintptr_t)hits & 7u) || ((uintptr_t)uv & 7u))
         return fail(RUMI_ERR_INVALID, "rumi_trace_closest: hits and uv must be 8-byte aligned");
     if (n == 0) return RUMI_OK;
-    TraceArgs a = base_args(ctx, bvh, rays, n);
+    TraceArgs a = base_args(ctx, bvh, rays, n, ordered);
     a.hits = reinterpret_cast<int2*>(hits);
     a.uv = reinterpret_cast<float2*>(uv);
-    return finish(launch_order<false, false>(a, ordered, bvh->flat.max_depth, (hipStream_t)stream),
+    return finish(launch_order<false, false>(a, ordered, (hipStream_t)stream),
                   "rumi_trace_closest");
 }
 
@@ -329,9 +355,9 @@ extern "C" int rumi_trace_any(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray
     if (int rc = check_common(ctx, bvh, rays, n, flags, &ordered)) return rc;
     if (n && !prim) return fail(RUMI_ERR_INVALID, "rumi_trace_any: null output");
     if (n == 0) return RUMI_OK;
-    TraceArgs a = base_args(ctx, bvh, rays, n);
+    TraceArgs a = base_args(ctx, bvh, rays, n, ordered);
     a.any_out = prim;
-    return finish(launch_order<true, false>(a, ordered, bvh->flat.max_depth, (hipStream_t)stream),
+    return finish(launch_order<true, false>(a, ordered, (hipStream_t)stream),
                   "rumi_trace_any");
 }
 
@@ -345,26 +371,29 @@ extern "C" int rumi_trace_count(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_r
         return fail(RUMI_ERR_INVALID, "rumi_trace_count: null output");
     if ((uintptr_t)hits & 7u) return fail(RUMI_ERR_INVALID, "rumi_trace_count: hits must be 8-byte aligned");
     if (n == 0) return RUMI_OK;
-    TraceArgs a = base_args(ctx, bvh, rays, n);
+    TraceArgs a = base_args(ctx, bvh, rays, n, ordered);
     a.cnt_nodes = node_count;
     a.cnt_prims = prim_count;
     a.hits = reinterpret_cast<int2*>(hits);
     const hipError_t e =
-        any ? launch_order<true, true>(a, ordered, bvh->flat.max_depth, (hipStream_t)stream)
-            : launch_order<false, true>(a, ordered, bvh->flat.max_depth, (hipStream_t)stream);
+        any ? launch_order<true, true>(a, ordered, (hipStream_t)stream)
+            : launch_order<false, true>(a, ordered, (hipStream_t)stream);
     return finish(e, "rumi_trace_count");
 }
 
 extern "C" int rumi_trace_set_tuning(rumi_ctx* ctx, const rumi_trace_tuning* t) {
     clear_error();
     if (!ctx) return fail(RUMI_ERR_NO_DEVICE, "rumi_trace_set_tuning: null context");
-    rumi_trace_tuning v = {128, 16, 16, 2};
+    rumi_trace_tuning v = {128, 16, 16, 2, 2};
     if (t) {
         if (t->block) v.block = t->block;
         if (t->stack) v.stack = t->stack;
         if (t->refill) v.refill = t->refill;
         if (t->persistent) v.persistent = t->persistent;
+        if (t->width) v.width = t->width;
     }
+    if (v.width != 2 && v.width != 4)
+        return fail(RUMI_ERR_INVALID, "rumi_trace_set_tuning: width %d not 2 or 4", v.width);
     if (v.block != 64 && v.block != 128 && v.block != 256)
         return fail(RUMI_ERR_INVALID, "rumi_trace_set_tuning: block %d not 64, 128 or 256", v.block);
     if (v.stack != 8 && v.stack != 16)

@@ -26,14 +26,15 @@ def test_trace_choice_follows_the_clock():
 
     def cost(t):  # a made-up landscape: best at block 128, stack 8, refill 32, kind 1
         return (abs(t["block"] - 128) / 64 + abs(t["stack"] - 8) / 8 + abs(t["refill"] - 32) / 8
-                + (5.0 if t["persistent"] < 0 else 0.0) + (0.5 if t["persistent"] == 2 else 0.0))
+                + (5.0 if t["persistent"] < 0 else 0.0) + (0.5 if t["persistent"] == 2 else 0.0)
+                + (0.1 if t["width"] == 4 else 0.0))
 
     timer = lambda fn: (fn(), cost(ctx.current))[1]
     calls = []
     best, table = autotune.tune_trace(ctx, lambda: calls.append(1), timer=timer)
-    assert best == dict(block=128, stack=8, refill=32, persistent=1)
+    assert best == dict(block=128, stack=8, refill=32, persistent=1, width=2)
     assert ctx.current == best
-    assert len(table) == len(autotune.trace_candidates()) == 1 + 3 * 2 + 3 * 2 * 4
+    assert len(table) == len(autotune.trace_candidates()) == 1 + 2 * 3 * 2 + 3 * 2 * 4
     assert len(calls) == len(table)
     assert table[0][0]["persistent"] == -1
 

@@ -214,6 +214,30 @@ def test_product_sah_traversal_matches_brute_force(rumi, oracle, small_mesh):
     assert ((p == bp) | tie_ok(t, bt)).all()
 
 
+@pytest.mark.parametrize("max_leaf", [1, 4])
+def test_oracle_wide_traversal_matches_binary_and_brute_force(rumi, oracle, small_mesh, max_leaf):
+    # The 4-wide form visits the same primitives' boxes in fewer node steps;
+    # the closest distance is order-independent, so it equals the binary
+    # traversal's bit for bit, and the ids equal brute force up to ties.
+    xyz, idx = small_mesh
+    b = rumi.Bvh(None, xyz, idx, max_leaf=max_leaf)
+    ob = flat_from_export(oracle, b.export(), idx.shape[0])
+    rays = rumi.make_rays_uniform(5000, 17, 2.1)
+    bp, bt = oracle.brute_mesh(xyz, idx, rays)
+    p2, t2, n2, c2 = ob.trace_mesh(xyz, idx, rays, ordered=1, counts=True)
+    p4, t4, n4, c4 = ob.trace_mesh(xyz, idx, rays, ordered=2, counts=True)
+    np.testing.assert_array_equal(t4.view(np.int32), t2.view(np.int32))
+    assert ((p4 == bp) | tie_ok(t4, bt)).all()
+    assert (p4 >= 0).sum() > 500
+    assert n4.sum() < 0.7 * n2.sum()
+    pa, _ = ob.trace_mesh(xyz, idx, rays, any_hit=True, ordered=2)
+    np.testing.assert_array_equal(pa >= 0, bp >= 0)
+    # the median tree too
+    lo, hi = oracle.mesh_bounds(xyz, idx)
+    pm, tm = oracle.MedianBvh(lo, hi).trace_mesh(xyz, idx, rays, ordered=2)
+    np.testing.assert_array_equal(tm.view(np.int32), t2.view(np.int32))
+
+
 def test_degenerate_meshes_build(rumi):
     # empty mesh, one triangle, many coincident triangles
     xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
