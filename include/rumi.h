@@ -159,10 +159,13 @@ int rumi_trace_count(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* dev_ray
  * ray per lane.  width 4 traverses SAH trees in their 4-wide form (near-first
  * order; persistent 1 then runs as 2).  Zero fields keep the defaults: block
  * 128 threads, stack 16 LDS entries per lane (deeper entries spill to private
- * memory), refill 16, persistent 2, width 2.  Every setting returns the same
- * hit distances and, within one width, the same bits; between widths 2 and 4
+ * memory), refill 16, persistent 2, width 2.  Within one width every setting
+ * returns the same bits.  Between widths 2 and 4 the hit distances agree and
  * only which of two primitives at exactly equal distance is reported can
- * differ (rumi_trace_count follows the width). */
+ * differ, except for rays with a zero direction component that start on the
+ * plane of a box face: the reference's box test rejects such a box when the
+ * plane is an x face, so the hit can depend on which boxes the tree holds
+ * (rumi_trace_count follows the width). */
 typedef struct {
     int32_t block;       /* 64, 128 or 256 */
     int32_t stack;       /* 8 or 16 */

@@ -62,13 +62,28 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
     return true;
 }
 
-// The tmin-independent part of the node accept test (bvh.cpp:167-168).
+// The tmin-independent part of the node accept test (bvh.cpp:167-168): the
+// decision of slab() followed by tf >= 0 and tmax >= tn, and, for an accepted
+// box, slab()'s tn up to the sign of a zero (tn only meets comparisons), but
+// without branches: each axis's pair is ordered with min/max and the three
+// merged with max/min.  The pair can hold a NaN (0 * inf: origin on a face's
+// plane, direction component zero), where min/max and slab()'s swap-and-
+// compare part ways; slab() never replaces a NaN on x (tn or tf stays NaN and
+// the box is rejected), and never takes one on y or z (no comparison with it
+// is true), which here is an entry of -inf or an exit of +inf.
 __device__ __forceinline__ bool box_static(float lx, float ly, float lz, float hx, float hy,
                                            float hz, float ox, float oy, float oz, float ix,
                                            float iy, float iz, float tmax, float& tn) {
-    float tf;
-    return slab(lx, ly, lz, hx, hy, hz, ox, oy, oz, ix, iy, iz, tn, tf) && tf >= 0.0f &&
-           tmax >= tn;
+    const float x0 = (lx - ox) * ix, x1 = (hx - ox) * ix;
+    const float y0 = (ly - oy) * iy, y1 = (hy - oy) * iy;
+    const float z0 = (lz - oz) * iz, z1 = (hz - oz) * iz;
+    // maxnum/minnum return the other operand for a NaN
+    const float y0n = fmaxf(y0, -INFINITY), y1n = fminf(y1, INFINITY);
+    const float z0n = fmaxf(z0, -INFINITY), z1n = fminf(z1, INFINITY);
+    tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0n, y1n)), fminf(z0n, z1n));
+    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0n, y1n)), fmaxf(z0n, z1n));
+    const bool x_ok = !__builtin_isunordered(x0, x1);
+    return x_ok & (tn <= tf) & (tf >= 0.0f) & (tmax >= tn);
 }
 
 // The tmin part (bvh.cpp:169): no hit yet, or the box starts before it.
@@ -88,24 +103,25 @@ __device__ __forceinline__ bool hit_triangle(const float4& ta, const float4& tb,
     const float py = r.dz * e2x - e2z * r.dx;
     const float pz = r.dx * e2y - e2x * r.dy;
     const float det = px * e1x + py * e1y + pz * e1z;
-    if (-1e-8f < det && det < 1e-8f) return false;
     const float inv_det = 1.0f / det;
     const float tx = r.ox - ta.x, ty = r.oy - ta.y, tz = r.oz - ta.z;
     const float u = (px * tx + py * ty + pz * tz) * inv_det;
-    if (u < 0.0f || u > 1.0f) return false;
     const float qx = ty * e1z - e1y * tz;
     const float qy = tz * e1x - e1z * tx;
     const float qz = tx * e1y - e1x * ty;
     const float v = (qx * r.dx + qy * r.dy + qz * r.dz) * inv_det;
-    if (v < 0.0f || u + v > 1.0f) return false;
     const float t = (qx * e2x + qy * e2y + qz * e2z) * inv_det;
-    if (t < kEps) return false;
-    if (r.tmin > kEps && t > r.tmin) return false;
-    if (r.tmax - t < kEps) return false;
-    r.tmin = t;
-    u_out = u;
-    v_out = v;
-    return true;
+    // the early returns of the sequential test, as one condition: in a wave
+    // some lane nearly always passes each of them, so branching saves nothing
+    const bool ok = !(-1e-8f < det && det < 1e-8f) & !(u < 0.0f || u > 1.0f) &
+                    !(v < 0.0f || u + v > 1.0f) & !(t < kEps) &
+                    !(r.tmin > kEps && t > r.tmin) & !(r.tmax - t < kEps);
+    if (ok) {
+        r.tmin = t;
+        u_out = u;
+        v_out = v;
+    }
+    return ok;
 }
 
 // Sphere::Intersect: a = centre.xyz, radius.  A = 1 (unit direction assumed),
@@ -224,12 +240,37 @@ __device__ __forceinline__ void stack_pop(TravState& s, int32_t& ref, float& tn,
     int2 e;
     if (!SPILL || s.sp < STACK) {
         e = s_stk[s.sp * BLOCK + tid];
+        // one 8-byte read: without this the reference's read is sunk below
+        // the distance test, a second LDS round trip
+        __asm__ volatile("" : "+v"(e.x), "+v"(e.y));
     } else {
         e = p_stk[s.sp - STACK];
         __asm__ volatile("" : "+v"(e.x), "+v"(e.y));  // as in stack_push
     }
     ref = e.x;
     tn = __int_as_float(e.y);
+}
+
+// Picks the next node to visit: a leaf in s.cur moves to the free leaf slot
+// (an empty leaf is dropped), and while s.cur is empty stack entries are
+// popped, those farther than the closest hit so far dropped.  A leaf found
+// while the leaf slot is taken waits in s.cur.
+template <int BLOCK, int STACK, bool SPILL>
+__device__ __forceinline__ void next_node(TravState& s, const int2* s_stk, uint32_t tid,
+                                          const int2* p_stk) {
+    while (true) {
+        const int32_t c = s.cur;
+        const bool is_leaf = c < 0 && c != kNoRef;
+        const bool empty = (~c & 15) == 0;
+        const bool take = is_leaf && !empty && s.leaf == 0;
+        s.leaf = take ? c : s.leaf;
+        s.cur = is_leaf && (empty || take) ? kNoRef : c;
+        if (s.cur != kNoRef || s.sp == 0) break;
+        int32_t rr;
+        float t;
+        stack_pop<BLOCK, STACK, SPILL>(s, rr, t, s_stk, tid, p_stk);
+        s.cur = closer(s.r.tmin, t) ? rr : kNoRef;
+    }
 }
 
 // Near-first traversal with postponed leaves.  A lane holds one node to
@@ -314,24 +355,7 @@ __device__ __forceinline__ bool trav_step_near(const float4* __restrict__ nodes,
             s.cur = a0 ? ref0 : (a1 ? ref1 : kNoRef);
         }
     }
-    // next node to visit; a leaf moves to the free leaf slot
-    while (true) {
-        if (s.cur != kNoRef && s.cur < 0) {
-            if ((~s.cur & 15) == 0) {  // an empty leaf
-                s.cur = kNoRef;
-            } else if (s.leaf != 0) {
-                break;
-            } else {
-                s.leaf = s.cur;
-                s.cur = kNoRef;
-            }
-        }
-        if (s.cur != kNoRef || s.sp == 0) break;
-        int32_t rr;
-        float t;
-        stack_pop<BLOCK, STACK, SPILL>(s, rr, t, s_stk, tid, p_stk);
-        if (closer(r.tmin, t)) s.cur = rr;
-    }
+    next_node<BLOCK, STACK, SPILL>(s, s_stk, tid, p_stk);
     return s.cur != kNoRef || s.leaf != 0;
 }
 
@@ -446,24 +470,7 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
         for (int c = 3; c >= 1; --c)
             if (acc[c]) stack_push<BLOCK, STACK, SPILL>(s, ref[c], tn[c], s_stk, tid, p_stk);
     }
-    // next node to visit; a leaf moves to the free leaf slot
-    while (true) {
-        if (s.cur != kNoRef && s.cur < 0) {
-            if ((~s.cur & 15) == 0) {  // an empty leaf
-                s.cur = kNoRef;
-            } else if (s.leaf != 0) {
-                break;
-            } else {
-                s.leaf = s.cur;
-                s.cur = kNoRef;
-            }
-        }
-        if (s.cur != kNoRef || s.sp == 0) break;
-        int32_t rr;
-        float t;
-        stack_pop<BLOCK, STACK, SPILL>(s, rr, t, s_stk, tid, p_stk);
-        if (closer(r.tmin, t)) s.cur = rr;
-    }
+    next_node<BLOCK, STACK, SPILL>(s, s_stk, tid, p_stk);
     return s.cur != kNoRef || s.leaf != 0;
 }
 

@@ -182,6 +182,76 @@ def test_wide_edge_cases(rumi, oracle, ctx, rays, wide4):
     np.testing.assert_array_equal(bits(t), bits(ot))
 
 
+def axis_rays_on_planes(boxes, n, seed):
+    """Axis-aligned rays whose origins lie on face planes of real boxes in
+    both perpendicular axes: slab products 0 * inf = NaN, with +0 and -0
+    direction components."""
+    rng = np.random.default_rng(seed)
+    rays = np.zeros((n, 8), np.float32)
+    axis = rng.integers(0, 3, n)
+    sign = rng.choice(np.float32([-1.0, 1.0]), n)
+    pick = boxes[rng.integers(0, len(boxes), n)]
+    for k in range(3):
+        rays[:, k] = np.where(rng.integers(0, 2, n) == 0, pick[:, k], pick[:, 3 + k])
+    rays[np.arange(n), axis] = -3.0 * sign
+    d = np.zeros((n, 3), np.float32)
+    d[np.arange(n), axis] = sign
+    d[(d == 0) & (rng.integers(0, 2, (n, 3)) == 1)] = np.float32(-0.0)
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    return rays
+
+
+def seq_vs_minmax_box_tests_differ(rays, boxes):
+    """Count ray/box pairs where the reference's sequential slab test
+    (src/Acceleration/bbox.cpp) and a plain min/max slab test disagree."""
+    with np.errstate(all="ignore"):
+        o, d = rays[:, None, 0:3], rays[:, None, 4:7]
+        inv = np.float32(1.0) / d
+        t0 = (boxes[None, :, 0:3] - o) * inv
+        t1 = (boxes[None, :, 3:6] - o) * inv
+        lo = np.where(t0 > t1, t1, t0)
+        hi = np.where(t0 > t1, t0, t1)
+        tn, tf = lo[..., 0], hi[..., 0]
+        ok = np.ones(tn.shape, bool)
+        for k in (1, 2):
+            ok &= ~((tn > hi[..., k]) | (lo[..., k] > tf))
+            tn = np.where(tn < lo[..., k], lo[..., k], tn)
+            tf = np.where(tf > hi[..., k], hi[..., k], tf)
+        seq = ok & (tf >= 0) & (np.inf >= tn)
+        mn, mx = np.fmin(t0, t1), np.fmax(t0, t1)
+        tn2, tf2 = np.fmax.reduce(mn, axis=-1), np.fmin.reduce(mx, axis=-1)
+        naive = (tn2 <= tf2) & (tf2 >= 0) & (np.inf >= tn2)
+    return int((seq != naive).sum())
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_axis_aligned_rays_on_box_planes(rumi, oracle, ctx, mesh, width):
+    """The branch-free box test keeps the sequential test's NaN behaviour:
+    rays with zero direction components starting on face planes of the
+    tree's own boxes give the oracle's ids and distances bit for bit."""
+    xyz, idx = mesh
+    b = rumi.Bvh(ctx, xyz, idx, max_leaf=4)
+    ex = b.export()
+    boxes = ex["child_boxes"].reshape(-1, 6)
+    ar = axis_rays_on_planes(boxes, 20000, 5)
+    # the inputs reach the cases where a min/max test would decide otherwise
+    assert seq_vs_minmax_box_tests_differ(ar[:400], boxes[::7]) > 0
+    ob = oracle.MedianBvh.from_export(ex, idx.shape[0])
+    ctx.set_trace_tuning(width=width)
+    try:
+        p, t = closest(rumi, ctx, b, ar)
+        pa = anyhit(rumi, ctx, b, ar)
+    finally:
+        ctx.set_trace_tuning()
+    op, ot = ob.trace_mesh(xyz, idx, ar, ordered=1 if width == 2 else 2)
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(bits(t), bits(ot))
+    oa, _ = ob.trace_mesh(xyz, idx, ar, any_hit=True, ordered=1 if width == 2 else 2)
+    np.testing.assert_array_equal(pa, oa)
+    assert (p >= 0).sum() > 1000
+
+
 def test_barycentrics_reconstruct_hit_point(rumi, ctx, mesh, rays):
     xyz, idx = mesh
     b = rumi.Bvh(ctx, xyz, idx)
