@@ -158,8 +158,8 @@ int rumi_trace_count(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* dev_ray
  * a wave from a shared counter once `refill` of its lanes are idle; -1 is one
  * ray per lane.  width 4 traverses SAH trees in their 4-wide form (near-first
  * order; persistent 1 then runs as 2).  Zero fields keep the defaults: block
- * 128 threads, stack 16 LDS entries per lane (deeper entries spill to private
- * memory), refill 16, persistent 2, width 2.  Within one width every setting
+ * 256 threads, stack 16 LDS entries per lane (deeper entries spill to private
+ * memory), refill 16, persistent 2, width 4.  Within one width every setting
  * returns the same bits.  Between widths 2 and 4 the hit distances agree and
  * only which of two primitives at exactly equal distance is reported can
  * differ, except for rays with a zero direction component that start on the

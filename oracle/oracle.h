@@ -116,8 +116,9 @@ typedef struct {
  *              leaves (the GPU's order for SAH trees): the same closest
  *              distance; ids differ only between primitives at exactly
  *              equal distance.
- *            = 2: the same order over the tree's 4-wide form (the GPU's
- *              width-4 traversal).
+ *            = 2: near-first over the tree's 4-wide form, children
+ *              ordered by order_key, the stack keeping key_distance (the
+ *              GPU's width-4 traversal).
  * Returns the primitive index (scene order) or -1; ray->tmin holds t. */
 int or_travel(const or_bvh* bvh, const or_prim* prims, or_ray* ray, int any,
               int ordered, int in_object, or_stats* st);

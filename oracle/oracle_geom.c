@@ -596,9 +596,35 @@ static int wide_children(const or_bvh* b, int node, int out[4])
     return n;
 }
 
+/* Order key of an accepted child of a 4-wide node: the entry distance's
+ * bits mapped to an unsigned integer of the same order (either zero as +0),
+ * the two low bits replaced by the child's slot.  key_distance reads back
+ * the distance with those two bits cleared, never larger than the entry
+ * distance; it is what the stack keeps. */
+#define NO_KEY 0xffffffffu
+
+static uint32_t order_key(float tn, int slot)
+{
+    float z = tn + 0.0f;
+    int32_t bits;
+    uint32_t u;
+    memcpy(&bits, &z, 4);
+    u = bits < 0 ? ~(uint32_t)bits : (uint32_t)bits | 0x80000000u;
+    return (u & ~3u) | (uint32_t)slot;
+}
+
+static float key_distance(uint32_t key)
+{
+    uint32_t u = key & ~3u;
+    uint32_t bits = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    memcpy(&f, &bits, 4);
+    return f;
+}
+
 /* travel_near over the 4-wide form: the accepted children are ordered by
- * entry distance (ties by slot), the nearest is visited next and the others
- * stacked farthest first (trav_step_near4 in csrc/traverse.h). */
+ * order_key, the first is visited next and the others stacked last first
+ * with their key_distance (trav_step_near4 in csrc/traverse.h). */
 static int travel_near4(const or_bvh* b, const or_prim* prims, or_ray* ray, int any,
                         int in_object, or_stats* st)
 {
@@ -619,36 +645,28 @@ static int travel_near4(const or_bvh* b, const or_prim* prims, or_ray* ray, int 
             }
         }
         if (at_node) {
-            int ch[4], acc[4], slot[4], i, n;
-            float tn[4];
+            int ch[4], i, j, n;
+            uint32_t key[4];
             if (st) st->nodes++;
             n = wide_children(b, cur, ch);
             for (i = 0; i < 4; ++i) {
-                slot[i] = i;
-                tn[i] = 0.0f;
-                acc[i] = 0;
-                if (i < n) acc[i] = static_accept(b, ch[i], ray, &tn[i]) && tmin_accept(ray, tn[i]);
-                else ch[i] = -1;
+                float tn = 0.0f;
+                key[i] = NO_KEY;
+                if (i < n && static_accept(b, ch[i], ray, &tn) && tmin_accept(ray, tn))
+                    key[i] = order_key(tn, i);
             }
-            static const int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
-            for (i = 0; i < 5; ++i) {
-                int x = net[i][0], y = net[i][1], before;
-                if (acc[y] != acc[x]) before = acc[y];
-                else if (tn[y] < tn[x]) before = 1;
-                else if (tn[x] < tn[y]) before = 0;
-                else before = slot[y] < slot[x];
-                if (before) {
-                    int ta = acc[x], tc = ch[x], ts = slot[x];
-                    float tt = tn[x];
-                    acc[x] = acc[y]; ch[x] = ch[y]; slot[x] = slot[y]; tn[x] = tn[y];
-                    acc[y] = ta; ch[y] = tc; slot[y] = ts; tn[y] = tt;
+            /* ascending keys (distinct when accepted), rejected last */
+            for (i = 1; i < 4; ++i)
+                for (j = i; j > 0 && key[j] < key[j - 1]; --j) {
+                    uint32_t tk = key[j]; int tc = ch[j];
+                    key[j] = key[j - 1]; ch[j] = ch[j - 1];
+                    key[j - 1] = tk; ch[j - 1] = tc;
                 }
-            }
-            cur = acc[0] ? ch[0] : -1;
+            cur = key[0] != NO_KEY ? ch[0] : -1;
             for (i = 3; i >= 1; --i)
-                if (acc[i]) {
+                if (key[i] != NO_KEY) {
                     stack[sp].node = ch[i];
-                    stack[sp].tnear = tn[i];
+                    stack[sp].tnear = key_distance(key[i]);
                     ++sp;
                 }
         }

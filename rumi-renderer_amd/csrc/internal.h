@@ -141,7 +141,7 @@ struct rumi_ctx {
     uint32_t* dev_counters = nullptr;
     std::atomic<uint32_t> next_counter{0};
     uint32_t* take_counter() { return dev_counters + (next_counter.fetch_add(1) % kCounterSlots); }
-    rumi_trace_tuning tuning{128, 16, 16, 2, 2};
+    rumi_trace_tuning tuning{256, 16, 16, 2, 4};
 };
 
 struct rumi_bvh {

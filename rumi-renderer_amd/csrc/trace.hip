@@ -384,7 +384,7 @@ extern "C" int rumi_trace_count(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_r
 extern "C" int rumi_trace_set_tuning(rumi_ctx* ctx, const rumi_trace_tuning* t) {
     clear_error();
     if (!ctx) return fail(RUMI_ERR_NO_DEVICE, "rumi_trace_set_tuning: null context");
-    rumi_trace_tuning v = {128, 16, 16, 2, 2};
+    rumi_trace_tuning v = {256, 16, 16, 2, 4};
     if (t) {
         if (t->block) v.block = t->block;
         if (t->stack) v.stack = t->stack;

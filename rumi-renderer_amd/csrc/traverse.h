@@ -359,9 +359,28 @@ __device__ __forceinline__ bool trav_step_near(const float4* __restrict__ nodes,
     return s.cur != kNoRef || s.leaf != 0;
 }
 
-// The same order over 4-wide nodes (DevNode4, 8 float4 per node): the
-// accepted children are ordered by entry distance (ties by slot), the nearest
-// becomes s.cur and the others are pushed farthest first.  Half the node
+// Order keys of a 4-wide node's accepted children: the entry distance's bits
+// mapped to an unsigned integer of the same order (a zero of either sign as
+// +0), the two low bits replaced by the child's slot, so keys are distinct
+// and sort by distance, ties and near-ties (within 4 units in the last place)
+// by slot.  key_distance() reads back a distance no larger than the entry
+// distance (the two bits cleared), which the stack keeps for the test on pop.
+constexpr uint32_t kNoKey = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t order_key(float tn, int slot) {
+    const int32_t b = __float_as_int(tn + 0.0f);
+    const uint32_t u = b < 0 ? ~(uint32_t)b : (uint32_t)b | 0x80000000u;
+    return (u & ~3u) | (uint32_t)slot;
+}
+
+__device__ __forceinline__ float key_distance(uint32_t key) {
+    const uint32_t u = key & ~3u;
+    return __int_as_float((int32_t)((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u));
+}
+
+// Near-first order over 4-wide nodes (DevNode4, 8 float4 per node): the
+// accepted children are ordered by order_key, the first becomes s.cur and the
+// others are pushed last first, each with its key_distance.  Half the node
 // steps of the binary tree, four boxes per fetch.
 template <int BLOCK, int STACK, bool ANY, bool MIXED, bool COUNT, bool SPILL>
 __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes,
@@ -425,50 +444,44 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
         const float blx[4] = {lx.x, lx.y, lx.z, lx.w}, bly[4] = {ly.x, ly.y, ly.z, ly.w},
                     blz[4] = {lz.x, lz.y, lz.z, lz.w}, bhx[4] = {hx.x, hx.y, hx.z, hx.w},
                     bhy[4] = {hy.x, hy.y, hy.z, hy.w}, bhz[4] = {hz.x, hz.y, hz.z, hz.w};
-        float tn[4];
-        bool acc[4];
-        int slot[4];
+        uint32_t key[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            tn[c] = 0.0f;
-            acc[c] = ref[c] != kNoRef &&
-                     box_static(blx[c], bly[c], blz[c], bhx[c], bhy[c], bhz[c], r.ox, r.oy, r.oz, s.ix,
-                                s.iy, s.iz, r.tmax, tn[c]) &&
-                     closer(r.tmin, tn[c]);
-            slot[c] = c;
+            float tn;
+            const bool acc = (ref[c] != kNoRef) &
+                             box_static(blx[c], bly[c], blz[c], bhx[c], bhy[c], bhz[c], r.ox, r.oy,
+                                        r.oz, s.ix, s.iy, s.iz, r.tmax, tn) &
+                             closer(r.tmin, tn);
+            key[c] = acc ? order_key(tn, c) : kNoKey;
         }
-        // order: accepted first, then by entry distance, then by slot
-        auto before = [&](int a, int b) {
-            if (acc[a] != acc[b]) return acc[a];
-            if (tn[a] < tn[b]) return true;
-            if (tn[b] < tn[a]) return false;
-            return slot[a] < slot[b];
-        };
+        // sorting network over (key, ref): ascending keys, rejected last
         auto cmpx = [&](int a, int b) {
-            if (before(b, a)) {
-                const bool xa = acc[a];
-                acc[a] = acc[b];
-                acc[b] = xa;
-                const float xt = tn[a];
-                tn[a] = tn[b];
-                tn[b] = xt;
-                const int32_t xr = ref[a];
-                ref[a] = ref[b];
-                ref[b] = xr;
-                const int xs = slot[a];
-                slot[a] = slot[b];
-                slot[b] = xs;
-            }
+            const bool sw = key[b] < key[a];
+            const uint32_t ka = key[a], kb = key[b];
+            key[a] = sw ? kb : ka;
+            key[b] = sw ? ka : kb;
+            const int32_t ra = ref[a], rb = ref[b];
+            ref[a] = sw ? rb : ra;
+            ref[b] = sw ? ra : rb;
         };
         cmpx(0, 1);
         cmpx(2, 3);
         cmpx(0, 2);
         cmpx(1, 3);
         cmpx(1, 2);
-        s.cur = acc[0] ? ref[0] : kNoRef;
+        s.cur = key[0] != kNoKey ? ref[0] : kNoRef;
+        if (!SPILL || __all(s.sp + 3 <= STACK)) {
+            // no lane of the wave can reach private memory: LDS pushes only
 #pragma unroll
-        for (int c = 3; c >= 1; --c)
-            if (acc[c]) stack_push<BLOCK, STACK, SPILL>(s, ref[c], tn[c], s_stk, tid, p_stk);
+            for (int c = 3; c >= 1; --c)
+                if (key[c] != kNoKey)
+                    stack_push<BLOCK, STACK, false>(s, ref[c], key_distance(key[c]), s_stk, tid, p_stk);
+        } else {
+#pragma unroll
+            for (int c = 3; c >= 1; --c)
+                if (key[c] != kNoKey)
+                    stack_push<BLOCK, STACK, SPILL>(s, ref[c], key_distance(key[c]), s_stk, tid, p_stk);
+        }
     }
     next_node<BLOCK, STACK, SPILL>(s, s_stk, tid, p_stk);
     return s.cur != kNoRef || s.leaf != 0;

@@ -130,6 +130,15 @@ def test_counts_match_oracle(rumi, oracle, ctx, mesh, rays, any_hit):
     np.testing.assert_array_equal(pc, opc)
 
 
+@pytest.fixture(autouse=True)
+def binary(ctx):
+    """Tests compare with the oracle's binary near-first order unless they
+    ask for the 4-wide tree (wide4) or set a tuning themselves."""
+    ctx.set_trace_tuning(width=2)
+    yield
+    ctx.set_trace_tuning()
+
+
 @pytest.fixture
 def wide4(ctx):
     ctx.set_trace_tuning(width=4)
@@ -408,4 +417,4 @@ def test_every_tuning_gives_identical_hits(rumi, ctx, small_mesh):
     best, table = autotune.tune_trace(ctx, lambda: rumi.trace_closest(ctx, bvh, rays, ref_h))
     print("tuning", best, [(c, round(s * 1e6, 1)) for c, s in table])
     ctx.set_trace_tuning()
-    assert ctx.trace_tuning() == dict(block=128, stack=16, refill=16, persistent=2, width=2)
+    assert ctx.trace_tuning() == dict(block=256, stack=16, refill=16, persistent=2, width=4)
