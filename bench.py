@@ -228,6 +228,72 @@ def measured_traffic(label, rays):
     return int(round(k["bytes_per_launch"]))
 
 
+def model_leg(rumi, ctx, args, log, scenes):
+    """The reference's model scene (src/Scene/Build Model.txt): its
+    100,000-triangle dragon.obj loaded by rumi_obj_load with the scene's
+    transform, in glass with absorption, 1200 x 1000, depth 20; 4 spp per
+    render (the reference's film takes 1).  GPU samples/s; the same seed at
+    60 x 50 on the GPU and in the oracle (relative L1 between the two: the
+    same samples up to rounding); CPU baseline = that oracle render on one
+    thread."""
+    import gzip
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    w, h, spp, depth = 1200, 1000, 4, 20
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "dragon.obj")
+        with open(path, "wb") as f:
+            f.write(gzip.open(os.path.join(ROOT, "tests", "golden", "models", "dragon.obj.gz")).read())
+        t0 = time.perf_counter()
+        model = rumi.load_obj(path, scenes.model_transform(0.3))
+        load_s = time.perf_counter() - t0
+    sc = scenes.model_scene(model, w, h)
+    scene = rumi.Scene(ctx, sc)
+    img, st = scene.render(w, h, spp, depth, seed=1)
+    torch.cuda.synchronize()
+    times = []
+    for k in range(args.render_runs):
+        t0 = time.perf_counter()
+        img, st = scene.render(w, h, spp, depth, seed=10 + k)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    scene.close()
+    sec = min(times)
+    sps = st.samples / sec
+    import oracle
+    cw, ch = 60, 50
+    small = scenes.model_scene(model, cw, ch)
+    s_small = rumi.Scene(ctx, small)
+    gi, _ = s_small.render(cw, ch, spp, depth, seed=7)
+    torch.cuda.synchronize()
+    gi = gi.cpu().numpy()
+    s_small.close()
+    t0 = time.perf_counter()
+    ci, _ = oracle.render_pt(small, cw, ch, spp, depth, seed=7, nthreads=1)
+    cpu_sec = time.perf_counter() - t0
+    cpu_sps = cw * ch * spp / cpu_sec
+    l1 = float(np.abs(gi - ci).sum() / max(np.abs(ci).sum(), 1e-30))
+    out = {
+        "samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4),
+        "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
+        "max_depth": depth, "triangles": int(model["idx"].shape[0]),
+        "obj_load_s": round(load_s, 3),
+        "extension_rays": int(st.extension_rays), "shadow_rays": int(st.shadow_rays),
+        "rays_per_s": round((st.extension_rays + st.shadow_rays) / sec, 1),
+        "same_seed_rel_l1_60x50": l1,
+        "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1, "kind": "port",
+                         "sample": f"60x50 at {spp} spp, depth {depth}, the oracle's FP64 "
+                                   f"restatement of PathTracer::Trace ({cpu_sec:.2f}s)"},
+        "speedup_vs_cpu": round(sps / cpu_sps, 1),
+    }
+    log(f"model_pt (dragon): {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.1f} ms), same-seed L1 {l1:.2e}, "
+        f"CPU {cpu_sps:.0f} samples/s")
+    return out
+
+
 def render_legs(rumi, ctx, args, log):
     """Path tracing: the 512^2 x 16 spp Cornell box (depth 5) and the 512^2 x
     32 spp media scene (depth 16), samples/s on the GPU; rel-MSE against the
@@ -297,6 +363,7 @@ def render_legs(rumi, ctx, args, log):
         log(f"{name}: {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.1f} ms), "
             f"rel-MSE ratio {out[name]['rel_mse_64px']['ratio']:.3f}, CPU {cpu_sps:.0f} samples/s")
         scene.close()
+    out["model_pt"] = model_leg(rumi, ctx, args, log, scenes)
     out["media_gradient"] = gradient_leg(rumi, ctx, args, log, scenes)
     out["cornell_bdpt"] = bdpt_leg(rumi, ctx, args, log, scenes)
     if not args.no_batch:

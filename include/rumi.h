@@ -390,6 +390,39 @@ int rumi_make_icosphere(uint32_t freq, uint64_t seed, float amplitude, float* ho
  * tmax = +inf; PCG32(seed, stream = ray index).  Host memory. */
 int rumi_make_rays_uniform(uint64_t n, uint64_t seed, float half_extent, rumi_ray* host_rays);
 
+/* ------------------------------------------------------------------ models */
+/* An OBJ model, loaded as Model::Model(file, transform) loads it
+ * (Model::LoadObjectFile, src/Shape/model.cpp:72-338): "v", "vt", "vn" and
+ * "f" records found by the reference's lexical rules (a record tag is up to
+ * four bytes ending at a space or newline; a face keeps its first three
+ * corners; face indices are parsed digit by digit in FP32, a '-' makes a
+ * corner relative to the file's total counts); vertices go through the
+ * row-major 4x4 `transform` (Mat4x4 * Point3, fourth column added), normals
+ * through the transposed inverse computed by the reference's unpivoted
+ * elimination (Shape::Shape, src/Shape/shape.cpp:29-37; Inverse,
+ * src/Kernel/matrix.cpp:67-133), then normalised; a file without "vn" lines,
+ * or a zero normal, gets the transformed face's normal.  Texture
+ * coordinates are read and dropped, as the reference's triangles carry none.
+ *
+ * The reference places a Model in its scene tree as one shape with its own
+ * BVHAccel inside (Model::Intersect, model.cpp:71-74).  Here the triangles
+ * join the scene as one RUMI_SHAPE_MESH (xyz = 3 vertices per triangle,
+ * idx = 0, 1, 2, ..., normals = the vertex normals) under the scene's single
+ * tree: the same closest hit, the same traversal kernels.
+ *
+ * Malformed faces whose reference behaviour is undefined (fewer than three
+ * corners, a fourth '/' field, an index outside the file's records) are
+ * rejected with RUMI_ERR_INVALID.  transform NULL = identity. */
+typedef struct rumi_obj rumi_obj;
+
+int rumi_obj_load(const char* path, const float* transform, rumi_obj** out);
+/* n_triangles; xyz and normals: 9 floats per triangle (corners 0, 1, 2),
+ * host memory owned by the model; counts: the file's v, vt, vn and f record
+ * counts (4 values).  Any output may be NULL. */
+int rumi_obj_get(const rumi_obj* model, uint32_t* n_triangles, const float** xyz,
+                 const float** normals, uint32_t* counts);
+int rumi_obj_destroy(rumi_obj* model);
+
 /* PCG32 (O'Neill, pcg32_srandom_r / pcg32_random_r): writes n outputs of
  * the generator seeded with (initstate, initseq).  Host memory. */
 int rumi_pcg32_sequence(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* host_out);

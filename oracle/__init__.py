@@ -74,6 +74,11 @@ def lib() -> ctypes.CDLL:
         L.or_time_median6_mesh.argtypes = [vp, vp, c_int, vp, c_int, c_int, vp, vp,
                                            POINTER(c_double)]
         L.or_time_median6_mesh.restype = c_double
+        L.or_obj_load.argtypes = [ctypes.c_char_p, vp, POINTER(vp), POINTER(vp), POINTER(c_int), vp]
+        L.or_obj_free.argtypes = [vp]
+        L.or_obj_free.restype = None
+        L.or_mat4_inverse.argtypes = [vp, vp]
+        L.or_mat4_inverse.restype = None
         _lib = L
     return _lib
 
@@ -209,6 +214,35 @@ class MedianBvh:
             lib().or_bvh_free(byref(self.b))
         except Exception:
             pass
+
+
+def load_obj(path, transform=None):
+    """or_obj_load: (xyz (n, 9) float32, normals (n, 9) float32, counts (4,)
+    int32: the file's v / vt / vn / f records).  Raises ValueError where the
+    reference's behaviour is undefined."""
+    xf = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
+    px, pn = c_void_p(), c_void_p()
+    nt = c_int(0)
+    counts = np.zeros(4, np.int32)
+    rc = lib().or_obj_load(os.fsencode(path), _p(xf), byref(px), byref(pn), byref(nt), _p(counts))
+    if rc != 0:
+        raise ValueError(f"or_obj_load: {path}: malformed face or index")
+    n = nt.value
+    try:
+        xyz = np.ctypeslib.as_array(ctypes.cast(px, POINTER(ctypes.c_float)), (max(n, 1) * 9,))[:9 * n]
+        nrm = np.ctypeslib.as_array(ctypes.cast(pn, POINTER(ctypes.c_float)), (max(n, 1) * 9,))[:9 * n]
+        return xyz.reshape(n, 9).copy(), nrm.reshape(n, 9).copy(), counts
+    finally:
+        lib().or_obj_free(px)
+        lib().or_obj_free(pn)
+
+
+def mat4_inverse(m):
+    """The reference's Inverse (unpivoted elimination) of a row-major 4x4."""
+    a = np.ascontiguousarray(m, np.float32).reshape(16)
+    out = np.empty(16, np.float32)
+    lib().or_mat4_inverse(_p(a), _p(out))
+    return out.reshape(4, 4)
 
 
 def mesh_bounds(xyz, idx):

@@ -202,6 +202,17 @@ int or_render_bdpt(const float* shapes, int n_shapes, const float* mats, int n_m
                    int n_eye, int n_light, int max_len, double mis_beta, uint64_t seed,
                    int nthreads, double* mean_rgb);
 
+/* OBJ model loading as Model::Model(file, transform) does it
+ * (src/Shape/model.cpp:72-338): 9 vertex and 9 normal floats per triangle
+ * (malloc'ed; free with or_obj_free), the triangle count and the file's
+ * v / vt / vn / f record counts.  transform: row-major 4x4 or NULL.
+ * Returns -1 on a face whose reference behaviour is undefined. */
+int or_obj_load(const char* path, const float* transform, float** xyz, float** normals,
+                int* n_triangles, int counts[4]);
+void or_obj_free(float* p);
+/* Inverse (src/Kernel/matrix.cpp:67-133), row-major 4x4. */
+void or_mat4_inverse(const float* a, float* out);
+
 #ifdef __cplusplus
 }
 #endif

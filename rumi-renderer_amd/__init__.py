@@ -157,6 +157,10 @@ _SIGNATURES = {
     "rumi_make_icosphere": (c_int, [c_uint32, c_uint64, c_float, c_void_p, c_void_p]),
     "rumi_make_rays_uniform": (c_int, [c_uint64, c_uint64, c_float, c_void_p]),
     "rumi_pcg32_sequence": (c_int, [c_uint64, c_uint64, c_uint32, c_void_p]),
+    "rumi_obj_load": (c_int, [c_char_p, c_void_p, POINTER(c_void_p)]),
+    "rumi_obj_get": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_void_p), POINTER(c_void_p),
+                             c_void_p]),
+    "rumi_obj_destroy": (c_int, [c_void_p]),
 }
 
 _lib = None
@@ -210,6 +214,30 @@ def _stream(stream) -> int:
 
 
 # --------------------------------------------------------------- workloads
+def load_obj(path, transform=None) -> dict:
+    """rumi_obj_load: an OBJ model as Model(file, transform) loads it.
+    Returns dict(xyz=(3n, 3) float32 vertices, three per triangle,
+    normals=(3n, 3) float32 vertex normals, idx=(n, 3) uint32 = 0, 1, 2, ...,
+    counts=(v, vt, vn, f) records): the fields of a "mesh" scene shape."""
+    import os
+    xf = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
+    h = c_void_p()
+    check(lib().rumi_obj_load(os.fsencode(str(path)), _ptr(xf), byref(h)))
+    try:
+        nt = c_uint32(0)
+        px, pn = c_void_p(), c_void_p()
+        counts = np.zeros(4, np.uint32)
+        check(lib().rumi_obj_get(h, byref(nt), byref(px), byref(pn), _ptr(counts)))
+        n = nt.value
+        fp = POINTER(c_float)
+        xyz = np.ctypeslib.as_array(ctypes.cast(px, fp), (max(n, 1) * 9,))[:9 * n].copy()
+        nrm = np.ctypeslib.as_array(ctypes.cast(pn, fp), (max(n, 1) * 9,))[:9 * n].copy()
+    finally:
+        lib().rumi_obj_destroy(h)
+    return dict(xyz=xyz.reshape(-1, 3), normals=nrm.reshape(-1, 3),
+                idx=np.arange(3 * n, dtype=np.uint32).reshape(n, 3), counts=counts)
+
+
 def make_icosphere(freq: int, seed: int, amplitude: float):
     nv, nt = 10 * freq * freq + 2, 20 * freq * freq
     xyz = np.empty((nv, 3), np.float32)

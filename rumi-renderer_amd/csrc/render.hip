@@ -125,7 +125,7 @@ __device__ inline V3 frame(V3 u, V3 v, V3 w, float su, float sv, float sw) {
 
 // ------------------------------------------------------------------ scene view
 struct SceneView {
-    const float4* nodes;
+    const float4* nodes;  // the scene tree's 4-wide form (DevNode4)
     const float4* prims;
     const float4* rects;
     const DevShape* shapes;
@@ -335,7 +335,7 @@ struct ExtendItem {
     }
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
-        return trav_step<kBlock, kStack, false, true, true, false, SPILL>(
+        return trav_step<kBlock, kStack, false, true, true, false, SPILL, true>(
             nodes, prims, rects, s, in_obj, nn, np, s_stk, tid, p_stk);
     }
     __device__ void finish(const TravState& s) {
@@ -1296,7 +1296,7 @@ struct ShadowItem {
     }
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
-        return trav_step<kBlock, kStack, true, true, true, false, SPILL>(
+        return trav_step<kBlock, kStack, true, true, true, false, SPILL, true>(
             nodes, prims, rects, s, false, nn, np, s_stk, tid, p_stk);
     }
     __device__ void finish(const TravState& s) {
@@ -1543,9 +1543,9 @@ __device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_
         float bu = 0.0f, bv = 0.0f;
         int32_t nn = 0, np = 0;
         const int32_t prim =
-            spill ? traverse<kBlock, kStack, false, true, true, false, true>(
+            spill ? traverse<kBlock, kStack, false, true, true, false, true, true>(
                         sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_stk, tid)
-                  : traverse<kBlock, kStack, false, true, true, false, false>(
+                  : traverse<kBlock, kStack, false, true, true, false, false, true>(
                         sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_stk, tid);
         // the medium along the segment, sampled as the path tracer does
         // (pathtracer.cpp:156-178; bidirectionalpathtracer.cpp:452-473)
@@ -1829,14 +1829,17 @@ __device__ inline float remap0(float f) { return f != 0.0f ? f : 1.0f; }
 
 __device__ inline float mis_pow(float r, float beta) { return beta == 1.0f ? r : powf(r, beta); }
 
+// Connections of strategies q0 .. q0 + n_strat - 1: contributions that need
+// no visibility test go to the film, the others are appended to the shadow
+// queue `sh` (one atomic per wave) for k_shadow, which resolves them with
+// persistent lanes as the path tracer's shadow rays are.
 __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t base, BdptParams bp,
                                                          SceneView sv, const int2* strategies,
-                                                         uint32_t n_strat, float* film, bool spill) {
-    __shared__ int2 s_stk[kStack * kBlock];
-    const uint32_t tid = threadIdx.x;
-    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + tid;
+                                                         uint32_t q0, uint32_t n_strat, float* film,
+                                                         ShadowBuf sh, uint32_t* sh_count) {
+    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (gid >= (uint64_t)n_strat * bb.nb) return;
-    const uint32_t q = (uint32_t)(gid / bb.nb);
+    const uint32_t q = q0 + (uint32_t)(gid / bb.nb);
     const uint32_t j = (uint32_t)(gid % bb.nb);
     const int s = strategies[q].x, t = strategies[q].y;
     const int nE = bb.n_eye[j], nL = bb.n_light[j];
@@ -1984,22 +1987,20 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
                           ((uint64_t)s << 8) | (uint64_t)t);
         if (gc.uniform() < 1.0f - expf(-md.avg_sigma_t * medL)) return;
     }
-    if (visible_test) {
-        const V3 dv = vb - va;
-        const float dist = length(dv);
-        const V3 dir = dv * (1.0f / dist);
-        Ray r{va.x, va.y, va.z, dir.x, dir.y, dir.z, dist, 0.0f};
-        float hu, hv;
-        int32_t nn = 0, np = 0;
-        if (spill)
-            traverse<kBlock, kStack, true, true, true, false, true>(sv.nodes, sv.prims, sv.rects, sv.root,
-                                                                    r, false, hu, hv, nn, np, s_stk, tid);
-        else
-            traverse<kBlock, kStack, true, true, true, false, false>(sv.nodes, sv.prims, sv.rects, sv.root,
-                                                                     r, false, hu, hv, nn, np, s_stk, tid);
-        if (r.tmin > 0.0f) return;
+    const V3 cw = c * w;
+    if (!visible_test) {
+        film_add(film, pixel, cw);
+        return;
     }
-    film_add(film, pixel, c * w);
+    // VisibilityTest (bidirectionalpathtracer.cpp:1018-1048): the segment
+    // va -> vb as an any-hit query, queued; every lane still here appends
+    const V3 dv = vb - va;
+    const float dist = length(dv);
+    const V3 dir = dv * (1.0f / dist);
+    const uint32_t slot = wave_append(sh_count, true);
+    sh.ray[2 * (size_t)slot + 0] = make_float4(va.x, va.y, va.z, 0.0f);
+    sh.ray[2 * (size_t)slot + 1] = make_float4(dir.x, dir.y, dir.z, dist);
+    sh.contrib[slot] = make_float4(cw.x, cw.y, cw.z, __int_as_float((int32_t)pixel));
 }
 
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
@@ -2027,7 +2028,6 @@ SceneView scene_view(const rumi_scene* sc) {
 
 int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
                 rumi_render_stats* stats, hipStream_t stream) {
-    (void)ctx;
     const int nE = desc->n_eye > 0 ? desc->n_eye : desc->max_depth + 1;
     const int nL = desc->n_light > 0 ? desc->n_light : desc->max_depth + 1;
     if (nE < 2 || nE > 64 || nL < 1 || nL > 64)
@@ -2048,8 +2048,13 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     if ((uint64_t)nb > total) nb = (uint32_t)total;
     const size_t vbytes = sizeof(BVert) * (size_t)nb;
     const size_t mbytes = sizeof(float4) * (size_t)nb;
+    // connections are made kConnChunk strategies at a time; each chunk's
+    // visibility tests are queued (48 B each) and resolved before the next
+    const uint32_t kConnChunk = std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)strat.size(),
+                                                                           (1u << 25) / nb));
+    const size_t max_conn = (size_t)kConnChunk * nb;
     const size_t need = (vbytes + mbytes) * (size_t)(nE + nL) + 2 * (size_t)nb + sizeof(int2) * strat.size() +
-                        8192;
+                        3 * sizeof(float4) * max_conn + 8192;
     if (sc->work_bytes < need) {
         if (sc->work) (void)hipFree(sc->work);
         sc->work = nullptr;
@@ -2072,6 +2077,10 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     bb.n_eye = (uint8_t*)take(nb);
     bb.n_light = (uint8_t*)take(nb);
     int2* d_strat = (int2*)take(sizeof(int2) * strat.size());
+    ShadowBuf sh{};
+    sh.ray = (float4*)take(2 * sizeof(float4) * max_conn);
+    sh.contrib = (float4*)take(sizeof(float4) * max_conn);
+    uint32_t* qctl = (uint32_t*)take(256);  // [0] queued connections, [1] k_shadow's fetch counter
     if ((size_t)(w - (char*)sc->work) > sc->work_bytes)
         return fail(RUMI_ERR_INVALID, "rumi_render: internal BDPT work-buffer layout overflow");
     if (hipMemcpyAsync(d_strat, strat.data(), sizeof(int2) * strat.size(), hipMemcpyHostToDevice,
@@ -2091,7 +2100,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     bp.beta = desc->mis_beta;
     bp.film_area = (2.0f * sc->camera.tanfov) * (2.0f * sc->camera.tanfov * sc->camera.y_ratio);
     bp.light_seed = desc->seed ^ 0x2545F4914F6CDD1Dull;
-    const bool spill = sc->max_depth > kStack;
+    const bool spill = sc->stack_need > kStack;
 
     hipEvent_t ev0, ev1;
     (void)hipEventCreate(&ev0);
@@ -2105,9 +2114,21 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
         hipLaunchKernelGGL(k_bdpt_eye, dim3(grid_of(bb.nb)), dim3(kBlock), 0, stream, bb, base, bp, sv, spill);
         hipLaunchKernelGGL(k_bdpt_light, dim3(grid_of(bb.nb)), dim3(kBlock), 0, stream, bb, base, bp, sv,
                            spill);
-        const uint64_t threads = (uint64_t)bb.nb * strat.size();
-        hipLaunchKernelGGL(k_bdpt_connect, dim3(grid_of(threads)), dim3(kBlock), 0, stream, bb, base, bp,
-                           sv, (const int2*)d_strat, (uint32_t)strat.size(), film, spill);
+        for (uint32_t q0 = 0; q0 < (uint32_t)strat.size(); q0 += kConnChunk) {
+            const uint32_t nq = std::min<uint32_t>(kConnChunk, (uint32_t)strat.size() - q0);
+            const uint64_t threads = (uint64_t)bb.nb * nq;
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, qctl, 2u);
+            hipLaunchKernelGGL(k_bdpt_connect, dim3(grid_of(threads)), dim3(kBlock), 0, stream, bb, base, bp,
+                               sv, (const int2*)d_strat, q0, nq, film, sh, qctl);
+            const uint32_t g = (uint32_t)std::min<uint64_t>((threads + kBlock - 1) / kBlock,
+                                                            (uint64_t)ctx->cu_count * 16u);
+            if (spill)
+                hipLaunchKernelGGL((k_shadow<true, kRender, false>), dim3(g), dim3(kBlock), 0, stream, sh,
+                                   qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{});
+            else
+                hipLaunchKernelGGL((k_shadow<false, kRender, false>), dim3(g), dim3(kBlock), 0, stream, sh,
+                                   qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{});
+        }
         ++batches;
     }
     hipLaunchKernelGGL(k_scale, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n,
@@ -2298,7 +2319,7 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     ps.film = film;
     ps.adj = adj;
     ps.tr = Track{nullptr, nullptr};
-    ps.spill = sc->max_depth > kStack;
+    ps.spill = sc->stack_need > kStack;
     ps.mode = mode;
     ps.grid_cap = (uint32_t)ctx->cu_count * 16u;
 
@@ -2656,7 +2677,7 @@ extern "C" int rumi_batch_create(rumi_ctx* ctx, const rumi_batch_item* items, ui
         samples += (uint64_t)t.pixels * (uint64_t)desc->spp;
         pixels += t.pixels;
         texels += sc->texel_count;
-        if (sc->max_depth > kStack) b->spill = true;
+        if (sc->stack_need > kStack) b->spill = true;
         if (samples > 0xffffffffull || pixels > 0xffffffffull / 4) {
             delete b;
             return fail(RUMI_ERR_INVALID, "rumi_batch_create: more than 2^32 - 1 samples in the batch");

@@ -208,6 +208,13 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
     flatten(br, &flat);
     if (flat.max_depth > kMaxDepth)
         return fail(RUMI_ERR_UNSUPPORTED, "scene BVH depth %d exceeds %d", flat.max_depth, kMaxDepth);
+    // the renderer traverses the tree's 4-wide form (near-first, up to three
+    // stack entries per level)
+    FlatBvh4 wide;
+    widen(flat, &wide);
+    if (3 * wide.max_depth > kMaxDepth)
+        return fail(RUMI_ERR_UNSUPPORTED, "scene BVH: 4-wide depth %d needs more than %d stack entries",
+                    wide.max_depth, kMaxDepth);
 
     // rect side table (Rect ctor, src/Shape/rect.cpp:6-15) and leaf records
     std::vector<float> rects;
@@ -352,7 +359,8 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
     sc->n_prims = n;
     sc->n_lights = (uint32_t)lights.size();
     sc->max_depth = flat.max_depth;
-    sc->root_ref = flat.root_ref;
+    sc->stack_need = 3 * wide.max_depth;
+    sc->root_ref = wide.root_ref;
     std::memcpy(sc->root_lo, flat.root_lo, sizeof(sc->root_lo));
     std::memcpy(sc->root_hi, flat.root_hi, sizeof(sc->root_hi));
     sc->material_kinds = kinds;
@@ -405,7 +413,7 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
             break;
         }
         sc->device = ctx->device;
-        if ((rc = upload(&sc->nodes, flat.nodes))) break;
+        if ((rc = upload(&sc->nodes, wide.nodes))) break;
         if ((rc = upload(&sc->prims, leaf))) break;
         if ((rc = upload(&sc->rects, rects))) break;
         if ((rc = upload(&sc->shapes, shapes))) break;

@@ -71,14 +71,15 @@ struct rumi_scene {
     int device = -1;
     uint32_t n_prims = 0;
     uint32_t n_lights = 0;
-    int32_t max_depth = 0;
-    int32_t root_ref = 0;
+    int32_t max_depth = 0;   // binary tree depth
+    int32_t stack_need = 0;  // traversal stack entries of the 4-wide form (3 per level)
+    int32_t root_ref = 0;    // root of the 4-wide form
     float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
     rumi::DevMedium medium{};
     rumi::DevCamera camera{};
     uint32_t material_kinds = 0;  // bit mask of RUMI_MAT_* present
     // device arrays
-    void* nodes = nullptr;      // DevNode[n_inner]
+    void* nodes = nullptr;      // DevNode4[]: the scene tree in its 4-wide form
     void* prims = nullptr;      // leaf-ordered 48-B records
     void* rects = nullptr;      // 64-B rect records
     void* shapes = nullptr;     // DevShape[n_prims] in input order

@@ -448,10 +448,10 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             float tn;
-            const bool acc = (ref[c] != kNoRef) &
-                             box_static(blx[c], bly[c], blz[c], bhx[c], bhy[c], bhz[c], r.ox, r.oy,
-                                        r.oz, s.ix, s.iy, s.iz, r.tmax, tn) &
-                             closer(r.tmin, tn);
+            const int acc = (int)(ref[c] != kNoRef) &
+                            (int)box_static(blx[c], bly[c], blz[c], bhx[c], bhy[c], bhz[c], r.ox, r.oy,
+                                            r.oz, s.ix, s.iy, s.iz, r.tmax, tn) &
+                            (int)closer(r.tmin, tn);
             key[c] = acc ? order_key(tn, c) : kNoKey;
         }
         // sorting network over (key, ref): ascending keys, rejected last

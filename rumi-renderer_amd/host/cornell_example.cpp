@@ -84,8 +84,22 @@ int main(int argc, char** argv) {
         describe();
         return 0;
     }
+    if (argc == 4 && std::strcmp(argv[1], "--model") == 0) {
+        // the model scene's transform: RotationX(0) * Translation(0, -0.99, 0) * Scaling(s, s, s)
+        const float sc = (float)std::atof(argv[3]);
+        const Mat4x4 t = RotationX(0.0f) * Translation(0.0f, -0.99f, 0.0f) * Scaling(sc, sc, sc);
+        Model m(argv[2], t);
+        std::printf("model %u\n", m.Triangles());
+        for (size_t i = 0; i < m._xyz.size(); ++i) {
+            uint32_t a, b;
+            std::memcpy(&a, &m._xyz[i], 4);
+            std::memcpy(&b, &m._normals[i], 4);
+            std::printf("%08x %08x\n", a, b);
+        }
+        return m.Triangles() > 0 ? 0 : 1;
+    }
     if (argc != 6) {
-        std::fprintf(stderr, "usage: %s --describe | W H SPP SEED OUT\n", argv[0]);
+        std::fprintf(stderr, "usage: %s --describe | --model OBJ SCALE | W H SPP SEED OUT\n", argv[0]);
         return 2;
     }
     const unsigned w = (unsigned)std::atoi(argv[1]), h = (unsigned)std::atoi(argv[2]);

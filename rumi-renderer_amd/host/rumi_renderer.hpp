@@ -74,6 +74,55 @@ inline Mat4x4 RotationY(float a) {
     return m;
 }
 
+inline Mat4x4 RotationX(float a) {
+    Mat4x4 m;
+    const float c = std::cos(a), s = std::sin(a);
+    m._m[5] = c;
+    m._m[6] = -s;
+    m._m[9] = s;
+    m._m[10] = c;
+    return m;
+}
+
+inline Mat4x4 RotationZ(float a) {
+    Mat4x4 m;
+    const float c = std::cos(a), s = std::sin(a);
+    m._m[0] = c;
+    m._m[1] = -s;
+    m._m[4] = s;
+    m._m[5] = c;
+    return m;
+}
+
+inline Mat4x4 Translation(float x, float y, float z) {
+    Mat4x4 m;
+    m._m[3] = x;
+    m._m[7] = y;
+    m._m[11] = z;
+    return m;
+}
+
+inline Mat4x4 Scaling(float x, float y, float z) {
+    Mat4x4 m;
+    m._m[0] = x;
+    m._m[5] = y;
+    m._m[10] = z;
+    return m;
+}
+
+// Mat4x4::operator * (src/Kernel/matrix.cpp:18-40): each entry summed left
+// to right in FP32.
+inline Mat4x4 operator*(const Mat4x4& a, const Mat4x4& b) {
+    Mat4x4 o;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            float acc = a._m[i * 4] * b._m[j];
+            for (int k = 1; k < 4; ++k) acc = acc + a._m[i * 4 + k] * b._m[k * 4 + j];
+            o._m[i * 4 + j] = acc;
+        }
+    return o;
+}
+
 struct BBox {  // src/Kernel/bbox.h
     Point3 _min, _max;
     BBox() = default;
@@ -264,6 +313,41 @@ class Mesh : public Shape {
     std::vector<float> _xyz;
     std::vector<uint32_t> _idx;
     std::vector<float> _uv;
+};
+
+// Model(file, transform) (src/Shape/model.cpp:35-43): an OBJ file loaded by
+// rumi_obj_load with the reference's parsing and transforms; its triangles
+// join the scene as one mesh with per-corner normals.
+class Model : public Shape {
+  public:
+    Model(const char* file, const Mat4x4& transform) {
+        rumi_obj* m = nullptr;
+        if (rumi_obj_load(file, transform._m, &m) != RUMI_OK) {
+            std::fprintf(stderr, "Model: %s\n", rumi_last_error());
+            return;
+        }
+        uint32_t n = 0;
+        const float *xyz = nullptr, *nrm = nullptr;
+        rumi_obj_get(m, &n, &xyz, &nrm, nullptr);
+        _xyz.assign(xyz, xyz + 9 * (size_t)n);
+        _normals.assign(nrm, nrm + 9 * (size_t)n);
+        _idx.resize(3 * (size_t)n);
+        for (size_t i = 0; i < _idx.size(); ++i) _idx[i] = (uint32_t)i;
+        rumi_obj_destroy(m);
+    }
+    rumi_shape Desc() const override {
+        rumi_shape s{};
+        s.kind = RUMI_SHAPE_MESH;
+        s.xyz = _xyz.data();
+        s.nv = (uint32_t)(_xyz.size() / 3);
+        s.idx = _idx.data();
+        s.nt = (uint32_t)(_idx.size() / 3);
+        s.normals = _normals.empty() ? nullptr : _normals.data();
+        return s;
+    }
+    uint32_t Triangles() const { return (uint32_t)(_idx.size() / 3); }
+    std::vector<float> _xyz, _normals;
+    std::vector<uint32_t> _idx;
 };
 
 // ------------------------------------------------------------------- lights

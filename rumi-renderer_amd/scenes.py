@@ -161,3 +161,89 @@ def _look_at(position, target):
     cx /= np.linalg.norm(cx)
     cy = np.cross(d, cx)
     return np.stack([cx, cy, d], axis=1).tolist()
+
+
+# ------------------------------------------------------------------ models
+# Mat4x4 helpers with the reference's FP32 arithmetic (src/Kernel/matrix.cpp:
+# Translation, Scaling, RotationX/Y/Z, operator *), row-major 4x4 float32.
+def mat_translation(x, y, z):
+    m = np.eye(4, dtype=np.float32)
+    m[0, 3], m[1, 3], m[2, 3] = np.float32(x), np.float32(y), np.float32(z)
+    return m
+
+
+def mat_scaling(x, y, z):
+    return np.diag(np.float32([x, y, z, 1.0]))
+
+
+def _cs(radian):
+    r = np.float32(radian)
+    return np.float32(np.cos(np.float64(r))), np.float32(np.sin(np.float64(r)))
+
+
+def mat_rotation_x(radian):
+    c, s = _cs(radian)
+    return np.array([[1, 0, 0, 0], [0, c, -s, 0], [0, s, c, 0], [0, 0, 0, 1]], np.float32)
+
+
+def mat_rotation_y(radian):
+    c, s = _cs(radian)
+    return np.array([[c, 0, s, 0], [0, 1, 0, 0], [-s, 0, c, 0], [0, 0, 0, 1]], np.float32)
+
+
+def mat_rotation_z(radian):
+    c, s = _cs(radian)
+    return np.array([[c, -s, 0, 0], [s, c, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], np.float32)
+
+
+def mat_mul(a, b):
+    """Mat4x4::operator * : each entry summed left to right in FP32."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    out = np.empty((4, 4), np.float32)
+    for i in range(4):
+        for j in range(4):
+            acc = a[i, 0] * b[0, j]
+            for k in range(1, 4):
+                acc = np.float32(acc + a[i, k] * b[k, j])
+            out[i, j] = acc
+    return out
+
+
+# The model scene's transforms: RotationX(0) * Translation(0, -0.99, 0) *
+# Scaling(s, s, s), s = 0.3 for the dragon and 0.65 for the teapot.
+def model_transform(scale):
+    return mat_mul(mat_mul(mat_rotation_x(0.0), mat_translation(0.0, -0.99, 0.0)),
+                   mat_scaling(scale, scale, scale))
+
+
+def model_scene(model: dict, width: int = 1200, height: int = 1000,
+                absorption=(0.05, 0.25, 1.0)) -> dict:
+    """The reference's model scene ("Build Model", src/Scene/Build Model.txt):
+    a 6 x 5 x 6 room of Rect walls (white 0.7 floor, ceiling and back wall,
+    green 0.67 right, red 0.67 left), a 3 x 3 Rect light of radiance 6.6666
+    just under the ceiling, the model (load_obj's mesh fields) in a
+    Dielectric(1.5, 1.00029, absorption) material, and a Pinhole at
+    (0, 1.5, 20.2) turned by RotationY(pi), 20 degree field of view,
+    y_ratio = height / width."""
+    mats = [
+        dict(kind="matte", kd=0.7, color=(1.0, 1.0, 1.0)),          # 0 floor, ceiling, back
+        dict(kind="matte", kd=0.67, color=(0.01, 0.9, 0.01)),        # 1 right
+        dict(kind="matte", kd=0.67, color=(0.9, 0.01, 0.01)),        # 2 left
+        dict(kind="emissive", kd=6.6666, color=(1.0, 1.0, 1.0)),     # 3 light
+        dict(kind="dielectric", eta_in=1.5, eta_out=1.00029, absorption=tuple(absorption)),  # 4
+    ]
+    shapes = [
+        dict(kind="rect", p=(-3.0, -1.0, 3.0), l=(6.0, 0.0, 0.0), w=(0.0, 0.0, -6.0), material=0),
+        dict(kind="rect", p=(3.0, -1.0, -3.0), l=(0.0, 0.0, 6.0), w=(0.0, 5.0, 0.0), material=1),
+        dict(kind="rect", p=(-3.0, -1.0, 3.0), l=(0.0, 0.0, -6.0), w=(0.0, 5.0, 0.0), material=2),
+        dict(kind="rect", p=(-3.0, 4.0, -3.0), l=(6.0, 0.0, 0.0), w=(0.0, 0.0, 6.0), material=0),
+        dict(kind="rect", p=(-3.0, -1.0, -3.0), l=(6.0, 0.0, 0.0), w=(0.0, 5.0, 0.0), material=0),
+        dict(kind="mesh", xyz=model["xyz"], idx=model["idx"], normals=model["normals"], material=4),
+        dict(kind="rect", p=(-1.5, 3.999, -1.5), l=(3.0, 0.0, 0.0), w=(0.0, 0.0, 3.0), material=3,
+             light=True),
+    ]
+    r = mat_rotation_y(np.pi)
+    cam = dict(position=(0.0, 1.5, 20.2), basis=r[:3, :3].astype(np.float64).tolist(), fov_deg=20.0,
+               y_ratio=height / width)
+    return dict(materials=mats, shapes=shapes, camera=cam, medium=None)

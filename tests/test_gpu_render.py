@@ -101,6 +101,28 @@ def test_same_seed_matches_oracle_sample_for_sample(rumi, oracle, ctx, scenes, c
     assert d_same < 1e-4 and d_other > 0.01, (d_same, d_other)
 
 
+def test_model_scene_same_seed_matches_oracle(rumi, oracle, ctx, scenes, tmp_path):
+    """The reference's model scene with its teapot (teapot.obj through
+    rumi_obj_load, glass with absorption): GPU and oracle trace the same
+    samples for one seed."""
+    import gzip
+    import os
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models", "teapot.obj.gz")
+    path = tmp_path / "teapot.obj"
+    path.write_bytes(gzip.open(src).read())
+    model = rumi.load_obj(str(path), scenes.model_transform(0.65))
+    assert model["idx"].shape[0] == 2304
+    w, h = 40, 32
+    sc = scenes.model_scene(model, w, h)
+    c = oracle.render_pt(sc, w, h, 32, 10, seed=5, nthreads=8)[0]
+    same = render(rumi, ctx, sc, w, h, 32, 10, seed=5)[0]
+    other = render(rumi, ctx, sc, w, h, 32, 10, seed=6)[0]
+    d_same = np.abs(same - c).sum() / np.abs(c).sum()
+    d_other = np.abs(other - c).sum() / np.abs(c).sum()
+    print("model relative L1, same seed", d_same, "other seed", d_other)
+    assert d_same < 1e-4 and d_other > 0.01, (d_same, d_other)
+
+
 def test_queue_capacity_and_regeneration_lose_no_samples(rumi, ctx, scenes):
     """Sample k's random stream depends only on (seed, k), so the image is
     the same whatever the queue size or refill threshold, up to the order of
