@@ -228,6 +228,55 @@ def measured_traffic(label, rays):
     return int(round(k["bytes_per_launch"]))
 
 
+def film_leg(rumi, ctx, img, w, h, log):
+    """The film output path on the rendered image: Film::Save's byte
+    conversion (rumi_film_encode, 32-bit pixels, gamma 2.2) and
+    FirefliesFiltering (rumi_film_fireflies), each timed with HIP events on
+    the launch stream over 50 launches; bytes moved per launch (film read +
+    pixels written; pixels read + written) over that time; the oracle's C
+    restatements of the same two steps timed on one host thread."""
+    import numpy as np
+    import torch
+
+    stream = torch.cuda.current_stream()
+    px = rumi.film_encode(ctx, img, w, h, rumi.RUMI_FILM_RGB_MEAN, 1, 2.2, 4, stream=stream)
+    out = rumi.film_fireflies(ctx, px, w, h, 4, stream=stream)
+
+    def timed(fn, reps=50):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    t_enc = timed(lambda: rumi.film_encode(ctx, img, w, h, rumi.RUMI_FILM_RGB_MEAN, 1, 2.2, 4, out=px,
+                                           stream=stream))
+    t_ff = timed(lambda: rumi.film_fireflies(ctx, px, w, h, 4, out=out, stream=stream))
+    n = w * h
+    import oracle
+    host_img = img.cpu().numpy()
+    t0 = time.perf_counter()
+    host_px = oracle.film_encode(host_img, False, 1, 2.2, 4)
+    c_enc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.fireflies(host_px)
+    c_ff = time.perf_counter() - t0
+    same = bool(np.array_equal(px.cpu().numpy(), host_px))
+    res = {"pixels": n, "encode_us": round(t_enc * 1e6, 2), "encode_GBs": round(16 * n / t_enc / 1e9, 1),
+           "fireflies_us": round(t_ff * 1e6, 2), "fireflies_GBs": round(8 * n / t_ff / 1e9, 1),
+           "bytes_equal_oracle": same,
+           "cpu_baseline": {"encode_us": round(c_enc * 1e6, 1), "fireflies_us": round(c_ff * 1e6, 1),
+                            "cores": 1, "kind": "port",
+                            "sample": f"the same {w}x{h} image through the oracle's C restatements"}}
+    log(f"film: encode {t_enc * 1e6:.1f} us, fireflies {t_ff * 1e6:.1f} us "
+        f"(CPU {c_enc * 1e6:.0f} / {c_ff * 1e6:.0f} us), bytes equal oracle: {same}")
+    return res
+
+
 def model_leg(rumi, ctx, args, log, scenes):
     """The reference's model scene (src/Scene/Build Model.txt): its
     100,000-triangle dragon.obj loaded by rumi_obj_load with the scene's
@@ -263,6 +312,7 @@ def model_leg(rumi, ctx, args, log, scenes):
     scene.close()
     sec = min(times)
     sps = st.samples / sec
+    film = film_leg(rumi, ctx, img.contiguous(), w, h, log)
     import oracle
     cw, ch = 60, 50
     small = scenes.model_scene(model, cw, ch)
@@ -288,6 +338,7 @@ def model_leg(rumi, ctx, args, log, scenes):
                          "sample": f"60x50 at {spp} spp, depth {depth}, the oracle's FP64 "
                                    f"restatement of PathTracer::Trace ({cpu_sec:.2f}s)"},
         "speedup_vs_cpu": round(sps / cpu_sps, 1),
+        "film_output": film,
     }
     log(f"model_pt (dragon): {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.1f} ms), same-seed L1 {l1:.2e}, "
         f"CPU {cpu_sps:.0f} samples/s")

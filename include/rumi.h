@@ -390,6 +390,47 @@ int rumi_make_icosphere(uint32_t freq, uint64_t seed, float amplitude, float* ho
  * tmax = +inf; PCG32(seed, stream = ray index).  Host memory. */
 int rumi_make_rays_uniform(uint64_t n, uint64_t seed, float half_extent, rumi_ray* host_rays);
 
+/* -------------------------------------------------------------------- film */
+/* Film::Save's conversion of the float film to pixel bytes
+ * (src/Camera/film.cpp:152-203), on the GPU: per channel
+ *   v = film value (RUMI_FILM_BGR_SUM: the reference's _pixel_buffer, B, G, R
+ *       sums, times 1 / d^2 with d = (int)sqrt((float)n_samples), as Save
+ *       divides; RUMI_FILM_RGB_MEAN: rumi_render's image, R, G, B means,
+ *       taken as they are),
+ *   v = powf(v, 1 / gamma) * 255, clamped to [0, 255], truncated;
+ * powf is taken correctly rounded (the double power rounded once), and a
+ * NaN (a negative film value) gives 0.  dev_out: width * height *
+ * bytes_per_pixel bytes B, G, R (+ 0 for 4 bytes per pixel), in the film's
+ * row order -- the pixel data SaveBmp writes (src/Kernel/bmp.cpp:59-97). */
+enum { RUMI_FILM_BGR_SUM = 0, RUMI_FILM_RGB_MEAN = 1 };
+
+int rumi_film_encode(rumi_ctx* ctx, const float* dev_film, uint32_t width, uint32_t height,
+                     int32_t layout, uint32_t n_samples, float gamma, uint32_t bytes_per_pixel,
+                     uint8_t* dev_out, void* stream);
+
+/* FirefliesFiltering (src/Scene/scene.cpp:261-431) over the pixel bytes of a
+ * loaded image, on the GPU: a pixel with a channel at 255 none of whose
+ * first n_copy gathered neighbours is white is replaced by their FP32 mean
+ * (truncated); the reference's gathering rules are kept, including its
+ * flipped row order and its edge conditions.  Where the reference reads
+ * outside its buffer (saturated pixels of the first and last image rows),
+ * those neighbours read as 0; the fourth byte of a 32-bit pixel is written
+ * as 0 (the reference leaves it unset).  dev_src and dev_dst must differ. */
+int rumi_film_fireflies(rumi_ctx* ctx, const uint8_t* dev_src, uint32_t width, uint32_t height,
+                        uint32_t bytes_per_pixel, uint8_t* dev_dst, void* stream);
+
+/* SaveBmp's 54-byte header (src/Kernel/bmp.cpp:59-97): "BM", sizes, offset
+ * 54, a 40-byte info header with positive height and no row padding counted.
+ * Host memory. */
+int rumi_bmp_header(uint32_t width, uint32_t height, uint32_t bytes_per_pixel, uint8_t* host_out);
+
+/* LoadBmp's reading rule (src/Kernel/bmp.cpp:3-57) over a file image in host
+ * memory: size fields from the header; rows back to back when the width is a
+ * multiple of 4, else each row followed by (width % 4) skipped bytes.
+ * pixels NULL: only the size fields. */
+int rumi_bmp_read(const uint8_t* host_file, uint64_t file_bytes, uint32_t* width, uint32_t* height,
+                  uint32_t* bytes_per_pixel, uint8_t* host_pixels, uint64_t pixel_capacity);
+
 /* ------------------------------------------------------------------ models */
 /* An OBJ model, loaded as Model::Model(file, transform) loads it
  * (Model::LoadObjectFile, src/Shape/model.cpp:72-338): "v", "vt", "vn" and

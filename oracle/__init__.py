@@ -79,6 +79,12 @@ def lib() -> ctypes.CDLL:
         L.or_obj_free.restype = None
         L.or_mat4_inverse.argtypes = [vp, vp]
         L.or_mat4_inverse.restype = None
+        L.or_film_encode.argtypes = [vp, c_int, c_int, c_int, c_int, ctypes.c_float, c_int, vp]
+        L.or_film_encode.restype = None
+        L.or_bmp_header.argtypes = [c_int, c_int, c_int, vp]
+        L.or_bmp_header.restype = None
+        L.or_fireflies.argtypes = [vp, c_int, c_int, c_int, vp]
+        L.or_fireflies.restype = None
         _lib = L
     return _lib
 
@@ -235,6 +241,30 @@ def load_obj(path, transform=None):
     finally:
         lib().or_obj_free(px)
         lib().or_obj_free(pn)
+
+
+def film_encode(film, bgr_sum, n_samples, gamma, bpp):
+    """or_film_encode: film (h, w, 3) float32 -> (h, w, bpp) uint8."""
+    film = np.ascontiguousarray(film, np.float32)
+    h, w = film.shape[:2]
+    out = np.empty((h, w, bpp), np.uint8)
+    lib().or_film_encode(_p(film), w, h, int(bgr_sum), int(n_samples), float(gamma), bpp, _p(out))
+    return out
+
+
+def bmp_header(w, h, bpp):
+    out = np.empty(54, np.uint8)
+    lib().or_bmp_header(w, h, bpp, _p(out))
+    return out
+
+
+def fireflies(src):
+    """or_fireflies: (h, w, bpp) uint8 -> filtered copy."""
+    src = np.ascontiguousarray(src, np.uint8)
+    h, w, bpp = src.shape
+    dst = np.empty_like(src)
+    lib().or_fireflies(_p(src), w, h, bpp, _p(dst))
+    return dst
 
 
 def mat4_inverse(m):

@@ -210,6 +210,16 @@ int or_render_bdpt(const float* shapes, int n_shapes, const float* mats, int n_m
 int or_obj_load(const char* path, const float* transform, float** xyz, float** normals,
                 int* n_triangles, int counts[4]);
 void or_obj_free(float* p);
+/* Film::Save's pixel conversion (src/Camera/film.cpp:152-203): film is
+ * width * height * 3 floats, B, G, R sums (bgr_sum = 1, divided by
+ * (int)sqrt(n_samples)^2) or R, G, B means (bgr_sum = 0); out:
+ * width * height * bpp bytes B, G, R (, 0). */
+void or_film_encode(const float* film, int width, int height, int bgr_sum, int n_samples,
+                    float gamma, int bpp, unsigned char* out);
+/* SaveBmp's 54-byte header (src/Kernel/bmp.cpp:59-97). */
+void or_bmp_header(int width, int height, int bpp, unsigned char* header);
+/* FirefliesFiltering (src/Scene/scene.cpp:261-431) over pixel bytes. */
+void or_fireflies(const unsigned char* src, int width, int height, int bpp, unsigned char* dst);
 /* Inverse (src/Kernel/matrix.cpp:67-133), row-major 4x4. */
 void or_mat4_inverse(const float* a, float* out);
 

@@ -1,0 +1,132 @@
+/* Oracle: the film output path, restated from Film::Save
+ * (src/Camera/film.cpp:152-203), SaveBmp (src/Kernel/bmp.cpp:59-97) and
+ * FirefliesFiltering (src/Scene/scene.cpp:261-431).  Test infrastructure
+ * only: the tests compare the library's GPU kernels with this and pin this
+ * against closed forms and hand-made images.
+ *
+ * powf is taken correctly rounded (the double power rounded once to float);
+ * NaN channel values become 0; neighbours the reference reads outside its
+ * buffer read as 0; the fourth byte of a 32-bit pixel is 0.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+static unsigned char channel_byte(float v)
+{
+    if (v > 255.0f) v = 255.0f;
+    else if (v < 0.0f) v = 0.0f;
+    if (isnan(v)) return 0;
+    return (unsigned char)(unsigned int)v;
+}
+
+void or_film_encode(const float* film, int width, int height, int bgr_sum, int n_samples,
+                    float gamma, int bpp, unsigned char* out)
+{
+    int depth = (int)sqrtf((float)n_samples);
+    float inv_n = 1.0f / (float)(depth * depth);
+    float g = 1.0f / gamma;
+    long i, n = (long)width * height;
+    memset(out, 0, (size_t)n * (size_t)bpp);
+    for (i = 0; i < n; ++i) {
+        const float* p = film + 3 * i;
+        float c[3];
+        int k;
+        if (bgr_sum) {
+            c[0] = p[0] * inv_n;
+            c[1] = p[1] * inv_n;
+            c[2] = p[2] * inv_n;
+        } else {
+            c[0] = p[2];
+            c[1] = p[1];
+            c[2] = p[0];
+        }
+        for (k = 0; k < 3; ++k) {
+            float v = (float)pow((double)c[k], (double)g) * 255.0f;
+            out[i * bpp + k] = channel_byte(v);
+        }
+    }
+}
+
+void or_bmp_header(int width, int height, int bpp, unsigned char* h)
+{
+    unsigned int data = (unsigned int)width * (unsigned int)height * (unsigned int)bpp;
+    unsigned int f[13];
+    int k;
+    memset(h, 0, 54);
+    h[0] = 'B';
+    h[1] = 'M';
+    /* file header (12 bytes) and info header (40), little-endian */
+    f[0] = 54 + data; /* file size */
+    f[1] = 0;         /* two 16-bit reserved words */
+    f[2] = 54;        /* data offset */
+    f[3] = 40;
+    f[4] = (unsigned int)width;
+    f[5] = (unsigned int)height;
+    f[6] = 1u | ((unsigned int)(bpp * 8) << 16); /* planes, bits per pixel */
+    f[7] = 0;
+    f[8] = data;
+    f[9] = f[10] = f[11] = f[12] = 0;
+    for (k = 0; k < 13; ++k) {
+        h[2 + 4 * k + 0] = (unsigned char)(f[k]);
+        h[2 + 4 * k + 1] = (unsigned char)(f[k] >> 8);
+        h[2 + 4 * k + 2] = (unsigned char)(f[k] >> 16);
+        h[2 + 4 * k + 3] = (unsigned char)(f[k] >> 24);
+    }
+}
+
+void or_fireflies(const unsigned char* src, int w, int h, int bpp, unsigned char* dst)
+{
+    long row = (long)w * bpp, total = row * h;
+    int y, x;
+    for (y = 0; y < h; ++y)
+        for (x = 0; x < w; ++x) {
+            long at = (long)(h - (y + 1)) * row + (long)x * bpp;
+            long pos[8];
+            int use[8], k, m = 0, nc = 8, white = 0;
+            float t[3], nb[8][3], s[3] = {0, 0, 0}, scale;
+            for (k = 0; k < 3; ++k) t[k] = (float)src[at + k];
+            if (bpp == 4) dst[at + 3] = 0;
+            if (t[0] < 255.0f && t[1] < 255.0f && t[2] < 255.0f) {
+                for (k = 0; k < 3; ++k) dst[at + k] = (unsigned char)(int)t[k];
+                continue;
+            }
+            if (x == 0 || y == 0 || x == w - 1 || y == h - 1) {
+                nc -= 3;
+                if (x == 0 && y == 0) nc -= 2;
+                if (x == 0 && y == h - 1) nc -= 2;
+                if (x == w - 1 && y == 0) nc -= 2;
+                if (x == w - 1 && y == h - 1) nc -= 2;
+            }
+            /* left, right; buffer row above: left, centre, right; below */
+            use[0] = x != 0;                 pos[0] = at - bpp;
+            use[1] = x != w - 1;             pos[1] = at + bpp;
+            use[2] = y != 0 || x != 0;       pos[2] = at - row - bpp;
+            use[3] = y != 0;                 pos[3] = at - row;
+            use[4] = y != 0 || x != w - 1;   pos[4] = at - row + bpp;
+            use[5] = y != h - 1 || x != 0;   pos[5] = at + row - bpp;
+            use[6] = y != h - 1;             pos[6] = at + row;
+            use[7] = y != h - 1 || x != w - 1; pos[7] = at + row + bpp;
+            for (k = 0; k < 8; ++k) {
+                int c;
+                if (!use[k]) continue;
+                for (c = 0; c < 3; ++c)
+                    nb[m][c] = (pos[k] >= 0 && pos[k] + 2 < total) ? (float)src[pos[k] + c] : 0.0f;
+                m++;
+            }
+            for (k = 0; k < nc && k < m; ++k)
+                if (nb[k][0] == 255.0f && nb[k][1] == 255.0f && nb[k][2] == 255.0f) white = 1;
+            if (white) {
+                for (k = 0; k < 3; ++k) dst[at + k] = (unsigned char)(int)t[k];
+                continue;
+            }
+            for (k = 0; k < nc && k < m; ++k) {
+                s[0] += nb[k][0];
+                s[1] += nb[k][1];
+                s[2] += nb[k][2];
+            }
+            scale = 1.0f / (float)nc;
+            for (k = 0; k < 3; ++k) dst[at + k] = channel_byte(s[k] * scale);
+        }
+}

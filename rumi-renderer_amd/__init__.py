@@ -123,6 +123,7 @@ RUMI_SHAPE_TRIANGLE, RUMI_SHAPE_SPHERE, RUMI_SHAPE_RECT, RUMI_SHAPE_MESH = 0, 1,
 MATERIAL_KINDS = {"matte": 0, "glossy": 1, "specular": 2, "emissive": 3, "dielectric": 4,
                   "subsurface": 5, "ggx": 6}
 RUMI_INTEGRATOR_PT, RUMI_INTEGRATOR_BDPT = 0, 1
+RUMI_FILM_BGR_SUM, RUMI_FILM_RGB_MEAN = 0, 1
 
 # Every function include/rumi.h declares, with its ctypes signature.
 _SIGNATURES = {
@@ -161,6 +162,13 @@ _SIGNATURES = {
     "rumi_obj_get": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_void_p), POINTER(c_void_p),
                              c_void_p]),
     "rumi_obj_destroy": (c_int, [c_void_p]),
+    "rumi_film_encode": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, c_int32, c_uint32, c_float,
+                                 c_uint32, c_void_p, c_void_p]),
+    "rumi_film_fireflies": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, c_void_p,
+                                    c_void_p]),
+    "rumi_bmp_header": (c_int, [c_uint32, c_uint32, c_uint32, c_void_p]),
+    "rumi_bmp_read": (c_int, [c_void_p, c_uint64, POINTER(c_uint32), POINTER(c_uint32),
+                              POINTER(c_uint32), c_void_p, c_uint64]),
 }
 
 _lib = None
@@ -214,6 +222,45 @@ def _stream(stream) -> int:
 
 
 # --------------------------------------------------------------- workloads
+def film_encode(ctx, film, width, height, layout=RUMI_FILM_RGB_MEAN, n_samples=1, gamma=2.2,
+                bytes_per_pixel=4, out=None, stream=None):
+    """rumi_film_encode on a device tensor (height x width x 3 float32);
+    returns the uint8 device tensor of pixel bytes (height x width x bpp)."""
+    import torch
+    if out is None:
+        out = torch.empty((height, width, bytes_per_pixel), dtype=torch.uint8, device=film.device)
+    check(lib().rumi_film_encode(ctx.handle, _ptr(film), width, height, layout, n_samples, gamma,
+                                 bytes_per_pixel, _ptr(out), _stream(stream)))
+    return out
+
+
+def film_fireflies(ctx, pixels, width, height, bytes_per_pixel=4, out=None, stream=None):
+    """rumi_film_fireflies: device uint8 pixels in, a new filtered tensor out."""
+    import torch
+    if out is None:
+        out = torch.empty_like(pixels)
+    check(lib().rumi_film_fireflies(ctx.handle, _ptr(pixels), width, height, bytes_per_pixel,
+                                    _ptr(out), _stream(stream)))
+    return out
+
+
+def bmp_bytes(pixels: np.ndarray, width: int, height: int, bytes_per_pixel: int) -> bytes:
+    """SaveBmp's file: rumi_bmp_header + the pixel bytes as they are."""
+    hdr = np.empty(54, np.uint8)
+    check(lib().rumi_bmp_header(width, height, bytes_per_pixel, _ptr(hdr)))
+    return hdr.tobytes() + np.ascontiguousarray(pixels, np.uint8).tobytes()
+
+
+def bmp_read(data: bytes):
+    """LoadBmp's reading rule: (pixels uint8 (h, w, bpp), width, height, bpp)."""
+    buf = np.frombuffer(data, np.uint8)
+    w, h, b = c_uint32(0), c_uint32(0), c_uint32(0)
+    check(lib().rumi_bmp_read(_ptr(buf), buf.size, byref(w), byref(h), byref(b), None, 0))
+    pix = np.empty((h.value, w.value, b.value), np.uint8)
+    check(lib().rumi_bmp_read(_ptr(buf), buf.size, byref(w), byref(h), byref(b), _ptr(pix), pix.size))
+    return pix, w.value, h.value, b.value
+
+
 def load_obj(path, transform=None) -> dict:
     """rumi_obj_load: an OBJ model as Model(file, transform) loads it.
     Returns dict(xyz=(3n, 3) float32 vertices, three per triangle,
