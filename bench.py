@@ -228,6 +228,52 @@ def measured_traffic(label, rays):
     return int(round(k["bytes_per_launch"]))
 
 
+def whitted_leg(rumi, ctx, args, log, scenes):
+    """The RayTracer (RUMI_INTEGRATOR_WHITTED, RayTracer(scene, 10)) on the
+    512^2 x 16 spp Cornell box: GPU samples/s; the same seed at 64^2 on the
+    GPU and in the oracle (relative L1 between them); CPU baseline = that
+    oracle render on one thread."""
+    import numpy as np
+    import torch
+
+    w = h = 512
+    spp, depth = 16, 10
+    scene = rumi.Scene(ctx, scenes.cornell_box(w, h))
+    kw = dict(integrator=rumi.RUMI_INTEGRATOR_WHITTED)
+    scene.render(w, h, spp, depth, seed=1, **kw)
+    torch.cuda.synchronize()
+    times = []
+    for k in range(args.render_runs):
+        t0 = time.perf_counter()
+        img, st = scene.render(w, h, spp, depth, seed=10 + k, **kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    scene.close()
+    sec = min(times)
+    sps = st.samples / sec
+    import oracle
+    small = scenes.cornell_box(64, 64)
+    s64 = rumi.Scene(ctx, small)
+    gi, _ = s64.render(64, 64, spp, depth, seed=7, **kw)
+    torch.cuda.synchronize()
+    gi = gi.cpu().numpy()
+    s64.close()
+    t0 = time.perf_counter()
+    ci, _ = oracle.render_whitted(small, 64, 64, spp, depth, seed=7, nthreads=1)
+    cpu_sec = time.perf_counter() - t0
+    cpu_sps = 64 * 64 * spp / cpu_sec
+    l1 = float(np.abs(gi - ci).sum() / max(np.abs(ci).sum(), 1e-30))
+    log(f"cornell_whitted: {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.2f} ms), same-seed L1 {l1:.2e}, "
+        f"CPU {cpu_sps:.0f} samples/s")
+    return {"samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 5),
+            "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
+            "max_depth": depth, "same_seed_rel_l1_64px": l1,
+            "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1, "kind": "port",
+                             "sample": f"64x64 at {spp} spp, the oracle's FP64 restatement of "
+                                       f"RayTracer::Trace ({cpu_sec:.2f}s)"},
+            "speedup_vs_cpu": round(sps / cpu_sps, 1)}
+
+
 def film_leg(rumi, ctx, img, w, h, log):
     """The film output path on the rendered image: Film::Save's byte
     conversion (rumi_film_encode, 32-bit pixels, gamma 2.2) and
@@ -414,6 +460,7 @@ def render_legs(rumi, ctx, args, log):
         log(f"{name}: {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.1f} ms), "
             f"rel-MSE ratio {out[name]['rel_mse_64px']['ratio']:.3f}, CPU {cpu_sps:.0f} samples/s")
         scene.close()
+    out["cornell_whitted"] = whitted_leg(rumi, ctx, args, log, scenes)
     out["model_pt"] = model_leg(rumi, ctx, args, log, scenes)
     out["media_gradient"] = gradient_leg(rumi, ctx, args, log, scenes)
     out["cornell_bdpt"] = bdpt_leg(rumi, ctx, args, log, scenes)

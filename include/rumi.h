@@ -270,7 +270,15 @@ int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* desc, rumi_scene** o
 int rumi_scene_destroy(rumi_scene* scene);
 
 /* ---------------------------------------------------------------- render */
-enum { RUMI_INTEGRATOR_PT = 0, RUMI_INTEGRATOR_BDPT = 1 };
+/* RUMI_INTEGRATOR_WHITTED: RayTracer(scene, max_depth)
+ * (src/Integrator/raytracer.cpp:30-86): Specular and Dielectric surfaces
+ * trace on; every other surface ends the path with its DirectLighting (a
+ * light sample for Matte, and for Glossy only when ks = 0, the RayTracer
+ * leaving the lobe variable at 1); emitters count when hit directly or
+ * through specular/dielectric chains; no medium sampling along rays
+ * (shadow rays keep the medium's transmittance); up to max_depth + 1
+ * surfaces. */
+enum { RUMI_INTEGRATOR_PT = 0, RUMI_INTEGRATOR_BDPT = 1, RUMI_INTEGRATOR_WHITTED = 2 };
 
 typedef struct {
     int32_t integrator;     /* RUMI_INTEGRATOR_* */

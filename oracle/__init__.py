@@ -435,6 +435,26 @@ def render_pt(scene: dict, width: int, height: int, spp: int, max_depth: int, se
     return mean, m2
 
 
+def render_whitted(scene: dict, width: int, height: int, spp: int, max_depth: int, seed: int = 0,
+                   nthreads: int = 1):
+    """FP64 restatement of RayTracer::Trace (RayTracer(scene, max_depth));
+    returns the per-pixel mean radiance and mean of squares."""
+    L = lib()
+    if not hasattr(L, "_whitted_bound"):
+        L.or_render_whitted.argtypes = _PT_ARGS + [c_void_p, c_void_p]
+        L._whitted_bound = True
+    shapes, mats, cam, med = scene_arrays(scene)
+    texels, table, n_tex, _ = texture_arrays(scene)
+    mean = np.zeros((height, width, 3), np.float64)
+    m2 = np.zeros((height, width, 3), np.float64)
+    rc = L.or_render_whitted(_p(shapes), shapes.shape[0], _p(mats), mats.shape[0], _p(cam),
+                             _p(med) if med is not None else None, _p(texels), _p(table), n_tex,
+                             width, height, spp, max_depth, seed, nthreads, _p(mean), _p(m2))
+    if rc != 0:
+        raise RuntimeError(f"or_render_whitted failed ({rc})")
+    return mean, m2
+
+
 def adjoint_pt(scene: dict, dloss_dimage, spp: int, max_depth: int, seed: int = 0,
                nthreads: int = 1, textures: bool = False):
     """FP64 derivative of the expected image contracted with dloss_dimage

@@ -181,6 +181,16 @@ int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mat
                  const int* tex_table, int n_tex, int width, int height, int spp, int max_depth,
                  uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb);
 
+/* RayTracer (src/Integrator/raytracer.cpp:30-86) over the same scene
+ * records: or_render_pt's walk where every surface but specular and
+ * dielectric ends the path after its light sample (Glossy's only when its
+ * lobe variable, left at 1, is within kd_pdf), no medium sampling along
+ * rays, trace depths 0 .. max_depth. */
+int or_render_whitted(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                      const float* camera, const float* medium, const float* texels,
+                      const int* tex_table, int n_tex, int width, int height, int spp,
+                      int max_depth, uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb);
+
 /* Derivative of the expected image of or_render_pt, contracted with
  * dloss_dimage (height x width x 3): grad7 = dLoss/d(sigma_a rgb, sigma_s
  * rgb, g) of the medium, grad7_var the variance of that estimate,

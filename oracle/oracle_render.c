@@ -149,6 +149,7 @@ typedef struct {
     const float* texels;    /* RGB texel pool */
     const int* tex_table;   /* per texture: width, height, first float of its texels */
     int n_tex;
+    int whitted;            /* RayTracer: only specular and dielectric surfaces continue */
 } scene_t;
 
 /* Ray + hit record, the reference's Ray/Intersection pair. */
@@ -911,7 +912,7 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
             cur->bv = h.v;
             surface(sc, &h, &r, &cur->p, &cur->n);
         }
-        if (md->kind && !cur->in_object) {
+        if (md->kind && !cur->in_object && !sc->whitted) {
             double tn, tf;
             if (med_intersect(md, &r, &tn, &tf)) {
                 cur->vsmax = med_maxdist(r.tmin, tn, tf);
@@ -933,6 +934,10 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
             if (cur->obj < 0) break;
             const mat_t* m = &sc->mats[sc->shapes[cur->obj].mat];
             if (m->kind == MAT_EMISSIVE) break;
+            /* RayTracer::Trace (raytracer.cpp:37-78): every material but
+               Specular and Dielectric gives its DirectLighting and ends the
+               path; the lobe variable stays 1 (intersection.cpp:18) */
+            if (sc->whitted && m->kind != MAT_SPECULAR && m->kind != MAT_DIELECTRIC) break;
             v3 wo = neg(r.d), n = cur->n;
             int sample_bsdf = 1;
             if (m->kind == MAT_SUBSURFACE && in_object) {
@@ -1394,7 +1399,7 @@ static int run_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
                   const int* tex_table, int n_tex, int width, int height, int spp, int max_depth,
                   uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb,
                   const double* dimg, double* grad7, double* grad7_var, double* grad_tex,
-                  size_t n_texels)
+                  size_t n_texels, int whitted)
 {
     if (width <= 0 || height <= 0 || spp <= 0 || max_depth <= 0 || !mean_rgb) return 1;
     if (n_tex > 0 && (!texels || !tex_table)) return 1;
@@ -1403,6 +1408,7 @@ static int run_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
     sc.texels = texels;
     sc.tex_table = tex_table;
     sc.n_tex = n_tex;
+    sc.whitted = whitted;
     int i, t, k;
     for (i = 0; i < sc.n_shapes; ++i)
         if (sc.shapes[i].tex >= n_tex) {
@@ -1463,7 +1469,20 @@ int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mat
                  uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb)
 {
     return run_pt(shapes, n_shapes, mats, n_mats, camera, medium, texels, tex_table, n_tex, width,
-                  height, spp, max_depth, seed, nthreads, mean_rgb, m2_rgb, 0, 0, 0, 0, 0);
+                  height, spp, max_depth, seed, nthreads, mean_rgb, m2_rgb, 0, 0, 0, 0, 0, 0);
+}
+
+/* RayTracer::Trace (src/Integrator/raytracer.cpp:37-86) with
+ * RayTracer(scene, max_depth): the path tracer's walk with every surface
+ * but Specular and Dielectric ending the path after its DirectLighting, no
+ * medium sampling along rays, and trace depths 0 .. max_depth. */
+int or_render_whitted(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                      const float* camera, const float* medium, const float* texels,
+                      const int* tex_table, int n_tex, int width, int height, int spp,
+                      int max_depth, uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb)
+{
+    return run_pt(shapes, n_shapes, mats, n_mats, camera, medium, texels, tex_table, n_tex, width,
+                  height, spp, max_depth + 1, seed, nthreads, mean_rgb, m2_rgb, 0, 0, 0, 0, 0, 1);
 }
 
 int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
@@ -1475,7 +1494,7 @@ int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
     if (!dloss_dimage || !grad7) return 1;
     return run_pt(shapes, n_shapes, mats, n_mats, camera, medium, texels, tex_table, n_tex, width,
                   height, spp, max_depth, seed, nthreads, mean_rgb, 0, dloss_dimage, grad7,
-                  grad7_var, grad_tex, n_texels);
+                  grad7_var, grad_tex, n_texels, 0);
 }
 
 /* ====================================================================== */

@@ -122,7 +122,7 @@ class GradBuffers(Structure):
 RUMI_SHAPE_TRIANGLE, RUMI_SHAPE_SPHERE, RUMI_SHAPE_RECT, RUMI_SHAPE_MESH = 0, 1, 2, 3
 MATERIAL_KINDS = {"matte": 0, "glossy": 1, "specular": 2, "emissive": 3, "dielectric": 4,
                   "subsurface": 5, "ggx": 6}
-RUMI_INTEGRATOR_PT, RUMI_INTEGRATOR_BDPT = 0, 1
+RUMI_INTEGRATOR_PT, RUMI_INTEGRATOR_BDPT, RUMI_INTEGRATOR_WHITTED = 0, 1, 2
 RUMI_FILM_BGR_SUM, RUMI_FILM_RGB_MEAN = 0, 1
 
 # Every function include/rumi.h declares, with its ctypes signature.

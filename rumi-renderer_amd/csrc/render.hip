@@ -163,6 +163,7 @@ struct Params {
     uint32_t pixels;
     uint64_t seed;
     uint32_t n_scenes;  // batches
+    int32_t whitted;    // RayTracer: only Specular and Dielectric continue a path
 };
 
 // One scene of a batch: its tables, image and offsets into the batch's
@@ -823,8 +824,9 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
         if (pend != 0.0f) G.v[6] += pend * S;  // the previous vertex's phase direction
     }
 
-    // Homogeneous medium along the segment (pathtracer.cpp:156-178).
-    if (md.bound && !in_obj) {
+    // Homogeneous medium along the segment (pathtracer.cpp:156-178); the
+    // RayTracer samples none (its only medium term is the shadow rays').
+    if (md.bound && !in_obj && !prm.whitted) {
         float tn, tf;
         if (med_intersect(md, o, d, prim >= 0 ? t : 0.0f, tn, tf)) {
             const float smax = med_maxdist(prim >= 0 ? t : 0.0f, tn, tf);
@@ -930,6 +932,28 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
             if (MODE != kReplay) film_add(film, pixel, c);
             if (MODE == kPrimal) add3(adj.L_total + 3 * (size_t)sample, c);
             if (MODE == kReplay) add3(adj.L_sofar + 3 * (size_t)sample, c);
+        }
+        return;
+    }
+
+    // RayTracer::Trace (raytracer.cpp:37-78): Specular and Dielectric trace
+    // on (GlobalIllumination); every other material returns its
+    // DirectLighting and ends the path -- Matte's light sample, Glossy's
+    // only when the lobe variable, left at 1 by the RayTracer
+    // (intersection.cpp:18), is within kd / (kd + ks); Subsurface and GGX
+    // none.
+    if (prm.whitted && m.kind != RUMI_MAT_SPECULAR && m.kind != RUMI_MAT_DIELECTRIC) {
+        if (m.kind == RUMI_MAT_MATTE || (m.kind == RUMI_MAT_GLOSSY && m.kd != 0.0f && 1.0f <= m.kd_pdf)) {
+            V3 lo, wi;
+            float dist, smax_s;
+            Rng gn = nee_stream(g, nee_key);
+            if (matte_direct(sv, m, p, n, gn, lo, wi, dist, smax_s)) {
+                if (m.kind == RUMI_MAT_GLOSSY) lo = lo * (1.0f / m.kd_pdf);
+                const V3 tex = s.info[2] >= 0 ? texture_at(p) : v3(1, 1, 1);
+                const V3 c = cmul(cmul(beta, tex), lo);
+                if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f)
+                    push_shadow(out, p, wi, dist, c, pixel, smax_s, 0.0f);
+            }
         }
         return;
     }
@@ -2088,7 +2112,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
         return fail(RUMI_ERR_HIP, "rumi_render: strategy upload failed");
 
     SceneView sv = scene_view(sc);
-    BdptParams bp;
+    BdptParams bp{};
     bp.base.width = desc->width;
     bp.base.height = desc->height;
     bp.base.max_depth = desc->max_depth;
@@ -2159,7 +2183,8 @@ int check_render_args(rumi_ctx* ctx, const rumi_scene* sc, const rumi_render_des
     if (!ctx) return fail(RUMI_ERR_NO_DEVICE, "%s: null context", fn);
     if (!sc || !desc) return fail(RUMI_ERR_INVALID, "%s: null argument", fn);
     if (sc->device != ctx->device) return fail(RUMI_ERR_INVALID, "%s: scene is on another device", fn);
-    if (desc->integrator != RUMI_INTEGRATOR_PT && desc->integrator != RUMI_INTEGRATOR_BDPT)
+    if (desc->integrator != RUMI_INTEGRATOR_PT && desc->integrator != RUMI_INTEGRATOR_BDPT &&
+        desc->integrator != RUMI_INTEGRATOR_WHITTED)
         return fail(RUMI_ERR_UNSUPPORTED, "%s: unknown integrator %d", fn, desc->integrator);
     if (desc->width <= 0 || desc->height <= 0 || desc->spp <= 0 || desc->max_depth <= 0)
         return fail(RUMI_ERR_INVALID, "%s: width, height, spp and max_depth must be > 0", fn);
@@ -2312,10 +2337,13 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     ps.tab = nullptr;
     ps.prm.width = desc->width;
     ps.prm.height = desc->height;
-    ps.prm.max_depth = desc->max_depth;
+    // RayTracer(scene, D) traces while trace_depth <= D: D + 1 surfaces
+    const bool whitted = desc->integrator == RUMI_INTEGRATOR_WHITTED;
+    ps.prm.max_depth = whitted ? desc->max_depth + 1 : desc->max_depth;
     ps.prm.pixels = pixels;
     ps.prm.seed = desc->seed;
     ps.prm.n_scenes = 1;
+    ps.prm.whitted = whitted ? 1 : 0;
     ps.film = film;
     ps.adj = adj;
     ps.tr = Track{nullptr, nullptr};

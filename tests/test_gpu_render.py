@@ -101,6 +101,24 @@ def test_same_seed_matches_oracle_sample_for_sample(rumi, oracle, ctx, scenes, c
     assert d_same < 1e-4 and d_other > 0.01, (d_same, d_other)
 
 
+@pytest.mark.parametrize("case", ["cornell", "media", "glossy"])
+def test_whitted_same_seed_matches_oracle(rumi, oracle, ctx, scenes, case):
+    """RayTracer (RUMI_INTEGRATOR_WHITTED): GPU and oracle trace the same
+    samples for one seed -- light samples at diffuse surfaces, chains through
+    the glass sphere, no medium sampling along rays."""
+    w = 32
+    sc = {"cornell": lambda: scenes.cornell_box(w, w), "media": lambda: scenes.cornell_media(w, w),
+          "glossy": lambda: rc.glossy_box(w, w)}[case]()
+    c = oracle.render_whitted(sc, w, w, 32, 6, seed=5, nthreads=8)[0]
+    kw = dict(integrator=rumi.RUMI_INTEGRATOR_WHITTED)
+    same = render(rumi, ctx, sc, w, w, 32, 6, seed=5, **kw)[0]
+    other = render(rumi, ctx, sc, w, w, 32, 6, seed=6, **kw)[0]
+    d_same = np.abs(same - c).sum() / np.abs(c).sum()
+    d_other = np.abs(other - c).sum() / np.abs(c).sum()
+    print(case, "whitted relative L1, same seed", d_same, "other seed", d_other)
+    assert d_same < 1e-4 and d_other > 0.003, (d_same, d_other)
+
+
 def test_model_scene_same_seed_matches_oracle(rumi, oracle, ctx, scenes, tmp_path):
     """The reference's model scene with its teapot (teapot.obj through
     rumi_obj_load, glass with absorption): GPU and oracle trace the same
