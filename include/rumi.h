@@ -250,9 +250,18 @@ typedef struct {
     float g;            /* Henyey-Greenstein anisotropy; 0 = the reference's Isotropic phase */
 } rumi_medium;
 
+enum { RUMI_TEXTURE_IMAGE = 0, RUMI_TEXTURE_CHECKER = 1 };
+
 typedef struct {
     int32_t width, height;
-    const float* rgb;   /* host, width * height * 3, row 0 = v = 0 */
+    const float* rgb;   /* IMAGE: host, width * height * 3, row 0 = v = 0 (ImageTexture::Pixel,
+                           src/Texture/imagetexture.cpp:32-61: nearest texel) */
+    int32_t kind;       /* RUMI_TEXTURE_* (0 = IMAGE) */
+    /* CHECKER: Checker(w, h, color_1, color_2) (src/Texture/checker.cpp:16-32):
+     * color_2 where (int)(u w) and (int)(v h) are both even or both odd, else
+     * color_1; no texels (no texture gradient) */
+    float checker_w, checker_h;
+    float color_1[3], color_2[3];
 } rumi_texture;
 
 typedef struct {
@@ -438,6 +447,14 @@ int rumi_bmp_header(uint32_t width, uint32_t height, uint32_t bytes_per_pixel, u
  * pixels NULL: only the size fields. */
 int rumi_bmp_read(const uint8_t* host_file, uint64_t file_bytes, uint32_t* width, uint32_t* height,
                   uint32_t* bytes_per_pixel, uint8_t* host_pixels, uint64_t pixel_capacity);
+
+/* ImageTexture(file, gamma)'s texel values (src/Texture/imagetexture.cpp:
+ * 21-61) from loaded pixel bytes: byte * (1 / 255) in FP32, raised to gamma
+ * (powf taken correctly rounded) unless gamma is 1; rgb: width * height * 3
+ * floats R, G, B in the buffer's row order (row 0 = v = 0), a
+ * RUMI_TEXTURE_IMAGE's data.  Host memory. */
+int rumi_bmp_texels(const uint8_t* host_pixels, uint32_t width, uint32_t height,
+                    uint32_t bytes_per_pixel, float gamma, float* host_rgb);
 
 /* ------------------------------------------------------------------ models */
 /* An OBJ model, loaded as Model::Model(file, transform) loads it

@@ -85,6 +85,8 @@ def lib() -> ctypes.CDLL:
         L.or_bmp_header.restype = None
         L.or_fireflies.argtypes = [vp, c_int, c_int, c_int, vp]
         L.or_fireflies.restype = None
+        L.or_bmp_texels.argtypes = [vp, c_int, c_int, c_int, ctypes.c_float, vp]
+        L.or_bmp_texels.restype = None
         _lib = L
     return _lib
 
@@ -258,6 +260,15 @@ def bmp_header(w, h, bpp):
     return out
 
 
+def bmp_texels(pixels, gamma):
+    """or_bmp_texels: (h, w, bpp) uint8 -> (h, w, 3) float32."""
+    pixels = np.ascontiguousarray(pixels, np.uint8)
+    h, w, bpp = pixels.shape
+    out = np.empty((h, w, 3), np.float32)
+    lib().or_bmp_texels(_p(pixels), w, h, bpp, float(gamma), _p(out))
+    return out
+
+
 def fireflies(src):
     """or_fireflies: (h, w, bpp) uint8 -> filtered copy."""
     src = np.ascontiguousarray(src, np.uint8)
@@ -400,15 +411,26 @@ def scene_arrays(scene: dict):
 
 def texture_arrays(scene: dict):
     """The scene's textures as the oracle's pool: (texels float32, table
-    int32 (n, 3) of width, height, first float), sizes in texels."""
-    texs = [np.ascontiguousarray(t, np.float32) for t in scene.get("textures", [])]
-    table = np.zeros((max(1, len(texs)), 3), np.int32)
-    off = 0
-    for i, t in enumerate(texs):
-        table[i] = (t.shape[1], t.shape[0], off)
+    int32 (n, 4) of width, height, first float, kind), count, image texels.
+    Checker textures (dicts) keep w, h, c1, c2 after all image texels."""
+    src = scene.get("textures", [])
+    table = np.zeros((max(1, len(src)), 4), np.int32)
+    parts, off = [], 0
+    for i, t in enumerate(src):
+        if isinstance(t, dict):
+            continue
+        t = np.ascontiguousarray(t, np.float32)
+        table[i] = (t.shape[1], t.shape[0], off, 0)
+        parts.append(t.reshape(-1))
         off += t.size
-    pool = np.concatenate([t.reshape(-1) for t in texs]) if texs else np.zeros(3, np.float32)
-    return np.ascontiguousarray(pool), table, len(texs), off // 3
+    n_texels = off // 3
+    for i, t in enumerate(src):
+        if isinstance(t, dict):
+            table[i] = (0, 0, off, 1)
+            parts.append(np.float32([t["w"], t["h"], *t["c1"], *t["c2"]]))
+            off += 8
+    pool = np.concatenate(parts) if parts else np.zeros(3, np.float32)
+    return np.ascontiguousarray(pool), table, len(src), n_texels
 
 
 _PT_ARGS = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,

@@ -226,6 +226,9 @@ void or_obj_free(float* p);
  * width * height * bpp bytes B, G, R (, 0). */
 void or_film_encode(const float* film, int width, int height, int bgr_sum, int n_samples,
                     float gamma, int bpp, unsigned char* out);
+/* ImageTexture's texel values from BMP pixel bytes (imagetexture.cpp:32-61). */
+void or_bmp_texels(const unsigned char* pixels, int width, int height, int bpp, float gamma,
+                   float* rgb);
 /* SaveBmp's 54-byte header (src/Kernel/bmp.cpp:59-97). */
 void or_bmp_header(int width, int height, int bpp, unsigned char* header);
 /* FirefliesFiltering (src/Scene/scene.cpp:261-431) over pixel bytes. */

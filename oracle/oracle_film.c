@@ -130,3 +130,18 @@ void or_fireflies(const unsigned char* src, int w, int h, int bpp, unsigned char
             for (k = 0; k < 3; ++k) dst[at + k] = channel_byte(s[k] * scale);
         }
 }
+
+/* ImageTexture::Pixel's texel values (src/Texture/imagetexture.cpp:32-61):
+ * byte * (1.0f / 255.0f), raised to gamma unless it is 1; R, G, B from the
+ * B, G, R bytes. */
+void or_bmp_texels(const unsigned char* px, int w, int h, int bpp, float gamma, float* rgb)
+{
+    const float inv = 1.0f / 255.0f;
+    long i, n = (long)w * h;
+    int c;
+    for (i = 0; i < n; ++i)
+        for (c = 0; c < 3; ++c) {
+            float x = (float)px[i * bpp + 2 - c] * inv;
+            rgb[3 * i + c] = gamma == 1.0f ? x : (float)pow((double)x, (double)gamma);
+        }
+}

@@ -147,7 +147,8 @@ typedef struct {
     int n_nodes;
     int* order;
     const float* texels;    /* RGB texel pool */
-    const int* tex_table;   /* per texture: width, height, first float of its texels */
+    const int* tex_table;   /* per texture: width, height, first float of its texels, kind
+                               (1: checker, its w, h, c1, c2 as 8 floats at that offset) */
     int n_tex;
     int whitted;            /* RayTracer: only specular and dielectric surfaces continue */
 } scene_t;
@@ -777,7 +778,15 @@ static double dlog_hg(double g, double c)
  * the texel's position in the pool. */
 static col texel(const scene_t* sc, int t, double u, double v, int* index)
 {
-    int W = sc->tex_table[3 * t], H = sc->tex_table[3 * t + 1], off = sc->tex_table[3 * t + 2];
+    int W = sc->tex_table[4 * t], H = sc->tex_table[4 * t + 1], off = sc->tex_table[4 * t + 2];
+    if (sc->tex_table[4 * t + 3] == 1) {
+        /* Checker::Pixel (checker.cpp:24-32), FP32 products as the reference's */
+        const float* c = sc->texels + off;
+        int a = (int)((float)u * c[0]) % 2, b = (int)((float)v * c[1]) % 2;
+        const float* pick = ((a == 0 && b == 0) || (a != 0 && b != 0)) ? c + 5 : c + 2;
+        *index = -1;
+        return cmk(pick[0], pick[1], pick[2]);
+    }
     int row = (int)((1.0 - v) * (H - 1));
     int cl = (int)(u * (W - 1));
     int y = H - 1 - row, x = cl;

@@ -69,7 +69,12 @@ class Medium(Structure):
 
 
 class Texture(Structure):
-    _fields_ = [("width", c_int32), ("height", c_int32), ("rgb", c_void_p)]
+    _fields_ = [("width", c_int32), ("height", c_int32), ("rgb", c_void_p), ("kind", c_int32),
+                ("checker_w", c_float), ("checker_h", c_float), ("color_1", c_float * 3),
+                ("color_2", c_float * 3)]
+
+
+RUMI_TEXTURE_IMAGE, RUMI_TEXTURE_CHECKER = 0, 1
 
 
 class SceneDesc(Structure):
@@ -167,6 +172,7 @@ _SIGNATURES = {
     "rumi_film_fireflies": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, c_void_p,
                                     c_void_p]),
     "rumi_bmp_header": (c_int, [c_uint32, c_uint32, c_uint32, c_void_p]),
+    "rumi_bmp_texels": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, c_float, c_void_p]),
     "rumi_bmp_read": (c_int, [c_void_p, c_uint64, POINTER(c_uint32), POINTER(c_uint32),
                               POINTER(c_uint32), c_void_p, c_uint64]),
 }
@@ -259,6 +265,15 @@ def bmp_read(data: bytes):
     pix = np.empty((h.value, w.value, b.value), np.uint8)
     check(lib().rumi_bmp_read(_ptr(buf), buf.size, byref(w), byref(h), byref(b), _ptr(pix), pix.size))
     return pix, w.value, h.value, b.value
+
+
+def bmp_texture(data: bytes, gamma: float = 1.0) -> np.ndarray:
+    """ImageTexture(file, gamma): a BMP file's bytes -> the (h, w, 3)
+    float32 texel array of a scene texture."""
+    pix, w, h, b = bmp_read(data)
+    rgb = np.empty((h, w, 3), np.float32)
+    check(lib().rumi_bmp_texels(_ptr(pix), w, h, b, gamma, _ptr(rgb)))
+    return rgb
 
 
 def load_obj(path, transform=None) -> dict:
@@ -471,6 +486,12 @@ class Scene:
         texs = (Texture * max(1, len(desc.get("textures", []))))()
         self.n_texels = 0
         for i, t in enumerate(desc.get("textures", [])):
+            if isinstance(t, dict):  # dict(kind="checker", w=, h=, c1=(r, g, b), c2=(r, g, b))
+                texs[i].kind = RUMI_TEXTURE_CHECKER
+                texs[i].checker_w, texs[i].checker_h = t["w"], t["h"]
+                texs[i].color_1 = (c_float * 3)(*t["c1"])
+                texs[i].color_2 = (c_float * 3)(*t["c2"])
+                continue
             rgb = np.ascontiguousarray(t, np.float32)
             self.n_texels += rgb.shape[0] * rgb.shape[1]
             self._keep.append(rgb)

@@ -234,6 +234,25 @@ extern "C" int rumi_bmp_header(uint32_t width, uint32_t height, uint32_t bytes_p
     return RUMI_OK;
 }
 
+extern "C" int rumi_bmp_texels(const uint8_t* pixels, uint32_t width, uint32_t height,
+                               uint32_t bytes_per_pixel, float gamma, float* rgb) {
+    clear_error();
+    if (!pixels || !rgb) return fail(RUMI_ERR_INVALID, "rumi_bmp_texels: null buffer");
+    if (bytes_per_pixel != 3 && bytes_per_pixel != 4)
+        return fail(RUMI_ERR_INVALID, "rumi_bmp_texels: bytes_per_pixel must be 3 or 4");
+    // ImageTexture::Pixel (imagetexture.cpp:32-61): byte * (1 / 255), raised
+    // to gamma unless gamma is 1 (powf taken correctly rounded); B, G, R
+    // bytes to R, G, B floats, rows as they are (row 0 = v = 0)
+    const float inv = 1.0f / 255.0f;
+    const uint64_t n = (uint64_t)width * height;
+    for (uint64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const float x = (float)pixels[i * bytes_per_pixel + (2 - c)] * inv;
+            rgb[3 * i + c] = gamma == 1.0f ? x : (float)std::pow((double)x, (double)gamma);
+        }
+    return RUMI_OK;
+}
+
 extern "C" int rumi_bmp_read(const uint8_t* file, uint64_t file_bytes, uint32_t* width, uint32_t* height,
                              uint32_t* bytes_per_pixel, uint8_t* pixels, uint64_t pixel_capacity) {
     clear_error();

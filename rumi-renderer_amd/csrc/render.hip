@@ -533,6 +533,13 @@ __device__ inline float light_g(V3 x, const LightSample& ls, V3 wi) {
 // ------------------------------------------------------------ textures
 __device__ inline V3 texel(const SceneView& sv, int tex, float u, float v, int& index) {
     const DevTexture t = sv.textures[tex];
+    if (t.kind == RUMI_TEXTURE_CHECKER) {
+        // Checker::Pixel (checker.cpp:24-32)
+        index = -1;
+        const int a = (int)(u * t.cw) % 2, b = (int)(v * t.ch) % 2;
+        const bool second = (a == 0 && b == 0) || (a != 0 && b != 0);
+        return second ? v3(t.c2[0], t.c2[1], t.c2[2]) : v3(t.c1[0], t.c1[1], t.c1[2]);
+    }
     // ImageTexture::Pixel (imagetexture.cpp:32-61): nearest texel
     const int row = (int)((1.0f - v) * (float)(t.height - 1));
     const int col = (int)(u * (float)(t.width - 1));

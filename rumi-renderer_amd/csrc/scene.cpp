@@ -346,6 +346,17 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
     std::vector<float> texels;
     for (uint32_t i = 0; i < d->n_textures; ++i) {
         const rumi_texture& t = d->textures[i];
+        texs[i] = DevTexture{};
+        texs[i].kind = t.kind;
+        if (t.kind == RUMI_TEXTURE_CHECKER) {
+            texs[i].cw = t.checker_w;
+            texs[i].ch = t.checker_h;
+            std::memcpy(texs[i].c1, t.color_1, sizeof(texs[i].c1));
+            std::memcpy(texs[i].c2, t.color_2, sizeof(texs[i].c2));
+            texs[i].offset = (uint32_t)texels.size();
+            continue;
+        }
+        if (t.kind != RUMI_TEXTURE_IMAGE) return fail(RUMI_ERR_INVALID, "texture %u: unknown kind %d", i, t.kind);
         if (t.width <= 0 || t.height <= 0 || !t.rgb)
             return fail(RUMI_ERR_INVALID, "texture %u: bad size or data", i);
         texs[i].width = t.width;

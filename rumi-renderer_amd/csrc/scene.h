@@ -62,7 +62,9 @@ struct DevCamera {
 struct DevTexture {
     int32_t width, height;
     uint32_t offset;  // float offset of the RGB data in the texel pool
-    int32_t pad;
+    int32_t kind;     // RUMI_TEXTURE_*
+    float cw, ch;     // checker: Checker(w, h, c1, c2)
+    float c1[3], c2[3];
 };
 
 }  // namespace rumi

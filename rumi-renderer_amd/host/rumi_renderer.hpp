@@ -206,16 +206,71 @@ class GGX : public Material {
 };
 
 // ----------------------------------------------------------------- textures
-// ImageTexture (src/Texture/imagetexture.cpp) from RGB floats (row 0 = v = 0);
-// the reference loads 8-bit BMP files, which is file I/O outside this library.
-class ImageTexture {
+class Texture {
+  public:
+    virtual ~Texture() = default;
+    virtual rumi_texture Desc() const = 0;
+};
+
+// ImageTexture (src/Texture/imagetexture.cpp): from RGB floats (row 0 = v =
+// 0), or as the reference builds it, from a BMP file and a gamma.
+class ImageTexture : public Texture {
   public:
     ImageTexture(int width, int height, const float* rgb)
         : _width(width), _height(height), _rgb(rgb, rgb + 3 * (size_t)width * height) {}
-    int _width, _height;
+    explicit ImageTexture(const char* file_name, float gamma = 1.0f) {
+        FILE* f = std::fopen(file_name, "rb");
+        if (!f) {
+            std::fprintf(stderr, "ImageTexture: cannot open %s\n", file_name);
+            return;
+        }
+        std::vector<uint8_t> bytes;
+        uint8_t buf[65536];
+        size_t n;
+        while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) bytes.insert(bytes.end(), buf, buf + n);
+        std::fclose(f);
+        uint32_t w = 0, h = 0, b = 0;
+        if (rumi_bmp_read(bytes.data(), bytes.size(), &w, &h, &b, nullptr, 0) != RUMI_OK) return;
+        std::vector<uint8_t> pix((size_t)w * h * b);
+        if (rumi_bmp_read(bytes.data(), bytes.size(), &w, &h, &b, pix.data(), pix.size()) != RUMI_OK ||
+            (b != 3 && b != 4))
+            return;
+        _rgb.resize(3 * (size_t)w * h);
+        if (rumi_bmp_texels(pix.data(), w, h, b, gamma, _rgb.data()) != RUMI_OK) {
+            _rgb.clear();
+            return;
+        }
+        _width = (int)w;
+        _height = (int)h;
+    }
+    rumi_texture Desc() const override {
+        rumi_texture t{};
+        t.width = _width;
+        t.height = _height;
+        t.rgb = _rgb.data();
+        t.kind = RUMI_TEXTURE_IMAGE;
+        return t;
+    }
+    int _width = 0, _height = 0;
     std::vector<float> _rgb;
 };
-using Texture = ImageTexture;
+
+// Checker(width, height, color_1, color_2) (src/Texture/checker.cpp:16-32).
+class Checker : public Texture {
+  public:
+    Checker(float w, float h, const Color& c1, const Color& c2) : _w(w), _h(h), _c1(c1), _c2(c2) {}
+    rumi_texture Desc() const override {
+        rumi_texture t{};
+        t.kind = RUMI_TEXTURE_CHECKER;
+        t.checker_w = _w;
+        t.checker_h = _h;
+        t.color_1[0] = _c1._r, t.color_1[1] = _c1._g, t.color_1[2] = _c1._b;
+        t.color_2[0] = _c2._r, t.color_2[1] = _c2._g, t.color_2[2] = _c2._b;
+        return t;
+    }
+    float _w, _h;
+    Color _c1, _c2;
+};
 
 // ------------------------------------------------------------------- shapes
 class Shape {
@@ -597,7 +652,7 @@ class Scene {
                 if (ti < 0) {
                     ti = (int)tseen.size();
                     tseen.push_back(t);
-                    _texs.push_back(rumi_texture{t->_width, t->_height, t->_rgb.data()});
+                    _texs.push_back(t->Desc());
                 }
                 s.texture = ti;
             }

@@ -147,3 +147,27 @@ def glossy_box(width=16, height=16):
     shapes[-2]["material"] = 5                          # the large sphere
     sc["shapes"] = shapes
     return sc
+
+
+def checker_scene(width=16, height=16):
+    """The textured scene with the floor rect under Checker(12, 12, 0.7, 1.0)
+    (the reference's commented Cornell floor) and the sphere under
+    Checker(48, 0, 0.5, 1.0), image textures elsewhere."""
+    sc = textured_scene(width, height)
+    sc["textures"][0] = dict(kind="checker", w=12.0, h=12.0, c1=(0.7, 0.7, 0.7), c2=(1.0, 1.0, 1.0))
+    sc["textures"][1] = dict(kind="checker", w=48.0, h=0.0, c1=(0.5, 0.5, 0.5), c2=(1.0, 1.0, 1.0))
+    return sc
+
+
+def checker_light_scene(u, v, w=12.0, h=12.0):
+    """An upward-facing emissive rect (Le = 1, 2, 3) on the floor textured with Checker(w, h, c1 = (0.25, 0.5, 0.75),
+    c2 = (1, 1, 1)); the camera's single pixel looks straight down at the
+    rect point with texture coordinates (u, v) through a 0.01 degree field."""
+    mats = [dict(kind="emissive", kd=1.0, color=(1.0, 2.0, 3.0))]
+    shapes = [dict(kind="rect", p=(0.0, 0.0, 0.0), l=(1.0, 0, 0), w=(0, 0, -1.0), material=0,
+                   texture=0)]
+    tex = [dict(kind="checker", w=w, h=h, c1=(0.25, 0.5, 0.75), c2=(1.0, 1.0, 1.0))]
+    target = (u, 0.0, -v)
+    pos = (u + 1e-4, 1.0, -v)
+    cam = dict(position=pos, basis=look_at_basis(pos, target), fov_deg=0.01, y_ratio=1.0)
+    return dict(materials=mats, shapes=shapes, camera=cam, medium=None, textures=tex)

@@ -112,3 +112,34 @@ def test_fireflies_leaves_unsaturated_images_alone(oracle):
     want = img.copy()
     want[..., 3] = 0
     np.testing.assert_array_equal(out, want)
+
+
+@pytest.fixture(scope="module")
+def earth_bmp():
+    import gzip
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    return gzip.open(os.path.join(here, "golden", "textures", "earth.bmp.gz")).read()
+
+
+@pytest.mark.parametrize("gamma", [1.0, 2.2])
+def test_image_texture_from_the_references_bmp(rumi, oracle, earth_bmp, gamma):
+    """ImageTexture("earth.bmp", gamma): LoadBmp's pixels, then byte / 255
+    (to the power gamma); the library's texels equal the oracle's bit for
+    bit and the float64 values to rounding, R, G, B from B, G, R bytes."""
+    pix, w, h, b = rumi.bmp_read(earth_bmp)
+    assert (w, h, b) == (1000, 500, 3)
+    np.testing.assert_array_equal(pix.reshape(-1), np.frombuffer(earth_bmp[54:], np.uint8))
+    tex = rumi.bmp_texture(earth_bmp, gamma)
+    np.testing.assert_array_equal(tex.view(np.int32), oracle.bmp_texels(pix, gamma).view(np.int32))
+    want = (pix[..., ::-1].astype(np.float64) / 255.0) ** gamma
+    # 1/255 and the product each round once in FP32 (2.4e-7), the power scales it
+    np.testing.assert_allclose(tex, want, rtol=2.5e-7 * max(gamma, 1.0), atol=1e-9)
+
+
+def test_checker_c_class_and_texture_kinds(rumi):
+    """A scene description takes checker textures (no texels) beside image
+    textures; unknown kinds are rejected."""
+    t = rumi.Texture()
+    t.kind = 7
+    assert t.kind == 7 and rumi.RUMI_TEXTURE_CHECKER == 1

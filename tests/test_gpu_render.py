@@ -82,7 +82,7 @@ def test_matches_oracle_statistically(rumi, oracle, ctx, scenes, name, factory, 
     assert abs(g.mean() - c.mean()) < 3 * se + 0.05 * c.mean(), (g, c)
 
 
-@pytest.mark.parametrize("case", ["cornell", "media", "glossy", "textured"])
+@pytest.mark.parametrize("case", ["cornell", "media", "glossy", "textured", "checker"])
 def test_same_seed_matches_oracle_sample_for_sample(rumi, oracle, ctx, scenes, case):
     """The GPU and the oracle draw every sample's numbers from the same PCG32
     streams in the same order (a per-sample path stream, a per-vertex light
@@ -91,7 +91,8 @@ def test_same_seed_matches_oracle_sample_for_sample(rumi, oracle, ctx, scenes, c
     that a different seed shows."""
     w = 32
     sc = {"cornell": lambda: scenes.cornell_box(w, w), "media": lambda: scenes.cornell_media(w, w),
-          "glossy": lambda: rc.glossy_box(w, w), "textured": lambda: rc.textured_scene(w, w)}[case]()
+          "glossy": lambda: rc.glossy_box(w, w), "textured": lambda: rc.textured_scene(w, w),
+          "checker": lambda: rc.checker_scene(w, w)}[case]()
     c = oracle.render_pt(sc, w, w, 64, 16, seed=5, nthreads=8)[0]
     same = render(rumi, ctx, sc, w, w, 64, 16, seed=5)[0]
     other = render(rumi, ctx, sc, w, w, 64, 16, seed=6)[0]

@@ -84,3 +84,16 @@ def test_oracle_consistent_with_its_high_spp_reference(oracle, scenes, name, fac
     want = rc.expected_rel_mse(ref["mean"], ref["m2"], 256, int(ref["spp"]))
     se = vals.std(ddof=1) / np.sqrt(len(vals))
     assert abs(vals.mean() - want) < 3 * se + 0.05 * want, (vals, want)
+
+
+@pytest.mark.parametrize("u,v,w,h", [(0.1, 0.1, 12, 12), (0.1, 0.2, 12, 12), (0.2, 0.1, 12, 12),
+                                     (0.55, 0.95, 12, 12), (0.03, 0.7, 48, 0), (0.05, 0.7, 48, 0)])
+def test_checker_texture_on_an_emitter(oracle, u, v, w, h):
+    """Checker::Pixel (checker.cpp:24-32) through Rect::TextureMapping: the
+    emitter's radiance Le times color_2 where (int)(u w) and (int)(v h) have
+    the same parity, else color_1."""
+    sc = rc.checker_light_scene(u, v, w, h)
+    mean, _ = oracle.render_pt(sc, 1, 1, 16, 2, seed=1, nthreads=1)
+    a, b = int(u * w) % 2, int(v * h) % 2
+    c = np.ones(3) if a == b else np.array([0.25, 0.5, 0.75])
+    np.testing.assert_allclose(mean[0, 0], np.array([1.0, 2.0, 3.0]) * c, rtol=1e-6)
