@@ -218,11 +218,18 @@ typedef struct {
 typedef struct {
     int32_t kind;       /* RUMI_SHAPE_* */
     int32_t material;   /* index into rumi_scene_desc.materials */
-    int32_t is_light;   /* 1: an AreaLight on this shape (rect or sphere, emissive material) */
+    int32_t is_light;   /* 1: an AreaLight on this shape (rect or sphere, emissive material);
+                           2: an ImageBasedLight (src/Light/imagebasedlight.cpp:24-64) on
+                           a sphere with an emissive material and a texture: light
+                           samples cosine-distributed about the shading normal, the
+                           texture's value where they leave the sphere, pdf cos/pi;
+                           path tracer and RayTracer only, no medium */
     int32_t texture;    /* index into rumi_scene_desc.textures or -1 (multiplies radiance,
                            PathTracer::Trace pathtracer.cpp:343-344) */
     float v[9];         /* triangle v0 v1 v2 | sphere centre in v[0..2] | rect p, l, w */
-    float n[9];         /* triangle vertex normals (all zero: geometric normal) */
+    float n[9];         /* triangle vertex normals (all zero: geometric normal); sphere:
+                           the rows of Shape::_transform's 3x3 its TextureMapping measures
+                           angles against (sphere.cpp:172-195; all zero: identity) */
     float radius;       /* sphere */
     /* RUMI_SHAPE_MESH (host memory, copied at scene creation): */
     const float* xyz;   uint32_t nv;
@@ -250,13 +257,15 @@ typedef struct {
     float g;            /* Henyey-Greenstein anisotropy; 0 = the reference's Isotropic phase */
 } rumi_medium;
 
-enum { RUMI_TEXTURE_IMAGE = 0, RUMI_TEXTURE_CHECKER = 1 };
+enum { RUMI_TEXTURE_IMAGE = 0, RUMI_TEXTURE_CHECKER = 1, RUMI_TEXTURE_HDR = 2 };
 
 typedef struct {
     int32_t width, height;
     const float* rgb;   /* IMAGE: host, width * height * 3, row 0 = v = 0 (ImageTexture::Pixel,
                            src/Texture/imagetexture.cpp:32-61: nearest texel) */
-    int32_t kind;       /* RUMI_TEXTURE_* (0 = IMAGE) */
+    int32_t kind;       /* RUMI_TEXTURE_* (0 = IMAGE); HDR: HDRTexture (src/Texture/
+                           hdrtexture.cpp:21-41): rgb rows top-down as rumi_hdr_read gives
+                           them, texel (int)((1 - v)(h - 1)), (int)((1 - u)(w - 1)) */
     /* CHECKER: Checker(w, h, color_1, color_2) (src/Texture/checker.cpp:16-32):
      * color_2 where (int)(u w) and (int)(v h) are both even or both odd, else
      * color_1; no texels (no texture gradient) */
@@ -447,6 +456,17 @@ int rumi_bmp_header(uint32_t width, uint32_t height, uint32_t bytes_per_pixel, u
  * pixels NULL: only the size fields. */
 int rumi_bmp_read(const uint8_t* host_file, uint64_t file_bytes, uint32_t* width, uint32_t* height,
                   uint32_t* bytes_per_pixel, uint8_t* host_pixels, uint64_t pixel_capacity);
+
+/* LoadHDR (src/Kernel/hdr.cpp:4-177) over a Radiance RGBE file image in host
+ * memory: "#?RADIANCE", '#' lines, "FORMAT=32-bit_rle_rgbe", lines up to an
+ * empty one, "-Y h +X w", then h new-style run-length scanlines; each pixel
+ * (r, g, b, e) -> (r, g, b) * 2^(e - 136), 0 when e = 0.  rgb: w * h * 3
+ * floats, rows top-down as in the file (a RUMI_TEXTURE_HDR's data); NULL:
+ * only the size.  A file the reference cannot read (another format, old
+ * run-length scanlines, a width below 8 or above 32767 or whose low byte is
+ * 128 or more -- the reference compares it as a signed char) is rejected. */
+int rumi_hdr_read(const uint8_t* host_file, uint64_t file_bytes, uint32_t* width, uint32_t* height,
+                  float* host_rgb, uint64_t rgb_capacity);
 
 /* ImageTexture(file, gamma)'s texel values (src/Texture/imagetexture.cpp:
  * 21-61) from loaded pixel bytes: byte * (1 / 255) in FP32, raised to gamma

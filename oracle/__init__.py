@@ -87,6 +87,7 @@ def lib() -> ctypes.CDLL:
         L.or_fireflies.restype = None
         L.or_bmp_texels.argtypes = [vp, c_int, c_int, c_int, ctypes.c_float, vp]
         L.or_bmp_texels.restype = None
+        L.or_hdr_read.argtypes = [vp, ctypes.c_size_t, POINTER(c_int), POINTER(c_int), vp]
         _lib = L
     return _lib
 
@@ -269,6 +270,19 @@ def bmp_texels(pixels, gamma):
     return out
 
 
+def hdr_read(data: bytes):
+    """or_hdr_read: RGBE file bytes -> (h, w, 3) float32 rows top-down;
+    ValueError where the reference would fail."""
+    buf = np.frombuffer(data, np.uint8)
+    w, h = c_int(0), c_int(0)
+    if lib().or_hdr_read(_p(buf), buf.size, byref(w), byref(h), None) != 0:
+        raise ValueError("or_hdr_read: header")
+    rgb = np.empty((h.value, w.value, 3), np.float32)
+    if lib().or_hdr_read(_p(buf), buf.size, byref(w), byref(h), _p(rgb)) != 0:
+        raise ValueError("or_hdr_read: scanlines")
+    return rgb
+
+
 def fireflies(src):
     """or_fireflies: (h, w, bpp) uint8 -> filtered copy."""
     src = np.ascontiguousarray(src, np.uint8)
@@ -356,7 +370,7 @@ def scene_arrays(scene: dict):
         r = np.zeros((1, 32), np.float32)
         r[0, 1] = s["material"]
         r[0, 22] = s.get("texture", -1)
-        r[0, 2] = 1.0 if s.get("light") else 0.0
+        r[0, 2] = 2.0 if s.get("light") == "ibl" else (1.0 if s.get("light") else 0.0)
         if k == "tri":
             r[0, 0] = OR_TRI
             r[0, 3:12] = np.asarray(s["v"], np.float32).reshape(9)
@@ -365,6 +379,8 @@ def scene_arrays(scene: dict):
             r[0, 0] = OR_SPHERE
             r[0, 3:6] = s["center"]
             r[0, 12] = s["radius"]
+            if s.get("basis") is not None:
+                r[0, 13:22] = np.asarray(s["basis"], np.float32).reshape(9)
         elif k == "rect":
             r[0, 0] = OR_RECT
             r[0, 3:6] = s["p"]
@@ -417,15 +433,18 @@ def texture_arrays(scene: dict):
     table = np.zeros((max(1, len(src)), 4), np.int32)
     parts, off = [], 0
     for i, t in enumerate(src):
-        if isinstance(t, dict):
+        if isinstance(t, dict) and t.get("kind") == "checker":
             continue
+        kind = 0
+        if isinstance(t, dict):  # dict(kind="hdr", rgb=(h, w, 3) rows top-down)
+            kind, t = 2, t["rgb"]
         t = np.ascontiguousarray(t, np.float32)
-        table[i] = (t.shape[1], t.shape[0], off, 0)
+        table[i] = (t.shape[1], t.shape[0], off, kind)
         parts.append(t.reshape(-1))
         off += t.size
     n_texels = off // 3
     for i, t in enumerate(src):
-        if isinstance(t, dict):
+        if isinstance(t, dict) and t.get("kind") == "checker":
             table[i] = (0, 0, off, 1)
             parts.append(np.float32([t["w"], t["h"], *t["c1"], *t["c2"]]))
             off += 8

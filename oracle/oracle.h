@@ -229,6 +229,9 @@ void or_film_encode(const float* film, int width, int height, int bgr_sum, int n
 /* ImageTexture's texel values from BMP pixel bytes (imagetexture.cpp:32-61). */
 void or_bmp_texels(const unsigned char* pixels, int width, int height, int bpp, float gamma,
                    float* rgb);
+/* LoadHDR (src/Kernel/hdr.cpp:4-177) over a Radiance RGBE file image:
+ * rows top-down, (r, g, b) * 2^(e - 136); rgb NULL: only the size. */
+int or_hdr_read(const unsigned char* data, size_t n, int* width, int* height, float* rgb);
 /* SaveBmp's 54-byte header (src/Kernel/bmp.cpp:59-97). */
 void or_bmp_header(int width, int height, int bpp, unsigned char* header);
 /* FirefliesFiltering (src/Scene/scene.cpp:261-431) over pixel bytes. */

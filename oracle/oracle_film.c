@@ -11,6 +11,8 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 static unsigned char channel_byte(float v)
@@ -144,4 +146,93 @@ void or_bmp_texels(const unsigned char* px, int w, int h, int bpp, float gamma, 
             float x = (float)px[i * bpp + 2 - c] * inv;
             rgb[3 * i + c] = gamma == 1.0f ? x : (float)pow((double)x, (double)gamma);
         }
+}
+
+/* LoadHDR + ScanLine (src/Kernel/hdr.cpp:4-177) over a file image in memory:
+ * rows top-down, (r, g, b) * 2^(e - 136).  Returns -1 where the reference
+ * would fail or read garbage (see the library's rumi_hdr_read). */
+static int next_line(const unsigned char* d, size_t n, size_t* at, char* out)
+{
+    int k = 0;
+    if (*at >= n) return 0;
+    while (k < 255 && *at < n) {
+        char c = (char)d[(*at)++];
+        out[k++] = c;
+        if (c == '\n') break;
+    }
+    out[k] = 0;
+    return 1;
+}
+
+int or_hdr_read(const unsigned char* d, size_t n, int* width, int* height, float* rgb)
+{
+    char line[256];
+    size_t at = 0;
+    int w = 0, h = 0, y, x, c;
+    if (!next_line(d, n, &at, line) || strcmp(line, "#?RADIANCE\n") != 0) return -1;
+    do {
+        if (!next_line(d, n, &at, line)) return -1;
+    } while (line[0] == '#');
+    if (strcmp(line, "FORMAT=32-bit_rle_rgbe\n") != 0) return -1;
+    do {
+        if (!next_line(d, n, &at, line)) return -1;
+    } while (line[0] != '\n');
+    if (!next_line(d, n, &at, line) || sscanf(line, "-Y %d + X  %d", &h, &w) != 2 || h <= 0 || w <= 0)
+        return -1;
+    *width = w;
+    *height = h;
+    if (!rgb) return 0;
+    {
+        unsigned char* pl = (unsigned char*)malloc(4 * (size_t)w);
+        if (!pl) return -1;
+        for (y = 0; y < h; ++y) {
+            signed char hd[4];
+            unsigned char* p = pl;
+            if (w < 8 || w > 0x7fff || at + 4 > n) { free(pl); return -1; }
+            for (c = 0; c < 4; ++c) hd[c] = (signed char)d[at + c];
+            at += 4;
+            if (hd[0] != 2 || hd[1] != 2 || (hd[2] & 0x80) || ((((int)hd[2]) << 8) | (int)hd[3]) != w) {
+                free(pl);
+                return -1;
+            }
+            for (c = 0; c < 4; ++c) {
+                unsigned char* end = pl + (size_t)(c + 1) * w;
+                while (p < end) {
+                    int cnt;
+                    unsigned char a, b;
+                    if (at + 2 > n) { free(pl); return -1; }
+                    a = d[at];
+                    b = d[at + 1];
+                    at += 2;
+                    if (a > 128) {
+                        cnt = a - 128;
+                        if (cnt > end - p) { free(pl); return -1; }
+                        while (cnt-- > 0) *p++ = b;
+                    } else {
+                        cnt = a;
+                        if (cnt == 0 || cnt > end - p) { free(pl); return -1; }
+                        *p++ = b;
+                        if (--cnt > 0) {
+                            if (at + (size_t)cnt > n) { free(pl); return -1; }
+                            memcpy(p, d + at, (size_t)cnt);
+                            at += (size_t)cnt;
+                            p += cnt;
+                        }
+                    }
+                }
+            }
+            for (x = 0; x < w; ++x) {
+                float* o = rgb + 3 * ((size_t)y * w + x);
+                int e = pl[3 * (size_t)w + x];
+                if (e) {
+                    float f = (float)ldexp(1.0, e - 136);
+                    for (c = 0; c < 3; ++c) o[c] = (float)pl[(size_t)c * w + x] * f;
+                } else {
+                    o[0] = o[1] = o[2] = 0.0f;
+                }
+            }
+        }
+        free(pl);
+    }
+    return 0;
 }

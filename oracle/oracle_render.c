@@ -184,7 +184,7 @@ static int hit_tri(const shape_t* s, ray_t* ray, double* u_out, double* v_out, d
     return 1;
 }
 
-static int hit_sphere(const shape_t* s, ray_t* ray, int in_object)
+static int hit_sphere(const shape_t* s, ray_t* ray, int in_object, int ibl)
 {
     v3 oc = sub(ray->o, s->a);
     double B = 2.0 * dot(ray->d, oc);
@@ -201,6 +201,13 @@ static int hit_sphere(const shape_t* s, ray_t* ray, int in_object)
     }
     if (t0 < 0.0 && t1 < 0.0) return 0;
     if (in_object) {
+        double tmp = t0;
+        t0 = t1;
+        t1 = tmp;
+    }
+    /* image-based light: an emissive textured sphere seen from inside
+       (Sphere::Intersect, sphere.cpp:85-92) swaps the roots again */
+    if (ibl && sqrt(dot(oc, oc)) < s->r) {
         double tmp = t0;
         t0 = t1;
         t1 = tmp;
@@ -244,7 +251,7 @@ static int hit_prim(const scene_t* sc, int i, ray_t* ray, int in_object, hitrec_
     if (s->kind == OR_TRI)
         ok = hit_tri(s, ray, &u, &v, &det);
     else if (s->kind == OR_SPHERE)
-        ok = hit_sphere(s, ray, in_object);
+        ok = hit_sphere(s, ray, in_object, sc->mats[s->mat].kind == MAT_EMISSIVE && s->tex >= 0);
     else
         ok = hit_rect(s, &sc->mats[s->mat], ray);
     if (ok && h) {
@@ -648,9 +655,13 @@ static v3 phase_sample(double g, v3 d, pcg_t* gen, double* pdf)
 /* ------------------------------------------------------------ lights */
 typedef struct {
     v3 p, n;
-    double pdf; /* (1 / n_lights) * (1 / area) */
+    double pdf; /* (1 / n_lights) * (1 / area); image-based light: (1 / n_lights) cos / pi */
     col le;
+    int ibl;    /* image-based light: the direction is wi, G = 1 */
+    v3 wi;
 } lsample_t;
+
+static col pattern(const scene_t* sc, int obj, v3 p, double bu, double bv, int* index);
 
 /* The stream of a vertex's light sample: a second PCG32 stream seeded from
  * the path stream's state on arrival at the vertex (recorded in the forward
@@ -667,13 +678,35 @@ static void nee_stream(pcg_t* n, const pcg_t* path, uint64_t key)
     pcg_next(n);
 }
 
-static void light_sample(const scene_t* sc, pcg_t* g, lsample_t* ls)
+static void light_sample(const scene_t* sc, v3 x, v3 nrm, pcg_t* g, lsample_t* ls)
 {
     int li = (int)(rnd(g) * sc->n_lights);
     if (li >= sc->n_lights) li = sc->n_lights - 1;
     const shape_t* s = &sc->shapes[sc->lights[li]];
     const mat_t* m = &sc->mats[s->mat];
     double area;
+    ls->ibl = 0;
+    if (s->light == 2) {
+        /* ImageBasedLight::UniformSampling (imagebasedlight.cpp:35-64) */
+        v3 u, w;
+        basis(nrm, &u, &w);
+        double phi = rnd(g) * 2.0 * PI_D;
+        double theta = acos(sqrt(1.0 - rnd(g)));
+        double st = sin(theta), c;
+        v3 wi = frame(u, nrm, w, st * sin(phi), cos(theta), st * cos(phi));
+        ray_t r = {x, wi, 0.0, 1e300};
+        int idx;
+        hit_sphere(s, &r, 0, 1);
+        ls->p = add(x, mul(wi, r.tmin));
+        ls->n = neg(wi);
+        ls->le = pattern(sc, sc->lights[li], ls->p, 0.0, 0.0, &idx);
+        c = dot(wi, nrm);
+        if (c <= 0.0) c = 1e-4;
+        ls->pdf = (1.0 / sc->n_lights) * (INV_PI_D * c > 0.0 ? INV_PI_D * c : 1e-4);
+        ls->ibl = 1;
+        ls->wi = wi;
+        return;
+    }
     if (s->kind == OR_RECT) {
         double a = rnd(g), b = rnd(g);
         ls->p = add(add(s->a, mul(s->b, a)), mul(s->c, b));
@@ -708,10 +741,10 @@ static double shadow_tr(const scene_t* sc, const ray_t* shadow, pcg_t* g, double
 }
 
 /* Scene::VisibilityTest (src/Scene/scene.cpp:99-123). */
-static int visibility(const scene_t* sc, v3 x, pcg_t* g, lsample_t* ls, v3* wi, ray_t* shadow)
+static int visibility(const scene_t* sc, v3 x, v3 nrm, pcg_t* g, lsample_t* ls, v3* wi, ray_t* shadow)
 {
-    light_sample(sc, g, ls);
-    *wi = norm(sub(ls->p, x));
+    light_sample(sc, x, nrm, g, ls);
+    *wi = ls->ibl ? ls->wi : norm(sub(ls->p, x));
     shadow->o = x;
     shadow->d = *wi;
     shadow->tmin = 0.0;
@@ -726,6 +759,7 @@ static int visibility(const scene_t* sc, v3 x, pcg_t* g, lsample_t* ls, v3* wi, 
 /* AreaLight::G (arealight.cpp:106-112). */
 static double geom(v3 x, const lsample_t* ls, v3 wi)
 {
+    if (ls->ibl) return 1.0; /* ImageBasedLight::G */
     double d = len(sub(x, ls->p));
     double c = -dot(wi, ls->n);
     if (c < 0.0) c = 0.0;
@@ -740,7 +774,7 @@ static col matte_direct(const scene_t* sc, const mat_t* m, v3 p, v3 n, pcg_t* g,
     v3 wi;
     ray_t sh;
     *smax_s = 0.0;
-    if (!visibility(sc, p, g, &ls, &wi, &sh)) return cs(0.0);
+    if (!visibility(sc, p, n, g, &ls, &wi, &sh)) return cs(0.0);
     col f = cscale(m->color, INV_PI_D * m->kd);
     double c = dot(wi, n);
     if (c < 0.0) c = 0.0;
@@ -760,7 +794,7 @@ static col medium_direct0(const scene_t* sc, v3 p, v3 d, pcg_t* g, double* smax_
     ray_t sh;
     *smax_s = 0.0;
     *cos_l = 0.0;
-    if (!visibility(sc, p, g, &ls, &wi, &sh)) return cs(0.0);
+    if (!visibility(sc, p, mk(0, 1, 0), g, &ls, &wi, &sh)) return cs(0.0); /* no image-based light with media */
     double tau = shadow_tr(sc, &sh, g, smax_s);
     *cos_l = dot(d, wi);
     double fp = sc->med.g == 0.0 ? 1.0 / (4.0 * PI_D) : hg_eval(sc->med.g, *cos_l);
@@ -779,6 +813,17 @@ static double dlog_hg(double g, double c)
 static col texel(const scene_t* sc, int t, double u, double v, int* index)
 {
     int W = sc->tex_table[4 * t], H = sc->tex_table[4 * t + 1], off = sc->tex_table[4 * t + 2];
+    if (sc->tex_table[4 * t + 3] == 2) {
+        /* HDRTexture::Pixel (hdrtexture.cpp:29-41): rows as loaded, u mirrored */
+        int y = (int)((1.0 - v) * (H - 1)), x = (int)((1.0 - u) * (W - 1));
+        if (y < 0) y = 0;
+        if (y > H - 1) y = H - 1;
+        if (x < 0) x = 0;
+        if (x > W - 1) x = W - 1;
+        *index = off / 3 + y * W + x;
+        const float* q = sc->texels + off + 3 * ((size_t)y * W + x);
+        return cmk(q[0], q[1], q[2]);
+    }
     if (sc->tex_table[4 * t + 3] == 1) {
         /* Checker::Pixel (checker.cpp:24-32), FP32 products as the reference's */
         const float* c = sc->texels + off;
@@ -812,10 +857,18 @@ static col pattern(const scene_t* sc, int obj, v3 p, double bu, double bv, int* 
         u = w0 * s->uv[0] + bu * s->uv[2] + bv * s->uv[4];
         v = w0 * s->uv[1] + bu * s->uv[3] + bv * s->uv[5];
     } else if (s->kind == OR_SPHERE) {
+        /* angles against the rows of the shape's transform (n0, n1, n2;
+           all zero: identity) */
+        v3 uu = s->n0, vv = s->n1, ww = s->n2;
+        if (dot(uu, uu) == 0.0 && dot(vv, vv) == 0.0 && dot(ww, ww) == 0.0) {
+            uu = mk(1, 0, 0);
+            vv = mk(0, 1, 0);
+            ww = mk(0, 0, 1);
+        }
         v3 pc = norm(sub(p, s->a));
-        double theta = acos(pc.y);
+        double theta = acos(dot(pc, vv));
         v3 pxz = mul(pc, sin(theta));
-        double phi = atan2(pxz.x, pxz.z);
+        double phi = atan2(dot(pxz, uu), dot(pxz, ww));
         u = phi >= 0.0 ? phi / (2.0 * PI_D) : (phi + 2.0 * PI_D) / (2.0 * PI_D);
         v = 1.0 - theta / PI_D;
     } else {
@@ -1252,7 +1305,7 @@ static void load_scene(scene_t* sc, const float* shapes, int n_shapes, const flo
         shape_t* o = &sc->shapes[i];
         o->kind = (int)s[0];
         o->mat = (int)s[1];
-        o->light = s[2] != 0.0f;
+        o->light = (int)s[2]; /* 1 area light, 2 image-based light */
         o->a = mk(s[3], s[4], s[5]);
         o->b = mk(s[6], s[7], s[8]);
         o->c = mk(s[9], s[10], s[11]);
@@ -1420,7 +1473,8 @@ static int run_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
     sc.whitted = whitted;
     int i, t, k;
     for (i = 0; i < sc.n_shapes; ++i)
-        if (sc.shapes[i].tex >= n_tex) {
+        if (sc.shapes[i].tex >= n_tex || (sc.shapes[i].light == 2 && sc.med.kind)) {
+            /* a texture index out of range; an image-based light with a medium */
             free_scene(&sc);
             return 1;
         }
@@ -1881,7 +1935,7 @@ static void* bdpt_rows(void* arg)
                 int nL = 0;
                 if (sc->n_lights > 0) {
                     lsample_t ls;
-                    light_sample(sc, &gl, &ls);
+                    light_sample(sc, mk(0, 0, 0), mk(0, 1, 0), &gl, &ls);
                     Lv[0].p = ls.p;
                     Lv[0].n = ls.n;
                     Lv[0].pdf_fwd = ls.pdf;
@@ -2049,8 +2103,13 @@ int or_render_bdpt(const float* shapes, int n_shapes, const float* mats, int n_m
     if (width <= 0 || height <= 0 || spp <= 0 || n_eye < 2 || n_light < 1 || !mean_rgb) return 1;
     scene_t sc;
     load_scene(&sc, shapes, n_shapes, mats, n_mats, camera, medium);
-    int* light_of = (int*)malloc(sizeof(int) * (size_t)(n_shapes > 0 ? n_shapes : 1));
     int i, t;
+    for (i = 0; i < sc.n_shapes; ++i)
+        if (sc.shapes[i].light == 2) { /* no image-based light in the bidirectional tracer */
+            free_scene(&sc);
+            return 1;
+        }
+    int* light_of = (int*)malloc(sizeof(int) * (size_t)(n_shapes > 0 ? n_shapes : 1));
     for (i = 0; i < n_shapes; ++i) light_of[i] = -1;
     for (i = 0; i < sc.n_lights; ++i) light_of[sc.lights[i]] = i;
     if (nthreads < 1) nthreads = 1;

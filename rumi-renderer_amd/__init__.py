@@ -74,7 +74,7 @@ class Texture(Structure):
                 ("color_2", c_float * 3)]
 
 
-RUMI_TEXTURE_IMAGE, RUMI_TEXTURE_CHECKER = 0, 1
+RUMI_TEXTURE_IMAGE, RUMI_TEXTURE_CHECKER, RUMI_TEXTURE_HDR = 0, 1, 2
 
 
 class SceneDesc(Structure):
@@ -173,6 +173,8 @@ _SIGNATURES = {
                                     c_void_p]),
     "rumi_bmp_header": (c_int, [c_uint32, c_uint32, c_uint32, c_void_p]),
     "rumi_bmp_texels": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, c_float, c_void_p]),
+    "rumi_hdr_read": (c_int, [c_void_p, c_uint64, POINTER(c_uint32), POINTER(c_uint32), c_void_p,
+                              c_uint64]),
     "rumi_bmp_read": (c_int, [c_void_p, c_uint64, POINTER(c_uint32), POINTER(c_uint32),
                               POINTER(c_uint32), c_void_p, c_uint64]),
 }
@@ -273,6 +275,17 @@ def bmp_texture(data: bytes, gamma: float = 1.0) -> np.ndarray:
     pix, w, h, b = bmp_read(data)
     rgb = np.empty((h, w, 3), np.float32)
     check(lib().rumi_bmp_texels(_ptr(pix), w, h, b, gamma, _ptr(rgb)))
+    return rgb
+
+
+def hdr_read(data: bytes) -> np.ndarray:
+    """LoadHDR: a Radiance RGBE file's bytes -> (h, w, 3) float32, rows
+    top-down (the data of a dict(kind="hdr", rgb=...) scene texture)."""
+    buf = np.frombuffer(data, np.uint8)
+    w, h = c_uint32(0), c_uint32(0)
+    check(lib().rumi_hdr_read(_ptr(buf), buf.size, byref(w), byref(h), None, 0))
+    rgb = np.empty((h.value, w.value, 3), np.float32)
+    check(lib().rumi_hdr_read(_ptr(buf), buf.size, byref(w), byref(h), _ptr(rgb), rgb.size))
     return rgb
 
 
@@ -452,7 +465,7 @@ class Scene:
         for i, sd in enumerate(desc["shapes"]):
             sh = shapes[i]
             sh.material = sd["material"]
-            sh.is_light = 1 if sd.get("light") else 0
+            sh.is_light = 2 if sd.get("light") == "ibl" else (1 if sd.get("light") else 0)
             sh.texture = sd.get("texture", -1)
             k = sd["kind"]
             if k == "tri":
@@ -464,6 +477,8 @@ class Scene:
                 sh.kind = RUMI_SHAPE_SPHERE
                 sh.v = (c_float * 9)(*(list(map(float, sd["center"])) + [0.0] * 6))
                 sh.radius = sd["radius"]
+                if sd.get("basis") is not None:  # rows of the shape's transform (texture mapping)
+                    sh.n = (c_float * 9)(*np.asarray(sd["basis"], np.float32).reshape(9).tolist())
             elif k == "rect":
                 sh.kind = RUMI_SHAPE_RECT
                 sh.v = (c_float * 9)(*[float(x) for x in (*sd["p"], *sd["l"], *sd["w"])])
@@ -486,6 +501,14 @@ class Scene:
         texs = (Texture * max(1, len(desc.get("textures", []))))()
         self.n_texels = 0
         for i, t in enumerate(desc.get("textures", [])):
+            if isinstance(t, dict) and t.get("kind") == "hdr":  # dict(kind="hdr", rgb=rows top-down)
+                rgb = np.ascontiguousarray(t["rgb"], np.float32)
+                self.n_texels += rgb.shape[0] * rgb.shape[1]
+                self._keep.append(rgb)
+                texs[i].kind = RUMI_TEXTURE_HDR
+                texs[i].height, texs[i].width = rgb.shape[0], rgb.shape[1]
+                texs[i].rgb = _ptr(rgb)
+                continue
             if isinstance(t, dict):  # dict(kind="checker", w=, h=, c1=(r, g, b), c2=(r, g, b))
                 texs[i].kind = RUMI_TEXTURE_CHECKER
                 texs[i].checker_w, texs[i].checker_h = t["w"], t["h"]

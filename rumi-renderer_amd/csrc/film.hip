@@ -7,7 +7,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "internal.h"
 
@@ -250,6 +252,108 @@ extern "C" int rumi_bmp_texels(const uint8_t* pixels, uint32_t width, uint32_t h
             const float x = (float)pixels[i * bytes_per_pixel + (2 - c)] * inv;
             rgb[3 * i + c] = gamma == 1.0f ? x : (float)std::pow((double)x, (double)gamma);
         }
+    return RUMI_OK;
+}
+
+namespace {
+
+// A line as fgets(buf, 256) reads it from a memory image: up to 255 bytes,
+// through the first '\n'.  Returns false at the end of the data.
+bool mem_line(const uint8_t* data, uint64_t n, uint64_t& at, char out[256]) {
+    if (at >= n) return false;
+    int k = 0;
+    while (k < 255 && at < n) {
+        const char c = (char)data[at++];
+        out[k++] = c;
+        if (c == '\n') break;
+    }
+    out[k] = 0;
+    return true;
+}
+
+// ScanLine (hdr.cpp:114-177): a new-style run-length scanline of `w` RGBE
+// pixels into four planes.  The width word is compared with char
+// arithmetic, as the reference does with a signed char (a low byte of 128 or
+// more never matches).
+bool scan_line(const uint8_t* data, uint64_t n, uint64_t& at, int w, uint8_t* planes) {
+    if (w < 8 || w > 0x7fff) return false;
+    if (at + 4 > n) return false;
+    const signed char h0 = (signed char)data[at], h1 = (signed char)data[at + 1];
+    const signed char h2 = (signed char)data[at + 2], h3 = (signed char)data[at + 3];
+    at += 4;
+    if (h0 != 2 || h1 != 2 || (h2 & 0x80)) return false;
+    if ((((int)h2 << 8) | (int)h3) != w) return false;
+    uint8_t* p = planes;
+    for (int c = 0; c < 4; ++c) {
+        uint8_t* end = planes + (size_t)(c + 1) * w;
+        while (p < end) {
+            if (at + 2 > n) return false;
+            const uint8_t a = data[at], b = data[at + 1];
+            at += 2;
+            if (a > 128) {
+                const int count = a - 128;
+                if (count > end - p) return false;
+                for (int k = 0; k < count; ++k) *p++ = b;
+            } else {
+                const int count = a;
+                if (count == 0 || count > end - p) return false;
+                *p++ = b;
+                if (count > 1) {
+                    if (at + (uint64_t)(count - 1) > n) return false;
+                    std::memcpy(p, data + at, (size_t)(count - 1));
+                    at += (uint64_t)(count - 1);
+                    p += count - 1;
+                }
+            }
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" int rumi_hdr_read(const uint8_t* file, uint64_t file_bytes, uint32_t* width, uint32_t* height,
+                             float* rgb, uint64_t rgb_capacity) {
+    clear_error();
+    if (!file) return fail(RUMI_ERR_INVALID, "rumi_hdr_read: null data");
+    // LoadHDR (hdr.cpp:4-112): the header lines, then the resolution line
+    char line[256];
+    uint64_t at = 0;
+    if (!mem_line(file, file_bytes, at, line) || std::strcmp(line, "#?RADIANCE\n") != 0)
+        return fail(RUMI_ERR_INVALID, "rumi_hdr_read: no #?RADIANCE line");
+    do {
+        if (!mem_line(file, file_bytes, at, line)) return fail(RUMI_ERR_INVALID, "rumi_hdr_read: header ends");
+    } while (line[0] == '#');
+    if (std::strcmp(line, "FORMAT=32-bit_rle_rgbe\n") != 0)
+        return fail(RUMI_ERR_INVALID, "rumi_hdr_read: not FORMAT=32-bit_rle_rgbe");
+    for (;;) {  // EXPOSURE, GAMMA and any other line until an empty one
+        if (!mem_line(file, file_bytes, at, line)) return fail(RUMI_ERR_INVALID, "rumi_hdr_read: header ends");
+        if (line[0] == '\n') break;
+    }
+    int h = 0, w = 0;
+    if (!mem_line(file, file_bytes, at, line) || std::sscanf(line, "-Y %d + X  %d", &h, &w) != 2 || h <= 0 ||
+        w <= 0)
+        return fail(RUMI_ERR_INVALID, "rumi_hdr_read: no -Y h +X w line");
+    if (width) *width = (uint32_t)w;
+    if (height) *height = (uint32_t)h;
+    if (!rgb) return RUMI_OK;
+    if (rgb_capacity < 3ull * (uint64_t)w * (uint64_t)h)
+        return fail(RUMI_ERR_INVALID, "rumi_hdr_read: rgb buffer too small");
+    std::vector<uint8_t> planes(4 * (size_t)w);
+    for (int y = 0; y < h; ++y) {
+        if (!scan_line(file, file_bytes, at, w, planes.data()))
+            return fail(RUMI_ERR_INVALID, "rumi_hdr_read: scanline %d is not one the reference reads", y);
+        for (int x = 0; x < w; ++x) {
+            const int e = planes[3 * (size_t)w + x];
+            float* o = rgb + 3 * ((size_t)y * w + x);
+            if (e) {
+                const float f = (float)std::ldexp(1.0, e - (128 + 8));
+                for (int c = 0; c < 3; ++c) o[c] = (float)planes[(size_t)c * w + x] * f;
+            } else {
+                o[0] = o[1] = o[2] = 0.0f;
+            }
+        }
+    }
     return RUMI_OK;
 }
 

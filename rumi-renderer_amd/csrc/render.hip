@@ -495,17 +495,53 @@ __device__ inline float shadow_tr(const DevMedium& m, V3 o, V3 d, float tmax, Rn
 struct LightSample {
     V3 p, n, le;
     float pdf;
+    float g;  // the light's G toward x (AreaLight::G; 1 for the image-based light)
 };
+
+__device__ inline V3 pattern(const SceneView& sv, const DevShape& s, V3 p, float bu, float bv, int& index);
+
+// AreaLight::G (arealight.cpp:106-112).
+__device__ inline float light_g(V3 x, const LightSample& ls, V3 wi) {
+    const float d = length(x - ls.p);
+    const float c = fmaxf(-dot(wi, ls.n), 0.0f);
+    return c / (d * d);
+}
 
 // Scene::VisibilityTest (scene.cpp:99-123) up to the shadow ray: uniform
 // light choice, a uniform point on the emitter (the reference draws from a
-// fixed stratified set), the one-sided test.  Returns false if unlit.
-__device__ inline bool sample_light(const SceneView& sv, V3 x, Rng& g, LightSample& ls, V3& wi,
+// fixed stratified set), the one-sided test.  Returns false if unlit.  n:
+// the shading normal, which the image-based light samples around.
+__device__ inline bool sample_light(const SceneView& sv, V3 x, V3 nrm, Rng& g, LightSample& ls, V3& wi,
                                     float& dist) {
     if (sv.n_lights == 0) return false;
     uint32_t li = (uint32_t)(g.uniform() * (float)sv.n_lights);
     if (li >= sv.n_lights) li = sv.n_lights - 1;
     const DevLight& L = sv.lights[li];
+    if (L.kind == kLightIbl) {
+        // ImageBasedLight::UniformSampling (imagebasedlight.cpp:35-64): a
+        // cosine-weighted direction about the normal, the sphere's far
+        // side along it, its texture there; pdf cos / pi, G = 1
+        V3 u, w;
+        basis(nrm, u, w);
+        const float phi = g.uniform() * 2.0f * kPiF;
+        const float theta = acosf(sqrtf(1.0f - g.uniform()));
+        const float st = sinf(theta);
+        wi = frame(u, nrm, w, st * sinf(phi), cosf(theta), st * cosf(phi));
+        Ray r{x.x, x.y, x.z, wi.x, wi.y, wi.z, FLT_MAX, 0.0f};
+        hit_sphere(make_float4(L.p[0], L.p[1], L.p[2], L.l[0]), r, false, true);
+        ls.p = x + wi * r.tmin;
+        ls.n = -wi;
+        int index;
+        ls.le = pattern(sv, sv.shapes[L.prim], ls.p, 0.0f, 0.0f, index);
+        float c = dot(wi, nrm);
+        if (c <= 0.0f) c = 1e-4f;
+        float pdf = kInvPiF * c;
+        if (pdf <= 0.0f) pdf = 1e-4f;
+        ls.pdf = 1.0f / (float)sv.n_lights * pdf;
+        ls.g = 1.0f;
+        dist = length(ls.p - x);
+        return !(dot(wi, ls.n) >= 0.0f);
+    }
     if (L.kind == RUMI_SHAPE_RECT) {
         const float a = g.uniform(), b = g.uniform();
         ls.p = (ld3(L.p) + ld3(L.l) * a) + ld3(L.w) * b;
@@ -520,14 +556,8 @@ __device__ inline bool sample_light(const SceneView& sv, V3 x, Rng& g, LightSamp
     ls.le = ld3(L.le);
     wi = normalize(ls.p - x);
     dist = length(ls.p - x);
+    ls.g = light_g(x, ls, wi);
     return !(dot(wi, ls.n) >= 0.0f);
-}
-
-// AreaLight::G (arealight.cpp:106-112).
-__device__ inline float light_g(V3 x, const LightSample& ls, V3 wi) {
-    const float d = length(x - ls.p);
-    const float c = fmaxf(-dot(wi, ls.n), 0.0f);
-    return c / (d * d);
 }
 
 // ------------------------------------------------------------ textures
@@ -540,6 +570,15 @@ __device__ inline V3 texel(const SceneView& sv, int tex, float u, float v, int& 
         const bool second = (a == 0 && b == 0) || (a != 0 && b != 0);
         return second ? v3(t.c2[0], t.c2[1], t.c2[2]) : v3(t.c1[0], t.c1[1], t.c1[2]);
     }
+    if (t.kind == RUMI_TEXTURE_HDR) {
+        // HDRTexture::Pixel (hdrtexture.cpp:29-41): rows top-down as loaded, u mirrored
+        const int row = (int)((1.0f - v) * (float)(t.height - 1));
+        const int col = (int)((1.0f - u) * (float)(t.width - 1));
+        const int y = min(max(row, 0), t.height - 1), x = min(max(col, 0), t.width - 1);
+        index = (int)(t.offset / 3u) + y * t.width + x;
+        const float* p = sv.texels + t.offset + 3 * ((size_t)y * t.width + x);
+        return v3(p[0], p[1], p[2]);
+    }
     // ImageTexture::Pixel (imagetexture.cpp:32-61): nearest texel
     const int row = (int)((1.0f - v) * (float)(t.height - 1));
     const int col = (int)(u * (float)(t.width - 1));
@@ -551,7 +590,8 @@ __device__ inline V3 texel(const SceneView& sv, int tex, float u, float v, int& 
 }
 
 // Shape::TextureMapping overrides: Rect (rect.cpp:183-197), Sphere
-// (sphere.cpp:172-195, identity transform); triangles interpolate vertex uv.
+// (sphere.cpp:172-195: angles against the rows uu, vv, ww of the shape's
+// transform, kept in n0..n2); triangles interpolate vertex uv.
 __device__ inline V3 pattern(const SceneView& sv, const DevShape& s, V3 p, float bu, float bv,
                              int& index) {
     const int tex = s.info[2];
@@ -564,9 +604,10 @@ __device__ inline V3 pattern(const SceneView& sv, const DevShape& s, V3 p, float
         v = w0 * s.n1[3] + bu * s.uv2[0] + bv * s.uv2[2];
     } else if (s.info[0] == RUMI_SHAPE_SPHERE) {
         const V3 pc = normalize(p - v3(s.g0[0], s.g0[1], s.g0[2]));
-        const float theta = acosf(pc.y);
+        const V3 uu = ld3(s.n0), vv = ld3(s.n1), ww = ld3(s.n2);
+        const float theta = acosf(dot(pc, vv));
         const V3 pxz = pc * sinf(theta);
-        const float phi = atan2f(pxz.x, pxz.z);
+        const float phi = atan2f(dot(pxz, uu), dot(pxz, ww));
         u = phi >= 0.0f ? phi / (2.0f * kPiF) : (phi + 2.0f * kPiF) / (2.0f * kPiF);
         v = 1.0f - theta / kPiF;
     } else {
@@ -797,10 +838,10 @@ __device__ inline bool matte_direct(const SceneView& sv, const DevMaterial& m, V
                                     V3& lo, V3& wi, float& dist, float& smax_s) {
     LightSample ls;
     smax_s = 0.0f;
-    if (!sample_light(sv, p, g, ls, wi, dist)) return false;
+    if (!sample_light(sv, p, n, g, ls, wi, dist)) return false;
     const V3 f = ld3(m.color) * (kInvPiF * m.kd);
     const float c = fmaxf(dot(wi, n), 0.0f);
-    lo = cmul(f, ls.le) * (c * light_g(p, ls, wi) / ls.pdf);
+    lo = cmul(f, ls.le) * (c * ls.g / ls.pdf);
     if (sv.med.bound) lo = lo * shadow_tr(sv.med, p, wi, dist, g, smax_s);
     return true;
 }
@@ -870,13 +911,13 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
                 V3 wi;
                 float dist;
                 Rng gn = nee_stream(g, nee_key);
-                if (sample_light(sv, p, gn, ls, wi, dist)) {
+                if (sample_light(sv, p, v3(0.0f, 1.0f, 0.0f), gn, ls, wi, dist)) {  // no image-based light with media
                     float smax_s;
                     const float tr = shadow_tr(md, p, wi, dist, gn, smax_s);
                     const float cl = dot(d, wi);
                     const float fp = md.g == 0.0f ? 1.0f / (4.0f * kPiF) : hg_eval(md.g, cl);
                     const V3 c = cmul(cmul(beta, ls.le), ld3(md.sigma_s)) *
-                                 (tr * fp * light_g(p, ls, wi) / ls.pdf);
+                                 (tr * fp * ls.g / ls.pdf);
                     if (tr > 0.0f) push_shadow(out, p, wi, dist, c, pixel, smax_s, dlog_hg(md.g, cl));
                 }
                 if (last) return;
@@ -2195,6 +2236,9 @@ int check_render_args(rumi_ctx* ctx, const rumi_scene* sc, const rumi_render_des
         return fail(RUMI_ERR_UNSUPPORTED, "%s: unknown integrator %d", fn, desc->integrator);
     if (desc->width <= 0 || desc->height <= 0 || desc->spp <= 0 || desc->max_depth <= 0)
         return fail(RUMI_ERR_INVALID, "%s: width, height, spp and max_depth must be > 0", fn);
+    if (sc->has_ibl && (desc->integrator == RUMI_INTEGRATOR_BDPT || sc->medium.bound))
+        return fail(RUMI_ERR_UNSUPPORTED, "%s: the image-based light works with the path tracer and the "
+                    "RayTracer, without a medium", fn);
     if (desc->max_depth > 1000) return fail(RUMI_ERR_INVALID, "%s: max_depth > 1000", fn);
     if ((uint64_t)desc->width * desc->height > 0xffffffffull / 4)
         return fail(RUMI_ERR_INVALID, "%s: image too large", fn);
@@ -2695,6 +2739,10 @@ extern "C" int rumi_batch_create(rumi_ctx* ctx, const rumi_batch_item* items, ui
         if (it.width <= 0 || it.height <= 0) {
             delete b;
             return fail(RUMI_ERR_INVALID, "rumi_batch_create: item %u has an empty image", i);
+        }
+        if (sc->has_ibl && sc->medium.bound) {
+            delete b;
+            return fail(RUMI_ERR_UNSUPPORTED, "rumi_batch_create: item %u: image-based light with a medium", i);
         }
         SceneTab& t = tab[i];
         t.sv = scene_view(sc);

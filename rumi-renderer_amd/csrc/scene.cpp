@@ -143,6 +143,13 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
             p.kind = RUMI_SHAPE_SPHERE;
             std::memcpy(p.v, s.v, sizeof(float) * 3);
             p.radius = s.radius;
+            // texture basis: the rows of Shape::_transform's 3x3 (zero: identity)
+            std::memcpy(p.n, s.n, sizeof(p.n));
+            {
+                bool zero = true;
+                for (int k = 0; k < 9; ++k) zero = zero && p.n[k] == 0.0f;
+                if (zero) p.n[0] = p.n[4] = p.n[8] = 1.0f;
+            }
             prims.push_back(p);
             break;
         case RUMI_SHAPE_RECT:
@@ -287,8 +294,12 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
         std::memcpy(rec + 4, s.g1, 16);
         std::memcpy(rec + 8, s.g2, 16);
         const int32_t tag = s.info[0] == RUMI_SHAPE_TRIANGLE ? 0 : (s.info[0] == RUMI_SHAPE_SPHERE ? 1 : 2);
-        const int32_t flags =
-            (tag == 2 && mats[s.info[1]].kind == RUMI_MAT_EMISSIVE) ? 1 : 0;
+        // rect: emissive (no back-face cull); sphere: emissive and textured (the
+        // image-based-light root swap of Sphere::Intersect, sphere.cpp:85-92)
+        const int32_t flags = ((tag == 2 && mats[s.info[1]].kind == RUMI_MAT_EMISSIVE) ||
+                               (tag == 1 && mats[s.info[1]].kind == RUMI_MAT_EMISSIVE && s.info[2] >= 0))
+                                  ? 1
+                                  : 0;
         if (tag == 2) std::memcpy(&rec[8], &s.info[3], 4);
         std::memcpy(&rec[9], &i, 4);
         std::memcpy(&rec[10], &tag, 4);
@@ -298,6 +309,7 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
 
     // ------------------------------------------------------------ lights
     std::vector<DevLight> lights;
+    bool sc_has_ibl = false;
     for (uint32_t i = 0; i < n; ++i) {
         const Prim& p = prims[i];
         if (!d->shapes[p.shape].is_light) continue;
@@ -320,6 +332,14 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
             L.n[1] = r[4];
             L.n[2] = r[5];
             L.p[3] = 1.0f / (r[14] * r[15]);  // 1 / area (Rect::Pdf)
+        } else if (p.kind == RUMI_SHAPE_SPHERE && d->shapes[p.shape].is_light == 2) {
+            // ImageBasedLight(sphere, emissive, texture) (imagebasedlight.cpp:24-33)
+            if (p.texture < 0)
+                return fail(RUMI_ERR_INVALID, "shape %d: an image-based light needs a texture", p.shape);
+            L.kind = kLightIbl;
+            for (int k = 0; k < 3; ++k) L.p[k] = p.v[k];
+            L.l[0] = p.radius;
+            sc_has_ibl = true;
         } else if (p.kind == RUMI_SHAPE_SPHERE) {
             L.kind = RUMI_SHAPE_SPHERE;
             for (int k = 0; k < 3; ++k) L.p[k] = p.v[k];
@@ -356,7 +376,8 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
             texs[i].offset = (uint32_t)texels.size();
             continue;
         }
-        if (t.kind != RUMI_TEXTURE_IMAGE) return fail(RUMI_ERR_INVALID, "texture %u: unknown kind %d", i, t.kind);
+        if (t.kind != RUMI_TEXTURE_IMAGE && t.kind != RUMI_TEXTURE_HDR)
+            return fail(RUMI_ERR_INVALID, "texture %u: unknown kind %d", i, t.kind);
         if (t.width <= 0 || t.height <= 0 || !t.rgb)
             return fail(RUMI_ERR_INVALID, "texture %u: bad size or data", i);
         texs[i].width = t.width;
@@ -375,6 +396,7 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
     std::memcpy(sc->root_lo, flat.root_lo, sizeof(sc->root_lo));
     std::memcpy(sc->root_hi, flat.root_hi, sizeof(sc->root_hi));
     sc->material_kinds = kinds;
+    sc->has_ibl = sc_has_ibl;
     sc->leaf_prims = flat.leaf_prims;
 
     // ------------------------------------------------------------ medium + camera

@@ -35,6 +35,8 @@ struct alignas(16) DevShape {
 };
 
 // Area light, 96 B: rect (p, l, w, n) or sphere (centre, radius).
+constexpr int32_t kLightIbl = 16;  // DevLight::kind of the image-based light (a sphere)
+
 struct alignas(16) DevLight {
     int32_t kind, prim, pad0, pad1;
     float p[4];   // rect p / sphere centre + radius in w
@@ -80,6 +82,7 @@ struct rumi_scene {
     rumi::DevMedium medium{};
     rumi::DevCamera camera{};
     uint32_t material_kinds = 0;  // bit mask of RUMI_MAT_* present
+    bool has_ibl = false;         // an image-based light (path tracer and RayTracer only, no medium)
     // device arrays
     void* nodes = nullptr;      // DevNode4[]: the scene tree in its 4-wide form
     void* prims = nullptr;      // leaf-ordered 48-B records

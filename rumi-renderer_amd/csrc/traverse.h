@@ -125,8 +125,9 @@ __device__ __forceinline__ bool hit_triangle(const float4& ta, const float4& tb,
 }
 
 // Sphere::Intersect: a = centre.xyz, radius.  A = 1 (unit direction assumed),
-// far root first inside an object.
-__device__ __forceinline__ bool hit_sphere(const float4& a, Ray& r, bool in_object) {
+// far root first inside an object, and swapped (again) for an emissive
+// textured sphere seen from inside (ibl: the image-based light's sphere).
+__device__ __forceinline__ bool hit_sphere(const float4& a, Ray& r, bool in_object, bool ibl = false) {
     const float ocx = r.ox - a.x, ocy = r.oy - a.y, ocz = r.oz - a.z;
     const float A = 1.0f;
     const float B = 2.0f * (r.dx * ocx + r.dy * ocy + r.dz * ocz);
@@ -144,6 +145,11 @@ __device__ __forceinline__ bool hit_sphere(const float4& a, Ray& r, bool in_obje
     }
     if (t0 < 0.0f && t1 < 0.0f) return false;
     if (in_object) {
+        const float tmp = t0;
+        t0 = t1;
+        t1 = tmp;
+    }
+    if (ibl && sqrtf(ocx * ocx + ocy * ocy + ocz * ocz) < a.w) {
         const float tmp = t0;
         t0 = t1;
         t1 = tmp;
@@ -321,7 +327,7 @@ __device__ __forceinline__ bool trav_step_near(const float4* __restrict__ nodes,
             if (tag == kTagTri)
                 ok = hit_triangle(ta, tb, tc, r, u, v);
             else if (tag == kTagSphere)
-                ok = hit_sphere(ta, r, in_object);
+                ok = hit_sphere(ta, r, in_object, __float_as_int(tc.w) != 0);
             else
                 ok = hit_rect(rects + 4 * (size_t)__float_as_int(tc.x), __float_as_int(tc.w) != 0, r);
         }
@@ -420,7 +426,7 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
             if (tag == kTagTri)
                 ok = hit_triangle(ta, tb, tc, r, u, v);
             else if (tag == kTagSphere)
-                ok = hit_sphere(ta, r, in_object);
+                ok = hit_sphere(ta, r, in_object, __float_as_int(tc.w) != 0);
             else
                 ok = hit_rect(rects + 4 * (size_t)__float_as_int(tc.x), __float_as_int(tc.w) != 0, r);
         }
@@ -581,7 +587,7 @@ __device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes,
                 if (tag == kTagTri)
                     ok = hit_triangle(ta, tb, tc, r, u, v);
                 else if (tag == kTagSphere)
-                    ok = hit_sphere(ta, r, in_object);
+                    ok = hit_sphere(ta, r, in_object, __float_as_int(tc.w) != 0);
                 else
                     ok = hit_rect(rects + 4 * (size_t)__float_as_int(tc.x), __float_as_int(tc.w) != 0,
                                   r);

@@ -171,3 +171,33 @@ def checker_light_scene(u, v, w=12.0, h=12.0):
     pos = (u + 1e-4, 1.0, -v)
     cam = dict(position=pos, basis=look_at_basis(pos, target), fov_deg=0.01, y_ratio=1.0)
     return dict(materials=mats, shapes=shapes, camera=cam, medium=None, textures=tex)
+
+
+def ibl_scene(width=16, height=16, seed=9, constant=None):
+    """The reference's image-based-lighting scene in small (src/Scene/Build
+    Image Based Lighting.txt): an emissive sphere of radius 15 at (0, 5, 0)
+    with RotationY(-pi) as its transform and an HDR texture (random 16 x 8,
+    or one constant value), lighting a matte floor rect, a matte sphere and a
+    glass sphere through the ImageBasedLight; the camera looks down at them
+    from inside the sphere."""
+    rng = np.random.default_rng(seed)
+    if constant is None:
+        tex = rng.uniform(0.2, 3.0, (8, 16, 3)).astype(np.float32)
+    else:
+        tex = np.tile(np.float32(constant), (8, 16, 1))
+    c, s = np.float32(np.cos(np.float32(-np.pi))), np.float32(np.sin(np.float32(-np.pi)))
+    basis = np.float32([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    mats = [dict(kind="matte", kd=0.6, color=(1.0, 1.0, 1.0)),
+            dict(kind="emissive", kd=1.0, color=(1.0, 1.0, 1.0)),
+            dict(kind="matte", kd=1.0, color=(0.7, 0.3, 0.2)),
+            dict(kind="dielectric", eta_in=1.5, eta_out=1.0, absorption=(0.1, 0.1, 0.1))]
+    shapes = [dict(kind="rect", p=(-2.0, 0.0, 2.0), l=(4.0, 0, 0), w=(0, 0, -4.0), material=0),
+              dict(kind="sphere", center=(0.0, 5.0, 0.0), radius=15.0, material=1, texture=0,
+                   basis=basis, light="ibl")]
+    if constant is None:
+        shapes += [dict(kind="sphere", center=(-0.6, 0.5, 0.0), radius=0.5, material=2),
+                   dict(kind="sphere", center=(0.7, 0.4, 0.3), radius=0.4, material=3)]
+    pos = (0.0, 3.0, 4.0)
+    cam = dict(position=pos, basis=look_at_basis(pos, (0, 0.3, 0)), fov_deg=30.0, y_ratio=height / width)
+    return dict(materials=mats, shapes=shapes, camera=cam, medium=None,
+                textures=[dict(kind="hdr", rgb=tex)])

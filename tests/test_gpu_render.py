@@ -120,6 +120,39 @@ def test_whitted_same_seed_matches_oracle(rumi, oracle, ctx, scenes, case):
     assert d_same < 1e-4 and d_other > 0.003, (d_same, d_other)
 
 
+@pytest.mark.parametrize("integrator", ["pt", "whitted"])
+def test_image_based_light_same_seed_matches_oracle(rumi, oracle, ctx, integrator):
+    """ImageBasedLight on an HDR-textured sphere with a rotated texture
+    frame: GPU and oracle trace the same samples for one seed."""
+    w = 32
+    sc = rc.ibl_scene(w, w)
+    if integrator == "pt":
+        c = oracle.render_pt(sc, w, w, 32, 5, seed=5, nthreads=8)[0]
+        kw = {}
+    else:
+        c = oracle.render_whitted(sc, w, w, 32, 5, seed=5, nthreads=8)[0]
+        kw = dict(integrator=rumi.RUMI_INTEGRATOR_WHITTED)
+    same = render(rumi, ctx, sc, w, w, 32, 5, seed=5, **kw)[0]
+    other = render(rumi, ctx, sc, w, w, 32, 5, seed=6, **kw)[0]
+    d_same = np.abs(same - c).sum() / np.abs(c).sum()
+    d_other = np.abs(other - c).sum() / np.abs(c).sum()
+    print("ibl", integrator, "relative L1, same seed", d_same, "other seed", d_other)
+    assert d_same < 1e-4 and d_other > 0.003, (d_same, d_other)
+
+
+def test_image_based_light_constant_sky_and_limits(rumi, ctx):
+    E = np.array([1.0, 2.0, 3.0])
+    sc = rc.ibl_scene(8, 8, constant=E)
+    sc["camera"] = dict(position=(0.0, 1.0, 0.3), basis=rc.look_at_basis((0, 1, 0.3), (0, 0, 0)),
+                        fov_deg=0.5, y_ratio=1.0)
+    img = render(rumi, ctx, sc, 2, 2, 64, 3, seed=2)[0]
+    np.testing.assert_allclose(img, np.broadcast_to(0.6 * E, img.shape), rtol=1e-5)
+    scene = rumi.Scene(ctx, rc.ibl_scene(8, 8))
+    with pytest.raises(rumi.RumiError):
+        scene.render(8, 8, 4, 4, integrator=rumi.RUMI_INTEGRATOR_BDPT)
+    scene.close()
+
+
 def test_model_scene_same_seed_matches_oracle(rumi, oracle, ctx, scenes, tmp_path):
     """The reference's model scene with its teapot (teapot.obj through
     rumi_obj_load, glass with absorption): GPU and oracle trace the same
