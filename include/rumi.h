@@ -244,6 +244,14 @@ typedef struct {
                            the camera looks along its +z (Pinhole, src/Camera/pinhole.cpp:30-51) */
     float fov_deg;      /* full field of view; the reference fixes 20 (pinhole.cpp:36) */
     float y_ratio;      /* image height / width */
+    /* Pinhole::DepthOfField(focal_distance, lens_radius) (pinhole.cpp:89-95,
+     * 111-122): with focal_distance > 0 each camera ray starts on a lens disk
+     * (r = sqrt(u) lens_radius, angle 2 pi v, along the camera's up and
+     * right axes) and aims at its pinhole ray's point on the focal plane.
+     * 0 = a pinhole.  The bidirectional tracer ignores the lens, as the
+     * reference's BidirectionalPathTracer resets it to a pinhole
+     * (bidirectionalpathtracer.cpp:56). */
+    float focal_distance, lens_radius;
 } rumi_camera;
 
 enum { RUMI_MEDIUM_NONE = 0, RUMI_MEDIUM_BOX = 1, RUMI_MEDIUM_SPHERE = 2 };

@@ -408,6 +408,8 @@ def scene_arrays(scene: dict):
     c[3:12] = np.asarray(cam["basis"], np.float32).reshape(9)
     c[12] = np.float32(np.tan(np.pi / np.float32(180.0) * np.float64(cam["fov_deg"]) * 0.5))
     c[13] = cam.get("y_ratio", 1.0)
+    c[14] = cam.get("focal_distance", 0.0)
+    c[15] = cam.get("lens_radius", 0.0)
     med = None
     if scene.get("medium"):
         md = scene["medium"]

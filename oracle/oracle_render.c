@@ -127,6 +127,7 @@ typedef struct {
     v3 p;
     double R[9];
     double tanfov, y_ratio;
+    double focal, lens; /* Pinhole::DepthOfField; focal 0 = pinhole */
 } camera_t;
 
 typedef struct {
@@ -1247,8 +1248,9 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
     return L;
 }
 
-/* Pinhole::CastViewRay (src/Camera/pinhole.cpp:97-136), no lens. */
-static ray_t camera_ray(const camera_t* c, int W, int H, int x, int y, double sx, double sy)
+/* Pinhole::CastViewRay (src/Camera/pinhole.cpp:97-136); with g and a focal
+ * distance > 0 the thin lens draws two more numbers after the pixel jitter. */
+static ray_t camera_ray(const camera_t* c, int W, int H, int x, int y, double sx, double sy, pcg_t* g)
 {
     double hw = W * 0.5, hh = H * 0.5;
     double px = -(((double)x - hw) + sx) / hw * c->tanfov;
@@ -1259,6 +1261,15 @@ static ray_t camera_ray(const camera_t* c, int W, int H, int x, int y, double sx
     r.d = mk(c->R[0] * l.x + c->R[1] * l.y + c->R[2] * l.z,
              c->R[3] * l.x + c->R[4] * l.y + c->R[5] * l.z,
              c->R[6] * l.x + c->R[7] * l.y + c->R[8] * l.z);
+    if (c->focal > 0.0 && g) {
+        /* thin lens (Pinhole::CastViewRay, pinhole.cpp:111-122) */
+        v3 fp = add(r.o, mul(r.d, c->focal / l.z));
+        double rr = sqrt(rnd(g)) * c->lens;
+        double th = rnd(g) * 2.0 * PI_D;
+        v3 up = mk(c->R[1], c->R[4], c->R[7]), right = mk(c->R[0], c->R[3], c->R[6]);
+        r.o = add(add(r.o, mul(mul(up, rr), sin(th))), mul(mul(right, rr), cos(th)));
+        r.d = norm(sub(fp, r.o));
+    }
     r.tmin = 0.0;
     r.tmax = 1e300;
     return r;
@@ -1345,6 +1356,8 @@ static void load_scene(scene_t* sc, const float* shapes, int n_shapes, const flo
     for (k = 0; k < 9; ++k) sc->cam.R[k] = cam[3 + k];
     sc->cam.tanfov = cam[12];
     sc->cam.y_ratio = cam[13];
+    sc->cam.focal = cam[14];
+    sc->cam.lens = cam[15];
     medium_t* md = &sc->med;
     md->kind = med ? (int)med[0] : 0;
     if (md->kind) {
@@ -1410,7 +1423,7 @@ static void* render_rows(void* arg)
                 pcg_seed(&g, j->seed + (uint64_t)s * 0x9E3779B97F4A7C15ull,
                          (uint64_t)y * (uint64_t)j->W + (uint64_t)x);
                 double sx = rnd(&g), sy = rnd(&g);
-                ray_t r = camera_ray(&j->sc->cam, j->W, j->H, x, y, sx, sy);
+                ray_t r = camera_ray(&j->sc->cam, j->W, j->H, x, y, sx, sy, &g);
                 col L = trace_pt(j->sc, r, j->max_depth, &g, P, Gp);
                 if (cnan(L)) continue; /* Scene::Rendering drops NaN samples */
                 sum = cadd(sum, L);
@@ -1918,7 +1931,7 @@ static void* bdpt_rows(void* arg)
                 pcg_seed(&gl, (j->seed ^ BDPT_LIGHT_MIX) + (uint64_t)smp * 0x9E3779B97F4A7C15ull, pixel);
                 /* eye subpath */
                 double sx = rnd(&g), sy = rnd(&g);
-                ray_t cr = camera_ray(cam, j->W, j->H, x, y, sx, sy);
+                ray_t cr = camera_ray(cam, j->W, j->H, x, y, sx, sy, 0);
                 double hw = j->W * 0.5, hh = j->H * 0.5;
                 double lpx = -(((double)x - hw) + sx) / hw * cam->tanfov;
                 double lpy = -(((double)y - hh) + sy) / hh * cam->tanfov * cam->y_ratio;

@@ -59,7 +59,7 @@ class Shape(Structure):
 
 class Camera(Structure):
     _fields_ = [("position", c_float * 3), ("basis", c_float * 9), ("fov_deg", c_float),
-                ("y_ratio", c_float)]
+                ("y_ratio", c_float), ("focal_distance", c_float), ("lens_radius", c_float)]
 
 
 class Medium(Structure):
@@ -529,6 +529,8 @@ class Scene:
         d.camera.basis = (c_float * 9)(*np.asarray(cam["basis"], np.float32).reshape(9).tolist())
         d.camera.fov_deg = cam["fov_deg"]
         d.camera.y_ratio = cam.get("y_ratio", 1.0)
+        d.camera.focal_distance = cam.get("focal_distance", 0.0)
+        d.camera.lens_radius = cam.get("lens_radius", 0.0)
         med = desc.get("medium")
         if med:
             d.medium.bound = {"box": 1, "sphere": 2}[med["bound"]]

@@ -284,10 +284,21 @@ __global__ __launch_bounds__(kBlock) void k_regen(PathBuf buf, uint32_t start, u
         const float px = -(((float)x - hw) + sx) / hw * c.tanfov;
         const float py = -(((float)y - hh) + sy) / hh * c.tanfov * c.y_ratio;
         const V3 l = normalize(v3(px, py, 1.0f));
-        const V3 d = v3(c.R[0] * l.x + c.R[1] * l.y + c.R[2] * l.z,
-                        c.R[3] * l.x + c.R[4] * l.y + c.R[5] * l.z,
-                        c.R[6] * l.x + c.R[7] * l.y + c.R[8] * l.z);
-        buf.ray[2 * (size_t)slot + 0] = make_float4(c.p[0], c.p[1], c.p[2], 0.0f);
+        V3 d = v3(c.R[0] * l.x + c.R[1] * l.y + c.R[2] * l.z,
+                  c.R[3] * l.x + c.R[4] * l.y + c.R[5] * l.z,
+                  c.R[6] * l.x + c.R[7] * l.y + c.R[8] * l.z);
+        V3 o = v3(c.p[0], c.p[1], c.p[2]);
+        if (c.focal > 0.0f) {
+            // the thin lens of Pinhole::CastViewRay (pinhole.cpp:111-122): a
+            // point on the lens disk aiming at the focal-plane point
+            const V3 fp = o + d * (c.focal / l.z);
+            const float rr = sqrtf(g.uniform()) * c.lens;
+            const float th = g.uniform() * 2.0f * kPiF;
+            const V3 up = v3(c.R[1], c.R[4], c.R[7]), right = v3(c.R[0], c.R[3], c.R[6]);
+            o = (o + (up * rr) * sinf(th)) + (right * rr) * cosf(th);
+            d = normalize(fp - o);
+        }
+        buf.ray[2 * (size_t)slot + 0] = make_float4(o.x, o.y, o.z, 0.0f);
         buf.ray[2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, FLT_MAX);
         buf.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         buf.rng[slot] = g.state;

@@ -438,6 +438,8 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
     // Pinhole ctor: _tanfov = tan(PI / 180.f * _fov * 0.5f) (pinhole.cpp:45)
     C.tanfov = (float)std::tan(kPi / 180.0f * (double)d->camera.fov_deg * 0.5f);
     C.y_ratio = d->camera.y_ratio;
+    C.focal = d->camera.focal_distance;
+    C.lens = d->camera.lens_radius;
 
     int rc = RUMI_OK;
     do {

@@ -59,6 +59,7 @@ struct DevCamera {
     float p[3];
     float R[9];
     float tanfov, y_ratio;
+    float focal, lens;  // Pinhole::DepthOfField; focal 0 = pinhole
 };
 
 struct DevTexture {

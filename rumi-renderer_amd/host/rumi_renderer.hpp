@@ -453,11 +453,19 @@ class Pinhole {  // src/Camera/pinhole.cpp:30-51
             for (int k = 0; k < 3; ++k) c.basis[3 * r + k] = _basis._m[4 * r + k];
         c.fov_deg = _fov;
         c.y_ratio = _y_ratio;
+        c.focal_distance = _focal_distance;
+        c.lens_radius = _lens_radius;
         return c;
+    }
+    // the thin lens (pinhole.cpp:89-95); focal_distance 0 = a pinhole
+    void DepthOfField(float focal_distance, float lens_radius) {
+        _focal_distance = focal_distance;
+        _lens_radius = lens_radius;
     }
     Point3 _pos;
     Mat4x4 _basis;
     float _y_ratio;
+    float _focal_distance = 0.0f, _lens_radius = 0.0f;
     float _fov = 20.0f;  // the reference's fixed field of view (pinhole.cpp:36); public, as there
 };
 

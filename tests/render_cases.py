@@ -201,3 +201,30 @@ def ibl_scene(width=16, height=16, seed=9, constant=None):
     cam = dict(position=pos, basis=look_at_basis(pos, (0, 0.3, 0)), fov_deg=30.0, y_ratio=height / width)
     return dict(materials=mats, shapes=shapes, camera=cam, medium=None,
                 textures=[dict(kind="hdr", rgb=tex)])
+
+
+def lens_edge_scene(depth, focal, radius, edge, axis="x", fov_deg=0.001, width_x=None):
+    """A thin-lens camera at the origin looking along +z (identity basis, so
+    its right axis is +x and its up axis +y) at a one-sided emitter in the
+    plane z = depth that covers the half-plane {axis >= edge} (or, with
+    width_x, the strip edge <= x <= edge + width_x)."""
+    mats = [dict(kind="emissive", kd=1.0, color=(1.0, 2.0, 3.0))]
+    big = 40.0
+    if axis == "x":
+        lo = (edge, -big / 2)
+        span = (width_x if width_x else big, big)
+    else:
+        lo = (-big / 2, edge)
+        span = (big, big)
+    # l along y and w along x: l x w points to -z, toward the camera
+    rect = dict(kind="rect", p=(lo[0], lo[1], depth), l=(0.0, span[1], 0.0), w=(span[0], 0.0, 0.0),
+                material=0, light=True)
+    cam = dict(position=(0.0, 0.0, 0.0), basis=np.eye(3).tolist(), fov_deg=fov_deg, y_ratio=1.0,
+               focal_distance=focal, lens_radius=radius)
+    return dict(materials=mats, shapes=[rect], camera=cam, medium=None)
+
+
+def disk_segment_fraction(t):
+    """Share of a unit disk, uniform in area, on one side of a chord at
+    distance t from its centre: (acos t - t sqrt(1 - t^2)) / pi."""
+    return (math.acos(t) - t * math.sqrt(1.0 - t * t)) / math.pi

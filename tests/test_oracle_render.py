@@ -97,3 +97,40 @@ def test_checker_texture_on_an_emitter(oracle, u, v, w, h):
     a, b = int(u * w) % 2, int(v * h) % 2
     c = np.ones(3) if a == b else np.array([0.25, 0.5, 0.75])
     np.testing.assert_allclose(mean[0, 0], np.array([1.0, 2.0, 3.0]) * c, rtol=1e-6)
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_thin_lens_blur_is_a_disk_segment(oracle, axis):
+    """Pinhole::DepthOfField out of focus (pinhole.cpp:111-122): with the
+    focal plane at F = 2 and an emitter edge at distance e = 0.2 off axis in
+    the plane z = D = 4, a lens ray from lens point (a, b) lands at
+    (-a, -b) (1 - D/F = -1), so the pixel sees the emitter for the share of
+    the lens disk (radius R, uniform in area) beyond the chord at e / R: a
+    disk segment.  A lens radius drawn linearly in u, or an angle over half
+    a turn, lands elsewhere."""
+    R, e = 0.5, 0.2
+    sc = rc.lens_edge_scene(4.0, 2.0, R, e, axis=axis)
+    n = 200000
+    mean, _ = oracle.render_pt(sc, 1, 1, n, 1, seed=3, nthreads=8)
+    f = rc.disk_segment_fraction(e / R)
+    sigma = np.sqrt(f * (1 - f) / n)
+    got = mean[0, 0] / np.array([1.0, 2.0, 3.0])
+    assert np.all(np.abs(got - f) < 4 * sigma), (got, f, sigma)
+    # in focus (F = D), every lens ray meets the pixel's focal-plane point
+    sc = rc.lens_edge_scene(4.0, 4.0, R, -0.01, axis=axis)
+    mean, _ = oracle.render_pt(sc, 1, 1, 4096, 1, seed=3, nthreads=8)
+    np.testing.assert_allclose(mean[0, 0], [1.0, 2.0, 3.0], rtol=1e-12)
+
+
+def test_thin_lens_focal_plane_is_a_plane(oracle):
+    """The focal point of an off-axis pixel sits on the plane z = F
+    (p + d F / d_z), not at distance F along d: a 60-degree view whose
+    leftmost pixel looks 30 degrees off axis keeps a sharp edge in focus.
+    Its pinhole rays land at x in [0.999, 1] * 4 tan 30 on the plane z = 4;
+    an emitter from x = 2.30 covers them all, and a focal point at distance
+    F along d would blur that pixel to about 0.58 of Le."""
+    sc = rc.lens_edge_scene(4.0, 4.0, 0.5, 2.30, axis="x", fov_deg=60.0)
+    sc["camera"]["y_ratio"] = 1e-3
+    mean, _ = oracle.render_pt(sc, 2001, 1, 64, 1, seed=4, nthreads=8)
+    np.testing.assert_allclose(mean[0, 0], [1.0, 2.0, 3.0], rtol=1e-12)
+    assert np.all(mean[0, -1] == 0)
