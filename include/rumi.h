@@ -281,12 +281,24 @@ typedef struct {
     float color_1[3], color_2[3];
 } rumi_texture;
 
+/* PointLight(position, ls, rgb) (src/Light/pointlight.cpp:16-72): a light
+ * with no shape (never hit, not in the scene tree).  Its light sample is the
+ * point itself: Le = rgb * ls, G = 1 / d^2, Pdf = 1, so next-event estimation
+ * from it is deterministic.  The path tracer and the RayTracer take it, with
+ * or without a medium; the bidirectional tracer rejects it. */
+typedef struct {
+    float position[3];
+    float ls;
+    float color[3];
+} rumi_point_light;
+
 typedef struct {
     const rumi_shape* shapes;       uint32_t n_shapes;
     const rumi_material* materials; uint32_t n_materials;
     const rumi_texture* textures;   uint32_t n_textures;
     rumi_camera camera;
     rumi_medium medium;
+    const rumi_point_light* point_lights; uint32_t n_point_lights;  /* after the shapes' lights */
 } rumi_scene_desc;
 
 /* Builds the scene's BVH (binned SAH over all primitives) and uploads it

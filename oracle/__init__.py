@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-OR_TRI, OR_SPHERE, OR_RECT = 0, 1, 2
+OR_TRI, OR_SPHERE, OR_RECT, OR_POINT = 0, 1, 2, 3
 
 
 class OrRay(Structure):
@@ -386,6 +386,17 @@ def scene_arrays(scene: dict):
             r[0, 3:6] = s["p"]
             r[0, 6:9] = s["l"]
             r[0, 9:12] = s["w"]
+        rows.append(r)
+    for q in scene.get("point_lights") or []:
+        # PointLight(position, ls, rgb): after every shape, as the C ABI
+        # appends point lights after the shapes' lights
+        r = np.zeros((1, 32), np.float32)
+        r[0, 0] = OR_POINT
+        r[0, 2] = 1.0
+        r[0, 22] = -1
+        r[0, 3:6] = q["position"]
+        r[0, 13:16] = (np.asarray(q.get("color", (1.0, 1.0, 1.0)), np.float32) *
+                       np.float32(q.get("ls", 1.0)))
         rows.append(r)
     shapes = np.ascontiguousarray(np.concatenate(rows, 0))
     mats = np.zeros((len(scene["materials"]), 32), np.float32)

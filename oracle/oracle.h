@@ -38,7 +38,7 @@ typedef struct {
 } or_ray;
 
 /* Primitive kinds of the shape table. */
-enum { OR_TRI = 0, OR_SPHERE = 1, OR_RECT = 2 };
+enum { OR_TRI = 0, OR_SPHERE = 1, OR_RECT = 2, OR_POINT = 3 };
 
 /* One scene primitive, restating the data members the reference's shapes
  * keep (src/Shape/triangle.h:11-17, sphere.h, rect.h). */
@@ -160,14 +160,17 @@ double or_time_median6_mesh(const float* xyz, const uint32_t* idx, int nt,
  *             [1] material, [2] area-light flag, [3..11] a,b,c (tri v0 v1 v2;
  *             sphere centre in a; rect p, l, w), [12] radius, [13..21] tri
  *             vertex normals n0 n1 n2, [22] texture index (-1 none),
- *             [23..28] tri vertex uv u0 v0 u1 v1 u2 v2.
+ *             [23..28] tri vertex uv u0 v0 u1 v1 u2 v2.  OR_POINT: a
+ *             PointLight, never hit: position in a, Le = rgb * ls in
+ *             [13..15], flag [2] = 1.
  *   mats      n_mats x 32 floats: [0] kind (0 matte, 1 glossy, 2 specular,
  *             3 emissive, 4 dielectric, 5 subsurface, 6 GGX), [1] kd
  *             (specular kr, emissive ls), [2] ks, [3] Phong exponent,
  *             [4..6] rgb, [7] eta_in, [8] eta_out, [9..11] absorption,
  *             [12..14] sigma_s, [15..17] sigma_a, [18] density, [19] GGX
  *             roughness.
- *   camera    16 floats: position, row-major 3x3 basis, tan(fov/2), y_ratio.
+ *   camera    16 floats: position, row-major 3x3 basis, tan(fov/2), y_ratio,
+ *             focal distance and lens radius (thin lens; focal 0 = pinhole).
  *   medium    16 floats or NULL: [0] 0 none / 1 box / 2 sphere, [1..3] lo or
  *             centre, [4..6] hi, [7] radius, [8..10] sigma_s, [11..13]
  *             sigma_a, [14] Henyey-Greenstein g (0 = the reference's isotropic).

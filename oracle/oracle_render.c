@@ -253,8 +253,9 @@ static int hit_prim(const scene_t* sc, int i, ray_t* ray, int in_object, hitrec_
         ok = hit_tri(s, ray, &u, &v, &det);
     else if (s->kind == OR_SPHERE)
         ok = hit_sphere(s, ray, in_object, sc->mats[s->mat].kind == MAT_EMISSIVE && s->tex >= 0);
-    else
+    else if (s->kind == OR_RECT)
         ok = hit_rect(s, &sc->mats[s->mat], ray);
+    /* OR_POINT: a PointLight is never hit */
     if (ok && h) {
         h->obj = i;
         h->u = u;
@@ -687,6 +688,15 @@ static void light_sample(const scene_t* sc, v3 x, v3 nrm, pcg_t* g, lsample_t* l
     const mat_t* m = &sc->mats[s->mat];
     double area;
     ls->ibl = 0;
+    if (s->kind == OR_POINT) {
+        /* PointLight (pointlight.cpp:44-72): N points from the light toward
+         * x, so the area-light G, max(-wi.n, 0) / d^2, is its 1 / d^2 */
+        ls->p = s->a;
+        ls->n = norm(sub(x, s->a));
+        ls->pdf = 1.0 / sc->n_lights; /* Pdf() = 1 */
+        ls->le = cmk(s->n0.x, s->n0.y, s->n0.z);
+        return;
+    }
     if (s->light == 2) {
         /* ImageBasedLight::UniformSampling (imagebasedlight.cpp:35-64) */
         v3 u, w;
@@ -1337,6 +1347,8 @@ static void load_scene(scene_t* sc, const float* shapes, int n_shapes, const flo
                 o->lo[k] = vget(o->a, k) - o->r;
                 o->hi[k] = vget(o->a, k) + o->r;
             }
+        } else if (o->kind == OR_POINT) {
+            for (k = 0; k < 3; ++k) o->lo[k] = o->hi[k] = vget(o->a, k);
         } else {
             o->n = norm(cross(o->b, o->c));
             o->length = len(o->b);

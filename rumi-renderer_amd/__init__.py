@@ -77,10 +77,15 @@ class Texture(Structure):
 RUMI_TEXTURE_IMAGE, RUMI_TEXTURE_CHECKER, RUMI_TEXTURE_HDR = 0, 1, 2
 
 
+class PointLight(Structure):
+    _fields_ = [("position", c_float * 3), ("ls", c_float), ("color", c_float * 3)]
+
+
 class SceneDesc(Structure):
     _fields_ = [("shapes", c_void_p), ("n_shapes", c_uint32), ("materials", c_void_p),
                 ("n_materials", c_uint32), ("textures", c_void_p), ("n_textures", c_uint32),
-                ("camera", Camera), ("medium", Medium)]
+                ("camera", Camera), ("medium", Medium), ("point_lights", c_void_p),
+                ("n_point_lights", c_uint32)]
 
 
 class RenderDesc(Structure):
@@ -531,6 +536,13 @@ class Scene:
         d.camera.y_ratio = cam.get("y_ratio", 1.0)
         d.camera.focal_distance = cam.get("focal_distance", 0.0)
         d.camera.lens_radius = cam.get("lens_radius", 0.0)
+        pls = desc.get("point_lights") or []
+        pl = (PointLight * max(1, len(pls)))()
+        for i, q in enumerate(pls):
+            pl[i].position = _f3(q["position"])
+            pl[i].ls = q.get("ls", 1.0)
+            pl[i].color = _f3(q.get("color", (1.0, 1.0, 1.0)))
+        d.point_lights, d.n_point_lights = ctypes.addressof(pl), len(pls)
         med = desc.get("medium")
         if med:
             d.medium.bound = {"box": 1, "sphere": 2}[med["bound"]]

@@ -528,6 +528,18 @@ __device__ inline bool sample_light(const SceneView& sv, V3 x, V3 nrm, Rng& g, L
     uint32_t li = (uint32_t)(g.uniform() * (float)sv.n_lights);
     if (li >= sv.n_lights) li = sv.n_lights - 1;
     const DevLight& L = sv.lights[li];
+    if (L.kind == kLightPoint) {
+        // PointLight::P / Wi / N / G / Pdf (pointlight.cpp:44-72): the point
+        // itself, N from the light toward x, G = 1 / d^2, Pdf 1
+        ls.p = ld3(L.p);
+        wi = normalize(ls.p - x);
+        ls.n = normalize(x - ls.p);
+        ls.le = ld3(L.le);
+        ls.pdf = 1.0f / (float)sv.n_lights;
+        dist = length(ls.p - x);
+        ls.g = 1.0f / (dist * dist);
+        return !(dot(wi, ls.n) >= 0.0f);
+    }
     if (L.kind == kLightIbl) {
         // ImageBasedLight::UniformSampling (imagebasedlight.cpp:35-64): a
         // cosine-weighted direction about the normal, the sphere's far
@@ -2247,6 +2259,8 @@ int check_render_args(rumi_ctx* ctx, const rumi_scene* sc, const rumi_render_des
         return fail(RUMI_ERR_UNSUPPORTED, "%s: unknown integrator %d", fn, desc->integrator);
     if (desc->width <= 0 || desc->height <= 0 || desc->spp <= 0 || desc->max_depth <= 0)
         return fail(RUMI_ERR_INVALID, "%s: width, height, spp and max_depth must be > 0", fn);
+    if (sc->has_point && desc->integrator == RUMI_INTEGRATOR_BDPT)
+        return fail(RUMI_ERR_UNSUPPORTED, "%s: the bidirectional tracer takes area lights, not point lights", fn);
     if (sc->has_ibl && (desc->integrator == RUMI_INTEGRATOR_BDPT || sc->medium.bound))
         return fail(RUMI_ERR_UNSUPPORTED, "%s: the image-based light works with the path tracer and the "
                     "RayTracer, without a medium", fn);

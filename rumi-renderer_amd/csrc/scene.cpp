@@ -350,6 +350,22 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
         }
         lights.push_back(L);
     }
+    if (d->n_point_lights && !d->point_lights)
+        return fail(RUMI_ERR_INVALID, "rumi_scene_create: null point_lights with n_point_lights > 0");
+    for (uint32_t i = 0; i < d->n_point_lights; ++i) {
+        // PointLight(position, ls, rgb) (pointlight.cpp:16-21): L = cl * ls
+        const rumi_point_light& q = d->point_lights[i];
+        DevLight L;
+        std::memset(&L, 0, sizeof(L));
+        L.kind = kLightPoint;
+        L.prim = -1;
+        for (int k = 0; k < 3; ++k) {
+            L.p[k] = q.position[k];
+            L.le[k] = q.color[k] * q.ls;
+        }
+        L.p[3] = 1.0f;  // PointLight::Pdf
+        lights.push_back(L);
+    }
 
     // light index of each emitter primitive (word uv2[3]; -1 = not a light)
     for (uint32_t i = 0; i < n; ++i) {
@@ -357,6 +373,7 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
         std::memcpy(&shapes[i].uv2[3], &none, 4);
     }
     for (size_t li = 0; li < lights.size(); ++li) {
+        if (lights[li].prim < 0) continue;
         const int32_t idx = (int32_t)li;
         std::memcpy(&shapes[lights[li].prim].uv2[3], &idx, 4);
     }
@@ -397,6 +414,7 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
     std::memcpy(sc->root_hi, flat.root_hi, sizeof(sc->root_hi));
     sc->material_kinds = kinds;
     sc->has_ibl = sc_has_ibl;
+    sc->has_point = d->n_point_lights > 0;
     sc->leaf_prims = flat.leaf_prims;
 
     // ------------------------------------------------------------ medium + camera

@@ -35,7 +35,8 @@ struct alignas(16) DevShape {
 };
 
 // Area light, 96 B: rect (p, l, w, n) or sphere (centre, radius).
-constexpr int32_t kLightIbl = 16;  // DevLight::kind of the image-based light (a sphere)
+constexpr int32_t kLightIbl = 16;    // DevLight::kind of the image-based light (a sphere)
+constexpr int32_t kLightPoint = 32;  // DevLight::kind of a PointLight: p, le; prim -1
 
 struct alignas(16) DevLight {
     int32_t kind, prim, pad0, pad1;
@@ -83,6 +84,7 @@ struct rumi_scene {
     rumi::DevMedium medium{};
     rumi::DevCamera camera{};
     uint32_t material_kinds = 0;  // bit mask of RUMI_MAT_* present
+    bool has_point = false;       // a PointLight (not for the bidirectional tracer)
     bool has_ibl = false;         // an image-based light (path tracer and RayTracer only, no medium)
     // device arrays
     void* nodes = nullptr;      // DevNode4[]: the scene tree in its 4-wide form

@@ -406,7 +406,12 @@ class Model : public Shape {
 };
 
 // ------------------------------------------------------------------- lights
-class AreaLight {  // src/Light/arealight.cpp: borrows its shape and material
+class Light {  // src/Light/light.h
+  public:
+    virtual ~Light() = default;
+};
+
+class AreaLight : public Light {  // src/Light/arealight.cpp: borrows its shape and material
   public:
     AreaLight(Shape* shape, Material* material) : _shape(shape) {
         shape->_is_light = true;
@@ -414,7 +419,16 @@ class AreaLight {  // src/Light/arealight.cpp: borrows its shape and material
     }
     Shape* _shape;
 };
-using Light = AreaLight;
+
+class PointLight : public Light {  // src/Light/pointlight.cpp:16-21; no shape
+  public:
+    PointLight(const Point3& position, float ls, const Color& rgb) {
+        desc.position[0] = position._x, desc.position[1] = position._y, desc.position[2] = position._z;
+        desc.ls = ls;
+        desc.color[0] = rgb._r, desc.color[1] = rgb._g, desc.color[2] = rgb._b;
+    }
+    rumi_point_light desc{};
+};
 
 // -------------------------------------------------------------------- media
 class HomogeneousVolume {  // src/Volume/homogeneousvolume.cpp:27-38
@@ -666,7 +680,12 @@ class Scene {
             }
             _shapes.push_back(s);
         }
+        _points.clear();
+        for (Light* l : _lights)
+            if (const PointLight* p = dynamic_cast<const PointLight*>(l)) _points.push_back(p->desc);
         _desc = rumi_scene_desc{};
+        _desc.point_lights = _points.empty() ? nullptr : _points.data();
+        _desc.n_point_lights = (uint32_t)_points.size();
         _desc.shapes = _shapes.data();
         _desc.n_shapes = (uint32_t)_shapes.size();
         _desc.materials = _mats.data();
@@ -732,6 +751,7 @@ class Scene {
     Pinhole* _camera = nullptr;
     std::vector<Shape*> _objects;
     std::vector<Light*> _lights;
+    std::vector<rumi_point_light> _points;
     HomogeneousVolume* _volume_ptr = nullptr;
     Integrator* _tracer_ptr = nullptr;
 

@@ -228,3 +228,21 @@ def disk_segment_fraction(t):
     """Share of a unit disk, uniform in area, on one side of a chord at
     distance t from its centre: (acos t - t sqrt(1 - t^2)) / pi."""
     return (math.acos(t) - t * math.sqrt(1.0 - t * t)) / math.pi
+
+
+def point_light_scene(rho=0.5, ls=2.0, color=(1.0, 2.0, 3.0), position=(0.3, 1.0, -0.2)):
+    """A Lambertian floor lit by one PointLight; the single pixel sees the
+    origin through a 0.001 degree field of view."""
+    mats = [dict(kind="matte", kd=1.0, color=(rho, rho, rho))]
+    pos = (0.0, 0.5, 0.3)
+    cam = dict(position=pos, basis=look_at_basis(pos, (0, 0, 0)), fov_deg=0.001, y_ratio=1.0)
+    return dict(materials=mats, shapes=_floor(0), camera=cam, medium=None,
+                point_lights=[dict(position=position, ls=ls, color=color)])
+
+
+def point_light_radiance(rho, ls, color, position):
+    """Matte next-event estimation from a PointLight at the origin:
+    (rho / pi) (color ls) cos(theta) / d^2, cos(theta) = y / d (floor normal +y)."""
+    p = np.asarray(position, np.float64)
+    d2 = float(p @ p)
+    return rho / math.pi * np.asarray(color, np.float64) * ls * (p[1] / math.sqrt(d2)) / d2

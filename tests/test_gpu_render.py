@@ -395,3 +395,37 @@ def test_thin_lens_same_seed_matches_oracle(rumi, oracle, ctx, scenes):
     # light-path splats land through float atomics, so repeat renders agree
     # to rounding, not bit for bit
     np.testing.assert_allclose(lens_bdpt, pin_bdpt, rtol=1e-5, atol=1e-9)
+
+
+def test_point_light_is_deterministic_inverse_square(rumi, ctx):
+    for pos, rho in (((0.3, 1.0, -0.2), 0.5), ((0.0, 2.0, 0.0), 0.8)):
+        sc = rc.point_light_scene(rho=rho, position=pos)
+        img = render(rumi, ctx, sc, 1, 1, 64, 1, seed=2)[0]
+        want = rc.point_light_radiance(rho, 2.0, (1.0, 2.0, 3.0), pos)
+        np.testing.assert_allclose(img[0, 0], want, rtol=1e-5)
+    scene = rumi.Scene(ctx, rc.point_light_scene())
+    with pytest.raises(rumi.RumiError):
+        scene.render(4, 4, 4, 4, integrator=rumi.RUMI_INTEGRATOR_BDPT)
+    scene.close()
+
+
+@pytest.mark.parametrize("case", ["pt", "whitted", "media"])
+def test_point_light_same_seed_matches_oracle(rumi, oracle, ctx, scenes, case):
+    """The Cornell box with a PointLight beside its area light (uniform
+    choice between the two): GPU and oracle trace the same samples for one
+    seed, with the path tracer, the RayTracer and through the medium."""
+    w = 32
+    sc = scenes.cornell_media(w, w) if case == "media" else scenes.cornell_box(w, w)
+    sc["point_lights"] = [dict(position=(0.3, 0.8, 0.2), ls=0.6, color=(1.0, 0.8, 0.5))]
+    kw = {}
+    if case == "whitted":
+        c = oracle.render_whitted(sc, w, w, 32, 5, seed=5, nthreads=8)[0]
+        kw = dict(integrator=rumi.RUMI_INTEGRATOR_WHITTED)
+    else:
+        c = oracle.render_pt(sc, w, w, 32, 5, seed=5, nthreads=8)[0]
+    same = render(rumi, ctx, sc, w, w, 32, 5, seed=5, **kw)[0]
+    other = render(rumi, ctx, sc, w, w, 32, 5, seed=6, **kw)[0]
+    d_same = np.abs(same - c).sum() / np.abs(c).sum()
+    d_other = np.abs(other - c).sum() / np.abs(c).sum()
+    print("point light", case, "relative L1, same seed", d_same, "other seed", d_other)
+    assert d_same < 1e-4 and d_other > 0.003, (d_same, d_other)

@@ -134,3 +134,22 @@ def test_thin_lens_focal_plane_is_a_plane(oracle):
     mean, _ = oracle.render_pt(sc, 2001, 1, 64, 1, seed=4, nthreads=8)
     np.testing.assert_allclose(mean[0, 0], [1.0, 2.0, 3.0], rtol=1e-12)
     assert np.all(mean[0, -1] == 0)
+
+
+def test_point_light_is_deterministic_inverse_square(oracle):
+    """PointLight (pointlight.cpp:44-72): G = 1 / d^2 and Pdf 1, so one
+    light sample gives (rho / pi) Le cos / d^2 exactly; moving the light
+    follows the closed form, and a blocker between the floor and the light
+    leaves the pixel black."""
+    for pos, rho in (((0.3, 1.0, -0.2), 0.5), ((0.0, 2.0, 0.0), 0.8)):
+        sc = rc.point_light_scene(rho=rho, position=pos)
+        mean, _ = oracle.render_pt(sc, 1, 1, 16, 1, seed=2, nthreads=1)
+        want = rc.point_light_radiance(rho, 2.0, (1.0, 2.0, 3.0), pos)
+        np.testing.assert_allclose(mean[0, 0], want, rtol=1e-5)
+    sc = rc.point_light_scene(position=(0.0, 2.0, 0.0))
+    sc["materials"].append(dict(kind="matte", kd=1.0, color=(1.0, 1.0, 1.0)))
+    # facing the floor: Rect::Intersect culls a non-emissive rect's back
+    # face for 0 < d.n < 1 (rect.cpp:80-81)
+    sc["shapes"].append(dict(kind="rect", p=(-0.5, 1.0, -0.5), l=(1.0, 0, 0), w=(0, 0, 1.0), material=1))
+    mean, _ = oracle.render_pt(sc, 1, 1, 16, 1, seed=2, nthreads=1)
+    assert np.all(mean == 0)
