@@ -518,10 +518,42 @@ __device__ inline float light_g(V3 x, const LightSample& ls, V3 wi) {
     return c / (d * d);
 }
 
+// ImageBasedLight::UniformSampling (imagebasedlight.cpp:35-64): a
+// cosine-weighted direction about the normal, the sphere's far side along
+// it, its texture there; pdf cos / pi, G = 1.  Compiled only into the
+// shading kernels of scenes that hold one (the IBL template flag): its
+// sphere test and texture lookup would raise every scene's shading kernel
+// from about 90 VGPRs to 150 (5 waves per SIMD to 3).
+__device__ inline bool sample_ibl(const SceneView& sv, const DevLight& L, V3 x, V3 nrm, Rng& g,
+                                  LightSample& ls, V3& wi, float& dist) {
+    V3 u, w;
+    basis(nrm, u, w);
+    const float phi = g.uniform() * 2.0f * kPiF;
+    const float theta = acosf(sqrtf(1.0f - g.uniform()));
+    const float st = sinf(theta);
+    wi = frame(u, nrm, w, st * sinf(phi), cosf(theta), st * cosf(phi));
+    Ray r{x.x, x.y, x.z, wi.x, wi.y, wi.z, FLT_MAX, 0.0f};
+    hit_sphere(make_float4(L.p[0], L.p[1], L.p[2], L.l[0]), r, false, true);
+    ls.p = x + wi * r.tmin;
+    ls.n = -wi;
+    int index;
+    ls.le = pattern(sv, sv.shapes[L.prim], ls.p, 0.0f, 0.0f, index);
+    float c = dot(wi, nrm);
+    if (c <= 0.0f) c = 1e-4f;
+    float pdf = kInvPiF * c;
+    if (pdf <= 0.0f) pdf = 1e-4f;
+    ls.pdf = 1.0f / (float)sv.n_lights * pdf;
+    ls.g = 1.0f;
+    dist = length(ls.p - x);
+    return !(dot(wi, ls.n) >= 0.0f);
+}
+
 // Scene::VisibilityTest (scene.cpp:99-123) up to the shadow ray: uniform
 // light choice, a uniform point on the emitter (the reference draws from a
 // fixed stratified set), the one-sided test.  Returns false if unlit.  n:
-// the shading normal, which the image-based light samples around.
+// the shading normal, which the image-based light samples around.  IBL:
+// the scene may hold an image-based light.
+template <bool IBL>
 __device__ inline bool sample_light(const SceneView& sv, V3 x, V3 nrm, Rng& g, LightSample& ls, V3& wi,
                                     float& dist) {
     if (sv.n_lights == 0) return false;
@@ -540,31 +572,7 @@ __device__ inline bool sample_light(const SceneView& sv, V3 x, V3 nrm, Rng& g, L
         ls.g = 1.0f / (dist * dist);
         return !(dot(wi, ls.n) >= 0.0f);
     }
-    if (L.kind == kLightIbl) {
-        // ImageBasedLight::UniformSampling (imagebasedlight.cpp:35-64): a
-        // cosine-weighted direction about the normal, the sphere's far
-        // side along it, its texture there; pdf cos / pi, G = 1
-        V3 u, w;
-        basis(nrm, u, w);
-        const float phi = g.uniform() * 2.0f * kPiF;
-        const float theta = acosf(sqrtf(1.0f - g.uniform()));
-        const float st = sinf(theta);
-        wi = frame(u, nrm, w, st * sinf(phi), cosf(theta), st * cosf(phi));
-        Ray r{x.x, x.y, x.z, wi.x, wi.y, wi.z, FLT_MAX, 0.0f};
-        hit_sphere(make_float4(L.p[0], L.p[1], L.p[2], L.l[0]), r, false, true);
-        ls.p = x + wi * r.tmin;
-        ls.n = -wi;
-        int index;
-        ls.le = pattern(sv, sv.shapes[L.prim], ls.p, 0.0f, 0.0f, index);
-        float c = dot(wi, nrm);
-        if (c <= 0.0f) c = 1e-4f;
-        float pdf = kInvPiF * c;
-        if (pdf <= 0.0f) pdf = 1e-4f;
-        ls.pdf = 1.0f / (float)sv.n_lights * pdf;
-        ls.g = 1.0f;
-        dist = length(ls.p - x);
-        return !(dot(wi, ls.n) >= 0.0f);
-    }
+    if (IBL && L.kind == kLightIbl) return sample_ibl(sv, L, x, nrm, g, ls, wi, dist);
     if (L.kind == RUMI_SHAPE_RECT) {
         const float a = g.uniform(), b = g.uniform();
         ls.p = (ld3(L.p) + ld3(L.l) * a) + ld3(L.w) * b;
@@ -857,11 +865,12 @@ __device__ inline uint32_t mk_flags(bool le, bool in_obj, uint32_t depth) {
 // Matte::DirectLighting (matte.cpp:54-82) without the occlusion test, which
 // the shadow stage performs.  smax_s: the shadow segment's length inside
 // the medium when its Bernoulli transmittance survived (0 if none).
+template <bool IBL>
 __device__ inline bool matte_direct(const SceneView& sv, const DevMaterial& m, V3 p, V3 n, Rng& g,
                                     V3& lo, V3& wi, float& dist, float& smax_s) {
     LightSample ls;
     smax_s = 0.0f;
-    if (!sample_light(sv, p, n, g, ls, wi, dist)) return false;
+    if (!sample_light<IBL>(sv, p, n, g, ls, wi, dist)) return false;
     const V3 f = ld3(m.color) * (kInvPiF * m.kd);
     const float c = fmaxf(dot(wi, n), 0.0f);
     lo = cmul(f, ls.le) * (c * ls.g / ls.pdf);
@@ -869,7 +878,7 @@ __device__ inline bool matte_direct(const SceneView& sv, const DevMaterial& m, V
     return true;
 }
 
-template <int MODE>
+template <int MODE, bool IBL>
 // pixel: the film pixel (the batch's, for batches); tex_base: the scene's
 // first texel in the batch's texel-gradient pool.
 __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32_t flags, V3 beta,
@@ -934,7 +943,7 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
                 V3 wi;
                 float dist;
                 Rng gn = nee_stream(g, nee_key);
-                if (sample_light(sv, p, v3(0.0f, 1.0f, 0.0f), gn, ls, wi, dist)) {  // no image-based light with media
+                if (sample_light<false>(sv, p, v3(0.0f, 1.0f, 0.0f), gn, ls, wi, dist)) {  // no image-based light with media
                     float smax_s;
                     const float tr = shadow_tr(md, p, wi, dist, gn, smax_s);
                     const float cl = dot(d, wi);
@@ -1018,7 +1027,7 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
             V3 lo, wi;
             float dist, smax_s;
             Rng gn = nee_stream(g, nee_key);
-            if (matte_direct(sv, m, p, n, gn, lo, wi, dist, smax_s)) {
+            if (matte_direct<IBL>(sv, m, p, n, gn, lo, wi, dist, smax_s)) {
                 if (m.kind == RUMI_MAT_GLOSSY) lo = lo * (1.0f / m.kd_pdf);
                 const V3 tex = s.info[2] >= 0 ? texture_at(p) : v3(1, 1, 1);
                 const V3 c = cmul(cmul(beta, tex), lo);
@@ -1135,7 +1144,7 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
         V3 lo, wi;
         float dist, smax_s;
         Rng gn = nee_stream(g, nee_key);
-        if (matte_direct(sv, m, p, n, gn, lo, wi, dist, smax_s)) {
+        if (matte_direct<IBL>(sv, m, p, n, gn, lo, wi, dist, smax_s)) {
             if (m.kind == RUMI_MAT_GLOSSY) lo = lo * (1.0f / m.kd_pdf);
             const V3 c = cmul(cmul(beta, tex), lo);
             if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f)
@@ -1249,7 +1258,7 @@ __device__ inline uint32_t block_append(uint32_t* counter, bool pred, uint32_t* 
 // orders them by material key (kind 0..6, 7 = miss) with a counting sort in
 // LDS, so the waves of a tile mostly run one BSDF; continuing paths go to the
 // other path buffer and shadow rays to the shadow queue.
-template <int MODE, bool MULTI>
+template <int MODE, bool MULTI, bool IBL>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(PathBuf in, PathBuf out, const uint32_t* n_ptr,
                                                        const int2* hits, const float2* uvs,
                                                        SceneView sv, const SceneTab* tab,
@@ -1321,7 +1330,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(PathBuf in, PathBuf out, 
             const float2 uv = uvs[i];
             const uint32_t fpix = MULTI ? tab[scene].pixel_base + pixel : pixel;
             const uint32_t tbase = MULTI ? tab[scene].texel_base : 0u;
-            shade<MODE>(view_of<MULTI>(sv, tab, scene), prm, v3(r0.x, r0.y, r0.z),
+            shade<MODE, IBL>(view_of<MULTI>(sv, tab, scene), prm, v3(r0.x, r0.y, r0.z),
                         v3(r1.x, r1.y, r1.z), in.flags[i], v3(bt.x, bt.y, bt.z), g, fpix, sample,
                         tbase, MODE == kReplay ? in.pend[i] : 0.0f, h.x, __int_as_float(h.y),
                         uv.x, uv.y, film, adj, G, so);
@@ -2328,6 +2337,7 @@ struct Pass {
     AdjCtx adj;
     Track tr;
     bool spill;
+    bool ibl;             // some scene holds an image-based light
     int mode;
     uint32_t grid_cap;    // most blocks a grid-stride stage is given
 };
@@ -2351,9 +2361,14 @@ void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t 
                            &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs);
     // one 256-path tile per block; blocks past the queue's size return at once
     const uint32_t gs = (uint32_t)((bound + kShadeBlock - 1) / kShadeBlock);
-    hipLaunchKernelGGL((k_shade<MODE, MULTI>), dim3(gs < 1 ? 1 : gs), dim3(kShadeBlock), 0, s, w.pb[cur],
-                       w.pb[cur ^ 1], &ctl->queue[cur], w.hits, w.uvs, ps.sv, ps.tab, ps.prm, ps.film,
-                       w.sh, &ctl->queue[cur ^ 1], &ctl->shadow, ps.adj, ps.tr);
+    if (ps.ibl)
+        hipLaunchKernelGGL((k_shade<MODE, MULTI, true>), dim3(gs < 1 ? 1 : gs), dim3(kShadeBlock), 0, s,
+                           w.pb[cur], w.pb[cur ^ 1], &ctl->queue[cur], w.hits, w.uvs, ps.sv, ps.tab, ps.prm,
+                           ps.film, w.sh, &ctl->queue[cur ^ 1], &ctl->shadow, ps.adj, ps.tr);
+    else
+        hipLaunchKernelGGL((k_shade<MODE, MULTI, false>), dim3(gs < 1 ? 1 : gs), dim3(kShadeBlock), 0, s,
+                           w.pb[cur], w.pb[cur ^ 1], &ctl->queue[cur], w.hits, w.uvs, ps.sv, ps.tab, ps.prm,
+                           ps.film, w.sh, &ctl->queue[cur ^ 1], &ctl->shadow, ps.adj, ps.tr);
     if (ps.spill)
         hipLaunchKernelGGL((k_shadow<true, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh, &ctl->shadow,
                            &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj);
@@ -2424,6 +2439,7 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     ps.adj = adj;
     ps.tr = Track{nullptr, nullptr};
     ps.spill = sc->stack_need > kStack;
+    ps.ibl = sc->has_ibl;
     ps.mode = mode;
     ps.grid_cap = (uint32_t)ctx->cu_count * 16u;
 
@@ -2545,6 +2561,7 @@ struct rumi_batch {
     uint32_t cap = 0, generations = 0;
     uint64_t total_samples = 0, total_pixels = 0, total_texels = 0;
     bool spill = false;
+    bool ibl = false;  // some scene holds an image-based light
     std::vector<uint32_t> item_samples, item_pixels;
     // device
     void* mem = nullptr;
@@ -2592,6 +2609,7 @@ void batch_enqueue(rumi_batch* b, const rumi_batch_io& io, hipStream_t s) {
     ps.prm.pixels = 0;
     ps.prm.n_scenes = n;
     ps.spill = b->spill;
+    ps.ibl = b->ibl;
     ps.grid_cap = (uint32_t)b->cu_count * 16u;
     ps.adj = AdjCtx{};
     ps.adj.gstripes = b->stripes;
@@ -2769,6 +2787,7 @@ extern "C" int rumi_batch_create(rumi_ctx* ctx, const rumi_batch_item* items, ui
             delete b;
             return fail(RUMI_ERR_UNSUPPORTED, "rumi_batch_create: item %u: image-based light with a medium", i);
         }
+        b->ibl = b->ibl || sc->has_ibl;
         SceneTab& t = tab[i];
         t.sv = scene_view(sc);
         t.width = it.width;
