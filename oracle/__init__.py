@@ -88,6 +88,9 @@ def lib() -> ctypes.CDLL:
         L.or_bmp_texels.argtypes = [vp, c_int, c_int, c_int, ctypes.c_float, vp]
         L.or_bmp_texels.restype = None
         L.or_hdr_read.argtypes = [vp, ctypes.c_size_t, POINTER(c_int), POINTER(c_int), vp]
+        L.or_probe.argtypes = [c_int, vp, vp, c_int, ctypes.c_uint64, vp]
+        L.or_plant.argtypes = [c_int]
+        L.or_plant.restype = None
         _lib = L
     return _lib
 
@@ -556,3 +559,42 @@ def render_bdpt(scene: dict, width: int, height: int, spp: int, max_len: int, n_
     if rc != 0:
         raise RuntimeError(f"or_render_bdpt failed ({rc})")
     return mean
+
+
+# ---------------------------------------------------------------- probes
+PROBE = {"lambert": 0, "phong": 1, "phase": 2, "ggx": 3, "fresnel": 4, "transmit": 5,
+         "reflect": 6, "distance": 7}
+PLANT = {"none": 0, "eta_swap": 1, "no_eta2": 2, "phong_norm": 3, "hg_sign": 4,
+         "grey_weight": 5, "no_roulette": 6, "no_beer": 7, "lambert_pdf": 8,
+         "ggx_jacobian": 9, "shadow_tr": 10, "snell": 11, "density": 12}
+
+
+def probe(kind: str, par, n: int = 0, seed: int = 1, inp=None) -> np.ndarray:
+    """Runs one of the oracle's samplers or evaluators (or_probe in
+    oracle.h) n times, or once per row of inp (rows of 6 doubles); returns an
+    (n, 8) array of its outputs."""
+    par = np.ascontiguousarray(np.asarray(par, np.float64).ravel())
+    if inp is not None:
+        inp = np.ascontiguousarray(np.asarray(inp, np.float64).reshape(-1, 6))
+        n = inp.shape[0]
+    out = np.zeros((n, 8), np.float64)
+    rc = lib().or_probe(PROBE[kind], _p(par), _p(inp), n, seed, _p(out))
+    if rc != 0:
+        raise RuntimeError(f"or_probe({kind}) failed")
+    return out
+
+
+class planted:
+    """Context manager: runs its body with one of the oracle's planted
+    mistakes switched on (or_plant in oracle.h), then switches it off."""
+
+    def __init__(self, name: str):
+        self.id = PLANT[name]
+
+    def __enter__(self):
+        lib().or_plant(self.id)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_plant(0)
+        return False

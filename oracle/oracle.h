@@ -242,6 +242,51 @@ void or_fireflies(const unsigned char* src, int width, int height, int bpp, unsi
 /* Inverse (src/Kernel/matrix.cpp:67-133), row-major 4x4. */
 void or_mat4_inverse(const float* a, float* out);
 
+/* ------------------------------------------------------------------ */
+/* Probes of oracle_render.c's samplers and evaluators, for the tests that
+ * pin them against closed forms.  n samples (or n inputs of 6 doubles in
+ * `in`), 8 doubles out per sample; PCG32 stream (seed, 1).
+ *   LAMBERT   par n[3]: Lambertian::BSDFSampling (bsdf.cpp:43-69) ->
+ *             wi[3], cos, pdf
+ *   PHONG     par e, wo[3], n[3]: Phong::BSDFSampling (bsdf.cpp:127-170)
+ *             -> wi[3], cos (1), pdf, Phong::F(n, wi, wo) (bsdf.cpp:91-104)
+ *   PHASE     par g, d[3]: Isotropic::PhaseSampling (phase.cpp:40-63) at
+ *             g = 0, Henyey-Greenstein otherwise -> wi[3], d.wi, pdf
+ *   GGX       par alpha, wo[3], n[3]; in: wi[3] or NULL (sampled) -> wi[3],
+ *             n.wi, pdf, f (F0 = 1)
+ *   FRESNEL   par eta_in, eta_out; in: direction[3], normal[3]:
+ *             FresnelReflection::Fresnel (bsdf.cpp:255-283) -> kr
+ *   TRANSMIT  par eta_in, eta_out; in: wo[3], n[3]:
+ *             FresnelTransparency::BSDFSampling (bsdf.cpp:322-342) -> w[3], cos
+ *   REFLECT   in: wo[3], n[3]: FresnelReflection::BSDFSampling
+ *             (bsdf.cpp:232-253) -> w[3], cos
+ *   DISTANCE  par sigma_t[3], avg sigma_t, absorb, smax:
+ *             HomogeneousVolume::DistanceSampling (homogeneousvolume.cpp:
+ *             168-215) -> scattered, s, absorbed, tau/pdf[3] */
+enum { OR_PROBE_LAMBERT = 0, OR_PROBE_PHONG = 1, OR_PROBE_PHASE = 2, OR_PROBE_GGX = 3,
+       OR_PROBE_FRESNEL = 4, OR_PROBE_TRANSMIT = 5, OR_PROBE_REFLECT = 6, OR_PROBE_DISTANCE = 7 };
+int or_probe(int kind, const double* par, const double* in, int n, uint64_t seed, double* out);
+
+/* Planted mistakes, for the tests that show each pin fails when the
+ * restated step is wrong; 0 (none) in every ordinary use.  Global to the
+ * library: set it only around single-threaded test code. */
+enum {
+    OR_PLANT_NONE = 0,
+    OR_PLANT_ETA_SWAP = 1,      /* Fresnel with eta_in and eta_out exchanged */
+    OR_PLANT_NO_ETA2 = 2,       /* transmission without FresnelTransparency::F's 1/eta^2 */
+    OR_PLANT_PHONG_NORM = 3,    /* Phong normalised by (e + 2) / 2 pi */
+    OR_PLANT_HG_SIGN = 4,       /* Henyey-Greenstein cosine with the wrong sign */
+    OR_PLANT_GREY_WEIGHT = 5,   /* scattering weight tau / (tau avg sigma_t), not tau / (tau sigma_t) */
+    OR_PLANT_NO_ROULETTE = 6,   /* no absorption roulette at a scattering event */
+    OR_PLANT_NO_BEER = 7,       /* no Beer-Lambert factor inside a dielectric */
+    OR_PLANT_LAMBERT_PDF = 8,   /* Lambertian pdf cos / 2 pi */
+    OR_PLANT_GGX_JACOBIAN = 9,  /* GGX pdf without the half-vector Jacobian 1 / (4 wo.h) */
+    OR_PLANT_SHADOW_TR = 10,    /* shadow rays through the medium always survive */
+    OR_PLANT_SNELL = 11,        /* refraction with eta in place of 1 / eta */
+    OR_PLANT_DENSITY = 12       /* Subsurface density ignored by the sampling sigma */
+};
+void or_plant(int id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,11 @@ static void pcg_seed(pcg_t* g, uint64_t initstate, uint64_t initseq)
 
 static inline double rnd(pcg_t* g) { return (double)pcg_next(g) * (1.0 / 4294967296.0); }
 
+/* Planted mistakes (or_plant, oracle.h): 0 in every ordinary use.  A test
+ * sets one to show that a pin fails when the restated step is wrong. */
+static int g_plant = 0;
+void or_plant(int id) { g_plant = id; }
+
 /* ------------------------------------------------------------------ scene */
 enum { MAT_MATTE = 0, MAT_GLOSSY = 1, MAT_SPECULAR = 2, MAT_EMISSIVE = 3,
        MAT_DIELECTRIC = 4, MAT_SUBSURFACE = 5, MAT_GGX = 6 };
@@ -419,6 +424,7 @@ static v3 lambert_sample(v3 n, pcg_t* g, double* cost, double* pdf)
     *cost = dot(wi, n);
     if (*cost <= 0.0) *cost = 1e-4;
     *pdf = INV_PI_D * *cost;
+    if (g_plant == OR_PLANT_LAMBERT_PDF) *pdf *= 0.5;
     if (*pdf <= 0.0) *pdf = 1e-4;
     return wi;
 }
@@ -462,6 +468,7 @@ static v3 phong_sample(const mat_t* m, v3 wo, v3 n, pcg_t* g, double* cost, doub
 static double fresnel(v3 dir, v3 n, double eta_in, double eta_out)
 {
     double eta_i = eta_out, eta_t = eta_in, ci = -dot(dir, n);
+    if (g_plant == OR_PLANT_ETA_SWAP) eta_i = eta_in, eta_t = eta_out;
     if (ci < 0.0) {
         eta_i = eta_in;
         eta_t = eta_out;
@@ -502,7 +509,7 @@ static v3 fresnel_transmit(const mat_t* m, v3 wo, v3 n, double* cost)
         eta = m->eta_out / m->eta_in;
         ci = -ci;
     }
-    double inv = 1.0 / eta;
+    double inv = g_plant == OR_PLANT_SNELL ? eta : 1.0 / eta;
     double ct = sqrt(1.0 - (1.0 - ci * ci) * (inv * inv));
     return add(mul(neg(wo), inv), mul(n, ci * inv - ct));
 }
@@ -540,6 +547,7 @@ static double ggx_pdf(const mat_t* m, v3 n, v3 wi, v3 wo)
     if (ci <= 0.0 || co <= 0.0) return 0.0;
     v3 h = norm(add(wi, wo));
     double ch = dot(n, h);
+    if (g_plant == OR_PLANT_GGX_JACOBIAN) return ggx_d(m->alpha, ch) * ch;
     return ggx_d(m->alpha, ch) * ch / (4.0 * dot(wo, h));
 }
 
@@ -613,14 +621,14 @@ static int distance_sample(col sigma_t, double avg, double absorb, double smax, 
         *pdf = *tau;
         return 0;
     }
-    if (rnd(g) < absorb || ds < 1e-4) {
+    if ((g_plant == OR_PLANT_NO_ROULETTE ? rnd(g) < 0.0 : rnd(g) < absorb) || ds < 1e-4) {
         *s = -1.0;
         *absorbed = 1;
         return 0;
     }
     *s = ds;
     *tau = exp_tau(sigma_t, ds);
-    *pdf = cmul(*tau, sigma_t);
+    *pdf = g_plant == OR_PLANT_GREY_WEIGHT ? cscale(*tau, avg) : cmul(*tau, sigma_t);
     if (cis0(*pdf)) *pdf = cs(1e-4);
     return 1;
 }
@@ -646,6 +654,7 @@ static v3 phase_sample(double g, v3 d, pcg_t* gen, double* pdf)
         double x = rnd(gen);
         double sq = (1.0 - g * g) / (1.0 - g + 2.0 * g * x);
         ct = (1.0 + g * g - sq * sq) / (2.0 * g);
+        if (g_plant == OR_PLANT_HG_SIGN) ct = -ct;
         if (ct > 1.0) ct = 1.0;
         if (ct < -1.0) ct = -1.0;
         *pdf = hg_eval(g, ct);
@@ -746,7 +755,7 @@ static double shadow_tr(const scene_t* sc, const ray_t* shadow, pcg_t* g, double
     if (!med_intersect(md, shadow, &tn, &tf)) return 1.0;
     double smax = med_maxdist(shadow->tmax, tn, tf);
     double xi = rnd(g);
-    if (xi < 1.0 - exp(md->avg_sigma_t * -smax)) return 0.0;
+    if (g_plant != OR_PLANT_SHADOW_TR && xi < 1.0 - exp(md->avg_sigma_t * -smax)) return 0.0;
     *smax_out = smax; /* the segment survived a Bernoulli trial of this length */
     return 1.0;
 }
@@ -1214,10 +1223,11 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
                     fr = cs(cur->pdf_path);
                 } else {
                     double eta = rinfo_in ? m->eta_in / m->eta_out : m->eta_out / m->eta_in;
+                    if (g_plant == OR_PLANT_NO_ETA2) eta = 1.0;
                     fr = cs(1.0 / (eta * eta) * cur->pdf_path);
                 }
                 if (rinfo_in) {
-                    if (m->kind == MAT_DIELECTRIC)
+                    if (m->kind == MAT_DIELECTRIC && g_plant != OR_PLANT_NO_BEER)
                         fr = cmul(fr, exp_tau(m->absorption, cur->tmin));
                     else
                         fr = cmul(fr, cdiv(cur->vtau, cur->vpdf));
@@ -1311,13 +1321,13 @@ static void load_scene(scene_t* sc, const float* shapes, int n_shapes, const flo
         o->sigma_s = cmk(m[12], m[13], m[14]);
         o->sigma_a = cmk(m[15], m[16], m[17]);
         o->sigma_t = cadd(o->sigma_s, o->sigma_a);
-        double density = m[18] > 0 ? m[18] : 1.0;
+        double density = m[18] > 0 && g_plant != OR_PLANT_DENSITY ? m[18] : 1.0;
         o->avg_sigmat = (o->sigma_t.r + o->sigma_t.g + o->sigma_t.b) / 3.0 * density;
         if (o->kind == MAT_SUBSURFACE)
             o->absorb = (o->sigma_a.r / o->sigma_t.r + o->sigma_a.g / o->sigma_t.g +
                          o->sigma_a.b / o->sigma_t.b) / 3.0;
         o->kd_pdf = (o->kd + o->ks) > 0 ? o->kd / (o->kd + o->ks) : 0.0;
-        o->phong_norm = (o->e + 1.0) / (2.0 * PI_D);
+        o->phong_norm = (o->e + (g_plant == OR_PLANT_PHONG_NORM ? 2.0 : 1.0)) / (2.0 * PI_D);
         o->alpha = m[19];
         o->opaque = !(o->kind == MAT_DIELECTRIC || o->kind == MAT_SUBSURFACE);
     }
@@ -2174,5 +2184,92 @@ int or_render_bdpt(const float* shapes, int n_shapes, const float* mats, int n_m
     free(jobs);
     free(light_of);
     free_scene(&sc);
+    return 0;
+}
+
+/* ------------------------------------------------------------ probes */
+/* or_probe (oracle.h): the samplers and evaluators above, one call per
+ * sample, for the tests that pin them against closed-form densities. */
+static mat_t probe_mat(const double* par)
+{
+    mat_t m;
+    memset(&m, 0, sizeof(m));
+    m.kind = MAT_GLOSSY;
+    m.e = par[0];
+    m.phong_norm = (m.e + (g_plant == OR_PLANT_PHONG_NORM ? 2.0 : 1.0)) / (2.0 * PI_D);
+    m.alpha = par[0];
+    m.color = cs(1.0);
+    m.eta_in = par[0];
+    m.eta_out = par[1];
+    return m;
+}
+
+int or_probe(int kind, const double* par, const double* in, int n, uint64_t seed, double* out)
+{
+    pcg_t g;
+    int i;
+    if (n < 0 || !par || !out) return 1;
+    pcg_seed(&g, seed, 1u);
+    mat_t m = probe_mat(par);
+    for (i = 0; i < n; ++i) {
+        double* o = out + 8 * (size_t)i;
+        const double* x = in ? in + 6 * (size_t)i : 0;
+        memset(o, 0, 8 * sizeof(double));
+        switch (kind) {
+        case OR_PROBE_LAMBERT: {
+            double c, p;
+            v3 w = lambert_sample(mk(par[0], par[1], par[2]), &g, &c, &p);
+            o[0] = w.x, o[1] = w.y, o[2] = w.z, o[3] = c, o[4] = p;
+            break;
+        }
+        case OR_PROBE_PHONG: {
+            double c, p;
+            v3 wo = mk(par[1], par[2], par[3]), nn = mk(par[4], par[5], par[6]);
+            v3 w = phong_sample(&m, wo, nn, &g, &c, &p);
+            o[0] = w.x, o[1] = w.y, o[2] = w.z, o[3] = c, o[4] = p;
+            o[5] = phong_f(&m, nn, w, wo);
+            break;
+        }
+        case OR_PROBE_PHASE: {
+            double p;
+            v3 d = mk(par[1], par[2], par[3]);
+            v3 w = phase_sample(par[0], d, &g, &p);
+            o[0] = w.x, o[1] = w.y, o[2] = w.z, o[3] = dot(d, w), o[4] = p;
+            break;
+        }
+        case OR_PROBE_GGX: {
+            v3 wo = mk(par[1], par[2], par[3]), nn = mk(par[4], par[5], par[6]);
+            v3 w = x ? mk(x[0], x[1], x[2]) : ggx_sample(&m, nn, wo, &g);
+            o[0] = w.x, o[1] = w.y, o[2] = w.z, o[3] = dot(nn, w);
+            o[4] = ggx_pdf(&m, nn, w, wo);
+            o[5] = ggx_f(&m, nn, w, wo).r;
+            break;
+        }
+        case OR_PROBE_FRESNEL:
+            if (!x) return 1;
+            o[0] = fresnel(mk(x[0], x[1], x[2]), mk(x[3], x[4], x[5]), par[0], par[1]);
+            break;
+        case OR_PROBE_TRANSMIT:
+        case OR_PROBE_REFLECT: {
+            double c = 0.0;
+            if (!x) return 1;
+            v3 wo = mk(x[0], x[1], x[2]), nn = mk(x[3], x[4], x[5]);
+            v3 w = kind == OR_PROBE_TRANSMIT ? fresnel_transmit(&m, wo, nn, &c) : fresnel_reflect(wo, nn, &c);
+            o[0] = w.x, o[1] = w.y, o[2] = w.z, o[3] = c;
+            break;
+        }
+        case OR_PROBE_DISTANCE: {
+            col st = cmk(par[0], par[1], par[2]), tau, pdf;
+            double s = 0.0;
+            int ab = 0;
+            o[0] = distance_sample(st, par[3], par[4], par[5], &g, &s, &ab, &tau, &pdf);
+            o[1] = s, o[2] = ab;
+            if (!ab) o[3] = tau.r / pdf.r, o[4] = tau.g / pdf.g, o[5] = tau.b / pdf.b;
+            break;
+        }
+        default:
+            return 1;
+        }
+    }
     return 0;
 }
