@@ -351,6 +351,23 @@ typedef struct {
 int rumi_render(rumi_ctx* ctx, const rumi_scene* scene, const rumi_render_desc* desc,
                 float* dev_rgb, rumi_render_stats* stats, void* stream);
 
+/* rumi_render without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The path tracer and the RayTracer only (BDPT:
+ * RUMI_ERR_UNSUPPORTED).  The wavefront queues live in one statically sized
+ * device array per GPU (RUMI_NOALLOC_MAX_PATHS paths, about 61 MB, held
+ * until the process ends); desc->queue_capacity 0 means that maximum, and a
+ * larger one is rejected with RUMI_ERR_INVALID.  The image equals
+ * rumi_render's with the same seed and queue capacity (up to the order of
+ * float atomics).  Calls on one GPU share the array: calls that overlap on
+ * one GPU (other threads, other streams) race.  stats->milliseconds is host
+ * time around the call (no HIP events are created).  Synchronises the
+ * stream before returning. */
+#define RUMI_NOALLOC_MAX_PATHS (1u << 18)
+int rumi_render_noalloc(rumi_ctx* ctx, const rumi_scene* scene, const rumi_render_desc* desc,
+                        float* dev_rgb, rumi_render_stats* stats, void* stream);
+
 /* ---------------------------------------------------------------- adjoint */
 /* Device buffers that rumi_adjoint overwrites; any may be NULL. */
 typedef struct {

@@ -158,6 +158,8 @@ _SIGNATURES = {
     "rumi_scene_destroy": (c_int, [c_void_p]),
     "rumi_render": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
                             POINTER(RenderStats), c_void_p]),
+    "rumi_render_noalloc": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
+                                    POINTER(RenderStats), c_void_p]),
     "rumi_adjoint": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
                              POINTER(GradBuffers), POINTER(RenderStats), c_void_p]),
     "rumi_batch_create": (c_int, [c_void_p, POINTER(BatchItem), c_uint32, POINTER(BatchDesc),
@@ -571,6 +573,17 @@ class Scene:
         check(lib().rumi_render(self.ctx.handle, self.handle, byref(rd), _ptr(out), byref(st),
                                 _stream(stream)))
         return out, st
+
+    def render_noalloc(self, width, height, spp, max_depth, out, seed=0, queue_capacity=0,
+                       regen_threshold=0, integrator=RUMI_INTEGRATOR_PT, stream=None):
+        """rumi_render_noalloc into the caller's device tensor out
+        (height x width x 3); returns RenderStats."""
+        rd = RenderDesc(integrator, width, height, spp, max_depth, 0, 0, 1.0, seed, queue_capacity,
+                        regen_threshold)
+        st = RenderStats()
+        check(lib().rumi_render_noalloc(self.ctx.handle, self.handle, byref(rd), _ptr(out), byref(st),
+                                        _stream(stream)))
+        return st
 
     def adjoint(self, dloss_dimage, spp, max_depth, seed=0, textures=False, queue_capacity=0,
                 regen_threshold=0, stream=None):

@@ -21,7 +21,9 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "scene.h"
@@ -2401,8 +2403,13 @@ void regen(const Wave& w, int cur, uint32_t start, uint32_t count, uint64_t samp
 // MODE selects plain rendering, the adjoint's primal pass or its replay.
 // The stages read their sizes from the device control block; the host reads
 // one word per iteration to decide on regeneration and the end.
+// `fixed` (or nullptr): a caller-provided work buffer of fixed_bytes that the
+// queue must fit (the no-allocation entry point); the pass then allocates
+// nothing -- no work buffer and no HIP events (its time is taken on the host,
+// the call being synchronous).
 int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
-              rumi_render_stats* stats, hipStream_t stream, int mode, const AdjCtx& adj) {
+              rumi_render_stats* stats, hipStream_t stream, int mode, const AdjCtx& adj,
+              void* fixed = nullptr, size_t fixed_bytes = 0) {
     const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
     const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
     uint32_t cap = desc->queue_capacity ? desc->queue_capacity : (1u << 21);
@@ -2411,16 +2418,25 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     uint32_t regen_at = desc->regen_threshold ? desc->regen_threshold : cap;
 
     const size_t need = wave_bytes(cap);
-    if (sc->work_bytes < need) {
-        if (sc->work) (void)hipFree(sc->work);
-        sc->work = nullptr;
-        sc->work_bytes = 0;
-        if (hipMalloc(&sc->work, need) != hipSuccess)
-            return fail(RUMI_ERR_NO_MEMORY, "rumi_render: hipMalloc of %zu work bytes failed", need);
-        sc->work_bytes = need;
+    void* work = fixed;
+    size_t work_bytes = fixed_bytes;
+    if (!fixed) {
+        if (sc->work_bytes < need) {
+            if (sc->work) (void)hipFree(sc->work);
+            sc->work = nullptr;
+            sc->work_bytes = 0;
+            if (hipMalloc(&sc->work, need) != hipSuccess)
+                return fail(RUMI_ERR_NO_MEMORY, "rumi_render: hipMalloc of %zu work bytes failed", need);
+            sc->work_bytes = need;
+        }
+        work = sc->work;
+        work_bytes = sc->work_bytes;
+    } else if (need > fixed_bytes) {
+        return fail(RUMI_ERR_INVALID, "rumi_render: queue of %u paths needs %zu work bytes, %zu available",
+                    cap, need, fixed_bytes);
     }
     Wave w;
-    if (!wave_carve(w, sc->work, sc->work_bytes, cap))
+    if (!wave_carve(w, work, work_bytes, cap))
         return fail(RUMI_ERR_INVALID, "rumi_render: internal work-buffer layout overflow");
 
     Pass ps{};
@@ -2443,10 +2459,14 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     ps.mode = mode;
     ps.grid_cap = (uint32_t)ctx->cu_count * 16u;
 
-    hipEvent_t ev0, ev1;
-    (void)hipEventCreate(&ev0);
-    (void)hipEventCreate(&ev1);
-    (void)hipEventRecord(ev0, stream);
+    const bool events = fixed == nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    const auto host_t0 = std::chrono::steady_clock::now();
+    if (events) {
+        (void)hipEventCreate(&ev0);
+        (void)hipEventCreate(&ev1);
+        (void)hipEventRecord(ev0, stream);
+    }
     const size_t film_n = 3 * (size_t)pixels;
     if (mode != kReplay)
         hipLaunchKernelGGL(k_zero, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n);
@@ -2481,16 +2501,20 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
         if (mode != kReplay)
             hipLaunchKernelGGL(k_scale, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film,
                                film_n, 1.0f / (float)desc->spp);
-        (void)hipEventRecord(ev1, stream);
+        if (events) (void)hipEventRecord(ev1, stream);
         if (hipStreamSynchronize(stream) != hipSuccess)
             rc = fail(RUMI_ERR_HIP, "rumi_render: %s", hipGetErrorString(hipGetLastError()));
         else if (hipGetLastError() != hipSuccess)
             rc = fail(RUMI_ERR_HIP, "rumi_render: a kernel launch failed");
     }
     float ms = 0.0f;
-    if (rc == RUMI_OK) (void)hipEventElapsedTime(&ms, ev0, ev1);
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
+    if (events) {
+        if (rc == RUMI_OK) (void)hipEventElapsedTime(&ms, ev0, ev1);
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    } else {
+        ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - host_t0).count();
+    }
     if (stats) {
         stats->samples = total;
         stats->extension_rays = ext_rays;
@@ -2680,6 +2704,60 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
     if (desc->integrator == RUMI_INTEGRATOR_BDPT) return render_bdpt(ctx, sc, desc, film, stats, stream);
     AdjCtx none{};
     return render_pt(ctx, sc, desc, film, stats, stream, kRender, none);
+}
+
+// ------------------------------------------------------------ no allocation
+// The work buffer of rumi_render_noalloc: one statically sized array per
+// GPU, resident from the library's load until the process ends.  Its
+// device address comes from hipGetSymbolAddress once per device (the
+// array's name is only a host-side stand-in); every call on a device shares
+// the array, so calls that overlap on one device race.
+namespace {
+constexpr uint32_t kNoAllocPaths = RUMI_NOALLOC_MAX_PATHS;
+constexpr size_t kScratchBytes = kBytesPerPath * (size_t)kNoAllocPaths + sizeof(Ctl) + 256 * kCarved;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The calling thread's current device must be `device`.
+int scratch_for(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(RUMI_ERR_INVALID, "rumi_render_noalloc: device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(RUMI_ERR_HIP, "rumi_render_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return RUMI_OK;
+}
+}  // namespace
+
+extern "C" int rumi_render_noalloc(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_render_desc* desc,
+                                   float* film, rumi_render_stats* stats, void* stream_v) {
+    clear_error();
+    rumi_scene* sc = const_cast<rumi_scene*>(scene_c);
+    int rc = check_render_args(ctx, sc, desc, "rumi_render_noalloc");
+    if (rc != RUMI_OK) return rc;
+    if (!film) return fail(RUMI_ERR_INVALID, "rumi_render_noalloc: null film");
+    if (desc->integrator == RUMI_INTEGRATOR_BDPT)
+        return fail(RUMI_ERR_UNSUPPORTED,
+                    "rumi_render_noalloc: the path tracer and the RayTracer only, not the bidirectional tracer");
+    if (desc->queue_capacity > kNoAllocPaths)
+        return fail(RUMI_ERR_INVALID, "rumi_render_noalloc: queue_capacity %u exceeds the maximum of %u paths",
+                    desc->queue_capacity, kNoAllocPaths);
+    void* work = nullptr;
+    if ((rc = scratch_for(ctx->device, &work)) != RUMI_OK) return rc;
+    rumi_render_desc d = *desc;
+    if (d.queue_capacity == 0) d.queue_capacity = kNoAllocPaths;
+    AdjCtx none{};
+    return render_pt(ctx, sc, &d, film, stats, (hipStream_t)stream_v, kRender, none, work, kScratchBytes);
 }
 
 extern "C" int rumi_adjoint(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_render_desc* desc,
