@@ -158,17 +158,30 @@ def main():
     pt = None if args.no_render else render_legs(rumi, ctx, args, log)
 
     cpu = None
+    ref_check = None
     if args.cpu_rays > 0:
         import oracle
         k = min(args.cpu_rays, n)
         log(f"CPU baseline: reference median-6 build + Travel on {k} rays, 1 thread ...")
-        _, _, secs, build_s = oracle.time_median6_mesh(xyz, idx, rays_h[:k])
+        ref_p, ref_t, secs, build_s = oracle.time_median6_mesh(xyz, idx, rays_h[:k])
         cpu = {"value": k / secs / 1e6, "unit": "Mrays/s", "cores": 1, "kind": "port",
                "sample": f"first {k} rays of the same batch, same mesh; the oracle's C "
                          f"restatement of BVHAccel::BuildBVH (median-6) + Travel; "
                          f"traversal timed only ({secs:.2f}s; build {build_s:.2f}s untimed)",
                "host_cpu": _cpu_model(), "nproc": os.cpu_count()}
         log(f"CPU baseline: {cpu['value']:.3f} Mrays/s")
+        # the benchmarked kernel's hits on those rays against the reference's
+        # own tree and order: ids equal except at exact ties (1e-6 relative t)
+        rumi.trace_closest(ctx, bvh, rays[:k], hits[:k], None, 0, stream)
+        torch.cuda.synchronize()
+        gp, gt = rumi.split_hits(hits[:k].cpu().numpy())
+        t_same = gt.view(np.int32) == ref_t.view(np.int32)
+        diff = gp != ref_p
+        tie = np.abs(gt.astype(np.float64) - ref_t) <= 1e-6 * np.abs(ref_t.astype(np.float64))
+        ref_check = {"rays": k, "ids_equal": int((~diff).sum()), "ties": int((diff & tie).sum()),
+                     "mismatches": int((diff & ~tie).sum()), "t_bit_equal": int(t_same.sum()),
+                     "tree": "reference median-6 (BVHAccel::BuildBVH), reference order"}
+        log(f"reference-tree check: {ref_check}")
 
     traffic = measured_traffic("closest-hit", n)
 
@@ -198,6 +211,7 @@ def main():
             "tris_per_ray": round(tris_c, 3), "node_bytes": node_bc,
         },
         "tuning": tuning,
+        "reference_tree_check": ref_check,
         "cpu_baseline": cpu,
         "any_hit": {
             "value": round(mrays_a, 2), "unit": "Mrays/s", "ms_per_step": round(ms_a, 4),

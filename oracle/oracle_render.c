@@ -670,6 +670,7 @@ typedef struct {
     col le;
     int ibl;    /* image-based light: the direction is wi, G = 1 */
     v3 wi;
+    int tex;    /* image-based light: the texel le came from (pool index), else -1 */
 } lsample_t;
 
 static col pattern(const scene_t* sc, int obj, v3 p, double bu, double bv, int* index);
@@ -697,6 +698,7 @@ static void light_sample(const scene_t* sc, v3 x, v3 nrm, pcg_t* g, lsample_t* l
     const mat_t* m = &sc->mats[s->mat];
     double area;
     ls->ibl = 0;
+    ls->tex = -1;
     if (s->kind == OR_POINT) {
         /* PointLight (pointlight.cpp:44-72): N points from the light toward
          * x, so the area-light G, max(-wi.n, 0) / d^2, is its 1 / d^2 */
@@ -719,7 +721,9 @@ static void light_sample(const scene_t* sc, v3 x, v3 nrm, pcg_t* g, lsample_t* l
         hit_sphere(s, &r, 0, 1);
         ls->p = add(x, mul(wi, r.tmin));
         ls->n = neg(wi);
+        idx = -1;
         ls->le = pattern(sc, sc->lights[li], ls->p, 0.0, 0.0, &idx);
+        ls->tex = idx;
         c = dot(wi, nrm);
         if (c <= 0.0) c = 1e-4;
         ls->pdf = (1.0 / sc->n_lights) * (INV_PI_D * c > 0.0 ? INV_PI_D * c : 1e-4);
@@ -787,14 +791,18 @@ static double geom(v3 x, const lsample_t* ls, v3 wi)
 }
 
 /* Matte::DirectLighting (matte.cpp:54-82).  *smax_s: medium length of the
- * shadow segment when its Bernoulli transmittance survived (else 0). */
-static col matte_direct(const scene_t* sc, const mat_t* m, v3 p, v3 n, pcg_t* g, double* smax_s)
+ * shadow segment when its Bernoulli transmittance survived (else 0); *ltex:
+ * the image-based light's texel the estimate is linear in (else -1). */
+static col matte_direct(const scene_t* sc, const mat_t* m, v3 p, v3 n, pcg_t* g, double* smax_s,
+                        int* ltex)
 {
     lsample_t ls;
     v3 wi;
     ray_t sh;
     *smax_s = 0.0;
+    *ltex = -1;
     if (!visibility(sc, p, n, g, &ls, &wi, &sh)) return cs(0.0);
+    *ltex = ls.tex;
     col f = cscale(m->color, INV_PI_D * m->kd);
     double c = dot(wi, n);
     if (c < 0.0) c = 0.0;
@@ -965,7 +973,7 @@ static void surface(const scene_t* sc, const hitrec_t* h, const ray_t* r, v3* p,
 typedef struct {
     col dL[7];
     int n_tex;
-    int* tix;   /* max_depth entries */
+    int* tix;   /* 2 max_depth entries */
     col* tval;
 } grad_t;
 
@@ -1185,17 +1193,18 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
             const mat_t* m = &sc->mats[s->mat];
             int rinfo_in = cur->in_object;
             double smax_s = 0.0;
+            int ltex = -1;
             pcg_t gn;
             nee_stream(&gn, g, cur->nee);
             switch (m->kind) {
             case MAT_MATTE:
                 fr = cscale(m->color, INV_PI_D * m->kd);
-                direct = matte_direct(sc, m, cur->p, n, &gn, &smax_s);
+                direct = matte_direct(sc, m, cur->p, n, &gn, &smax_s, &ltex);
                 break;
             case MAT_GLOSSY:
                 if (m->kd != 0.0 && cur->rand <= m->kd_pdf) {
                     fr = cscale(m->color, INV_PI_D * m->kd);
-                    direct = cscale(matte_direct(sc, m, cur->p, n, &gn, &smax_s), 1.0 / m->kd_pdf);
+                    direct = cscale(matte_direct(sc, m, cur->p, n, &gn, &smax_s, &ltex), 1.0 / m->kd_pdf);
                 } else {
                     fr = cs(phong_f(m, n, w, wo) * m->ks);
                 }
@@ -1259,6 +1268,16 @@ static col trace_pt(const scene_t* sc, ray_t cam_ray, int max_depth, pcg_t* g, p
                 if (tix >= 0) {
                     G->tix[G->n_tex] = tix;
                     G->tval[G->n_tex] = L; /* d (L tx) / d tx per channel */
+                    ++G->n_tex;
+                }
+                if (ltex >= 0) {
+                    /* the light sample is linear in the image-based light's
+                       texel: d direct / d texel = direct / texel per channel */
+                    const float* tv = sc->texels + 3 * (size_t)ltex;
+                    col d = cmk(tv[0] > 0 ? direct.r / tv[0] : 0.0, tv[1] > 0 ? direct.g / tv[1] : 0.0,
+                                tv[2] > 0 ? direct.b / tv[2] : 0.0);
+                    G->tix[G->n_tex] = ltex;
+                    G->tval[G->n_tex] = cmul(cmul(d, pass), tx);
                     ++G->n_tex;
                 }
             }
@@ -1428,8 +1447,9 @@ static void* render_rows(void* arg)
     grad_t G, *Gp = 0;
     int x, y, s, k, e;
     if (j->dimg) {
-        G.tix = (int*)malloc(sizeof(int) * (size_t)md);
-        G.tval = (col*)malloc(sizeof(col) * (size_t)md);
+        /* per vertex: its own texel and an image-based light's */
+        G.tix = (int*)malloc(sizeof(int) * 2 * (size_t)md);
+        G.tval = (col*)malloc(sizeof(col) * 2 * (size_t)md);
         Gp = &G;
         for (k = 0; k < 7; ++k) j->grad[k] = j->grad_var[k] = 0.0;
     }

@@ -157,6 +157,7 @@ struct ShadowBuf {
     float4* contrib;   // rgb, film pixel (int bits) in w
     uint32_t* sample;  // adjoint passes: the sample the contribution belongs to
     float2* score;     // adjoint replay: the connection's own score terms
+    int32_t* tex;      // adjoint replay: the image-based light's texel (scene pool index) or -1
     uint32_t* scene;   // batches: the entry's scene
 };
 
@@ -508,7 +509,8 @@ __device__ inline float shadow_tr(const DevMedium& m, V3 o, V3 d, float tmax, Rn
 struct LightSample {
     V3 p, n, le;
     float pdf;
-    float g;  // the light's G toward x (AreaLight::G; 1 for the image-based light)
+    float g;    // the light's G toward x (AreaLight::G; 1 for the image-based light)
+    int tex;    // image-based light: the texel le came from (scene pool index), else -1
 };
 
 __device__ inline V3 pattern(const SceneView& sv, const DevShape& s, V3 p, float bu, float bv, int& index);
@@ -538,8 +540,9 @@ __device__ inline bool sample_ibl(const SceneView& sv, const DevLight& L, V3 x, 
     hit_sphere(make_float4(L.p[0], L.p[1], L.p[2], L.l[0]), r, false, true);
     ls.p = x + wi * r.tmin;
     ls.n = -wi;
-    int index;
+    int index = -1;
     ls.le = pattern(sv, sv.shapes[L.prim], ls.p, 0.0f, 0.0f, index);
+    ls.tex = index;
     float c = dot(wi, nrm);
     if (c <= 0.0f) c = 1e-4f;
     float pdf = kInvPiF * c;
@@ -558,6 +561,7 @@ __device__ inline bool sample_ibl(const SceneView& sv, const DevLight& L, V3 x, 
 template <bool IBL>
 __device__ inline bool sample_light(const SceneView& sv, V3 x, V3 nrm, Rng& g, LightSample& ls, V3& wi,
                                     float& dist) {
+    ls.tex = -1;
     if (sv.n_lights == 0) return false;
     uint32_t li = (uint32_t)(g.uniform() * (float)sv.n_lights);
     if (li >= sv.n_lights) li = sv.n_lights - 1;
@@ -840,15 +844,17 @@ struct ShadeOut {
     bool shadow;
     float4 s0, s1, contrib;
     float2 score;   // shadow entry: medium length of the segment (Bernoulli), d log fp / d g
+    int32_t ltex;   // shadow entry: the image-based light's texel, or -1
 };
 
 __device__ inline void push_shadow(ShadeOut& o, V3 x, V3 wi, float dist, V3 c, uint32_t pixel,
-                                   float smax_s, float dlog_fp) {
+                                   float smax_s, float dlog_fp, int ltex) {
     o.shadow = true;
     o.s0 = make_float4(x.x, x.y, x.z, 0.0f);
     o.s1 = make_float4(wi.x, wi.y, wi.z, dist);
     o.contrib = make_float4(c.x, c.y, c.z, __int_as_float((int)pixel));
     o.score = make_float2(smax_s, dlog_fp);
+    o.ltex = ltex;
 }
 
 __device__ inline void push_next(ShadeOut& o, V3 p, V3 w, V3 beta, uint32_t flags, float pend) {
@@ -869,10 +875,12 @@ __device__ inline uint32_t mk_flags(bool le, bool in_obj, uint32_t depth) {
 // the medium when its Bernoulli transmittance survived (0 if none).
 template <bool IBL>
 __device__ inline bool matte_direct(const SceneView& sv, const DevMaterial& m, V3 p, V3 n, Rng& g,
-                                    V3& lo, V3& wi, float& dist, float& smax_s) {
+                                    V3& lo, V3& wi, float& dist, float& smax_s, int& ltex) {
     LightSample ls;
     smax_s = 0.0f;
-    if (!sample_light<IBL>(sv, p, n, g, ls, wi, dist)) return false;
+    const bool lit = sample_light<IBL>(sv, p, n, g, ls, wi, dist);
+    ltex = ls.tex;
+    if (!lit) return false;
     const V3 f = ld3(m.color) * (kInvPiF * m.kd);
     const float c = fmaxf(dot(wi, n), 0.0f);
     lo = cmul(f, ls.le) * (c * ls.g / ls.pdf);
@@ -952,7 +960,7 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
                     const float fp = md.g == 0.0f ? 1.0f / (4.0f * kPiF) : hg_eval(md.g, cl);
                     const V3 c = cmul(cmul(beta, ls.le), ld3(md.sigma_s)) *
                                  (tr * fp * ls.g / ls.pdf);
-                    if (tr > 0.0f) push_shadow(out, p, wi, dist, c, pixel, smax_s, dlog_hg(md.g, cl));
+                    if (tr > 0.0f) push_shadow(out, p, wi, dist, c, pixel, smax_s, dlog_hg(md.g, cl), -1);
                 }
                 if (last) return;
                 float pdf;
@@ -1028,13 +1036,14 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
         if (m.kind == RUMI_MAT_MATTE || (m.kind == RUMI_MAT_GLOSSY && m.kd != 0.0f && 1.0f <= m.kd_pdf)) {
             V3 lo, wi;
             float dist, smax_s;
+            int ltex;
             Rng gn = nee_stream(g, nee_key);
-            if (matte_direct<IBL>(sv, m, p, n, gn, lo, wi, dist, smax_s)) {
+            if (matte_direct<IBL>(sv, m, p, n, gn, lo, wi, dist, smax_s, ltex)) {
                 if (m.kind == RUMI_MAT_GLOSSY) lo = lo * (1.0f / m.kd_pdf);
                 const V3 tex = s.info[2] >= 0 ? texture_at(p) : v3(1, 1, 1);
                 const V3 c = cmul(cmul(beta, tex), lo);
                 if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f)
-                    push_shadow(out, p, wi, dist, c, pixel, smax_s, 0.0f);
+                    push_shadow(out, p, wi, dist, c, pixel, smax_s, 0.0f, ltex);
             }
         }
         return;
@@ -1145,12 +1154,13 @@ __device__ void shade(const SceneView& sv, const Params& prm, V3 o, V3 d, uint32
     if (m.kind == RUMI_MAT_MATTE || (m.kind == RUMI_MAT_GLOSSY && m.kd != 0.0f && xi <= m.kd_pdf)) {
         V3 lo, wi;
         float dist, smax_s;
+        int ltex;
         Rng gn = nee_stream(g, nee_key);
-        if (matte_direct<IBL>(sv, m, p, n, gn, lo, wi, dist, smax_s)) {
+        if (matte_direct<IBL>(sv, m, p, n, gn, lo, wi, dist, smax_s, ltex)) {
             if (m.kind == RUMI_MAT_GLOSSY) lo = lo * (1.0f / m.kd_pdf);
             const V3 c = cmul(cmul(beta, tex), lo);
             if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f)
-                push_shadow(out, p, wi, dist, c, pixel, smax_s, 0.0f);
+                push_shadow(out, p, wi, dist, c, pixel, smax_s, 0.0f, ltex);
         }
     }
     if (last) return;
@@ -1360,7 +1370,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(PathBuf in, PathBuf out, 
             sh.ray[2 * (size_t)ss + 1] = so.s1;
             sh.contrib[ss] = so.contrib;
             if (MODE != kRender) sh.sample[ss] = so.sample;
-            if (MODE == kReplay) sh.score[ss] = so.score;
+            if (MODE == kReplay) {
+                sh.score[ss] = so.score;
+                sh.tex[ss] = so.ltex;
+            }
             if (MULTI) sh.scene[ss] = scene;
         }
     }
@@ -1374,13 +1387,14 @@ struct ShadowItem {
     ShadowBuf sh;
     const SceneTab* tab;
     const float4 *nodes, *prims, *rects;
+    const float* texels;  // the scene's texel pool (single scene)
     RootInfo root;
     float* film;
     AdjCtx adj;
     Grad7 G;
     int2* s_stk;
     int2* p_stk;
-    uint32_t tid, id, nid, nscene;
+    uint32_t tid, id, nid, nscene, scene;
     float4 n0, n1;
     __device__ void prefetch(uint32_t k) {
         n0 = sh.ray[2 * (size_t)k + 0];
@@ -1392,6 +1406,7 @@ struct ShadowItem {
         s.r = Ray{n0.x, n0.y, n0.z, n1.x, n1.y, n1.z, n1.w, 0.0f};
         id = nid;
         if (MULTI) {
+            scene = nscene;
             const SceneView& v = tab[nscene].sv;
             nodes = v.nodes;
             prims = v.prims;
@@ -1422,6 +1437,17 @@ struct ShadowItem {
             const float2 sc = sh.score[id];
             if (sc.x > 0.0f) grad_grey(G, -sc.x, Ss);
             G.v[6] += sc.y * Ss;
+            // an image-based light's texel scales this contribution linearly
+            const int lt = sh.tex[id];
+            if (lt >= 0 && adj.tex_grad) {
+                const float* pool = MULTI ? tab[scene].sv.texels : texels;
+                const uint32_t base = MULTI ? tab[scene].texel_base : 0u;
+                const V3 tx = ld3(pool + 3 * (size_t)lt);
+                float* tg = adj.tex_grad + 3 * ((size_t)base + (size_t)lt);
+                if (tx.x > 0.0f && w3.x != 0.0f) atomicAdd(tg + 0, w3.x / tx.x);
+                if (tx.y > 0.0f && w3.y != 0.0f) atomicAdd(tg + 1, w3.y / tx.y);
+                if (tx.z > 0.0f && w3.z != 0.0f) atomicAdd(tg + 2, w3.z / tx.z);
+            }
         }
     }
 };
@@ -1439,6 +1465,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
     it.nodes = sv.nodes;
     it.prims = sv.prims;
     it.rects = sv.rects;
+    it.texels = sv.texels;
     it.root = sv.root;
     it.film = film;
     it.adj = adj;
@@ -1447,7 +1474,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
     it.s_stk = s_stk;
     it.p_stk = p_stk;
     it.tid = threadIdx.x;
-    it.id = it.nid = it.nscene = 0;
+    it.id = it.nid = it.nscene = it.scene = 0;
     run_lanes(*n_ptr, counter, it);
     if (MODE == kReplay) flush_grad(it.G, adj.gstripes);
 }
@@ -2295,9 +2322,9 @@ struct Wave {
 
 // Bytes per queued path: 2 path sets (ray 32, beta 16, rng 8, pixel, sample,
 // flags, pend, scene 4 each), hit 8, barycentrics 8, shadow entry (ray 32,
-// contrib 16, sample 4, score 8, scene 4).
-constexpr size_t kBytesPerPath = 2 * (32 + 16 + 8 + 4 * 5) + 8 + 8 + (32 + 16 + 4 + 8 + 4);
-constexpr int kCarved = 24;  // arrays carved below, each aligned to 256 B
+// contrib 16, sample 4, score 8, texel 4, scene 4).
+constexpr size_t kBytesPerPath = 2 * (32 + 16 + 8 + 4 * 5) + 8 + 8 + (32 + 16 + 4 + 8 + 4 + 4);
+constexpr int kCarved = 25;  // arrays carved below, each aligned to 256 B
 
 size_t wave_bytes(uint32_t cap) { return kBytesPerPath * (size_t)cap + sizeof(Ctl) + 256 * kCarved; }
 
@@ -2324,6 +2351,7 @@ bool wave_carve(Wave& w, void* mem, size_t bytes, uint32_t cap) {
     w.sh.contrib = (float4*)take(16 * (size_t)cap);
     w.sh.sample = (uint32_t*)take(4 * (size_t)cap);
     w.sh.score = (float2*)take(8 * (size_t)cap);
+    w.sh.tex = (int32_t*)take(4 * (size_t)cap);
     w.sh.scene = (uint32_t*)take(4 * (size_t)cap);
     w.ctl = (Ctl*)take(sizeof(Ctl));
     w.cap = cap;

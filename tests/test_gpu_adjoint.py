@@ -170,6 +170,21 @@ def test_gradients_equal_oracle_on_common_random_numbers(rumi, oracle, ctx, scen
     np.testing.assert_allclose(gt, ct, rtol=0, atol=1e-3 * np.abs(ct).max())
 
 
+def test_image_based_light_texel_gradients_equal_oracle(rumi, oracle, ctx):
+    """The sky's texels through the light samples of matte vertices (the
+    shadow entry carries the texel; the shadow stage adds dLoss x contrib /
+    texel when unoccluded) and through dielectric chains: the GPU adjoint
+    equals the oracle's FP64 derivative with one seed (the oracle's is pinned
+    by exact differences in test_oracle_adjoint.py)."""
+    w = 16
+    sc = rc.ibl_scene(w, w, seed=9)
+    dimg = np.random.default_rng(6).uniform(0.5, 1.5, (w, w, 3))
+    gt = adjoint(rumi, ctx, sc, dimg, 128, 4, seed=31, textures=True)[2]
+    ct = oracle.adjoint_pt(sc, dimg, 128, 4, seed=31, nthreads=8, textures=True)[3]
+    assert np.abs(ct).max() > 0
+    np.testing.assert_allclose(gt[:ct.shape[0]], ct, rtol=0, atol=1e-3 * np.abs(ct).max())
+
+
 def test_medium_gradient_matches_finite_differences(rumi, ctx, scenes):
     """Central differences of the GPU renderer (step 10% of each parameter,
     common random numbers) against the adjoint, over independent seeds."""

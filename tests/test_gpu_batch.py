@@ -27,7 +27,8 @@ def _items(scenes):
            (rc.textured_scene(32, 20), 32, 20),
            (scenes.cornell_media(24, 24), 24, 24),
            (scenes.batch_scene(3, 6, 36, tex_size=16), 36, 36),
-           (scenes.batch_scene(7, 12, 20, tex_size=8), 20, 20)]
+           (scenes.batch_scene(7, 12, 20, tex_size=8), 20, 20),
+           (rc.ibl_scene(24, 16), 24, 16)]
     for sc, w, h in out:
         sc["camera"]["y_ratio"] = h / w
     return out
@@ -76,7 +77,8 @@ def test_batched_adjoint_equals_per_scene_adjoint(rumi, ctx, scenes):
     import torch
     items = [(rc.textured_scene(24, 16), 24, 16),
              (scenes.batch_scene(2, 5, 28, tex_size=12), 28, 28),
-             (scenes.batch_scene(5, 9, 20, tex_size=8), 20, 20)]
+             (scenes.batch_scene(5, 9, 20, tex_size=8), 20, 20),
+             (rc.ibl_scene(20, 16), 20, 16)]
     for sc, w, h in items:
         sc["camera"]["y_ratio"] = h / w
     objs = [rumi.Scene(ctx, sc) for sc, _, _ in items]

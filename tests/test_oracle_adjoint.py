@@ -104,3 +104,48 @@ def test_medium_gradient_matches_finite_differences(oracle):
     z = (ga.mean(0) - fd.mean(0)) / se
     print("adjoint", ga.mean(0), "\nFD", fd.mean(0), "\nz", z)
     assert np.all(np.abs(z) < 4.0), z
+
+
+def test_image_based_light_texels_get_their_light_sample_gradients(oracle):
+    """The image-based light's texels enter the image twice: where the sky is
+    seen or reached through a specular chain (the texture of the hit vertex)
+    and through every light sample of a matte vertex (ImageBasedLight::L,
+    imagebasedlight.cpp:35-64, linear in the texel it reads).  No sampling
+    decision depends on a texel, so renders with one seed are linear in each
+    texel and central differences along any texel direction are exact: the
+    adjoint must equal them, for random directions over the whole sky and
+    for single texels lit only through light samples."""
+    w = 8
+    sc = rc.ibl_scene(w, w, seed=9)
+    sky = sc["textures"][0]["rgb"]
+    rng = np.random.default_rng(3)
+    dimg = rng.uniform(0.5, 1.5, (w, w, 3))
+    spp, depth, seed = 64, 4, 17
+    _, _, _, gt = oracle.adjoint_pt(sc, dimg, spp, depth, seed=seed, nthreads=8, textures=True)
+    assert gt.shape == (sky.shape[0] * sky.shape[1], 3)
+
+    def loss_at(delta):
+        s2 = dict(sc)
+        s2["textures"] = [dict(kind="hdr", rgb=(sky.astype(np.float64) + delta).astype(np.float32))]
+        img = oracle.render_pt(s2, w, w, spp, depth, seed=seed, nthreads=8)[0]
+        actual = s2["textures"][0]["rgb"].astype(np.float64) - sky
+        return (img * dimg).sum(), actual
+
+    for k in range(2):
+        v = rng.uniform(-1.0, 1.0, sky.shape) * 1e-2
+        lp, ap = loss_at(v)
+        lm, am = loss_at(-v)
+        fd = lp - lm
+        want = float((gt.reshape(sky.shape) * (ap - am)).sum())
+        np.testing.assert_allclose(want, fd, rtol=1e-5)
+    # the camera never sees the sky directly here: a texel's whole gradient
+    # comes from light samples (and dielectric chains)
+    big = np.argsort(-np.abs(gt).sum(1))[:3]
+    for i in big:
+        y, x = divmod(int(i), sky.shape[1])
+        d = np.zeros_like(sky, dtype=np.float64)
+        d[y, x, :] = 1e-2 * sky[y, x, :]
+        lp, ap = loss_at(d)
+        lm, am = loss_at(-d)
+        want = float((gt[i] * (ap - am)[y, x]).sum())
+        np.testing.assert_allclose(want, lp - lm, rtol=1e-5)
