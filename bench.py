@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-scene leg")
     ap.add_argument("--grad-check", type=float, default=0.0, metavar="SECONDS",
                     help="only compare the media adjoint with central differences for about SECONDS")
+    ap.add_argument("--grad-steps", default="0.2,0.1",
+                    help="relative finite-difference steps of --grad-check (Richardson over the two smallest)")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default traversal launch shape instead of autotuning it")
@@ -594,19 +596,29 @@ def batch_leg(rumi, ctx, args, log, scenes):
 
 def grad_check(rumi, ctx, args, log):
     """Adjoint vs central finite differences on the differentiable-media
-    workload (512^2, depth 16), step eps = 1e-3 relative per parameter, common
-    random numbers (one seed per +/- pair).  The loss is linearised at a fixed
-    weight image w = 2 (I0 - T) (I0 at theta, T at theta x 1.10, both 256 spp)
-    so that the adjoint and the differences estimate the same quantity,
-    sum w dE[I]/dtheta.  Runs each side for the given seconds; prints one JSON
-    line with the per-parameter relative difference and its standard error."""
+    workload (512^2, 32 spp, depth 16), common random numbers (one seed per
+    replicate, shared by every step and sign).  The loss is linearised at a
+    fixed weight image w = 2 (I0 - T) (I0 at theta, T at theta x 1.10, both
+    256 spp), so the adjoint and the differences estimate the same quantity,
+    sum w dE[I]/dtheta.
+
+    A difference with common random numbers is dominated by sampling
+    decisions that flip between theta - h and theta + h (a distance, lobe or
+    roulette choice): its variance grows as 1/h, so a step of 1e-3 cannot
+    resolve 1% in minutes.  The steps here are relative (h_k = step theta_k),
+    and the estimate is the Richardson combination (4 D(h/2) - D(h)) / 3 of
+    the two smallest, whose remaining bias is O(h^4); the O(h^2) bias of D(h)
+    itself, (D(h) - D(h/2)) 4/3, is reported beside it.  Runs for about the
+    given seconds; prints one JSON line."""
     import numpy as np
     import torch
     sys.path.insert(0, os.path.join(ROOT, "rumi-renderer_amd"))
     import scenes
     w = h = 512
-    spp, depth, eps = 32, 16, 1e-3
+    spp, depth = 32, 16
+    steps = sorted({float(x) for x in args.grad_steps.split(",")}, reverse=True)
     theta = np.array([0.05, 0.2, 0.4, 1.0, 1.0, 1.0, 0.7])
+    names = ["sigma_a.r", "sigma_a.g", "sigma_a.b", "sigma_s.r", "sigma_s.g", "sigma_s.b", "g"]
 
     def scene(p):
         return rumi.Scene(ctx, scenes.cornell_media(w, h, sigma_a=tuple(p[:3]), sigma_s=tuple(p[3:6]),
@@ -617,43 +629,62 @@ def grad_check(rumi, ctx, args, log):
     tg = st.render(w, h, 256, depth, seed=12)[0].clone()
     st.close()
     wgt = (2.0 * (i0 - tg)).contiguous()
+    wgt64 = wgt.double()
     torch.cuda.synchronize()
     adj = []
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < args.grad_check * 0.2 or len(adj) < 8:
+    while time.perf_counter() - t0 < args.grad_check * 0.15 or len(adj) < 8:
         adj.append(s0.adjoint(wgt, spp, depth, seed=1000 + len(adj))[1].cpu().double().numpy())
     log(f"grad check: {len(adj)} adjoint runs")
     adj = np.array(adj)
-    fd = [[] for _ in range(7)]
+    # scenes at every perturbed point, built once
+    pts = {}
+    for k in range(7):
+        for hs in steps:
+            for sg in (1, -1):
+                p = theta.copy()
+                p[k] += sg * hs * theta[k]
+                pts[(k, hs, sg)] = scene(p)
+    img = torch.empty((h, w, 3), dtype=torch.float32, device="cuda")
+    fd = []   # replicate x step x param
     t0 = time.perf_counter()
-    r = 0
-    budget = args.grad_check * 0.8
-    while time.perf_counter() - t0 < budget or r < 4:
+    budget = args.grad_check * 0.85
+    while time.perf_counter() - t0 < budget or len(fd) < 4:
+        r = len(fd)
+        rep = np.zeros((len(steps), 7))
         for k in range(7):
-            hk = eps * abs(theta[k])
-            tp, tm = theta.copy(), theta.copy()
-            tp[k] += hk
-            tm[k] -= hk
-            sp, sm = scene(tp), scene(tm)
-            ip = sp.render(w, h, spp, depth, seed=5000 + r)[0]
-            im = sm.render(w, h, spp, depth, seed=5000 + r)[0]
-            fd[k].append(float(((ip - im).double() * wgt.double()).sum()) / (2 * hk))
-            sp.close()
-            sm.close()
-        r += 1
-        log(f"grad check: {r} difference rounds, {time.perf_counter() - t0:.0f}s")
+            for j, hs in enumerate(steps):
+                lp = float((pts[(k, hs, 1)].render(w, h, spp, depth, seed=5000 + r, out=img)[0]
+                            .double() * wgt64).sum())
+                lm = float((pts[(k, hs, -1)].render(w, h, spp, depth, seed=5000 + r, out=img)[0]
+                            .double() * wgt64).sum())
+                rep[j, k] = (lp - lm) / (2 * hs * theta[k])
+        fd.append(rep)
+        if r % 10 == 9:
+            log(f"grad check: {r + 1} difference rounds, {time.perf_counter() - t0:.0f}s")
+    for o in pts.values():
+        o.close()
     s0.close()
-    fd = np.array(fd).T
+    fd = np.array(fd)
+    n = fd.shape[0]
     a_m, a_se = adj.mean(0), adj.std(0, ddof=1) / np.sqrt(len(adj))
-    f_m, f_se = fd.mean(0), fd.std(0, ddof=1) / np.sqrt(len(fd))
-    rel = np.abs(a_m - f_m) / np.abs(f_m)
-    rel_se = np.sqrt(a_se ** 2 + f_se ** 2) / np.abs(f_m)
-    out = {"metric": "adjoint vs central FD (eps 1e-3), media 512^2 depth 16",
-           "params": ["sigma_a.r", "sigma_a.g", "sigma_a.b", "sigma_s.r", "sigma_s.g", "sigma_s.b", "g"],
-           "adjoint": a_m.round(5).tolist(), "adjoint_se": a_se.round(5).tolist(),
-           "fd": f_m.round(5).tolist(), "fd_se": f_se.round(5).tolist(),
-           "rel_diff": rel.round(4).tolist(), "rel_diff_se": rel_se.round(4).tolist(),
-           "adjoint_runs": len(adj), "fd_pairs_per_param": len(fd), "spp": spp}
+    per_step = {str(hs): {"fd": fd[:, j].mean(0).round(6).tolist(),
+                          "fd_se": (fd[:, j].std(0, ddof=1) / np.sqrt(n)).round(6).tolist()}
+                for j, hs in enumerate(steps)}
+    rich = (4.0 * fd[:, -1] - fd[:, -2]) / 3.0 if len(steps) > 1 else fd[:, -1]
+    r_m, r_se = rich.mean(0), rich.std(0, ddof=1) / np.sqrt(n)
+    bias_h = ((fd[:, -2] - fd[:, -1]).mean(0) * 4.0 / 3.0) if len(steps) > 1 else np.zeros(7)
+    rel = np.abs(a_m - r_m) / np.abs(r_m)
+    rel_se = np.sqrt(a_se ** 2 + r_se ** 2) / np.abs(r_m)
+    out = {"metric": "adjoint vs central FD (Richardson over relative steps), media 512^2 32 spp depth 16",
+           "params": names, "theta": theta.tolist(), "steps": steps,
+           "adjoint": a_m.round(6).tolist(), "adjoint_se": a_se.round(6).tolist(),
+           "fd_richardson": r_m.round(6).tolist(), "fd_richardson_se": r_se.round(6).tolist(),
+           "fd_rel_se": (r_se / np.abs(r_m)).round(5).tolist(),
+           "bias_of_largest_step_estimate": bias_h.round(6).tolist(),
+           "per_step": per_step,
+           "rel_diff": rel.round(5).tolist(), "rel_diff_se": rel_se.round(5).tolist(),
+           "adjoint_runs": len(adj), "fd_replicates": n, "spp": spp}
     print(json.dumps(out), flush=True)
 
 

@@ -492,6 +492,50 @@ def render_pt(scene: dict, width: int, height: int, spp: int, max_depth: int, se
     return mean, m2
 
 
+def render_pt_refsampling(scene: dict, width: int, height: int, spp: int, max_depth: int,
+                          light_points: int = 256, seed: int = 0, nthreads: int = 1):
+    """or_render_pt_refsampling: the path tracer with the reference's fixed
+    pixel and light point sets (Q9); (int)sqrt(spp)^2 samples per pixel.
+    Returns the per-pixel mean and mean of squares."""
+    L = lib()
+    if not hasattr(L, "_refs_bound"):
+        L.or_render_pt_refsampling.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                                               c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
+                                               c_int, ctypes.c_uint64, c_int, c_void_p, c_void_p]
+        L.or_probe_refsets.argtypes = [c_int, c_int, c_double, c_double, ctypes.c_uint64, c_void_p,
+                                       POINTER(c_int)]
+        L._refs_bound = True
+    shapes, mats, cam, med = scene_arrays(scene)
+    texels, table, n_tex, _ = texture_arrays(scene)
+    mean = np.zeros((height, width, 3), np.float64)
+    m2 = np.zeros((height, width, 3), np.float64)
+    rc = L.or_render_pt_refsampling(_p(shapes), shapes.shape[0], _p(mats), mats.shape[0], _p(cam),
+                                    _p(med) if med is not None else None, _p(texels), _p(table),
+                                    n_tex, width, height, spp, max_depth, light_points, seed,
+                                    nthreads, _p(mean), _p(m2))
+    if rc != 0:
+        raise RuntimeError(f"or_render_pt_refsampling failed ({rc})")
+    return mean, m2
+
+
+def reference_point_set(kind: str, n: int, length: float = 1.0, width: float = 1.0,
+                        seed: int = 1) -> np.ndarray:
+    """The reference's fixed point sets as the oracle makes them: "pixel"
+    (UniformSample), "rect" (RectSample) or "sphere" (SphereSample; phi,
+    theta); (count, 2) float64."""
+    L = lib()
+    if not hasattr(L, "_refs_bound"):
+        L.or_probe_refsets.argtypes = [c_int, c_int, c_double, c_double, ctypes.c_uint64, c_void_p,
+                                       POINTER(c_int)]
+    k = {"pixel": 0, "rect": 1, "sphere": 2}[kind]
+    cnt = c_int()
+    if L.or_probe_refsets(k, n, length, width, seed, None, byref(cnt)) != 0:
+        raise RuntimeError("or_probe_refsets failed")
+    xy = np.zeros((max(1, cnt.value), 2), np.float64)
+    L.or_probe_refsets(k, n, length, width, seed, _p(xy), byref(cnt))
+    return xy[:cnt.value]
+
+
 def render_whitted(scene: dict, width: int, height: int, spp: int, max_depth: int, seed: int = 0,
                    nthreads: int = 1):
     """FP64 restatement of RayTracer::Trace (RayTracer(scene, max_depth));

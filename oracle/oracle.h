@@ -184,6 +184,28 @@ int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mat
                  const int* tex_table, int n_tex, int width, int height, int spp, int max_depth,
                  uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb);
 
+/* or_render_pt with the reference's sampling (Q9): (int)sqrt(spp)^2
+ * samples per pixel (Scene::Rendering takes SampleCount(), scene.cpp:214,
+ * and the film divides by it), their jitter drawn by Sampler::Sampling
+ * (sampler.cpp:17-38) from UniformSample's fixed jittered set
+ * (uniformsample.cpp:18-58), and area-light points from RectSample's
+ * (rectsample.cpp:18-102, light_points requested) or SphereSample's
+ * (spheresample.cpp:20-48) fixed sets, made once per frame and shuffled
+ * (Sampler::ShuffleSamples).  The pixel's pass over its set restarts at
+ * every pixel as in the reference; a light set is met in a random
+ * arrangement and cursor at each pixel (where the frame's earlier draws
+ * would have left it). */
+int or_render_pt_refsampling(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                             const float* camera, const float* medium, const float* texels,
+                             const int* tex_table, int n_tex, int width, int height, int spp,
+                             int max_depth, int light_points, uint64_t seed, int nthreads,
+                             double* mean_rgb, double* m2_rgb);
+/* The point sets themselves: kind 0 UniformSample(n, n > 1), 1
+ * RectSample(n, length, width), 2 SphereSample(n) (phi, theta); xy: 2 per
+ * point (may be NULL), count: the set's size. */
+int or_probe_refsets(int kind, int n, double length, double width, uint64_t seed, double* xy,
+                     int* count);
+
 /* RayTracer (src/Integrator/raytracer.cpp:30-86) over the same scene
  * records: or_render_pt's walk where every surface but specular and
  * dielectric ends the path after its light sample (Glossy's only when its

@@ -663,6 +663,158 @@ static v3 phase_sample(double g, v3 d, pcg_t* gen, double* pdf)
     return frame(u, v, w, st * sin(phi), ct, st * cos(phi));
 }
 
+/* ------------------------------------------- the reference's point sets */
+/* Q9: the reference draws pixel jitter and area-light points from fixed
+ * point sets made once per frame, each draw taken by Sampler::Sampling
+ * (sampler.cpp:17-38): an incremental shuffle of the x and of the y values,
+ * independently, restarting after every pass over the set.  ref_set_t is
+ * one such set and its cursor. */
+typedef struct {
+    int n, cursor;
+    double *x, *y;
+} ref_set_t;
+
+static void ref_set_alloc(ref_set_t* s, int n)
+{
+    s->n = n;
+    s->cursor = 0;
+    s->x = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    s->y = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+}
+
+static void ref_set_free(ref_set_t* s)
+{
+    free(s->x);
+    free(s->y);
+    s->x = s->y = 0;
+    s->n = 0;
+}
+
+static void ref_set_copy(ref_set_t* d, const ref_set_t* s)
+{
+    memcpy(d->x, s->x, sizeof(double) * (size_t)s->n);
+    memcpy(d->y, s->y, sizeof(double) * (size_t)s->n);
+    d->n = s->n;
+    d->cursor = 0;
+}
+
+/* Sampler::Sampling (sampler.cpp:17-38); rand() % m becomes a uniform
+ * integer below m. */
+static void ref_set_draw(ref_set_t* s, pcg_t* g, double* x, double* y)
+{
+    if (s->cursor >= s->n) s->cursor = 0;
+    int m = s->n - s->cursor;
+    int i1 = s->cursor + (int)(rnd(g) * m), i2 = s->cursor + (int)(rnd(g) * m);
+    if (i1 >= s->n) i1 = s->n - 1;
+    if (i2 >= s->n) i2 = s->n - 1;
+    double sx = s->x[i1], sy = s->y[i2];
+    s->x[i1] = s->x[s->cursor];
+    s->y[i2] = s->y[s->cursor];
+    s->x[s->cursor] = sx;
+    s->y[s->cursor] = sy;
+    ++s->cursor;
+    *x = sx;
+    *y = sy;
+}
+
+/* Sampler::ShuffleSamples (sampler.cpp:40-66). */
+static void ref_set_shuffle(ref_set_t* s, pcg_t* g)
+{
+    int i;
+    for (i = 0; i < s->n; ++i) {
+        int j = (int)(rnd(g) * s->n);
+        if (j >= s->n) j = s->n - 1;
+        double t = s->x[i];
+        s->x[i] = s->x[j];
+        s->x[j] = t;
+    }
+    for (i = 0; i < s->n; ++i) {
+        int j = (int)(rnd(g) * s->n);
+        if (j >= s->n) j = s->n - 1;
+        double t = s->y[i];
+        s->y[i] = s->y[j];
+        s->y[j] = t;
+    }
+}
+
+/* UniformSample(n, jitter = n > 1) + GenerateSamples (uniformsample.cpp:
+ * 18-58): d = (int)sqrt(n), d^2 points, one per cell of a d x d grid,
+ * jittered in the cell or at its centre. */
+static void ref_uniform_set(ref_set_t* s, int n, pcg_t* g)
+{
+    int d = (int)sqrt((float)n), ix, iy;
+    ref_set_alloc(s, d * d);
+    double inv = 1.0 / d;
+    for (iy = 0; iy < d; ++iy)
+        for (ix = 0; ix < d; ++ix) {
+            double jx = n > 1 ? rnd(g) : 0.5, jy = n > 1 ? rnd(g) : 0.5;
+            s->x[iy * d + ix] = (ix + jx) * inv;
+            s->y[iy * d + ix] = (iy + jy) * inv;
+        }
+}
+
+/* RectSample(n, length, width) + GenerateSamples (rectsample.cpp:18-102):
+ * a jittered (int)sqrt(n) grid on the unit square, the rows' points with
+ * y < ratio (short side / long side) kept until a row keeps none, mapped to
+ * (x, y / ratio) -- or (y / ratio, x) when width > length. */
+static void ref_rect_set(ref_set_t* s, int n, double length, double width, pcg_t* g)
+{
+    int d = (int)sqrt((float)n), ix, iy, kept = 0, k;
+    double ratio = length >= width ? width / length : length / width;
+    double* tx = (double*)malloc(sizeof(double) * (size_t)(d * d > 0 ? d * d : 1));
+    double* ty = (double*)malloc(sizeof(double) * (size_t)(d * d > 0 ? d * d : 1));
+    int rows = 0;
+    for (iy = 0; iy < d; ++iy) {
+        int any = 0;
+        for (ix = 0; ix < d; ++ix) {
+            tx[iy * d + ix] = (ix + rnd(g)) / d;
+            ty[iy * d + ix] = (iy + rnd(g)) / d;
+            if (ty[iy * d + ix] < ratio) {
+                any = 1;
+                ++kept;
+            }
+        }
+        ++rows;
+        if (!any) break;
+    }
+    ref_set_alloc(s, kept);
+    for (k = 0, ix = 0; k < kept && ix < rows * d; ++ix)
+        if (ty[ix] < ratio) {
+            if (length >= width) {
+                s->x[k] = tx[ix];
+                s->y[k] = ty[ix] / ratio;
+            } else {
+                s->x[k] = ty[ix] / ratio;
+                s->y[k] = tx[ix];
+            }
+            ++k;
+        }
+    free(tx);
+    free(ty);
+}
+
+/* SphereSample(n) + GenerateSamples (spheresample.cpp:20-48): phi, theta
+ * of a golden-ratio spiral (one random direction when n = 1). */
+static void ref_sphere_set(ref_set_t* s, int n, pcg_t* g)
+{
+    int i;
+    ref_set_alloc(s, n);
+    if (n == 1) {
+        s->x[0] = rnd(g) * 2.0 * PI_D;
+        s->y[0] = acos(1.0 - rnd(g) * 2.0);
+        return;
+    }
+    double golden = (1.0 + sqrt(5.0)) * 0.5;
+    for (i = 0; i < n; ++i) {
+        s->x[i] = PI_D * golden * (i + 0.5);
+        s->y[i] = acos(1.0 - 2.0 * (i + 0.5) / n);
+    }
+}
+
+/* The light sets' state of the pixel a thread is rendering (reference
+ * sampling only): one set per scene shape, NULL where the shape has none. */
+static __thread ref_set_t* tl_light_sets = 0;
+
 /* ------------------------------------------------------------ lights */
 typedef struct {
     v3 p, n;
@@ -731,11 +883,25 @@ static void light_sample(const scene_t* sc, v3 x, v3 nrm, pcg_t* g, lsample_t* l
         ls->wi = wi;
         return;
     }
+    ref_set_t* set = tl_light_sets ? &tl_light_sets[sc->lights[li]] : 0;
+    if (set && set->n == 0) set = 0;
     if (s->kind == OR_RECT) {
-        double a = rnd(g), b = rnd(g);
+        double a, b;
+        if (set)
+            ref_set_draw(set, g, &a, &b); /* Rect::ShapeSampling (rect.cpp:176-181) */
+        else
+            a = rnd(g), b = rnd(g);
         ls->p = add(add(s->a, mul(s->b, a)), mul(s->c, b));
         ls->n = s->n;
         area = s->length * s->width;
+    } else if (set) {
+        /* Sphere::ShapeSampling (sphere.cpp:152-170), identity transform */
+        double phi, theta;
+        ref_set_draw(set, g, &phi, &theta);
+        v3 dir = mk(sin(theta) * sin(phi), cos(theta), sin(theta) * cos(phi));
+        ls->p = add(s->a, mul(dir, s->r));
+        ls->n = norm(sub(ls->p, s->a));
+        area = 4.0 * PI_D * s->r * s->r;
     } else {
         double z = 1.0 - 2.0 * rnd(g), phi = 2.0 * PI_D * rnd(g);
         double r = sqrt(1.0 - z * z > 0 ? 1.0 - z * z : 0.0);
@@ -1437,6 +1603,11 @@ typedef struct {
     const double* dimg;
     double grad[7], grad_var[7];
     double* gtex; /* 3 per texel, or NULL */
+    /* reference sampling: the frame's pixel set and one light set per shape
+       (NULL: continuous sampling) */
+    const ref_set_t* pixel_set;
+    const ref_set_t* light_sets;
+    int n_shapes;
 } job_t;
 
 static void* render_rows(void* arg)
@@ -1453,11 +1624,34 @@ static void* render_rows(void* arg)
         Gp = &G;
         for (k = 0; k < 7; ++k) j->grad[k] = j->grad_var[k] = 0.0;
     }
+    /* reference sampling: this thread's copies of the frame's sets */
+    ref_set_t pix = {0, 0, 0, 0}, *lsets = 0;
+    if (j->pixel_set) {
+        ref_set_alloc(&pix, j->pixel_set->n);
+        lsets = (ref_set_t*)calloc((size_t)(j->n_shapes > 0 ? j->n_shapes : 1), sizeof(ref_set_t));
+        for (k = 0; k < j->n_shapes; ++k)
+            if (j->light_sets[k].n) ref_set_alloc(&lsets[k], j->light_sets[k].n);
+        tl_light_sets = lsets;
+    }
     for (y = j->row0; y < j->row1; ++y)
         for (x = 0; x < j->W; ++x) {
             size_t o = 3 * ((size_t)y * j->W + x);
             col sum = cs(0.0), sq = cs(0.0);
             double psum[7] = {0}, psq[7] = {0};
+            pcg_t gp;
+            if (j->pixel_set) {
+                /* Scene::Rendering takes the pixel's samples in one pass over
+                   the pixel set; the light sets are met wherever the frame's
+                   earlier draws left them: a random arrangement and cursor */
+                pcg_seed(&gp, j->seed ^ 0xA0761D6478BD642Full, (uint64_t)y * (uint64_t)j->W + (uint64_t)x);
+                ref_set_copy(&pix, j->pixel_set);
+                for (k = 0; k < j->n_shapes; ++k)
+                    if (lsets[k].n) {
+                        ref_set_copy(&lsets[k], &j->light_sets[k]);
+                        ref_set_shuffle(&lsets[k], &gp);
+                        lsets[k].cursor = (int)(rnd(&gp) * lsets[k].n);
+                    }
+            }
             for (s = 0; s < j->spp; ++s) {
                 /* one stream per sample, so that renders at nearby parameters
                    with one seed stay correlated sample by sample */
@@ -1465,6 +1659,7 @@ static void* render_rows(void* arg)
                 pcg_seed(&g, j->seed + (uint64_t)s * 0x9E3779B97F4A7C15ull,
                          (uint64_t)y * (uint64_t)j->W + (uint64_t)x);
                 double sx = rnd(&g), sy = rnd(&g);
+                if (j->pixel_set) ref_set_draw(&pix, &gp, &sx, &sy); /* Film::PixelSampler()->Sampling() */
                 ray_t r = camera_ray(&j->sc->cam, j->W, j->H, x, y, sx, sy, &g);
                 col L = trace_pt(j->sc, r, j->max_depth, &g, P, Gp);
                 if (cnan(L)) continue; /* Scene::Rendering drops NaN samples */
@@ -1507,6 +1702,12 @@ static void* render_rows(void* arg)
         free(G.tix);
         free(G.tval);
     }
+    if (j->pixel_set) {
+        ref_set_free(&pix);
+        for (k = 0; k < j->n_shapes; ++k) ref_set_free(&lsets[k]);
+        free(lsets);
+        tl_light_sets = 0;
+    }
     free(P);
     return 0;
 }
@@ -1516,7 +1717,7 @@ static int run_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
                   const int* tex_table, int n_tex, int width, int height, int spp, int max_depth,
                   uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb,
                   const double* dimg, double* grad7, double* grad7_var, double* grad_tex,
-                  size_t n_texels, int whitted)
+                  size_t n_texels, int whitted, const ref_set_t* pixel_set, const ref_set_t* light_sets)
 {
     if (width <= 0 || height <= 0 || spp <= 0 || max_depth <= 0 || !mean_rgb) return 1;
     if (n_tex > 0 && (!texels || !tex_table)) return 1;
@@ -1550,6 +1751,9 @@ static int run_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
         jobs[t].m2 = m2_rgb;
         jobs[t].dimg = dimg;
         jobs[t].gtex = (dimg && grad_tex) ? (double*)calloc(3 * n_texels + 1, sizeof(double)) : 0;
+        jobs[t].pixel_set = pixel_set;
+        jobs[t].light_sets = light_sets;
+        jobs[t].n_shapes = sc.n_shapes;
         if (nthreads == 1)
             render_rows(&jobs[t]);
         else
@@ -1587,7 +1791,71 @@ int or_render_pt(const float* shapes, int n_shapes, const float* mats, int n_mat
                  uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb)
 {
     return run_pt(shapes, n_shapes, mats, n_mats, camera, medium, texels, tex_table, n_tex, width,
-                  height, spp, max_depth, seed, nthreads, mean_rgb, m2_rgb, 0, 0, 0, 0, 0, 0);
+                  height, spp, max_depth, seed, nthreads, mean_rgb, m2_rgb, 0, 0, 0, 0, 0, 0, 0, 0);
+}
+
+int or_render_pt_refsampling(const float* shapes, int n_shapes, const float* mats, int n_mats,
+                             const float* camera, const float* medium, const float* texels,
+                             const int* tex_table, int n_tex, int width, int height, int spp,
+                             int max_depth, int light_points, uint64_t seed, int nthreads,
+                             double* mean_rgb, double* m2_rgb)
+{
+    if (spp <= 0 || light_points <= 0 || n_shapes < 0) return 1;
+    int i, rc, d = (int)sqrt((float)spp);
+    pcg_t gf;
+    /* the frame's sets, made once as Scene::Build makes them (scene.cpp:
+       145, 193): the film's UniformSample, each area light's set, shuffled */
+    pcg_seed(&gf, seed ^ 0x2545F4914F6CDD1Dull, 0xF00Du);
+    ref_set_t pix;
+    ref_uniform_set(&pix, spp, &gf);
+    ref_set_t* lights = (ref_set_t*)calloc((size_t)(n_shapes > 0 ? n_shapes : 1), sizeof(ref_set_t));
+    for (i = 0; i < n_shapes; ++i) {
+        const float* sh = shapes + 32 * (size_t)i;
+        if ((int)sh[2] != 1) continue;
+        if ((int)sh[0] == OR_RECT) {
+            double l = sqrt((double)sh[6] * sh[6] + (double)sh[7] * sh[7] + (double)sh[8] * sh[8]);
+            double w = sqrt((double)sh[9] * sh[9] + (double)sh[10] * sh[10] + (double)sh[11] * sh[11]);
+            ref_rect_set(&lights[i], light_points, l, w, &gf);
+        } else if ((int)sh[0] == OR_SPHERE) {
+            ref_sphere_set(&lights[i], light_points, &gf);
+        } else {
+            continue;
+        }
+        ref_set_shuffle(&lights[i], &gf);
+    }
+    rc = run_pt(shapes, n_shapes, mats, n_mats, camera, medium, texels, tex_table, n_tex, width,
+                height, d * d, max_depth, seed, nthreads, mean_rgb, m2_rgb, 0, 0, 0, 0, 0, 0, &pix,
+                lights);
+    for (i = 0; i < n_shapes; ++i) ref_set_free(&lights[i]);
+    free(lights);
+    ref_set_free(&pix);
+    return rc;
+}
+
+int or_probe_refsets(int kind, int n, double length, double width, uint64_t seed, double* xy,
+                     int* count)
+{
+    pcg_t g;
+    ref_set_t s;
+    pcg_seed(&g, seed, 7u);
+    if (kind == 0)
+        ref_uniform_set(&s, n, &g);
+    else if (kind == 1)
+        ref_rect_set(&s, n, length, width, &g);
+    else if (kind == 2)
+        ref_sphere_set(&s, n, &g);
+    else
+        return 1;
+    *count = s.n;
+    if (xy) {
+        int i;
+        for (i = 0; i < s.n; ++i) {
+            xy[2 * i] = s.x[i];
+            xy[2 * i + 1] = s.y[i];
+        }
+    }
+    ref_set_free(&s);
+    return 0;
 }
 
 /* RayTracer::Trace (src/Integrator/raytracer.cpp:37-86) with
@@ -1600,7 +1868,7 @@ int or_render_whitted(const float* shapes, int n_shapes, const float* mats, int 
                       int max_depth, uint64_t seed, int nthreads, double* mean_rgb, double* m2_rgb)
 {
     return run_pt(shapes, n_shapes, mats, n_mats, camera, medium, texels, tex_table, n_tex, width,
-                  height, spp, max_depth + 1, seed, nthreads, mean_rgb, m2_rgb, 0, 0, 0, 0, 0, 1);
+                  height, spp, max_depth + 1, seed, nthreads, mean_rgb, m2_rgb, 0, 0, 0, 0, 0, 1, 0, 0);
 }
 
 int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_mats,
@@ -1612,7 +1880,7 @@ int or_adjoint_pt(const float* shapes, int n_shapes, const float* mats, int n_ma
     if (!dloss_dimage || !grad7) return 1;
     return run_pt(shapes, n_shapes, mats, n_mats, camera, medium, texels, tex_table, n_tex, width,
                   height, spp, max_depth, seed, nthreads, mean_rgb, 0, dloss_dimage, grad7,
-                  grad7_var, grad_tex, n_texels, 0);
+                  grad7_var, grad_tex, n_texels, 0, 0, 0);
 }
 
 /* ====================================================================== */
