@@ -43,6 +43,9 @@ def main():
     ap.add_argument("--cpu-rays", type=int, default=1 << 20,
                     help="rays in the bounded CPU-baseline sample (0 = skip)")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing legs")
+    ap.add_argument("--only-leg", default="", metavar="NAME",
+                    help="run only this rendering leg (cornell_pt, media_pt, cornell_whitted, model_pt, "
+                         "media_gradient, cornell_bdpt, batch) and print its JSON: for profiling")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-scene leg")
     ap.add_argument("--grad-check", type=float, default=0.0, metavar="SECONDS",
                     help="only compare the media adjoint with central differences for about SECONDS")
@@ -82,6 +85,18 @@ def main():
 
     if args.grad_check > 0:
         grad_check(rumi, rumi.Context(0), args, log)
+        return
+    if args.only_leg:
+        sys.path.insert(0, os.path.join(ROOT, "rumi-renderer_amd"))
+        import scenes
+        legs = {"cornell_whitted": whitted_leg, "model_pt": model_leg, "media_gradient": gradient_leg,
+                "cornell_bdpt": bdpt_leg, "batch": batch_leg}
+        ctx = rumi.Context(0)
+        if args.only_leg in ("cornell_pt", "media_pt"):
+            res = render_legs(rumi, ctx, args, log, only=args.only_leg)
+        else:
+            res = legs[args.only_leg](rumi, ctx, args, log, scenes)
+        print(json.dumps({args.only_leg: res}), flush=True)
         return
 
     t0 = time.time()
@@ -128,10 +143,7 @@ def main():
     if not args.no_tune:
         # launch shape chosen by the autotuner on this workload (untimed setup;
         # every candidate returns the same bits)
-        sys.path.insert(0, os.path.join(ROOT, "rumi-renderer_amd"))
-        import autotune
-        best, table = autotune.tune_trace(
-            ctx, lambda: rumi.trace_closest(ctx, bvh, rays, hits, None, 0, stream))
+        best, table = rumi.autotune_trace(ctx, bvh, rays, hits, stream=stream)
         tuning = {"closest": best,
                   "candidates_us": [[c["block"], c["stack"], c["refill"], c["persistent"],
                                      c["width"], round(t * 1e6, 1)] for c, t in table]}
@@ -143,7 +155,7 @@ def main():
                  args.steps, args.warmup)
     log(f"closest-hit: {ms_c:.3f} ms/step, {n / ms_c / 1e3:.1f} Mrays/s")
     if tuning is not None:
-        best_a, _ = autotune.tune_trace(ctx, lambda: rumi.trace_any(ctx, bvh, rays, anyp, 0, stream))
+        best_a, _ = rumi.autotune_trace(ctx, bvh, rays, anyp, any_hit=True, stream=stream)
         tuning["any"] = best_a
         log(f"autotuned any-hit: {best_a}")
     nodes_a, tris_a, node_ba = work(True)
@@ -407,7 +419,7 @@ def model_leg(rumi, ctx, args, log, scenes):
     return out
 
 
-def render_legs(rumi, ctx, args, log):
+def render_legs(rumi, ctx, args, log, only=""):
     """Path tracing: the 512^2 x 16 spp Cornell box (depth 5) and the 512^2 x
     32 spp media scene (depth 16), samples/s on the GPU; rel-MSE against the
     committed 4096-spp oracle reference at 64^2, GPU vs the oracle at equal
@@ -422,51 +434,64 @@ def render_legs(rumi, ctx, args, log):
     legs = [("cornell_pt", scenes.cornell_box, 512, 512, 16, 5, "cornell64_pt_d5"),
             ("media_pt", scenes.cornell_media, 512, 512, 32, 16, "media64_pt_d16")]
     for name, factory, w, h, spp, depth, golden in legs:
+        if only and name != only:
+            continue
         sc = factory(w, h)
         scene = rumi.Scene(ctx, sc)
-        img, st = scene.render(w, h, spp, depth, seed=1)  # warm-up (allocates work buffers)
+        # queue sizes chosen by the native autotuner on this frame (untimed)
+        qtune, qtable = scene.autotune(w, h, spp, depth, seed=1)
+        img, st = scene.render(w, h, spp, depth, seed=1, **qtune)  # warm-up
         torch.cuda.synchronize()
         times = []
         for k in range(args.render_runs):
             t0 = time.perf_counter()
-            img, st = scene.render(w, h, spp, depth, seed=10 + k)
+            img, st = scene.render(w, h, spp, depth, seed=10 + k, **qtune)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
         sec = min(times)
         sps = st.samples / sec
-        # rel-MSE at equal spp against the committed 64^2 4096-spp reference:
-        # the GPU's renders here, the CPU's from the baseline leg below, with
-        # the same seeds (the two number their random draws alike, so each
-        # pair of renders holds the same samples up to rounding).
+        # rel-MSE at equal spp against the committed 64^2 4096-spp reference,
+        # GPU and CPU on disjoint seeds (independent renders): 32 GPU renders,
+        # the 4 timed one-thread CPU renders below and 20 more on all threads
         ref = np.load(os.path.join(ROOT, "tests", "golden", golden + ".npz"))
         sc64 = factory(64, 64)
         s64 = rumi.Scene(ctx, sc64)
         g_vals = []
-        for k in range(4):
+        for k in range(32):
             gi, _ = s64.render(64, 64, spp, depth, seed=200 + k)
             torch.cuda.synchronize()
             g_vals.append(rel_mse(gi.cpu().numpy(), ref["mean"]))
         s64.close()
         # CPU baseline leg: the oracle (FP64 restatement of PathTracer::Trace)
-        # on one thread, 4 renders of 64^2 at the same spp; their images give
-        # the CPU rel-MSE at equal spp.
+        # on one thread, 4 renders of 64^2 at the same spp
         import oracle
         c_vals = []
         t0 = time.perf_counter()
         for k in range(4):
-            ci, _ = oracle.render_pt(sc64, 64, 64, spp, depth, seed=200 + k, nthreads=1)
+            ci, _ = oracle.render_pt(sc64, 64, 64, spp, depth, seed=600 + k, nthreads=1)
             c_vals.append(rel_mse(ci, ref["mean"]))
         cpu_sec = time.perf_counter() - t0
         cpu_sps = 4 * 64 * 64 * spp / cpu_sec
+        threads = min(16, os.cpu_count() or 1)
+        for k in range(4, 24):
+            ci, _ = oracle.render_pt(sc64, 64, 64, spp, depth, seed=600 + k, nthreads=threads)
+            c_vals.append(rel_mse(ci, ref["mean"]))
+        ab = rel_mse_ab(g_vals, c_vals)
+        if name == "cornell_pt":
+            # the reference's own estimator (fixed pixel and light point sets,
+            # Q9), restated in the oracle
+            r_vals = [rel_mse(oracle.render_pt_refsampling(sc64, 64, 64, spp, depth, seed=900 + k,
+                                                           nthreads=threads)[0], ref["mean"])
+                      for k in range(24)]
+            ab["vs_reference_sampling"] = rel_mse_ab(g_vals, r_vals)
         out[name] = {
             "samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4),
             "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
             "max_depth": depth, "extension_rays": int(st.extension_rays),
             "shadow_rays": int(st.shadow_rays), "iterations": int(st.iterations),
             "rays_per_s": round((st.extension_rays + st.shadow_rays) / sec, 1),
-            "rel_mse_64px": {"gpu": float(np.mean(g_vals)), "cpu_oracle": float(np.mean(c_vals)),
-                             "ratio": float(np.mean(g_vals) / np.mean(c_vals)),
-                             "spp": spp, "renders": 4, "same_seeds": True},
+            "rel_mse_64px": dict(ab, spp=spp),
+            "queue_tuning": dict(qtune, candidates_ms=[None if t is None else round(t * 1e3, 2) for t in qtable]),
             "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
                              "kind": "port",
                              "sample": f"4 renders of 64x64 at {spp} spp, the oracle's FP64 "
@@ -474,8 +499,10 @@ def render_legs(rumi, ctx, args, log):
             "speedup_vs_cpu": round(sps / cpu_sps, 1),
         }
         log(f"{name}: {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.1f} ms), "
-            f"rel-MSE ratio {out[name]['rel_mse_64px']['ratio']:.3f}, CPU {cpu_sps:.0f} samples/s")
+            f"rel-MSE ratio {ab['ratio']:.3f} +- {ab['ratio_se']:.3f}, CPU {cpu_sps:.0f} samples/s")
         scene.close()
+    if only:
+        return out[only]
     out["cornell_whitted"] = whitted_leg(rumi, ctx, args, log, scenes)
     out["model_pt"] = model_leg(rumi, ctx, args, log, scenes)
     out["media_gradient"] = gradient_leg(rumi, ctx, args, log, scenes)
@@ -764,26 +791,26 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     sc = scenes.cornell_bdpt(w, h)
     scene = rumi.Scene(ctx, sc)
     kw = dict(integrator=rumi.RUMI_INTEGRATOR_BDPT, mis_beta=2.0, n_eye=9, n_light=9)
-    img, st = scene.render(64, 64, 1, max_len, seed=1, **kw)  # warm-up
-    torch.cuda.synchronize()
+    # samples per batch chosen by the native autotuner on this frame (untimed)
+    qtune, qtable = scene.autotune(w, h, spp, max_len, seed=1, **kw)
+    log(f"cornell_bdpt: batch sizes {qtable} -> {qtune}")
     times = []
     for k in range(max(1, args.render_runs - 1)):
         t0 = time.perf_counter()
-        img, st = scene.render(w, h, spp, max_len, seed=10 + k, **kw)
+        img, st = scene.render(w, h, spp, max_len, seed=10 + k, queue_capacity=qtune["queue_capacity"], **kw)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     sec = min(times)
     sps = st.samples / sec
     scene.close()
     # rel-MSE at equal spp against the committed 64^2 4096-spp oracle
-    # reference: the GPU's renders here, the CPU's from the baseline leg, with
-    # the same seeds (equal samples up to rounding)
+    # reference, GPU and CPU on disjoint seeds (independent renders)
     import numpy as np
     ref = np.load(os.path.join(ROOT, "tests", "golden", "bdpt64_l8.npz"))
     sc64 = scenes.cornell_bdpt(64, 64)
     s64 = rumi.Scene(ctx, sc64)
     g_vals = []
-    for k in range(6):
+    for k in range(24):
         gi, _ = s64.render(64, 64, spp, max_len, seed=300 + k, **kw)
         torch.cuda.synchronize()
         g_vals.append(rel_mse(gi.cpu().numpy(), ref["mean"]))
@@ -798,21 +825,35 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     cpu_sec = time.perf_counter() - t0
     cpu_sps = 32 * 32 * spp / cpu_sec
     c_vals = [rel_mse(oracle.render_bdpt(sc64, 64, 64, spp, max_len, n_eye=9, n_light=9,
-                                         mis_beta=2.0, seed=300 + k, nthreads=min(16, os.cpu_count() or 1)),
-                      ref["mean"]) for k in range(6)]
+                                         mis_beta=2.0, seed=700 + k, nthreads=min(16, os.cpu_count() or 1)),
+                      ref["mean"]) for k in range(8)]
+    ab = rel_mse_ab(g_vals, c_vals)
     log(f"cornell_bdpt: {sps / 1e6:.3f} Msamples/s ({sec * 1e3:.1f} ms), CPU {cpu_sps:.0f} samples/s, "
-        f"rel-MSE ratio {np.mean(g_vals) / np.mean(c_vals):.3f}")
+        f"rel-MSE ratio {ab['ratio']:.3f} +- {ab['ratio_se']:.3f}")
     return {"samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4),
             "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
             "max_path_length": max_len, "mis_beta": 2.0, "triangles": 200000 + 10,
-            "rel_mse_64px": {"gpu": float(np.mean(g_vals)), "cpu_oracle": float(np.mean(c_vals)),
-                             "ratio": float(np.mean(g_vals) / np.mean(c_vals)), "spp": spp,
-                             "renders": [6, 6], "same_seeds": True},
+            "rel_mse_64px": dict(ab, spp=spp),
+            "batch_tuning": dict(samples_per_batch=qtune["queue_capacity"],
+                                 candidates_ms=[None if t is None else round(t * 1e3, 1) for t in qtable]),
             "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
                              "kind": "port",
                              "sample": f"32x32 at {spp} spp, the oracle's FP64 bidirectional "
                                        f"tracer ({cpu_sec:.2f}s)"},
             "speedup_vs_cpu": round(sps / cpu_sps, 1)}
+
+
+def rel_mse_ab(g_vals, c_vals):
+    """Mean rel-MSE of independent GPU and CPU renders and their ratio with
+    its standard error (delta method)."""
+    import numpy as np
+    g, c = np.asarray(g_vals, float), np.asarray(c_vals, float)
+    gm, cm = g.mean(), c.mean()
+    gse, cse = g.std(ddof=1) / np.sqrt(len(g)), c.std(ddof=1) / np.sqrt(len(c))
+    r = gm / cm
+    return {"gpu": float(gm), "gpu_se": float(gse), "cpu_oracle": float(cm), "cpu_se": float(cse),
+            "ratio": float(r), "ratio_se": float(r * np.sqrt((gse / gm) ** 2 + (cse / cm) ** 2)),
+            "renders": [len(g), len(c)], "independent_seeds": True}
 
 
 def rel_mse(img, ref):

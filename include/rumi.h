@@ -326,7 +326,8 @@ typedef struct {
     int32_t n_light, n_eye; /* BidirectionalPathTracer(scene, nL, nE) vertex caps */
     float mis_beta;         /* BDPT MIS exponent: 1 balance (the reference's), 2 power */
     uint64_t seed;
-    uint32_t queue_capacity;    /* wavefront paths in flight, 0 = automatic */
+    uint32_t queue_capacity;    /* wavefront paths in flight (BDPT: samples per batch), 0 = automatic
+                                   (2^21 paths; BDPT 2^22 samples) */
     uint32_t regen_threshold;   /* refill the queue when fewer paths are alive, 0 = always */
 } rumi_render_desc;
 
@@ -367,6 +368,46 @@ int rumi_render(rumi_ctx* ctx, const rumi_scene* scene, const rumi_render_desc* 
 #define RUMI_NOALLOC_MAX_PATHS (1u << 18)
 int rumi_render_noalloc(rumi_ctx* ctx, const rumi_scene* scene, const rumi_render_desc* desc,
                         float* dev_rgb, rumi_render_stats* stats, void* stream);
+
+/* -------------------------------------------------------------- autotuner */
+/* The launch-shape autotuner (no counterpart in the reference, which has one
+ * code path per call): times candidate settings on the caller's own
+ * workload with HIP events and keeps the fastest.  Every candidate gives the
+ * same results (traversal: the same hit distances, the same bits within one
+ * node width; rendering: the same image up to the order of float atomics).
+ * Synchronous on stream. */
+typedef struct {
+    uint32_t n_candidates;        /* out: candidates timed */
+    float best_us;                /* out: the winner's median time */
+    float* host_candidate_us;     /* caller array or NULL: each candidate's median time in us
+                                     (-2: skipped as equal to the previous candidate) */
+    uint32_t candidates_capacity; /* entries host_candidate_us holds */
+    int32_t repeats;              /* timed runs per candidate after one untimed run (0: 3 for
+                                     traversal, 1 for rendering) */
+} rumi_autotune_report;
+
+/* The traversal candidates in the order rumi_autotune_trace times them:
+ * one ray per lane; prefetching lanes for width 2 and 4, blocks 64/128/256
+ * and LDS stacks 8/16; threshold refill for every block, stack and refill
+ * 16/32/48/64.  out may be NULL (count only). */
+int rumi_autotune_trace_candidates(rumi_trace_tuning* out, uint32_t capacity, uint32_t* count);
+
+/* Times rumi_trace_closest (any = 0; dev_out: n rumi_hit) or rumi_trace_any
+ * (any = 1; dev_out: n int32) on the caller's ray batch under every
+ * candidate and leaves the fastest set on the context (rumi_trace_set_tuning);
+ * best and report may be NULL.  The batch the reference would trace with
+ * BVHAccel::Travel (src/Acceleration/bvh.cpp:150-212) ray by ray. */
+int rumi_autotune_trace(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* dev_rays, uint32_t n,
+                        int32_t any, uint32_t flags, void* dev_out, rumi_trace_tuning* best,
+                        rumi_autotune_report* report, void* stream);
+
+/* Renders desc's frame with candidate queue sizes and writes the fastest
+ * into desc (path tracer and RayTracer: queue_capacity 2^19..2^22 with the
+ * refill threshold at the capacity or half of it; bidirectional tracer:
+ * samples per batch 2^20..2^23); dev_rgb receives the last render.  The
+ * frame of Scene::Rendering (src/Scene/scene.cpp:206-258). */
+int rumi_autotune_render(rumi_ctx* ctx, const rumi_scene* scene, rumi_render_desc* desc, float* dev_rgb,
+                         rumi_autotune_report* report, void* stream);
 
 /* ---------------------------------------------------------------- adjoint */
 /* Device buffers that rumi_adjoint overwrites; any may be NULL. */

@@ -105,6 +105,11 @@ class TraceTuning(Structure):
                 ("persistent", c_int32), ("width", c_int32)]
 
 
+class AutotuneReport(Structure):
+    _fields_ = [("n_candidates", c_uint32), ("best_us", c_float), ("host_candidate_us", c_void_p),
+                ("candidates_capacity", c_uint32), ("repeats", c_int32)]
+
+
 class BatchItem(Structure):
     _fields_ = [("scene", c_void_p), ("width", c_int32), ("height", c_int32)]
 
@@ -154,6 +159,11 @@ _SIGNATURES = {
                                  c_void_p, c_void_p, c_uint32, c_void_p]),
     "rumi_trace_set_tuning": (c_int, [c_void_p, POINTER(TraceTuning)]),
     "rumi_trace_get_tuning": (c_int, [c_void_p, POINTER(TraceTuning)]),
+    "rumi_autotune_trace_candidates": (c_int, [POINTER(TraceTuning), c_uint32, POINTER(c_uint32)]),
+    "rumi_autotune_trace": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_int32, c_uint32, c_void_p,
+                                    POINTER(TraceTuning), POINTER(AutotuneReport), c_void_p]),
+    "rumi_autotune_render": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
+                                     POINTER(AutotuneReport), c_void_p]),
     "rumi_scene_create": (c_int, [c_void_p, POINTER(SceneDesc), POINTER(c_void_p)]),
     "rumi_scene_destroy": (c_int, [c_void_p]),
     "rumi_render": (c_int, [c_void_p, c_void_p, POINTER(RenderDesc), c_void_p,
@@ -435,6 +445,33 @@ def trace_count(ctx: Context, bvh: Bvh, rays, any_hit: bool, hits, node_cnt, pri
                                  _stream(stream)))
 
 
+_TUNE_KEYS = ("block", "stack", "refill", "persistent", "width")
+
+
+def autotune_trace_candidates() -> list[dict]:
+    """rumi_autotune_trace_candidates: the traversal tunings the native
+    autotuner times, in its order."""
+    n = c_uint32()
+    check(lib().rumi_autotune_trace_candidates(None, 0, byref(n)))
+    arr = (TraceTuning * n.value)()
+    check(lib().rumi_autotune_trace_candidates(arr, n.value, byref(n)))
+    return [{k: int(getattr(t, k)) for k in _TUNE_KEYS} for t in arr]
+
+
+def autotune_trace(ctx, bvh, rays, out, any_hit: bool = False, flags: int = 0, repeats: int = 0,
+                   stream=None):
+    """rumi_autotune_trace: times every candidate on this batch and leaves
+    the fastest on ctx.  Returns (best tuning, [(tuning, seconds), ...])."""
+    cands = autotune_trace_candidates()
+    us = np.zeros(len(cands), np.float32)
+    rep = AutotuneReport(0, 0.0, us.ctypes.data, len(cands), repeats)
+    best = TraceTuning()
+    check(lib().rumi_autotune_trace(ctx.handle, bvh.handle, _ptr(rays), int(rays.shape[0]), int(any_hit),
+                                    flags, _ptr(out), byref(best), byref(rep), _stream(stream)))
+    return ({k: int(getattr(best, k)) for k in _TUNE_KEYS},
+            [(c, float(t) * 1e-6) for c, t in zip(cands, us[:rep.n_candidates])])
+
+
 def split_hits(hits_i32: np.ndarray):
     """(n, 2) int32 hit records -> (prim int32, t float32)."""
     h = np.ascontiguousarray(hits_i32, np.int32)
@@ -573,6 +610,22 @@ class Scene:
         check(lib().rumi_render(self.ctx.handle, self.handle, byref(rd), _ptr(out), byref(st),
                                 _stream(stream)))
         return out, st
+
+    def autotune(self, width, height, spp, max_depth, seed=0, integrator=RUMI_INTEGRATOR_PT,
+                 n_light=0, n_eye=0, mis_beta=1.0, repeats=0, out=None, stream=None):
+        """rumi_autotune_render: times the frame under the candidate queue
+        sizes.  Returns ({queue_capacity, regen_threshold}, [seconds per
+        candidate, None where skipped])."""
+        import torch
+        if out is None:
+            out = torch.empty((height, width, 3), dtype=torch.float32, device="cuda")
+        rd = RenderDesc(integrator, width, height, spp, max_depth, n_light, n_eye, mis_beta, seed, 0, 0)
+        us = np.zeros(16, np.float32)
+        rep = AutotuneReport(0, 0.0, us.ctypes.data, 16, repeats)
+        check(lib().rumi_autotune_render(self.ctx.handle, self.handle, byref(rd), _ptr(out), byref(rep),
+                                         _stream(stream)))
+        return (dict(queue_capacity=int(rd.queue_capacity), regen_threshold=int(rd.regen_threshold)),
+                [None if t < 0 else float(t) * 1e-6 for t in us[:rep.n_candidates]])
 
     def render_noalloc(self, width, height, spp, max_depth, out, seed=0, queue_capacity=0,
                        regen_threshold=0, integrator=RUMI_INTEGRATOR_PT, stream=None):

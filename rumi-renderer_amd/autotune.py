@@ -1,5 +1,12 @@
 """Autotuner for the launch shapes of the traversal and the wavefront renderer.
 
+The product's tuner is native (rumi_autotune_trace / rumi_autotune_render in
+the C ABI, bound as rumi.autotune_trace and Scene.autotune): it times the
+same traversal candidates as trace_candidates() below, in the same order.
+This module keeps the tuning policy with a pluggable clock (so tests can
+drive it with a fake one) and tunes what only the Python layer builds: the
+generation size of a batch (tune_batch).
+
 Traversal: threads per block, LDS stack entries per lane, the refill
 threshold of the persistent-lane kernels and the node width (binary or
 4-wide) of SAH trees (rumi_trace_set_tuning).  Renderer: the wavefront queue

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "scene.h"
@@ -35,7 +36,27 @@ using namespace rumi::dev;
 namespace {
 
 constexpr int kBlock = 128;
-constexpr int kStack = 16;
+
+// Traversal stack of the renderer's persistent stages (k_extend, k_shadow):
+// 16 LDS entries per lane and private-memory entries beyond them, chosen
+// per scene from its 4-wide tree's bound (3 entries per level, stack_need):
+// variant 0: 16 + 0, 1: 16 + 16, 2: 16 + 32, 3: 16 + 80.  The private array is
+// allocated for every lane whether it is reached or not, so it is kept to
+// what the bound needs.  (32 LDS entries cost more in occupancy than the
+// spills they save: the batch graph went from 734 to 864 ms.)
+constexpr int sv_lds(int) { return 16; }
+constexpr int sv_priv(int v) { return v == 1 ? 16 : (v == 2 ? 32 : (v == 3 ? kMaxDepth - 16 : 0)); }
+inline int stack_variant(int need) { return need <= 16 ? 0 : need <= 32 ? 1 : need <= 48 ? 2 : 3; }
+
+template <class F>
+void with_stack(int v, F&& f) {
+    switch (v) {
+    case 0: f(std::integral_constant<int, 0>{}); break;
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    default: f(std::integral_constant<int, 3>{}); break;
+    }
+}
 constexpr int kKeys = 8;     // material kinds 0..6, 7 = miss
 constexpr int kKeyMiss = 7;
 constexpr float kPiF = 3.14159265358979323846f;
@@ -315,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_regen(PathBuf buf, uint32_t start, u
 
 // Closest hit of every queued path (BVHAccel::Travel with the path's
 // in_object flag, bvh.cpp:150-213), persistent lanes (run_lanes).
-template <bool SPILL, bool MULTI>
+template <int SV, bool MULTI>
 struct ExtendItem {
     PathBuf buf;
     const SceneTab* tab;
@@ -350,7 +371,7 @@ struct ExtendItem {
     }
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
-        return trav_step<kBlock, kStack, false, true, true, false, SPILL, true>(
+        return trav_step<kBlock, sv_lds(SV), false, true, true, false, (sv_priv(SV) > 0), true>(
             nodes, prims, rects, s, in_obj, nn, np, s_stk, tid, p_stk);
     }
     __device__ void finish(const TravState& s) {
@@ -359,14 +380,13 @@ struct ExtendItem {
     }
 };
 
-template <bool SPILL, bool MULTI>
+template <int SV, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* n_ptr,
                                                    uint32_t* counter, SceneView sv,
                                                    const SceneTab* tab, int2* hits, float2* uvs) {
-    __shared__ int2 s_stk[kStack * kBlock];
-    constexpr int kSpill = SPILL ? kMaxDepth - kStack : 1;
-    int2 p_stk[kSpill];
-    ExtendItem<SPILL, MULTI> it;
+    __shared__ int2 s_stk[sv_lds(SV) * kBlock];
+    int2 p_stk[sv_priv(SV) > 0 ? sv_priv(SV) : 1];
+    ExtendItem<SV, MULTI> it;
     it.buf = buf;
     it.tab = tab;
     it.nodes = sv.nodes;
@@ -1382,7 +1402,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(PathBuf in, PathBuf out, 
 // Any-hit test of every shadow ray (Scene::VisibilityTest, scene.cpp:91-123),
 // persistent lanes; unoccluded contributions go to the film (the adjoint
 // passes: to the per-sample sums and the connection's own gradient terms).
-template <bool SPILL, int MODE, bool MULTI>
+template <int SV, int MODE, bool MULTI>
 struct ShadowItem {
     ShadowBuf sh;
     const SceneTab* tab;
@@ -1417,7 +1437,7 @@ struct ShadowItem {
     }
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
-        return trav_step<kBlock, kStack, true, true, true, false, SPILL, true>(
+        return trav_step<kBlock, sv_lds(SV), true, true, true, false, (sv_priv(SV) > 0), true>(
             nodes, prims, rects, s, false, nn, np, s_stk, tid, p_stk);
     }
     __device__ void finish(const TravState& s) {
@@ -1452,14 +1472,13 @@ struct ShadowItem {
     }
 };
 
-template <bool SPILL, int MODE, bool MULTI>
+template <int SV, int MODE, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t* n_ptr,
                                                    uint32_t* counter, SceneView sv,
                                                    const SceneTab* tab, float* film, AdjCtx adj) {
-    __shared__ int2 s_stk[kStack * kBlock];
-    constexpr int kSpill = SPILL ? kMaxDepth - kStack : 1;
-    int2 p_stk[kSpill];
-    ShadowItem<SPILL, MODE, MULTI> it;
+    __shared__ int2 s_stk[sv_lds(SV) * kBlock];
+    int2 p_stk[sv_priv(SV) > 0 ? sv_priv(SV) : 1];
+    ShadowItem<SV, MODE, MULTI> it;
     it.sh = sh;
     it.tab = tab;
     it.nodes = sv.nodes;
@@ -1662,300 +1681,375 @@ __device__ inline void store_vert(BVert* arr, float4* mis, uint32_t idx, V3 p, f
     mis[idx] = make_float4(pdf_fwd, bw, __int_as_float((int)flags), 0.0f);
 }
 
-// One random walk (eye: radiance transport; light: importance transport,
-// no 1/eta^2 on refraction) writing vertices start..; returns the count.
+// One segment of a walk (eye: radiance transport; light: importance
+// transport, no 1/eta^2 on refraction) given its closest hit (prim < 0: a
+// miss; t the hit distance): the medium along it, the vertex it reaches
+// (stored at st.nv) and the next direction.  Returns true when the walk
+// goes on with st's next segment.  The body of the reference's vertex loop
+// (bidirectionalpathtracer.cpp:396-711, :713-1016), one traversal apart.
+struct WalkState {
+    V3 o, d, beta, prev_p;
+    float pdf_w;
+    int nv;
+    bool in_obj;
+};
+
 template <bool EYE>
-__device__ int random_walk(const SceneView& sv, BVert* arr, float4* mis, uint32_t nb, uint32_t j, int start,
-                           int max_verts, V3 o, V3 d, V3 beta, float pdf_w, Rng& g,
-                           int2* s_stk, uint32_t tid, bool spill) {
-    int nv = start;
-    bool in_obj = false;
-    V3 prev_p = o;
-    while (nv < max_verts) {
-        Ray r{o.x, o.y, o.z, d.x, d.y, d.z, FLT_MAX, 0.0f};
-        float bu = 0.0f, bv = 0.0f;
-        int32_t nn = 0, np = 0;
-        const int32_t prim =
-            spill ? traverse<kBlock, kStack, false, true, true, false, true, true>(
-                        sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_stk, tid)
-                  : traverse<kBlock, kStack, false, true, true, false, false, true>(
-                        sv.nodes, sv.prims, sv.rects, sv.root, r, in_obj, bu, bv, nn, np, s_stk, tid);
-        // the medium along the segment, sampled as the path tracer does
-        // (pathtracer.cpp:156-178; bidirectionalpathtracer.cpp:452-473)
-        const DevMedium& md = sv.med;
-        float seg_L = 0.0f;
-        if (md.bound && !in_obj) {
-            const float thit = prim >= 0 ? r.tmin : 0.0f;
-            float tn, tf;
-            if (med_intersect(md, o, d, thit, tn, tf)) {
-                const float smax = med_maxdist(thit, tn, tf);
-                float sd = 0.0f;
-                V3 ratio;
-                const int ev = distance_sample(md.sigma_t, md.avg_sigma_t, md.absorb, smax, g, sd, ratio);
-                if (ev == kAbsorb) break;
-                beta = cmul(beta, ratio);
-                if (ev == kScatter) {
-                    // a medium vertex: HomogeneousVolume::PhaseSampling, Fp / pdf = sigma_s
-                    const V3 mp = o + d * (tn < 0.0f ? sd : tn + sd);
-                    const uint32_t mslot = (uint32_t)nv * nb + j;
-                    store_vert(arr, mis, mslot, mp, to_meas(md, pdf_w, prev_p, mp, d, true, sd), d, beta,
-                               kVMedium, v3(1, 1, 1), -1);
-                    ++nv;
-                    float pdf_ph;
-                    const V3 w = phase_sample(md.g, d, g, pdf_ph);
-                    if (nv >= 2) {
-                        const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
-                        BVert& pv = arr[pi];
-                        pv.b.w = to_meas(md, pdf_ph, mp, pv, sd);
-                        mis[pi].y = pv.b.w;
-                    }
-                    beta = cmul(beta, ld3(md.sigma_s));
-                    pdf_w = pdf_ph;
-                    prev_p = mp;
-                    o = mp;
-                    d = w;
-                    in_obj = false;
-                    continue;
-                }
-                seg_L = smax;
-            }
-        }
-        if (prim < 0) break;
-        V3 p, n, tex;
-        surface_at(sv, prim, o, d, r.tmin, bu, bv, p, n, tex);
-        const int mi = sv.shapes[prim].info[1];
-        const DevMaterial& m = sv.mats[mi];
-        V3 vol = v3(1, 1, 1);  // subsurface pass-through tau / pdf at the exit boundary
-        if (m.kind == RUMI_MAT_SUBSURFACE && in_obj) {
-            // Subsurface::PhaseSampling (subsurface.cpp:135-155): the interior
-            // walk over the segment to the boundary, as the path tracer takes
-            // it.  Interior vertices are never connected (a connection from
-            // inside ends on the boundary), so they enter the MIS walk like
-            // Dirac vertices.
+__device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint32_t nb, uint32_t j,
+                             WalkState& st, int32_t prim, float t, float bu, float bv, Rng& g) {
+    int& nv = st.nv;
+    bool& in_obj = st.in_obj;
+    V3& prev_p = st.prev_p;
+    V3& o = st.o;
+    V3& d = st.d;
+    V3& beta = st.beta;
+    float& pdf_w = st.pdf_w;
+    // the medium along the segment, sampled as the path tracer does
+    // (pathtracer.cpp:156-178; bidirectionalpathtracer.cpp:452-473)
+    const DevMedium& md = sv.med;
+    float seg_L = 0.0f;
+    if (md.bound && !in_obj) {
+        const float thit = prim >= 0 ? t : 0.0f;
+        float tn, tf;
+        if (med_intersect(md, o, d, thit, tn, tf)) {
+            const float smax = med_maxdist(thit, tn, tf);
             float sd = 0.0f;
             V3 ratio;
-            const int ev = distance_sample(m.sigma_t, m.avg_sigmat, m.absorb, r.tmin, g, sd, ratio);
-            if (ev == kAbsorb) break;
+            const int ev = distance_sample(md.sigma_t, md.avg_sigma_t, md.absorb, smax, g, sd, ratio);
+            if (ev == kAbsorb) return false;
+            beta = cmul(beta, ratio);
             if (ev == kScatter) {
-                const V3 ps = o + d * sd;
-                int tix;
-                const V3 txs = sv.shapes[prim].info[2] >= 0 ? pattern(sv, sv.shapes[prim], ps, bu, bv, tix)
-                                                           : v3(1, 1, 1);
-                store_vert(arr, mis, (uint32_t)nv * nb + j, ps, 0.0f, d, beta, kVDelta, txs, mi);
+                // a medium vertex: HomogeneousVolume::PhaseSampling, Fp / pdf = sigma_s
+                const V3 mp = o + d * (tn < 0.0f ? sd : tn + sd);
+                const uint32_t mslot = (uint32_t)nv * nb + j;
+                store_vert(arr, mis, mslot, mp, to_meas(md, pdf_w, prev_p, mp, d, true, sd), d, beta,
+                           kVMedium, v3(1, 1, 1), -1);
                 ++nv;
+                float pdf_ph;
+                const V3 w = phase_sample(md.g, d, g, pdf_ph);
                 if (nv >= 2) {
                     const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
-                    arr[pi].b.w = 0.0f;
-                    mis[pi].y = 0.0f;
+                    BVert& pv = arr[pi];
+                    pv.b.w = to_meas(md, pdf_ph, mp, pv, sd);
+                    mis[pi].y = pv.b.w;
                 }
-                beta = cmul(cmul(cmul(beta, ratio), ld3(m.sigma_s)), txs);
-                float pdf_ph;
-                const V3 w = phase_sample(0.0f, d, g, pdf_ph);
-                pdf_w = 0.0f;
-                prev_p = ps;
-                o = ps;
+                beta = cmul(beta, ld3(md.sigma_s));
+                pdf_w = pdf_ph;
+                prev_p = mp;
+                o = mp;
                 d = w;
-                continue;
+                in_obj = false;
+                return true;
             }
-            vol = ratio;
+            seg_L = smax;
         }
-        const float pdf_fwd = to_meas(md, pdf_w, prev_p, p, n, false, seg_L);
-        if (m.kind == RUMI_MAT_EMISSIVE) {
-            // The walk ends on an emitter (bidirectionalpathtracer.cpp:693-699).
-            // an emitter vertex is never scattered from, so its reverse-pdf
-            // word carries the primitive id instead
-            store_vert(arr, mis, (uint32_t)nv * nb + j, p, pdf_fwd, n, beta, kVEmitter, tex, mi,
-                       __int_as_float(prim));
+    }
+    if (prim < 0) return false;
+    V3 p, n, tex;
+    surface_at(sv, prim, o, d, t, bu, bv, p, n, tex);
+    const int mi = sv.shapes[prim].info[1];
+    const DevMaterial& m = sv.mats[mi];
+    V3 vol = v3(1, 1, 1);  // subsurface pass-through tau / pdf at the exit boundary
+    if (m.kind == RUMI_MAT_SUBSURFACE && in_obj) {
+        // Subsurface::PhaseSampling (subsurface.cpp:135-155): the interior
+        // walk over the segment to the boundary, as the path tracer takes
+        // it.  Interior vertices are never connected (a connection from
+        // inside ends on the boundary), so they enter the MIS walk like
+        // Dirac vertices.
+        float sd = 0.0f;
+        V3 ratio;
+        const int ev = distance_sample(m.sigma_t, m.avg_sigmat, m.absorb, t, g, sd, ratio);
+        if (ev == kAbsorb) return false;
+        if (ev == kScatter) {
+            const V3 ps = o + d * sd;
+            int tix;
+            const V3 txs = sv.shapes[prim].info[2] >= 0 ? pattern(sv, sv.shapes[prim], ps, bu, bv, tix)
+                                                       : v3(1, 1, 1);
+            store_vert(arr, mis, (uint32_t)nv * nb + j, ps, 0.0f, d, beta, kVDelta, txs, mi);
             ++nv;
-            break;
+            if (nv >= 2) {
+                const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
+                arr[pi].b.w = 0.0f;
+                mis[pi].y = 0.0f;
+            }
+            beta = cmul(cmul(cmul(beta, ratio), ld3(m.sigma_s)), txs);
+            float pdf_ph;
+            const V3 w = phase_sample(0.0f, d, g, pdf_ph);
+            pdf_w = 0.0f;
+            prev_p = ps;
+            o = ps;
+            d = w;
+            return true;
         }
-        const uint32_t slot = (uint32_t)nv * nb + j;
+        vol = ratio;
+    }
+    const float pdf_fwd = to_meas(md, pdf_w, prev_p, p, n, false, seg_L);
+    if (m.kind == RUMI_MAT_EMISSIVE) {
+        // The walk ends on an emitter (bidirectionalpathtracer.cpp:693-699).
+        // an emitter vertex is never scattered from, so its reverse-pdf
+        // word carries the primitive id instead
+        store_vert(arr, mis, (uint32_t)nv * nb + j, p, pdf_fwd, n, beta, kVEmitter, tex, mi,
+                   __int_as_float(prim));
         ++nv;
-        const bool was_in = in_obj;
-        const V3 wo = -d;
-        V3 w;
-        float pdf_next = 0.0f, pdf_rev = 0.0f;
-        bool delta = false;
-        uint32_t vflags = 0u;
-        V3 weight;  // f |cos| / pdf of this bounce
-        if (m.kind == RUMI_MAT_MATTE) {
+        return false;
+    }
+    const uint32_t slot = (uint32_t)nv * nb + j;
+    ++nv;
+    const bool was_in = in_obj;
+    const V3 wo = -d;
+    V3 w;
+    float pdf_next = 0.0f, pdf_rev = 0.0f;
+    bool delta = false;
+    uint32_t vflags = 0u;
+    V3 weight;  // f |cos| / pdf of this bounce
+    if (m.kind == RUMI_MAT_MATTE) {
+        float cost, pdf;
+        w = lambert_sample(n, g, cost, pdf);
+        weight = cmul(ld3(m.color), tex) * (m.kd * kInvPiF * cost / pdf);
+        pdf_next = pdf;
+        const float pr = kInvPiF * dot(n, wo);
+        pdf_rev = pr <= 0.0f ? 1e-4f : pr;
+    } else if (m.kind == RUMI_MAT_GLOSSY) {
+        if (m.kd != 0.0f && g.uniform() <= m.kd_pdf) {
             float cost, pdf;
             w = lambert_sample(n, g, cost, pdf);
-            weight = cmul(ld3(m.color), tex) * (m.kd * kInvPiF * cost / pdf);
+            weight = cmul(ld3(m.color), tex) * (m.kd * kInvPiF * cost / (pdf * m.kd_pdf));
             pdf_next = pdf;
             const float pr = kInvPiF * dot(n, wo);
             pdf_rev = pr <= 0.0f ? 1e-4f : pr;
-        } else if (m.kind == RUMI_MAT_GLOSSY) {
-            if (m.kd != 0.0f && g.uniform() <= m.kd_pdf) {
-                float cost, pdf;
-                w = lambert_sample(n, g, cost, pdf);
-                weight = cmul(ld3(m.color), tex) * (m.kd * kInvPiF * cost / (pdf * m.kd_pdf));
-                pdf_next = pdf;
-                const float pr = kInvPiF * dot(n, wo);
-                pdf_rev = pr <= 0.0f ? 1e-4f : pr;
-            } else {
-                if (!EYE) {  // a light walk ends at a Phong-lobe vertex
-                    store_vert(arr, mis, slot, p, pdf_fwd, n, beta, kVDelta | kVPhong, tex, mi);
-                    break;
-                }
-                float cost, pdf;
-                w = phong_sample(m, wo, n, g, cost, pdf);
-                weight = tex * (phong_f(m, n, w, wo) * m.ks * cost / (pdf * (1.0f - m.kd_pdf)));
-                delta = true;
-                vflags = kVPhong;
+        } else {
+            if (!EYE) {  // a light walk ends at a Phong-lobe vertex
+                store_vert(arr, mis, slot, p, pdf_fwd, n, beta, kVDelta | kVPhong, tex, mi);
+                return false;
             }
-        } else if (m.kind == RUMI_MAT_GGX) {
-            w = ggx_sample(m, n, wo, g);
-            const float cost = dot(n, w);
-            pdf_next = ggx_pdf(m, n, w, wo);
-            if (!(cost > 0.0f && pdf_next > 0.0f)) {
-                store_vert(arr, mis, slot, p, pdf_fwd, n, beta, 0u, tex, mi);
-                break;
-            }
-            weight = cmul(ggx_f(m, n, w, wo), tex) * (cost / pdf_next);
-            pdf_rev = ggx_pdf(m, n, wo, w);
-        } else if (m.kind == RUMI_MAT_SPECULAR) {
-            float won = dot(wo, n);
-            if (won <= 0.0f) won = 1e-4f;
-            w = -wo + n * (2.0f * won);
-            weight = cmul(ld3(m.color), tex) * m.kd;
+            float cost, pdf;
+            w = phong_sample(m, wo, n, g, cost, pdf);
+            weight = tex * (phong_f(m, n, w, wo) * m.ks * cost / (pdf * (1.0f - m.kd_pdf)));
             delta = true;
-        } else {  // dielectric, subsurface boundary
-            const float kr = fresnel(-wo, n, m.eta_in, m.eta_out);
-            const float c = dot(wo, n);
-            delta = true;
-            if (kr != 0.0f && g.uniform() <= kr) {
-                float won = c;
-                V3 nn = n;
-                if (won < 1e-4f) {
-                    if (won > -1e-4f)
-                        won = 1e-4f;
-                    else {
-                        nn = -n;
-                        won = -won;
-                    }
-                }
-                w = -wo + nn * (2.0f * won);
-                in_obj = c < 0.0f;
-                weight = v3(1, 1, 1);
-            } else {
-                float eta = m.eta_in / m.eta_out;
-                V3 nn = n;
-                float ci = c;
-                if (ci < 0.0f) {
+            vflags = kVPhong;
+        }
+    } else if (m.kind == RUMI_MAT_GGX) {
+        w = ggx_sample(m, n, wo, g);
+        const float cost = dot(n, w);
+        pdf_next = ggx_pdf(m, n, w, wo);
+        if (!(cost > 0.0f && pdf_next > 0.0f)) {
+            store_vert(arr, mis, slot, p, pdf_fwd, n, beta, 0u, tex, mi);
+            return false;
+        }
+        weight = cmul(ggx_f(m, n, w, wo), tex) * (cost / pdf_next);
+        pdf_rev = ggx_pdf(m, n, wo, w);
+    } else if (m.kind == RUMI_MAT_SPECULAR) {
+        float won = dot(wo, n);
+        if (won <= 0.0f) won = 1e-4f;
+        w = -wo + n * (2.0f * won);
+        weight = cmul(ld3(m.color), tex) * m.kd;
+        delta = true;
+    } else {  // dielectric, subsurface boundary
+        const float kr = fresnel(-wo, n, m.eta_in, m.eta_out);
+        const float c = dot(wo, n);
+        delta = true;
+        if (kr != 0.0f && g.uniform() <= kr) {
+            float won = c;
+            V3 nn = n;
+            if (won < 1e-4f) {
+                if (won > -1e-4f)
+                    won = 1e-4f;
+                else {
                     nn = -n;
-                    eta = m.eta_out / m.eta_in;
-                    ci = -ci;
+                    won = -won;
                 }
-                const float inv = 1.0f / eta;
-                const float ct = sqrtf(1.0f - (1.0f - ci * ci) * (inv * inv));
-                w = -wo * inv + nn * (ci * inv - ct);
-                in_obj = c > 0.0f;
-                // Dielectric::Fr (dielectric.cpp:73-98): radiance is scaled by
-                // 1/eta^2, eta = eta_in/eta_out from inside, eta_out/eta_in from
-                // outside; importance carried by light paths is not scaled.
-                const float er = was_in ? m.eta_in / m.eta_out : m.eta_out / m.eta_in;
-                const float scale = EYE ? 1.0f / (er * er) : 1.0f;
-                weight = v3(scale, scale, scale);
             }
-            weight = cmul(weight, tex);
-            // Beer-Lambert along a segment travelled inside (dielectric.cpp:94-95);
-            // a subsurface interior passed through without scattering
-            if (was_in && m.kind == RUMI_MAT_DIELECTRIC)
-                weight = cmul(weight, v3(expf(-m.absorption[0] * r.tmin), expf(-m.absorption[1] * r.tmin),
-                                         expf(-m.absorption[2] * r.tmin)));
-            if (was_in && m.kind == RUMI_MAT_SUBSURFACE) weight = cmul(weight, vol);
+            w = -wo + nn * (2.0f * won);
+            in_obj = c < 0.0f;
+            weight = v3(1, 1, 1);
+        } else {
+            float eta = m.eta_in / m.eta_out;
+            V3 nn = n;
+            float ci = c;
+            if (ci < 0.0f) {
+                nn = -n;
+                eta = m.eta_out / m.eta_in;
+                ci = -ci;
+            }
+            const float inv = 1.0f / eta;
+            const float ct = sqrtf(1.0f - (1.0f - ci * ci) * (inv * inv));
+            w = -wo * inv + nn * (ci * inv - ct);
+            in_obj = c > 0.0f;
+            // Dielectric::Fr (dielectric.cpp:73-98): radiance is scaled by
+            // 1/eta^2, eta = eta_in/eta_out from inside, eta_out/eta_in from
+            // outside; importance carried by light paths is not scaled.
+            const float er = was_in ? m.eta_in / m.eta_out : m.eta_out / m.eta_in;
+            const float scale = EYE ? 1.0f / (er * er) : 1.0f;
+            weight = v3(scale, scale, scale);
         }
-        store_vert(arr, mis, slot, p, pdf_fwd, n, beta, vflags | (delta ? kVDelta : 0u), tex, mi);
-        // reverse pdf of the previous vertex (area measure at prev)
-        if (nv >= 2) {
-            const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
-            BVert& pv = arr[pi];
-            pv.b.w = delta ? 0.0f : to_meas(md, pdf_rev, p, pv, seg_L);
-            mis[pi].y = pv.b.w;
-        }
-        beta = cmul(beta, weight);
-        pdf_w = delta ? 0.0f : pdf_next;
-        prev_p = p;
-        o = p;
-        d = w;
-        if (m.kind != RUMI_MAT_DIELECTRIC && m.kind != RUMI_MAT_SUBSURFACE) in_obj = false;
+        weight = cmul(weight, tex);
+        // Beer-Lambert along a segment travelled inside (dielectric.cpp:94-95);
+        // a subsurface interior passed through without scattering
+        if (was_in && m.kind == RUMI_MAT_DIELECTRIC)
+            weight = cmul(weight, v3(expf(-m.absorption[0] * t), expf(-m.absorption[1] * t),
+                                     expf(-m.absorption[2] * t)));
+        if (was_in && m.kind == RUMI_MAT_SUBSURFACE) weight = cmul(weight, vol);
     }
-    return nv;
+    store_vert(arr, mis, slot, p, pdf_fwd, n, beta, vflags | (delta ? kVDelta : 0u), tex, mi);
+    // reverse pdf of the previous vertex (area measure at prev)
+    if (nv >= 2) {
+        const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
+        BVert& pv = arr[pi];
+        pv.b.w = delta ? 0.0f : to_meas(md, pdf_rev, p, pv, seg_L);
+        mis[pi].y = pv.b.w;
+    }
+    beta = cmul(beta, weight);
+    pdf_w = delta ? 0.0f : pdf_next;
+    prev_p = p;
+    o = p;
+    d = w;
+    if (m.kind != RUMI_MAT_DIELECTRIC && m.kind != RUMI_MAT_SUBSURFACE) in_obj = false;
+    return true;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bdpt_eye(BdptBufs bb, uint64_t base, BdptParams bp,
-                                                     SceneView sv, bool spill) {
-    __shared__ int2 s_stk[kStack * kBlock];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t j = blockIdx.x * kBlock + tid;
-    if (j >= bb.nb) return;
+// Walk state of the wavefront BDPT walks: eye and light walks of a batch in
+// one queue, double-buffered.  ray and flags are laid out as PathBuf's, so
+// the path tracer's persistent-lane extension stage (k_extend) traces them.
+struct WalkBuf {
+    float4* ray;       // 2 per walk: o.xyz 0 | d.xyz FLT_MAX
+    uint32_t* flags;   // bit 1 in_object, bit 2 light walk, bits 8.. vertices stored
+    float4* beta;      // throughput rgb, pdf_w of the segment's direction
+    float4* prev;      // the previous vertex's position
+    uint64_t* rng;     // PCG32 state (increment from the pixel)
+    uint32_t* slot;    // sample index within the batch
+};
+
+constexpr uint32_t kWalkLight = 4u;
+
+__device__ inline void push_walk(const WalkBuf& wb, uint32_t* count, bool pred, const WalkState& st,
+                                 uint64_t rng, uint32_t light, uint32_t j) {
+    const uint32_t q = wave_append(count, pred);
+    if (!pred) return;
+    wb.ray[2 * (size_t)q + 0] = make_float4(st.o.x, st.o.y, st.o.z, 0.0f);
+    wb.ray[2 * (size_t)q + 1] = make_float4(st.d.x, st.d.y, st.d.z, FLT_MAX);
+    wb.flags[q] = (st.in_obj ? 2u : 0u) | light | ((uint32_t)st.nv << 8);
+    wb.beta[q] = make_float4(st.beta.x, st.beta.y, st.beta.z, st.pdf_w);
+    wb.prev[q] = make_float4(st.prev_p.x, st.prev_p.y, st.prev_p.z, 0.0f);
+    wb.rng[q] = rng;
+    wb.slot[q] = j;
+}
+
+// The first vertex of every walk of the batch (thread w < nb: the eye walk
+// of sample w from the camera, ConstructViewPath :396-420; w >= nb: the
+// light walk of sample w - nb from a light point with a cosine-weighted
+// direction, Scene::LightsUniformSampling scene.cpp:91-97 +
+// AreaLight::SolidAngleSampling arealight.cpp:31-60), queued for the walk
+// stages.
+__global__ __launch_bounds__(kBlock) void k_bdpt_init(BdptBufs bb, uint64_t base, BdptParams bp,
+                                                      SceneView sv, WalkBuf wb, uint32_t* count) {
+    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+    if (w >= 2 * bb.nb) return;
     const Params& prm = bp.base;
+    const bool eye = w < bb.nb;
+    const uint32_t j = eye ? w : w - bb.nb;
     const uint64_t k = base + j;
     const uint32_t pixel = (uint32_t)(k % prm.pixels);
     const uint64_t s = k / prm.pixels;
-    Rng g = rng_seed(prm.seed + s * 0x9E3779B97F4A7C15ull, pixel);
-    const DevCamera& cam = sv.cam;
-    const int x = (int)(pixel % (uint32_t)prm.width);
-    const int y = (int)(pixel / (uint32_t)prm.width);
-    const float sx = g.uniform(), sy = g.uniform();
-    const float hw = prm.width * 0.5f, hh = prm.height * 0.5f;
-    const float px = -(((float)x - hw) + sx) / hw * cam.tanfov;
-    const float py = -(((float)y - hh) + sy) / hh * cam.tanfov * cam.y_ratio;
-    const V3 l = normalize(v3(px, py, 1.0f));
-    const V3 d = v3(cam.R[0] * l.x + cam.R[1] * l.y + cam.R[2] * l.z,
-                    cam.R[3] * l.x + cam.R[4] * l.y + cam.R[5] * l.z,
-                    cam.R[6] * l.x + cam.R[7] * l.y + cam.R[8] * l.z);
-    const V3 cp = v3(cam.p[0], cam.p[1], cam.p[2]);
-    const V3 cdir = v3(cam.R[2], cam.R[5], cam.R[8]);
-    store_vert(bb.eye, bb.eye_mis, j, cp, 1.0f, cdir, v3(1, 1, 1), kVCamera, v3(1, 1, 1), -1);
-    // Pinhole::Pdfw: 1 / (cos^3 film_area) at plane distance 1
-    const float pdf_dir = 1.0f / (l.z * l.z * l.z * bp.film_area);
-    const int nv = random_walk<true>(sv, bb.eye, bb.eye_mis, bb.nb, j, 1, bp.n_eye, cp, d, v3(1, 1, 1), pdf_dir,
-                                     g, s_stk, tid, spill);
-    bb.n_eye[j] = (uint8_t)nv;
-}
-
-__global__ __launch_bounds__(kBlock) void k_bdpt_light(BdptBufs bb, uint64_t base, BdptParams bp,
-                                                       SceneView sv, bool spill) {
-    __shared__ int2 s_stk[kStack * kBlock];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t j = blockIdx.x * kBlock + tid;
-    if (j >= bb.nb) return;
-    const Params& prm = bp.base;
-    const uint64_t k = base + j;
-    const uint32_t pixel = (uint32_t)(k % prm.pixels);
-    const uint64_t s = k / prm.pixels;
-    Rng g = rng_seed(bp.light_seed + s * 0x9E3779B97F4A7C15ull, pixel);
-    if (sv.n_lights == 0) {
-        bb.n_light[j] = 0;
-        return;
-    }
-    // Scene::LightsUniformSampling (scene.cpp:91-97) + AreaLight::SolidAngleSampling
-    // (arealight.cpp:31-60): a point on the emitter, a cosine-weighted direction.
-    uint32_t li = (uint32_t)(g.uniform() * (float)sv.n_lights);
-    if (li >= sv.n_lights) li = sv.n_lights - 1;
-    const DevLight& L = sv.lights[li];
-    V3 p0, n0;
-    if (L.kind == RUMI_SHAPE_RECT) {
-        const float a = g.uniform(), b = g.uniform();
-        p0 = (ld3(L.p) + ld3(L.l) * a) + ld3(L.w) * b;
-        n0 = ld3(L.n);
+    WalkState st;
+    st.nv = 1;
+    st.in_obj = false;
+    bool go = false;
+    Rng g;
+    if (eye) {
+        g = rng_seed(prm.seed + s * 0x9E3779B97F4A7C15ull, pixel);
+        const DevCamera& cam = sv.cam;
+        const int x = (int)(pixel % (uint32_t)prm.width);
+        const int y = (int)(pixel / (uint32_t)prm.width);
+        const float sx = g.uniform(), sy = g.uniform();
+        const float hw = prm.width * 0.5f, hh = prm.height * 0.5f;
+        const float px = -(((float)x - hw) + sx) / hw * cam.tanfov;
+        const float py = -(((float)y - hh) + sy) / hh * cam.tanfov * cam.y_ratio;
+        const V3 l = normalize(v3(px, py, 1.0f));
+        st.d = v3(cam.R[0] * l.x + cam.R[1] * l.y + cam.R[2] * l.z, cam.R[3] * l.x + cam.R[4] * l.y + cam.R[5] * l.z,
+                  cam.R[6] * l.x + cam.R[7] * l.y + cam.R[8] * l.z);
+        const V3 cp = v3(cam.p[0], cam.p[1], cam.p[2]);
+        const V3 cdir = v3(cam.R[2], cam.R[5], cam.R[8]);
+        store_vert(bb.eye, bb.eye_mis, j, cp, 1.0f, cdir, v3(1, 1, 1), kVCamera, v3(1, 1, 1), -1);
+        // Pinhole::Pdfw: 1 / (cos^3 film_area) at plane distance 1
+        st.pdf_w = 1.0f / (l.z * l.z * l.z * bp.film_area);
+        st.o = st.prev_p = cp;
+        st.beta = v3(1, 1, 1);
+        bb.n_eye[j] = 1;
+        go = bp.n_eye > 1;
     } else {
-        const float z = 1.0f - 2.0f * g.uniform(), phi = 2.0f * kPiF * g.uniform();
-        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-        p0 = ld3(L.p) + v3(rr * cosf(phi), rr * sinf(phi), z) * L.l[0];
-        n0 = normalize(p0 - ld3(L.p));
+        g = rng_seed(bp.light_seed + s * 0x9E3779B97F4A7C15ull, pixel);
+        if (sv.n_lights == 0) {
+            bb.n_light[j] = 0;
+        } else {
+            uint32_t li = (uint32_t)(g.uniform() * (float)sv.n_lights);
+            if (li >= sv.n_lights) li = sv.n_lights - 1;
+            const DevLight& L = sv.lights[li];
+            V3 p0, n0;
+            if (L.kind == RUMI_SHAPE_RECT) {
+                const float a = g.uniform(), b = g.uniform();
+                p0 = (ld3(L.p) + ld3(L.l) * a) + ld3(L.w) * b;
+                n0 = ld3(L.n);
+            } else {
+                const float z = 1.0f - 2.0f * g.uniform(), phi = 2.0f * kPiF * g.uniform();
+                const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+                p0 = ld3(L.p) + v3(rr * cosf(phi), rr * sinf(phi), z) * L.l[0];
+                n0 = normalize(p0 - ld3(L.p));
+            }
+            const float pdf_pos = 1.0f / (float)sv.n_lights * L.p[3];
+            const V3 le = ld3(L.le);
+            store_vert(bb.light, bb.light_mis, j, p0, pdf_pos, n0, le * (1.0f / pdf_pos), kVLight, v3(1, 1, 1),
+                       -1);
+            float cost, pdf_dir;
+            st.d = lambert_sample(n0, g, cost, pdf_dir);
+            st.beta = le * (cost / (pdf_pos * pdf_dir));
+            st.pdf_w = pdf_dir;
+            st.o = st.prev_p = p0;
+            bb.n_light[j] = 1;
+            go = bp.n_light > 1;
+        }
     }
-    const float pdf_pos = 1.0f / (float)sv.n_lights * L.p[3];
-    const V3 le = ld3(L.le);
-    store_vert(bb.light, bb.light_mis, j, p0, pdf_pos, n0, le * (1.0f / pdf_pos), kVLight, v3(1, 1, 1), -1);
-    float cost, pdf_dir;
-    const V3 w = lambert_sample(n0, g, cost, pdf_dir);
-    const V3 beta = le * (cost / (pdf_pos * pdf_dir));
-    const int nv = random_walk<false>(sv, bb.light, bb.light_mis, bb.nb, j, 1, bp.n_light, p0, w, beta, pdf_dir, g,
-                                      s_stk, tid, spill);
-    bb.n_light[j] = (uint8_t)nv;
+    push_walk(wb, count, go, st, g.state, eye ? 0u : kWalkLight, j);
+}
+
+// One segment of every queued walk, its closest hit already found by
+// k_extend; walks that go on are appended to the next queue, the others
+// record their vertex count.
+__global__ __launch_bounds__(kBlock) void k_bdpt_walk(BdptBufs bb, uint64_t base, BdptParams bp, SceneView sv,
+                                                      WalkBuf in, const uint32_t* n_ptr, const int2* hits,
+                                                      const float2* uvs, WalkBuf out, uint32_t* next_count) {
+    const uint32_t n = *n_ptr;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint32_t flags = in.flags[i];
+        const uint32_t j = in.slot[i];
+        const bool light = (flags & kWalkLight) != 0;
+        const uint32_t pixel = (uint32_t)((base + j) % bp.base.pixels);
+        Rng g;
+        g.state = in.rng[i];
+        g.inc = ((uint64_t)pixel << 1u) | 1u;
+        WalkState st;
+        const float4 r0 = in.ray[2 * (size_t)i + 0], r1 = in.ray[2 * (size_t)i + 1];
+        const float4 b4 = in.beta[i], p4 = in.prev[i];
+        st.o = v3(r0.x, r0.y, r0.z);
+        st.d = v3(r1.x, r1.y, r1.z);
+        st.beta = v3(b4.x, b4.y, b4.z);
+        st.pdf_w = b4.w;
+        st.prev_p = v3(p4.x, p4.y, p4.z);
+        st.nv = (int)(flags >> 8);
+        st.in_obj = (flags & 2u) != 0;
+        const int2 h = hits[i];
+        const float2 uv = uvs[i];
+        const float t = __int_as_float(h.y);
+        bool go = light ? walk_segment<false>(sv, bb.light, bb.light_mis, bb.nb, j, st, h.x, t, uv.x, uv.y, g)
+                        : walk_segment<true>(sv, bb.eye, bb.eye_mis, bb.nb, j, st, h.x, t, uv.x, uv.y, g);
+        go = go && st.nv < (light ? bp.n_light : bp.n_eye);
+        if (!go) {
+            if (light)
+                bb.n_light[j] = (uint8_t)st.nv;
+            else
+                bb.n_eye[j] = (uint8_t)st.nv;
+        }
+        push_walk(out, next_count, go, st, g.state, light ? kWalkLight : 0u, j);
+    }
 }
 
 __device__ inline float remap0(float f) { return f != 0.0f ? f : 1.0f; }
@@ -2137,6 +2231,12 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
 }
 
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+// Clears the walk queue the next iteration appends to and k_extend's counter.
+__global__ void k_walk_ctl(uint32_t* wctl, int next) {
+    if (threadIdx.x == 0) wctl[next] = 0u;
+    if (threadIdx.x == 1) wctl[2] = 0u;
+}
 SceneView scene_view(const rumi_scene* sc) {
     SceneView sv;
     sv.nodes = (const float4*)sc->nodes;
@@ -2177,7 +2277,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
         }
     const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
     const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
-    uint32_t nb = desc->queue_capacity ? desc->queue_capacity : (1u << 19);
+    uint32_t nb = desc->queue_capacity ? desc->queue_capacity : (1u << 22);
     if ((uint64_t)nb > total) nb = (uint32_t)total;
     const size_t vbytes = sizeof(BVert) * (size_t)nb;
     const size_t mbytes = sizeof(float4) * (size_t)nb;
@@ -2186,8 +2286,11 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     const uint32_t kConnChunk = std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)strat.size(),
                                                                            (1u << 25) / nb));
     const size_t max_conn = (size_t)kConnChunk * nb;
+    // walks: eye and light walks of a batch in one double-buffered queue
+    // (80 B per walk and buffer) with their closest hits (16 B)
+    const size_t walks = 2 * (size_t)nb;
     const size_t need = (vbytes + mbytes) * (size_t)(nE + nL) + 2 * (size_t)nb + sizeof(int2) * strat.size() +
-                        3 * sizeof(float4) * max_conn + 8192;
+                        3 * sizeof(float4) * max_conn + walks * (2 * 80 + 16) + 32 * 256;
     if (sc->work_bytes < need) {
         if (sc->work) (void)hipFree(sc->work);
         sc->work = nullptr;
@@ -2214,6 +2317,18 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     sh.ray = (float4*)take(2 * sizeof(float4) * max_conn);
     sh.contrib = (float4*)take(sizeof(float4) * max_conn);
     uint32_t* qctl = (uint32_t*)take(256);  // [0] queued connections, [1] k_shadow's fetch counter
+    WalkBuf wb[2];
+    for (int q = 0; q < 2; ++q) {
+        wb[q].ray = (float4*)take(2 * sizeof(float4) * walks);
+        wb[q].flags = (uint32_t*)take(4 * walks);
+        wb[q].beta = (float4*)take(sizeof(float4) * walks);
+        wb[q].prev = (float4*)take(sizeof(float4) * walks);
+        wb[q].rng = (uint64_t*)take(8 * walks);
+        wb[q].slot = (uint32_t*)take(4 * walks);
+    }
+    int2* whits = (int2*)take(8 * walks);
+    float2* wuvs = (float2*)take(8 * walks);
+    uint32_t* wctl = (uint32_t*)take(256);  // [0], [1] walk queue sizes, [2] k_extend's fetch counter
     if ((size_t)(w - (char*)sc->work) > sc->work_bytes)
         return fail(RUMI_ERR_INVALID, "rumi_render: internal BDPT work-buffer layout overflow");
     if (hipMemcpyAsync(d_strat, strat.data(), sizeof(int2) * strat.size(), hipMemcpyHostToDevice,
@@ -2233,7 +2348,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     bp.beta = desc->mis_beta;
     bp.film_area = (2.0f * sc->camera.tanfov) * (2.0f * sc->camera.tanfov * sc->camera.y_ratio);
     bp.light_seed = desc->seed ^ 0x2545F4914F6CDD1Dull;
-    const bool spill = sc->stack_need > kStack;
+    const int stack = stack_variant(sc->stack_need);
 
     hipEvent_t ev0, ev1;
     (void)hipEventCreate(&ev0);
@@ -2244,9 +2359,28 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     uint64_t batches = 0;
     for (uint64_t base = 0; base < total; base += nb) {
         bb.nb = (uint32_t)std::min<uint64_t>(nb, total - base);
-        hipLaunchKernelGGL(k_bdpt_eye, dim3(grid_of(bb.nb)), dim3(kBlock), 0, stream, bb, base, bp, sv, spill);
-        hipLaunchKernelGGL(k_bdpt_light, dim3(grid_of(bb.nb)), dim3(kBlock), 0, stream, bb, base, bp, sv,
-                           spill);
+        // the walks as a wavefront: each iteration finds the closest hit of
+        // every live walk with persistent lanes (k_extend), then takes its
+        // segment (k_bdpt_walk); every iteration stores one vertex per live
+        // walk, so max(nE, nL) - 1 iterations end them all
+        const uint32_t nw = 2 * bb.nb;
+        const uint32_t gw = std::min<uint32_t>(grid_of(nw), (uint32_t)ctx->cu_count * 16u);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, wctl, 3u);
+        hipLaunchKernelGGL(k_bdpt_init, dim3(grid_of(nw)), dim3(kBlock), 0, stream, bb, base, bp, sv, wb[0],
+                           wctl);
+        for (int it = 0, cur = 0; it < std::max(nE, nL) - 1; ++it, cur ^= 1) {
+            hipLaunchKernelGGL(k_walk_ctl, dim3(1), dim3(64), 0, stream, wctl, cur ^ 1);
+            PathBuf pb{};
+            pb.ray = wb[cur].ray;
+            pb.flags = wb[cur].flags;
+            with_stack(stack, [&](auto v) {
+                hipLaunchKernelGGL((k_extend<decltype(v)::value, false>), dim3(gw), dim3(kBlock), 0, stream, pb,
+                                   wctl + cur, wctl + 2, sv, (const SceneTab*)nullptr, whits, wuvs);
+            });
+            hipLaunchKernelGGL(k_bdpt_walk, dim3(gw), dim3(kBlock), 0, stream, bb, base, bp, sv, wb[cur],
+                               (const uint32_t*)(wctl + cur), (const int2*)whits, (const float2*)wuvs,
+                               wb[cur ^ 1], wctl + (cur ^ 1));
+        }
         for (uint32_t q0 = 0; q0 < (uint32_t)strat.size(); q0 += kConnChunk) {
             const uint32_t nq = std::min<uint32_t>(kConnChunk, (uint32_t)strat.size() - q0);
             const uint64_t threads = (uint64_t)bb.nb * nq;
@@ -2255,12 +2389,10 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
                                sv, (const int2*)d_strat, q0, nq, film, sh, qctl);
             const uint32_t g = (uint32_t)std::min<uint64_t>((threads + kBlock - 1) / kBlock,
                                                             (uint64_t)ctx->cu_count * 16u);
-            if (spill)
-                hipLaunchKernelGGL((k_shadow<true, kRender, false>), dim3(g), dim3(kBlock), 0, stream, sh,
-                                   qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{});
-            else
-                hipLaunchKernelGGL((k_shadow<false, kRender, false>), dim3(g), dim3(kBlock), 0, stream, sh,
-                                   qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{});
+            with_stack(stack, [&](auto v) {
+                hipLaunchKernelGGL((k_shadow<decltype(v)::value, kRender, false>), dim3(g), dim3(kBlock), 0,
+                                   stream, sh, qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{});
+            });
         }
         ++batches;
     }
@@ -2366,7 +2498,7 @@ struct Pass {
     float* film;
     AdjCtx adj;
     Track tr;
-    bool spill;
+    int stack;            // traversal stack variant (stack_variant)
     bool ibl;             // some scene holds an image-based light
     int mode;
     uint32_t grid_cap;    // most blocks a grid-stride stage is given
@@ -2383,12 +2515,10 @@ void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t 
     const uint32_t g = grid_for(bound, ps.grid_cap);
     Ctl* ctl = w.ctl;
     hipLaunchKernelGGL(k_clear_ctl, dim3(1), dim3(64), 0, s, ctl, cur ^ 1);
-    if (ps.spill)
-        hipLaunchKernelGGL((k_extend<true, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
+    with_stack(ps.stack, [&](auto v) {
+        hipLaunchKernelGGL((k_extend<decltype(v)::value, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
                            &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs);
-    else
-        hipLaunchKernelGGL((k_extend<false, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
-                           &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs);
+    });
     // one 256-path tile per block; blocks past the queue's size return at once
     const uint32_t gs = (uint32_t)((bound + kShadeBlock - 1) / kShadeBlock);
     if (ps.ibl)
@@ -2399,12 +2529,10 @@ void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t 
         hipLaunchKernelGGL((k_shade<MODE, MULTI, false>), dim3(gs < 1 ? 1 : gs), dim3(kShadeBlock), 0, s,
                            w.pb[cur], w.pb[cur ^ 1], &ctl->queue[cur], w.hits, w.uvs, ps.sv, ps.tab, ps.prm,
                            ps.film, w.sh, &ctl->queue[cur ^ 1], &ctl->shadow, ps.adj, ps.tr);
-    if (ps.spill)
-        hipLaunchKernelGGL((k_shadow<true, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh, &ctl->shadow,
-                           &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj);
-    else
-        hipLaunchKernelGGL((k_shadow<false, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh,
+    with_stack(ps.stack, [&](auto v) {
+        hipLaunchKernelGGL((k_shadow<decltype(v)::value, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh,
                            &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj);
+    });
 }
 
 // One wavefront iteration over queue `cur` (at most `bound` paths).
@@ -2482,7 +2610,7 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     ps.film = film;
     ps.adj = adj;
     ps.tr = Track{nullptr, nullptr};
-    ps.spill = sc->stack_need > kStack;
+    ps.stack = stack_variant(sc->stack_need);
     ps.ibl = sc->has_ibl;
     ps.mode = mode;
     ps.grid_cap = (uint32_t)ctx->cu_count * 16u;
@@ -2612,7 +2740,7 @@ struct rumi_batch {
     rumi_batch_desc desc{};
     uint32_t cap = 0, generations = 0;
     uint64_t total_samples = 0, total_pixels = 0, total_texels = 0;
-    bool spill = false;
+    int stack = 0;        // traversal stack variant of the deepest scene
     bool ibl = false;  // some scene holds an image-based light
     std::vector<uint32_t> item_samples, item_pixels;
     // device
@@ -2660,7 +2788,7 @@ void batch_enqueue(rumi_batch* b, const rumi_batch_io& io, hipStream_t s) {
     ps.prm.max_depth = b->desc.max_depth;
     ps.prm.pixels = 0;
     ps.prm.n_scenes = n;
-    ps.spill = b->spill;
+    ps.stack = b->stack;
     ps.ibl = b->ibl;
     ps.grid_cap = (uint32_t)b->cu_count * 16u;
     ps.adj = AdjCtx{};
@@ -2910,7 +3038,7 @@ extern "C" int rumi_batch_create(rumi_ctx* ctx, const rumi_batch_item* items, ui
         samples += (uint64_t)t.pixels * (uint64_t)desc->spp;
         pixels += t.pixels;
         texels += sc->texel_count;
-        if (sc->stack_need > kStack) b->spill = true;
+        b->stack = std::max(b->stack, stack_variant(sc->stack_need));
         if (samples > 0xffffffffull || pixels > 0xffffffffull / 4) {
             delete b;
             return fail(RUMI_ERR_INVALID, "rumi_batch_create: more than 2^32 - 1 samples in the batch");

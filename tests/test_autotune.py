@@ -73,3 +73,11 @@ def test_render_choice_and_candidates():
 def test_empty_candidates_raise():
     with pytest.raises(ValueError):
         autotune.tune_trace(FakeCtx(), lambda: None, [], timer=lambda fn: 1.0)
+
+
+def test_native_candidates_equal_the_python_list():
+    """The C ABI's tuner (rumi_autotune_trace) times exactly these
+    candidates, in this order (a host-only call: no GPU needed)."""
+    import importlib
+    rumi = importlib.import_module("rumi-renderer_amd")
+    assert rumi.autotune_trace_candidates() == autotune.trace_candidates()

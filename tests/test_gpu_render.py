@@ -429,3 +429,22 @@ def test_point_light_same_seed_matches_oracle(rumi, oracle, ctx, scenes, case):
     d_other = np.abs(other - c).sum() / np.abs(c).sum()
     print("point light", case, "relative L1, same seed", d_same, "other seed", d_other)
     assert d_same < 1e-4 and d_other > 0.003, (d_same, d_other)
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_native_render_autotuner_choice_renders_the_same_image(rumi, ctx, scenes, integrator):
+    """rumi_autotune_render picks a queue size (path tracer) or batch size
+    (BDPT) among its candidates; the frame rendered with it equals the
+    default render with the same seed up to the order of float atomics."""
+    import torch
+    sc = rumi.Scene(ctx, scenes.cornell_box(64, 64))
+    kw = dict(integrator=integrator, mis_beta=1.0) if integrator == 1 else {}
+    tune, table = sc.autotune(64, 64, 64, 5, seed=3, **kw)
+    assert tune["queue_capacity"] in ([1 << 20, 1 << 21, 1 << 22, 1 << 23] if integrator == 1
+                                      else [1 << 19, 1 << 20, 1 << 21, 1 << 22])
+    assert len(table) == (4 if integrator == 1 else 8) and table[0] is not None
+    a, _ = sc.render(64, 64, 64, 5, seed=3, **tune, **kw)
+    b, _ = sc.render(64, 64, 64, 5, seed=3, **kw)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(a.cpu().numpy(), b.cpu().numpy(), rtol=2e-4, atol=1e-6)
+    sc.close()

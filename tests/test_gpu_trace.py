@@ -441,3 +441,26 @@ def test_every_tuning_gives_identical_hits(rumi, ctx, small_mesh):
     print("tuning", best, [(c, round(s * 1e6, 1)) for c, s in table])
     ctx.set_trace_tuning()
     assert ctx.trace_tuning() == dict(block=256, stack=16, refill=16, persistent=2, width=4)
+
+
+def test_native_autotuner_picks_a_candidate_and_keeps_results(rumi, ctx, small_mesh):
+    """rumi_autotune_trace times every candidate on the caller's batch, leaves
+    the fastest on the context and the hits are those of any tuning."""
+    import torch
+    xyz, idx = small_mesh
+    bvh = rumi.Bvh(ctx, xyz, idx)
+    rays = torch.from_numpy(rumi.make_rays_uniform(1 << 15, 4, 2.1)).cuda()
+    ref = torch.empty((1 << 15, 2), dtype=torch.int32, device="cuda")
+    ctx.set_trace_tuning(width=4)
+    rumi.trace_closest(ctx, bvh, rays, ref)
+    out = torch.empty_like(ref)
+    best, table = rumi.autotune_trace(ctx, bvh, rays, out)
+    cands = rumi.autotune_trace_candidates()
+    assert best in cands and ctx.trace_tuning() == best
+    assert [c for c, _ in table] == cands and all(t > 0 for _, t in table)
+    assert min(t for _, t in table) == dict((tuple(c.items()), t) for c, t in table)[tuple(best.items())]
+    h = torch.full_like(ref, 7)
+    rumi.trace_closest(ctx, bvh, rays, h)
+    torch.cuda.synchronize()
+    assert torch.equal(h[:, 1], ref[:, 1])       # same distances for every width
+    ctx.set_trace_tuning()
