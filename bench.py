@@ -198,6 +198,7 @@ def main():
         log(f"reference-tree check: {ref_check}")
 
     traffic = measured_traffic("closest-hit", n)
+    copy_gbs = hbm_copy_bandwidth(log)
 
     out = {
         "metric": METRIC,
@@ -223,6 +224,8 @@ def main():
             "unit": "GB/s", "frac": round(achieved_c / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_ray": round(b_ray_c, 1), "nodes_per_ray": round(nodes_c, 3),
             "tris_per_ray": round(tris_c, 3), "node_bytes": node_bc,
+            "achievable_copy_GBs": copy_gbs,
+            "frac_of_copy": round(achieved_c / copy_gbs, 4) if copy_gbs else None,
         },
         "tuning": tuning,
         "reference_tree_check": ref_check,
@@ -841,6 +844,31 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
                              "sample": f"32x32 at {spp} spp, the oracle's FP64 bidirectional "
                                        f"tracer ({cpu_sec:.2f}s)"},
             "speedup_vs_cpu": round(sps / cpu_sps, 1)}
+
+
+def hbm_copy_bandwidth(log, mib=4096, runs=5):
+    """Achievable HBM bandwidth: a device-to-device copy of `mib` MiB (16x
+    the 256 MB Infinity Cache, so it streams from HBM), read + write bytes
+    over the best of `runs` timed copies (HIP events)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(runs):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    gbs = 2.0 * n * 4 / best / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    log(f"HBM copy: {mib} MiB in {best * 1e3:.3f} ms = {gbs:.0f} GB/s (read + write)")
+    return round(gbs, 1)
 
 
 def rel_mse_ab(g_vals, c_vals):
