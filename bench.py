@@ -453,6 +453,10 @@ def render_legs(rumi, ctx, args, log, only=""):
             times.append(time.perf_counter() - t0)
         sec = min(times)
         sps = st.samples / sec
+        # one counting render (untimed): the per-ray work behind the roofline
+        _, cst = scene.render(w, h, spp, depth, seed=10, count_traversal=True, **qtune)
+        torch.cuda.synchronize()
+        roof = render_roofline(cst, sps)
         # rel-MSE at equal spp against the committed 64^2 4096-spp reference,
         # GPU and CPU on disjoint seeds (independent renders): 32 GPU renders,
         # the 4 timed one-thread CPU renders below and 20 more on all threads
@@ -494,6 +498,7 @@ def render_legs(rumi, ctx, args, log, only=""):
             "shadow_rays": int(st.shadow_rays), "iterations": int(st.iterations),
             "rays_per_s": round((st.extension_rays + st.shadow_rays) / sec, 1),
             "rel_mse_64px": dict(ab, spp=spp),
+            "roofline": roof,
             "queue_tuning": dict(qtune, candidates_ms=[None if t is None else round(t * 1e3, 2) for t in qtable]),
             "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
                              "kind": "port",
@@ -805,6 +810,10 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
         times.append(time.perf_counter() - t0)
     sec = min(times)
     sps = st.samples / sec
+    _, cst = scene.render(w, h, spp, max_len, seed=10, queue_capacity=qtune["queue_capacity"],
+                          count_traversal=True, **kw)
+    torch.cuda.synchronize()
+    roof = render_roofline(cst, sps)
     scene.close()
     # rel-MSE at equal spp against the committed 64^2 4096-spp oracle
     # reference, GPU and CPU on disjoint seeds (independent renders)
@@ -837,6 +846,9 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
             "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
             "max_path_length": max_len, "mis_beta": 2.0, "triangles": 200000 + 10,
             "rel_mse_64px": dict(ab, spp=spp),
+            "roofline": roof,
+            "walk_segments_per_sample": round(cst.extension_rays / cst.samples, 3),
+            "connection_rays_per_sample": round(cst.shadow_rays / cst.samples, 3),
             "batch_tuning": dict(samples_per_batch=qtune["queue_capacity"],
                                  candidates_ms=[None if t is None else round(t * 1e3, 1) for t in qtable]),
             "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1,
@@ -844,6 +856,33 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
                              "sample": f"32x32 at {spp} spp, the oracle's FP64 bidirectional "
                                        f"tracer ({cpu_sec:.2f}s)"},
             "speedup_vs_cpu": round(sps / cpu_sps, 1)}
+
+
+def render_roofline(st, samples_per_s, node_bytes=NODE_BYTES4, prim_bytes=TRI_BYTES):
+    """Roofline of a render leg from one counting render's stats (st:
+    RenderStats with the traversal totals): algorithmic bytes per sample =
+    sum over the sample's rays of B_ray (ray 32 B in, hit 8 B out -- 4 for a
+    shadow ray -- 128 B per 4-wide node entered, 48 B per primitive tested)
+    + 128 B of path state read and written per extension ray (per walk
+    segment for BDPT) + 12 B of film; achieved = that x samples/s against
+    the 8 TB/s peak.  These scenes' trees are cache-resident, so the bytes
+    are touched, not DRAM traffic."""
+    R = float(st.samples)
+    e, sh = float(st.extension_rays), float(st.shadow_rays)
+    ne = st.ext_nodes / e if e else 0.0
+    pe = st.ext_prims / e if e else 0.0
+    ns = st.shadow_nodes / sh if sh else 0.0
+    ps = st.shadow_prims / sh if sh else 0.0
+    b_ext = RAY_IN_BYTES + HIT_OUT_BYTES + node_bytes * ne + prim_bytes * pe
+    b_sh = RAY_IN_BYTES + 4 + node_bytes * ns + prim_bytes * ps
+    per_sample = (e * b_ext + sh * b_sh) / R + 128.0 * (e / R) + 12.0
+    achieved = per_sample * samples_per_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_sample": round(per_sample, 1),
+            "extension_rays_per_sample": round(e / R, 3), "shadow_rays_per_sample": round(sh / R, 3),
+            "nodes_per_extension_ray": round(ne, 3), "prims_per_extension_ray": round(pe, 3),
+            "nodes_per_shadow_ray": round(ns, 3), "prims_per_shadow_ray": round(ps, 3)}
 
 
 def hbm_copy_bandwidth(log, mib=4096, runs=5):

@@ -329,14 +329,26 @@ typedef struct {
     uint32_t queue_capacity;    /* wavefront paths in flight (BDPT: samples per batch), 0 = automatic
                                    (2^21 paths; BDPT 2^22 samples) */
     uint32_t regen_threshold;   /* refill the queue when fewer paths are alive, 0 = always */
+    uint32_t flags;             /* RUMI_RENDER_* */
 } rumi_render_desc;
+
+/* rumi_render_desc.flags: count the traversal work of every extension and
+ * shadow ray into rumi_render_stats (interior nodes entered, primitives
+ * tested; for the bidirectional tracer also its walk segments and
+ * connection rays) -- the per-ray work behind the renderer's algorithmic
+ * bytes.  Same image, slower kernels; path tracer and bidirectional tracer. */
+#define RUMI_RENDER_COUNT_TRAVERSAL 1u
 
 typedef struct {
     uint64_t samples;
-    uint64_t extension_rays;
-    uint64_t shadow_rays;
+    uint64_t extension_rays;  /* BDPT: walk segments, counting renders only */
+    uint64_t shadow_rays;     /* BDPT: connection rays, counting renders only */
     uint64_t iterations;
     double milliseconds;   /* device time between the first and last stage (HIP events) */
+    /* RUMI_RENDER_COUNT_TRAVERSAL only (else 0): interior nodes entered and
+     * primitives tested by all extension (BDPT: walk) rays and all shadow
+     * (BDPT: connection) rays */
+    uint64_t ext_nodes, ext_prims, shadow_nodes, shadow_prims;
 } rumi_render_stats;
 
 /* Renders one frame: Scene::Rendering's pixel x sample loop

@@ -92,12 +92,13 @@ class RenderDesc(Structure):
     _fields_ = [("integrator", c_int32), ("width", c_int32), ("height", c_int32),
                 ("spp", c_int32), ("max_depth", c_int32), ("n_light", c_int32),
                 ("n_eye", c_int32), ("mis_beta", c_float), ("seed", c_uint64),
-                ("queue_capacity", c_uint32), ("regen_threshold", c_uint32)]
+                ("queue_capacity", c_uint32), ("regen_threshold", c_uint32), ("flags", c_uint32)]
 
 
 class RenderStats(Structure):
     _fields_ = [("samples", c_uint64), ("extension_rays", c_uint64), ("shadow_rays", c_uint64),
-                ("iterations", c_uint64), ("milliseconds", c_double)]
+                ("iterations", c_uint64), ("milliseconds", c_double), ("ext_nodes", c_uint64),
+                ("ext_prims", c_uint64), ("shadow_nodes", c_uint64), ("shadow_prims", c_uint64)]
 
 
 class TraceTuning(Structure):
@@ -138,6 +139,7 @@ RUMI_SHAPE_TRIANGLE, RUMI_SHAPE_SPHERE, RUMI_SHAPE_RECT, RUMI_SHAPE_MESH = 0, 1,
 MATERIAL_KINDS = {"matte": 0, "glossy": 1, "specular": 2, "emissive": 3, "dielectric": 4,
                   "subsurface": 5, "ggx": 6}
 RUMI_INTEGRATOR_PT, RUMI_INTEGRATOR_BDPT, RUMI_INTEGRATOR_WHITTED = 0, 1, 2
+RUMI_RENDER_COUNT_TRAVERSAL = 1
 RUMI_FILM_BGR_SUM, RUMI_FILM_RGB_MEAN = 0, 1
 
 # Every function include/rumi.h declares, with its ctypes signature.
@@ -599,13 +601,15 @@ class Scene:
 
     def render(self, width, height, spp, max_depth, seed=0, out=None, queue_capacity=0,
                regen_threshold=0, integrator=RUMI_INTEGRATOR_PT, n_light=0, n_eye=0,
-               mis_beta=1.0, stream=None):
-        """Returns (device tensor height x width x 3 of mean radiance, RenderStats)."""
+               mis_beta=1.0, count_traversal=False, stream=None):
+        """Returns (device tensor height x width x 3 of mean radiance, RenderStats);
+        count_traversal fills the stats' node and primitive totals."""
         import torch
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device="cuda")
         rd = RenderDesc(integrator, width, height, spp, max_depth, n_light, n_eye, mis_beta,
-                        seed, queue_capacity, regen_threshold)
+                        seed, queue_capacity, regen_threshold,
+                        RUMI_RENDER_COUNT_TRAVERSAL if count_traversal else 0)
         st = RenderStats()
         check(lib().rumi_render(self.ctx.handle, self.handle, byref(rd), _ptr(out), byref(st),
                                 _stream(stream)))

@@ -227,6 +227,19 @@ __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + (uint32_t)__popcll(below);
 }
 
+// Adds a lane's traversal counts to two 64-bit totals, one atomic per wave.
+__device__ inline void add_counts(unsigned long long* counts, uint32_t nodes, uint32_t prims) {
+    uint64_t a = nodes, b = prims;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor((unsigned long long)a, off);
+        b += __shfl_xor((unsigned long long)b, off);
+    }
+    if (__lane_id() == 0) {
+        if (a) atomicAdd(counts + 0, (unsigned long long)a);
+        if (b) atomicAdd(counts + 1, (unsigned long long)b);
+    }
+}
+
 __device__ inline void film_add(float* film, uint32_t pixel, V3 c) {
     if (is_nan(c)) return;  // Scene::Rendering drops NaN radiance (scene.cpp:229-238)
     float* f = film + 3 * (size_t)pixel;
@@ -336,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_regen(PathBuf buf, uint32_t start, u
 
 // Closest hit of every queued path (BVHAccel::Travel with the path's
 // in_object flag, bvh.cpp:150-213), persistent lanes (run_lanes).
-template <int SV, bool MULTI>
+template <int SV, bool MULTI, bool COUNT = false>
 struct ExtendItem {
     PathBuf buf;
     const SceneTab* tab;
@@ -371,22 +384,27 @@ struct ExtendItem {
     }
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
-        return trav_step<kBlock, sv_lds(SV), false, true, true, false, (sv_priv(SV) > 0), true>(
+        const bool go = trav_step<kBlock, sv_lds(SV), false, true, true, COUNT, (sv_priv(SV) > 0), true>(
             nodes, prims, rects, s, in_obj, nn, np, s_stk, tid, p_stk);
+        if (COUNT) cn += (uint32_t)nn, cp += (uint32_t)np;
+        return go;
     }
     __device__ void finish(const TravState& s) {
         hits[id] = make_int2(s.hit, __float_as_int(s.r.tmin));
         uvs[id] = make_float2(s.hu, s.hv);
     }
+    uint32_t cn = 0, cp = 0;  // COUNT: interior nodes entered and primitives tested
 };
 
-template <int SV, bool MULTI>
+// COUNT: also adds every lane's node visits and primitive tests to counts[0..1].
+template <int SV, bool MULTI, bool COUNT = false>
 __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* n_ptr,
                                                    uint32_t* counter, SceneView sv,
-                                                   const SceneTab* tab, int2* hits, float2* uvs) {
+                                                   const SceneTab* tab, int2* hits, float2* uvs,
+                                                   unsigned long long* counts = nullptr) {
     __shared__ int2 s_stk[sv_lds(SV) * kBlock];
     int2 p_stk[sv_priv(SV) > 0 ? sv_priv(SV) : 1];
-    ExtendItem<SV, MULTI> it;
+    ExtendItem<SV, MULTI, COUNT> it;
     it.buf = buf;
     it.tab = tab;
     it.nodes = sv.nodes;
@@ -401,6 +419,10 @@ __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* 
     it.id = it.nid = it.nflags = it.nscene = 0;
     it.in_obj = false;
     run_lanes(*n_ptr, counter, it);
+    if (COUNT) {  // counts: [0] nodes, [1] primitives, [4] rays
+        add_counts(counts, it.cn, it.cp);
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(counts + 4, (unsigned long long)*n_ptr);
+    }
 }
 
 // Clears the counters the next wavefront iteration appends to.
@@ -1402,7 +1424,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(PathBuf in, PathBuf out, 
 // Any-hit test of every shadow ray (Scene::VisibilityTest, scene.cpp:91-123),
 // persistent lanes; unoccluded contributions go to the film (the adjoint
 // passes: to the per-sample sums and the connection's own gradient terms).
-template <int SV, int MODE, bool MULTI>
+template <int SV, int MODE, bool MULTI, bool COUNT = false>
 struct ShadowItem {
     ShadowBuf sh;
     const SceneTab* tab;
@@ -1437,9 +1459,12 @@ struct ShadowItem {
     }
     __device__ bool step(TravState& s) {
         int32_t nn = 0, np = 0;
-        return trav_step<kBlock, sv_lds(SV), true, true, true, false, (sv_priv(SV) > 0), true>(
+        const bool go = trav_step<kBlock, sv_lds(SV), true, true, true, COUNT, (sv_priv(SV) > 0), true>(
             nodes, prims, rects, s, false, nn, np, s_stk, tid, p_stk);
+        if (COUNT) cn += (uint32_t)nn, cp += (uint32_t)np;
+        return go;
     }
+    uint32_t cn = 0, cp = 0;  // COUNT: interior nodes entered and primitives tested
     __device__ void finish(const TravState& s) {
         // Scene::VisibilityTest: in shadow iff the any-hit query found a hit.
         if (s.r.tmin > 0.0f) return;
@@ -1472,13 +1497,14 @@ struct ShadowItem {
     }
 };
 
-template <int SV, int MODE, bool MULTI>
+template <int SV, int MODE, bool MULTI, bool COUNT = false>
 __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t* n_ptr,
                                                    uint32_t* counter, SceneView sv,
-                                                   const SceneTab* tab, float* film, AdjCtx adj) {
+                                                   const SceneTab* tab, float* film, AdjCtx adj,
+                                                   unsigned long long* counts = nullptr) {
     __shared__ int2 s_stk[sv_lds(SV) * kBlock];
     int2 p_stk[sv_priv(SV) > 0 ? sv_priv(SV) : 1];
-    ShadowItem<SV, MODE, MULTI> it;
+    ShadowItem<SV, MODE, MULTI, COUNT> it;
     it.sh = sh;
     it.tab = tab;
     it.nodes = sv.nodes;
@@ -1496,6 +1522,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
     it.id = it.nid = it.nscene = it.scene = 0;
     run_lanes(*n_ptr, counter, it);
     if (MODE == kReplay) flush_grad(it.G, adj.gstripes);
+    if (COUNT) {  // counts: [2] nodes, [3] primitives, [5] rays
+        add_counts(counts + 2, it.cn, it.cp);
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(counts + 5, (unsigned long long)*n_ptr);
+    }
 }
 
 // ============================================================ BDPT
@@ -2329,6 +2359,8 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     int2* whits = (int2*)take(8 * walks);
     float2* wuvs = (float2*)take(8 * walks);
     uint32_t* wctl = (uint32_t*)take(256);  // [0], [1] walk queue sizes, [2] k_extend's fetch counter
+    unsigned long long* counts = (unsigned long long*)take(64);  // counting render: traversal totals
+    const bool counting = (desc->flags & RUMI_RENDER_COUNT_TRAVERSAL) != 0;
     if ((size_t)(w - (char*)sc->work) > sc->work_bytes)
         return fail(RUMI_ERR_INVALID, "rumi_render: internal BDPT work-buffer layout overflow");
     if (hipMemcpyAsync(d_strat, strat.data(), sizeof(int2) * strat.size(), hipMemcpyHostToDevice,
@@ -2356,6 +2388,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     (void)hipEventRecord(ev0, stream);
     const size_t film_n = 3 * (size_t)pixels;
     hipLaunchKernelGGL(k_zero, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n);
+    if (counting) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)counts, 12u);
     uint64_t batches = 0;
     for (uint64_t base = 0; base < total; base += nb) {
         bb.nb = (uint32_t)std::min<uint64_t>(nb, total - base);
@@ -2374,8 +2407,14 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
             pb.ray = wb[cur].ray;
             pb.flags = wb[cur].flags;
             with_stack(stack, [&](auto v) {
-                hipLaunchKernelGGL((k_extend<decltype(v)::value, false>), dim3(gw), dim3(kBlock), 0, stream, pb,
-                                   wctl + cur, wctl + 2, sv, (const SceneTab*)nullptr, whits, wuvs);
+                if (counting)
+                    hipLaunchKernelGGL((k_extend<decltype(v)::value, false, true>), dim3(gw), dim3(kBlock), 0,
+                                       stream, pb, wctl + cur, wctl + 2, sv, (const SceneTab*)nullptr, whits, wuvs,
+                                       counts);
+                else
+                    hipLaunchKernelGGL((k_extend<decltype(v)::value, false>), dim3(gw), dim3(kBlock), 0, stream,
+                                       pb, wctl + cur, wctl + 2, sv, (const SceneTab*)nullptr, whits, wuvs,
+                                       (unsigned long long*)nullptr);
             });
             hipLaunchKernelGGL(k_bdpt_walk, dim3(gw), dim3(kBlock), 0, stream, bb, base, bp, sv, wb[cur],
                                (const uint32_t*)(wctl + cur), (const int2*)whits, (const float2*)wuvs,
@@ -2390,8 +2429,14 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
             const uint32_t g = (uint32_t)std::min<uint64_t>((threads + kBlock - 1) / kBlock,
                                                             (uint64_t)ctx->cu_count * 16u);
             with_stack(stack, [&](auto v) {
-                hipLaunchKernelGGL((k_shadow<decltype(v)::value, kRender, false>), dim3(g), dim3(kBlock), 0,
-                                   stream, sh, qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{});
+                if (counting)
+                    hipLaunchKernelGGL((k_shadow<decltype(v)::value, kRender, false, true>), dim3(g), dim3(kBlock),
+                                       0, stream, sh, qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{},
+                                       counts);
+                else
+                    hipLaunchKernelGGL((k_shadow<decltype(v)::value, kRender, false>), dim3(g), dim3(kBlock), 0,
+                                       stream, sh, qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{},
+                                       (unsigned long long*)nullptr);
             });
         }
         ++batches;
@@ -2410,10 +2455,19 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     (void)hipEventDestroy(ev1);
     if (stats) {
         stats->samples = total;
-        stats->extension_rays = 0;
-        stats->shadow_rays = 0;
         stats->iterations = batches;
         stats->milliseconds = ms;
+        // walk segments and connection rays are counted on the device only
+        // by counting renders
+        unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
+        if (rc == RUMI_OK && counting && hipMemcpy(c, counts, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(RUMI_ERR_HIP, "rumi_render: traversal count read failed");
+        stats->extension_rays = c[4];
+        stats->shadow_rays = c[5];
+        stats->ext_nodes = c[0];
+        stats->ext_prims = c[1];
+        stats->shadow_nodes = c[2];
+        stats->shadow_prims = c[3];
     }
     return rc;
 }
@@ -2449,6 +2503,7 @@ struct Wave {
     float2* uvs;
     ShadowBuf sh;
     Ctl* ctl;
+    unsigned long long* counts;  // counting renders: extension nodes, prims; shadow nodes, prims; rays
     uint32_t cap;
 };
 
@@ -2456,9 +2511,9 @@ struct Wave {
 // flags, pend, scene 4 each), hit 8, barycentrics 8, shadow entry (ray 32,
 // contrib 16, sample 4, score 8, texel 4, scene 4).
 constexpr size_t kBytesPerPath = 2 * (32 + 16 + 8 + 4 * 5) + 8 + 8 + (32 + 16 + 4 + 8 + 4 + 4);
-constexpr int kCarved = 25;  // arrays carved below, each aligned to 256 B
+constexpr int kCarved = 26;  // arrays carved below, each aligned to 256 B
 
-size_t wave_bytes(uint32_t cap) { return kBytesPerPath * (size_t)cap + sizeof(Ctl) + 256 * kCarved; }
+size_t wave_bytes(uint32_t cap) { return kBytesPerPath * (size_t)cap + sizeof(Ctl) + 64 + 256 * kCarved; }
 
 bool wave_carve(Wave& w, void* mem, size_t bytes, uint32_t cap) {
     char* p = (char*)mem;
@@ -2486,6 +2541,7 @@ bool wave_carve(Wave& w, void* mem, size_t bytes, uint32_t cap) {
     w.sh.tex = (int32_t*)take(4 * (size_t)cap);
     w.sh.scene = (uint32_t*)take(4 * (size_t)cap);
     w.ctl = (Ctl*)take(sizeof(Ctl));
+    w.counts = (unsigned long long*)take(64);
     w.cap = cap;
     return (size_t)(p - (char*)mem) <= bytes;
 }
@@ -2499,6 +2555,7 @@ struct Pass {
     AdjCtx adj;
     Track tr;
     int stack;            // traversal stack variant (stack_variant)
+    unsigned long long* counts;  // counting render: traversal totals, else null
     bool ibl;             // some scene holds an image-based light
     int mode;
     uint32_t grid_cap;    // most blocks a grid-stride stage is given
@@ -2516,8 +2573,17 @@ void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t 
     Ctl* ctl = w.ctl;
     hipLaunchKernelGGL(k_clear_ctl, dim3(1), dim3(64), 0, s, ctl, cur ^ 1);
     with_stack(ps.stack, [&](auto v) {
+        if constexpr (MODE == kRender && !MULTI) {
+            if (ps.counts) {
+                hipLaunchKernelGGL((k_extend<decltype(v)::value, false, true>), dim3(g), dim3(kBlock), 0, s,
+                                   w.pb[cur], &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs,
+                                   ps.counts);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_extend<decltype(v)::value, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
-                           &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs);
+                           &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs,
+                           (unsigned long long*)nullptr);
     });
     // one 256-path tile per block; blocks past the queue's size return at once
     const uint32_t gs = (uint32_t)((bound + kShadeBlock - 1) / kShadeBlock);
@@ -2530,8 +2596,17 @@ void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t 
                            w.pb[cur], w.pb[cur ^ 1], &ctl->queue[cur], w.hits, w.uvs, ps.sv, ps.tab, ps.prm,
                            ps.film, w.sh, &ctl->queue[cur ^ 1], &ctl->shadow, ps.adj, ps.tr);
     with_stack(ps.stack, [&](auto v) {
+        if constexpr (MODE == kRender && !MULTI) {
+            if (ps.counts) {
+                hipLaunchKernelGGL((k_shadow<decltype(v)::value, kRender, false, true>), dim3(g), dim3(kBlock), 0,
+                                   s, w.sh, &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj,
+                                   ps.counts);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_shadow<decltype(v)::value, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh,
-                           &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj);
+                           &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj,
+                           (unsigned long long*)nullptr);
     });
 }
 
@@ -2611,6 +2686,8 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     ps.adj = adj;
     ps.tr = Track{nullptr, nullptr};
     ps.stack = stack_variant(sc->stack_need);
+    const bool counting = (desc->flags & RUMI_RENDER_COUNT_TRAVERSAL) != 0 && mode == kRender;
+    ps.counts = counting ? w.counts : nullptr;
     ps.ibl = sc->has_ibl;
     ps.mode = mode;
     ps.grid_cap = (uint32_t)ctx->cu_count * 16u;
@@ -2626,6 +2703,7 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     const size_t film_n = 3 * (size_t)pixels;
     if (mode != kReplay)
         hipLaunchKernelGGL(k_zero, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n);
+    if (counting) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)w.counts, 12u);
 
     uint64_t issued = 0, ext_rays = 0, shadow_rays = 0, iters = 0;
     uint32_t n_cur = 0;
@@ -2677,6 +2755,13 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
         stats->shadow_rays = shadow_rays;
         stats->iterations = iters;
         stats->milliseconds = ms;
+        unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
+        if (rc == RUMI_OK && counting && hipMemcpy(c, w.counts, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(RUMI_ERR_HIP, "rumi_render: traversal count read failed");
+        stats->ext_nodes = c[0];
+        stats->ext_prims = c[1];
+        stats->shadow_nodes = c[2];
+        stats->shadow_prims = c[3];
     }
     return rc;
 }
@@ -2870,7 +2955,7 @@ extern "C" int rumi_render(rumi_ctx* ctx, const rumi_scene* scene_c, const rumi_
 // the array, so calls that overlap on one device race.
 namespace {
 constexpr uint32_t kNoAllocPaths = RUMI_NOALLOC_MAX_PATHS;
-constexpr size_t kScratchBytes = kBytesPerPath * (size_t)kNoAllocPaths + sizeof(Ctl) + 256 * kCarved;
+constexpr size_t kScratchBytes = kBytesPerPath * (size_t)kNoAllocPaths + sizeof(Ctl) + 64 + 256 * kCarved;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 constexpr int kMaxDevices = 64;

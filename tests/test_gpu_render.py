@@ -448,3 +448,24 @@ def test_native_render_autotuner_choice_renders_the_same_image(rumi, ctx, scenes
     torch.cuda.synchronize()
     np.testing.assert_allclose(a.cpu().numpy(), b.cpu().numpy(), rtol=2e-4, atol=1e-6)
     sc.close()
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_counting_render_gives_the_same_image_and_the_work(rumi, ctx, scenes, integrator):
+    """RUMI_RENDER_COUNT_TRAVERSAL: the counting kernels render the same image
+    and report the rays and their node/primitive work (every ray of the
+    closed Cornell box enters the root, so at least one node per ray)."""
+    import torch
+    sc = rumi.Scene(ctx, scenes.cornell_box(48, 48))
+    kw = dict(integrator=integrator, mis_beta=1.0) if integrator == 1 else {}
+    a, sa = sc.render(48, 48, 16, 5, seed=4, **kw)
+    b, sb = sc.render(48, 48, 16, 5, seed=4, count_traversal=True, **kw)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(b.cpu().numpy(), a.cpu().numpy(), rtol=2e-4, atol=1e-6)
+    assert sa.ext_nodes == 0 and sb.ext_nodes > 0
+    assert sb.extension_rays > 0 and sb.shadow_rays > 0
+    if integrator == 0:
+        assert sb.extension_rays == sa.extension_rays and sb.shadow_rays == sa.shadow_rays
+    assert sb.ext_nodes >= sb.extension_rays and sb.ext_prims > 0
+    assert sb.shadow_nodes > 0 and sb.shadow_prims > 0
+    sc.close()
