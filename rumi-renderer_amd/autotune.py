@@ -122,7 +122,7 @@ def tune_render(run: Callable[..., object], candidates: Iterable[dict] | None = 
 
 
 def tune_batch(build: Callable[[int], object], measure: Callable[[object], float],
-               capacities: Sequence[int] = (1 << 20, 1 << 21, 1 << 22, 1 << 23),
+               capacities: Sequence[int] = (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25),
                release: Callable[[object], None] | None = None):
     """Generation size of a batch: build(capacity) makes the batch,
     measure(batch) returns the seconds of one run (the timer: for a captured
