@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default traversal launch shape instead of autotuning it")
     ap.add_argument("--render-runs", type=int, default=3)
+    ap.add_argument("--gpu-only", action="store_true",
+                    help="render legs: GPU renders only, with the default queue sizes (no autotuner, "
+                         "counting render, rel-MSE renders or CPU baseline): for profiling")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -442,7 +445,10 @@ def render_legs(rumi, ctx, args, log, only=""):
         sc = factory(w, h)
         scene = rumi.Scene(ctx, sc)
         # queue sizes chosen by the native autotuner on this frame (untimed)
-        qtune, qtable = scene.autotune(w, h, spp, depth, seed=1)
+        if args.gpu_only:
+            qtune, qtable = {}, []
+        else:
+            qtune, qtable = scene.autotune(w, h, spp, depth, seed=1)
         img, st = scene.render(w, h, spp, depth, seed=1, **qtune)  # warm-up
         torch.cuda.synchronize()
         times = []
@@ -453,6 +459,11 @@ def render_legs(rumi, ctx, args, log, only=""):
             times.append(time.perf_counter() - t0)
         sec = min(times)
         sps = st.samples / sec
+        if args.gpu_only:
+            out[name] = {"samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4)}
+            log(f"{name}: {sps / 1e6:.2f} Msamples/s ({sec * 1e3:.1f} ms)")
+            scene.close()
+            continue
         # one counting render (untimed): the per-ray work behind the roofline
         _, cst = scene.render(w, h, spp, depth, seed=10, count_traversal=True, **qtune)
         torch.cuda.synchronize()
@@ -805,7 +816,10 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     scene = rumi.Scene(ctx, sc)
     kw = dict(integrator=rumi.RUMI_INTEGRATOR_BDPT, mis_beta=2.0, n_eye=9, n_light=9)
     # samples per batch chosen by the native autotuner on this frame (untimed)
-    qtune, qtable = scene.autotune(w, h, spp, max_len, seed=1, **kw)
+    if args.gpu_only:
+        qtune, qtable = {"queue_capacity": 0}, []
+    else:
+        qtune, qtable = scene.autotune(w, h, spp, max_len, seed=1, **kw)
     log(f"cornell_bdpt: batch sizes {qtable} -> {qtune}")
     times = []
     for k in range(max(1, args.render_runs - 1)):
@@ -815,6 +829,10 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
         times.append(time.perf_counter() - t0)
     sec = min(times)
     sps = st.samples / sec
+    if args.gpu_only:
+        scene.close()
+        log(f"cornell_bdpt: {sps / 1e6:.3f} Msamples/s ({sec * 1e3:.1f} ms)")
+        return {"samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4)}
     _, cst = scene.render(w, h, spp, max_len, seed=10, queue_capacity=qtune["queue_capacity"],
                           count_traversal=True, **kw)
     torch.cuda.synchronize()

@@ -1539,8 +1539,11 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
 // over all strategies with exponent beta (beta = 1 is the reference's
 // balance heuristic, MISWeight :1096-1586).  Dirac vertices (mirror,
 // dielectric) are never connected and enter the MIS sums with remapped
-// pdfs.  Stages: k_bdpt_eye, k_bdpt_light, k_bdpt_connect (one thread per
-// sample and strategy, shadow ray inline).
+// pdfs.  Stages: k_bdpt_init and rounds of k_extend + k_bdpt_walk build both
+// subpaths of every sample of a batch as one wavefront; k_bdpt_hist and
+// k_bdpt_scatter order the samples by their subpath lengths; k_bdpt_connect
+// (one thread per sample and strategy that sample can make) evaluates the
+// connections and queues their visibility tests for k_shadow.
 //
 // A glossy vertex keeps the one lobe Glossy::BSDFSampling picks
 // (src/Material/glossy.cpp:56-78).  Lambertian lobe (probability kd_pdf): a
@@ -1564,9 +1567,9 @@ struct BVert {
 };
 
 struct BdptBufs {
-    BVert* eye;       // [vertex][sample]
+    BVert* eye;       // [sample][vertex], vslot()
     BVert* light;
-    float4* eye_mis;  // [vertex][sample]: pdf_fwd, pdf_rev, flags, 0 -- what the MIS walk reads
+    float4* eye_mis;  // [sample][vertex]: pdf_fwd, pdf_rev, flags, 0 -- what the MIS walk reads
     float4* light_mis;
     uint8_t* n_eye;
     uint8_t* n_light;
@@ -1580,6 +1583,11 @@ struct BdptParams {
     float film_area;  // (2 tan)(2 tan y_ratio) at plane distance 1
     uint64_t light_seed;
 };
+
+// A subpath vertex's slot: the vertices of one sample's subpath are
+// contiguous (cap = the subpath's vertex cap), so the connections of a
+// sample read its few vertices and MIS records from a few cache lines.
+__device__ inline size_t vslot(uint32_t j, int v, uint32_t cap) { return (size_t)j * cap + (uint32_t)v; }
 
 __device__ inline V3 bv_p(const BVert& v) { return v3(v.a.x, v.a.y, v.a.z); }
 __device__ inline V3 bv_n(const BVert& v) { return v3(v.b.x, v.b.y, v.b.z); }
@@ -1700,7 +1708,7 @@ __device__ inline void surface_at(const SceneView& sv, int32_t prim, V3 o, V3 d,
     tex = pattern(sv, s, p, bu, bv, tex_index);
 }
 
-__device__ inline void store_vert(BVert* arr, float4* mis, uint32_t idx, V3 p, float pdf_fwd, V3 n,
+__device__ inline void store_vert(BVert* arr, float4* mis, size_t idx, V3 p, float pdf_fwd, V3 n,
                                   V3 beta, uint32_t flags, V3 tex, int mat, float bw = 0.0f) {
     BVert v;
     v.a = make_float4(p.x, p.y, p.z, pdf_fwd);
@@ -1725,7 +1733,7 @@ struct WalkState {
 };
 
 template <bool EYE>
-__device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint32_t nb, uint32_t j,
+__device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint32_t cap, uint32_t j,
                              WalkState& st, int32_t prim, float t, float bu, float bv, Rng& g) {
     int& nv = st.nv;
     bool& in_obj = st.in_obj;
@@ -1751,14 +1759,14 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
             if (ev == kScatter) {
                 // a medium vertex: HomogeneousVolume::PhaseSampling, Fp / pdf = sigma_s
                 const V3 mp = o + d * (tn < 0.0f ? sd : tn + sd);
-                const uint32_t mslot = (uint32_t)nv * nb + j;
+                const size_t mslot = vslot(j, nv, cap);
                 store_vert(arr, mis, mslot, mp, to_meas(md, pdf_w, prev_p, mp, d, true, sd), d, beta,
                            kVMedium, v3(1, 1, 1), -1);
                 ++nv;
                 float pdf_ph;
                 const V3 w = phase_sample(md.g, d, g, pdf_ph);
                 if (nv >= 2) {
-                    const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
+                    const size_t pi = vslot(j, nv - 2, cap);
                     BVert& pv = arr[pi];
                     pv.b.w = to_meas(md, pdf_ph, mp, pv, sd);
                     mis[pi].y = pv.b.w;
@@ -1795,10 +1803,10 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
             int tix;
             const V3 txs = sv.shapes[prim].info[2] >= 0 ? pattern(sv, sv.shapes[prim], ps, bu, bv, tix)
                                                        : v3(1, 1, 1);
-            store_vert(arr, mis, (uint32_t)nv * nb + j, ps, 0.0f, d, beta, kVDelta, txs, mi);
+            store_vert(arr, mis, vslot(j, nv, cap), ps, 0.0f, d, beta, kVDelta, txs, mi);
             ++nv;
             if (nv >= 2) {
-                const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
+                const size_t pi = vslot(j, nv - 2, cap);
                 arr[pi].b.w = 0.0f;
                 mis[pi].y = 0.0f;
             }
@@ -1818,12 +1826,12 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
         // The walk ends on an emitter (bidirectionalpathtracer.cpp:693-699).
         // an emitter vertex is never scattered from, so its reverse-pdf
         // word carries the primitive id instead
-        store_vert(arr, mis, (uint32_t)nv * nb + j, p, pdf_fwd, n, beta, kVEmitter, tex, mi,
+        store_vert(arr, mis, vslot(j, nv, cap), p, pdf_fwd, n, beta, kVEmitter, tex, mi,
                    __int_as_float(prim));
         ++nv;
         return false;
     }
-    const uint32_t slot = (uint32_t)nv * nb + j;
+    const size_t slot = vslot(j, nv, cap);
     ++nv;
     const bool was_in = in_obj;
     const V3 wo = -d;
@@ -1923,7 +1931,7 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
     store_vert(arr, mis, slot, p, pdf_fwd, n, beta, vflags | (delta ? kVDelta : 0u), tex, mi);
     // reverse pdf of the previous vertex (area measure at prev)
     if (nv >= 2) {
-        const uint32_t pi = (uint32_t)(nv - 2) * nb + j;
+        const size_t pi = vslot(j, nv - 2, cap);
         BVert& pv = arr[pi];
         pv.b.w = delta ? 0.0f : to_meas(md, pdf_rev, p, pv, seg_L);
         mis[pi].y = pv.b.w;
@@ -1999,7 +2007,8 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_init(BdptBufs bb, uint64_t base
                   cam.R[6] * l.x + cam.R[7] * l.y + cam.R[8] * l.z);
         const V3 cp = v3(cam.p[0], cam.p[1], cam.p[2]);
         const V3 cdir = v3(cam.R[2], cam.R[5], cam.R[8]);
-        store_vert(bb.eye, bb.eye_mis, j, cp, 1.0f, cdir, v3(1, 1, 1), kVCamera, v3(1, 1, 1), -1);
+        store_vert(bb.eye, bb.eye_mis, vslot(j, 0, bp.n_eye), cp, 1.0f, cdir, v3(1, 1, 1), kVCamera, v3(1, 1, 1),
+                   -1);
         // Pinhole::Pdfw: 1 / (cos^3 film_area) at plane distance 1
         st.pdf_w = 1.0f / (l.z * l.z * l.z * bp.film_area);
         st.o = st.prev_p = cp;
@@ -2027,8 +2036,8 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_init(BdptBufs bb, uint64_t base
             }
             const float pdf_pos = 1.0f / (float)sv.n_lights * L.p[3];
             const V3 le = ld3(L.le);
-            store_vert(bb.light, bb.light_mis, j, p0, pdf_pos, n0, le * (1.0f / pdf_pos), kVLight, v3(1, 1, 1),
-                       -1);
+            store_vert(bb.light, bb.light_mis, vslot(j, 0, bp.n_light), p0, pdf_pos, n0, le * (1.0f / pdf_pos),
+                       kVLight, v3(1, 1, 1), -1);
             float cost, pdf_dir;
             st.d = lambert_sample(n0, g, cost, pdf_dir);
             st.beta = le * (cost / (pdf_pos * pdf_dir));
@@ -2069,8 +2078,8 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_walk(BdptBufs bb, uint64_t base
         const int2 h = hits[i];
         const float2 uv = uvs[i];
         const float t = __int_as_float(h.y);
-        bool go = light ? walk_segment<false>(sv, bb.light, bb.light_mis, bb.nb, j, st, h.x, t, uv.x, uv.y, g)
-                        : walk_segment<true>(sv, bb.eye, bb.eye_mis, bb.nb, j, st, h.x, t, uv.x, uv.y, g);
+        bool go = light ? walk_segment<false>(sv, bb.light, bb.light_mis, bp.n_light, j, st, h.x, t, uv.x, uv.y, g)
+                        : walk_segment<true>(sv, bb.eye, bb.eye_mis, bp.n_eye, j, st, h.x, t, uv.x, uv.y, g);
         go = go && st.nv < (light ? bp.n_light : bp.n_eye);
         if (!go) {
             if (light)
@@ -2082,28 +2091,72 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_walk(BdptBufs bb, uint64_t base
     }
 }
 
+// Connection work list.  After the walks, the samples of a batch are put in
+// the order of their subpath lengths, (n_eye, n_light) both descending
+// (k_bdpt_hist counts each pair, the host turns the counts into the first
+// position of each pair, k_bdpt_scatter writes every sample's index at its
+// position).  k_bdpt_connect then takes the samples of one pair 64 at a time
+// ("tiles"): every sample of a tile can make the same strategies, so each
+// wave runs one strategy over a whole tile (no idle lanes, no divergence
+// between strategies), and the block's waves go through all of the tile's
+// strategies while its subpaths sit in the caches.  Both kernels give every
+// block the same contiguous range of samples.
+constexpr uint32_t kMaxBins = 65u * 65u;  // (n_eye, n_light) pairs, caps of 64
+
+__device__ inline void bin_range(uint32_t nb, uint32_t& j0, uint32_t& j1) {
+    const uint32_t per = (nb + gridDim.x - 1) / gridDim.x;
+    j0 = blockIdx.x * per;
+    j1 = min(nb, j0 + per);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bdpt_hist(const uint8_t* n_eye, const uint8_t* n_light, uint32_t nb,
+                                                      uint32_t l1, uint32_t nbins, uint32_t* hist) {
+    __shared__ uint32_t h[kMaxBins];
+    for (uint32_t i = threadIdx.x; i < nbins; i += kBlock) h[i] = 0u;
+    __syncthreads();
+    uint32_t j0, j1;
+    bin_range(nb, j0, j1);
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += kBlock) atomicAdd(&h[n_eye[j] * l1 + n_light[j]], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nbins; i += kBlock)
+        if (h[i]) atomicAdd(hist + i, h[i]);
+}
+
+// cursor[bin] holds the bin's first position on entry; each block reserves
+// a run per bin for its samples, then places them.
+__global__ __launch_bounds__(kBlock) void k_bdpt_scatter(const uint8_t* n_eye, const uint8_t* n_light,
+                                                         uint32_t nb, uint32_t l1, uint32_t nbins,
+                                                         uint32_t* cursor, uint32_t* perm) {
+    __shared__ uint32_t h[kMaxBins];
+    for (uint32_t i = threadIdx.x; i < nbins; i += kBlock) h[i] = 0u;
+    __syncthreads();
+    uint32_t j0, j1;
+    bin_range(nb, j0, j1);
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += kBlock) atomicAdd(&h[n_eye[j] * l1 + n_light[j]], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nbins; i += kBlock)
+        if (h[i]) h[i] = atomicAdd(cursor + i, h[i]);
+    __syncthreads();
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += kBlock)
+        perm[atomicAdd(&h[n_eye[j] * l1 + n_light[j]], 1u)] = j;
+}
+
 __device__ inline float remap0(float f) { return f != 0.0f ? f : 1.0f; }
 
-__device__ inline float mis_pow(float r, float beta) { return beta == 1.0f ? r : powf(r, beta); }
+__device__ inline float mis_pow(float r, float beta) {
+    return beta == 1.0f ? r : (beta == 2.0f ? r * r : powf(r, beta));
+}
 
-// Connections of strategies q0 .. q0 + n_strat - 1: contributions that need
-// no visibility test go to the film, the others are appended to the shadow
-// queue `sh` (one atomic per wave) for k_shadow, which resolves them with
-// persistent lanes as the path tracer's shadow rays are.
-__global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t base, BdptParams bp,
-                                                         SceneView sv, const int2* strategies,
-                                                         uint32_t q0, uint32_t n_strat, float* film,
-                                                         ShadowBuf sh, uint32_t* sh_count) {
-    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (gid >= (uint64_t)n_strat * bb.nb) return;
-    const uint32_t q = q0 + (uint32_t)(gid / bb.nb);
-    const uint32_t j = (uint32_t)(gid % bb.nb);
-    const int s = strategies[q].x, t = strategies[q].y;
-    const int nE = bb.n_eye[j], nL = bb.n_light[j];
-    if (t > nE || s > nL) return;
-    const uint32_t nb = bb.nb;
+// Connection (s, t) of sample j, whose subpaths have at least t eye and s
+// light vertices: a contribution that needs no visibility test goes to the
+// film, the others are appended to the shadow queue `sh` (one atomic per
+// wave) for k_shadow, which resolves them with persistent lanes as the path
+// tracer's shadow rays are.
+__device__ void connect_one(const BdptBufs& bb, uint64_t base, const BdptParams& bp, const SceneView& sv,
+                            int s, int t, uint32_t j, float* film, const ShadowBuf& sh, uint32_t* sh_count) {
+    const uint32_t nEc = (uint32_t)bp.n_eye, nLc = (uint32_t)bp.n_light;
     const Params& prm = bp.base;
-    const BVert pt = bb.eye[(uint32_t)(t - 1) * nb + j];
+    const BVert pt = bb.eye[vslot(j, t - 1, nEc)];
     const uint32_t ptf = bv_flags(pt);
     V3 c;
     // the sample's own pixel; a t = 1 connection splats elsewhere
@@ -2116,7 +2169,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
     float medL = 0.0f;  // medium length of the connecting segment
     if (s == 0) {
         if (!(ptf & kVEmitter) || t < 2) return;
-        const BVert pm = bb.eye[(uint32_t)(t - 2) * nb + j];
+        const BVert pm = bb.eye[vslot(j, t - 2, nEc)];
         const V3 to_prev = normalize(bv_p(pm) - bv_p(pt));
         const DevMaterial& m = sv.mats[bv_mat(pt)];
         // Emissive::DirectLighting front-face rule (emissive.cpp:60-74)
@@ -2135,7 +2188,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
                 ptm_rev = to_meas(md, kInvPiF * cl, bv_p(pt), pm, seg_medium(md, bv_p(pt), bv_p(pm)));
         }
     } else {
-        const BVert qs = bb.light[(uint32_t)(s - 1) * nb + j];
+        const BVert qs = bb.light[vslot(j, s - 1, nLc)];
         const uint32_t qsf = bv_flags(qs);
         if ((qsf & kVDelta) || (qsf & kVEmitter)) return;
         if (t == 1) {
@@ -2152,7 +2205,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
             if (s == 1) {
                 fL = v3(1, 1, 1);
             } else {
-                const BVert qm = bb.light[(uint32_t)(s - 2) * nb + j];
+                const BVert qm = bb.light[vslot(j, s - 2, nLc)];
                 fL = bsdf_f(sv, qs, to_cam, normalize(bv_p(qm) - bv_p(qs)));
                 qsm_rev = to_meas(md, bsdf_pdf(sv, qs, normalize(bv_p(qm) - bv_p(qs)), to_cam), bv_p(qs), qm,
                                   seg_medium(md, bv_p(qs), bv_p(qm)));
@@ -2168,7 +2221,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
             vb = cp;
         } else {
             if ((ptf & kVDelta) || (ptf & kVEmitter)) return;
-            const BVert pm = bb.eye[(uint32_t)(t - 2) * nb + j];
+            const BVert pm = bb.eye[vslot(j, t - 2, nEc)];
             const V3 d_pq = bv_p(qs) - bv_p(pt);
             const float d2 = dot(d_pq, d_pq);
             const V3 w_pq = d_pq * (1.0f / sqrtf(d2));
@@ -2183,7 +2236,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
                 fL = v3(1, 1, 1);
                 pt_rev = to_meas(md, kInvPiF * cos_q, bv_p(qs), pt, medL);
             } else {
-                const BVert qm = bb.light[(uint32_t)(s - 2) * nb + j];
+                const BVert qm = bb.light[vslot(j, s - 2, nLc)];
                 const V3 to_qm = normalize(bv_p(qm) - bv_p(qs));
                 fL = bsdf_f(sv, qs, -w_pq, to_qm);
                 pt_rev = to_meas(md, bsdf_pdf(sv, qs, -w_pq, to_qm), bv_p(qs), pt, medL);
@@ -2209,9 +2262,9 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
         // each vertex once
         float sum = 0.0f, ri = 1.0f;
         if (t - 1 > 0) {
-            float4 mi = bb.eye_mis[(uint32_t)(t - 1) * nb + j];
+            float4 mi = bb.eye_mis[vslot(j, t - 1, nEc)];
             for (int i = t - 1; i > 0; --i) {
-                const float4 mp = bb.eye_mis[(uint32_t)(i - 1) * nb + j];
+                const float4 mp = bb.eye_mis[vslot(j, i - 1, nEc)];
                 const float pr = i == t - 1 ? pt_rev : (i == t - 2 ? ptm_rev : mi.y);
                 ri *= remap0(pr) / remap0(mi.x);
                 const bool di = i == t - 1 ? false : ((uint32_t)__float_as_int(mi.z) & kVDelta) != 0;
@@ -2223,9 +2276,9 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
         }
         ri = 1.0f;
         if (s - 1 >= 0) {
-            float4 mi = bb.light_mis[(uint32_t)(s - 1) * nb + j];
+            float4 mi = bb.light_mis[vslot(j, s - 1, nLc)];
             for (int i = s - 1; i >= 0; --i) {
-                const float4 mp = i > 0 ? bb.light_mis[(uint32_t)(i - 1) * nb + j] : make_float4(0, 0, 0, 0);
+                const float4 mp = i > 0 ? bb.light_mis[vslot(j, i - 1, nLc)] : make_float4(0, 0, 0, 0);
                 const float pr = i == s - 1 ? qs_rev : (i == s - 2 ? qsm_rev : mi.y);
                 ri *= remap0(pr) / remap0(mi.x);
                 const bool di = i == s - 1 ? false : ((uint32_t)__float_as_int(mi.z) & kVDelta) != 0;
@@ -2260,6 +2313,43 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(BdptBufs bb, uint64_t b
     sh.contrib[slot] = make_float4(cw.x, cw.y, cw.z, __int_as_float((int32_t)pixel));
 }
 
+constexpr int kConnBlock = 256;  // 4 waves per tile of 64 samples
+
+// The connections of tiles tile0 .. tile0 + gridDim.x - 1, one tile per
+// block.  bins[k] = (n_eye, n_light, first position, sample count) of the
+// k-th non-empty length pair in position order, bin_tile[k] its first tile.
+// The strategies of a tile are all (s, t) with t <= n_eye, s <= n_light,
+// s + t >= 2, s + t - 1 <= max_len, except (1, 1) (the camera ray reaching
+// the light covers that path); wave w of the block takes every fourth.
+__global__ __launch_bounds__(kConnBlock) void k_bdpt_connect(BdptBufs bb, uint64_t base, BdptParams bp,
+                                                             SceneView sv, const int4* bins,
+                                                             const uint32_t* bin_tile, uint32_t n_bins,
+                                                             uint32_t tile0, const uint32_t* perm, float* film,
+                                                             ShadowBuf sh, uint32_t* sh_count) {
+    const uint32_t tile = tile0 + blockIdx.x;
+    uint32_t lo = 0, hi = n_bins - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (bin_tile[mid] <= tile)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int4 bn = bins[lo];
+    const uint32_t first = (tile - bin_tile[lo]) * 64u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const bool live = first + lane < (uint32_t)bn.w;
+    const uint32_t j = live ? perm[(uint32_t)bn.z + first + lane] : 0u;
+    const int e = bn.x, l = bn.y;
+    uint32_t k = 0;
+    for (int t = 1; t <= e; ++t)
+        for (int s = 0; s <= l && s + t - 1 <= bp.max_len; ++s) {
+            if (s + t < 2 || (s == 1 && t == 1)) continue;
+            if ((k++ & 3u) != wave) continue;
+            if (live) connect_one(bb, base, bp, sv, s, t, j, film, sh, sh_count);
+        }
+}
+
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 // Clears the walk queue the next iteration appends to and k_extend's counter.
@@ -2289,6 +2379,62 @@ SceneView scene_view(const rumi_scene* sc) {
 }
 
 
+// One launch of k_bdpt_connect: tiles tile0 .. tile1 - 1 of the batch.
+struct ConnChunk {
+    uint32_t tile0, tile1;
+    size_t conns;  // connections of the chunk's tiles (64 per tile and strategy)
+};
+
+// Plans a batch's connections from its (n_eye, n_light) counts (hist, index
+// n_eye * (nL + 1) + n_light; strategies as k_bdpt_connect enumerates them).
+// cursor[0 .. nbins) = the first position of each pair in the (n_eye,
+// n_light) descending order (k_bdpt_scatter's cursors); bins = (n_eye,
+// n_light, first position, count) of each non-empty pair in that order with
+// its first tile in bin_tile; chunks of tiles whose connections (64 per
+// tile and strategy) fit max_conn.
+void conn_plan(const std::vector<uint32_t>& hist, int nE, int nL, int max_len, size_t max_conn,
+               std::vector<uint32_t>& cursor, std::vector<int4>& bins, std::vector<uint32_t>& bin_tile,
+               std::vector<ConnChunk>& chunks) {
+    const uint32_t l1 = (uint32_t)nL + 1u, nbins = (uint32_t)(nE + 1) * l1;
+    cursor.assign(nbins, 0u);
+    bins.clear();
+    bin_tile.clear();
+    chunks.clear();
+    uint32_t pos = 0, tiles = 0;
+    size_t in_chunk = 0;
+    ConnChunk cur{0, 0, 0};
+    for (int e = nE; e >= 0; --e)
+        for (int l = nL; l >= 0; --l) {
+            const uint32_t cnt = hist[e * l1 + l];
+            cursor[e * l1 + l] = pos;
+            if (cnt > 0) {
+                // the strategies k_bdpt_connect enumerates for this pair
+                size_t n_s = 0;
+                for (int t = 1; t <= e; ++t)
+                    for (int s = 0; s <= l && s + t - 1 <= max_len; ++s)
+                        if (s + t >= 2 && !(s == 1 && t == 1)) ++n_s;
+                bins.push_back(make_int4(e, l, (int)pos, (int)cnt));
+                bin_tile.push_back(tiles);
+                const uint32_t nt = (cnt + 63u) / 64u;
+                for (uint32_t k = 0; k < nt; ++k) {
+                    if (in_chunk > 0 && in_chunk + 64 * n_s > max_conn) {
+                        cur.tile1 = tiles + k;
+                        cur.conns = in_chunk;
+                        chunks.push_back(cur);
+                        cur.tile0 = tiles + k;
+                        in_chunk = 0;
+                    }
+                    in_chunk += 64 * n_s;
+                }
+                tiles += nt;
+            }
+            pos += cnt;
+        }
+    cur.tile1 = tiles;
+    cur.conns = in_chunk;
+    if (cur.tile1 > cur.tile0) chunks.push_back(cur);
+}
+
 int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
                 rumi_render_stats* stats, hipStream_t stream) {
     const int nE = desc->n_eye > 0 ? desc->n_eye : desc->max_depth + 1;
@@ -2296,31 +2442,29 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     if (nE < 2 || nE > 64 || nL < 1 || nL > 64)
         return fail(RUMI_ERR_INVALID, "rumi_render: BDPT vertex caps must be in [2,64] (eye), [1,64] (light)");
     if (!(desc->mis_beta > 0.0f)) return fail(RUMI_ERR_INVALID, "rumi_render: mis_beta must be > 0");
+    // strategies: all (s, t) with t >= 1, s + t >= 2, path edges
+    // s + t - 1 <= max_len, except (1, 1), that the sample's subpaths reach
+    // (k_bdpt_connect)
     const int max_len = desc->max_depth;
-    // all (s, t) with t >= 1, s + t >= 2, path edges s + t - 1 <= max_len,
-    // except (1, 1) (the camera ray hitting the light covers that path)
-    std::vector<int2> strat;
-    for (int t = 1; t <= nE; ++t)
-        for (int s = 0; s <= nL; ++s) {
-            if (s + t < 2 || (s == 1 && t == 1) || s + t - 1 > max_len) continue;
-            strat.push_back(make_int2(s, t));
-        }
     const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
     const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
     uint32_t nb = desc->queue_capacity ? desc->queue_capacity : (1u << 22);
     if ((uint64_t)nb > total) nb = (uint32_t)total;
     const size_t vbytes = sizeof(BVert) * (size_t)nb;
     const size_t mbytes = sizeof(float4) * (size_t)nb;
-    // connections are made kConnChunk strategies at a time; each chunk's
-    // visibility tests are queued (48 B each) and resolved before the next
-    const uint32_t kConnChunk = std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)strat.size(),
-                                                                           (1u << 25) / nb));
-    const size_t max_conn = (size_t)kConnChunk * nb;
+    // connections are made in chunks of at most max_conn (sample, strategy)
+    // pairs; each chunk's visibility tests are queued (48 B each) and
+    // resolved before the next
+    const size_t max_conn = std::max<size_t>(nb, (size_t)1 << 25);
     // walks: eye and light walks of a batch in one double-buffered queue
     // (80 B per walk and buffer) with their closest hits (16 B)
     const size_t walks = 2 * (size_t)nb;
-    const size_t need = (vbytes + mbytes) * (size_t)(nE + nL) + 2 * (size_t)nb + sizeof(int2) * strat.size() +
-                        3 * sizeof(float4) * max_conn + walks * (2 * 80 + 16) + 32 * 256;
+    // the connection work list: (n_eye, n_light) counts and cursors, the
+    // samples in that order, the non-empty pairs and their first tiles
+    const uint32_t l1 = (uint32_t)nL + 1u, nbins = (uint32_t)(nE + 1) * l1;
+    const size_t need = (vbytes + mbytes) * (size_t)(nE + nL) + 2 * (size_t)nb + 4 * (size_t)nb +
+                        (4 + 4 + 16 + 4) * (size_t)nbins + 3 * sizeof(float4) * max_conn +
+                        walks * (2 * 80 + 16) + 40 * 256;
     if (sc->work_bytes < need) {
         if (sc->work) (void)hipFree(sc->work);
         sc->work = nullptr;
@@ -2342,7 +2486,11 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     bb.light_mis = (float4*)take(mbytes * (size_t)nL);
     bb.n_eye = (uint8_t*)take(nb);
     bb.n_light = (uint8_t*)take(nb);
-    int2* d_strat = (int2*)take(sizeof(int2) * strat.size());
+    uint32_t* perm = (uint32_t*)take(4 * (size_t)nb);
+    uint32_t* d_hist = (uint32_t*)take(4 * (size_t)nbins);
+    uint32_t* d_cursor = (uint32_t*)take(4 * (size_t)nbins);
+    int4* d_bins = (int4*)take(16 * (size_t)nbins);
+    uint32_t* d_bin_tile = (uint32_t*)take(4 * (size_t)nbins);
     ShadowBuf sh{};
     sh.ray = (float4*)take(2 * sizeof(float4) * max_conn);
     sh.contrib = (float4*)take(sizeof(float4) * max_conn);
@@ -2363,9 +2511,6 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     const bool counting = (desc->flags & RUMI_RENDER_COUNT_TRAVERSAL) != 0;
     if ((size_t)(w - (char*)sc->work) > sc->work_bytes)
         return fail(RUMI_ERR_INVALID, "rumi_render: internal BDPT work-buffer layout overflow");
-    if (hipMemcpyAsync(d_strat, strat.data(), sizeof(int2) * strat.size(), hipMemcpyHostToDevice,
-                       stream) != hipSuccess)
-        return fail(RUMI_ERR_HIP, "rumi_render: strategy upload failed");
 
     SceneView sv = scene_view(sc);
     BdptParams bp{};
@@ -2390,6 +2535,10 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     hipLaunchKernelGGL(k_zero, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n);
     if (counting) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)counts, 12u);
     uint64_t batches = 0;
+    int rc = RUMI_OK;
+    std::vector<uint32_t> h_hist(nbins), h_cursor, h_bin_tile;
+    std::vector<int4> h_bins;
+    std::vector<ConnChunk> chunks;
     for (uint64_t base = 0; base < total; base += nb) {
         bb.nb = (uint32_t)std::min<uint64_t>(nb, total - base);
         // the walks as a wavefront: each iteration finds the closest hit of
@@ -2420,13 +2569,37 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
                                (const uint32_t*)(wctl + cur), (const int2*)whits, (const float2*)wuvs,
                                wb[cur ^ 1], wctl + (cur ^ 1));
         }
-        for (uint32_t q0 = 0; q0 < (uint32_t)strat.size(); q0 += kConnChunk) {
-            const uint32_t nq = std::min<uint32_t>(kConnChunk, (uint32_t)strat.size() - q0);
-            const uint64_t threads = (uint64_t)bb.nb * nq;
+        // the samples in (n_eye, n_light) order, and the chunks of tiles
+        const uint32_t gb = std::min<uint32_t>(grid_of(bb.nb), 1024u);
+        hipLaunchKernelGGL(k_zero_u32, dim3((nbins + 255) / 256), dim3(256), 0, stream, d_hist, nbins);
+        hipLaunchKernelGGL(k_bdpt_hist, dim3(gb), dim3(kBlock), 0, stream, (const uint8_t*)bb.n_eye,
+                           (const uint8_t*)bb.n_light, bb.nb, l1, nbins, d_hist);
+        if (hipMemcpyAsync(h_hist.data(), d_hist, 4 * (size_t)nbins, hipMemcpyDeviceToHost, stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            rc = fail(RUMI_ERR_HIP, "rumi_render: subpath length counts: %s", hipGetErrorString(hipGetLastError()));
+            break;
+        }
+        conn_plan(h_hist, nE, nL, max_len, max_conn, h_cursor, h_bins, h_bin_tile, chunks);
+        if (hipMemcpyAsync(d_cursor, h_cursor.data(), 4 * h_cursor.size(), hipMemcpyHostToDevice, stream) !=
+                hipSuccess ||
+            (!h_bins.empty() &&
+             (hipMemcpyAsync(d_bins, h_bins.data(), sizeof(int4) * h_bins.size(), hipMemcpyHostToDevice,
+                             stream) != hipSuccess ||
+              hipMemcpyAsync(d_bin_tile, h_bin_tile.data(), 4 * h_bin_tile.size(), hipMemcpyHostToDevice,
+                             stream) != hipSuccess))) {
+            rc = fail(RUMI_ERR_HIP, "rumi_render: work-list upload failed");
+            break;
+        }
+        hipLaunchKernelGGL(k_bdpt_scatter, dim3(gb), dim3(kBlock), 0, stream, (const uint8_t*)bb.n_eye,
+                           (const uint8_t*)bb.n_light, bb.nb, l1, nbins, d_cursor, perm);
+        for (const ConnChunk& ch : chunks) {
+            const uint32_t tiles = ch.tile1 - ch.tile0;
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, qctl, 2u);
-            hipLaunchKernelGGL(k_bdpt_connect, dim3(grid_of(threads)), dim3(kBlock), 0, stream, bb, base, bp,
-                               sv, (const int2*)d_strat, q0, nq, film, sh, qctl);
-            const uint32_t g = (uint32_t)std::min<uint64_t>((threads + kBlock - 1) / kBlock,
+            hipLaunchKernelGGL(k_bdpt_connect, dim3(tiles), dim3(kConnBlock), 0, stream, bb, base, bp, sv,
+                               (const int4*)d_bins, (const uint32_t*)d_bin_tile, (uint32_t)h_bins.size(), ch.tile0,
+                               (const uint32_t*)perm, film, sh, qctl);
+            const uint32_t g = (uint32_t)std::min<uint64_t>((ch.conns + kBlock - 1) / kBlock,
                                                             (uint64_t)ctx->cu_count * 16u);
             with_stack(stack, [&](auto v) {
                 if (counting)
@@ -2441,13 +2614,13 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
         }
         ++batches;
     }
-    hipLaunchKernelGGL(k_scale, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n,
-                       1.0f / (float)desc->spp);
+    if (rc == RUMI_OK)
+        hipLaunchKernelGGL(k_scale, dim3((uint32_t)((film_n + 255) / 256)), dim3(256), 0, stream, film, film_n,
+                           1.0f / (float)desc->spp);
     (void)hipEventRecord(ev1, stream);
-    int rc = RUMI_OK;
-    if (hipStreamSynchronize(stream) != hipSuccess)
+    if (hipStreamSynchronize(stream) != hipSuccess && rc == RUMI_OK)
         rc = fail(RUMI_ERR_HIP, "rumi_render: %s", hipGetErrorString(hipGetLastError()));
-    else if (hipGetLastError() != hipSuccess)
+    else if (hipGetLastError() != hipSuccess && rc == RUMI_OK)
         rc = fail(RUMI_ERR_HIP, "rumi_render: a BDPT kernel launch failed");
     float ms = 0.0f;
     if (rc == RUMI_OK) (void)hipEventElapsedTime(&ms, ev0, ev1);
