@@ -214,19 +214,6 @@ struct Ctl {
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
-// Wave-aggregated append: one atomic per wave, lanes get consecutive slots.
-__device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0;
-    const uint32_t cnt = (uint32_t)__popcll(mask);
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane_id() == leader) base = atomicAdd(counter, cnt);
-    base = __shfl(base, leader);
-    const uint64_t below = lane_id() == 0 ? 0ull : (mask & (~0ull >> (64 - lane_id())));
-    return base + (uint32_t)__popcll(below);
-}
-
 // Adds a lane's traversal counts to two 64-bit totals, one atomic per wave.
 __device__ inline void add_counts(unsigned long long* counts, uint32_t nodes, uint32_t prims) {
     uint64_t a = nodes, b = prims;
@@ -1959,10 +1946,8 @@ struct WalkBuf {
 
 constexpr uint32_t kWalkLight = 4u;
 
-__device__ inline void push_walk(const WalkBuf& wb, uint32_t* count, bool pred, const WalkState& st,
-                                 uint64_t rng, uint32_t light, uint32_t j) {
-    const uint32_t q = wave_append(count, pred);
-    if (!pred) return;
+__device__ inline void write_walk(const WalkBuf& wb, uint32_t q, const WalkState& st, uint64_t rng, uint32_t light,
+                                  uint32_t j) {
     wb.ray[2 * (size_t)q + 0] = make_float4(st.o.x, st.o.y, st.o.z, 0.0f);
     wb.ray[2 * (size_t)q + 1] = make_float4(st.d.x, st.d.y, st.d.z, FLT_MAX);
     wb.flags[q] = (st.in_obj ? 2u : 0u) | light | ((uint32_t)st.nv << 8);
@@ -1978,21 +1963,17 @@ __device__ inline void push_walk(const WalkBuf& wb, uint32_t* count, bool pred, 
 // direction, Scene::LightsUniformSampling scene.cpp:91-97 +
 // AreaLight::SolidAngleSampling arealight.cpp:31-60), queued for the walk
 // stages.
-__global__ __launch_bounds__(kBlock) void k_bdpt_init(BdptBufs bb, uint64_t base, BdptParams bp,
-                                                      SceneView sv, WalkBuf wb, uint32_t* count) {
-    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
-    if (w >= 2 * bb.nb) return;
+__device__ inline bool init_walk(const BdptBufs& bb, uint64_t base, const BdptParams& bp, const SceneView& sv,
+                                 uint32_t w, WalkState& st, Rng& g, uint32_t& j_out, bool& eye_out) {
     const Params& prm = bp.base;
     const bool eye = w < bb.nb;
     const uint32_t j = eye ? w : w - bb.nb;
     const uint64_t k = base + j;
     const uint32_t pixel = (uint32_t)(k % prm.pixels);
     const uint64_t s = k / prm.pixels;
-    WalkState st;
     st.nv = 1;
     st.in_obj = false;
     bool go = false;
-    Rng g;
     if (eye) {
         g = rng_seed(prm.seed + s * 0x9E3779B97F4A7C15ull, pixel);
         const DevCamera& cam = sv.cam;
@@ -2047,48 +2028,87 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_init(BdptBufs bb, uint64_t base
             go = bp.n_light > 1;
         }
     }
-    push_walk(wb, count, go, st, g.state, eye ? 0u : kWalkLight, j);
+    j_out = j;
+    eye_out = eye;
+    return go;
 }
 
-// One segment of every queued walk, its closest hit already found by
-// k_extend; walks that go on are appended to the next queue, the others
-// record their vertex count.
+__global__ __launch_bounds__(kBlock) void k_bdpt_init(BdptBufs bb, uint64_t base, BdptParams bp,
+                                                      SceneView sv, WalkBuf wb, uint32_t* count) {
+    __shared__ uint32_t s_tmp[kBlock / 64 + 1];
+    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+    WalkState st;
+    Rng g;
+    uint32_t j = 0;
+    bool eye = true, go = false;
+    if (w < 2 * bb.nb) go = init_walk(bb, base, bp, sv, w, st, g, j, eye);
+    const uint32_t q = block_append<kBlock>(count, go, s_tmp);
+    if (go) write_walk(wb, q, st, g.state, eye ? 0u : kWalkLight, j);
+}
+
+// The segment of queued walk i, its closest hit already found by k_extend
+// (k_bdpt_walk); true when the walk goes on, else its vertex count is
+// recorded.
+__device__ inline bool walk_one(const BdptBufs& bb, uint64_t base, const BdptParams& bp, const SceneView& sv,
+                                const WalkBuf& in, uint32_t i, const int2* hits, const float2* uvs, WalkState& st,
+                                Rng& g, uint32_t& j_out, bool& light_out) {
+    const uint32_t flags = in.flags[i];
+    const uint32_t j = in.slot[i];
+    const bool light = (flags & kWalkLight) != 0;
+    const uint32_t pixel = (uint32_t)((base + j) % bp.base.pixels);
+    g.state = in.rng[i];
+    g.inc = ((uint64_t)pixel << 1u) | 1u;
+    const float4 r0 = in.ray[2 * (size_t)i + 0], r1 = in.ray[2 * (size_t)i + 1];
+    const float4 b4 = in.beta[i], p4 = in.prev[i];
+    st.o = v3(r0.x, r0.y, r0.z);
+    st.d = v3(r1.x, r1.y, r1.z);
+    st.beta = v3(b4.x, b4.y, b4.z);
+    st.pdf_w = b4.w;
+    st.prev_p = v3(p4.x, p4.y, p4.z);
+    st.nv = (int)(flags >> 8);
+    st.in_obj = (flags & 2u) != 0;
+    const int2 h = hits[i];
+    const float2 uv = uvs[i];
+    const float t = __int_as_float(h.y);
+    bool go = light ? walk_segment<false>(sv, bb.light, bb.light_mis, bp.n_light, j, st, h.x, t, uv.x, uv.y, g)
+                    : walk_segment<true>(sv, bb.eye, bb.eye_mis, bp.n_eye, j, st, h.x, t, uv.x, uv.y, g);
+    go = go && st.nv < (light ? bp.n_light : bp.n_eye);
+    if (!go) {
+        if (light)
+            bb.n_light[j] = (uint8_t)st.nv;
+        else
+            bb.n_eye[j] = (uint8_t)st.nv;
+    }
+    j_out = j;
+    light_out = light;
+    return go;
+}
+
+// One segment of every queued walk; walks that go on are appended to the
+// next queue (one atomic per block).
 __global__ __launch_bounds__(kBlock) void k_bdpt_walk(BdptBufs bb, uint64_t base, BdptParams bp, SceneView sv,
                                                       WalkBuf in, const uint32_t* n_ptr, const int2* hits,
                                                       const float2* uvs, WalkBuf out, uint32_t* next_count) {
+    __shared__ uint32_t s_tmp[kBlock / 64 + 1];
     const uint32_t n = *n_ptr;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const uint32_t flags = in.flags[i];
-        const uint32_t j = in.slot[i];
-        const bool light = (flags & kWalkLight) != 0;
-        const uint32_t pixel = (uint32_t)((base + j) % bp.base.pixels);
-        Rng g;
-        g.state = in.rng[i];
-        g.inc = ((uint64_t)pixel << 1u) | 1u;
+    // block-uniform trip count: every thread reaches the block-aggregated append
+    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += gridDim.x * kBlock) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool go = false;
         WalkState st;
-        const float4 r0 = in.ray[2 * (size_t)i + 0], r1 = in.ray[2 * (size_t)i + 1];
-        const float4 b4 = in.beta[i], p4 = in.prev[i];
-        st.o = v3(r0.x, r0.y, r0.z);
-        st.d = v3(r1.x, r1.y, r1.z);
-        st.beta = v3(b4.x, b4.y, b4.z);
-        st.pdf_w = b4.w;
-        st.prev_p = v3(p4.x, p4.y, p4.z);
-        st.nv = (int)(flags >> 8);
-        st.in_obj = (flags & 2u) != 0;
-        const int2 h = hits[i];
-        const float2 uv = uvs[i];
-        const float t = __int_as_float(h.y);
-        bool go = light ? walk_segment<false>(sv, bb.light, bb.light_mis, bp.n_light, j, st, h.x, t, uv.x, uv.y, g)
-                        : walk_segment<true>(sv, bb.eye, bb.eye_mis, bp.n_eye, j, st, h.x, t, uv.x, uv.y, g);
-        go = go && st.nv < (light ? bp.n_light : bp.n_eye);
-        if (!go) {
-            if (light)
-                bb.n_light[j] = (uint8_t)st.nv;
-            else
-                bb.n_eye[j] = (uint8_t)st.nv;
-        }
-        push_walk(out, next_count, go, st, g.state, light ? kWalkLight : 0u, j);
+        Rng g;
+        uint32_t j = 0;
+        bool light = false;
+        if (i < n) go = walk_one(bb, base, bp, sv, in, i, hits, uvs, st, g, j, light);
+        const uint32_t q = block_append<kBlock>(next_count, go, s_tmp);
+        if (go) write_walk(out, q, st, g.state, light ? kWalkLight : 0u, j);
     }
+}
+
+__device__ inline float remap0(float f) { return f != 0.0f ? f : 1.0f; }
+
+__device__ inline float mis_pow(float r, float beta) {
+    return beta == 1.0f ? r : (beta == 2.0f ? r * r : powf(r, beta));
 }
 
 // Connection work list.  After the walks, the samples of a batch are put in
@@ -2141,19 +2161,20 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_scatter(const uint8_t* n_eye, c
         perm[atomicAdd(&h[n_eye[j] * l1 + n_light[j]], 1u)] = j;
 }
 
-__device__ inline float remap0(float f) { return f != 0.0f ? f : 1.0f; }
 
-__device__ inline float mis_pow(float r, float beta) {
-    return beta == 1.0f ? r : (beta == 2.0f ? r * r : powf(r, beta));
-}
+// A queued visibility test of a connection: the segment (origin, unit
+// direction, length) and the weighted contribution with its pixel.
+struct ShadowEntry {
+    float4 o, d, c;
+};
 
 // Connection (s, t) of sample j, whose subpaths have at least t eye and s
 // light vertices: a contribution that needs no visibility test goes to the
-// film, the others are appended to the shadow queue `sh` (one atomic per
-// wave) for k_shadow, which resolves them with persistent lanes as the path
-// tracer's shadow rays are.
-__device__ void connect_one(const BdptBufs& bb, uint64_t base, const BdptParams& bp, const SceneView& sv,
-                            int s, int t, uint32_t j, float* film, const ShadowBuf& sh, uint32_t* sh_count) {
+// film; for the others the visibility test is returned in e (true), to be
+// queued for k_shadow, which resolves them with persistent lanes as the
+// path tracer's shadow rays are.
+__device__ bool connect_one(const BdptBufs& bb, uint64_t base, const BdptParams& bp, const SceneView& sv, int s,
+                            int t, uint32_t j, float* film, ShadowEntry& e) {
     const uint32_t nEc = (uint32_t)bp.n_eye, nLc = (uint32_t)bp.n_light;
     const Params& prm = bp.base;
     const BVert pt = bb.eye[vslot(j, t - 1, nEc)];
@@ -2168,12 +2189,12 @@ __device__ void connect_one(const BdptBufs& bb, uint64_t base, const BdptParams&
     const DevMedium& md = sv.med;
     float medL = 0.0f;  // medium length of the connecting segment
     if (s == 0) {
-        if (!(ptf & kVEmitter) || t < 2) return;
+        if (!(ptf & kVEmitter) || t < 2) return false;
         const BVert pm = bb.eye[vslot(j, t - 2, nEc)];
         const V3 to_prev = normalize(bv_p(pm) - bv_p(pt));
         const DevMaterial& m = sv.mats[bv_mat(pt)];
         // Emissive::DirectLighting front-face rule (emissive.cpp:60-74)
-        if (!(dot(bv_n(pt), to_prev) > 0.0f)) return;
+        if (!(dot(bv_n(pt), to_prev) > 0.0f)) return false;
         c = cmul(v3(pt.c.x, pt.c.y, pt.c.z), ld3(m.color) * m.kd);
         const int prim = __float_as_int(pt.b.w);
         const int li = __float_as_int(sv.shapes[prim].uv2[3]);
@@ -2190,18 +2211,18 @@ __device__ void connect_one(const BdptBufs& bb, uint64_t base, const BdptParams&
     } else {
         const BVert qs = bb.light[vslot(j, s - 1, nLc)];
         const uint32_t qsf = bv_flags(qs);
-        if ((qsf & kVDelta) || (qsf & kVEmitter)) return;
+        if ((qsf & kVDelta) || (qsf & kVEmitter)) return false;
         if (t == 1) {
             // light tracing: connect qs to the pinhole, splat (t_idx == 0 case, :367-380)
             const V3 cp = v3(sv.cam.p[0], sv.cam.p[1], sv.cam.p[2]);
             const V3 dc = normalize(bv_p(qs) - cp);
             float cos_c;
-            if (!camera_project(sv.cam, prm.width, prm.height, dc, pixel, cos_c)) return;
+            if (!camera_project(sv.cam, prm.width, prm.height, dc, pixel, cos_c)) return false;
             const V3 to_cam = -dc;
             const float d2 = dot(bv_p(qs) - cp, bv_p(qs) - cp);
             V3 fL;
             const float cos_q = (qsf & kVMedium) ? 1.0f : dot(bv_n(qs), to_cam);
-            if (!(cos_q > 0.0f)) return;
+            if (!(cos_q > 0.0f)) return false;
             if (s == 1) {
                 fL = v3(1, 1, 1);
             } else {
@@ -2220,14 +2241,14 @@ __device__ void connect_one(const BdptBufs& bb, uint64_t base, const BdptParams&
             va = bv_p(qs);
             vb = cp;
         } else {
-            if ((ptf & kVDelta) || (ptf & kVEmitter)) return;
+            if ((ptf & kVDelta) || (ptf & kVEmitter)) return false;
             const BVert pm = bb.eye[vslot(j, t - 2, nEc)];
             const V3 d_pq = bv_p(qs) - bv_p(pt);
             const float d2 = dot(d_pq, d_pq);
             const V3 w_pq = d_pq * (1.0f / sqrtf(d2));
             const float cos_p = (ptf & kVMedium) ? 1.0f : dot(bv_n(pt), w_pq);
             const float cos_q = (qsf & kVMedium) ? 1.0f : -dot(bv_n(qs), w_pq);
-            if (!(cos_p > 0.0f && cos_q > 0.0f)) return;
+            if (!(cos_p > 0.0f && cos_q > 0.0f)) return false;
             const V3 to_pm = normalize(bv_p(pm) - bv_p(pt));
             const V3 fE = bsdf_f(sv, pt, w_pq, to_pm);
             V3 fL;
@@ -2254,7 +2275,7 @@ __device__ void connect_one(const BdptBufs& bb, uint64_t base, const BdptParams&
             vb = bv_p(pt);
         }
     }
-    if (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f) return;
+    if (c.x == 0.0f && c.y == 0.0f && c.z == 0.0f) return false;
     // MIS weight over all strategies of this path (power heuristic with exponent beta)
     float w = 1.0f;
     if (s + t > 2 && !mis_trivial) {
@@ -2295,22 +2316,22 @@ __device__ void connect_one(const BdptBufs& bb, uint64_t base, const BdptParams&
     if (medL > 0.0f) {
         Rng gc = rng_seed((prm.seed ^ kConnMix) + (base + j) * 0x9E3779B97F4A7C15ull,
                           ((uint64_t)s << 8) | (uint64_t)t);
-        if (gc.uniform() < 1.0f - expf(-md.avg_sigma_t * medL)) return;
+        if (gc.uniform() < 1.0f - expf(-md.avg_sigma_t * medL)) return false;
     }
     const V3 cw = c * w;
     if (!visible_test) {
         film_add(film, pixel, cw);
-        return;
+        return false;
     }
     // VisibilityTest (bidirectionalpathtracer.cpp:1018-1048): the segment
-    // va -> vb as an any-hit query, queued; every lane still here appends
+    // va -> vb as an any-hit query
     const V3 dv = vb - va;
     const float dist = length(dv);
     const V3 dir = dv * (1.0f / dist);
-    const uint32_t slot = wave_append(sh_count, true);
-    sh.ray[2 * (size_t)slot + 0] = make_float4(va.x, va.y, va.z, 0.0f);
-    sh.ray[2 * (size_t)slot + 1] = make_float4(dir.x, dir.y, dir.z, dist);
-    sh.contrib[slot] = make_float4(cw.x, cw.y, cw.z, __int_as_float((int32_t)pixel));
+    e.o = make_float4(va.x, va.y, va.z, 0.0f);
+    e.d = make_float4(dir.x, dir.y, dir.z, dist);
+    e.c = make_float4(cw.x, cw.y, cw.z, __int_as_float((int32_t)pixel));
+    return true;
 }
 
 constexpr int kConnBlock = 256;  // 4 waves per tile of 64 samples
@@ -2320,12 +2341,14 @@ constexpr int kConnBlock = 256;  // 4 waves per tile of 64 samples
 // k-th non-empty length pair in position order, bin_tile[k] its first tile.
 // The strategies of a tile are all (s, t) with t <= n_eye, s <= n_light,
 // s + t >= 2, s + t - 1 <= max_len, except (1, 1) (the camera ray reaching
-// the light covers that path); wave w of the block takes every fourth.
+// the light covers that path); the block takes them four at a time, one per
+// wave, and queues the group's visibility tests with one atomic.
 __global__ __launch_bounds__(kConnBlock) void k_bdpt_connect(BdptBufs bb, uint64_t base, BdptParams bp,
                                                              SceneView sv, const int4* bins,
                                                              const uint32_t* bin_tile, uint32_t n_bins,
                                                              uint32_t tile0, const uint32_t* perm, float* film,
                                                              ShadowBuf sh, uint32_t* sh_count) {
+    __shared__ uint32_t s_tmp[kConnBlock / 64 + 1];
     const uint32_t tile = tile0 + blockIdx.x;
     uint32_t lo = 0, hi = n_bins - 1;
     while (lo < hi) {
@@ -2341,13 +2364,32 @@ __global__ __launch_bounds__(kConnBlock) void k_bdpt_connect(BdptBufs bb, uint64
     const bool live = first + lane < (uint32_t)bn.w;
     const uint32_t j = live ? perm[(uint32_t)bn.z + first + lane] : 0u;
     const int e = bn.x, l = bn.y;
+    // the block-uniform enumeration; (ms, mt) = this wave's strategy of the
+    // current group of four
     uint32_t k = 0;
+    int ms = -1, mt = 0;
+    auto flush = [&]() {
+        ShadowEntry se;
+        const bool q = live && ms >= 0 && connect_one(bb, base, bp, sv, ms, mt, j, film, se);
+        const uint32_t slot = block_append<kConnBlock>(sh_count, q, s_tmp);
+        if (q) {
+            sh.ray[2 * (size_t)slot + 0] = se.o;
+            sh.ray[2 * (size_t)slot + 1] = se.d;
+            sh.contrib[slot] = se.c;
+        }
+        ms = -1;
+    };
     for (int t = 1; t <= e; ++t)
         for (int s = 0; s <= l && s + t - 1 <= bp.max_len; ++s) {
             if (s + t < 2 || (s == 1 && t == 1)) continue;
-            if ((k++ & 3u) != wave) continue;
-            if (live) connect_one(bb, base, bp, sv, s, t, j, film, sh, sh_count);
+            if ((k & 3u) == wave) {
+                ms = s;
+                mt = t;
+            }
+            if ((k & 3u) == 3u) flush();
+            ++k;
         }
+    if (k & 3u) flush();
 }
 
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }

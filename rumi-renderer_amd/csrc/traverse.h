@@ -646,6 +646,11 @@ __device__ __forceinline__ int32_t traverse(const float4* __restrict__ nodes,
 #define RUMI_RUN_CHUNK 256
 #endif
 constexpr uint32_t kRunChunk = RUMI_RUN_CHUNK;
+// Rays per atomic while more than 16 rays per lane of the grid remain:
+// queues many times the grid's width (the renderer's connection queues hold
+// up to 2^25 rays) would otherwise wait on the counter's same-address
+// atomics, which serialise.
+constexpr uint32_t kRunChunkBig = 1024;
 // Lanes with a fetched ray that start together (see run_lanes).
 #ifndef RUMI_START_BATCH
 #define RUMI_START_BATCH 16
@@ -682,6 +687,7 @@ __device__ __forceinline__ void run_lanes(uint32_t n, uint32_t* counter, Item& i
     // chunks keep them rare, and chunks shrink to a wave's width once fewer
     // than two rays per lane of the grid remain, to balance the tail.
     const uint32_t tail = gridDim.x * blockDim.x * 2u;
+    const uint64_t big_tail = (uint64_t)gridDim.x * blockDim.x * 16u;
     TravState s;
     bool active = false, has_next = false, drained = false, unfinished = false;
     uint32_t cbase = 0, cleft = 0, pend_val = 0, pend_sz = 0;  // wave-uniform chunk state
@@ -706,7 +712,7 @@ __device__ __forceinline__ void run_lanes(uint32_t n, uint32_t* counter, Item& i
             }
         }
         if (!pend && cleft < kRunChunk) {
-            pend_sz = cbase + tail < n ? kRunChunk : 64u;
+            pend_sz = (uint64_t)cbase + big_tail < n ? kRunChunkBig : (cbase + tail < n ? kRunChunk : 64u);
             if (lane == 0) pend_val = atomicAdd(counter, pend_sz);
             pend = true;
         }
