@@ -1582,7 +1582,7 @@ __device__ inline uint32_t bv_flags(const BVert& v) { return (uint32_t)__float_a
 __device__ inline int bv_mat(const BVert& v) { return __float_as_int(v.d.w); }
 
 // Medium length of the segment a -> b (0 without a medium).
-__device__ inline float seg_medium(const DevMedium& md, V3 a, V3 b) {
+__device__ __forceinline__ float seg_medium(const DevMedium& md, V3 a, V3 b) {
     if (!md.bound) return 0.0f;
     const V3 dv = b - a;
     const float dist = length(dv);
@@ -2033,17 +2033,22 @@ __device__ inline bool init_walk(const BdptBufs& bb, uint64_t base, const BdptPa
     return go;
 }
 
+// Every eye walk goes on after its first vertex (n_eye >= 2), and every
+// light walk does when the scene has a light and n_light >= 2, so the
+// queue's slots follow from the walk's index: eye walks first, then light
+// walks, and no counter is needed.
 __global__ __launch_bounds__(kBlock) void k_bdpt_init(BdptBufs bb, uint64_t base, BdptParams bp,
                                                       SceneView sv, WalkBuf wb, uint32_t* count) {
-    __shared__ uint32_t s_tmp[kBlock / 64 + 1];
     const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+    const bool eye_go = bp.n_eye > 1, light_go = sv.n_lights > 0 && bp.n_light > 1;
+    if (w == 0) *count = (eye_go ? bb.nb : 0u) + (light_go ? bb.nb : 0u);
+    if (w >= 2 * bb.nb) return;
     WalkState st;
     Rng g;
     uint32_t j = 0;
-    bool eye = true, go = false;
-    if (w < 2 * bb.nb) go = init_walk(bb, base, bp, sv, w, st, g, j, eye);
-    const uint32_t q = block_append<kBlock>(count, go, s_tmp);
-    if (go) write_walk(wb, q, st, g.state, eye ? 0u : kWalkLight, j);
+    bool eye = true;
+    const bool go = init_walk(bb, base, bp, sv, w, st, g, j, eye);
+    if (go) write_walk(wb, eye ? j : (eye_go ? bb.nb : 0u) + j, st, g.state, eye ? 0u : kWalkLight, j);
 }
 
 // The segment of queued walk i, its closest hit already found by k_extend
@@ -2085,14 +2090,17 @@ __device__ inline bool walk_one(const BdptBufs& bb, uint64_t base, const BdptPar
 }
 
 // One segment of every queued walk; walks that go on are appended to the
-// next queue (one atomic per block).
-__global__ __launch_bounds__(kBlock) void k_bdpt_walk(BdptBufs bb, uint64_t base, BdptParams bp, SceneView sv,
-                                                      WalkBuf in, const uint32_t* n_ptr, const int2* hits,
-                                                      const float2* uvs, WalkBuf out, uint32_t* next_count) {
-    __shared__ uint32_t s_tmp[kBlock / 64 + 1];
+// next queue (one atomic per block of 256: the queue counter's same-address
+// atomics serialise).
+constexpr int kWalkBlock = 256;
+
+__global__ __launch_bounds__(kWalkBlock) void k_bdpt_walk(BdptBufs bb, uint64_t base, BdptParams bp, SceneView sv,
+                                                          WalkBuf in, const uint32_t* n_ptr, const int2* hits,
+                                                          const float2* uvs, WalkBuf out, uint32_t* next_count) {
+    __shared__ uint32_t s_tmp[kWalkBlock / 64 + 1];
     const uint32_t n = *n_ptr;
     // block-uniform trip count: every thread reaches the block-aggregated append
-    for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += gridDim.x * kBlock) {
+    for (uint32_t i0 = blockIdx.x * kWalkBlock; i0 < n; i0 += gridDim.x * kWalkBlock) {
         const uint32_t i = i0 + threadIdx.x;
         bool go = false;
         WalkState st;
@@ -2100,7 +2108,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_walk(BdptBufs bb, uint64_t base
         uint32_t j = 0;
         bool light = false;
         if (i < n) go = walk_one(bb, base, bp, sv, in, i, hits, uvs, st, g, j, light);
-        const uint32_t q = block_append<kBlock>(next_count, go, s_tmp);
+        const uint32_t q = block_append<kWalkBlock>(next_count, go, s_tmp);
         if (go) write_walk(out, q, st, g.state, light ? kWalkLight : 0u, j);
     }
 }
@@ -2173,7 +2181,7 @@ struct ShadowEntry {
 // film; for the others the visibility test is returned in e (true), to be
 // queued for k_shadow, which resolves them with persistent lanes as the
 // path tracer's shadow rays are.
-__device__ bool connect_one(const BdptBufs& bb, uint64_t base, const BdptParams& bp, const SceneView& sv, int s,
+__device__ __forceinline__ bool connect_one(const BdptBufs& bb, uint64_t base, const BdptParams& bp, const SceneView& sv, int s,
                             int t, uint32_t j, float* film, ShadowEntry& e) {
     const uint32_t nEc = (uint32_t)bp.n_eye, nLc = (uint32_t)bp.n_light;
     const Params& prm = bp.base;
@@ -2335,6 +2343,7 @@ __device__ bool connect_one(const BdptBufs& bb, uint64_t base, const BdptParams&
 }
 
 constexpr int kConnBlock = 256;  // 4 waves per tile of 64 samples
+constexpr uint32_t kConnStage = 640;  // staged visibility tests per block (30 KB of LDS)
 
 // The connections of tiles tile0 .. tile0 + gridDim.x - 1, one tile per
 // block.  bins[k] = (n_eye, n_light, first position, sample count) of the
@@ -2342,13 +2351,17 @@ constexpr int kConnBlock = 256;  // 4 waves per tile of 64 samples
 // The strategies of a tile are all (s, t) with t <= n_eye, s <= n_light,
 // s + t >= 2, s + t - 1 <= max_len, except (1, 1) (the camera ray reaching
 // the light covers that path); the block takes them four at a time, one per
-// wave, and queues the group's visibility tests with one atomic.
+// wave.
 __global__ __launch_bounds__(kConnBlock) void k_bdpt_connect(BdptBufs bb, uint64_t base, BdptParams bp,
                                                              SceneView sv, const int4* bins,
                                                              const uint32_t* bin_tile, uint32_t n_bins,
                                                              uint32_t tile0, const uint32_t* perm, float* film,
                                                              ShadowBuf sh, uint32_t* sh_count) {
+    // visibility tests are staged in LDS and written out kConnStage at a
+    // time, so the block takes its queue slots with few atomics
+    __shared__ float4 s_stage[3 * kConnStage];  // ray halves [0, 2 kConnStage), contributions after
     __shared__ uint32_t s_tmp[kConnBlock / 64 + 1];
+    __shared__ uint32_t s_fill, s_base;
     const uint32_t tile = tile0 + blockIdx.x;
     uint32_t lo = 0, hi = n_bins - 1;
     while (lo < hi) {
@@ -2364,6 +2377,46 @@ __global__ __launch_bounds__(kConnBlock) void k_bdpt_connect(BdptBufs bb, uint64
     const bool live = first + lane < (uint32_t)bn.w;
     const uint32_t j = live ? perm[(uint32_t)bn.z + first + lane] : 0u;
     const int e = bn.x, l = bn.y;
+    if (threadIdx.x == 0) s_fill = 0u;
+    __syncthreads();
+    // every thread calls these two at the same points
+    auto drain = [&]() {
+        const uint32_t n = s_fill;
+        if (threadIdx.x == 0) s_base = atomicAdd(sh_count, n);
+        __syncthreads();
+        const size_t b = s_base;
+        for (uint32_t k = threadIdx.x; k < 2 * n; k += kConnBlock) sh.ray[2 * b + k] = s_stage[k];
+        for (uint32_t k = threadIdx.x; k < n; k += kConnBlock) sh.contrib[b + k] = s_stage[2 * kConnStage + k];
+        __syncthreads();
+        if (threadIdx.x == 0) s_fill = 0u;
+        __syncthreads();
+    };
+    auto stage = [&](bool q, const ShadowEntry& se) {
+        const uint64_t mask = __ballot(q);
+        if (lane == 0) s_tmp[wave] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int w = 0; w < kConnBlock / 64; ++w) {
+                const uint32_t c = s_tmp[w];
+                s_tmp[w] = tot;
+                tot += c;
+            }
+            s_tmp[kConnBlock / 64] = tot;
+        }
+        __syncthreads();
+        const uint32_t tot = s_tmp[kConnBlock / 64];
+        if (s_fill + tot > kConnStage) drain();
+        if (q) {
+            const uint32_t k = s_fill + s_tmp[wave] + lane_rank(mask);
+            s_stage[2 * k + 0] = se.o;
+            s_stage[2 * k + 1] = se.d;
+            s_stage[2 * kConnStage + k] = se.c;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_fill += tot;
+        __syncthreads();
+    };
     // the block-uniform enumeration; (ms, mt) = this wave's strategy of the
     // current group of four
     uint32_t k = 0;
@@ -2371,12 +2424,7 @@ __global__ __launch_bounds__(kConnBlock) void k_bdpt_connect(BdptBufs bb, uint64
     auto flush = [&]() {
         ShadowEntry se;
         const bool q = live && ms >= 0 && connect_one(bb, base, bp, sv, ms, mt, j, film, se);
-        const uint32_t slot = block_append<kConnBlock>(sh_count, q, s_tmp);
-        if (q) {
-            sh.ray[2 * (size_t)slot + 0] = se.o;
-            sh.ray[2 * (size_t)slot + 1] = se.d;
-            sh.contrib[slot] = se.c;
-        }
+        stage(q, se);
         ms = -1;
     };
     for (int t = 1; t <= e; ++t)
@@ -2390,6 +2438,7 @@ __global__ __launch_bounds__(kConnBlock) void k_bdpt_connect(BdptBufs bb, uint64
             ++k;
         }
     if (k & 3u) flush();
+    if (s_fill > 0) drain();
 }
 
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
@@ -2607,7 +2656,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
                                        pb, wctl + cur, wctl + 2, sv, (const SceneTab*)nullptr, whits, wuvs,
                                        (unsigned long long*)nullptr);
             });
-            hipLaunchKernelGGL(k_bdpt_walk, dim3(gw), dim3(kBlock), 0, stream, bb, base, bp, sv, wb[cur],
+            hipLaunchKernelGGL(k_bdpt_walk, dim3((gw + 1) / 2), dim3(kWalkBlock), 0, stream, bb, base, bp, sv, wb[cur],
                                (const uint32_t*)(wctl + cur), (const int2*)whits, (const float2*)wuvs,
                                wb[cur ^ 1], wctl + (cur ^ 1));
         }
