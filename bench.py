@@ -724,6 +724,24 @@ def grad_check(rumi, ctx, args, log):
     bias_h = ((fd[:, -2] - fd[:, -1]).mean(0) * 4.0 / 3.0) if len(steps) > 1 else np.zeros(7)
     rel = np.abs(a_m - r_m) / np.abs(r_m)
     rel_se = np.sqrt(a_se ** 2 + r_se ** 2) / np.abs(r_m)
+    # the whole gradient: relative L2 distance between the adjoint's and the
+    # differences' 7-vectors, with a bootstrap over adjoint runs and
+    # difference replicates for its spread
+    vec = float(np.linalg.norm(a_m - r_m) / np.linalg.norm(r_m))
+    rng = np.random.default_rng(7)
+    boot = []
+    for _ in range(2000):
+        am = adj[rng.integers(0, len(adj), len(adj))].mean(0)
+        rm = rich[rng.integers(0, n, n)].mean(0)
+        boot.append(np.linalg.norm(am - rm) / np.linalg.norm(rm))
+    boot = np.array(boot)
+    # what pure noise gives: the same distance with the adjoint replaced by
+    # the differences' own mean (a perfect adjoint), bootstrapped
+    null = []
+    for _ in range(2000):
+        rm = rich[rng.integers(0, n, n)].mean(0)
+        null.append(np.linalg.norm(r_m - rm) / np.linalg.norm(rm))
+    null = np.array(null)
     out = {"metric": "adjoint vs central FD (Richardson over relative steps), media 512^2 32 spp depth 16",
            "params": names, "theta": theta.tolist(), "steps": steps,
            "adjoint": a_m.round(6).tolist(), "adjoint_se": a_se.round(6).tolist(),
@@ -732,6 +750,11 @@ def grad_check(rumi, ctx, args, log):
            "bias_of_largest_step_estimate": bias_h.round(6).tolist(),
            "per_step": per_step,
            "rel_diff": rel.round(5).tolist(), "rel_diff_se": rel_se.round(5).tolist(),
+           "vector_rel_l2": round(vec, 5),
+           "vector_rel_l2_bootstrap_p50_p95": [round(float(np.percentile(boot, 50)), 5),
+                                               round(float(np.percentile(boot, 95)), 5)],
+           "vector_rel_l2_noise_only_p50_p95": [round(float(np.percentile(null, 50)), 5),
+                                                round(float(np.percentile(null, 95)), 5)],
            "adjoint_runs": len(adj), "fd_replicates": n, "spp": spp}
     print(json.dumps(out), flush=True)
 

@@ -416,7 +416,7 @@ int rumi_autotune_trace(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* dev_
 /* Renders desc's frame with candidate queue sizes and writes the fastest
  * into desc (path tracer and RayTracer: queue_capacity 2^19..2^22 with the
  * refill threshold at the capacity or half of it; bidirectional tracer:
- * samples per batch 2^20..2^23); dev_rgb receives the last render.  The
+ * samples per batch 2^20..2^24); dev_rgb receives the last render.  The
  * frame of Scene::Rendering (src/Scene/scene.cpp:206-258). */
 int rumi_autotune_render(rumi_ctx* ctx, const rumi_scene* scene, rumi_render_desc* desc, float* dev_rgb,
                          rumi_autotune_report* report, void* stream);

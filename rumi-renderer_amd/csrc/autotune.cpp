@@ -138,7 +138,7 @@ extern "C" int rumi_autotune_render(rumi_ctx* ctx, const rumi_scene* scene, rumi
     // bidirectional tracer: samples per batch
     std::vector<std::pair<uint32_t, uint32_t>> cands;
     if (desc->integrator == RUMI_INTEGRATOR_BDPT) {
-        for (uint32_t c : {1u << 20, 1u << 21, 1u << 22, 1u << 23}) cands.push_back({c, 0u});
+        for (uint32_t c : {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24}) cands.push_back({c, 0u});
     } else {
         for (uint32_t c : {1u << 19, 1u << 20, 1u << 21, 1u << 22}) {
             cands.push_back({c, c});
