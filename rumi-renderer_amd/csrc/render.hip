@@ -2446,6 +2446,29 @@ __global__ __launch_bounds__(kConnBlock) void k_bdpt_connect(BdptBufs bb, uint64
 
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
+uint32_t grid_for(uint64_t n, uint32_t cap_blocks) {
+    uint64_t g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    return (uint32_t)(g < cap_blocks ? g : cap_blocks);
+}
+
+// Blocks of kernel K (kBlock threads) that stay resident on one compute
+// unit.  The single-scene path tracer's persistent-lane stages get that
+// many per CU and no more: a block beyond them would only start once others
+// end, and its first draw on the work counter (same-address atomics
+// serialise) would come at the tail (Cornell +5%, media +8%).  Batches and
+// the bidirectional tracer keep 16 blocks per CU, which measured faster
+// there (batch graph 536 against 561 ms, BDPT 146 against 140 Msamples/s).
+template <auto K>
+uint32_t resident_per_cu() {
+    static const uint32_t per_cu = [] {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, K, kBlock, 0) != hipSuccess || v < 1) v = 1;
+        return (uint32_t)v;
+    }();
+    return per_cu;
+}
+
 // Clears the walk queue the next iteration appends to and k_extend's counter.
 __global__ void k_walk_ctl(uint32_t* wctl, int next) {
     if (threadIdx.x == 0) wctl[next] = 0u;
@@ -2826,31 +2849,27 @@ struct Pass {
     bool ibl;             // some scene holds an image-based light
     int mode;
     uint32_t grid_cap;    // most blocks a grid-stride stage is given
+    uint32_t cu;          // compute units (persistent-lane grids: resident blocks per CU x cu)
 };
 
-uint32_t grid_for(uint64_t n, uint32_t cap_blocks) {
-    uint64_t g = (n + kBlock - 1) / kBlock;
-    if (g < 1) g = 1;
-    return (uint32_t)(g < cap_blocks ? g : cap_blocks);
-}
 
 template <int MODE, bool MULTI>
 void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t s) {
-    const uint32_t g = grid_for(bound, ps.grid_cap);
     Ctl* ctl = w.ctl;
     hipLaunchKernelGGL(k_clear_ctl, dim3(1), dim3(64), 0, s, ctl, cur ^ 1);
     with_stack(ps.stack, [&](auto v) {
+        constexpr int SV = decltype(v)::value;
         if constexpr (MODE == kRender && !MULTI) {
             if (ps.counts) {
-                hipLaunchKernelGGL((k_extend<decltype(v)::value, false, true>), dim3(g), dim3(kBlock), 0, s,
-                                   w.pb[cur], &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs,
-                                   ps.counts);
+                const uint32_t g = grid_for(bound, resident_per_cu<k_extend<SV, false, true>>() * ps.cu);
+                hipLaunchKernelGGL((k_extend<SV, false, true>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
+                                   &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs, ps.counts);
                 return;
             }
         }
-        hipLaunchKernelGGL((k_extend<decltype(v)::value, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
-                           &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs,
-                           (unsigned long long*)nullptr);
+        const uint32_t g = grid_for(bound, MULTI ? ps.grid_cap : resident_per_cu<k_extend<SV, MULTI>>() * ps.cu);
+        hipLaunchKernelGGL((k_extend<SV, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur], &ctl->queue[cur],
+                           &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs, (unsigned long long*)nullptr);
     });
     // one tile per block; blocks past the queue's size return at once
     const uint32_t gs = (uint32_t)((bound + kShadeBlock - 1) / kShadeBlock);
@@ -2863,17 +2882,18 @@ void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t 
                            w.pb[cur], w.pb[cur ^ 1], &ctl->queue[cur], w.hits, w.uvs, ps.sv, ps.tab, ps.prm,
                            ps.film, w.sh, &ctl->queue[cur ^ 1], &ctl->shadow, ps.adj, ps.tr);
     with_stack(ps.stack, [&](auto v) {
+        constexpr int SV = decltype(v)::value;
         if constexpr (MODE == kRender && !MULTI) {
             if (ps.counts) {
-                hipLaunchKernelGGL((k_shadow<decltype(v)::value, kRender, false, true>), dim3(g), dim3(kBlock), 0,
-                                   s, w.sh, &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj,
-                                   ps.counts);
+                const uint32_t g = grid_for(bound, resident_per_cu<k_shadow<SV, kRender, false, true>>() * ps.cu);
+                hipLaunchKernelGGL((k_shadow<SV, kRender, false, true>), dim3(g), dim3(kBlock), 0, s, w.sh,
+                                   &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj, ps.counts);
                 return;
             }
         }
-        hipLaunchKernelGGL((k_shadow<decltype(v)::value, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh,
-                           &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj,
-                           (unsigned long long*)nullptr);
+        const uint32_t g = grid_for(bound, MULTI ? ps.grid_cap : resident_per_cu<k_shadow<SV, MODE, MULTI>>() * ps.cu);
+        hipLaunchKernelGGL((k_shadow<SV, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh, &ctl->shadow,
+                           &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj, (unsigned long long*)nullptr);
     });
 }
 
@@ -2958,6 +2978,7 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     ps.ibl = sc->has_ibl;
     ps.mode = mode;
     ps.grid_cap = (uint32_t)ctx->cu_count * 16u;
+    ps.cu = (uint32_t)ctx->cu_count;
 
     const bool events = fixed == nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -3143,6 +3164,7 @@ void batch_enqueue(rumi_batch* b, const rumi_batch_io& io, hipStream_t s) {
     ps.stack = b->stack;
     ps.ibl = b->ibl;
     ps.grid_cap = (uint32_t)b->cu_count * 16u;
+    ps.cu = (uint32_t)b->cu_count;
     ps.adj = AdjCtx{};
     ps.adj.gstripes = b->stripes;
     ps.adj.inv_spp = 1.0f / (float)b->desc.spp;
