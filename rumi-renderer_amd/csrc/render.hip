@@ -48,12 +48,20 @@ constexpr int sv_lds(int) { return 16; }
 constexpr int sv_priv(int v) { return v == 1 ? 16 : (v == 2 ? 32 : (v == 3 ? kMaxDepth - 16 : 0)); }
 inline int stack_variant(int need) { return need <= 16 ? 0 : need <= 32 ? 1 : need <= 48 ? 2 : 3; }
 
+// Variant 4: variant 0 with the whole scene tree (nodes, primitive and rect
+// records, at most kTreeF4 float4s) copied into LDS by every block of the
+// single-scene extension and shadow stages, so their node and primitive
+// fetches are LDS reads (the Cornell box and media scenes: ~1.5 KB).
+constexpr uint32_t kTreeF4 = 256;  // 4 KB
+constexpr bool sv_tree(int v) { return v == 4; }
+
 template <class F>
 void with_stack(int v, F&& f) {
     switch (v) {
     case 0: f(std::integral_constant<int, 0>{}); break;
     case 1: f(std::integral_constant<int, 1>{}); break;
     case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
     default: f(std::integral_constant<int, 3>{}); break;
     }
 }
@@ -158,6 +166,7 @@ struct SceneView {
     const float* texels;
     RootInfo root;
     uint32_t n_lights;
+    uint32_t tree_f4[3];  // float4s of nodes, primitive records, rect records
     DevMedium med;
     DevCamera cam;
 };
@@ -383,6 +392,20 @@ struct ExtendItem {
     uint32_t cn = 0, cp = 0;  // COUNT: interior nodes entered and primitives tested
 };
 
+// Copies a single scene's tree into LDS (stack variant 4) and points
+// nodes/prims/rects there; every thread of the block calls it.
+__device__ inline void tree_to_lds(const SceneView& sv, float4* s_tree, const float4*& nodes,
+                                   const float4*& prims, const float4*& rects) {
+    const uint32_t nf = sv.tree_f4[0], pf = sv.tree_f4[1], rf = sv.tree_f4[2];
+    for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x) s_tree[k] = sv.nodes[k];
+    for (uint32_t k = threadIdx.x; k < pf; k += blockDim.x) s_tree[nf + k] = sv.prims[k];
+    for (uint32_t k = threadIdx.x; k < rf; k += blockDim.x) s_tree[nf + pf + k] = sv.rects[k];
+    __syncthreads();
+    nodes = s_tree;
+    prims = s_tree + nf;
+    rects = s_tree + nf + pf;
+}
+
 // COUNT: also adds every lane's node visits and primitive tests to counts[0..1].
 template <int SV, bool MULTI, bool COUNT = false>
 __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* n_ptr,
@@ -397,6 +420,10 @@ __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* 
     it.nodes = sv.nodes;
     it.prims = sv.prims;
     it.rects = sv.rects;
+    if constexpr (sv_tree(SV) && !MULTI) {
+        __shared__ float4 s_tree[kTreeF4];
+        tree_to_lds(sv, s_tree, it.nodes, it.prims, it.rects);
+    }
     it.root = sv.root;
     it.hits = hits;
     it.uvs = uvs;
@@ -1500,6 +1527,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
     it.nodes = sv.nodes;
     it.prims = sv.prims;
     it.rects = sv.rects;
+    if constexpr (sv_tree(SV) && !MULTI) {
+        __shared__ float4 s_tree[kTreeF4];
+        tree_to_lds(sv, s_tree, it.nodes, it.prims, it.rects);
+    }
     it.texels = sv.texels;
     it.root = sv.root;
     it.film = film;
@@ -2446,6 +2477,14 @@ __global__ __launch_bounds__(kConnBlock) void k_bdpt_connect(BdptBufs bb, uint64
 
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
+// A single scene's stack variant: 4 (tree in LDS) when the tree is shallow
+// enough for variant 0 and fits kTreeF4.
+inline int scene_stack_variant(const rumi_scene* sc) {
+    const int v = stack_variant(sc->stack_need);
+    const uint64_t f4 = (uint64_t)sc->tree_f4[0] + sc->tree_f4[1] + sc->tree_f4[2];
+    return v == 0 && f4 <= kTreeF4 ? 4 : v;
+}
+
 uint32_t grid_for(uint64_t n, uint32_t cap_blocks) {
     uint64_t g = (n + kBlock - 1) / kBlock;
     if (g < 1) g = 1;
@@ -2490,6 +2529,7 @@ SceneView scene_view(const rumi_scene* sc) {
     }
     sv.root.ref = sc->root_ref;
     sv.n_lights = sc->n_lights;
+    for (int k = 0; k < 3; ++k) sv.tree_f4[k] = sc->tree_f4[k];
     sv.med = sc->medium;
     sv.cam = sc->camera;
     return sv;
@@ -2642,7 +2682,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     bp.beta = desc->mis_beta;
     bp.film_area = (2.0f * sc->camera.tanfov) * (2.0f * sc->camera.tanfov * sc->camera.y_ratio);
     bp.light_seed = desc->seed ^ 0x2545F4914F6CDD1Dull;
-    const int stack = stack_variant(sc->stack_need);
+    const int stack = scene_stack_variant(sc);
 
     hipEvent_t ev0, ev1;
     (void)hipEventCreate(&ev0);
@@ -2972,7 +3012,7 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     ps.film = film;
     ps.adj = adj;
     ps.tr = Track{nullptr, nullptr};
-    ps.stack = stack_variant(sc->stack_need);
+    ps.stack = scene_stack_variant(sc);
     const bool counting = (desc->flags & RUMI_RENDER_COUNT_TRAVERSAL) != 0 && mode == kRender;
     ps.counts = counting ? w.counts : nullptr;
     ps.ibl = sc->has_ibl;

@@ -475,6 +475,9 @@ extern "C" int rumi_scene_create(rumi_ctx* ctx, const rumi_scene_desc* d, rumi_s
         if ((rc = upload(&sc->textures, texs))) break;
         if ((rc = upload(&sc->texels, texels))) break;
         sc->texel_count = (uint32_t)(texels.size() / 3);
+        sc->tree_f4[0] = (uint32_t)(sizeof(DevNode4) / 16 * wide.nodes.size());
+        sc->tree_f4[1] = (uint32_t)(sizeof(DevTri) / 16 * leaf.size());
+        sc->tree_f4[2] = (uint32_t)(rects.size() * sizeof(rects[0]) / 16);
     } while (0);
     if (rc != RUMI_OK) {
         release(sc);

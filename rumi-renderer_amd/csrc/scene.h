@@ -80,6 +80,7 @@ struct rumi_scene {
     int32_t max_depth = 0;   // binary tree depth
     int32_t stack_need = 0;  // traversal stack entries of the 4-wide form (3 per level)
     int32_t root_ref = 0;    // root of the 4-wide form
+    uint32_t tree_f4[3] = {0, 0, 0};  // float4s of the device nodes, primitive records, rect records
     float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
     rumi::DevMedium medium{};
     rumi::DevCamera camera{};
