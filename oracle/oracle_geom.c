@@ -634,11 +634,16 @@ static int travel_near4(const or_bvh* b, const or_prim* prims, or_ray* ray, int 
     for (;;) {
         int at_node = cur >= 0 && b->left[cur] >= 0;
         if (leaf >= 0) {
-            int p = b->order[lpos];
-            if (st) st->prims++;
-            if (or_prim_intersect(&prims[p], ray, in_object))
-                hit = p;
-            if (++lpos == lend) {
+            /* the leaf's next two primitives while two remain, one after
+             * the other (the GPU's mesh kernels test two per step) */
+            int k, use = lend - lpos > 1 ? 2 : 1;
+            for (k = 0; k < use; ++k) {
+                int p = b->order[lpos++];
+                if (st) st->prims++;
+                if (or_prim_intersect(&prims[p], ray, in_object))
+                    hit = p;
+            }
+            if (lpos == lend) {
                 leaf = -1;
                 if (any && ray->tmin > 0.0f)
                     return hit;

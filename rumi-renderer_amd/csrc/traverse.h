@@ -402,7 +402,8 @@ __device__ __forceinline__ float key_distance(uint32_t key) {
 // Near-first order over 4-wide nodes (DevNode4, 8 float4 per node): the
 // accepted children are ordered by order_key, the first becomes s.cur and the
 // others are pushed last first, each with its key_distance.  Half the node
-// steps of the binary tree, four boxes per fetch.
+// steps of the binary tree, four boxes per fetch; for meshes, two triangles
+// of the postponed leaf per step.
 template <int BLOCK, int STACK, bool ANY, bool MIXED, bool COUNT, bool SPILL>
 __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes,
                                                 const float4* __restrict__ prims,
@@ -424,11 +425,22 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
         rf = np4[6];
     }
     const int32_t enc = ~s.leaf;
+    // meshes: a leaf's next two triangles in one step while two remain (the
+    // wave then waits on one fetch round trip for both, and tmin drops
+    // sooner for the node tests of the same step); mixed scenes keep one
+    // primitive per step
+    const bool two = !MIXED && at_leaf && (enc & 15) > 1;
+    float4 ta2, tb2, tc2;
     if (at_leaf) {
         const float4* t4 = prims + 3 * (size_t)(enc >> 4);
         ta = t4[0];
         tb = t4[1];
         tc = t4[2];
+        if (two) {
+            ta2 = t4[3];
+            tb2 = t4[4];
+            tc2 = t4[5];
+        }
     }
     if (at_leaf) {
         if (COUNT) ++np;
@@ -450,9 +462,19 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
             s.hu = u;
             s.hv = v;
         }
+        if (two) {
+            if (COUNT) ++np;
+            float u2 = 0.0f, v2 = 0.0f;
+            if (hit_triangle(ta2, tb2, tc2, r, u2, v2)) {
+                s.hit = __float_as_int(tc2.y);
+                s.hu = u2;
+                s.hv = v2;
+            }
+        }
+        const int32_t used = two ? 2 : 1;
         const int32_t count = enc & 15;
-        if (count > 1) {
-            s.leaf = ~((((enc >> 4) + 1) << 4) | (count - 1));
+        if (count > used) {
+            s.leaf = ~((((enc >> 4) + used) << 4) | (count - used));
         } else {
             s.leaf = 0;
             if (ANY && r.tmin > 0.0f) return false;
