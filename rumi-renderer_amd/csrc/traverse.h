@@ -425,11 +425,11 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
         rf = np4[6];
     }
     const int32_t enc = ~s.leaf;
-    // meshes: a leaf's next two triangles in one step while two remain (the
-    // wave then waits on one fetch round trip for both, and tmin drops
-    // sooner for the node tests of the same step); mixed scenes keep one
-    // primitive per step
-    const bool two = !MIXED && at_leaf && (enc & 15) > 1;
+    // a leaf's next two triangles in one step while two remain (the wave
+    // then waits on one fetch round trip for both, and tmin drops sooner for
+    // the node tests of the same step); in mixed scenes only when both
+    // records are triangles, other primitives one per step
+    bool two = at_leaf && (enc & 15) > 1;
     float4 ta2, tb2, tc2;
     if (at_leaf) {
         const float4* t4 = prims + 3 * (size_t)(enc >> 4);
@@ -442,6 +442,8 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
             tc2 = t4[5];
         }
     }
+    if (MIXED && two)
+        two = __float_as_int(tc.z) == kTagTri && __float_as_int(tc2.z) == kTagTri;
     if (at_leaf) {
         if (COUNT) ++np;
         float u = 0.0f, v = 0.0f;
