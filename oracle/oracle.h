@@ -117,9 +117,9 @@ typedef struct {
  *              distance; ids differ only between primitives at exactly
  *              equal distance.
  *            = 2: near-first over the tree's 4-wide form, children
- *              ordered by order_key, the stack keeping key_distance, a
- *              leaf's next two primitives per step while two remain (the
- *              GPU's width-4 mesh traversal).
+ *              ordered by order_key, the stack keeping key_distance, up
+ *              to three of a leaf's primitives per step (the GPU's
+ *              width-4 mesh traversal).
  * Returns the primitive index (scene order) or -1; ray->tmin holds t. */
 int or_travel(const or_bvh* bvh, const or_prim* prims, or_ray* ray, int any,
               int ordered, int in_object, or_stats* st);
