@@ -40,6 +40,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--freq", type=int, default=256)
     ap.add_argument("--rays", type=int, default=1 << 23)
+    ap.add_argument("--bins", type=int, default=32, help="binned-SAH bins of the mesh BVH")
     ap.add_argument("--cpu-rays", type=int, default=1 << 20,
                     help="rays in the bounded CPU-baseline sample (0 = skip)")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing legs")
@@ -106,7 +107,7 @@ def main():
     xyz, idx = rumi.make_icosphere(args.freq, 1, 0.05)
     rays_h = rumi.make_rays_uniform(args.rays, 2, 2.1)
     ctx = rumi.Context(0)
-    bvh = rumi.Bvh(ctx, xyz, idx, mode=rumi.RUMI_BVH_SAH_BINNED, max_leaf=4, n_bins=32)
+    bvh = rumi.Bvh(ctx, xyz, idx, mode=rumi.RUMI_BVH_SAH_BINNED, max_leaf=4, n_bins=args.bins)
     info = bvh.info()
     n = rays_h.shape[0]
     rays = torch.from_numpy(rays_h).cuda()

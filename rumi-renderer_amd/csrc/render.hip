@@ -1563,8 +1563,9 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
 // pdfs.  Stages: k_bdpt_init and rounds of k_extend + k_bdpt_walk build both
 // subpaths of every sample of a batch as one wavefront; k_bdpt_hist and
 // k_bdpt_scatter order the samples by their subpath lengths; k_bdpt_connect
-// (one thread per sample and strategy that sample can make) evaluates the
-// connections and queues their visibility tests for k_shadow.
+// (one block per 64 samples of equal subpath lengths, one wave per strategy
+// those lengths allow) evaluates the connections and queues their
+// visibility tests for k_shadow.
 //
 // A glossy vertex keeps the one lobe Glossy::BSDFSampling picks
 // (src/Material/glossy.cpp:56-78).  Lambertian lobe (probability kd_pdf): a
