@@ -432,7 +432,20 @@ __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* 
     it.tid = threadIdx.x;
     it.id = it.nid = it.nflags = it.nscene = 0;
     it.in_obj = false;
-    run_lanes(*n_ptr, counter, it);
+    if constexpr (sv_tree(SV) && !MULTI) {
+        // a tree in LDS: rays are short and cost about the same, so one ray
+        // per lane at a time does without the persistent lanes' bookkeeping
+        const uint32_t n = *n_ptr;
+        for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+            it.prefetch(k);
+            TravState ts;
+            bool act = it.start(ts);
+            while (act) act = it.step(ts);
+            it.finish(ts);
+        }
+    } else {
+        run_lanes(*n_ptr, counter, it);
+    }
     if (COUNT) {  // counts: [0] nodes, [1] primitives, [4] rays
         add_counts(counts, it.cn, it.cp);
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(counts + 4, (unsigned long long)*n_ptr);
@@ -1541,7 +1554,20 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
     it.p_stk = p_stk;
     it.tid = threadIdx.x;
     it.id = it.nid = it.nscene = it.scene = 0;
-    run_lanes(*n_ptr, counter, it);
+    if constexpr (sv_tree(SV) && !MULTI) {
+        // a tree in LDS: rays are short and cost about the same, so one ray
+        // per lane at a time does without the persistent lanes' bookkeeping
+        const uint32_t n = *n_ptr;
+        for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+            it.prefetch(k);
+            TravState ts;
+            bool act = it.start(ts);
+            while (act) act = it.step(ts);
+            it.finish(ts);
+        }
+    } else {
+        run_lanes(*n_ptr, counter, it);
+    }
     if (MODE == kReplay) flush_grad(it.G, adj.gstripes);
     if (COUNT) {  // counts: [2] nodes, [3] primitives, [5] rays
         add_counts(counts + 2, it.cn, it.cp);
