@@ -410,7 +410,7 @@ __device__ inline void tree_to_lds(const SceneView& sv, float4* s_tree, const fl
 template <int SV, bool MULTI, bool COUNT = false>
 __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* n_ptr,
                                                    uint32_t* counter, SceneView sv,
-                                                   const SceneTab* tab, int2* hits, float2* uvs,
+                                                   const SceneTab* tab, int2* hits, float2* uvs, bool lanes,
                                                    unsigned long long* counts = nullptr) {
     __shared__ int2 s_stk[sv_lds(SV) * kBlock];
     int2 p_stk[sv_priv(SV) > 0 ? sv_priv(SV) : 1];
@@ -432,9 +432,10 @@ __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* 
     it.tid = threadIdx.x;
     it.id = it.nid = it.nflags = it.nscene = 0;
     it.in_obj = false;
-    if constexpr (sv_tree(SV) && !MULTI) {
-        // a tree in LDS: rays are short and cost about the same, so one ray
-        // per lane at a time does without the persistent lanes' bookkeeping
+    if (!MULTI && (sv_tree(SV) || !lanes)) {
+        // one ray per lane at a time, without the persistent lanes'
+        // bookkeeping: a tree in LDS always (its rays are short and cost
+        // about the same), other trees when the caller asks (!lanes)
         const uint32_t n = *n_ptr;
         for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
             it.prefetch(k);
@@ -1530,7 +1531,7 @@ struct ShadowItem {
 template <int SV, int MODE, bool MULTI, bool COUNT = false>
 __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t* n_ptr,
                                                    uint32_t* counter, SceneView sv,
-                                                   const SceneTab* tab, float* film, AdjCtx adj,
+                                                   const SceneTab* tab, float* film, AdjCtx adj, bool lanes,
                                                    unsigned long long* counts = nullptr) {
     __shared__ int2 s_stk[sv_lds(SV) * kBlock];
     int2 p_stk[sv_priv(SV) > 0 ? sv_priv(SV) : 1];
@@ -1554,9 +1555,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
     it.p_stk = p_stk;
     it.tid = threadIdx.x;
     it.id = it.nid = it.nscene = it.scene = 0;
-    if constexpr (sv_tree(SV) && !MULTI) {
-        // a tree in LDS: rays are short and cost about the same, so one ray
-        // per lane at a time does without the persistent lanes' bookkeeping
+    if (!MULTI && (sv_tree(SV) || !lanes)) {
+        // one ray per lane at a time, without the persistent lanes'
+        // bookkeeping: a tree in LDS always (its rays are short and cost
+        // about the same), other trees when the caller asks (!lanes)
         const uint32_t n = *n_ptr;
         for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
             it.prefetch(k);
@@ -2743,11 +2745,11 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
                 if (counting)
                     hipLaunchKernelGGL((k_extend<decltype(v)::value, false, true>), dim3(gw), dim3(kBlock), 0,
                                        stream, pb, wctl + cur, wctl + 2, sv, (const SceneTab*)nullptr, whits, wuvs,
-                                       counts);
+                                       true, counts);
                 else
                     hipLaunchKernelGGL((k_extend<decltype(v)::value, false>), dim3(gw), dim3(kBlock), 0, stream,
                                        pb, wctl + cur, wctl + 2, sv, (const SceneTab*)nullptr, whits, wuvs,
-                                       (unsigned long long*)nullptr);
+                                       true, (unsigned long long*)nullptr);
             });
             hipLaunchKernelGGL(k_bdpt_walk, dim3((gw + 1) / 2), dim3(kWalkBlock), 0, stream, bb, base, bp, sv, wb[cur],
                                (const uint32_t*)(wctl + cur), (const int2*)whits, (const float2*)wuvs,
@@ -2789,11 +2791,11 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
                 if (counting)
                     hipLaunchKernelGGL((k_shadow<decltype(v)::value, kRender, false, true>), dim3(g), dim3(kBlock),
                                        0, stream, sh, qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{},
-                                       counts);
+                                       true, counts);
                 else
                     hipLaunchKernelGGL((k_shadow<decltype(v)::value, kRender, false>), dim3(g), dim3(kBlock), 0,
                                        stream, sh, qctl, qctl + 1, sv, (const SceneTab*)nullptr, film, AdjCtx{},
-                                       (unsigned long long*)nullptr);
+                                       true, (unsigned long long*)nullptr);
             });
         }
         ++batches;
@@ -2921,6 +2923,10 @@ struct Pass {
 
 
 template <int MODE, bool MULTI>
+// The single-scene path tracer traces one ray per lane (lanes = false): its
+// queues are compacted every iteration, so lanes rarely idle and the
+// persistent lanes' bookkeeping costs more than it saves (the bidirectional
+// tracer's walks and connections, of very uneven length, keep them).
 void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t s) {
     Ctl* ctl = w.ctl;
     hipLaunchKernelGGL(k_clear_ctl, dim3(1), dim3(64), 0, s, ctl, cur ^ 1);
@@ -2930,13 +2936,13 @@ void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t 
             if (ps.counts) {
                 const uint32_t g = grid_for(bound, resident_per_cu<k_extend<SV, false, true>>() * ps.cu);
                 hipLaunchKernelGGL((k_extend<SV, false, true>), dim3(g), dim3(kBlock), 0, s, w.pb[cur],
-                                   &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs, ps.counts);
+                                   &ctl->queue[cur], &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs, false, ps.counts);
                 return;
             }
         }
         const uint32_t g = grid_for(bound, MULTI ? ps.grid_cap : resident_per_cu<k_extend<SV, MULTI>>() * ps.cu);
         hipLaunchKernelGGL((k_extend<SV, MULTI>), dim3(g), dim3(kBlock), 0, s, w.pb[cur], &ctl->queue[cur],
-                           &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs, (unsigned long long*)nullptr);
+                           &ctl->fetch[0], ps.sv, ps.tab, w.hits, w.uvs, false, (unsigned long long*)nullptr);
     });
     // one tile per block; blocks past the queue's size return at once
     const uint32_t gs = (uint32_t)((bound + kShadeBlock - 1) / kShadeBlock);
@@ -2954,13 +2960,13 @@ void stages(const Wave& w, int cur, uint32_t bound, const Pass& ps, hipStream_t 
             if (ps.counts) {
                 const uint32_t g = grid_for(bound, resident_per_cu<k_shadow<SV, kRender, false, true>>() * ps.cu);
                 hipLaunchKernelGGL((k_shadow<SV, kRender, false, true>), dim3(g), dim3(kBlock), 0, s, w.sh,
-                                   &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj, ps.counts);
+                                   &ctl->shadow, &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj, false, ps.counts);
                 return;
             }
         }
         const uint32_t g = grid_for(bound, MULTI ? ps.grid_cap : resident_per_cu<k_shadow<SV, MODE, MULTI>>() * ps.cu);
         hipLaunchKernelGGL((k_shadow<SV, MODE, MULTI>), dim3(g), dim3(kBlock), 0, s, w.sh, &ctl->shadow,
-                           &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj, (unsigned long long*)nullptr);
+                           &ctl->fetch[1], ps.sv, ps.tab, ps.film, ps.adj, false, (unsigned long long*)nullptr);
     });
 }
 
