@@ -118,7 +118,7 @@ typedef struct {
  *              equal distance.
  *            = 2: near-first over the tree's 4-wide form, children
  *              ordered by order_key, the stack keeping key_distance, up
- *              to three of a leaf's primitives per step (the GPU's
+ *              to four of a leaf's primitives per step (the GPU's
  *              width-4 mesh traversal).
  * Returns the primitive index (scene order) or -1; ray->tmin holds t. */
 int or_travel(const or_bvh* bvh, const or_prim* prims, or_ray* ray, int any,

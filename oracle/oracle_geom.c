@@ -634,10 +634,10 @@ static int travel_near4(const or_bvh* b, const or_prim* prims, or_ray* ray, int 
     for (;;) {
         int at_node = cur >= 0 && b->left[cur] >= 0;
         if (leaf >= 0) {
-            /* the leaf's next three primitives (as many as remain), one
-             * after the other: the GPU's mesh kernels test up to three per
+            /* the leaf's next four primitives (as many as remain), one
+             * after the other: the GPU's mesh kernels test up to four per
              * step */
-            int k, use = lend - lpos < 3 ? lend - lpos : 3;
+            int k, use = lend - lpos < 4 ? lend - lpos : 4;
             for (k = 0; k < use; ++k) {
                 int p = b->order[lpos++];
                 if (st) st->prims++;

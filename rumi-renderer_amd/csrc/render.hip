@@ -347,6 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_regen(PathBuf buf, uint32_t start, u
 // in_object flag, bvh.cpp:150-213), persistent lanes (run_lanes).
 template <int SV, bool MULTI, bool COUNT = false>
 struct ExtendItem {
+    static constexpr bool kWaveStep = false;
     PathBuf buf;
     const SceneTab* tab;
     const float4 *nodes, *prims, *rects;
@@ -1457,6 +1458,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(PathBuf in, PathBuf out, 
 // passes: to the per-sample sums and the connection's own gradient terms).
 template <int SV, int MODE, bool MULTI, bool COUNT = false>
 struct ShadowItem {
+    static constexpr bool kWaveStep = false;
     ShadowBuf sh;
     const SceneTab* tab;
     const float4 *nodes, *prims, *rects;

@@ -139,6 +139,33 @@ __device__ __forceinline__ bool hit_triangle(const float4& ta, const float4& tb,
     return ok;
 }
 
+// hit_triangle's tests except the one against the current hit (t > tmin),
+// which depends on the order of the tests: the triangle's distance and
+// barycentrics are returned and r is not changed.  The same arithmetic, so
+// the same bits.
+__device__ __forceinline__ bool tri_base(const float4& ta, const float4& tb, const float4& tc, const Ray& r,
+                                         float& t_out, float& u_out, float& v_out) {
+    const float e1x = ta.w, e1y = tb.x, e1z = tb.y;
+    const float e2x = tb.z, e2y = tb.w, e2z = tc.x;
+    const float px = r.dy * e2z - e2y * r.dz;
+    const float py = r.dz * e2x - e2z * r.dx;
+    const float pz = r.dx * e2y - e2x * r.dy;
+    const float det = px * e1x + py * e1y + pz * e1z;
+    const float inv_det = 1.0f / det;
+    const float tx = r.ox - ta.x, ty = r.oy - ta.y, tz = r.oz - ta.z;
+    const float u = (px * tx + py * ty + pz * tz) * inv_det;
+    const float qx = ty * e1z - e1y * tz;
+    const float qy = tz * e1x - e1z * tx;
+    const float qz = tx * e1y - e1x * ty;
+    const float v = (qx * r.dx + qy * r.dy + qz * r.dz) * inv_det;
+    const float t = (qx * e2x + qy * e2y + qz * e2z) * inv_det;
+    t_out = t;
+    u_out = u;
+    v_out = v;
+    return !(-1e-8f < det && det < 1e-8f) & !(u < 0.0f || u > 1.0f) & !(v < 0.0f || u + v > 1.0f) &
+           !(t < kEps) & !(r.tmax - t < kEps);
+}
+
 // Sphere::Intersect: a = centre.xyz, radius.  A = 1 (unit direction assumed),
 // far root first inside an object, and swapped (again) for an emissive
 // textured sphere seen from inside (ibl: the image-based light's sphere).
@@ -394,6 +421,12 @@ __device__ __forceinline__ uint32_t order_key(float tn, int slot) {
     return (u & ~3u) | (uint32_t)slot;
 }
 
+// Set bits of mask in the lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 __device__ __forceinline__ float key_distance(uint32_t key) {
     const uint32_t u = key & ~3u;
     return __int_as_float((int32_t)((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u));
@@ -402,17 +435,27 @@ __device__ __forceinline__ float key_distance(uint32_t key) {
 // Near-first order over 4-wide nodes (DevNode4, 8 float4 per node): the
 // accepted children are ordered by order_key, the first becomes s.cur and the
 // others are pushed last first, each with its key_distance.  Half the node
-// steps of the binary tree, four boxes per fetch; up to three triangles of
+// steps of the binary tree, four boxes per fetch; up to four triangles of
 // the postponed leaf per step (meshes; two in mixed scenes).
-template <int BLOCK, int STACK, bool ANY, bool MIXED, bool COUNT, bool SPILL>
+//
+// RD (meshes only): every lane of the wave calls the step, `live` telling
+// whether it holds a ray, and the step's triangle tests -- up to four per
+// lane, tested in turn on that lane otherwise -- are spread over the wave's
+// lanes, 64 per pass (slots: K * 64 entries (triangle, owner lane) of LDS
+// per wave); each lane then takes its tests' outcomes back in their order
+// and applies the test against its current hit, so hits, distances and
+// counts are those of the per-lane steps.
+template <int BLOCK, int STACK, bool ANY, bool MIXED, bool COUNT, bool SPILL, bool RD = false>
 __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes,
                                                 const float4* __restrict__ prims,
                                                 const float4* __restrict__ rects, TravState& s,
                                                 bool in_object, int32_t& nn, int32_t& np,
-                                                int2* s_stk, uint32_t tid, int2* p_stk) {
+                                                int2* s_stk, uint32_t tid, int2* p_stk, bool live = true,
+                                                uint8_t* slots = nullptr) {
+    static_assert(!(RD && MIXED), "spread triangle tests are for meshes");
     Ray& r = s.r;
-    const bool at_node = s.cur >= 0;
-    const bool at_leaf = s.leaf != 0;
+    const bool at_node = (!RD || live) && s.cur >= 0;
+    const bool at_leaf = (!RD || live) && s.leaf != 0;
     float4 lx, ly, lz, hx, hy, hz, rf, ta, tb, tc;
     if (at_node) {
         const float4* np4 = nodes + 8 * (size_t)s.cur;
@@ -427,15 +470,91 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
     const int32_t enc = ~s.leaf;
     // A leaf's next K primitive records in one step (the wave waits on one
     // fetch round trip for all of them, and tmin drops sooner for the node
-    // tests of the same step): meshes up to three triangles (2.22 -> 1.94
-    // ms closest-hit on the 1.3M-triangle mesh; four gave the same closest
-    // and a slower any-hit), mixed scenes two when both are triangles,
-    // other primitives one per step.
-    constexpr int K = MIXED ? 2 : 3;
+    // tests of the same step): meshes up to four triangles (spread over the
+    // wave: 1.92 -> 1.87 ms closest-hit on the 1.3M-triangle mesh, against
+    // three), mixed scenes two when both are triangles, other primitives
+    // one per step.
+    constexpr int K = MIXED ? 2 : 4;
     const int32_t rem = at_leaf ? (enc & 15) : 0;
     int use = rem < K ? rem : K;
+    if constexpr (RD) {
+        const uint32_t lane = tid & 63u;
+        const uint64_t b0 = __ballot((use & 1) != 0), b1 = __ballot((use & 2) != 0),
+                       b2 = K > 3 ? __ballot((use & 4) != 0) : 0ull;
+        const uint32_t total =
+            (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
+        if (total != 0) {
+            // test k of lane L is the wave's test pre(L) + k
+            const uint32_t pre = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2);
+            int2* sl = reinterpret_cast<int2*>(slots);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (k < use) sl[pre + k] = make_int2((enc >> 4) + k, (int32_t)lane);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t r0 = 0; r0 < total; r0 += 64u) {
+                const uint32_t p = r0 + lane;
+                const bool tester = p < total;
+                const int2 e = tester ? sl[p] : make_int2(0, (int32_t)lane);
+                // the owner's ray (every lane takes part)
+                const int oa = e.y << 2;
+                Ray q;
+                q.ox = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.ox)));
+                q.oy = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.oy)));
+                q.oz = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.oz)));
+                q.dx = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.dx)));
+                q.dy = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.dy)));
+                q.dz = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.dz)));
+                q.tmax = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.tmax)));
+                int32_t rid = -1;
+                float rt = 0.0f, ru = 0.0f, rv = 0.0f;
+                if (tester) {
+                    const float4* t4 = prims + 3 * (size_t)e.x;
+                    const float4 a = t4[0], b = t4[1], c = t4[2];
+                    float t, u, v;
+                    if (tri_base(a, b, c, q, t, u, v)) {
+                        rid = __float_as_int(c.y);
+                        rt = t;
+                        ru = u;
+                        rv = v;
+                    }
+                }
+                // each lane takes back its tests of this pass, in order
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint32_t pos = pre + (uint32_t)k;
+                    const bool mine = k < use && pos >= r0 && pos < r0 + 64u;
+                    const int sa = (int)((mine ? pos - r0 : lane) << 2);
+                    const int32_t id = __builtin_amdgcn_ds_bpermute(sa, rid);
+                    const float t = __int_as_float(__builtin_amdgcn_ds_bpermute(sa, __float_as_int(rt)));
+                    const float u = __int_as_float(__builtin_amdgcn_ds_bpermute(sa, __float_as_int(ru)));
+                    const float v = __int_as_float(__builtin_amdgcn_ds_bpermute(sa, __float_as_int(rv)));
+                    if (mine && id >= 0 && !(r.tmin > kEps && t > r.tmin)) {
+                        r.tmin = t;
+                        s.hit = id;
+                        s.hu = u;
+                        s.hv = v;
+                    }
+                }
+            }
+            // the next step's slot writes stay after this step's reads
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (at_leaf) {
+            if (COUNT) np += use;
+            const int32_t count = enc & 15;
+            if (count > use) {
+                s.leaf = ~((((enc >> 4) + use) << 4) | (count - use));
+            } else {
+                s.leaf = 0;
+                if (ANY && r.tmin > 0.0f) return false;
+            }
+        }
+    }
     float4 tt[K][3];
-    if (at_leaf) {
+    if (!RD && at_leaf) {
         const float4* t4 = prims + 3 * (size_t)(enc >> 4);
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -447,7 +566,7 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
     }
     if (MIXED && use > 1 && !(__float_as_int(tt[0][2].z) == kTagTri && __float_as_int(tt[1][2].z) == kTagTri))
         use = 1;
-    if (at_leaf) {
+    if (!RD && at_leaf) {
         ta = tt[0][0];
         tb = tt[0][1];
         tc = tt[0][2];
@@ -549,6 +668,7 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
                     stack_push<BLOCK, STACK, SPILL>(s, ref[c], key_distance(key[c]), s_stk, tid, p_stk);
         }
     }
+    if (RD && !live) return false;
     next_node<BLOCK, STACK, SPILL>(s, s_stk, tid, p_stk);
     return s.cur != kNoRef || s.leaf != 0;
 }
@@ -717,12 +837,6 @@ constexpr uint32_t kRunChunkBig = 1024;
 #endif
 constexpr uint32_t kStartBatch = RUMI_START_BATCH;
 
-// Set bits of mask in the lanes below this one.
-__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
 // Persistent-lane driver over a queue of n rays.  Rays differ in cost by two
 // orders of magnitude, so one ray per lane leaves a wave waiting on its
 // slowest lane.  Here every lane steps its own ray one node or primitive at
@@ -748,7 +862,7 @@ __device__ __forceinline__ void run_lanes(uint32_t n, uint32_t* counter, Item& i
     // than two rays per lane of the grid remain, to balance the tail.
     const uint32_t tail = gridDim.x * blockDim.x * 2u;
     const uint64_t big_tail = (uint64_t)gridDim.x * blockDim.x * 16u;
-    TravState s;
+    TravState s = {};
     bool active = false, has_next = false, drained = false, unfinished = false;
     uint32_t cbase = 0, cleft = 0, pend_val = 0, pend_sz = 0;  // wave-uniform chunk state
     bool pend = false;
@@ -798,7 +912,14 @@ __device__ __forceinline__ void run_lanes(uint32_t n, uint32_t* counter, Item& i
             cbase += take;
             cleft -= take;
         }
-        if (active) {
+        if constexpr (Item::kWaveStep) {
+            // the whole wave steps; lanes without a ray only lend their ALUs
+            const bool a = item.step(s, active);
+            if (active) {
+                active = a;
+                unfinished = !a;
+            }
+        } else if (active) {
             active = item.step(s);
             unfinished = !active;
         }
