@@ -392,7 +392,8 @@ typedef struct {
     uint32_t n_candidates;        /* out: candidates timed */
     float best_us;                /* out: the winner's median time */
     float* host_candidate_us;     /* caller array or NULL: each candidate's median time in us
-                                     (-2: skipped as equal to the previous candidate) */
+                                     (-2: skipped as equal to the previous candidate,
+                                     -3: skipped, its buffers did not fit in device memory) */
     uint32_t candidates_capacity; /* entries host_candidate_us holds */
     int32_t repeats;              /* timed runs per candidate after one untimed run (0: 3 for
                                      traversal, 1 for rendering) */
@@ -416,7 +417,9 @@ int rumi_autotune_trace(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* dev_
 /* Renders desc's frame with candidate queue sizes and writes the fastest
  * into desc (path tracer and RayTracer: queue_capacity 2^19..2^22 with the
  * refill threshold at the capacity or half of it; bidirectional tracer:
- * samples per batch 2^20..2^24); dev_rgb receives the last render.  The
+ * samples per batch 2^20..2^26, about 1.8 KB of device memory per sample
+ * at 9 + 9 vertices; a candidate that does not fit in device memory is
+ * skipped); dev_rgb receives the last render.  The
  * frame of Scene::Rendering (src/Scene/scene.cpp:206-258). */
 int rumi_autotune_render(rumi_ctx* ctx, const rumi_scene* scene, rumi_render_desc* desc, float* dev_rgb,
                          rumi_autotune_report* report, void* stream);

@@ -135,10 +135,12 @@ extern "C" int rumi_autotune_render(rumi_ctx* ctx, const rumi_scene* scene, rumi
     if (repeats > 100) return fail(RUMI_ERR_INVALID, "rumi_autotune_render: more than 100 repeats");
     const hipStream_t s = (hipStream_t)stream;
     // path tracer / RayTracer: paths in flight and the refill threshold;
-    // bidirectional tracer: samples per batch
+    // bidirectional tracer: samples per batch (2^26 samples: about 120 GB of
+    // subpaths, walks and visibility tests at 9 + 9 vertices)
     std::vector<std::pair<uint32_t, uint32_t>> cands;
     if (desc->integrator == RUMI_INTEGRATOR_BDPT) {
-        for (uint32_t c : {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24}) cands.push_back({c, 0u});
+        for (uint32_t c : {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26})
+            cands.push_back({c, 0u});
     } else {
         for (uint32_t c : {1u << 19, 1u << 20, 1u << 21, 1u << 22}) {
             cands.push_back({c, c});
@@ -157,12 +159,26 @@ extern "C" int rumi_autotune_render(rumi_ctx* ctx, const rumi_scene* scene, rumi
         if ((uint64_t)cands[k].first > total && k > 0 && (uint64_t)cands[k - 1].first >= total) {
             t = -2.0f;  // the same queue as a smaller candidate: the frame fits either
         } else {
-            t = time_median([&]() { return rumi_render(ctx, scene, &d, dev_rgb, nullptr, stream); }, repeats, s);
-            if (t < 0.0f) return RUMI_ERR_HIP;
+            int rc = RUMI_OK;
+            t = time_median(
+                [&]() {
+                    rc = rumi_render(ctx, scene, &d, dev_rgb, nullptr, stream);
+                    return rc;
+                },
+                repeats, s);
+            if (t < 0.0f) {
+                // a candidate whose buffers do not fit in device memory is
+                // skipped; any other failure ends the tuning
+                if (rc != RUMI_ERR_NO_MEMORY) return rc != RUMI_OK ? rc : RUMI_ERR_HIP;
+                (void)hipGetLastError();
+                clear_error();
+                t = -3.0f;
+            }
         }
         us.push_back(t);
         if (t >= 0.0f && (bi < 0 || t < bt)) bi = (int)k, bt = t;
     }
+    if (bi < 0) return fail(RUMI_ERR_NO_MEMORY, "rumi_autotune_render: no candidate fits in device memory");
     desc->queue_capacity = cands[(size_t)bi].first;
     desc->regen_threshold = cands[(size_t)bi].second;
     report_times(rep, us, bt);
