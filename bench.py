@@ -275,6 +275,9 @@ def whitted_leg(rumi, ctx, args, log, scenes):
     spp, depth = 16, 10
     scene = rumi.Scene(ctx, scenes.cornell_box(w, h))
     kw = dict(integrator=rumi.RUMI_INTEGRATOR_WHITTED)
+    # queue size from the native autotuner on this frame (untimed)
+    qtune = {} if args.gpu_only else scene.autotune(w, h, spp, depth, seed=1, **kw)[0]
+    kw.update(qtune)
     scene.render(w, h, spp, depth, seed=1, **kw)
     torch.cuda.synchronize()
     times = []
@@ -284,6 +287,7 @@ def whitted_leg(rumi, ctx, args, log, scenes):
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     scene.close()
+    kw = dict(integrator=rumi.RUMI_INTEGRATOR_WHITTED)
     sec = min(times)
     sps = st.samples / sec
     import oracle
@@ -302,7 +306,7 @@ def whitted_leg(rumi, ctx, args, log, scenes):
         f"CPU {cpu_sps:.0f} samples/s")
     return {"samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 5),
             "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
-            "max_depth": depth, "same_seed_rel_l1_64px": l1,
+            "max_depth": depth, "queue_tuning": qtune, "same_seed_rel_l1_64px": l1,
             "cpu_baseline": {"value": round(cpu_sps, 1), "unit": "samples/s", "cores": 1, "kind": "port",
                              "sample": f"64x64 at {spp} spp, the oracle's FP64 restatement of "
                                        f"RayTracer::Trace ({cpu_sec:.2f}s)"},
@@ -382,12 +386,14 @@ def model_leg(rumi, ctx, args, log, scenes):
         load_s = time.perf_counter() - t0
     sc = scenes.model_scene(model, w, h)
     scene = rumi.Scene(ctx, sc)
-    img, st = scene.render(w, h, spp, depth, seed=1)
+    # queue size from the native autotuner on this frame (untimed)
+    qtune = {} if args.gpu_only else scene.autotune(w, h, spp, depth, seed=1)[0]
+    img, st = scene.render(w, h, spp, depth, seed=1, **qtune)
     torch.cuda.synchronize()
     times = []
     for k in range(args.render_runs):
         t0 = time.perf_counter()
-        img, st = scene.render(w, h, spp, depth, seed=10 + k)
+        img, st = scene.render(w, h, spp, depth, seed=10 + k, **qtune)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     scene.close()
@@ -411,7 +417,7 @@ def model_leg(rumi, ctx, args, log, scenes):
         "samples_per_s": round(sps, 1), "unit": "samples/s", "seconds": round(sec, 4),
         "device_ms": round(st.milliseconds, 3), "width": w, "height": h, "spp": spp,
         "max_depth": depth, "triangles": int(model["idx"].shape[0]),
-        "obj_load_s": round(load_s, 3),
+        "obj_load_s": round(load_s, 3), "queue_tuning": qtune,
         "extension_rays": int(st.extension_rays), "shadow_rays": int(st.shadow_rays),
         "rays_per_s": round((st.extension_rays + st.shadow_rays) / sec, 1),
         "same_seed_rel_l1_60x50": l1,

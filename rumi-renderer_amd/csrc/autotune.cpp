@@ -142,7 +142,7 @@ extern "C" int rumi_autotune_render(rumi_ctx* ctx, const rumi_scene* scene, rumi
         for (uint32_t c : {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26})
             cands.push_back({c, 0u});
     } else {
-        for (uint32_t c : {1u << 19, 1u << 20, 1u << 21, 1u << 22}) {
+        for (uint32_t c : {1u << 19, 1u << 20, 1u << 21, 1u << 22, 1u << 23}) {
             cands.push_back({c, c});
             cands.push_back({c, c / 2});
         }

@@ -441,8 +441,8 @@ def test_native_render_autotuner_choice_renders_the_same_image(rumi, ctx, scenes
     kw = dict(integrator=integrator, mis_beta=1.0) if integrator == 1 else {}
     tune, table = sc.autotune(64, 64, 64, 5, seed=3, **kw)
     assert tune["queue_capacity"] in ([1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26]
-                                      if integrator == 1 else [1 << 19, 1 << 20, 1 << 21, 1 << 22])
-    assert len(table) == (7 if integrator == 1 else 8) and table[0] is not None
+                                      if integrator == 1 else [1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23])
+    assert len(table) == (7 if integrator == 1 else 10) and table[0] is not None
     a, _ = sc.render(64, 64, 64, 5, seed=3, **tune, **kw)
     b, _ = sc.render(64, 64, 64, 5, seed=3, **kw)
     torch.cuda.synchronize()
