@@ -122,12 +122,14 @@ def tune_render(run: Callable[..., object], candidates: Iterable[dict] | None = 
 
 
 def tune_batch(build: Callable[[int], object], measure: Callable[[object], float],
-               capacities: Sequence[int] = (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25),
+               capacities: Sequence[int] = tuple(1 << k for k in range(20, 29)),
                release: Callable[[object], None] | None = None):
     """Generation size of a batch: build(capacity) makes the batch,
     measure(batch) returns the seconds of one run (the timer: for a captured
     graph, one replay after the capture), release(batch) frees it.  Returns
-    (best capacity, [(capacity, seconds), ...]); ties keep the smaller."""
+    (best capacity, [(capacity, seconds), ...]); ties keep the smaller.  The
+    default candidates reach 2^28 paths in flight (about 63 GB of wavefront
+    queues at 236 B per path)."""
     results = []
     best, best_t = None, float("inf")
     for cap in capacities:

@@ -81,3 +81,31 @@ def test_native_candidates_equal_the_python_list():
     import importlib
     rumi = importlib.import_module("rumi-renderer_amd")
     assert rumi.autotune_trace_candidates() == autotune.trace_candidates()
+
+
+def test_batch_generation_size_choice_ties_and_release():
+    """tune_batch builds, times and releases every candidate in order and
+    keeps the fastest; a tie keeps the smaller generation."""
+    built, released = [], []
+    cost = {1 << 20: 5.0, 1 << 21: 3.0, 1 << 22: 2.0, 1 << 23: 2.0, 1 << 24: 4.0}
+
+    def build(cap):
+        built.append(cap)
+        return {"cap": cap}
+
+    best, table = autotune.tune_batch(build, lambda b: cost[b["cap"]], capacities=sorted(cost),
+                                      release=lambda b: released.append(b["cap"]))
+    assert best == 1 << 22
+    assert built == released == sorted(cost)
+    assert table == [(c, cost[c]) for c in sorted(cost)]
+    with pytest.raises(ValueError):
+        autotune.tune_batch(build, lambda b: 1.0, capacities=[])
+
+
+def test_batch_default_candidates_reach_the_whole_config4_batch():
+    """The default generation sizes run from 2^20 to 2^28 paths (the 256-scene
+    workload's 170.8M samples fit one generation of 2^28)."""
+    import inspect
+    caps = inspect.signature(autotune.tune_batch).parameters["capacities"].default
+    assert list(caps) == [1 << k for k in range(20, 29)]
+    assert caps[-1] >= 170_755_624
