@@ -417,7 +417,7 @@ int rumi_autotune_trace(rumi_ctx* ctx, const rumi_bvh* bvh, const rumi_ray* dev_
 /* Renders desc's frame with candidate queue sizes and writes the fastest
  * into desc (path tracer and RayTracer: queue_capacity 2^19..2^23 with the
  * refill threshold at the capacity or half of it; bidirectional tracer:
- * samples per batch 2^20..2^26, about 1.8 KB of device memory per sample
+ * samples per batch 2^20..2^26, about 2 KB of device memory per sample
  * at 9 + 9 vertices; a candidate that does not fit in device memory is
  * skipped); dev_rgb receives the last render.  The
  * frame of Scene::Rendering (src/Scene/scene.cpp:206-258). */

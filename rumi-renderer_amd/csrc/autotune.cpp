@@ -135,7 +135,7 @@ extern "C" int rumi_autotune_render(rumi_ctx* ctx, const rumi_scene* scene, rumi
     if (repeats > 100) return fail(RUMI_ERR_INVALID, "rumi_autotune_render: more than 100 repeats");
     const hipStream_t s = (hipStream_t)stream;
     // path tracer / RayTracer: paths in flight and the refill threshold;
-    // bidirectional tracer: samples per batch (2^26 samples: about 120 GB of
+    // bidirectional tracer: samples per batch (2^26 samples: about 130 GB of
     // subpaths, walks and visibility tests at 9 + 9 vertices)
     std::vector<std::pair<uint32_t, uint32_t>> cands;
     if (desc->integrator == RUMI_INTEGRATOR_BDPT) {
