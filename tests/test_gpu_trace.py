@@ -275,15 +275,25 @@ def test_barycentrics_reconstruct_hit_point(rumi, ctx, mesh, rays):
     assert (u >= 0).all() and (v >= 0).all() and (u + v <= 1 + 1e-7).all()
 
 
-def test_edge_cases(rumi, oracle, ctx, mesh, rays):
+@pytest.mark.parametrize("width", [2, 4])
+def test_edge_cases(rumi, oracle, ctx, mesh, rays, width):
+    """Ragged batches, empty batches, zero directions, finite segments and a
+    caller's closest-so-far, on both tree widths (width 4: the persistent
+    kernel whose waves share their triangle tests, with partly filled waves
+    in the ragged batches)."""
     xyz, idx = mesh
+    ctx.set_trace_tuning(width=width)
+    ordered = 2 if width == 4 else True
     b = rumi.Bvh(ctx, xyz, idx)
     ob = oracle.MedianBvh.from_export(b.export(), idx.shape[0])
     # ragged sizes around the block size
     for n in (1, 63, 64, 127, 128, 129, 1031):
         p, t = closest(rumi, ctx, b, rays[:n])
-        op, ot = ob.trace_mesh(xyz, idx, rays[:n], ordered=True)
+        op, ot = ob.trace_mesh(xyz, idx, rays[:n], ordered=ordered)
         np.testing.assert_array_equal(p, op)
+        np.testing.assert_array_equal(bits(t), bits(ot))
+        oa, _ = ob.trace_mesh(xyz, idx, rays[:n], any_hit=True, ordered=ordered)
+        np.testing.assert_array_equal(anyhit(rumi, ctx, b, rays[:n]), oa)
     # n = 0 is a no-op
     p, t = closest(rumi, ctx, b, rays[:0])
     assert p.size == 0
@@ -301,18 +311,20 @@ def test_edge_cases(rumi, oracle, ctx, mesh, rays):
     assert (p1[hit] == -1).all()
     a1 = anyhit(rumi, ctx, b, seg)
     assert (a1[hit] == -1).all()
-    op1, _ = ob.trace_mesh(xyz, idx, seg, ordered=True)
+    op1, _ = ob.trace_mesh(xyz, idx, seg, ordered=ordered)
     np.testing.assert_array_equal(p1, op1)
     # a caller-supplied "closest so far" (tmin > 0) is honoured
     pre = rays[:4000].copy()
     pre[hit, 3] = t0[hit] * np.float32(0.999)
     p2, t2 = closest(rumi, ctx, b, pre)
-    op2, ot2 = ob.trace_mesh(xyz, idx, pre, ordered=True)
+    op2, ot2 = ob.trace_mesh(xyz, idx, pre, ordered=ordered)
     np.testing.assert_array_equal(p2, op2)
     np.testing.assert_array_equal(bits(t2), bits(ot2))
 
 
-def test_empty_and_single_triangle_meshes(rumi, oracle, ctx, rays):
+@pytest.mark.parametrize("width", [2, 4])
+def test_empty_and_single_triangle_meshes(rumi, oracle, ctx, rays, width):
+    ctx.set_trace_tuning(width=width)
     xyz = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
     empty = rumi.Bvh(ctx, xyz, np.zeros((0, 3), np.uint32))
     p, t = closest(rumi, ctx, empty, rays[:500])
