@@ -228,6 +228,9 @@ def main():
             "unit": "GB/s", "frac": round(achieved_c / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_ray": round(b_ray_c, 1), "nodes_per_ray": round(nodes_c, 3),
             "tris_per_ray": round(tris_c, 3), "node_bytes": node_bc,
+            # the I/O-only floor: ray in + hit out, no tree bytes
+            "io_floor_bytes_per_ray": RAY_IN_BYTES + HIT_OUT_BYTES,
+            "io_floor_mrays": round(HBM_PEAK_GBS * 1e9 / (RAY_IN_BYTES + HIT_OUT_BYTES) / 1e6, 1),
             "achievable_copy_GBs": copy_gbs,
             "frac_of_copy": round(achieved_c / copy_gbs, 4) if copy_gbs else None,
         },
