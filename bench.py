@@ -188,6 +188,12 @@ def main():
                          f"traversal timed only ({secs:.2f}s; build {build_s:.2f}s untimed)",
                "host_cpu": _cpu_model(), "nproc": os.cpu_count()}
         log(f"CPU baseline: {cpu['value']:.3f} Mrays/s")
+        # for context: the same sample on the host's share of cores (at most
+        # 16, the CPU share of one GPU on the box), one shared tree
+        nth = max(1, min(os.cpu_count() or 1, 16))
+        _, _, secs_mt, _ = oracle.time_median6_mesh(xyz, idx, rays_h[:k], nthreads=nth)
+        cpu["all_cores"] = {"value": k / secs_mt / 1e6, "unit": "Mrays/s", "cores": nth}
+        log(f"CPU baseline on {nth} threads: {cpu['all_cores']['value']:.3f} Mrays/s")
         # the benchmarked kernel's hits on those rays against the reference's
         # own tree and order: ids equal except at exact ties (1e-6 relative t)
         rumi.trace_closest(ctx, bvh, rays[:k], hits[:k], None, 0, stream)

@@ -74,6 +74,9 @@ def lib() -> ctypes.CDLL:
         L.or_time_median6_mesh.argtypes = [vp, vp, c_int, vp, c_int, c_int, vp, vp,
                                            POINTER(c_double)]
         L.or_time_median6_mesh.restype = c_double
+        L.or_time_median6_mesh_mt.argtypes = [vp, vp, c_int, vp, c_int, c_int, vp, vp,
+                                              POINTER(c_double), c_int]
+        L.or_time_median6_mesh_mt.restype = c_double
         L.or_obj_load.argtypes = [ctypes.c_char_p, vp, POINTER(vp), POINTER(vp), POINTER(c_int), vp]
         L.or_obj_free.argtypes = [vp]
         L.or_obj_free.restype = None
@@ -323,9 +326,10 @@ def brute_mesh(xyz, idx, rays):
     return prim, t
 
 
-def time_median6_mesh(xyz, idx, rays, any_hit=False):
-    """Reference CPU path: median-6 build, then Travel per ray (1 thread).
-    Returns (prim, t, traversal_seconds, build_seconds)."""
+def time_median6_mesh(xyz, idx, rays, any_hit=False, nthreads=1):
+    """Reference CPU path: median-6 build, then Travel per ray (on nthreads
+    host threads, the batch split in contiguous ranges).  Returns (prim, t,
+    traversal_seconds, build_seconds)."""
     xyz = np.ascontiguousarray(xyz, np.float32)
     idx = np.ascontiguousarray(idx, np.uint32)
     rays = np.ascontiguousarray(rays, np.float32)
@@ -333,8 +337,8 @@ def time_median6_mesh(xyz, idx, rays, any_hit=False):
     prim = np.empty(n, np.int32)
     t = np.empty(n, np.float32)
     bs = c_double()
-    secs = lib().or_time_median6_mesh(_p(xyz), _p(idx), idx.shape[0], _p(rays), n,
-                                      int(any_hit), _p(prim), _p(t), byref(bs))
+    secs = lib().or_time_median6_mesh_mt(_p(xyz), _p(idx), idx.shape[0], _p(rays), n,
+                                         int(any_hit), _p(prim), _p(t), byref(bs), int(nthreads))
     return prim, t, secs, bs.value
 
 

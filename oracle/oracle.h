@@ -153,6 +153,12 @@ void or_bvh_copy(const or_bvh* b, float* lo, float* hi, int* left, int* right,
 double or_time_median6_mesh(const float* xyz, const uint32_t* idx, int nt,
                             const float* rays, int n, int any,
                             int32_t* prim_out, float* t_out, double* build_seconds);
+/* The same over nthreads host threads (the ray batch split in equal
+ * contiguous ranges, one shared tree); the wall time of the ray loop. */
+double or_time_median6_mesh_mt(const float* xyz, const uint32_t* idx, int nt,
+                               const float* rays, int n, int any,
+                               int32_t* prim_out, float* t_out, double* build_seconds,
+                               int nthreads);
 
 /* ------------------------------------------------------------------ */
 /* FP64 path tracer (oracle_render.c): PathTracer::Trace(w, h, ...)

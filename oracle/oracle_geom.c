@@ -6,6 +6,7 @@
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, so the FP32 operation
  * order written below is the order executed).
  */
+#include <pthread.h>
 #include "oracle.h"
 
 #include <float.h>
@@ -882,9 +883,42 @@ static double now_seconds(void)
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* One thread's share of or_time_median6_mesh's ray loop: rays [i0, i1). */
+typedef struct {
+    const or_bvh* b;
+    const or_prim* prims;
+    const float* rays;
+    int i0, i1, any;
+    int32_t* prim_out;
+    float* t_out;
+} travel_job;
+
+static void* travel_rays(void* arg)
+{
+    const travel_job* j = (const travel_job*)arg;
+    int i;
+    for (i = j->i0; i < j->i1; ++i) {
+        or_ray ray;
+        load_ray(j->rays + 8 * (size_t)i, &ray);
+        float tin = ray.tmin;
+        int h = or_travel(j->b, j->prims, &ray, j->any, 0, 0, 0);
+        if (j->prim_out) j->prim_out[i] = h;
+        if (j->t_out) j->t_out[i] = h >= 0 ? ray.tmin : tin;
+    }
+    return 0;
+}
+
 double or_time_median6_mesh(const float* xyz, const uint32_t* idx, int nt,
                             const float* rays, int n, int any,
                             int32_t* prim_out, float* t_out, double* build_seconds)
+{
+    return or_time_median6_mesh_mt(xyz, idx, nt, rays, n, any, prim_out, t_out, build_seconds, 1);
+}
+
+double or_time_median6_mesh_mt(const float* xyz, const uint32_t* idx, int nt,
+                               const float* rays, int n, int any,
+                               int32_t* prim_out, float* t_out, double* build_seconds,
+                               int nthreads)
 {
     float* lo = (float*)malloc(sizeof(float) * 3 * (size_t)nt);
     float* hi = (float*)malloc(sizeof(float) * 3 * (size_t)nt);
@@ -898,15 +932,30 @@ double or_time_median6_mesh(const float* xyz, const uint32_t* idx, int nt,
     or_bvh_build_median6(lo, hi, nt, &b);
     double t1 = now_seconds();
     if (build_seconds) *build_seconds = t1 - t0;
-    int i;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    travel_job jobs[256];
+    pthread_t th[256];
+    int t, started = 0;
     double s0 = now_seconds();
-    for (i = 0; i < n; ++i) {
-        or_ray ray;
-        load_ray(rays + 8 * (size_t)i, &ray);
-        float tin = ray.tmin;
-        int h = or_travel(&b, prims, &ray, any, 0, 0, 0);
-        if (prim_out) prim_out[i] = h;
-        if (t_out) t_out[i] = h >= 0 ? ray.tmin : tin;
+    for (t = 0; t < nthreads; ++t) {
+        jobs[t].b = &b;
+        jobs[t].prims = prims;
+        jobs[t].rays = rays;
+        jobs[t].i0 = (int)((long long)n * t / nthreads);
+        jobs[t].i1 = (int)((long long)n * (t + 1) / nthreads);
+        jobs[t].any = any;
+        jobs[t].prim_out = prim_out;
+        jobs[t].t_out = t_out;
+    }
+    if (nthreads == 1) {
+        travel_rays(&jobs[0]);
+    } else {
+        for (t = 0; t < nthreads; ++t)
+            if (pthread_create(&th[t], 0, travel_rays, &jobs[t]) == 0) started = t + 1;
+            else break;
+        for (t = 0; t < started; ++t) pthread_join(th[t], 0);
+        for (t = started; t < nthreads; ++t) travel_rays(&jobs[t]);  /* threads that did not start */
     }
     double s1 = now_seconds();
     or_bvh_free(&b);

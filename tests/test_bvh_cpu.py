@@ -85,6 +85,22 @@ def test_oracle_traversal_matches_brute_force(rumi, oracle, small_mesh):
     assert np.array_equal(pa >= 0, bp >= 0)
 
 
+def test_oracle_timed_traversal_threads_give_the_same_hits(rumi, oracle, small_mesh):
+    """The bench's CPU-baseline driver (median-6 build + Travel per ray) on one
+    thread and on several: the same hits bit for bit, and those of the
+    oracle's own traversal in reference order."""
+    xyz, idx = small_mesh
+    rays = rumi.make_rays_uniform(5000, 6, 2.1)
+    p1, t1, s1, b1 = oracle.time_median6_mesh(xyz, idx, rays)
+    p4, t4, s4, b4 = oracle.time_median6_mesh(xyz, idx, rays, nthreads=4)
+    lo, hi = oracle.mesh_bounds(xyz, idx)
+    op, ot = oracle.MedianBvh(lo, hi).trace_mesh(xyz, idx, rays, ordered=False)
+    np.testing.assert_array_equal(p1, p4)
+    np.testing.assert_array_equal(t1.view(np.int32), t4.view(np.int32))
+    np.testing.assert_array_equal(p1, op)
+    assert s1 > 0 and s4 > 0 and b1 > 0
+
+
 def test_oracle_traversal_inside_sphere_all_hit(rumi, oracle, small_mesh):
     xyz, idx = small_mesh
     rays = rumi.make_rays_uniform(2000, 9, 0.5)  # origins well inside the r >= 0.95 shell
