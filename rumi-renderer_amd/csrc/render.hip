@@ -1613,7 +1613,7 @@ constexpr uint64_t kConnMix = 0x9FB21C651E98DF25ull;  // connection streams
 
 struct BVert {
     float4 a;  // p.xyz, pdf_fwd (area)
-    float4 b;  // shading normal.xyz, pdf_rev (area)
+    float4 b;  // shading normal.xyz, w: an emitter vertex's primitive id
     float4 c;  // beta.rgb, flags
     float4 d;  // texture multiplier rgb, material (int bits)
 };
@@ -1621,7 +1621,8 @@ struct BVert {
 struct BdptBufs {
     BVert* eye;       // [sample][vertex], vslot()
     BVert* light;
-    float4* eye_mis;  // [sample][vertex]: pdf_fwd, pdf_rev, flags, 0 -- what the MIS walk reads
+    float4* eye_mis;  // [sample][vertex]: pdf_fwd, -, flags, pdf_rev of the vertex before -- what
+                      // the MIS walk reads
     float4* light_mis;
     uint8_t* n_eye;
     uint8_t* n_light;
@@ -1761,14 +1762,15 @@ __device__ inline void surface_at(const SceneView& sv, int32_t prim, V3 o, V3 d,
 }
 
 __device__ inline void store_vert(BVert* arr, float4* mis, size_t idx, V3 p, float pdf_fwd, V3 n,
-                                  V3 beta, uint32_t flags, V3 tex, int mat, float bw = 0.0f) {
+                                  V3 beta, uint32_t flags, V3 tex, int mat, float bw = 0.0f,
+                                  float prev_rev = 0.0f) {
     BVert v;
     v.a = make_float4(p.x, p.y, p.z, pdf_fwd);
     v.b = make_float4(n.x, n.y, n.z, bw);
     v.c = make_float4(beta.x, beta.y, beta.z, __int_as_float((int)flags));
     v.d = make_float4(tex.x, tex.y, tex.z, __int_as_float(mat));
     arr[idx] = v;
-    mis[idx] = make_float4(pdf_fwd, bw, __int_as_float((int)flags), 0.0f);
+    mis[idx] = make_float4(pdf_fwd, bw, __int_as_float((int)flags), prev_rev);
 }
 
 // One segment of a walk (eye: radiance transport; light: importance
@@ -1779,6 +1781,8 @@ __device__ inline void store_vert(BVert* arr, float4* mis, size_t idx, V3 p, flo
 // (bidirectionalpathtracer.cpp:396-711, :713-1016), one traversal apart.
 struct WalkState {
     V3 o, d, beta, prev_p;
+    V3 prev_n;      // the previous vertex's stored normal (medium vertices: the direction)
+    bool prev_med;  // the previous vertex is a medium vertex
     float pdf_w;
     int nv;
     bool in_obj;
@@ -1790,6 +1794,8 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
     int& nv = st.nv;
     bool& in_obj = st.in_obj;
     V3& prev_p = st.prev_p;
+    V3& prev_n = st.prev_n;
+    bool& prev_med = st.prev_med;
     V3& o = st.o;
     V3& d = st.d;
     V3& beta = st.beta;
@@ -1812,20 +1818,19 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
                 // a medium vertex: HomogeneousVolume::PhaseSampling, Fp / pdf = sigma_s
                 const V3 mp = o + d * (tn < 0.0f ? sd : tn + sd);
                 const size_t mslot = vslot(j, nv, cap);
-                store_vert(arr, mis, mslot, mp, to_meas(md, pdf_w, prev_p, mp, d, true, sd), d, beta,
-                           kVMedium, v3(1, 1, 1), -1);
-                ++nv;
                 float pdf_ph;
                 const V3 w = phase_sample(md.g, d, g, pdf_ph);
-                if (nv >= 2) {
-                    const size_t pi = vslot(j, nv - 2, cap);
-                    BVert& pv = arr[pi];
-                    pv.b.w = to_meas(md, pdf_ph, mp, pv, sd);
-                    mis[pi].y = pv.b.w;
-                }
+                // the previous vertex's reverse pdf (area measure there),
+                // kept in this vertex's MIS record
+                const float prev_rev = nv >= 1 ? to_meas(md, pdf_ph, mp, prev_p, prev_n, prev_med, sd) : 0.0f;
+                store_vert(arr, mis, mslot, mp, to_meas(md, pdf_w, prev_p, mp, d, true, sd), d, beta,
+                           kVMedium, v3(1, 1, 1), -1, 0.0f, prev_rev);
+                ++nv;
                 beta = cmul(beta, ld3(md.sigma_s));
                 pdf_w = pdf_ph;
                 prev_p = mp;
+                prev_n = d;
+                prev_med = true;
                 o = mp;
                 d = w;
                 in_obj = false;
@@ -1855,18 +1860,16 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
             int tix;
             const V3 txs = sv.shapes[prim].info[2] >= 0 ? pattern(sv, sv.shapes[prim], ps, bu, bv, tix)
                                                        : v3(1, 1, 1);
+            // (a Dirac vertex: the previous vertex's reverse pdf is 0)
             store_vert(arr, mis, vslot(j, nv, cap), ps, 0.0f, d, beta, kVDelta, txs, mi);
             ++nv;
-            if (nv >= 2) {
-                const size_t pi = vslot(j, nv - 2, cap);
-                arr[pi].b.w = 0.0f;
-                mis[pi].y = 0.0f;
-            }
             beta = cmul(cmul(cmul(beta, ratio), ld3(m.sigma_s)), txs);
             float pdf_ph;
             const V3 w = phase_sample(0.0f, d, g, pdf_ph);
             pdf_w = 0.0f;
             prev_p = ps;
+            prev_n = d;
+            prev_med = false;
             o = ps;
             d = w;
             return true;
@@ -1980,17 +1983,15 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
                                      expf(-m.absorption[2] * t)));
         if (was_in && m.kind == RUMI_MAT_SUBSURFACE) weight = cmul(weight, vol);
     }
-    store_vert(arr, mis, slot, p, pdf_fwd, n, beta, vflags | (delta ? kVDelta : 0u), tex, mi);
-    // reverse pdf of the previous vertex (area measure at prev)
-    if (nv >= 2) {
-        const size_t pi = vslot(j, nv - 2, cap);
-        BVert& pv = arr[pi];
-        pv.b.w = delta ? 0.0f : to_meas(md, pdf_rev, p, pv, seg_L);
-        mis[pi].y = pv.b.w;
-    }
+    // the previous vertex's reverse pdf (area measure there), kept in this
+    // vertex's MIS record
+    const float prev_rev = nv >= 2 && !delta ? to_meas(md, pdf_rev, p, prev_p, prev_n, prev_med, seg_L) : 0.0f;
+    store_vert(arr, mis, slot, p, pdf_fwd, n, beta, vflags | (delta ? kVDelta : 0u), tex, mi, 0.0f, prev_rev);
     beta = cmul(beta, weight);
     pdf_w = delta ? 0.0f : pdf_next;
     prev_p = p;
+    prev_n = n;
+    prev_med = false;
     o = p;
     d = w;
     if (m.kind != RUMI_MAT_DIELECTRIC && m.kind != RUMI_MAT_SUBSURFACE) in_obj = false;
@@ -2005,6 +2006,7 @@ struct WalkBuf {
     uint32_t* flags;   // bit 1 in_object, bit 2 light walk, bits 8.. vertices stored
     float4* beta;      // throughput rgb, pdf_w of the segment's direction
     float4* prev;      // the previous vertex's position
+    float4* prevn;     // the previous vertex's stored normal, w: 1 for a medium vertex
     uint64_t* rng;     // PCG32 state (increment from the pixel)
     uint32_t* slot;    // sample index within the batch
 };
@@ -2018,6 +2020,7 @@ __device__ inline void write_walk(const WalkBuf& wb, uint32_t q, const WalkState
     wb.flags[q] = (st.in_obj ? 2u : 0u) | light | ((uint32_t)st.nv << 8);
     wb.beta[q] = make_float4(st.beta.x, st.beta.y, st.beta.z, st.pdf_w);
     wb.prev[q] = make_float4(st.prev_p.x, st.prev_p.y, st.prev_p.z, 0.0f);
+    wb.prevn[q] = make_float4(st.prev_n.x, st.prev_n.y, st.prev_n.z, st.prev_med ? 1.0f : 0.0f);
     wb.rng[q] = rng;
     wb.slot[q] = j;
 }
@@ -2058,6 +2061,8 @@ __device__ inline bool init_walk(const BdptBufs& bb, uint64_t base, const BdptPa
         // Pinhole::Pdfw: 1 / (cos^3 film_area) at plane distance 1
         st.pdf_w = 1.0f / (l.z * l.z * l.z * bp.film_area);
         st.o = st.prev_p = cp;
+        st.prev_n = cdir;
+        st.prev_med = false;
         st.beta = v3(1, 1, 1);
         bb.n_eye[j] = 1;
         go = bp.n_eye > 1;
@@ -2089,6 +2094,8 @@ __device__ inline bool init_walk(const BdptBufs& bb, uint64_t base, const BdptPa
             st.beta = le * (cost / (pdf_pos * pdf_dir));
             st.pdf_w = pdf_dir;
             st.o = st.prev_p = p0;
+            st.prev_n = n0;
+            st.prev_med = false;
             bb.n_light[j] = 1;
             go = bp.n_light > 1;
         }
@@ -2135,6 +2142,9 @@ __device__ inline bool walk_one(const BdptBufs& bb, uint64_t base, const BdptPar
     st.beta = v3(b4.x, b4.y, b4.z);
     st.pdf_w = b4.w;
     st.prev_p = v3(p4.x, p4.y, p4.z);
+    const float4 n4 = in.prevn[i];
+    st.prev_n = v3(n4.x, n4.y, n4.z);
+    st.prev_med = n4.w != 0.0f;
     st.nv = (int)(flags >> 8);
     st.in_obj = (flags & 2u) != 0;
     const int2 h = hits[i];
@@ -2356,28 +2366,32 @@ __device__ __forceinline__ bool connect_one(const BdptBufs& bb, uint64_t base, c
         // each vertex once
         float sum = 0.0f, ri = 1.0f;
         if (t - 1 > 0) {
-            float4 mi = bb.eye_mis[vslot(j, t - 1, nEc)];
+            // record i holds vertex i's pdf_fwd and flags and vertex i - 1's
+            // pdf_rev: nx is record i + 1
+            float4 mi = bb.eye_mis[vslot(j, t - 1, nEc)], nx = mi;
             for (int i = t - 1; i > 0; --i) {
                 const float4 mp = bb.eye_mis[vslot(j, i - 1, nEc)];
-                const float pr = i == t - 1 ? pt_rev : (i == t - 2 ? ptm_rev : mi.y);
+                const float pr = i == t - 1 ? pt_rev : (i == t - 2 ? ptm_rev : nx.w);
                 ri *= remap0(pr) / remap0(mi.x);
                 const bool di = i == t - 1 ? false : ((uint32_t)__float_as_int(mi.z) & kVDelta) != 0;
                 const bool dprev = ((uint32_t)__float_as_int(mp.z) & kVDelta) != 0;
                 if (!di && !dprev) sum += mis_pow(ri, bp.beta);
                 if ((uint32_t)__float_as_int(mi.z) & kVPhong) break;
+                nx = mi;
                 mi = mp;
             }
         }
         ri = 1.0f;
         if (s - 1 >= 0) {
-            float4 mi = bb.light_mis[vslot(j, s - 1, nLc)];
+            float4 mi = bb.light_mis[vslot(j, s - 1, nLc)], nx = mi;
             for (int i = s - 1; i >= 0; --i) {
                 const float4 mp = i > 0 ? bb.light_mis[vslot(j, i - 1, nLc)] : make_float4(0, 0, 0, 0);
-                const float pr = i == s - 1 ? qs_rev : (i == s - 2 ? qsm_rev : mi.y);
+                const float pr = i == s - 1 ? qs_rev : (i == s - 2 ? qsm_rev : nx.w);
                 ri *= remap0(pr) / remap0(mi.x);
                 const bool di = i == s - 1 ? false : ((uint32_t)__float_as_int(mi.z) & kVDelta) != 0;
                 const bool dprev = i > 0 ? ((uint32_t)__float_as_int(mp.z) & kVDelta) != 0 : false;
                 if (!di && !dprev) sum += mis_pow(ri, bp.beta);
+                nx = mi;
                 mi = mp;
             }
         }
@@ -2654,7 +2668,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     const uint32_t l1 = (uint32_t)nL + 1u, nbins = (uint32_t)(nE + 1) * l1;
     const size_t need = (vbytes + mbytes) * (size_t)(nE + nL) + 2 * (size_t)nb + 4 * (size_t)nb +
                         (4 + 4 + 16 + 4) * (size_t)nbins + 3 * sizeof(float4) * max_conn +
-                        walks * (2 * 80 + 16) + 40 * 256;
+                        walks * (2 * 96 + 16) + 40 * 256;
     if (sc->work_bytes < need) {
         if (sc->work) (void)hipFree(sc->work);
         sc->work = nullptr;
@@ -2691,6 +2705,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
         wb[q].flags = (uint32_t*)take(4 * walks);
         wb[q].beta = (float4*)take(sizeof(float4) * walks);
         wb[q].prev = (float4*)take(sizeof(float4) * walks);
+        wb[q].prevn = (float4*)take(sizeof(float4) * walks);
         wb[q].rng = (uint64_t*)take(8 * walks);
         wb[q].slot = (uint32_t*)take(4 * walks);
     }
