@@ -59,6 +59,10 @@ def main():
     ap.add_argument("--gpu-only", action="store_true",
                     help="render legs: GPU renders only, with the default queue sizes (no autotuner, "
                          "counting render, rel-MSE renders or CPU baseline): for profiling")
+    ap.add_argument("--bdpt-batch", type=int, default=0, metavar="SAMPLES",
+                    help="with --gpu-only: the BDPT leg's samples per batch (0: the library default)")
+    ap.add_argument("--queue", type=int, default=0, metavar="PATHS",
+                    help="with --gpu-only: the path-tracing legs' queue capacity (0: the library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -462,7 +466,7 @@ def render_legs(rumi, ctx, args, log, only=""):
         scene = rumi.Scene(ctx, sc)
         # queue sizes chosen by the native autotuner on this frame (untimed)
         if args.gpu_only:
-            qtune, qtable = {}, []
+            qtune, qtable = ({"queue_capacity": args.queue} if args.queue else {}), []
         else:
             qtune, qtable = scene.autotune(w, h, spp, depth, seed=1)
         img, st = scene.render(w, h, spp, depth, seed=1, **qtune)  # warm-up
@@ -856,7 +860,7 @@ def bdpt_leg(rumi, ctx, args, log, scenes):
     kw = dict(integrator=rumi.RUMI_INTEGRATOR_BDPT, mis_beta=2.0, n_eye=9, n_light=9)
     # samples per batch chosen by the native autotuner on this frame (untimed)
     if args.gpu_only:
-        qtune, qtable = {"queue_capacity": 0}, []
+        qtune, qtable = {"queue_capacity": args.bdpt_batch}, []
     else:
         qtune, qtable = scene.autotune(w, h, spp, max_len, seed=1, **kw)
     log(f"cornell_bdpt: batch sizes {qtable} -> {qtune}")
