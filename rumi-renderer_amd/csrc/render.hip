@@ -1609,6 +1609,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(ShadowBuf sh, const uint32_t*
 
 constexpr uint32_t kVDelta = 1u, kVEmitter = 2u, kVCamera = 4u, kVLight = 8u, kVPhong = 16u,
                    kVMedium = 32u;
+// MIS records only: the vertex before this one is a Dirac vertex
+constexpr uint32_t kVPrevDelta = 64u;
 constexpr uint64_t kConnMix = 0x9FB21C651E98DF25ull;  // connection streams
 
 struct BVert {
@@ -1621,8 +1623,8 @@ struct BVert {
 struct BdptBufs {
     BVert* eye;       // [sample][vertex], vslot()
     BVert* light;
-    float4* eye_mis;  // [sample][vertex]: pdf_fwd, -, flags, pdf_rev of the vertex before -- what
-                      // the MIS walk reads
+    float4* eye_mis;  // [sample][vertex]: pdf_fwd, the MIS prefix of the vertex before, flags (+
+                      // kVPrevDelta), pdf_rev of the vertex before -- what the MIS weights read
     float4* light_mis;
     uint8_t* n_eye;
     uint8_t* n_light;
@@ -1761,16 +1763,22 @@ __device__ inline void surface_at(const SceneView& sv, int32_t prim, V3 o, V3 d,
     tex = pattern(sv, s, p, bu, bv, tex_index);
 }
 
+__device__ inline float remap0(float f) { return f != 0.0f ? f : 1.0f; }
+
+__device__ inline float mis_pow(float r, float beta) {
+    return beta == 1.0f ? r : (beta == 2.0f ? r * r : powf(r, beta));
+}
+
 __device__ inline void store_vert(BVert* arr, float4* mis, size_t idx, V3 p, float pdf_fwd, V3 n,
                                   V3 beta, uint32_t flags, V3 tex, int mat, float bw = 0.0f,
-                                  float prev_rev = 0.0f) {
+                                  float prev_rev = 0.0f, float prefix = 0.0f, uint32_t mis_bits = 0u) {
     BVert v;
     v.a = make_float4(p.x, p.y, p.z, pdf_fwd);
     v.b = make_float4(n.x, n.y, n.z, bw);
     v.c = make_float4(beta.x, beta.y, beta.z, __int_as_float((int)flags));
     v.d = make_float4(tex.x, tex.y, tex.z, __int_as_float(mat));
     arr[idx] = v;
-    mis[idx] = make_float4(pdf_fwd, bw, __int_as_float((int)flags), prev_rev);
+    mis[idx] = make_float4(pdf_fwd, prefix, __int_as_float((int)(flags | mis_bits)), prev_rev);
 }
 
 // One segment of a walk (eye: radiance transport; light: importance
@@ -1783,14 +1791,46 @@ struct WalkState {
     V3 o, d, beta, prev_p;
     V3 prev_n;      // the previous vertex's stored normal (medium vertices: the direction)
     bool prev_med;  // the previous vertex is a medium vertex
+    // the MIS prefix recursion (mis_prefix): the previous vertex's pdf_fwd,
+    // Dirac and Phong flags, the Dirac flag of the vertex before it, and the
+    // prefix of the vertex before it
+    float prev_fwd, prev_sum;
+    bool prev_delta, prev_phong, prevprev_delta;
     float pdf_w;
     int nv;
     bool in_obj;
 };
 
+// The MIS weight of a connection sums, over the other strategies of its
+// path, products of ratios R_k = pdf_rev(k) / pdf_fwd(k) (0 read as 1) over
+// the subpath vertices k, raised to beta (connect_one).  The part that does
+// not depend on the connection is a prefix of each subpath, made when the
+// vertex after k is stored (its reverse pdf is known then) and kept in that
+// vertex's MIS record:
+//   eye:   E_k = R_k^b ([c_k] + [k not Phong] E_(k-1)),  E_0 = 0 (the camera)
+//   light: L_k = R_k^b ([c_k] + L_(k-1)),               L_(-1) = 0
+// c_k: neither vertex k nor k - 1 is a Dirac vertex.  An eye sum stops at a
+// Phong vertex, as the MIS walk of the reference stops there.
+template <bool EYE>
+__device__ inline float mis_prefix(const WalkState& st, int k, float rev, float beta) {
+    if (EYE && k < 1) return 0.0f;
+    const float rb = mis_pow(remap0(rev) / remap0(st.prev_fwd), beta);
+    const float c = !st.prev_delta && !st.prevprev_delta ? 1.0f : 0.0f;
+    return rb * (c + (EYE && st.prev_phong ? 0.0f : st.prev_sum));
+}
+
+// The state after vertex flags/pdf_fwd were stored with MIS prefix `prefix`.
+__device__ inline void mis_advance(WalkState& st, float pdf_fwd, uint32_t flags, float prefix) {
+    st.prevprev_delta = st.prev_delta;
+    st.prev_delta = (flags & kVDelta) != 0u;
+    st.prev_phong = (flags & kVPhong) != 0u;
+    st.prev_fwd = pdf_fwd;
+    st.prev_sum = prefix;
+}
+
 template <bool EYE>
 __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint32_t cap, uint32_t j,
-                             WalkState& st, int32_t prim, float t, float bu, float bv, Rng& g) {
+                             WalkState& st, int32_t prim, float t, float bu, float bv, Rng& g, float mis_beta) {
     int& nv = st.nv;
     bool& in_obj = st.in_obj;
     V3& prev_p = st.prev_p;
@@ -1823,8 +1863,11 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
                 // the previous vertex's reverse pdf (area measure there),
                 // kept in this vertex's MIS record
                 const float prev_rev = nv >= 1 ? to_meas(md, pdf_ph, mp, prev_p, prev_n, prev_med, sd) : 0.0f;
-                store_vert(arr, mis, mslot, mp, to_meas(md, pdf_w, prev_p, mp, d, true, sd), d, beta,
-                           kVMedium, v3(1, 1, 1), -1, 0.0f, prev_rev);
+                const float pre = mis_prefix<EYE>(st, nv - 1, prev_rev, mis_beta);
+                const float fwd = to_meas(md, pdf_w, prev_p, mp, d, true, sd);
+                store_vert(arr, mis, mslot, mp, fwd, d, beta, kVMedium, v3(1, 1, 1), -1, 0.0f, prev_rev, pre,
+                           st.prev_delta ? kVPrevDelta : 0u);
+                mis_advance(st, fwd, kVMedium, pre);
                 ++nv;
                 beta = cmul(beta, ld3(md.sigma_s));
                 pdf_w = pdf_ph;
@@ -1861,7 +1904,10 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
             const V3 txs = sv.shapes[prim].info[2] >= 0 ? pattern(sv, sv.shapes[prim], ps, bu, bv, tix)
                                                        : v3(1, 1, 1);
             // (a Dirac vertex: the previous vertex's reverse pdf is 0)
-            store_vert(arr, mis, vslot(j, nv, cap), ps, 0.0f, d, beta, kVDelta, txs, mi);
+            const float pre = mis_prefix<EYE>(st, nv - 1, 0.0f, mis_beta);
+            store_vert(arr, mis, vslot(j, nv, cap), ps, 0.0f, d, beta, kVDelta, txs, mi, 0.0f, 0.0f, pre,
+                       st.prev_delta ? kVPrevDelta : 0u);
+            mis_advance(st, 0.0f, kVDelta, pre);
             ++nv;
             beta = cmul(cmul(cmul(beta, ratio), ld3(m.sigma_s)), txs);
             float pdf_ph;
@@ -1881,8 +1927,8 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
         // The walk ends on an emitter (bidirectionalpathtracer.cpp:693-699).
         // an emitter vertex is never scattered from, so its reverse-pdf
         // word carries the primitive id instead
-        store_vert(arr, mis, vslot(j, nv, cap), p, pdf_fwd, n, beta, kVEmitter, tex, mi,
-                   __int_as_float(prim));
+        store_vert(arr, mis, vslot(j, nv, cap), p, pdf_fwd, n, beta, kVEmitter, tex, mi, __int_as_float(prim),
+                   0.0f, mis_prefix<EYE>(st, nv - 1, 0.0f, mis_beta), st.prev_delta ? kVPrevDelta : 0u);
         ++nv;
         return false;
     }
@@ -1912,7 +1958,8 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
             pdf_rev = pr <= 0.0f ? 1e-4f : pr;
         } else {
             if (!EYE) {  // a light walk ends at a Phong-lobe vertex
-                store_vert(arr, mis, slot, p, pdf_fwd, n, beta, kVDelta | kVPhong, tex, mi);
+                store_vert(arr, mis, slot, p, pdf_fwd, n, beta, kVDelta | kVPhong, tex, mi, 0.0f, 0.0f,
+                           mis_prefix<EYE>(st, nv - 2, 0.0f, mis_beta), st.prev_delta ? kVPrevDelta : 0u);
                 return false;
             }
             float cost, pdf;
@@ -1926,7 +1973,8 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
         const float cost = dot(n, w);
         pdf_next = ggx_pdf(m, n, w, wo);
         if (!(cost > 0.0f && pdf_next > 0.0f)) {
-            store_vert(arr, mis, slot, p, pdf_fwd, n, beta, 0u, tex, mi);
+            store_vert(arr, mis, slot, p, pdf_fwd, n, beta, 0u, tex, mi, 0.0f, 0.0f,
+                       mis_prefix<EYE>(st, nv - 2, 0.0f, mis_beta), st.prev_delta ? kVPrevDelta : 0u);
             return false;
         }
         weight = cmul(ggx_f(m, n, w, wo), tex) * (cost / pdf_next);
@@ -1986,7 +2034,11 @@ __device__ bool walk_segment(const SceneView& sv, BVert* arr, float4* mis, uint3
     // the previous vertex's reverse pdf (area measure there), kept in this
     // vertex's MIS record
     const float prev_rev = nv >= 2 && !delta ? to_meas(md, pdf_rev, p, prev_p, prev_n, prev_med, seg_L) : 0.0f;
-    store_vert(arr, mis, slot, p, pdf_fwd, n, beta, vflags | (delta ? kVDelta : 0u), tex, mi, 0.0f, prev_rev);
+    const uint32_t fl = vflags | (delta ? kVDelta : 0u);
+    const float pre = mis_prefix<EYE>(st, nv - 2, prev_rev, mis_beta);
+    store_vert(arr, mis, slot, p, pdf_fwd, n, beta, fl, tex, mi, 0.0f, prev_rev, pre,
+               st.prev_delta ? kVPrevDelta : 0u);
+    mis_advance(st, pdf_fwd, fl, pre);
     beta = cmul(beta, weight);
     pdf_w = delta ? 0.0f : pdf_next;
     prev_p = p;
@@ -2017,10 +2069,12 @@ __device__ inline void write_walk(const WalkBuf& wb, uint32_t q, const WalkState
                                   uint32_t j) {
     wb.ray[2 * (size_t)q + 0] = make_float4(st.o.x, st.o.y, st.o.z, 0.0f);
     wb.ray[2 * (size_t)q + 1] = make_float4(st.d.x, st.d.y, st.d.z, FLT_MAX);
-    wb.flags[q] = (st.in_obj ? 2u : 0u) | light | ((uint32_t)st.nv << 8);
+    wb.flags[q] = (st.in_obj ? 2u : 0u) | light | ((uint32_t)st.nv << 8) | (st.prev_med ? 1u << 16 : 0u) |
+                  (st.prev_delta ? 1u << 17 : 0u) | (st.prev_phong ? 1u << 18 : 0u) |
+                  (st.prevprev_delta ? 1u << 19 : 0u);
     wb.beta[q] = make_float4(st.beta.x, st.beta.y, st.beta.z, st.pdf_w);
-    wb.prev[q] = make_float4(st.prev_p.x, st.prev_p.y, st.prev_p.z, 0.0f);
-    wb.prevn[q] = make_float4(st.prev_n.x, st.prev_n.y, st.prev_n.z, st.prev_med ? 1.0f : 0.0f);
+    wb.prev[q] = make_float4(st.prev_p.x, st.prev_p.y, st.prev_p.z, st.prev_fwd);
+    wb.prevn[q] = make_float4(st.prev_n.x, st.prev_n.y, st.prev_n.z, st.prev_sum);
     wb.rng[q] = rng;
     wb.slot[q] = j;
 }
@@ -2041,6 +2095,9 @@ __device__ inline bool init_walk(const BdptBufs& bb, uint64_t base, const BdptPa
     const uint64_t s = k / prm.pixels;
     st.nv = 1;
     st.in_obj = false;
+    st.prev_sum = 0.0f;
+    st.prev_fwd = 1.0f;
+    st.prev_delta = st.prev_phong = st.prevprev_delta = st.prev_med = false;
     bool go = false;
     if (eye) {
         g = rng_seed(prm.seed + s * 0x9E3779B97F4A7C15ull, pixel);
@@ -2063,6 +2120,7 @@ __device__ inline bool init_walk(const BdptBufs& bb, uint64_t base, const BdptPa
         st.o = st.prev_p = cp;
         st.prev_n = cdir;
         st.prev_med = false;
+        st.prev_fwd = 1.0f;
         st.beta = v3(1, 1, 1);
         bb.n_eye[j] = 1;
         go = bp.n_eye > 1;
@@ -2096,6 +2154,7 @@ __device__ inline bool init_walk(const BdptBufs& bb, uint64_t base, const BdptPa
             st.o = st.prev_p = p0;
             st.prev_n = n0;
             st.prev_med = false;
+            st.prev_fwd = pdf_pos;
             bb.n_light[j] = 1;
             go = bp.n_light > 1;
         }
@@ -2142,16 +2201,22 @@ __device__ inline bool walk_one(const BdptBufs& bb, uint64_t base, const BdptPar
     st.beta = v3(b4.x, b4.y, b4.z);
     st.pdf_w = b4.w;
     st.prev_p = v3(p4.x, p4.y, p4.z);
+    st.prev_fwd = p4.w;
     const float4 n4 = in.prevn[i];
     st.prev_n = v3(n4.x, n4.y, n4.z);
-    st.prev_med = n4.w != 0.0f;
-    st.nv = (int)(flags >> 8);
+    st.prev_sum = n4.w;
+    st.prev_med = (flags >> 16) & 1u;
+    st.prev_delta = (flags >> 17) & 1u;
+    st.prev_phong = (flags >> 18) & 1u;
+    st.prevprev_delta = (flags >> 19) & 1u;
+    st.nv = (int)((flags >> 8) & 0xffu);
     st.in_obj = (flags & 2u) != 0;
     const int2 h = hits[i];
     const float2 uv = uvs[i];
     const float t = __int_as_float(h.y);
-    bool go = light ? walk_segment<false>(sv, bb.light, bb.light_mis, bp.n_light, j, st, h.x, t, uv.x, uv.y, g)
-                    : walk_segment<true>(sv, bb.eye, bb.eye_mis, bp.n_eye, j, st, h.x, t, uv.x, uv.y, g);
+    bool go = light ? walk_segment<false>(sv, bb.light, bb.light_mis, bp.n_light, j, st, h.x, t, uv.x, uv.y, g,
+                                          bp.beta)
+                    : walk_segment<true>(sv, bb.eye, bb.eye_mis, bp.n_eye, j, st, h.x, t, uv.x, uv.y, g, bp.beta);
     go = go && st.nv < (light ? bp.n_light : bp.n_eye);
     if (!go) {
         if (light)
@@ -2188,11 +2253,6 @@ __global__ __launch_bounds__(kWalkBlock) void k_bdpt_walk(BdptBufs bb, uint64_t 
     }
 }
 
-__device__ inline float remap0(float f) { return f != 0.0f ? f : 1.0f; }
-
-__device__ inline float mis_pow(float r, float beta) {
-    return beta == 1.0f ? r : (beta == 2.0f ? r * r : powf(r, beta));
-}
 
 // Connection work list.  After the walks, the samples of a batch are put in
 // the order of their subpath lengths, (n_eye, n_light) both descending
@@ -2362,37 +2422,34 @@ __device__ __forceinline__ bool connect_one(const BdptBufs& bb, uint64_t base, c
     // MIS weight over all strategies of this path (power heuristic with exponent beta)
     float w = 1.0f;
     if (s + t > 2 && !mis_trivial) {
-        // the walks read only the compact (pdf_fwd, pdf_rev, flags) records,
-        // each vertex once
-        float sum = 0.0f, ri = 1.0f;
-        if (t - 1 > 0) {
-            // record i holds vertex i's pdf_fwd and flags and vertex i - 1's
-            // pdf_rev: nx is record i + 1
-            float4 mi = bb.eye_mis[vslot(j, t - 1, nEc)], nx = mi;
-            for (int i = t - 1; i > 0; --i) {
-                const float4 mp = bb.eye_mis[vslot(j, i - 1, nEc)];
-                const float pr = i == t - 1 ? pt_rev : (i == t - 2 ? ptm_rev : nx.w);
-                ri *= remap0(pr) / remap0(mi.x);
-                const bool di = i == t - 1 ? false : ((uint32_t)__float_as_int(mi.z) & kVDelta) != 0;
-                const bool dprev = ((uint32_t)__float_as_int(mp.z) & kVDelta) != 0;
-                if (!di && !dprev) sum += mis_pow(ri, bp.beta);
-                if ((uint32_t)__float_as_int(mi.z) & kVPhong) break;
-                nx = mi;
-                mi = mp;
+        // the other strategies of this path through the subpaths' MIS
+        // prefixes (mis_prefix): the two vertices at each end of the
+        // connection take the connection's reverse pdfs, the rest is the
+        // prefix kept in the record of the second vertex from the end
+        float sum = 0.0f;
+        if (t >= 2) {
+            const float4 e1 = bb.eye_mis[vslot(j, t - 1, nEc)], e2 = bb.eye_mis[vslot(j, t - 2, nEc)];
+            const uint32_t f1 = (uint32_t)__float_as_int(e1.z), f2 = (uint32_t)__float_as_int(e2.z);
+            const float a = remap0(pt_rev) / remap0(e1.x);
+            if (!(f2 & kVDelta)) sum += mis_pow(a, bp.beta);
+            if (!(f1 & kVPhong) && t >= 3) {
+                const float ab = a * (remap0(ptm_rev) / remap0(e2.x));
+                const float abb = mis_pow(ab, bp.beta);
+                if (!(f2 & (kVDelta | kVPrevDelta))) sum += abb;
+                if (!(f2 & kVPhong) && t >= 4) sum += abb * e2.y;
             }
         }
-        ri = 1.0f;
-        if (s - 1 >= 0) {
-            float4 mi = bb.light_mis[vslot(j, s - 1, nLc)], nx = mi;
-            for (int i = s - 1; i >= 0; --i) {
-                const float4 mp = i > 0 ? bb.light_mis[vslot(j, i - 1, nLc)] : make_float4(0, 0, 0, 0);
-                const float pr = i == s - 1 ? qs_rev : (i == s - 2 ? qsm_rev : nx.w);
-                ri *= remap0(pr) / remap0(mi.x);
-                const bool di = i == s - 1 ? false : ((uint32_t)__float_as_int(mi.z) & kVDelta) != 0;
-                const bool dprev = i > 0 ? ((uint32_t)__float_as_int(mp.z) & kVDelta) != 0 : false;
-                if (!di && !dprev) sum += mis_pow(ri, bp.beta);
-                nx = mi;
-                mi = mp;
+        if (s >= 1) {
+            const float4 l1 = bb.light_mis[vslot(j, s - 1, nLc)];
+            const float4 l2 = s >= 2 ? bb.light_mis[vslot(j, s - 2, nLc)] : make_float4(0, 0, 0, 0);
+            const uint32_t f2 = (uint32_t)__float_as_int(l2.z);
+            const float a = remap0(qs_rev) / remap0(l1.x);
+            if (!(f2 & kVDelta)) sum += mis_pow(a, bp.beta);
+            if (s >= 2) {
+                const float ab = a * (remap0(qsm_rev) / remap0(l2.x));
+                const float abb = mis_pow(ab, bp.beta);
+                if (!(f2 & (kVDelta | kVPrevDelta))) sum += abb;
+                if (s >= 3) sum += abb * l2.y;
             }
         }
         w = 1.0f / (1.0f + sum);
