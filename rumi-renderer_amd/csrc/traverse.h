@@ -507,6 +507,7 @@ __device__ __forceinline__ bool trav_step_near4(const float4* __restrict__ nodes
                 q.dy = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.dy)));
                 q.dz = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.dz)));
                 q.tmax = __int_as_float(__builtin_amdgcn_ds_bpermute(oa, __float_as_int(r.tmax)));
+                q.tmin = 0.0f;  // not read: tri_base leaves the test against the hit to the owner
                 int32_t rid = -1;
                 float rt = 0.0f, ru = 0.0f, rv = 0.0f;
                 if (tester) {
