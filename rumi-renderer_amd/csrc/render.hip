@@ -2234,7 +2234,10 @@ __device__ inline bool walk_one(const BdptBufs& bb, uint64_t base, const BdptPar
 // atomics serialise).
 constexpr int kWalkBlock = 256;
 
-__global__ __launch_bounds__(kWalkBlock) void k_bdpt_walk(BdptBufs bb, uint64_t base, BdptParams bp, SceneView sv,
+// Three waves per SIMD (168 VGPRs, a few spills): left to itself the
+// compiler takes 180 and two waves (whole-frame BDPT 345 -> 341 ms; four
+// waves spill more, 345 ms).
+__global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bdpt_walk(BdptBufs bb, uint64_t base, BdptParams bp, SceneView sv,
                                                           WalkBuf in, const uint32_t* n_ptr, const int2* hits,
                                                           const float2* uvs, WalkBuf out, uint32_t* next_count) {
     __shared__ uint32_t s_tmp[kWalkBlock / 64 + 1];
