@@ -217,9 +217,13 @@ struct Ctl {
     uint32_t queue[2];      // paths in path buffer 0 / 1
     uint32_t shadow;        // entries in the shadow queue
     uint32_t fetch[2];      // ray counters of the persistent extension / shadow stages
-    uint32_t pad0[3];
-    uint32_t pad1[20];
+    uint32_t pad0;
+    // a frame's totals, added up at each iteration's start (k_clear_ctl):
+    // extension rays, shadow entries of the finished iterations, iterations
+    unsigned long long total[3];
+    uint32_t pad1[16];
 };
+static_assert(sizeof(Ctl) == 112, "Ctl layout");
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
@@ -457,8 +461,16 @@ __global__ __launch_bounds__(kBlock) void k_extend(PathBuf buf, const uint32_t* 
 // Clears the counters the next wavefront iteration appends to.
 __global__ void k_clear_ctl(Ctl* ctl, int next_queue) {
     const uint32_t t = threadIdx.x;
-    if (t == 0) ctl->queue[next_queue] = 0u;
-    if (t == 1) ctl->shadow = 0u;
+    if (t == 0) {
+        const uint32_t n = ctl->queue[next_queue ^ 1];
+        ctl->total[0] += n;
+        ctl->total[2] += n ? 1u : 0u;
+        ctl->queue[next_queue] = 0u;
+    }
+    if (t == 1) {
+        ctl->total[1] += ctl->shadow;
+        ctl->shadow = 0u;
+    }
     if (t == 2) ctl->fetch[0] = 0u;
     if (t == 3) ctl->fetch[1] = 0u;
 }
@@ -3072,11 +3084,17 @@ void regen(const Wave& w, int cur, uint32_t start, uint32_t count, uint64_t samp
 // shading, shadow rays, until every sample of the frame has terminated.
 // MODE selects plain rendering, the adjoint's primal pass or its replay.
 // The stages read their sizes from the device control block; the host reads
-// one word per iteration to decide on regeneration and the end.
+// the queue size after each iteration while samples remain to be issued (to
+// decide on regeneration), then after every kDrainIters iterations (queues
+// only shrink then, so the last size read bounds the launches, and stages
+// whose queue has emptied end at once).
 // `fixed` (or nullptr): a caller-provided work buffer of fixed_bytes that the
 // queue must fit (the no-allocation entry point); the pass then allocates
 // nothing -- no work buffer and no HIP events (its time is taken on the host,
 // the call being synchronous).
+// Iterations between host reads once every sample of the frame is issued.
+constexpr int kDrainIters = 4;
+
 int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
               rumi_render_stats* stats, hipStream_t stream, int mode, const AdjCtx& adj,
               void* fixed = nullptr, size_t fixed_bytes = 0) {
@@ -3148,8 +3166,9 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
     uint64_t issued = 0, ext_rays = 0, shadow_rays = 0, iters = 0;
     uint32_t n_cur = 0;
     int cur = 0;
-    uint32_t host_ctl[3];
+    Ctl host_ctl{};
     int rc = RUMI_OK;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)w.ctl->total, 6u);
     while (true) {
         if (issued < total && n_cur < cap && n_cur <= regen_at) {
             const uint32_t cnt = (uint32_t)std::min<uint64_t>(cap - n_cur, total - issued);
@@ -3158,18 +3177,24 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
             issued += cnt;
         }
         if (n_cur == 0) break;
-        ++iters;
-        ext_rays += n_cur;
-        iterate<false>(w, cur, n_cur, ps, stream);
-        if (hipMemcpyAsync(host_ctl, w.ctl, sizeof(host_ctl), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        const int batch = issued >= total ? kDrainIters : 1;
+        for (int k = 0; k < batch; ++k) {
+            iterate<false>(w, cur, n_cur, ps, stream);
+            cur ^= 1;
+        }
+        if (hipMemcpyAsync(&host_ctl, w.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream) != hipSuccess ||
             hipStreamSynchronize(stream) != hipSuccess) {
             rc = fail(RUMI_ERR_HIP, "rumi_render: counter read failed: %s",
                       hipGetErrorString(hipGetLastError()));
             break;
         }
-        shadow_rays += host_ctl[2];
-        cur ^= 1;
-        n_cur = host_ctl[cur];
+        n_cur = host_ctl.queue[cur];
+    }
+    if (rc == RUMI_OK) {
+        // the last iteration's shadow entries were not added up yet
+        ext_rays = host_ctl.total[0];
+        shadow_rays = host_ctl.total[1] + host_ctl.shadow;
+        iters = host_ctl.total[2];
     }
     if (rc == RUMI_OK) {
         if (mode != kReplay)
