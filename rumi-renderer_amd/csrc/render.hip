@@ -3085,15 +3085,17 @@ void regen(const Wave& w, int cur, uint32_t start, uint32_t count, uint64_t samp
 // MODE selects plain rendering, the adjoint's primal pass or its replay.
 // The stages read their sizes from the device control block; the host reads
 // the queue size after each iteration while samples remain to be issued (to
-// decide on regeneration), then after every kDrainIters iterations (queues
-// only shrink then, so the last size read bounds the launches, and stages
-// whose queue has emptied end at once).
+// decide on regeneration), then after batches of up to kDrainIters
+// iterations that stop at the paths' depth limit (queues only shrink then,
+// so the last size read bounds the launches, and stages whose queue has
+// emptied end at once).
 // `fixed` (or nullptr): a caller-provided work buffer of fixed_bytes that the
 // queue must fit (the no-allocation entry point); the pass then allocates
 // nothing -- no work buffer and no HIP events (its time is taken on the host,
 // the call being synchronous).
-// Iterations between host reads once every sample of the frame is issued.
-constexpr int kDrainIters = 4;
+// Iterations between host reads once every sample of the frame is issued
+// (at most; fewer when the paths' depth limit says fewer remain).
+constexpr int kDrainIters = 8;
 
 int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
               rumi_render_stats* stats, hipStream_t stream, int mode, const AdjCtx& adj,
@@ -3165,7 +3167,7 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
 
     uint64_t issued = 0, ext_rays = 0, shadow_rays = 0, iters = 0;
     uint32_t n_cur = 0;
-    int cur = 0;
+    int cur = 0, since_regen = 0;
     Ctl host_ctl{};
     int rc = RUMI_OK;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)w.ctl->total, 6u);
@@ -3175,9 +3177,14 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
             regen<false>(w, cur, n_cur, cnt, issued, ps, stream);
             n_cur += cnt;
             issued += cnt;
+            since_regen = 0;
         }
         if (n_cur == 0) break;
-        const int batch = issued >= total ? kDrainIters : 1;
+        // a path ends within max_depth + 1 iterations of its start (past
+        // that bound, one iteration per read again)
+        const int left = ps.prm.max_depth + 1 - since_regen;
+        const int batch = issued >= total ? std::min(kDrainIters, std::max(1, left)) : 1;
+        since_regen += batch;
         for (int k = 0; k < batch; ++k) {
             iterate<false>(w, cur, n_cur, ps, stream);
             cur ^= 1;
