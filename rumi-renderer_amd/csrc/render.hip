@@ -3094,8 +3094,10 @@ void regen(const Wave& w, int cur, uint32_t start, uint32_t count, uint64_t samp
 // nothing -- no work buffer and no HIP events (its time is taken on the host,
 // the call being synchronous).
 // Iterations between host reads once every sample of the frame is issued
-// (at most; fewer when the paths' depth limit says fewer remain).
-constexpr int kDrainIters = 8;
+// (at most; fewer when the paths' depth limit says fewer remain: the
+// RayTracer's 12 iterations go out in one batch, 4774 -> 5050 Msamples/s
+// against batches of 8).
+constexpr int kDrainIters = 24;
 
 int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float* film,
               rumi_render_stats* stats, hipStream_t stream, int mode, const AdjCtx& adj,
