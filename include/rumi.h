@@ -327,7 +327,8 @@ typedef struct {
     float mis_beta;         /* BDPT MIS exponent: 1 balance (the reference's), 2 power */
     uint64_t seed;
     uint32_t queue_capacity;    /* wavefront paths in flight (BDPT: samples per batch), 0 = automatic
-                                   (2^21 paths; BDPT 2^22 samples) */
+                                   (2^23 paths, at most the frame's samples;
+                                   BDPT 2^22 samples) */
     uint32_t regen_threshold;   /* refill the queue when fewer paths are alive, 0 = always */
     uint32_t flags;             /* RUMI_RENDER_* */
 } rumi_render_desc;

@@ -96,8 +96,8 @@ def tune_trace(ctx, run: Callable[[], object], candidates: Iterable[dict] | None
     return best, results
 
 
-def render_candidates(capacities: Sequence[int] = (1 << 18, 1 << 19, 1 << 20, 1 << 21),
-                      regen_fractions: Sequence[float] = (1.0, 0.5, 0.25)):
+def render_candidates(capacities: Sequence[int] = (1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23),
+                      regen_fractions: Sequence[float] = (1.0, 0.5)):
     """(queue_capacity, regen_threshold) pairs; the threshold is a fraction of
     the capacity (refill when fewer paths than that are alive)."""
     return [dict(queue_capacity=c, regen_threshold=max(1, int(c * f)))

@@ -3104,7 +3104,10 @@ int render_pt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, float
               void* fixed = nullptr, size_t fixed_bytes = 0) {
     const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
     const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
-    uint32_t cap = desc->queue_capacity ? desc->queue_capacity : (1u << 21);
+    // 2^23 paths (236 B each, about 2 GB) unless the caller says otherwise:
+    // the 512^2 x 16 Cornell frame renders at 1722 Msamples/s with it and
+    // 1493 with 2^21, the 512^2 x 32 media frame at 964 and 847.
+    uint32_t cap = desc->queue_capacity ? desc->queue_capacity : (1u << 23);
     if ((uint64_t)cap > total) cap = (uint32_t)total;
     if (cap < 64) cap = 64;
     uint32_t regen_at = desc->regen_threshold ? desc->regen_threshold : cap;
