@@ -289,7 +289,10 @@ def whitted_leg(rumi, ctx, args, log, scenes):
     scene = rumi.Scene(ctx, scenes.cornell_box(w, h))
     kw = dict(integrator=rumi.RUMI_INTEGRATOR_WHITTED)
     # queue size from the native autotuner on this frame (untimed)
-    qtune = {} if args.gpu_only else scene.autotune(w, h, spp, depth, seed=1, **kw)[0]
+    if args.gpu_only:
+        qtune = {"queue_capacity": args.queue} if args.queue else {}
+    else:
+        qtune = scene.autotune(w, h, spp, depth, seed=1, **kw)[0]
     kw.update(qtune)
     scene.render(w, h, spp, depth, seed=1, **kw)
     torch.cuda.synchronize()
@@ -400,7 +403,10 @@ def model_leg(rumi, ctx, args, log, scenes):
     sc = scenes.model_scene(model, w, h)
     scene = rumi.Scene(ctx, sc)
     # queue size from the native autotuner on this frame (untimed)
-    qtune = {} if args.gpu_only else scene.autotune(w, h, spp, depth, seed=1)[0]
+    if args.gpu_only:
+        qtune = {"queue_capacity": args.queue} if args.queue else {}
+    else:
+        qtune = scene.autotune(w, h, spp, depth, seed=1)[0]
     img, st = scene.render(w, h, spp, depth, seed=1, **qtune)
     torch.cuda.synchronize()
     times = []
