@@ -328,7 +328,8 @@ typedef struct {
     uint64_t seed;
     uint32_t queue_capacity;    /* wavefront paths in flight (BDPT: samples per batch), 0 = automatic
                                    (2^23 paths, at most the frame's samples;
-                                   BDPT 2^22 samples) */
+                                   BDPT 2^24 samples or what 90% of the free
+                                   device memory holds, down to 2^20) */
     uint32_t regen_threshold;   /* refill the queue when fewer paths are alive, 0 = always */
     uint32_t flags;             /* RUMI_RENDER_* */
 } rumi_render_desc;

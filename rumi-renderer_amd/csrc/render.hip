@@ -2722,7 +2722,32 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     const int max_len = desc->max_depth;
     const uint32_t pixels = (uint32_t)desc->width * (uint32_t)desc->height;
     const uint64_t total = (uint64_t)pixels * (uint64_t)desc->spp;
-    uint32_t nb = desc->queue_capacity ? desc->queue_capacity : (1u << 22);
+    // the connection work list: (n_eye, n_light) counts and cursors, the
+    // samples in that order, the non-empty pairs and their first tiles
+    const uint32_t l1 = (uint32_t)nL + 1u, nbins = (uint32_t)(nE + 1) * l1;
+    // device bytes of a batch of b samples (the layout taken below)
+    auto batch_bytes = [&](uint32_t b) {
+        const size_t conn = std::max<size_t>(4 * (size_t)b, (size_t)1 << 25);
+        return (sizeof(BVert) + sizeof(float4)) * (size_t)b * (size_t)(nE + nL) + 6 * (size_t)b +
+               (4 + 4 + 16 + 4) * (size_t)nbins + 3 * sizeof(float4) * conn +
+               2 * (size_t)b * (2 * 96 + 16) + 40 * 256;
+    };
+    uint32_t nb = desc->queue_capacity;
+    if (!nb) {
+        // 2^24 samples (about 35 GB at 9 + 9 vertices; the 1024^2 x 64 frame
+        // renders at 170 Msamples/s with it, 150 with 2^22), or the largest
+        // power of two down to 2^20 whose buffers fit in 90% of the free
+        // device memory and what this scene already holds
+        nb = total < (1u << 24) ? (uint32_t)total : (1u << 24);
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const size_t avail = (free_b + sc->work_bytes) / 10 * 9;
+            while (nb > (1u << 20) && batch_bytes(nb) > avail) nb >>= 1;
+        } else {
+            (void)hipGetLastError();
+            nb = 1u << 22;
+        }
+    }
     if ((uint64_t)nb > total) nb = (uint32_t)total;
     const size_t vbytes = sizeof(BVert) * (size_t)nb;
     const size_t mbytes = sizeof(float4) * (size_t)nb;
@@ -2735,12 +2760,7 @@ int render_bdpt(rumi_ctx* ctx, rumi_scene* sc, const rumi_render_desc* desc, flo
     // walks: eye and light walks of a batch in one double-buffered queue
     // (80 B per walk and buffer) with their closest hits (16 B)
     const size_t walks = 2 * (size_t)nb;
-    // the connection work list: (n_eye, n_light) counts and cursors, the
-    // samples in that order, the non-empty pairs and their first tiles
-    const uint32_t l1 = (uint32_t)nL + 1u, nbins = (uint32_t)(nE + 1) * l1;
-    const size_t need = (vbytes + mbytes) * (size_t)(nE + nL) + 2 * (size_t)nb + 4 * (size_t)nb +
-                        (4 + 4 + 16 + 4) * (size_t)nbins + 3 * sizeof(float4) * max_conn +
-                        walks * (2 * 96 + 16) + 40 * 256;
+    const size_t need = batch_bytes(nb);
     if (sc->work_bytes < need) {
         if (sc->work) (void)hipFree(sc->work);
         sc->work = nullptr;
