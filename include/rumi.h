@@ -471,7 +471,8 @@ typedef struct {
     int32_t spp;              /* samples per pixel of every item */
     int32_t max_depth;        /* PathTracer max depth of every item */
     uint64_t seed;            /* forward pass; the adjoint passes use seed + 1 */
-    uint32_t queue_capacity;  /* samples per generation (0 = 2^22) */
+    uint32_t queue_capacity;  /* samples per generation (0 = 2^26, or what 90% of the free
+                                 device memory holds, down to 2^22) */
     int32_t adjoint;          /* 1: the run also forms dLoss/dImage = 2 (I - target)
                                  and runs the path-replay adjoint */
 } rumi_batch_desc;

@@ -3630,19 +3630,33 @@ extern "C" int rumi_batch_create(rumi_ctx* ctx, const rumi_batch_item* items, ui
     b->total_samples = samples;
     b->total_pixels = pixels;
     b->total_texels = texels;
-    uint32_t cap = desc->queue_capacity ? desc->queue_capacity : (1u << 22);
+    // one allocation: table, per-item arrays, wave buffers, adjoint buffers
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t fixed = al(sizeof(SceneTab) * n) + 3 * al(4 * (size_t)n) + al(8 * ((size_t)n + 1)) +
+                   al(4 * 8 * kStripes);
+    if (desc->adjoint)
+        fixed += 2 * al(12 * (size_t)pixels) + 2 * al(12 * (size_t)samples);
+    uint32_t cap = desc->queue_capacity;
+    if (!cap) {
+        // 2^26 paths per generation (about 16 GB; the 256-scene forward +
+        // adjoint graph takes 470 ms with it, 867 with 2^22), or the largest
+        // power of two down to 2^22 that fits in 90% of the free device memory
+        cap = samples < (1u << 26) ? (uint32_t)samples : (1u << 26);
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const size_t avail = free_b / 10 * 9;
+            while (cap > (1u << 22) && fixed + al(wave_bytes(cap)) > avail) cap >>= 1;
+        } else {
+            (void)hipGetLastError();
+            cap = 1u << 22;
+        }
+    }
     if ((uint64_t)cap > samples) cap = (uint32_t)samples;
     if (cap < 64) cap = 64;
     b->cap = cap;
     b->generations = (uint32_t)((samples + cap - 1) / cap);
-
-    // one allocation: table, per-item arrays, wave buffers, adjoint buffers
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t wbytes = wave_bytes(cap);
-    size_t bytes = al(sizeof(SceneTab) * n) + 3 * al(4 * (size_t)n) + al(8 * ((size_t)n + 1)) + al(wbytes) +
-                   al(4 * 8 * kStripes);
-    if (desc->adjoint)
-        bytes += 2 * al(12 * (size_t)pixels) + 2 * al(12 * (size_t)samples);
+    const size_t bytes = fixed + al(wbytes);
     if (hipMalloc(&b->mem, bytes) != hipSuccess) {
         delete b;
         return fail(RUMI_ERR_NO_MEMORY, "rumi_batch_create: hipMalloc of %zu bytes failed", bytes);
